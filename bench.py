@@ -1,0 +1,160 @@
+"""Benchmark of the banded-extension hot path on one MI355X.
+
+Workload: 1,000,000 independent banded-extension pairs (uniform, int16 score
+range): 150-bp queries, targets = source window + 0..100-bp flank mutated at
+1% substitutions / 0.1% indels, h0 ~ U[19,60], w = 100, z-drop 100, end bonus 5,
+2-bit packed and resident in HBM before timing starts.
+
+A step is one pass of the extension kernels over the whole job set.  The
+headline value is GCUPS of the batched persistent form; the plain
+one-launch form is timed beside it.  Cells are counted exactly as the CPU
+loop counts them (one per inner iteration of BandedSWScalar.swift:112).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+OPS_PER_CELL = 11          # algorithmic VALU ops per extension cell
+PEAK_PK16_TOPS = 78.6      # 256 CU x 64 lanes x 2.4 GHz x 2 (packed int16)
+PEAK_I32_TOPS = 39.3       # 32-bit VALU lane-ops
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(js, sample):
+    """Oracle (C restatement of BandedSWScalar) on a bounded sample, all host cores."""
+    from oracle import pyoracle as orc
+    from swiftbwa_amd import workload as wl
+    idx = np.arange(min(sample, js["n"]))
+    sub = wl.slice_jobs(js, idx)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    _, cells = orc.ksw_extend_batch(sub, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": threads,
+            "kind": "port",
+            "sample": "first %d jobs of the same 1,000,000-pair set (%d cells), C oracle "
+                      "-O3, %d threads, %.2f s" % (len(idx), cells, threads, dt),
+            "ext_per_s": round(len(idx) / dt, 1)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1000000)
+    ap.add_argument("--cpu-sample", type=int, default=60000)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (each job set runs on one "
+              "device; see the sharding note in DESIGN.md). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import ctypes as C
+    from swiftbwa_amd import native, workload as wl
+    from swiftbwa_amd.metalsw import HipSWEngine, ScoringParameters, _ext_arrays
+
+    log("generating %d jobs" % args.n)
+    js = wl.generate(wl.EXT_UNIFORM, 2, args.n)
+    eng = HipSWEngine(0)
+    lib = eng.lib
+    dev_name, n_cu = eng.device_info()
+    sp = ScoringParameters()
+    sc = sp.to_c()
+    a = _ext_arrays(js)
+    log("uploading to %s" % dev_name)
+    t0 = time.perf_counter()
+    st = lib.hipsw_ext_upload(eng.handle, js["n"], native.ptr(a["queries"]), native.ptr(a["q_off"]),
+                              native.ptr(a["q_len"]), native.ptr(a["targets"]),
+                              native.ptr(a["t_off"]), native.ptr(a["t_len"]), native.ptr(a["h0"]),
+                              native.ptr(a["w"]), native.ptr(a["end_bonus"]), C.byref(sc))
+    if not st:
+        raise RuntimeError(eng.last_error())
+    upload_s = time.perf_counter() - t0
+
+    def run(mode, steps, warmup):
+        s = native.HipswStats()
+        for _ in range(warmup):
+            rc = lib.hipsw_ext_run(eng.handle, st, mode, C.byref(s))
+            if rc:
+                raise RuntimeError(eng.last_error())
+        kms = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            rc = lib.hipsw_ext_run(eng.handle, st, mode, C.byref(s))
+            if rc:
+                raise RuntimeError(eng.last_error())
+            kms.append(s.kernel_ms)
+        wall = (time.perf_counter() - t0) * 1000.0 / steps
+        return wall, float(np.mean(kms)), s
+
+    log("plain form: warmup %d, steps %d" % (args.warmup, args.steps))
+    p_wall, p_kms, p_st = run(native.MODE_PLAIN, args.steps, args.warmup)
+    log("plain: %.2f ms/step (kernel %.2f ms)" % (p_wall, p_kms))
+    log("batched form: warmup %d, steps %d" % (args.warmup, args.steps))
+    b_wall, b_kms, b_st = run(native.MODE_BATCHED, args.steps, args.warmup)
+    log("batched: %.2f ms/step (kernel %.2f ms)" % (b_wall, b_kms))
+    cells = b_st.cells
+    assert cells == p_st.cells, (cells, p_st.cells)
+    out = np.zeros((js["n"], 6), np.int32)
+    lib.hipsw_ext_download(eng.handle, st, native.ptr(out))
+    lib.hipsw_ext_free(eng.handle, st)
+
+    gcups = cells / (b_wall * 1e-3) / 1e9
+    achieved = cells * OPS_PER_CELL / (b_kms * 1e-3) / 1e12
+    rec = {
+        "metric": "banded-SW GCUPS (batched persistent form, 1 GPU)",
+        "value": round(gcups, 2),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(b_wall, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32 VALU on u8-packed h/e state (exact score tier per job)",
+        "data": "synthetic",
+        "config": {
+            "workload": "uniform banded extension, int16 range: 1,000,000 pairs, q=150 bp, "
+                        "t=150..250 bp (+indels), h0~U[19,60], w=100, zdrop=100, end bonus 5",
+            "jobs": js["n"], "cells": int(cells),
+            "ext_per_s": round(js["n"] / (b_wall * 1e-3), 1),
+            "kernel_ms": round(b_kms, 3),
+            "plain_gcups": round(cells / (p_wall * 1e-3) / 1e9, 2),
+            "plain_kernel_ms": round(p_kms, 3),
+            "batched_over_plain": round(p_wall / b_wall, 3),
+            "tier_jobs": list(b_st.tier_jobs),
+            "upload_s_incl_pack": round(upload_s, 3),
+            "device": dev_name, "n_cu": n_cu,
+        },
+        "roofline": {
+            "bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_PK16_TOPS,
+            "unit": "TOP/s", "frac": round(achieved / PEAK_PK16_TOPS, 4), "traffic": None,
+            "ops_per_cell": OPS_PER_CELL, "peak_int32": PEAK_I32_TOPS,
+            "frac_int32": round(achieved / PEAK_I32_TOPS, 4),
+            "kernel": "ext_persistent_kernel<u8 tier>",
+        },
+    }
+    if not args.no_cpu:
+        log("cpu baseline on %d jobs" % args.cpu_sample)
+        rec["cpu_baseline"] = cpu_baseline(js, args.cpu_sample)
+    eng.close()
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

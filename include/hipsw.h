@@ -1,0 +1,149 @@
+/*
+ * hipsw.h -- C ABI of the MI355X (gfx950) Smith-Waterman engine that replaces
+ * swiftbwa's MetalSW module behind its Swift call surface.
+ *
+ * Exposed to Swift as a system-library target (module map beside this header),
+ * in the style of swiftbwa's CHelpers target (Package.swift:54-58,
+ * Sources/CHelpers/include/CHelpers.h).
+ *
+ * Conventions
+ *  - The caller owns all host memory; nothing returned aliases engine memory.
+ *  - Status 0 = success.  A negative status means "use the CPU path": the Swift
+ *    shim maps it to nil / [] exactly as MetalSW's dispatchers signal failure
+ *    (BandedSWDispatcher.swift:70,83,129,142; GlobalSWDispatcher.swift:81).
+ *  - Sequences are one byte per base, 0..4 (A C G T N), the layout swiftbwa
+ *    holds (Base.swift:2-9).  Queries and targets are packed the way MetalSW's
+ *    QueryTargetPacker packs them (MetalDispatchHelper.swift:30-62): one byte
+ *    arena each, u32 offsets, u16 lengths.  The engine packs to 2 bits + an N
+ *    mask before the upload.
+ *  - Every call is thread-safe; concurrent calls on one engine serialise on its
+ *    stream slots.
+ */
+#ifndef HIPSW_H
+#define HIPSW_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define HIPSW_OK              0
+#define HIPSW_E_INVALID      -1   /* bad argument (null pointer, bad scoring) */
+#define HIPSW_E_DEVICE       -2   /* HIP runtime error (alloc, launch, copy)  */
+#define HIPSW_E_UNSUPPORTED  -3   /* input outside what the kernels handle    */
+#define HIPSW_E_BUDGET       -4   /* request exceeds the engine's HBM budget  */
+#define HIPSW_E_PENDING      -5   /* ticket not finished (poll)               */
+
+/* Dispatch modes of the extension and NW entry points. */
+#define HIPSW_MODE_PLAIN      0   /* one launch per call, input order            */
+#define HIPSW_MODE_BATCHED    1   /* area-binned persistent launch, tuned per bin */
+
+typedef struct hipsw_engine hipsw_engine;
+
+/* Scoring: the 5x5 matrix of ScoringParameters.scoringMatrix()
+ * (ScoringParameters.swift:65-73) plus the gap penalties and z-drop.
+ * `mat` is indexed [target_base * 5 + query_base] (BandedSWScalar.swift:45-50). */
+typedef struct {
+    int8_t  mat[25];
+    int32_t o_ins, e_ins;     /* gapOpenPenalty, gapExtendPenalty            */
+    int32_t o_del, e_del;     /* gapOpenPenaltyDeletion, gapExtendPenaltyDeletion */
+    int32_t zdrop;            /* zDrop                                        */
+} hipsw_scoring;
+
+/* Per-call timing/counters, filled when the caller passes a non-NULL pointer. */
+typedef struct {
+    double  kernel_ms;        /* device time of the DP kernels (HIP events)   */
+    double  total_ms;         /* wall time of the call incl. pack/copies      */
+    int64_t n_jobs;
+    int64_t launches;         /* kernel launches issued                        */
+    int64_t promoted;         /* jobs run in a wider score tier than int16     */
+    int64_t tier_jobs[3];     /* jobs per score tier: u8, u16, i32             */
+    int64_t cells;            /* DP cells evaluated, counted as the CPU loop    */
+                              /* counts them (BandedSWScalar.swift:112)         */
+} hipsw_stats;
+
+/* ---- engine (replaces MetalSWEngine.shared, MetalSWEngine.swift:8-17) ---- */
+
+/* Create an engine on HIP device `device`.  NULL when no usable device exists
+ * or the kernels cannot be loaded -- the same "engine is nil" signal MetalSW
+ * gives (MetalSWEngine.swift:8-17); callers then stay on the CPU. */
+hipsw_engine *hipsw_engine_create(int device);
+void          hipsw_engine_destroy(hipsw_engine *eng);
+/* Text of the last error on this engine (valid until the next call on it). */
+const char   *hipsw_last_error(const hipsw_engine *eng);
+/* MetalSWEngine.minBatchSize (MetalSWEngine.swift:39). */
+int32_t       hipsw_min_batch_size(void);
+/* Device name and compute-unit count, for reports. */
+int           hipsw_device_info(const hipsw_engine *eng, char *name, int name_cap,
+                                int32_t *n_cu);
+
+/* ---- banded extension (replaces BandedSWDispatcher.dispatchBatch8 and
+ *      dispatchBatch16, BandedSWDispatcher.swift:48-158) ----
+ * Semantics are BandedSWScalar.align (BandedSWScalar.swift:21-191) per task,
+ * with per-task h0, w and end bonus (the Metal path took w from tasks[0] only,
+ * BandedSWDispatcher.swift:61, and had no end bonus).  out is n rows of six
+ * int32 in SWResult order {score, queryEnd, targetEnd, globalTargetEnd,
+ * globalScore, maxOff} (SWResult.swift:2-25, BandedSWDispatcher.swift:92-99).
+ * Scores are exact in every case: the engine picks an 8-, 16- or 32-bit score
+ * tier per task from the bound h0 + qlen * max(mat). */
+int hipsw_banded_extend(hipsw_engine *eng, int32_t n,
+                        const uint8_t *queries, const uint32_t *q_off, const uint16_t *q_len,
+                        const uint8_t *targets, const uint32_t *t_off, const uint16_t *t_len,
+                        const int32_t *h0, const int32_t *w, const int32_t *end_bonus,
+                        const hipsw_scoring *sc, int32_t mode, int32_t *out,
+                        hipsw_stats *stats);
+
+/* Device-resident job sets: upload once, run many times (benchmarks, serving).
+ * hipsw_ext_run leaves results on the device; hipsw_ext_download copies them
+ * out.  No reference counterpart: MetalSW re-uploads every call. */
+typedef struct hipsw_ext_set hipsw_ext_set;
+hipsw_ext_set *hipsw_ext_upload(hipsw_engine *eng, int32_t n,
+                                const uint8_t *queries, const uint32_t *q_off,
+                                const uint16_t *q_len, const uint8_t *targets,
+                                const uint32_t *t_off, const uint16_t *t_len,
+                                const int32_t *h0, const int32_t *w, const int32_t *end_bonus,
+                                const hipsw_scoring *sc);
+int  hipsw_ext_run(hipsw_engine *eng, hipsw_ext_set *set, int32_t mode, hipsw_stats *stats);
+int  hipsw_ext_download(hipsw_engine *eng, hipsw_ext_set *set, int32_t *out);
+void hipsw_ext_free(hipsw_engine *eng, hipsw_ext_set *set);
+
+/* ---- banded global NW + traceback (replaces GlobalSWDispatcher.dispatchBatch,
+ *      GlobalSWDispatcher.swift:6-118) ----
+ * Per task, the NW loop of CIGARGenerator.generate (CIGARGenerator.swift:82-96):
+ * GlobalAligner.align (GlobalAligner.swift:19-232) at band w[k]; while the
+ * score is below true_score[k] and fewer than max_retries retries were made,
+ * w doubles and the alignment is redone.  Unlike the Metal kernel, trueScore
+ * and w are not clamped to int16 (GlobalSWDispatcher.swift:55-56) and the band
+ * widening follows the CPU (w' = max(w, |qlen-tlen|) inside every attempt).
+ * Results: out_score[k]; the packed CIGAR (len<<4 | op, M0 I1 D2) in
+ * out_cigar[k*cap ...], out_n_cigar[k] ops (the traceback's full count, which
+ * may exceed cap); out_flags[k] bit 0 = more ops than cap (the caller redoes
+ * that task on the CPU); out_final_w[k] (nullable) = band of the last attempt. */
+int hipsw_global_nw(hipsw_engine *eng, int32_t n,
+                    const uint8_t *queries, const uint32_t *q_off, const uint16_t *q_len,
+                    const uint8_t *targets, const uint32_t *t_off, const uint16_t *t_len,
+                    const int32_t *w, const int32_t *true_score, int32_t max_retries,
+                    int32_t cap, const hipsw_scoring *sc, int32_t mode,
+                    int32_t *out_score, uint32_t *out_cigar, int32_t *out_n_cigar,
+                    int32_t *out_flags, int32_t *out_final_w, hipsw_stats *stats);
+
+/* Device-resident NW job sets (benchmarks, serving); see hipsw_ext_upload. */
+typedef struct hipsw_nw_set hipsw_nw_set;
+hipsw_nw_set *hipsw_nw_upload(hipsw_engine *eng, int32_t n,
+                              const uint8_t *queries, const uint32_t *q_off,
+                              const uint16_t *q_len, const uint8_t *targets,
+                              const uint32_t *t_off, const uint16_t *t_len,
+                              const int32_t *w, const int32_t *true_score,
+                              int32_t max_retries, int32_t cap, const hipsw_scoring *sc);
+int  hipsw_nw_run(hipsw_engine *eng, hipsw_nw_set *set, int32_t mode, hipsw_stats *stats);
+int  hipsw_nw_download(hipsw_engine *eng, hipsw_nw_set *set, int32_t *out_score,
+                       uint32_t *out_cigar, int32_t *out_n_cigar, int32_t *out_flags,
+                       int32_t *out_final_w);
+void hipsw_nw_free(hipsw_engine *eng, hipsw_nw_set *set);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

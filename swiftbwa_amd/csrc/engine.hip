@@ -1,0 +1,273 @@
+// Host side of libhipsw: the C ABI declared in include/hipsw.h.
+//
+// Replaces MetalSW's engine and dispatchers (Sources/MetalSW/MetalSWEngine.swift,
+// BandedSWDispatcher.swift, GlobalSWDispatcher.swift, LocalSWDispatcher.swift)
+// with a HIP engine: one stream per engine, grow-only device arenas, 2-bit
+// sequence packing on the host, and length-binned persistent launches.
+#include "hipsw.h"
+#include "hipsw_internal.h"
+#include "engine_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+using namespace hipsw;
+
+bool hipsw_engine::fail(const char *what, hipError_t e)
+{
+    char buf[256];
+    snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+    last_error = buf;
+    return false;
+}
+
+bool hipsw_engine::check(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return true;
+    return fail(what, e);
+}
+
+void *hipsw_engine::scratch_get(size_t bytes)
+{
+    if (bytes <= scratch_bytes) return scratch;
+    if (scratch) hipFree(scratch);
+    scratch = nullptr;
+    scratch_bytes = 0;
+    if (hipMalloc(&scratch, bytes) != hipSuccess) return nullptr;
+    scratch_bytes = bytes;
+    return scratch;
+}
+
+namespace hipsw {
+
+double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+// Runs fn(lo, hi) over [0, n) on up to `threads` host threads.
+void parallel_for(int64_t n, int threads, const std::function<void(int64_t, int64_t)> &fn)
+{
+    if (threads <= 1 || n < 4096) {
+        fn(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    const int64_t chunk = (n + threads - 1) / threads;
+    for (int k = 0; k < threads; ++k) {
+        const int64_t lo = k * chunk, hi = std::min(n, lo + chunk);
+        if (lo >= hi) break;
+        th.emplace_back([&fn, lo, hi] { fn(lo, hi); });
+    }
+    for (auto &t : th) t.join();
+}
+
+int host_threads()
+{
+    unsigned h = std::thread::hardware_concurrency();
+    return (int)std::min(16u, std::max(1u, h));
+}
+
+bool scoring_valid(const hipsw_scoring *sc)
+{
+    return sc && sc->e_ins > 0 && sc->e_del > 0 && sc->o_ins >= 0 && sc->o_del >= 0;
+}
+
+ExtParams make_params(const hipsw_scoring *sc)
+{
+    ExtParams p;
+    memset(&p, 0, sizeof p);
+    int max_mat = 0;                       // BandedSWScalar.swift:54-58 (starts at 0)
+    for (int k = 0; k < 25; ++k) {
+        p.mat[k] = sc->mat[k];
+        max_mat = std::max(max_mat, (int)sc->mat[k]);
+    }
+    p.o_del = sc->o_del; p.e_del = sc->e_del;
+    p.o_ins = sc->o_ins; p.e_ins = sc->e_ins;
+    p.zdrop = sc->zdrop;
+    p.oe_del = sc->o_del + sc->e_del;
+    p.oe_ins = sc->o_ins + sc->e_ins;
+    p.max_mat = max_mat;
+    return p;
+}
+
+// 2-bit packing of one sequence into `dst` (16 bases per word, base k at bits
+// 2k..2k+1; N -> code 0 plus an N bit).  Returns false on a code > 4.
+bool pack2(const uint8_t *src, int len, uint32_t *dst, uint32_t *ndst)
+{
+    const int words = (len + 15) / 16;
+    for (int wdx = 0; wdx < words; ++wdx) {
+        uint32_t v = 0;
+        const int lo = wdx * 16, hi = std::min(len, lo + 16);
+        for (int k = lo; k < hi; ++k) {
+            const uint8_t b = src[k];
+            if (b > 4) return false;
+            v |= (uint32_t)(b & 3) << ((k - lo) * 2);
+        }
+        dst[wdx] = v;
+    }
+    if (ndst) {
+        const int nw = (len + 31) / 32;
+        for (int wdx = 0; wdx < nw; ++wdx) {
+            uint32_t v = 0;
+            const int lo = wdx * 32, hi = std::min(len, lo + 32);
+            for (int k = lo; k < hi; ++k)
+                if (src[k] == 4) v |= 1u << (k - lo);
+            ndst[wdx] = v;
+        }
+    }
+    return true;
+}
+
+bool has_n(const uint8_t *src, int len)
+{
+    for (int k = 0; k < len; ++k)
+        if (src[k] >= 4) return true;
+    return false;
+}
+
+bool pack_jobs(int n, const uint8_t *queries, const uint32_t *q_off, const uint16_t *q_len,
+               const uint8_t *targets, const uint32_t *t_off, const uint16_t *t_len,
+               PackedJobs &pj, std::string &err)
+{
+    const int threads = host_threads();
+    pj.qw.assign(n, 0); pj.tw.assign(n, 0); pj.qnw.assign(n, kNoN); pj.tnw.assign(n, kNoN);
+    pj.ql.assign(n, 0); pj.tl.assign(n, 0);
+    std::vector<uint8_t> qn_flag(n), tn_flag(n);
+    parallel_for(n, threads, [&](int64_t lo, int64_t hi) {
+        for (int64_t k = lo; k < hi; ++k) {
+            qn_flag[k] = has_n(queries + q_off[k], q_len[k]);
+            tn_flag[k] = has_n(targets + t_off[k], t_len[k]);
+        }
+    });
+    uint64_t words = 0, nwords = 0;
+    for (int k = 0; k < n; ++k) {
+        pj.ql[k] = q_len[k];
+        pj.tl[k] = t_len[k];
+        pj.qw[k] = (uint32_t)words;
+        words += (q_len[k] + 15) / 16;
+        pj.tw[k] = (uint32_t)words;
+        words += (t_len[k] + 15) / 16;
+        if (qn_flag[k]) { pj.qnw[k] = (uint32_t)nwords; nwords += (q_len[k] + 31) / 32; }
+        if (tn_flag[k]) { pj.tnw[k] = (uint32_t)nwords; nwords += (t_len[k] + 31) / 32; }
+    }
+    if (words + 64 > 0xFFFFFFF0ull) {
+        err = "job set exceeds 2^32 sequence words";
+        return false;
+    }
+    words += 64;
+    nwords += 64;
+    pj.seq.assign(words, 0);
+    pj.nmask.assign(nwords, 0);
+    bool bad = false;
+    parallel_for(n, threads, [&](int64_t lo, int64_t hi) {
+        for (int64_t k = lo; k < hi; ++k) {
+            if (!pack2(queries + q_off[k], q_len[k], pj.seq.data() + pj.qw[k],
+                       qn_flag[k] ? pj.nmask.data() + pj.qnw[k] : nullptr))
+                bad = true;
+            if (!pack2(targets + t_off[k], t_len[k], pj.seq.data() + pj.tw[k],
+                       tn_flag[k] ? pj.nmask.data() + pj.tnw[k] : nullptr))
+                bad = true;
+        }
+    });
+    if (bad) {
+        err = "base code > 4";
+        return false;
+    }
+    return true;
+}
+
+}  // namespace hipsw
+
+// ------------------------------------------------------------------------
+// Engine lifetime
+// ------------------------------------------------------------------------
+extern "C" hipsw_engine *hipsw_engine_create(int device)
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 || device < 0 || device >= ndev)
+        return nullptr;
+    if (hipSetDevice(device) != hipSuccess) return nullptr;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return nullptr;
+    auto *eng = new hipsw_engine();
+    eng->device = device;
+    eng->n_cu = prop.multiProcessorCount;
+    eng->lds_per_cu = (int)prop.maxSharedMemoryPerMultiProcessor;
+    // Dynamic LDS per block is kept within 64 KiB, the size every launch may
+    // request without an attribute change.
+    eng->lds_per_block = (int)std::min<size_t>(prop.sharedMemPerBlock, 65536);
+    eng->name = prop.name;
+    eng->arch = prop.gcnArchName;
+    if (hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete eng;
+        return nullptr;
+    }
+    for (auto &s : eng->side) {
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+            delete eng;
+            return nullptr;
+        }
+    }
+    if (hipEventCreate(&eng->ev0) != hipSuccess || hipEventCreate(&eng->ev1) != hipSuccess) {
+        delete eng;
+        return nullptr;
+    }
+    for (auto &e : eng->side_ev) hipEventCreateWithFlags(&e, hipEventDisableTiming);
+    if (hipMalloc((void **)&eng->counters, 64 * sizeof(uint32_t)) != hipSuccess) {
+        delete eng;
+        return nullptr;
+    }
+    return eng;
+}
+
+hipsw_engine::~hipsw_engine()
+{
+    hipSetDevice(device);
+    if (stream) hipStreamSynchronize(stream);
+    for (auto s : side)
+        if (s) hipStreamDestroy(s);
+    if (stream) hipStreamDestroy(stream);
+    if (ev0) hipEventDestroy(ev0);
+    if (ev1) hipEventDestroy(ev1);
+    for (auto e : side_ev)
+        if (e) hipEventDestroy(e);
+    if (counters) hipFree(counters);
+    if (scratch) hipFree(scratch);
+    free_nw_buffers();
+}
+
+extern "C" void hipsw_engine_destroy(hipsw_engine *eng) { delete eng; }
+
+extern "C" const char *hipsw_last_error(const hipsw_engine *eng)
+{
+    return eng ? eng->last_error.c_str() : "no engine";
+}
+
+extern "C" int32_t hipsw_min_batch_size(void) { return 32; }
+
+extern "C" int hipsw_device_info(const hipsw_engine *eng, char *name, int name_cap, int32_t *n_cu)
+{
+    if (!eng) return HIPSW_E_INVALID;
+    if (name && name_cap > 0) {
+        std::string s = eng->name + " (" + eng->arch + ")";
+        snprintf(name, (size_t)name_cap, "%s", s.c_str());
+    }
+    if (n_cu) *n_cu = eng->n_cu;
+    return HIPSW_OK;
+}
+
+void hipsw_engine::free_nw_buffers()
+{
+    if (nw_dev) hipFree(nw_dev);
+    nw_dev = nullptr;
+    nw_dev_bytes = 0;
+}

@@ -1,0 +1,375 @@
+// Host side of the banded-extension entry points (include/hipsw.h).
+//
+// Replaces BandedSWDispatcher.dispatchBatch8/16 (Sources/MetalSW/
+// BandedSWDispatcher.swift:48-158).  A call packs the tasks to 2-bit words,
+// picks an exact score tier per task, uploads one arena, runs the DP kernels
+// and copies SWResult rows back.
+#include "engine_internal.h"
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+using namespace hipsw;
+
+namespace {
+
+// Query-length classes of the batched form: each class is one persistent
+// launch whose LDS footprint is sized to the class' longest query.
+constexpr int kClassBounds[] = {64, 160, 256, 1 << 30};
+constexpr int kClasses = 4;
+
+int class_of(int qlen)
+{
+    for (int c = 0; c < kClasses; ++c)
+        if (qlen <= kClassBounds[c]) return c;
+    return kClasses - 1;
+}
+
+int tier_of(int64_t h0, int64_t qlen, int max_mat)
+{
+    const int64_t bound = h0 + qlen * (int64_t)max_mat;
+    if (h0 >= 0 && bound <= 255) return TIER_U8;
+    if (h0 >= 0 && bound <= 65535) return TIER_U16;
+    return TIER_I32;
+}
+
+// Waves (64-thread blocks) resident per CU for a given dynamic LDS size.
+int waves_per_cu(const hipsw_engine *eng, size_t lds)
+{
+    const size_t per_cu = eng->lds_per_cu > 0 ? (size_t)eng->lds_per_cu : 65536;
+    int w = (int)(per_cu / std::max<size_t>(lds + 512, 1));
+    return std::max(1, std::min(w, 8));
+}
+
+template <typename T>
+T *carve(char *&cur, size_t count)
+{
+    T *p = reinterpret_cast<T *>(cur);
+    cur += (count * sizeof(T) + 255) & ~(size_t)255;
+    return p;
+}
+
+template <typename T>
+size_t span(size_t count)
+{
+    return (count * sizeof(T) + 255) & ~(size_t)255;
+}
+
+}  // namespace
+
+extern "C" hipsw_ext_set *hipsw_ext_upload(hipsw_engine *eng, int32_t n,
+                                           const uint8_t *queries, const uint32_t *q_off,
+                                           const uint16_t *q_len, const uint8_t *targets,
+                                           const uint32_t *t_off, const uint16_t *t_len,
+                                           const int32_t *h0, const int32_t *w,
+                                           const int32_t *end_bonus, const hipsw_scoring *sc)
+{
+    if (!eng) return nullptr;
+    std::lock_guard<std::recursive_mutex> lock(eng->mu);
+    if (n < 0 || (n > 0 && (!queries || !q_off || !q_len || !targets || !t_off || !t_len ||
+                            !h0 || !w || !end_bonus))) {
+        eng->last_error = "hipsw_ext_upload: null argument";
+        return nullptr;
+    }
+    if (!scoring_valid(sc)) {
+        eng->last_error = "hipsw_ext_upload: gap extension penalties must be > 0";
+        return nullptr;
+    }
+    hipSetDevice(eng->device);
+    auto *set = new hipsw_ext_set();
+    set->n = n;
+    set->params = make_params(sc);
+    const int max_mat = set->params.max_mat;
+    const int threads = host_threads();
+
+    // Word layout: every query and target starts on a 16-base word boundary.
+    std::vector<uint32_t> qw(n), tw(n), qnw(n), tnw(n);
+    std::vector<uint8_t> qn_flag(n), tn_flag(n);
+    parallel_for(n, threads, [&](int64_t lo, int64_t hi) {
+        for (int64_t k = lo; k < hi; ++k) {
+            qn_flag[k] = has_n(queries + q_off[k], q_len[k]);
+            tn_flag[k] = has_n(targets + t_off[k], t_len[k]);
+        }
+    });
+    uint64_t words = 0, nwords = 0;
+    for (int k = 0; k < n; ++k) {
+        qw[k] = (uint32_t)words;
+        words += (q_len[k] + 15) / 16;
+        tw[k] = (uint32_t)words;
+        words += (t_len[k] + 15) / 16;
+        qnw[k] = qn_flag[k] ? (uint32_t)nwords : kNoN;
+        if (qn_flag[k]) nwords += (q_len[k] + 31) / 32;
+        tnw[k] = tn_flag[k] ? (uint32_t)nwords : kNoN;
+        if (tn_flag[k]) nwords += (t_len[k] + 31) / 32;
+    }
+    if (words + 64 > 0xFFFFFFF0ull) {
+        eng->last_error = "hipsw_ext_upload: job set exceeds 2^32 sequence words";
+        delete set;
+        return nullptr;
+    }
+    words += 64;    // tail padding: whole-word reads past the last sequence stay in bounds
+    nwords += 64;
+    std::vector<uint32_t> seq(words, 0), nmask(nwords, 0);
+    std::vector<int32_t> ql(n), tl(n), vh0(n), vw(n), veb(n);
+    bool bad = false;
+    parallel_for(n, threads, [&](int64_t lo, int64_t hi) {
+        for (int64_t k = lo; k < hi; ++k) {
+            ql[k] = q_len[k];
+            tl[k] = t_len[k];
+            vh0[k] = h0[k];
+            vw[k] = w[k];
+            veb[k] = end_bonus[k];
+            if (!pack2(queries + q_off[k], q_len[k], seq.data() + qw[k],
+                       qn_flag[k] ? nmask.data() + qnw[k] : nullptr))
+                bad = true;
+            if (!pack2(targets + t_off[k], t_len[k], seq.data() + tw[k],
+                       tn_flag[k] ? nmask.data() + tnw[k] : nullptr))
+                bad = true;
+        }
+    });
+    if (bad) {
+        eng->last_error = "hipsw_ext_upload: base code > 4";
+        delete set;
+        return nullptr;
+    }
+
+    // Exact score tier per job.
+    std::vector<uint8_t> tier(n);
+    int plain_tier = TIER_U8, qmax = 0;
+    for (int k = 0; k < n; ++k) {
+        tier[k] = (uint8_t)tier_of(vh0[k], ql[k], max_mat);
+        set->tier_jobs[tier[k]]++;
+        plain_tier = std::max(plain_tier, (int)tier[k]);
+        qmax = std::max(qmax, ql[k]);
+    }
+
+    // Plain form: input order, one tier and one LDS size for the whole launch.
+    set->plain_tier = plain_tier;
+    set->plain_cols = qmax + 1;
+    const size_t plain_lds = ext_lds_bytes(plain_tier, false, set->plain_cols);
+    set->plain_global = plain_lds > (size_t)eng->lds_per_block;
+    const int groups = (n + 63) / 64;
+    if (set->plain_global) {
+        set->plain_grid = std::max(1, std::min(groups, eng->n_cu * 8));
+        set->plain_scratch = (size_t)set->plain_grid * set->plain_cols * 64 * ext_state_bytes(plain_tier);
+    } else {
+        set->plain_grid = std::max(1, groups);
+    }
+
+    // Batched form: bin by (tier, query-length class); inside a bin order by
+    // estimated DP area, largest first; cut into 64-job work items.
+    std::vector<uint64_t> keyed(n);
+    for (int k = 0; k < n; ++k) {
+        const int bin = tier[k] * kClasses + class_of(ql[k]);
+        const uint64_t band = (uint64_t)std::min<int64_t>(ql[k], 2 * (int64_t)std::max(vw[k], 0) + 1);
+        const uint64_t area = std::min<uint64_t>((uint64_t)tl[k] * band, (1ull << 31) - 1);
+        // bin ascending, area descending, index ascending (stable)
+        keyed[k] = ((uint64_t)bin << 59) | ((((1ull << 31) - 1) - area) << 27) | 0;
+        (void)area;
+    }
+    std::vector<uint32_t> order(n);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t b) { return keyed[a] < keyed[b]; });
+    std::vector<WorkItem> items;
+    items.reserve((size_t)groups + 16);
+    size_t scratch_off = 0;
+    for (int s = 0; s < n;) {
+        const int bin = (int)(keyed[order[s]] >> 59);
+        int e = s;
+        int cmax = 0;
+        while (e < n && (int)(keyed[order[e]] >> 59) == bin) {
+            cmax = std::max(cmax, ql[order[e]]);
+            ++e;
+        }
+        ExtLaunch L;
+        L.tier = bin / kClasses;
+        L.cols = cmax + 1;
+        L.global_state = ext_lds_bytes(L.tier, false, L.cols) > (size_t)eng->lds_per_block;
+        L.item_begin = (int)items.size();
+        for (int g = s; g < e; g += 64) items.push_back(WorkItem{(uint32_t)g, (uint32_t)std::min(64, e - g)});
+        L.item_count = (int)items.size() - L.item_begin;
+        const int wpc = L.global_state ? 8 : waves_per_cu(eng, ext_lds_bytes(L.tier, false, L.cols));
+        L.grid = std::max(1, std::min(L.item_count, eng->n_cu * wpc));
+        L.scratch_off = scratch_off;
+        if (L.global_state) scratch_off += (size_t)L.grid * L.cols * 64 * ext_state_bytes(L.tier);
+        set->launches.push_back(L);
+        s = e;
+    }
+    set->batched_scratch = scratch_off;
+
+    // One device allocation for the whole set.
+    const size_t bytes = span<uint32_t>(words) + span<uint32_t>(nwords) + 4 * span<uint32_t>(n) +
+                         6 * span<int32_t>(n) + span<int32_t>((size_t)n * 6) +
+                         span<uint32_t>(n) + span<WorkItem>(items.size() + 1);
+    if (!eng->check(hipMalloc(&set->dmem, bytes), "hipMalloc(ext set)")) {
+        delete set;
+        return nullptr;
+    }
+    set->dmem_bytes = bytes;
+    char *cur = reinterpret_cast<char *>(set->dmem);
+    uint32_t *d_seq = carve<uint32_t>(cur, words);
+    uint32_t *d_nmask = carve<uint32_t>(cur, nwords);
+    uint32_t *d_qw = carve<uint32_t>(cur, n);
+    uint32_t *d_tw = carve<uint32_t>(cur, n);
+    uint32_t *d_qn = carve<uint32_t>(cur, n);
+    uint32_t *d_tn = carve<uint32_t>(cur, n);
+    int32_t *d_ql = carve<int32_t>(cur, n);
+    int32_t *d_tl = carve<int32_t>(cur, n);
+    int32_t *d_h0 = carve<int32_t>(cur, n);
+    int32_t *d_w = carve<int32_t>(cur, n);
+    int32_t *d_eb = carve<int32_t>(cur, n);
+    int32_t *d_unused = carve<int32_t>(cur, std::max(n, 2));   // 8-byte cell counter
+    int32_t *d_out = carve<int32_t>(cur, (size_t)n * 6);
+    set->d_order = carve<uint32_t>(cur, n);
+    set->d_items = carve<WorkItem>(cur, items.size() + 1);
+    bool ok = true;
+    auto up = [&](void *dst, const void *src, size_t b) {
+        if (ok && b) ok = eng->check(hipMemcpyAsync(dst, src, b, hipMemcpyHostToDevice, eng->stream), "upload");
+    };
+    up(d_seq, seq.data(), words * 4);
+    up(d_nmask, nmask.data(), nwords * 4);
+    up(d_qw, qw.data(), (size_t)n * 4);
+    up(d_tw, tw.data(), (size_t)n * 4);
+    up(d_qn, qnw.data(), (size_t)n * 4);
+    up(d_tn, tnw.data(), (size_t)n * 4);
+    up(d_ql, ql.data(), (size_t)n * 4);
+    up(d_tl, tl.data(), (size_t)n * 4);
+    up(d_h0, vh0.data(), (size_t)n * 4);
+    up(d_w, vw.data(), (size_t)n * 4);
+    up(d_eb, veb.data(), (size_t)n * 4);
+    up(set->d_order, order.data(), (size_t)n * 4);
+    up(set->d_items, items.data(), items.size() * sizeof(WorkItem));
+    if (ok) ok = eng->check(hipStreamSynchronize(eng->stream), "upload sync");
+    if (!ok) {
+        hipFree(set->dmem);
+        delete set;
+        return nullptr;
+    }
+    set->dev = ExtDev{d_seq, d_nmask, d_qw, d_tw, d_qn, d_tn, d_ql, d_tl, d_h0, d_w, d_eb, d_out,
+                      reinterpret_cast<unsigned long long *>(d_unused)};
+    return set;
+}
+
+extern "C" int hipsw_ext_run(hipsw_engine *eng, hipsw_ext_set *set, int32_t mode,
+                             hipsw_stats *stats)
+{
+    if (!eng || !set) return HIPSW_E_INVALID;
+    if (mode != HIPSW_MODE_PLAIN && mode != HIPSW_MODE_BATCHED) return HIPSW_E_INVALID;
+    std::lock_guard<std::recursive_mutex> lock(eng->mu);
+    hipSetDevice(eng->device);
+    const double t0 = now_ms();
+    int64_t launches = 0;
+    if (set->n > 0) {
+        const size_t need = mode == HIPSW_MODE_PLAIN ? (set->plain_global ? set->plain_scratch : 0)
+                                                     : set->batched_scratch;
+        void *gs = nullptr;
+        if (need) {
+            gs = eng->scratch_get(need);
+            if (!gs) {
+                eng->last_error = "hipsw_ext_run: scratch allocation failed";
+                return HIPSW_E_DEVICE;
+            }
+        }
+        if (mode == HIPSW_MODE_BATCHED && set->launches.size() > 64) {
+            eng->last_error = "hipsw_ext_run: too many bins";
+            return HIPSW_E_UNSUPPORTED;
+        }
+        if (!eng->check(launch_zero_u32(reinterpret_cast<uint32_t *>(set->dev.cells), 2, eng->stream),
+                        "launch zero"))
+            return HIPSW_E_DEVICE;
+        if (!eng->check(hipEventRecord(eng->ev0, eng->stream), "event")) return HIPSW_E_DEVICE;
+        if (mode == HIPSW_MODE_PLAIN) {
+            if (!eng->check(launch_ext_plain(set->plain_tier, set->plain_global, set->dev,
+                                             set->params, nullptr, set->n, set->plain_cols,
+                                             set->plain_grid, gs, eng->stream),
+                            "launch ext_plain"))
+                return HIPSW_E_DEVICE;
+            launches = 1;
+        } else {
+            if (!eng->check(launch_zero_u32(eng->counters, 64, eng->stream), "launch zero"))
+                return HIPSW_E_DEVICE;
+            int k = 0;
+            for (const ExtLaunch &L : set->launches) {
+                void *lgs = L.global_state ? (char *)gs + L.scratch_off : nullptr;
+                if (!eng->check(launch_ext_persistent(L.tier, L.global_state, set->dev, set->params,
+                                                      set->d_order, set->d_items + L.item_begin,
+                                                      L.item_count, eng->counters + k, L.cols,
+                                                      L.grid, lgs, eng->stream),
+                                "launch ext_persistent"))
+                    return HIPSW_E_DEVICE;
+                ++k;
+                ++launches;
+            }
+        }
+        if (!eng->check(hipEventRecord(eng->ev1, eng->stream), "event")) return HIPSW_E_DEVICE;
+        if (!eng->check(hipEventSynchronize(eng->ev1), "kernel")) return HIPSW_E_DEVICE;
+    }
+    if (stats) {
+        float ms = 0.f;
+        if (set->n > 0) hipEventElapsedTime(&ms, eng->ev0, eng->ev1);
+        stats->kernel_ms = ms;
+        stats->total_ms = now_ms() - t0;
+        stats->n_jobs = set->n;
+        stats->launches = launches;
+        stats->promoted = set->tier_jobs[TIER_I32];
+        for (int k = 0; k < 3; ++k) stats->tier_jobs[k] = set->tier_jobs[k];
+        unsigned long long c = 0;
+        if (set->n > 0)
+            hipMemcpy(&c, set->dev.cells, sizeof c, hipMemcpyDeviceToHost);
+        stats->cells = (int64_t)c;
+    }
+    return HIPSW_OK;
+}
+
+extern "C" int hipsw_ext_download(hipsw_engine *eng, hipsw_ext_set *set, int32_t *out)
+{
+    if (!eng || !set || (set->n > 0 && !out)) return HIPSW_E_INVALID;
+    std::lock_guard<std::recursive_mutex> lock(eng->mu);
+    if (set->n == 0) return HIPSW_OK;
+    hipSetDevice(eng->device);
+    if (!eng->check(hipMemcpy(out, set->dev.out, (size_t)set->n * 6 * sizeof(int32_t),
+                              hipMemcpyDeviceToHost),
+                    "download"))
+        return HIPSW_E_DEVICE;
+    return HIPSW_OK;
+}
+
+extern "C" void hipsw_ext_free(hipsw_engine *eng, hipsw_ext_set *set)
+{
+    if (!set) return;
+    if (eng) {
+        std::lock_guard<std::recursive_mutex> lock(eng->mu);
+        hipSetDevice(eng->device);
+        hipStreamSynchronize(eng->stream);
+        if (set->dmem) hipFree(set->dmem);
+    }
+    delete set;
+}
+
+extern "C" int hipsw_banded_extend(hipsw_engine *eng, int32_t n, const uint8_t *queries,
+                                   const uint32_t *q_off, const uint16_t *q_len,
+                                   const uint8_t *targets, const uint32_t *t_off,
+                                   const uint16_t *t_len, const int32_t *h0, const int32_t *w,
+                                   const int32_t *end_bonus, const hipsw_scoring *sc,
+                                   int32_t mode, int32_t *out, hipsw_stats *stats)
+{
+    if (!eng) return HIPSW_E_INVALID;
+    if (n == 0) return HIPSW_OK;
+    if (n < 0 || !out) return HIPSW_E_INVALID;
+    std::lock_guard<std::recursive_mutex> lock(eng->mu);
+    const double t0 = now_ms();
+    hipsw_ext_set *set = hipsw_ext_upload(eng, n, queries, q_off, q_len, targets, t_off, t_len,
+                                          h0, w, end_bonus, sc);
+    if (!set) return eng->last_error.find("hipMalloc") != std::string::npos ||
+                             eng->last_error.find("upload") == 0
+                         ? HIPSW_E_DEVICE
+                         : HIPSW_E_INVALID;
+    int rc = hipsw_ext_run(eng, set, mode, stats);
+    if (rc == HIPSW_OK) rc = hipsw_ext_download(eng, set, out);
+    hipsw_ext_free(eng, set);
+    if (stats) stats->total_ms = now_ms() - t0;
+    return rc;
+}

@@ -1,0 +1,347 @@
+// Banded seed-extension kernels for gfx950 (ksw_extend2 semantics).
+//
+// Semantics: BandedSWScalar.align, Sources/Alignment/BandedSWScalar.swift:21-191
+// (w cap :52-62, first row :68-77, row loop :90-181, cells :112-143, band
+// narrowing :175-180), restated per lane.
+//
+// Layout: one wave64 holds 64 independent DP matrices, one per lane, and sweeps
+// them row by row in lock step (inter-task SIMD).  Each lane owns the whole
+// row recurrence of its matrix, so the row-sequential parts of ksw_extend2 --
+// the band [beg,end) recomputed from the previous row, stale entries beyond
+// `end`, last-j row-max ties, z-drop and the rowMax==0 break -- are exact by
+// construction.  The per-column state h[j]/e[j] of all 64 matrices lives in LDS
+// as one word per (column, lane), column-major, so a column access by the wave
+// touches 64 consecutive words (conflict-free).  Scores are held in the
+// narrowest exact tier: two u8 (16-bit word), two u16 (32-bit word) or two i32.
+// Queries and targets are read as 2-bit words from HBM (16 bases per load).
+#include "hipsw_internal.h"
+
+namespace hipsw {
+
+template <int TIER> struct St;
+template <> struct St<TIER_U8> {
+    using W = uint16_t;
+    static __device__ __forceinline__ int h(W v) { return v & 0xFF; }
+    static __device__ __forceinline__ int e(W v) { return v >> 8; }
+    static __device__ __forceinline__ W pack(int hv, int ev) { return (W)(hv | (ev << 8)); }
+    static __device__ __forceinline__ bool zero(W v) { return v == 0; }
+};
+template <> struct St<TIER_U16> {
+    using W = uint32_t;
+    static __device__ __forceinline__ int h(W v) { return (int)(v & 0xFFFFu); }
+    static __device__ __forceinline__ int e(W v) { return (int)(v >> 16); }
+    static __device__ __forceinline__ W pack(int hv, int ev) { return (W)hv | ((W)ev << 16); }
+    static __device__ __forceinline__ bool zero(W v) { return v == 0; }
+};
+template <> struct St<TIER_I32> {
+    using W = int2;
+    static __device__ __forceinline__ int h(W v) { return v.x; }
+    static __device__ __forceinline__ int e(W v) { return v.y; }
+    static __device__ __forceinline__ W pack(int hv, int ev) { return make_int2(hv, ev); }
+    static __device__ __forceinline__ bool zero(W v) { return v.x == 0 && v.y == 0; }
+};
+
+size_t ext_state_bytes(int tier)
+{
+    return tier == TIER_U8 ? 2 : tier == TIER_U16 ? 4 : 8;
+}
+
+__device__ __forceinline__ int wave_min(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Extends the (<= 64) jobs idx[0..count) with one wave.  `col` points at this
+// lane's column-0 state word; column j is at col[j * 64].
+template <int TIER>
+__device__ __forceinline__ void ext_wave(const ExtDev &d, const ExtParams &p,
+                                         const uint32_t *idx, int count,
+                                         typename St<TIER>::W *col, const int8_t *smat)
+{
+    using S = St<TIER>;
+    using W = typename S::W;
+    const int lane = threadIdx.x & 63;
+    const bool valid = lane < count;
+    const int job = valid ? (int)idx[lane] : 0;
+    const int qlen = valid ? d.q_len[job] : 0;
+    const int tlen = valid ? d.t_len[job] : 0;
+    const int h0 = valid ? d.h0[job] : 0;
+    int w = valid ? d.w[job] : 0;
+    const int eb = valid ? d.eb[job] : 0;
+    const uint32_t qw0 = valid ? d.q_word[job] : 0;
+    const uint32_t tw0 = valid ? d.t_word[job] : 0;
+    const uint32_t qn0 = valid ? d.qn_word[job] : kNoN;
+    const uint32_t tn0 = valid ? d.tn_word[job] : kNoN;
+    const bool nonempty = valid && qlen > 0 && tlen > 0;     // :33-35
+    bool alive = nonempty;
+
+    if (alive) {                                               // w cap, :52-62
+        int mi = (int)((double)(qlen * p.max_mat + eb - p.o_ins) / (double)p.e_ins + 1.0);
+        w = min(w, max(1, mi));
+        int md = (int)((double)(qlen * p.max_mat + eb - p.o_del) / (double)p.e_del + 1.0);
+        w = min(w, max(1, md));
+    }
+
+    // First row (:68-77): h[0] = h0, h[j] = max(0, h0 - o_ins - e_ins*j), e = 0.
+    const int qmax = wave_max(alive ? qlen : 0);
+    for (int j = 0; j <= qmax; ++j) {
+        if (alive && j <= qlen) {
+            int hv = j == 0 ? h0 : max(0, h0 - p.o_ins - p.e_ins * j);
+            col[j * 64] = S::pack(hv, 0);
+        }
+    }
+
+    int beg = 0, end = qlen;                                   // :79-87
+    int maxv = h0, max_i = -1, max_j = -1;
+    int gsc = -1, gtle = -1, max_off = 0;
+    uint32_t tw = 0, tn = 0;
+    uint64_t ncells = 0;
+
+    for (int i = 0;; ++i) {
+        if (alive && i >= tlen) alive = false;
+        if (__ballot(alive) == 0) break;
+        if ((i & 15) == 0 && alive) tw = d.seq[tw0 + (i >> 4)];
+        int t = (tw >> ((i & 15) * 2)) & 3;
+        if (tn0 != kNoN) {
+            if ((i & 31) == 0 && alive) tn = d.nmask[tn0 + (i >> 5)];
+            if ((tn >> (i & 31)) & 1) t = 4;
+        }
+        const int t5 = t * 5;
+
+        int f = 0, h1 = 0;
+        uint32_t key = 0;                 // (rowMax << 16) | rowMaxJ, narrow tiers
+        int row_max = 0, row_max_j = -1;  // i32 tier
+        if (alive) {                                           // :95-107
+            beg = max(beg, i - w);
+            end = min(end, i + w + 1);
+            end = min(end, qlen);
+            h1 = beg == 0 ? max(0, h0 - (p.o_del + p.e_del * (i + 1))) : 0;
+        }
+        if (alive && end > beg) ncells += (uint64_t)(end - beg);
+        const int jlo = wave_min(alive ? beg : 0x7fffffff);
+        const int jhi = wave_max(alive ? end : 0);
+        uint32_t qw = 0, qn = 0;
+        for (int j = jlo; j < jhi; ++j) {                      // :112-143
+            if (((j & 15) == 0 || j == jlo) && alive && j < qlen) qw = d.seq[qw0 + (j >> 4)];
+            if (qn0 != kNoN && ((j & 31) == 0 || j == jlo) && alive && j < qlen)
+                qn = d.nmask[qn0 + (j >> 5)];
+            if (alive && j >= beg && j < end) {
+                int q = (qw >> ((j & 15) * 2)) & 3;
+                if (qn0 != kNoN && ((qn >> (j & 31)) & 1)) q = 4;
+                const W wd = col[j * 64];
+                const int M = S::h(wd), E = S::e(wd);
+                const int diag = M != 0 ? M + (int)smat[t5 + q] : 0;
+                const int H = max(max(diag, E), f);
+                if (TIER == TIER_I32) {
+                    row_max_j = row_max > H ? row_max_j : j;
+                    row_max = max(row_max, H);
+                } else {
+                    key = max(key, ((uint32_t)H << 16) | (uint32_t)j);
+                }
+                const int ne = max(max(E - p.e_del, diag - p.oe_del), 0);
+                f = max(max(f - p.e_ins, diag - p.oe_ins), 0);
+                col[j * 64] = S::pack(h1, ne);
+                h1 = H;
+            }
+        }
+        if (alive) {
+            if (TIER != TIER_I32) {
+                row_max = (int)(key >> 16);
+                row_max_j = (int)(key & 0xFFFFu);
+            }
+            col[end * 64] = S::pack(h1, 0);                    // :145-146
+            if (end == qlen) {                                 // :149-152
+                gtle = gsc > h1 ? gtle : i;
+                gsc = max(gsc, h1);
+            }
+            if (row_max == 0) {                                // :155
+                alive = false;
+            } else if (row_max > maxv) {                       // :158-163
+                maxv = row_max;
+                max_i = i;
+                max_j = row_max_j;
+                max_off = max(max_off, abs(row_max_j - i));
+            } else if (p.zdrop > 0) {                          // :164-172
+                const int di = i - max_i, dj = row_max_j - max_j;
+                const int drop = di > dj ? maxv - row_max - (di - dj) * p.e_del
+                                         : maxv - row_max - (dj - di) * p.e_ins;
+                if (drop > p.zdrop) alive = false;
+            }
+            if (alive) {                                       // :175-180
+                int nb = beg;
+                while (nb < end && S::zero(col[nb * 64])) ++nb;
+                beg = nb;
+                int ne = end;
+                while (ne >= beg && S::zero(col[ne * 64])) --ne;
+                end = min(ne + 2, qlen);
+            }
+        }
+    }
+    if (d.cells) {
+        unsigned long long c = ncells;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+        if (lane == 0) atomicAdd(d.cells, (unsigned long long)c);
+    }
+    if (valid) {                                               // :183-190
+        int32_t *o = d.out + (size_t)job * 6;
+        if (nonempty) {
+            o[0] = maxv; o[1] = max_j + 1; o[2] = max_i + 1;
+            o[3] = gtle + 1; o[4] = gsc; o[5] = max_off;
+        } else {
+            o[0] = 0; o[1] = 0; o[2] = 0; o[3] = 0; o[4] = 0; o[5] = 0;
+        }
+    }
+}
+
+// Per-wave state pointer: LDS when the columns fit, else a per-block slice of
+// a global scratch buffer (cols * 64 words per block).
+template <int TIER, bool GLOBAL>
+__device__ __forceinline__ typename St<TIER>::W *state_base(void *gscratch, int cols)
+{
+    using W = typename St<TIER>::W;
+    if (GLOBAL) {
+        return reinterpret_cast<W *>(gscratch) + (size_t)blockIdx.x * cols * 64 + threadIdx.x;
+    } else {
+        extern __shared__ __align__(16) unsigned char lds_raw[];
+        return reinterpret_cast<W *>(lds_raw + 32) + threadIdx.x;
+    }
+}
+
+__device__ __forceinline__ const int8_t *load_smat(const ExtParams &p)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    int8_t *smat = reinterpret_cast<int8_t *>(lds_raw);
+    if (threadIdx.x < 32) smat[threadIdx.x] = p.mat[threadIdx.x];
+    __syncthreads();
+    return smat;
+}
+
+// Plain form: one launch over the jobs in input order, 64 consecutive jobs per
+// wave.  With global state the grid is capped and waves stride over groups.
+template <int TIER, bool GLOBAL>
+__global__ __launch_bounds__(64) void ext_plain_kernel(ExtDev d, ExtParams p,
+                                                       const uint32_t *order, int n, int cols,
+                                                       void *gscratch)
+{
+    const int8_t *smat = load_smat(p);
+    typename St<TIER>::W *col = state_base<TIER, GLOBAL>(gscratch, cols);
+    __shared__ uint32_t idx[64];
+    const int lane = threadIdx.x;
+    const int groups = (n + 63) / 64;
+    for (int g = blockIdx.x; g < groups; g += gridDim.x) {
+        const int base = g * 64;
+        const int count = min(64, n - base);
+        if (lane < count) idx[lane] = order ? order[base + lane] : (uint32_t)(base + lane);
+        __syncthreads();
+        ext_wave<TIER>(d, p, idx, count, col, smat);
+        __syncthreads();
+    }
+}
+
+// Persistent work-stealing form: the grid is sized to the machine; each wave
+// takes the next WorkItem from a global counter until the list is drained.
+// Items are ordered by decreasing DP area, so the longest groups start first.
+template <int TIER, bool GLOBAL>
+__global__ __launch_bounds__(64) void ext_persistent_kernel(ExtDev d, ExtParams p,
+                                                            const uint32_t *order,
+                                                            const WorkItem *items, int n_items,
+                                                            uint32_t *counter, int cols,
+                                                            void *gscratch)
+{
+    const int8_t *smat = load_smat(p);
+    typename St<TIER>::W *col = state_base<TIER, GLOBAL>(gscratch, cols);
+    __shared__ uint32_t idx[64];
+    __shared__ int item_sh;
+    for (;;) {
+        if (threadIdx.x == 0) item_sh = (int)atomicAdd(counter, 1u);
+        __syncthreads();
+        const int item = item_sh;
+        if (item >= n_items) break;
+        const WorkItem wi = items[item];
+        if (threadIdx.x < wi.count) idx[threadIdx.x] = order[wi.start + threadIdx.x];
+        __syncthreads();
+        ext_wave<TIER>(d, p, idx, (int)wi.count, col, smat);
+        __syncthreads();
+    }
+}
+
+__global__ void zero_u32_kernel(uint32_t *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+hipError_t launch_zero_u32(uint32_t *p, int n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+size_t ext_lds_bytes(int tier, bool global, int cols)
+{
+    return 32 + (global ? 0 : (size_t)cols * 64 * ext_state_bytes(tier));
+}
+static size_t lds_bytes(int tier, bool global, int cols) { return ext_lds_bytes(tier, global, cols); }
+
+template <int TIER, bool GLOBAL>
+static hipError_t plain_t(const ExtDev &d, const ExtParams &p, const uint32_t *order, int n,
+                          int cols, int grid, void *gs, hipStream_t s)
+{
+    hipLaunchKernelGGL((ext_plain_kernel<TIER, GLOBAL>), dim3(grid), dim3(64),
+                       lds_bytes(TIER, GLOBAL, cols), s, d, p, order, n, cols, gs);
+    return hipGetLastError();
+}
+
+template <int TIER, bool GLOBAL>
+static hipError_t persist_t(const ExtDev &d, const ExtParams &p, const uint32_t *order,
+                            const WorkItem *items, int n_items, uint32_t *counter, int cols,
+                            int grid, void *gs, hipStream_t s)
+{
+    hipLaunchKernelGGL((ext_persistent_kernel<TIER, GLOBAL>), dim3(grid), dim3(64),
+                       lds_bytes(TIER, GLOBAL, cols), s, d, p, order, items, n_items, counter,
+                       cols, gs);
+    return hipGetLastError();
+}
+
+hipError_t launch_ext_plain(int tier, bool global_state, const ExtDev &d, const ExtParams &p,
+                            const uint32_t *order, int n, int cols, int grid, void *gscratch,
+                            hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    switch (tier * 2 + (global_state ? 1 : 0)) {
+    case 0: return plain_t<TIER_U8, false>(d, p, order, n, cols, grid, gscratch, s);
+    case 1: return plain_t<TIER_U8, true>(d, p, order, n, cols, grid, gscratch, s);
+    case 2: return plain_t<TIER_U16, false>(d, p, order, n, cols, grid, gscratch, s);
+    case 3: return plain_t<TIER_U16, true>(d, p, order, n, cols, grid, gscratch, s);
+    case 4: return plain_t<TIER_I32, false>(d, p, order, n, cols, grid, gscratch, s);
+    default: return plain_t<TIER_I32, true>(d, p, order, n, cols, grid, gscratch, s);
+    }
+}
+
+hipError_t launch_ext_persistent(int tier, bool global_state, const ExtDev &d,
+                                 const ExtParams &p, const uint32_t *order,
+                                 const WorkItem *items, int n_items, uint32_t *counter,
+                                 int cols, int grid, void *gscratch, hipStream_t s)
+{
+    if (n_items <= 0) return hipSuccess;
+    switch (tier * 2 + (global_state ? 1 : 0)) {
+    case 0: return persist_t<TIER_U8, false>(d, p, order, items, n_items, counter, cols, grid, gscratch, s);
+    case 1: return persist_t<TIER_U8, true>(d, p, order, items, n_items, counter, cols, grid, gscratch, s);
+    case 2: return persist_t<TIER_U16, false>(d, p, order, items, n_items, counter, cols, grid, gscratch, s);
+    case 3: return persist_t<TIER_U16, true>(d, p, order, items, n_items, counter, cols, grid, gscratch, s);
+    case 4: return persist_t<TIER_I32, false>(d, p, order, items, n_items, counter, cols, grid, gscratch, s);
+    default: return persist_t<TIER_I32, true>(d, p, order, items, n_items, counter, cols, grid, gscratch, s);
+    }
+}
+
+}  // namespace hipsw

@@ -1,0 +1,64 @@
+// Internal definitions shared by the HIP kernels and the host engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hipsw {
+
+// Score tiers.  Every H/E/F value of a banded extension lies in
+// [0, h0 + qlen * max(mat)] (BandedSWScalar.swift:112-143 clamps E and F at 0
+// and H = max(diag, E, F)), so the tier is an exact bound test per job.
+enum Tier : int { TIER_U8 = 0, TIER_U16 = 1, TIER_I32 = 2 };
+
+constexpr uint32_t kNoN = 0xFFFFFFFFu;   // "sequence has no N" marker
+
+// Device view of an uploaded extension job set (structure of arrays).
+// Sequences are 2-bit codes, 16 per 32-bit word, base k of a word at bits
+// 2k..2k+1; every query and target starts on a word boundary.  N bases carry
+// code 0 in the 2-bit stream and a set bit in the N-mask stream (32 per word).
+struct ExtDev {
+    const uint32_t *seq;      // 2-bit words
+    const uint32_t *nmask;    // N-bit words (never dereferenced when no N)
+    const uint32_t *q_word;   // first word of each query
+    const uint32_t *t_word;   // first word of each target
+    const uint32_t *qn_word;  // first N word of each query, kNoN if none
+    const uint32_t *tn_word;  // first N word of each target, kNoN if none
+    const int32_t  *q_len;
+    const int32_t  *t_len;
+    const int32_t  *h0;
+    const int32_t  *w;
+    const int32_t  *eb;       // end bonus
+    int32_t        *out;      // 6 per job, SWResult order
+    unsigned long long *cells; // DP cells evaluated (one atomic add per wave)
+};
+
+// Scoring constants broadcast to every lane.
+struct ExtParams {
+    int8_t  mat[32];          // 25 used, [target*5 + query]
+    int32_t o_del, e_del, o_ins, e_ins, zdrop;
+    int32_t oe_del, oe_ins, max_mat;
+};
+
+// A unit of persistent work: `count` (<= 64) consecutive entries of the
+// sorted job order starting at `start`.
+struct WorkItem {
+    uint32_t start;
+    uint32_t count;
+};
+
+}  // namespace hipsw
+
+// Kernel launchers (ext_kernel.hip).
+namespace hipsw {
+hipError_t launch_ext_plain(int tier, bool global_state, const ExtDev &d, const ExtParams &p,
+                            const uint32_t *order, int n, int cols, int grid, void *gscratch,
+                            hipStream_t s);
+hipError_t launch_ext_persistent(int tier, bool global_state, const ExtDev &d,
+                                 const ExtParams &p, const uint32_t *order,
+                                 const WorkItem *items, int n_items, uint32_t *counter,
+                                 int cols, int grid, void *gscratch, hipStream_t s);
+hipError_t launch_zero_u32(uint32_t *p, int n, hipStream_t s);
+size_t ext_state_bytes(int tier);   // bytes of h/e state per column per lane
+size_t ext_lds_bytes(int tier, bool global_state, int cols);   // dynamic LDS per block
+}  // namespace hipsw

@@ -1,0 +1,45 @@
+// Internal definitions of the banded global NW path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hipsw_internal.h"
+
+namespace hipsw {
+
+constexpr int32_t NW_FLAG_TRUNCATED = 1;   // more CIGAR ops than the output row holds
+
+// NW state tiers.  GlobalAligner keeps int32 H/E with the Int32.min/2 sentinel
+// (GlobalAligner.swift:52).  The 16-bit tier stores both as int16 with the
+// sentinel remapped to -16384; every comparison of the fill keeps its outcome
+// while |real value| + |sentinel drift| < 16384, which the host checks per job
+// with the bound (qlen + tlen) * max(max|mat|, o_del+e_del, o_ins+e_ins) <= 8000.
+enum NwTier : int { NW_I16 = 0, NW_I32 = 1 };
+
+struct NwDev {
+    const uint32_t *seq;
+    const uint32_t *nmask;
+    const uint32_t *q_word, *t_word, *qn_word, *tn_word;
+    const int32_t *q_len, *t_len;
+    const int32_t *w;          // band of this pass (doubled on retries)
+    int32_t *score;
+    uint32_t *cigar;           // cap ops per job
+    int32_t *n_cigar;
+    int32_t *flags;
+    unsigned long long *cells; // fill cells (GlobalAligner.swift:108 iterations)
+    int cap;
+};
+
+struct NwParams {
+    int8_t mat[32];
+    int32_t o_del, e_del, o_ins, e_ins, oe_del, oe_ins;
+};
+
+size_t nw_lds_bytes(int tier, bool global_state, int S);
+hipError_t launch_nw(int tier, bool global_state, const NwDev &d, const NwParams &p,
+                     const uint32_t *order, const WorkItem *items, int n_items,
+                     uint32_t *counter, int S, uint32_t *zbase, int zrows, int ncw, int grid,
+                     bool persistent, void *gstate, hipStream_t s);
+
+}  // namespace hipsw

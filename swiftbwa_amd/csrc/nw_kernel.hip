@@ -1,0 +1,317 @@
+// Banded global Needleman-Wunsch with traceback, for gfx950.
+//
+// Semantics: GlobalAligner.align + traceback, Sources/Alignment/GlobalAligner.swift
+// (band w' = max(w, |qlen - tlen|) :47, nCol = min(qlen, 2w'+1) :48, int32 state
+// with the Int32.min/2 sentinel :52, first row :80-84, fill :89-142, score
+// h[qlen] :144, traceback :160-232) restated per lane.
+//
+// State: int16 (sentinel remapped, exact under a per-job bound checked on the
+// host) or int32, as the tier of the launch says.
+//
+// Layout: one wave64 fills 64 independent DP matrices, one per lane, row by
+// row.  Lanes walk their own band in band-relative columns c = j - beg, so a
+// wave whose lanes share w' runs every lane on every step.  The h/e row state
+// of a lane is a circular buffer of S >= nCol + 2 slots in LDS (only the band
+// is ever live: column j lives in slot j mod S), so the LDS footprint follows
+// the band, not the query.  The 4-bit backpointers of each cell are packed 8
+// per dword and stored band-relative as [row][c/8][lane] in a per-wave HBM
+// region: every store of the wave writes 256 consecutive bytes.  Each lane then
+// walks its own traceback from that region and writes its packed CIGAR
+// (len << 4 | op, CIGARBuilder.swift:4-38) to its output row.
+#include "hipsw_internal.h"
+#include "nw_internal.h"
+
+namespace hipsw {
+
+template <int T> struct NwSt;
+template <> struct NwSt<NW_I32> {
+    using W = int2;
+    static constexpr int kNeg = INT32_MIN / 2;             // GlobalAligner.swift:52
+    static __device__ __forceinline__ int h(W v) { return v.x; }
+    static __device__ __forceinline__ int e(W v) { return v.y; }
+    static __device__ __forceinline__ W pack(int hv, int ev) { return make_int2(hv, ev); }
+};
+template <> struct NwSt<NW_I16> {
+    using W = uint32_t;
+    static constexpr int kNeg = -16384;
+    static __device__ __forceinline__ int h(W v) { return (int)(int16_t)(v & 0xFFFFu); }
+    static __device__ __forceinline__ int e(W v) { return (int)(int16_t)(v >> 16); }
+    static __device__ __forceinline__ W pack(int hv, int ev)
+    {
+        return ((uint32_t)hv & 0xFFFFu) | ((uint32_t)ev << 16);
+    }
+};
+
+__device__ __forceinline__ int wave_max_nw(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Fill + traceback for the (<= 64) jobs idx[0..count).
+template <int T>
+__device__ void nw_wave(const NwDev &d, const NwParams &p, const uint32_t *idx, int count,
+                        typename NwSt<T>::W *slots, int S, uint32_t *zreg, int zrows, int ncw,
+                        const int8_t *smat)
+{
+    using St = NwSt<T>;
+    using Wd = typename St::W;
+    constexpr int kNegInf = St::kNeg;
+    const int lane = threadIdx.x & 63;
+    const bool valid = lane < count;
+    const int job = valid ? (int)idx[lane] : 0;
+    const int qlen = valid ? d.q_len[job] : 0;
+    const int tlen = valid ? d.t_len[job] : 0;
+    const bool nonempty = valid && qlen > 0 && tlen > 0;     // :29-31
+    int w = valid ? d.w[job] : 0;
+    const int dl = qlen > tlen ? qlen - tlen : tlen - qlen;
+    w = max(w, dl);                                            // :47
+    const int ncol = min(qlen, 2 * w + 1);                     // :48
+    const uint32_t qw0 = valid ? d.q_word[job] : 0;
+    const uint32_t tw0 = valid ? d.t_word[job] : 0;
+    const uint32_t qn0 = valid ? d.qn_word[job] : kNoN;
+    const uint32_t tn0 = valid ? d.tn_word[job] : kNoN;
+    Wd *col = slots + lane;                                    // slot s at col[s * 64]
+
+    // First row (:75-84): h[0] = 0, h[j] = -(o_ins + e_ins*j) for j <= min(w,qlen),
+    // everything else (and all of e) the sentinel.
+    if (nonempty) {
+        const int lim = min(qlen, S - 1);
+        const int init_end = min(w, qlen);
+        for (int s = 0; s <= lim; ++s) {
+            const int hv = s == 0 ? 0 : (s <= init_end ? -(p.o_ins + p.e_ins * s) : kNegInf);
+            col[s * 64] = St::pack(hv, kNegInf);
+        }
+    }
+    const int cmax = wave_max_nw(nonempty ? ncol : 0);        // wave-uniform row width
+    const int nwords = (cmax + 7) >> 3;
+    uint32_t *zl = zreg + lane;                                // word (i, cw) at zl[(i*ncw+cw)*64]
+    const int tmax = wave_max_nw(nonempty ? tlen : 0);
+
+    uint32_t tw = 0, tn = 0;
+    uint64_t ncells = 0;
+    int sb = 0;                                                // slot of column beg
+    int beg_prev = 0;
+    for (int i = 0; i < tmax; ++i) {                           // :89
+        const bool act = nonempty && i < tlen;
+        if ((i & 15) == 0 && act) tw = d.seq[tw0 + (i >> 4)];
+        int t = (tw >> ((i & 15) * 2)) & 3;
+        if (tn0 != kNoN) {
+            if ((i & 31) == 0 && act) tn = d.nmask[tn0 + (i >> 5)];
+            if ((tn >> (i & 31)) & 1) t = 4;
+        }
+        const int t5 = t * 5;
+        const int beg = max(0, i - w);                         // :90-91
+        const int end = min(qlen, i + w + 1);
+        const int width = end - beg;
+        if (act && width > 0) ncells += (uint64_t)width;
+        if (act) {
+            for (int b = beg_prev; b < beg; ++b) sb = sb + 1 == S ? 0 : sb + 1;
+        }
+        beg_prev = beg;
+        int f = kNegInf, h_diag = 0, h_left = 0;
+        if (act) {                                             // :96-104
+            const Wd v = col[sb * 64];
+            h_diag = St::h(v);
+            h_left = beg == 0 ? 0 - (p.oe_del + p.e_del * i) : kNegInf;
+            col[sb * 64] = St::pack(h_left, St::e(v));
+        }
+        // query word cache: base j of the current cell
+        int j = beg;
+        uint32_t qw = 0, qn = 0;
+        if (act && j < qlen) qw = d.seq[qw0 + (j >> 4)];
+        if (act && qn0 != kNoN && j < qlen) qn = d.nmask[qn0 + (j >> 5)];
+        int s1 = sb + 1 == S ? 0 : sb + 1;                     // slot of column j+1
+        uint32_t zword = 0;
+        for (int c = 0; c < cmax; ++c) {                       // :108-141
+            if (act && c < width) {
+                int q = (qw >> ((j & 15) * 2)) & 3;
+                if (qn0 != kNoN && ((qn >> (j & 31)) & 1)) q = 4;
+                int h_cur = h_diag + (int)smat[t5 + q];
+                const Wd v = col[s1 * 64];
+                h_diag = St::h(v);
+                const int e_open = h_diag - p.oe_del;
+                const int e_ext = St::e(v) - p.e_del;
+                const uint32_t e_flag = e_open > e_ext ? 0u : 1u;
+                const int ev = e_open > e_ext ? e_open : e_ext;
+                const int f_open = h_left - p.oe_ins;
+                const int f_ext = f - p.e_ins;
+                const uint32_t f_flag = f_open > f_ext ? 0u : 2u;
+                f = f_open > f_ext ? f_open : f_ext;
+                const uint32_t from_e = ev > h_cur ? 4u : 0u;
+                h_cur = ev > h_cur ? ev : h_cur;
+                const uint32_t from_f = f > h_cur ? 8u : 0u;
+                h_cur = f > h_cur ? f : h_cur;
+                col[s1 * 64] = St::pack(h_cur, ev);
+                h_left = h_cur;
+                zword |= (e_flag | f_flag | from_e | from_f) << ((c & 7) * 4);
+                ++j;
+                s1 = s1 + 1 == S ? 0 : s1 + 1;
+                if ((j & 15) == 0 && j < qlen) qw = d.seq[qw0 + (j >> 4)];
+                if (qn0 != kNoN && (j & 31) == 0 && j < qlen) qn = d.nmask[qn0 + (j >> 5)];
+            }
+            if ((c & 7) == 7 || c == cmax - 1) {
+                if (i < zrows) zl[((size_t)i * ncw + (c >> 3)) * 64] = zword;
+                zword = 0;
+            }
+        }
+        if (act) {
+            // Column end+1 is read by the next row; in the full-width array it
+            // still holds its first-row value, the sentinel (end+1 > w').
+            const int s_next = (sb + width + 1) % S;
+            if (end + 1 <= qlen) col[s_next * 64] = St::pack(kNegInf, kNegInf);
+        }
+    }
+    (void)nwords;
+    if (d.cells) {
+        unsigned long long c = ncells;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+        if (lane == 0) atomicAdd(d.cells, c);
+    }
+
+    if (!valid) return;
+    int32_t *o_score = d.score + job;
+    uint32_t *cig = d.cigar + (size_t)job * d.cap;
+    if (!nonempty) {
+        *o_score = 0;
+        d.n_cigar[job] = 0;
+        d.flags[job] = 0;
+        return;
+    }
+    // score = h[qlen] (:144); column qlen sits in slot (sb + (qlen - beg_last)) mod S.
+    {
+        const int beg_last = max(0, tlen - 1 - w);
+        const int s = (sb + (qlen - beg_last)) % S;
+        *o_score = St::h(col[s * 64]);
+    }
+    // Traceback (:160-232) with CIGARBuilder merging; ops come out last-first.
+    int i = tlen - 1, jj = qlen - 1, state = 0;
+    int nops = 0;
+    uint32_t cur_op = 0xFu, cur_len = 0;
+    auto emit = [&](uint32_t op) {
+        if (op == cur_op) {
+            ++cur_len;
+        } else {
+            if (cur_len) {
+                if (nops < d.cap) cig[nops] = (cur_len << 4) | cur_op;
+                ++nops;
+            }
+            cur_op = op;
+            cur_len = 1;
+        }
+    };
+    while (i >= 0 || jj >= 0) {
+        if (i < 0) { emit(1u); --jj; continue; }
+        if (jj < 0) { emit(2u); --i; continue; }
+        const int beg = max(0, i - w);
+        const int c = jj - beg;
+        if (!(c >= 0 && c < ncol)) break;
+        const uint32_t word = zl[((size_t)i * ncw + (c >> 3)) * 64];
+        const uint32_t dd = (word >> ((c & 7) * 4)) & 0xFu;
+        if (state == 0) {
+            const uint32_t hdir = (dd >> 2) & 3u;
+            if (hdir == 0) { emit(0u); --i; --jj; }
+            else if (hdir == 1) state = 1;
+            else state = 2;
+        } else if (state == 1) {
+            emit(2u);
+            if ((dd & 1u) == 0) state = 0;
+            --i;
+        } else {
+            emit(1u);
+            if ((dd & 2u) == 0) state = 0;
+            --jj;
+        }
+    }
+    if (cur_len) {
+        if (nops < d.cap) cig[nops] = (cur_len << 4) | cur_op;
+        ++nops;
+    }
+    const int kept = min(nops, d.cap);
+    for (int a = 0, b = kept - 1; a < b; ++a, --b) {           // :230 reverse
+        const uint32_t tmp = cig[a];
+        cig[a] = cig[b];
+        cig[b] = tmp;
+    }
+    d.n_cigar[job] = nops;
+    d.flags[job] = nops > d.cap ? NW_FLAG_TRUNCATED : 0;
+}
+
+__device__ __forceinline__ const int8_t *nw_load_smat(const NwParams &p)
+{
+    extern __shared__ __align__(16) unsigned char nw_lds[];
+    int8_t *smat = reinterpret_cast<int8_t *>(nw_lds);
+    if (threadIdx.x < 32) smat[threadIdx.x] = p.mat[threadIdx.x];
+    __syncthreads();
+    return smat;
+}
+
+// Persistent work-stealing kernel (items ordered by decreasing area, taken
+// from an atomic counter).  The plain form runs the same body over items in
+// input order with a static round-robin assignment of items to blocks.
+template <int T, bool GLOBAL>
+__global__ __launch_bounds__(64) void nw_kernel(NwDev d, NwParams p, const uint32_t *order,
+                                                const WorkItem *items, int n_items,
+                                                uint32_t *counter, int S, uint32_t *zbase,
+                                                int zrows, int ncw, int persistent, void *gstate)
+{
+    using Wd = typename NwSt<T>::W;
+    extern __shared__ __align__(16) unsigned char nw_lds[];
+    const int8_t *smat = nw_load_smat(p);
+    Wd *slots = GLOBAL ? reinterpret_cast<Wd *>(gstate) + (size_t)blockIdx.x * S * 64
+                       : reinterpret_cast<Wd *>(nw_lds + 32);
+    uint32_t *zreg = zbase + (size_t)blockIdx.x * zrows * ncw * 64;
+    __shared__ uint32_t idx[64];
+    __shared__ int item_sh;
+    int it = blockIdx.x;
+    for (;;) {
+        if (persistent) {
+            if (threadIdx.x == 0) item_sh = (int)atomicAdd(counter, 1u);
+            __syncthreads();
+            it = item_sh;
+        }
+        if (it >= n_items) break;
+        const WorkItem wi = items[it];
+        if (threadIdx.x < wi.count) idx[threadIdx.x] = order[wi.start + threadIdx.x];
+        __syncthreads();
+        nw_wave<T>(d, p, idx, (int)wi.count, slots, S, zreg, zrows, ncw, smat);
+        __syncthreads();
+        if (!persistent) it += gridDim.x;      // plain form: static round-robin
+    }
+}
+
+size_t nw_lds_bytes(int tier, bool global_state, int S)
+{
+    const size_t w = tier == NW_I16 ? 4 : 8;
+    return 32 + (global_state ? 0 : (size_t)S * 64 * w);
+}
+
+template <int T, bool G>
+static hipError_t nw_t(const NwDev &d, const NwParams &p, const uint32_t *order,
+                       const WorkItem *items, int n_items, uint32_t *counter, int S,
+                       uint32_t *zbase, int zrows, int ncw, int grid, bool persistent,
+                       void *gstate, hipStream_t s)
+{
+    hipLaunchKernelGGL((nw_kernel<T, G>), dim3(grid), dim3(64), nw_lds_bytes(T, G, S), s, d, p,
+                       order, items, n_items, counter, S, zbase, zrows, ncw,
+                       persistent ? 1 : 0, gstate);
+    return hipGetLastError();
+}
+
+hipError_t launch_nw(int tier, bool global_state, const NwDev &d, const NwParams &p,
+                     const uint32_t *order, const WorkItem *items, int n_items,
+                     uint32_t *counter, int S, uint32_t *zbase, int zrows, int ncw, int grid,
+                     bool persistent, void *gstate, hipStream_t s)
+{
+    if (n_items <= 0) return hipSuccess;
+    switch (tier * 2 + (global_state ? 1 : 0)) {
+    case 0: return nw_t<NW_I16, false>(d, p, order, items, n_items, counter, S, zbase, zrows, ncw, grid, persistent, gstate, s);
+    case 1: return nw_t<NW_I16, true>(d, p, order, items, n_items, counter, S, zbase, zrows, ncw, grid, persistent, gstate, s);
+    case 2: return nw_t<NW_I32, false>(d, p, order, items, n_items, counter, S, zbase, zrows, ncw, grid, persistent, gstate, s);
+    default: return nw_t<NW_I32, true>(d, p, order, items, n_items, counter, S, zbase, zrows, ncw, grid, persistent, gstate, s);
+    }
+}
+
+}  // namespace hipsw

@@ -1,0 +1,191 @@
+"""Host mirror of swiftbwa's MetalSW call surface, backed by libhipsw.
+
+Names, argument meaning and failure signalling follow the Swift types:
+  ScoringParameters   Sources/BWACore/ScoringParameters.swift:2-73
+  SWResult            Sources/BWACore/SWResult.swift:2-25
+  BandedSWTask        Sources/MetalSW/BandedSWDispatcher.swift:6-20 (+ endBonus)
+  BandedSWDispatcher  Sources/MetalSW/BandedSWDispatcher.swift:23-158
+  GlobalSWTask/...    Sources/MetalSW/GlobalSWDispatcher.swift:6-118
+  LocalSWTask/...     Sources/MetalSW/LocalSWDispatcher.swift:6-95
+  HipSWEngine.shared  Sources/MetalSW/MetalSWEngine.swift:8-17 (None = no GPU)
+Failure is signalled the way MetalSW signals it: dispatchBatch16 and the
+global/local dispatchers return [] and the caller keeps its CPU path.
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import native
+
+
+@dataclass
+class ScoringParameters:
+    matchScore: int = 1
+    mismatchPenalty: int = 4
+    gapOpenPenalty: int = 6
+    gapExtendPenalty: int = 1
+    gapOpenPenaltyDeletion: int = 6
+    gapExtendPenaltyDeletion: int = 1
+    bandWidth: int = 100
+    zDrop: int = 100
+    minSeedLength: int = 19
+    penClip5: int = 5
+    penClip3: int = 5
+    maxMatesw: int = 50
+
+    def scoringMatrix(self) -> List[int]:
+        """ScoringParameters.swift:65-73: A on the diagonal, -B off it, -1 for N."""
+        mat = [-1] * 25
+        for i in range(4):
+            for j in range(4):
+                mat[i * 5 + j] = self.matchScore if i == j else -self.mismatchPenalty
+        return mat
+
+    def to_c(self) -> native.HipswScoring:
+        s = native.HipswScoring()
+        for i, v in enumerate(self.scoringMatrix()):
+            s.mat[i] = v
+        s.o_ins, s.e_ins = self.gapOpenPenalty, self.gapExtendPenalty
+        s.o_del, s.e_del = self.gapOpenPenaltyDeletion, self.gapExtendPenaltyDeletion
+        s.zdrop = self.zDrop
+        return s
+
+
+@dataclass(frozen=True)
+class SWResult:
+    score: int = 0
+    queryEnd: int = 0
+    targetEnd: int = 0
+    globalTargetEnd: int = 0
+    globalScore: int = 0
+    maxOff: int = 0
+
+
+@dataclass
+class BandedSWTask:
+    query: bytes
+    target: bytes
+    h0: int
+    w: int
+    scoring: ScoringParameters = field(default_factory=ScoringParameters)
+    endBonus: int = 0
+
+
+class HipSWEngine:
+    """Singleton GPU engine; ``HipSWEngine.shared()`` is None without a device."""
+    minBatchSize = 32
+    _shared = None
+    _tried = False
+
+    def __init__(self, device=0):
+        self.lib = native.load()
+        self.handle = self.lib.hipsw_engine_create(int(device))
+        if not self.handle:
+            raise RuntimeError("hipsw_engine_create failed (no HIP device %d)" % device)
+
+    @classmethod
+    def shared(cls) -> Optional["HipSWEngine"]:
+        if not cls._tried:
+            cls._tried = True
+            try:
+                cls._shared = HipSWEngine(0)
+            except RuntimeError:
+                cls._shared = None
+        return cls._shared
+
+    def last_error(self) -> str:
+        return (self.lib.hipsw_last_error(self.handle) or b"").decode()
+
+    def device_info(self):
+        buf = C.create_string_buffer(256)
+        ncu = C.c_int32()
+        self.lib.hipsw_device_info(self.handle, buf, 256, C.byref(ncu))
+        return buf.value.decode(), ncu.value
+
+    def close(self):
+        if self.handle:
+            self.lib.hipsw_engine_destroy(self.handle)
+            self.handle = None
+
+    # ---- array-level entry points (what the Swift shim calls through the C ABI) ----
+    def banded_extend(self, jobs, scoring: ScoringParameters, mode=native.MODE_BATCHED,
+                      stats=None):
+        """Runs hipsw_banded_extend on a job dict; returns int32[n, 6]."""
+        n = int(jobs["n"])
+        out = np.zeros((n, 6), dtype=np.int32)
+        if n == 0:
+            return out
+        a = _ext_arrays(jobs)
+        sc = scoring.to_c()
+        st = native.HipswStats()
+        rc = self.lib.hipsw_banded_extend(
+            self.handle, n, native.ptr(a["queries"]), native.ptr(a["q_off"]),
+            native.ptr(a["q_len"]), native.ptr(a["targets"]), native.ptr(a["t_off"]),
+            native.ptr(a["t_len"]), native.ptr(a["h0"]), native.ptr(a["w"]),
+            native.ptr(a["end_bonus"]), C.byref(sc), int(mode), native.ptr(out), C.byref(st))
+        if rc != native.HIPSW_OK:
+            raise native.HipswError(rc, self.last_error())
+        if stats is not None:
+            stats.update(st.as_dict())
+        return out
+
+
+def _ext_arrays(jobs):
+    def c(name, dt):
+        return np.ascontiguousarray(jobs[name], dtype=dt)
+    a = {"queries": c("queries", np.uint8), "q_off": c("q_off", np.uint32),
+         "q_len": c("q_len", np.uint16), "targets": c("targets", np.uint8),
+         "t_off": c("t_off", np.uint32), "t_len": c("t_len", np.uint16),
+         "h0": c("h0", np.int32), "w": c("w", np.int32), "end_bonus": c("end_bonus", np.int32)}
+    if a["queries"].size == 0:
+        a["queries"] = np.zeros(1, np.uint8)
+    if a["targets"].size == 0:
+        a["targets"] = np.zeros(1, np.uint8)
+    return a
+
+
+def tasks_to_jobs(tasks):
+    """Packs BandedSWTask objects the way QueryTargetPacker does
+    (MetalDispatchHelper.swift:30-46)."""
+    n = len(tasks)
+    ql = np.array([len(t.query) for t in tasks], dtype=np.uint16)
+    tl = np.array([len(t.target) for t in tasks], dtype=np.uint16)
+    qo = np.zeros(n, np.uint32)
+    to = np.zeros(n, np.uint32)
+    if n:
+        qo[1:] = np.cumsum(ql.astype(np.int64))[:-1]
+        to[1:] = np.cumsum(tl.astype(np.int64))[:-1]
+    q = np.frombuffer(b"".join(bytes(t.query) for t in tasks), dtype=np.uint8)
+    t = np.frombuffer(b"".join(bytes(t.target) for t in tasks), dtype=np.uint8)
+    return {"n": n, "queries": q, "q_off": qo, "q_len": ql, "targets": t, "t_off": to,
+            "t_len": tl, "h0": np.array([t.h0 for t in tasks], np.int32),
+            "w": np.array([t.w for t in tasks], np.int32),
+            "end_bonus": np.array([t.endBonus for t in tasks], np.int32)}
+
+
+class BandedSWDispatcher:
+    """BandedSWDispatcher.swift:23-158.  Both entry points return exact
+    BandedSWScalar results for every task; the engine widens the score type
+    itself, so dispatchBatch8 never reports an 8-bit overflow (nil)."""
+
+    @staticmethod
+    def dispatchBatch8(tasks: List[BandedSWTask], engine: HipSWEngine) -> List[Optional[SWResult]]:
+        res = BandedSWDispatcher.dispatchBatch16(tasks, engine)
+        if not res and tasks:
+            return [None] * len(tasks)
+        return res
+
+    @staticmethod
+    def dispatchBatch16(tasks: List[BandedSWTask], engine: HipSWEngine) -> List[SWResult]:
+        if not tasks:
+            return []
+        sc = tasks[0].scoring
+        if any(t.scoring != sc for t in tasks):
+            return []      # one scoring per batch, as the Metal kernels take it
+        try:
+            out = engine.banded_extend(tasks_to_jobs(tasks), sc)
+        except native.HipswError:
+            return []
+        return [SWResult(*map(int, row)) for row in out]

@@ -1,0 +1,96 @@
+"""ctypes binding of libhipsw.so, the C ABI declared in include/hipsw.h.
+
+The library is built in-tree by ``swiftbwa_amd.build``.  Loading fails loudly
+when it is missing: there is no CPU fallback inside this package (callers that
+want one keep their own CPU path, as swiftbwa does when MetalSW is nil).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhipsw.so")
+
+HIPSW_OK = 0
+HIPSW_E_INVALID = -1
+HIPSW_E_DEVICE = -2
+HIPSW_E_UNSUPPORTED = -3
+HIPSW_E_BUDGET = -4
+HIPSW_E_PENDING = -5
+
+MODE_PLAIN = 0
+MODE_BATCHED = 1
+
+
+class HipswScoring(C.Structure):
+    _fields_ = [("mat", C.c_int8 * 25), ("o_ins", C.c_int32), ("e_ins", C.c_int32),
+                ("o_del", C.c_int32), ("e_del", C.c_int32), ("zdrop", C.c_int32)]
+
+
+class HipswStats(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("total_ms", C.c_double), ("n_jobs", C.c_int64),
+                ("launches", C.c_int64), ("promoted", C.c_int64), ("tier_jobs", C.c_int64 * 3),
+                ("cells", C.c_int64)]
+
+    def as_dict(self):
+        return {"kernel_ms": self.kernel_ms, "total_ms": self.total_ms, "n_jobs": self.n_jobs,
+                "launches": self.launches, "promoted": self.promoted,
+                "tier_jobs": list(self.tier_jobs), "cells": self.cells}
+
+
+# Every symbol include/hipsw.h declares, with (restype, argtypes).
+_P = C.c_void_p
+_I32P = C.POINTER(C.c_int32)
+_SIGS = {
+    "hipsw_engine_create": (_P, [C.c_int]),
+    "hipsw_engine_destroy": (None, [_P]),
+    "hipsw_last_error": (C.c_char_p, [_P]),
+    "hipsw_min_batch_size": (C.c_int32, []),
+    "hipsw_device_info": (C.c_int, [_P, C.c_char_p, C.c_int, _I32P]),
+    "hipsw_banded_extend": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                      C.POINTER(HipswScoring), C.c_int32, _P,
+                                      C.POINTER(HipswStats)]),
+    "hipsw_ext_upload": (_P, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                              C.POINTER(HipswScoring)]),
+    "hipsw_ext_run": (C.c_int, [_P, _P, C.c_int32, C.POINTER(HipswStats)]),
+    "hipsw_ext_download": (C.c_int, [_P, _P, _P]),
+    "hipsw_ext_free": (None, [_P, _P]),
+}
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Loads libhipsw.so and binds every declared symbol; raises if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            "libhipsw.so is not built (%s); run `python -m swiftbwa_amd.build`" % path)
+    lib = C.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def declared_symbols():
+    return sorted(_SIGS)
+
+
+def ptr(a):
+    """Address of a contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+class HipswError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("hipsw status %d: %s" % (status, msg))
+        self.status = status

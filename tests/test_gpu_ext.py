@@ -1,0 +1,143 @@
+"""GPU parity of the banded-extension kernels against the CPU oracle
+(BandedSWScalar.align restated in oracle/sw_oracle.c).  Bit-exact on all six
+SWResult fields and on the DP cell count."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as orc
+from swiftbwa_amd import native, workload as wl
+from swiftbwa_amd.metalsw import (BandedSWDispatcher, BandedSWTask, ScoringParameters,
+                                  SWResult)
+
+pytestmark = pytest.mark.gpu
+
+MODES = [native.MODE_PLAIN, native.MODE_BATCHED]
+
+
+def oracle_sc(sp: ScoringParameters):
+    return orc.scoring(a=sp.matchScore, b=sp.mismatchPenalty, o_ins=sp.gapOpenPenalty,
+                       e_ins=sp.gapExtendPenalty, o_del=sp.gapOpenPenaltyDeletion,
+                       e_del=sp.gapExtendPenaltyDeletion, zdrop=sp.zDrop)
+
+
+def check(engine, js, sp=None, modes=MODES):
+    sp = sp or ScoringParameters()
+    ref, cells = orc.ksw_extend_batch(js, oracle_sc(sp), nthreads=8)
+    for mode in modes:
+        st = {}
+        got = engine.banded_extend(js, sp, mode=mode, stats=st)
+        bad = np.nonzero((got != ref).any(axis=1))[0]
+        assert len(bad) == 0, "mode %d: %d mismatches, first job %d: gpu %s oracle %s" % (
+            mode, len(bad), bad[0], got[bad[0]].tolist(), ref[bad[0]].tolist())
+        assert st["cells"] == cells, (mode, st["cells"], cells)
+    return ref
+
+
+def test_known_answers_through_dispatcher(engine):
+    # AlignmentTests.swift:48-63, :82-96, :115-136, :208-237 (closed forms)
+    sp = ScoringParameters()
+    tasks = [BandedSWTask(bytes([0, 1, 2, 3, 0, 1]), bytes([0, 1, 2, 3, 0, 1]), 1, 10, sp),
+             BandedSWTask(bytes([0, 1, 2]), bytes([0, 1, 2]), 5, 10, sp),
+             BandedSWTask(bytes([0, 1, 2, 3] * 2), bytes([0, 1, 2, 3] * 2), 1, 10, sp),
+             BandedSWTask(bytes([0, 1, 2, 3, 0, 1, 2, 3, 0, 1]),
+                          bytes([0, 1, 2, 3, 3, 1, 2, 3, 0, 1]), 1, 10, sp)]
+    res = BandedSWDispatcher.dispatchBatch16(tasks, engine)
+    assert [r.score for r in res] == [7, 8, 9, 6]
+    # z-drop case (AlignmentTests.swift:239-283): score 6, targetEnd < 50
+    sz = ScoringParameters(zDrop=10)
+    t = BandedSWTask(bytes([0, 1, 2, 3, 0] + [0] * 100), bytes([0, 1, 2, 3, 0] + [3] * 100), 1, 5, sz)
+    r = BandedSWDispatcher.dispatchBatch8([t], engine)[0]
+    assert r.score == 6 and r.targetEnd < 50
+
+
+def test_empty_tasks(engine):
+    sp = ScoringParameters()
+    tasks = [BandedSWTask(b"", bytes([0, 1]), 5, 10, sp), BandedSWTask(bytes([0, 1]), b"", 5, 10, sp),
+             BandedSWTask(bytes([0, 1]), bytes([0, 1]), 5, 10, sp)]
+    res = BandedSWDispatcher.dispatchBatch16(tasks, engine)
+    assert res[0] == SWResult() and res[1] == SWResult()
+    assert res[2].score == 7
+
+
+def test_uniform_subset(engine):
+    check(engine, wl.generate(wl.EXT_UNIFORM, 2, 20000))
+
+
+def test_mixed_subset(engine):
+    check(engine, wl.generate(wl.EXT_MIXED, 4, 30000))
+
+
+def _random_jobs(rng, n, qmax, tmax, n_rate=0.0, h0_range=(1, 60), w_choices=(1, 5, 20, 100)):
+    qs, ts, ql, tl, h0, w, eb = [], [], [], [], [], [], []
+    for _ in range(n):
+        q = rng.integers(0, 4, rng.integers(0, qmax + 1)).astype(np.uint8)
+        t = rng.integers(0, 4, rng.integers(0, tmax + 1)).astype(np.uint8)
+        # make targets similar to queries half of the time
+        if len(q) and len(t) and rng.random() < 0.5:
+            m = min(len(q), len(t))
+            t[:m] = q[:m]
+            flip = rng.random(m) < 0.05
+            t[:m][flip] = rng.integers(0, 4, flip.sum())
+        if n_rate:
+            q[rng.random(len(q)) < n_rate] = 4
+            t[rng.random(len(t)) < n_rate] = 4
+        qs.append(q); ts.append(t)
+        ql.append(len(q)); tl.append(len(t))
+        h0.append(int(rng.integers(h0_range[0], h0_range[1] + 1)))
+        w.append(int(rng.choice(w_choices)))
+        eb.append(int(rng.integers(0, 8)))
+    qo = np.zeros(n, np.uint32); to = np.zeros(n, np.uint32)
+    qo[1:] = np.cumsum(ql)[:-1]; to[1:] = np.cumsum(tl)[:-1]
+    cat = lambda xs: np.concatenate(xs) if sum(map(len, xs)) else np.zeros(1, np.uint8)
+    return {"n": n, "queries": cat(qs), "targets": cat(ts), "q_off": qo, "t_off": to,
+            "q_len": np.array(ql, np.uint16), "t_len": np.array(tl, np.uint16),
+            "h0": np.array(h0, np.int32), "w": np.array(w, np.int32),
+            "end_bonus": np.array(eb, np.int32)}
+
+
+def test_random_small_with_n(engine):
+    rng = np.random.default_rng(11)
+    check(engine, _random_jobs(rng, 3000, 40, 60, n_rate=0.05))
+
+
+def test_int32_tier_and_negative_h0(engine):
+    rng = np.random.default_rng(12)
+    js = _random_jobs(rng, 1500, 80, 120, h0_range=(-20, 70000))
+    check(engine, js)
+
+
+def test_long_queries_global_state(engine):
+    # qlen up to 700 pushes the u16 state past the per-block LDS limit
+    rng = np.random.default_rng(13)
+    check(engine, _random_jobs(rng, 300, 700, 900, h0_range=(20, 300), w_choices=(50, 100, 400)))
+
+
+def test_scoring_variants(engine):
+    rng = np.random.default_rng(14)
+    js = _random_jobs(rng, 2000, 100, 150)
+    for sp in (ScoringParameters(zDrop=0), ScoringParameters(matchScore=2, mismatchPenalty=3),
+               ScoringParameters(gapOpenPenalty=3, gapExtendPenalty=2, gapOpenPenaltyDeletion=8,
+                                 gapExtendPenaltyDeletion=3, zDrop=30)):
+        check(engine, js, sp)
+
+
+def test_full_size_uniform_properties(engine):
+    # 10^6 jobs: invariants that hold job by job at full size, plus
+    # plain == batched (two different launch shapes and orders).
+    js = wl.generate(wl.EXT_UNIFORM, 2, 1000000)
+    sp = ScoringParameters()
+    st_a, st_b = {}, {}
+    a = engine.banded_extend(js, sp, mode=native.MODE_PLAIN, stats=st_a)
+    b = engine.banded_extend(js, sp, mode=native.MODE_BATCHED, stats=st_b)
+    assert np.array_equal(a, b)
+    assert st_a["cells"] == st_b["cells"]
+    ql = js["q_len"].astype(np.int32); tl = js["t_len"].astype(np.int32)
+    assert (a[:, 0] >= js["h0"]).all()
+    assert ((a[:, 1] >= 0) & (a[:, 1] <= ql)).all()
+    assert ((a[:, 2] >= 0) & (a[:, 2] <= tl)).all()
+    assert ((a[:, 4] == -1) == (a[:, 3] == 0)).all()
+    # a random sample checked against the oracle
+    idx = np.random.default_rng(5).choice(js["n"], 5000, replace=False)
+    sub = wl.slice_jobs(js, np.sort(idx))
+    ref, _ = orc.ksw_extend_batch(sub, nthreads=8)
+    assert np.array_equal(a[np.sort(idx)], ref)
