@@ -49,13 +49,57 @@ def cpu_baseline(js, sample):
             "ext_per_s": round(len(idx) / dt, 1)}
 
 
+def bench_nw(eng, args):
+    """Banded global NW + traceback: 500,000 pairs, qlen U[50,250], target =
+    query +- a planted 0..20-bp indel, 1% subs / 0.2% indels; w = min(100,
+    2|dlen|+3); score + CIGAR per pair, retries while score < planted score."""
+    import ctypes as C
+    from swiftbwa_amd import native, workload as wl
+    from swiftbwa_amd.metalsw import ScoringParameters, _ext_arrays
+    lib = eng.lib
+    js = wl.generate(wl.NW, 3, args.nw_n)
+    a = _ext_arrays(js)
+    ts = np.ascontiguousarray(js["true_score"], np.int32)
+    sc = ScoringParameters().to_c()
+    st = lib.hipsw_nw_upload(eng.handle, js["n"], native.ptr(a["queries"]), native.ptr(a["q_off"]),
+                             native.ptr(a["q_len"]), native.ptr(a["targets"]),
+                             native.ptr(a["t_off"]), native.ptr(a["t_len"]), native.ptr(a["w"]),
+                             native.ptr(ts), 3, 512, C.byref(sc))
+    if not st:
+        raise RuntimeError(eng.last_error())
+    res = {}
+    for name, mode in (("plain", native.MODE_PLAIN), ("batched", native.MODE_BATCHED)):
+        s = native.HipswStats()
+        for _ in range(args.warmup):
+            if lib.hipsw_nw_run(eng.handle, st, mode, C.byref(s)):
+                raise RuntimeError(eng.last_error())
+        kms = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            if lib.hipsw_nw_run(eng.handle, st, mode, C.byref(s)):
+                raise RuntimeError(eng.last_error())
+            kms.append(s.kernel_ms)
+        wall = (time.perf_counter() - t0) * 1000.0 / args.steps
+        res[name] = {"ms_per_step": round(wall, 3), "kernel_ms": round(float(np.mean(kms)), 3),
+                     "gcups": round(s.cells / (wall * 1e-3) / 1e9, 2),
+                     "kernel_gcups": round(s.cells / (np.mean(kms) * 1e-3) / 1e9, 2),
+                     "launches": int(s.launches)}
+        res["cells"] = int(s.cells)
+        log("nw %s: %.2f ms/step (kernel %.2f ms)" % (name, wall, np.mean(kms)))
+    lib.hipsw_nw_free(eng.handle, st)
+    res["pairs"] = js["n"]
+    res["batched_over_plain"] = round(res["plain"]["ms_per_step"] / res["batched"]["ms_per_step"], 3)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1000000)
-    ap.add_argument("--cpu-sample", type=int, default=60000)
+    ap.add_argument("--cpu-sample", type=int, default=600000)
+    ap.add_argument("--nw-n", type=int, default=500000)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -113,6 +157,8 @@ def main():
     lib.hipsw_ext_download(eng.handle, st, native.ptr(out))
     lib.hipsw_ext_free(eng.handle, st)
 
+    nw = bench_nw(eng, args) if args.nw_n > 0 else None
+
     gcups = cells / (b_wall * 1e-3) / 1e9
     achieved = cells * OPS_PER_CELL / (b_kms * 1e-3) / 1e12
     rec = {
@@ -140,6 +186,7 @@ def main():
             "tier_jobs": list(b_st.tier_jobs),
             "upload_s_incl_pack": round(upload_s, 3),
             "device": dev_name, "n_cu": n_cu,
+            "nw_cigar": nw,
         },
         "roofline": {
             "bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_PK16_TOPS,

@@ -96,6 +96,12 @@ ExtParams make_params(const hipsw_scoring *sc)
     p.oe_del = sc->o_del + sc->e_del;
     p.oe_ins = sc->o_ins + sc->e_ins;
     p.max_mat = max_mat;
+    p.match_sc = sc->mat[0];
+    p.mis_sc = sc->mat[1];
+    p.std_mat = 1;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j)
+            if (sc->mat[i * 5 + j] != (i == j ? p.match_sc : p.mis_sc)) p.std_mat = 0;
     return p;
 }
 
@@ -183,6 +189,25 @@ bool pack_jobs(int n, const uint8_t *queries, const uint32_t *q_off, const uint1
         return false;
     }
     return true;
+}
+
+void radix_sort_by_key(std::vector<uint32_t> &keys, std::vector<uint32_t> &idx)
+{
+    const size_t n = keys.size();
+    std::vector<uint32_t> k2(n), i2(n);
+    for (int shift = 0; shift < 32; shift += 16) {
+        std::vector<uint32_t> cnt(65537, 0);
+        for (size_t a = 0; a < n; ++a) cnt[((keys[a] >> shift) & 0xFFFFu) + 1]++;
+        for (size_t b = 1; b < cnt.size(); ++b) cnt[b] += cnt[b - 1];
+        for (size_t a = 0; a < n; ++a) {
+            const uint32_t d = (keys[a] >> shift) & 0xFFFFu;
+            const uint32_t pos = cnt[d]++;
+            k2[pos] = keys[a];
+            i2[pos] = idx[a];
+        }
+        keys.swap(k2);
+        idx.swap(i2);
+    }
 }
 
 }  // namespace hipsw

@@ -59,6 +59,9 @@ bool pack_jobs(int n, const uint8_t *queries, const uint32_t *q_off, const uint1
                const uint8_t *targets, const uint32_t *t_off, const uint16_t *t_len,
                PackedJobs &pj, std::string &err);
 
+// Stable LSD radix sort of idx by 32-bit keys (two 16-bit passes).
+void radix_sort_by_key(std::vector<uint32_t> &keys, std::vector<uint32_t> &idx);
+
 // One persistent launch of the batched extension form.
 struct ExtLaunch {
     int tier;

@@ -59,10 +59,11 @@ __device__ __forceinline__ int wave_max(int v)
     return v;
 }
 
-// Extends the (<= 64) jobs idx[0..count) with one wave.  `col` points at this
-// lane's column-0 state word; column j is at col[j * 64].
+// Generic form: any 5x5 matrix, N bases, every tier.  Extends the (<= 64) jobs
+// idx[0..count) with one wave.  `col` points at this lane's column-0 state
+// word; column j is at col[j * 64].
 template <int TIER>
-__device__ __forceinline__ void ext_wave(const ExtDev &d, const ExtParams &p,
+__device__ __forceinline__ void ext_wave_generic(const ExtDev &d, const ExtParams &p,
                                          const uint32_t *idx, int count,
                                          typename St<TIER>::W *col, const int8_t *smat)
 {
@@ -202,8 +203,205 @@ __device__ __forceinline__ void ext_wave(const ExtDev &d, const ExtParams &p,
     }
 }
 
+
+// Fast form for the common case -- no N base in any of the wave's jobs, the
+// standard A/-B matrix, 8- or 16-bit tier.  Same recurrence as the generic
+// form; what changes is how a row is swept:
+//  - each lane's query is staged once per job in LDS as 2-bit words
+//    (qlds[b*64 + lane] = bases 16b..16b+15);
+//  - per 16-column block the match bits of the whole block are one XOR with
+//    the row's target base replicated 16 times;
+//  - the row range [jlo, jhi) is split into masked edges and an unmasked
+//    middle of whole 16-column blocks where every live lane is inside its band
+//    ([max beg, min end)); the middle is fully unrolled with no exec-mask
+//    changes.  Lanes already finished compute there too, harmlessly: they only
+//    touch their own LDS column state, which is re-initialised per job.
+template <int TIER>
+__device__ __forceinline__ void ext_wave_fast(const ExtDev &d, const ExtParams &p,
+                                              const uint32_t *idx, int count,
+                                              typename St<TIER>::W *col, uint32_t *qlds)
+{
+    using S = St<TIER>;
+    using W = typename S::W;
+    const int lane = threadIdx.x & 63;
+    const bool valid = lane < count;
+    const int job = valid ? (int)idx[lane] : 0;
+    const int qlen = valid ? d.q_len[job] : 0;
+    const int tlen = valid ? d.t_len[job] : 0;
+    const int h0 = valid ? d.h0[job] : 0;
+    int w = valid ? d.w[job] : 0;
+    const int eb = valid ? d.eb[job] : 0;
+    const uint32_t qw0 = valid ? d.q_word[job] : 0;
+    const uint32_t tw0 = valid ? d.t_word[job] : 0;
+    const bool nonempty = valid && qlen > 0 && tlen > 0;     // :33-35
+    bool alive = nonempty;
+    const int A = p.match_sc, MIS = p.mis_sc;
+    const int e_del = p.e_del, e_ins = p.e_ins, oe_del = p.oe_del, oe_ins = p.oe_ins;
+
+    if (alive) {                                               // w cap, :52-62
+        int mi = (int)((double)(qlen * p.max_mat + eb - p.o_ins) / (double)e_ins + 1.0);
+        w = min(w, max(1, mi));
+        int md = (int)((double)(qlen * p.max_mat + eb - p.o_del) / (double)e_del + 1.0);
+        w = min(w, max(1, md));
+    }
+    const int qmax = wave_max(alive ? qlen : 0);
+    // stage the query's 2-bit words
+    const int qblocks = (qmax + 15) >> 4;
+    uint32_t *ql = qlds + lane;
+    for (int b = 0; b < qblocks; ++b)
+        ql[b * 64] = (alive && b * 16 < qlen) ? d.seq[qw0 + b] : 0u;
+    // first row (:68-77)
+    for (int j = 0; j <= qmax; ++j) {
+        if (alive && j <= qlen) {
+            const int hv = j == 0 ? h0 : max(0, h0 - p.o_ins - e_ins * j);
+            col[j * 64] = S::pack(hv, 0);
+        }
+    }
+
+    int beg = 0, end = qlen;                                   // :79-87
+    int maxv = h0, max_i = -1, max_j = -1;
+    int gsc = -1, gtle = -1, max_off = 0;
+    uint32_t tw = 0;
+    uint64_t ncells = 0;
+
+    for (int i = 0;; ++i) {
+        if (alive && i >= tlen) alive = false;
+        if (__ballot(alive) == 0) break;
+        if ((i & 15) == 0 && alive) tw = d.seq[tw0 + (i >> 4)];
+        const uint32_t t = (tw >> ((i & 15) * 2)) & 3u;
+        const uint32_t tpat = t * 0x55555555u;
+
+        int f = 0, h1 = 0;
+        uint32_t key = 0;                 // (rowMax << 16) | rowMaxJ, last j wins ties
+        if (alive) {                                           // :95-107
+            beg = max(beg, i - w);
+            end = min(end, i + w + 1);
+            end = min(end, qlen);
+            h1 = beg == 0 ? max(0, h0 - (p.o_del + e_del * (i + 1))) : 0;
+            if (end > beg) ncells += (uint64_t)(end - beg);
+        }
+        const int jlo = wave_min(alive ? beg : 0x7fffffff);
+        const int jhi = wave_max(alive ? end : 0);
+        const int mlo = (wave_max(alive ? beg : 0) + 15) & ~15;   // middle: whole blocks
+        const int mhi = wave_min(alive ? end : 0x7fffffff) & ~15;
+        const bool has_mid = mlo < mhi && mlo >= jlo && mhi <= jhi;
+        const int e1 = has_mid ? mlo : jhi;                   // masked [jlo, e1)
+        const int s2 = has_mid ? mhi : jhi;                   // masked [s2, jhi)
+
+        // one cell with an explicit in-band test (edges of the row)
+        auto masked_cell = [&](int j) {
+            const bool inb = alive && (unsigned)(j - beg) < (unsigned)(end - beg);
+            const uint32_t qw = ql[(j >> 4) * 64];
+            const uint32_t q = (qw >> ((j & 15) * 2)) & 3u;
+            const W wd = col[j * 64];
+            const int M = S::h(wd), E = S::e(wd);
+            const int sc = q == t ? A : MIS;
+            const int diag = M != 0 ? M + sc : 0;
+            const int H = max(max(diag, E), f);
+            const uint32_t kk = ((uint32_t)H << 16) | (uint32_t)j;
+            const int ne = max(max(E - e_del, diag - oe_del), 0);
+            const int nf = max(max(f - e_ins, diag - oe_ins), 0);
+            col[j * 64] = inb ? S::pack(h1, ne) : wd;
+            key = inb ? max(key, kk) : key;
+            f = inb ? nf : f;
+            h1 = inb ? H : h1;
+        };
+        for (int j = jlo; j < e1; ++j) masked_cell(j);
+        if (has_mid) {
+            for (int b = mlo >> 4; b < (mhi >> 4); ++b) {
+                const uint32_t x = ql[b * 64] ^ tpat;
+                const uint32_t mb = ~(x | (x >> 1)) & 0x55555555u;   // bit 2k: base k matches
+                W *cb = col + b * 16 * 64;
+                const uint32_t jb = (uint32_t)(b * 16);
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const W wd = cb[k * 64];
+                    const int M = S::h(wd), E = S::e(wd);
+                    const int sc = ((mb >> (2 * k)) & 1u) ? A : MIS;
+                    const int diag = M != 0 ? M + sc : 0;
+                    const int H = max(max(diag, E), f);
+                    key = max(key, ((uint32_t)H << 16) | (jb + k));
+                    const int ne = max(max(E - e_del, diag - oe_del), 0);
+                    f = max(max(f - e_ins, diag - oe_ins), 0);
+                    cb[k * 64] = S::pack(h1, ne);
+                    h1 = H;
+                }
+            }
+        }
+        for (int j = s2; j < jhi; ++j) masked_cell(j);
+
+        if (alive) {
+            const int row_max = (int)(key >> 16);
+            const int row_max_j = (int)(key & 0xFFFFu);
+            col[end * 64] = S::pack(h1, 0);                    // :145-146
+            if (end == qlen) {                                 // :149-152
+                gtle = gsc > h1 ? gtle : i;
+                gsc = max(gsc, h1);
+            }
+            if (row_max == 0) {                                // :155
+                alive = false;
+            } else if (row_max > maxv) {                       // :158-163
+                maxv = row_max;
+                max_i = i;
+                max_j = row_max_j;
+                max_off = max(max_off, abs(row_max_j - i));
+            } else if (p.zdrop > 0) {                          // :164-172
+                const int di = i - max_i, dj = row_max_j - max_j;
+                const int drop = di > dj ? maxv - row_max - (di - dj) * e_del
+                                         : maxv - row_max - (dj - di) * e_ins;
+                if (drop > p.zdrop) alive = false;
+            }
+            if (alive) {                                       // :175-180
+                int nb = beg;
+                while (nb < end && S::zero(col[nb * 64])) ++nb;
+                beg = nb;
+                int ne = end;
+                while (ne >= beg && S::zero(col[ne * 64])) --ne;
+                end = min(ne + 2, qlen);
+            }
+        }
+    }
+    if (d.cells) {
+        unsigned long long c = ncells;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+        if (lane == 0) atomicAdd(d.cells, c);
+    }
+    if (valid) {                                               // :183-190
+        int32_t *o = d.out + (size_t)job * 6;
+        if (nonempty) {
+            o[0] = maxv; o[1] = max_j + 1; o[2] = max_i + 1;
+            o[3] = gtle + 1; o[4] = gsc; o[5] = max_off;
+        } else {
+            o[0] = 0; o[1] = 0; o[2] = 0; o[3] = 0; o[4] = 0; o[5] = 0;
+        }
+    }
+}
+
+// Picks the fast or the generic form for this wave's jobs (uniform branch).
+template <int TIER>
+__device__ __forceinline__ void ext_wave(const ExtDev &d, const ExtParams &p,
+                                         const uint32_t *idx, int count,
+                                         typename St<TIER>::W *col, const int8_t *smat,
+                                         uint32_t *qlds)
+{
+    if (TIER != TIER_I32 && p.std_mat) {
+        const int lane = threadIdx.x & 63;
+        const bool valid = lane < count;
+        const int job = valid ? (int)idx[lane] : 0;
+        const bool has_n = valid && (d.qn_word[job] != kNoN || d.tn_word[job] != kNoN);
+        if (__ballot(has_n) == 0) {
+            ext_wave_fast<TIER>(d, p, idx, count, col, qlds);
+            return;
+        }
+    }
+    ext_wave_generic<TIER>(d, p, idx, count, col, smat);
+}
+
 // Per-wave state pointer: LDS when the columns fit, else a per-block slice of
 // a global scratch buffer (cols * 64 words per block).
+__device__ __forceinline__ int query_blocks(int cols) { return (cols - 1 + 15) >> 4; }
+
 template <int TIER, bool GLOBAL>
 __device__ __forceinline__ typename St<TIER>::W *state_base(void *gscratch, int cols)
 {
@@ -212,8 +410,15 @@ __device__ __forceinline__ typename St<TIER>::W *state_base(void *gscratch, int 
         return reinterpret_cast<W *>(gscratch) + (size_t)blockIdx.x * cols * 64 + threadIdx.x;
     } else {
         extern __shared__ __align__(16) unsigned char lds_raw[];
-        return reinterpret_cast<W *>(lds_raw + 32) + threadIdx.x;
+        return reinterpret_cast<W *>(lds_raw + 32 + (size_t)query_blocks(cols) * 256) + threadIdx.x;
     }
+}
+
+__device__ __forceinline__ uint32_t *query_lds(int cols)
+{
+    (void)cols;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    return reinterpret_cast<uint32_t *>(lds_raw + 32);
 }
 
 __device__ __forceinline__ const int8_t *load_smat(const ExtParams &p)
@@ -242,7 +447,7 @@ __global__ __launch_bounds__(64) void ext_plain_kernel(ExtDev d, ExtParams p,
         const int count = min(64, n - base);
         if (lane < count) idx[lane] = order ? order[base + lane] : (uint32_t)(base + lane);
         __syncthreads();
-        ext_wave<TIER>(d, p, idx, count, col, smat);
+        ext_wave<TIER>(d, p, idx, count, col, smat, query_lds(cols));
         __syncthreads();
     }
 }
@@ -269,7 +474,7 @@ __global__ __launch_bounds__(64) void ext_persistent_kernel(ExtDev d, ExtParams 
         const WorkItem wi = items[item];
         if (threadIdx.x < wi.count) idx[threadIdx.x] = order[wi.start + threadIdx.x];
         __syncthreads();
-        ext_wave<TIER>(d, p, idx, (int)wi.count, col, smat);
+        ext_wave<TIER>(d, p, idx, (int)wi.count, col, smat, query_lds(cols));
         __syncthreads();
     }
 }
@@ -289,7 +494,8 @@ hipError_t launch_zero_u32(uint32_t *p, int n, hipStream_t s)
 
 size_t ext_lds_bytes(int tier, bool global, int cols)
 {
-    return 32 + (global ? 0 : (size_t)cols * 64 * ext_state_bytes(tier));
+    const size_t qblocks = (size_t)(cols - 1 + 15) / 16;
+    return 32 + qblocks * 256 + (global ? 0 : (size_t)cols * 64 * ext_state_bytes(tier));
 }
 static size_t lds_bytes(int tier, bool global, int cols) { return ext_lds_bytes(tier, global, cols); }
 

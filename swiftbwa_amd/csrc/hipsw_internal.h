@@ -38,6 +38,9 @@ struct ExtParams {
     int8_t  mat[32];          // 25 used, [target*5 + query]
     int32_t o_del, e_del, o_ins, e_ins, zdrop;
     int32_t oe_del, oe_ins, max_mat;
+    // Fast path: the 4x4 A/C/G/T block of mat is match_sc on the diagonal and
+    // mis_sc off it (the shape ScoringParameters.scoringMatrix() builds).
+    int32_t std_mat, match_sc, mis_sc;
 };
 
 // A unit of persistent work: `count` (<= 64) consecutive entries of the

@@ -27,6 +27,11 @@ struct hipsw_nw_set {
     uint32_t *d_order = nullptr;
     WorkItem *d_items = nullptr;
     std::vector<int32_t> final_w;
+    // first-pass plans, per mode (the retry passes are planned on the fly)
+    bool plan_ready[2] = {false, false};
+    std::vector<uint32_t> plan_order[2];
+    std::vector<hipsw::WorkItem> plan_items[2];
+    std::vector<int> plan_launch_raw[2];
 };
 
 namespace {
@@ -44,6 +49,7 @@ size_t span(size_t count) { return (count * sizeof(T) + 255) & ~(size_t)255; }
 struct NwLaunch {
     int tier, S, ncol_max, zrows, item_begin, item_count, grid;
     bool global_state;
+    int qw8;     // staged query words (8 bases each) per lane
 };
 
 int s_class(int S)
@@ -174,7 +180,7 @@ static void plan_pass(const hipsw_engine *eng, const hipsw_nw_set *set,
     for (int a = 0; a < m; ++a) {
         const int k = (int)active[a];
         const int q = set->ql[k], t = set->tl[k];
-        const int wp = std::max(w[k], std::abs(q - t));
+        const int wp = std::max(std::min(w[k], 1 << 20), std::abs(q - t));
         ncol[a] = std::min(q, 2 * wp + 1);
         S[a] = ncol[a] + 2;
         const int64_t bound = (int64_t)(q + t) * set->max_pen + 64;
@@ -183,58 +189,61 @@ static void plan_pass(const hipsw_engine *eng, const hipsw_nw_set *set,
     order.clear();
     items.clear();
     launches.clear();
-    auto add_launch = [&](int tr, int s_max, int c_max, int z_rows, int ib, int ic, bool persistent) {
+    auto add_launch = [&](int tr, int s_max, int c_max, int z_rows, int ib, int ic, int q_max) {
         NwLaunch L;
+        L.qw8 = std::max(1, (q_max + 7) / 8);
         L.tier = tr;
         L.S = s_max;
         L.ncol_max = c_max;
         L.zrows = z_rows;
         L.item_begin = ib;
         L.item_count = ic;
-        const size_t lds = nw_lds_bytes(tr, false, s_max);
+        const size_t lds = nw_lds_bytes(tr, false, s_max, L.qw8);
         L.global_state = lds > (size_t)eng->lds_per_block;
-        int wpc = 16;
-        if (!L.global_state) wpc = std::max(1, std::min(16, (int)(eng->lds_per_cu / (lds + 512))));
+        const size_t lds_used = nw_lds_bytes(tr, L.global_state, s_max, L.qw8);
+        const int wpc = std::max(1, std::min(16, (int)(eng->lds_per_cu / (lds_used + 512))));
         L.grid = std::max(1, std::min(ic, eng->n_cu * wpc));
-        (void)persistent;
         launches.push_back(L);
     };
     if (mode == HIPSW_MODE_PLAIN) {
-        int tr = NW_I16, smax = 3, cmax = 1, zr = 1;
+        int tr = NW_I16, smax = 3, cmax = 1, zr = 1, qm = 1;
         for (int a = 0; a < m; ++a) {
             tr = std::max(tr, tier[a]);
             smax = std::max(smax, S[a]);
             cmax = std::max(cmax, ncol[a]);
             zr = std::max(zr, set->tl[active[a]]);
+            qm = std::max(qm, set->ql[active[a]]);
             order.push_back(active[a]);
         }
         for (int g = 0; g < m; g += 64) items.push_back(WorkItem{(uint32_t)g, (uint32_t)std::min(64, m - g)});
-        add_launch(tr, smax, cmax, zr, 0, (int)items.size(), false);
+        add_launch(tr, smax, cmax, zr, 0, (int)items.size(), qm);
         return;
     }
     // Batched: bins by (tier, S class); inside a bin lanes of one wave share
-    // the band width and have similar target lengths (sorted by ncol, tlen).
-    std::vector<int> idx(m);
-    std::iota(idx.begin(), idx.end(), 0);
-    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
-        const int bx = tier[x] * 8 + s_class(S[x]), by = tier[y] * 8 + s_class(S[y]);
-        if (bx != by) return bx < by;
-        if (ncol[x] != ncol[y]) return ncol[x] > ncol[y];
-        return set->tl[active[x]] > set->tl[active[y]];
-    });
+    // the band width and have similar target lengths (ncol desc, tlen desc).
+    std::vector<uint32_t> keys(m), idx(m);
+    for (int a = 0; a < m; ++a) {
+        const uint32_t bin = (uint32_t)(tier[a] * 8 + s_class(S[a]));
+        const uint32_t nc = (uint32_t)std::min(ncol[a], 4095);
+        const uint32_t tl = (uint32_t)std::min(set->tl[active[a]], 65535);
+        keys[a] = (bin << 28) | ((4095u - nc) << 16) | (65535u - tl);
+        idx[a] = (uint32_t)a;
+    }
+    radix_sort_by_key(keys, idx);
     for (int a = 0; a < m; ++a) order.push_back(active[idx[a]]);
     for (int s = 0; s < m;) {
-        const int bin = tier[idx[s]] * 8 + s_class(S[idx[s]]);
-        int e = s, smax = 3, cmax = 1, zr = 1;
-        while (e < m && tier[idx[e]] * 8 + s_class(S[idx[e]]) == bin) {
+        const uint32_t bin = keys[s] >> 28;
+        int e = s, smax = 3, cmax = 1, zr = 1, qm = 1;
+        while (e < m && (keys[e] >> 28) == bin) {
             smax = std::max(smax, S[idx[e]]);
             cmax = std::max(cmax, ncol[idx[e]]);
             zr = std::max(zr, set->tl[active[idx[e]]]);
+            qm = std::max(qm, set->ql[active[idx[e]]]);
             ++e;
         }
         const int ib = (int)items.size();
         for (int g = s; g < e; g += 64) items.push_back(WorkItem{(uint32_t)g, (uint32_t)std::min(64, e - g)});
-        add_launch(bin / 8, smax, cmax, zr, ib, (int)items.size() - ib, true);
+        add_launch((int)bin / 8, smax, cmax, zr, ib, (int)items.size() - ib, qm);
         s = e;
     }
 }
@@ -262,7 +271,29 @@ extern "C" int hipsw_nw_run(hipsw_engine *eng, hipsw_nw_set *set, int32_t mode,
                                              eng->stream), "zero"))
         return HIPSW_E_DEVICE;
     for (int pass = 0; !active.empty(); ++pass) {
-        plan_pass(eng, set, active, w, mode, order, items, launches);
+        if (pass == 0 && set->plan_ready[mode]) {
+            order = set->plan_order[mode];
+            items = set->plan_items[mode];
+            launches.clear();
+            const std::vector<int> &raw = set->plan_launch_raw[mode];
+            for (size_t r = 0; r + 9 <= raw.size(); r += 9)
+                launches.push_back(NwLaunch{raw[r], raw[r + 1], raw[r + 2], raw[r + 3], raw[r + 4],
+                                            raw[r + 5], raw[r + 6], raw[r + 7] != 0, raw[r + 8]});
+        } else {
+            plan_pass(eng, set, active, w, mode, order, items, launches);
+            if (pass == 0) {
+                set->plan_order[mode] = order;
+                set->plan_items[mode] = items;
+                std::vector<int> raw;
+                for (const NwLaunch &L : launches) {
+                    const int v[9] = {L.tier, L.S, L.ncol_max, L.zrows, L.item_begin, L.item_count,
+                                      L.grid, L.global_state ? 1 : 0, L.qw8};
+                    raw.insert(raw.end(), v, v + 9);
+                }
+                set->plan_launch_raw[mode] = raw;
+                set->plan_ready[mode] = true;
+            }
+        }
         if (launches.size() > 64) {
             eng->last_error = "hipsw_nw_run: too many bins";
             return HIPSW_E_UNSUPPORTED;
@@ -300,7 +331,7 @@ extern "C" int hipsw_nw_run(hipsw_engine *eng, hipsw_nw_set *set, int32_t mode,
             ok = eng->check(launch_nw(L.tier, L.global_state, set->dev, set->params, set->d_order,
                                       set->d_items + L.item_begin, L.item_count,
                                       eng->counters + k, L.S, zbase, L.zrows, ncw, L.grid,
-                                      mode == HIPSW_MODE_BATCHED, gstate, eng->stream),
+                                      mode == HIPSW_MODE_BATCHED, gstate, L.qw8, eng->stream),
                             "launch nw");
             ++k;
             ++launches_total;
@@ -319,7 +350,7 @@ extern "C" int hipsw_nw_run(hipsw_engine *eng, hipsw_nw_set *set, int32_t mode,
         for (uint32_t j : active) {
             set->final_w[j] = w[j];
             if (score[j] < set->ts[j] && tries[j] < set->max_retries) {
-                w[j] *= 2;
+                w[j] = w[j] < (1 << 20) ? w[j] * 2 : w[j];
                 tries[j] += 1;
                 next.push_back(j);
             }

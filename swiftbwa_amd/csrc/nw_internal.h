@@ -36,10 +36,12 @@ struct NwParams {
     int32_t o_del, e_del, o_ins, e_ins, oe_del, oe_ins;
 };
 
-size_t nw_lds_bytes(int tier, bool global_state, int S);
+// LDS bytes of one block: scoring table, the staged query (qw8 words of 8
+// 4-bit codes per lane) and, unless held in HBM, the S-slot row state.
+size_t nw_lds_bytes(int tier, bool global_state, int S, int qw8);
 hipError_t launch_nw(int tier, bool global_state, const NwDev &d, const NwParams &p,
                      const uint32_t *order, const WorkItem *items, int n_items,
                      uint32_t *counter, int S, uint32_t *zbase, int zrows, int ncw, int grid,
-                     bool persistent, void *gstate, hipStream_t s);
+                     bool persistent, void *gstate, int qw8, hipStream_t s);
 
 }  // namespace hipsw

@@ -53,7 +53,7 @@ __device__ __forceinline__ int wave_max_nw(int v)
 template <int T>
 __device__ void nw_wave(const NwDev &d, const NwParams &p, const uint32_t *idx, int count,
                         typename NwSt<T>::W *slots, int S, uint32_t *zreg, int zrows, int ncw,
-                        const int8_t *smat)
+                        const int8_t *smat, uint32_t *qlds, int qw8)
 {
     using St = NwSt<T>;
     using Wd = typename St::W;
@@ -64,7 +64,7 @@ __device__ void nw_wave(const NwDev &d, const NwParams &p, const uint32_t *idx, 
     const int qlen = valid ? d.q_len[job] : 0;
     const int tlen = valid ? d.t_len[job] : 0;
     const bool nonempty = valid && qlen > 0 && tlen > 0;     // :29-31
-    int w = valid ? d.w[job] : 0;
+    int w = valid ? min(d.w[job], 1 << 20) : 0;   // wider than any band: same cells
     const int dl = qlen > tlen ? qlen - tlen : tlen - qlen;
     w = max(w, dl);                                            // :47
     const int ncol = min(qlen, 2 * w + 1);                     // :48
@@ -73,6 +73,25 @@ __device__ void nw_wave(const NwDev &d, const NwParams &p, const uint32_t *idx, 
     const uint32_t qn0 = valid ? d.qn_word[job] : kNoN;
     const uint32_t tn0 = valid ? d.tn_word[job] : kNoN;
     Wd *col = slots + lane;                                    // slot s at col[s * 64]
+
+    // Stage the query as 4-bit codes, 8 bases per LDS word: qlds[b*64 + lane]
+    // holds bases 8b..8b+7 (code 4 = N).
+    uint32_t *ql = qlds + lane;
+    for (int b = 0; b < qw8; ++b) {
+        uint32_t x = 0;
+        if (nonempty && b * 8 < qlen) {
+            x = (d.seq[qw0 + (b >> 1)] >> ((b & 1) * 16)) & 0xFFFFu;
+            x = (x | (x << 8)) & 0x00FF00FFu;                   // spread 2-bit codes
+            x = (x | (x << 4)) & 0x0F0F0F0Fu;                   // to 4-bit fields
+            x = (x | (x << 2)) & 0x33333333u;
+            if (qn0 != kNoN) {
+                const uint32_t nb = (d.nmask[qn0 + (b >> 2)] >> ((b & 3) * 8)) & 0xFFu;
+                for (int k = 0; k < 8; ++k)
+                    if ((nb >> k) & 1u) x = (x & ~(0xFu << (4 * k))) | (4u << (4 * k));
+            }
+        }
+        ql[b * 64] = x;
+    }
 
     // First row (:75-84): h[0] = 0, h[j] = -(o_ins + e_ins*j) for j <= min(w,qlen),
     // everything else (and all of e) the sentinel.
@@ -117,40 +136,36 @@ __device__ void nw_wave(const NwDev &d, const NwParams &p, const uint32_t *idx, 
             h_left = beg == 0 ? 0 - (p.oe_del + p.e_del * i) : kNegInf;
             col[sb * 64] = St::pack(h_left, St::e(v));
         }
-        // query word cache: base j of the current cell
         int j = beg;
-        uint32_t qw = 0, qn = 0;
-        if (act && j < qlen) qw = d.seq[qw0 + (j >> 4)];
-        if (act && qn0 != kNoN && j < qlen) qn = d.nmask[qn0 + (j >> 5)];
         int s1 = sb + 1 == S ? 0 : sb + 1;                     // slot of column j+1
         uint32_t zword = 0;
         for (int c = 0; c < cmax; ++c) {                       // :108-141
-            if (act && c < width) {
-                int q = (qw >> ((j & 15) * 2)) & 3;
-                if (qn0 != kNoN && ((qn >> (j & 31)) & 1)) q = 4;
-                int h_cur = h_diag + (int)smat[t5 + q];
-                const Wd v = col[s1 * 64];
-                h_diag = St::h(v);
-                const int e_open = h_diag - p.oe_del;
-                const int e_ext = St::e(v) - p.e_del;
-                const uint32_t e_flag = e_open > e_ext ? 0u : 1u;
-                const int ev = e_open > e_ext ? e_open : e_ext;
-                const int f_open = h_left - p.oe_ins;
-                const int f_ext = f - p.e_ins;
-                const uint32_t f_flag = f_open > f_ext ? 0u : 2u;
-                f = f_open > f_ext ? f_open : f_ext;
-                const uint32_t from_e = ev > h_cur ? 4u : 0u;
-                h_cur = ev > h_cur ? ev : h_cur;
-                const uint32_t from_f = f > h_cur ? 8u : 0u;
-                h_cur = f > h_cur ? f : h_cur;
-                col[s1 * 64] = St::pack(h_cur, ev);
-                h_left = h_cur;
-                zword |= (e_flag | f_flag | from_e | from_f) << ((c & 7) * 4);
-                ++j;
-                s1 = s1 + 1 == S ? 0 : s1 + 1;
-                if ((j & 15) == 0 && j < qlen) qw = d.seq[qw0 + (j >> 4)];
-                if (qn0 != kNoN && (j & 31) == 0 && j < qlen) qn = d.nmask[qn0 + (j >> 5)];
-            }
+            // Every lane computes every step; lanes past their band end (or
+            // without a job) keep their LDS state and record no flags.
+            const bool inb = act && c < width;
+            const int jq = max(0, min(j, qlen - 1));
+            const int q = (int)((ql[(jq >> 3) * 64] >> ((jq & 7) * 4)) & 0xFu);
+            int h_cur = h_diag + (int)smat[t5 + q];
+            const Wd v = col[s1 * 64];
+            const int hd = St::h(v);
+            const int e_open = hd - p.oe_del;
+            const int e_ext = St::e(v) - p.e_del;
+            const uint32_t e_flag = e_open > e_ext ? 0u : 1u;
+            const int ev = e_open > e_ext ? e_open : e_ext;
+            const int f_open = h_left - p.oe_ins;
+            const int f_ext = f - p.e_ins;
+            const uint32_t f_flag = f_open > f_ext ? 0u : 2u;
+            f = f_open > f_ext ? f_open : f_ext;
+            const uint32_t from_e = ev > h_cur ? 4u : 0u;
+            h_cur = ev > h_cur ? ev : h_cur;
+            const uint32_t from_f = f > h_cur ? 8u : 0u;
+            h_cur = f > h_cur ? f : h_cur;
+            col[s1 * 64] = inb ? St::pack(h_cur, ev) : v;
+            h_diag = hd;
+            h_left = h_cur;
+            zword |= inb ? (e_flag | f_flag | from_e | from_f) << ((c & 7) * 4) : 0u;
+            ++j;
+            s1 = s1 + 1 == S ? 0 : s1 + 1;
             if ((c & 7) == 7 || c == cmax - 1) {
                 if (i < zrows) zl[((size_t)i * ncw + (c >> 3)) * 64] = zword;
                 zword = 0;
@@ -187,17 +202,23 @@ __device__ void nw_wave(const NwDev &d, const NwParams &p, const uint32_t *idx, 
         *o_score = St::h(col[s * 64]);
     }
     // Traceback (:160-232) with CIGARBuilder merging; ops come out last-first.
+    // They go to a ring of `cap` slots, so when the CIGAR has more than cap ops
+    // the row still ends up holding the first cap ops of the final CIGAR.
     int i = tlen - 1, jj = qlen - 1, state = 0;
     int nops = 0;
     uint32_t cur_op = 0xFu, cur_len = 0;
+    const int cap = d.cap;
+    int slot = 0;
+    auto push = [&]() {
+        cig[slot] = (cur_len << 4) | cur_op;
+        slot = slot + 1 == cap ? 0 : slot + 1;
+        ++nops;
+    };
     auto emit = [&](uint32_t op) {
         if (op == cur_op) {
             ++cur_len;
         } else {
-            if (cur_len) {
-                if (nops < d.cap) cig[nops] = (cur_len << 4) | cur_op;
-                ++nops;
-            }
+            if (cur_len) push();
             cur_op = op;
             cur_len = 1;
         }
@@ -225,12 +246,17 @@ __device__ void nw_wave(const NwDev &d, const NwParams &p, const uint32_t *idx, 
             --jj;
         }
     }
-    if (cur_len) {
-        if (nops < d.cap) cig[nops] = (cur_len << 4) | cur_op;
-        ++nops;
+    if (cur_len) push();
+    // Final order = reverse of emission order (:230).  With the ring, slots
+    // [0, p) and [p, kept) are each reversed in place, p = nops mod cap.
+    const int kept = min(nops, cap);
+    const int p_split = nops > cap ? nops % cap : kept;
+    for (int a = 0, b = p_split - 1; a < b; ++a, --b) {
+        const uint32_t tmp = cig[a];
+        cig[a] = cig[b];
+        cig[b] = tmp;
     }
-    const int kept = min(nops, d.cap);
-    for (int a = 0, b = kept - 1; a < b; ++a, --b) {           // :230 reverse
+    for (int a = p_split, b = kept - 1; a < b; ++a, --b) {
         const uint32_t tmp = cig[a];
         cig[a] = cig[b];
         cig[b] = tmp;
@@ -255,13 +281,15 @@ template <int T, bool GLOBAL>
 __global__ __launch_bounds__(64) void nw_kernel(NwDev d, NwParams p, const uint32_t *order,
                                                 const WorkItem *items, int n_items,
                                                 uint32_t *counter, int S, uint32_t *zbase,
-                                                int zrows, int ncw, int persistent, void *gstate)
+                                                int zrows, int ncw, int persistent, void *gstate,
+                                                int qw8)
 {
     using Wd = typename NwSt<T>::W;
     extern __shared__ __align__(16) unsigned char nw_lds[];
     const int8_t *smat = nw_load_smat(p);
+    uint32_t *qlds = reinterpret_cast<uint32_t *>(nw_lds + 32);
     Wd *slots = GLOBAL ? reinterpret_cast<Wd *>(gstate) + (size_t)blockIdx.x * S * 64
-                       : reinterpret_cast<Wd *>(nw_lds + 32);
+                       : reinterpret_cast<Wd *>(nw_lds + 32 + (size_t)qw8 * 256);
     uint32_t *zreg = zbase + (size_t)blockIdx.x * zrows * ncw * 64;
     __shared__ uint32_t idx[64];
     __shared__ int item_sh;
@@ -276,41 +304,41 @@ __global__ __launch_bounds__(64) void nw_kernel(NwDev d, NwParams p, const uint3
         const WorkItem wi = items[it];
         if (threadIdx.x < wi.count) idx[threadIdx.x] = order[wi.start + threadIdx.x];
         __syncthreads();
-        nw_wave<T>(d, p, idx, (int)wi.count, slots, S, zreg, zrows, ncw, smat);
+        nw_wave<T>(d, p, idx, (int)wi.count, slots, S, zreg, zrows, ncw, smat, qlds, qw8);
         __syncthreads();
         if (!persistent) it += gridDim.x;      // plain form: static round-robin
     }
 }
 
-size_t nw_lds_bytes(int tier, bool global_state, int S)
+size_t nw_lds_bytes(int tier, bool global_state, int S, int qw8)
 {
     const size_t w = tier == NW_I16 ? 4 : 8;
-    return 32 + (global_state ? 0 : (size_t)S * 64 * w);
+    return 32 + (size_t)qw8 * 256 + (global_state ? 0 : (size_t)S * 64 * w);
 }
 
 template <int T, bool G>
 static hipError_t nw_t(const NwDev &d, const NwParams &p, const uint32_t *order,
                        const WorkItem *items, int n_items, uint32_t *counter, int S,
                        uint32_t *zbase, int zrows, int ncw, int grid, bool persistent,
-                       void *gstate, hipStream_t s)
+                       void *gstate, int qw8, hipStream_t s)
 {
-    hipLaunchKernelGGL((nw_kernel<T, G>), dim3(grid), dim3(64), nw_lds_bytes(T, G, S), s, d, p,
-                       order, items, n_items, counter, S, zbase, zrows, ncw,
-                       persistent ? 1 : 0, gstate);
+    hipLaunchKernelGGL((nw_kernel<T, G>), dim3(grid), dim3(64), nw_lds_bytes(T, G, S, qw8), s, d,
+                       p, order, items, n_items, counter, S, zbase, zrows, ncw,
+                       persistent ? 1 : 0, gstate, qw8);
     return hipGetLastError();
 }
 
 hipError_t launch_nw(int tier, bool global_state, const NwDev &d, const NwParams &p,
                      const uint32_t *order, const WorkItem *items, int n_items,
                      uint32_t *counter, int S, uint32_t *zbase, int zrows, int ncw, int grid,
-                     bool persistent, void *gstate, hipStream_t s)
+                     bool persistent, void *gstate, int qw8, hipStream_t s)
 {
     if (n_items <= 0) return hipSuccess;
     switch (tier * 2 + (global_state ? 1 : 0)) {
-    case 0: return nw_t<NW_I16, false>(d, p, order, items, n_items, counter, S, zbase, zrows, ncw, grid, persistent, gstate, s);
-    case 1: return nw_t<NW_I16, true>(d, p, order, items, n_items, counter, S, zbase, zrows, ncw, grid, persistent, gstate, s);
-    case 2: return nw_t<NW_I32, false>(d, p, order, items, n_items, counter, S, zbase, zrows, ncw, grid, persistent, gstate, s);
-    default: return nw_t<NW_I32, true>(d, p, order, items, n_items, counter, S, zbase, zrows, ncw, grid, persistent, gstate, s);
+    case 0: return nw_t<NW_I16, false>(d, p, order, items, n_items, counter, S, zbase, zrows, ncw, grid, persistent, gstate, qw8, s);
+    case 1: return nw_t<NW_I16, true>(d, p, order, items, n_items, counter, S, zbase, zrows, ncw, grid, persistent, gstate, qw8, s);
+    case 2: return nw_t<NW_I32, false>(d, p, order, items, n_items, counter, S, zbase, zrows, ncw, grid, persistent, gstate, qw8, s);
+    default: return nw_t<NW_I32, true>(d, p, order, items, n_items, counter, S, zbase, zrows, ncw, grid, persistent, gstate, qw8, s);
     }
 }
 

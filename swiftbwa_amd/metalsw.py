@@ -132,13 +132,45 @@ class HipSWEngine:
         return out
 
 
+    def global_nw(self, jobs, scoring: ScoringParameters, max_retries=3, cap=512,
+                  mode=native.MODE_BATCHED, stats=None):
+        """hipsw_global_nw on a job dict (w = starting band, true_score = retry
+        threshold).  Returns (score[n], cigar[n, cap], n_cigar[n], flags[n], final_w[n])."""
+        n = int(jobs["n"])
+        score = np.zeros(n, np.int32)
+        cig = np.zeros((n, cap), np.uint32)
+        ncig = np.zeros(n, np.int32)
+        flags = np.zeros(n, np.int32)
+        fw = np.zeros(n, np.int32)
+        if n == 0:
+            return score, cig, ncig, flags, fw
+        a = _ext_arrays(jobs)
+        ts = np.ascontiguousarray(jobs["true_score"], dtype=np.int32)
+        sc = scoring.to_c()
+        st = native.HipswStats()
+        rc = self.lib.hipsw_global_nw(
+            self.handle, n, native.ptr(a["queries"]), native.ptr(a["q_off"]),
+            native.ptr(a["q_len"]), native.ptr(a["targets"]), native.ptr(a["t_off"]),
+            native.ptr(a["t_len"]), native.ptr(a["w"]), native.ptr(ts), int(max_retries),
+            int(cap), C.byref(sc), int(mode), native.ptr(score), native.ptr(cig),
+            native.ptr(ncig), native.ptr(flags), native.ptr(fw), C.byref(st))
+        if rc != native.HIPSW_OK:
+            raise native.HipswError(rc, self.last_error())
+        if stats is not None:
+            stats.update(st.as_dict())
+        return score, cig, ncig, flags, fw
+
+
 def _ext_arrays(jobs):
     def c(name, dt):
         return np.ascontiguousarray(jobs[name], dtype=dt)
     a = {"queries": c("queries", np.uint8), "q_off": c("q_off", np.uint32),
          "q_len": c("q_len", np.uint16), "targets": c("targets", np.uint8),
          "t_off": c("t_off", np.uint32), "t_len": c("t_len", np.uint16),
-         "h0": c("h0", np.int32), "w": c("w", np.int32), "end_bonus": c("end_bonus", np.int32)}
+         "w": c("w", np.int32)}
+    n = int(jobs["n"])
+    for name in ("h0", "end_bonus"):
+        a[name] = c(name, np.int32) if name in jobs else np.zeros(n, np.int32)
     if a["queries"].size == 0:
         a["queries"] = np.zeros(1, np.uint8)
     if a["targets"].size == 0:
@@ -189,3 +221,47 @@ class BandedSWDispatcher:
         except native.HipswError:
             return []
         return [SWResult(*map(int, row)) for row in out]
+
+
+@dataclass
+class GlobalSWTask:
+    """GlobalSWDispatcher.swift:6-20."""
+    query: bytes
+    target: bytes
+    trueScore: int
+    initialW: int
+    scoring: ScoringParameters = field(default_factory=ScoringParameters)
+
+
+@dataclass
+class GlobalSWResult:
+    """GlobalSWDispatcher.swift:23-31: score plus packed CIGAR (len << 4 | op)."""
+    score: int
+    cigar: List[int]
+
+
+class GlobalSWDispatcher:
+    """GlobalSWDispatcher.swift:37-118.  Per task: NW at initialW, then w
+    doubling while score < trueScore, at most 3 retries (the CPU loop,
+    CIGARGenerator.swift:88-96).  Returns [] on failure; a task whose CIGAR
+    exceeds MAX_CIGAR_OPS also fails the batch, as the caller must redo it."""
+    MAX_CIGAR_OPS = 512
+
+    @staticmethod
+    def dispatchBatch(tasks: List[GlobalSWTask], engine: HipSWEngine) -> List[GlobalSWResult]:
+        if not tasks:
+            return []
+        sc = tasks[0].scoring
+        if any(t.scoring != sc for t in tasks):
+            return []
+        jobs = tasks_to_jobs([BandedSWTask(t.query, t.target, 0, t.initialW, sc) for t in tasks])
+        jobs["true_score"] = np.array([t.trueScore for t in tasks], np.int32)
+        try:
+            score, cig, ncig, flags, _ = engine.global_nw(
+                jobs, sc, max_retries=3, cap=GlobalSWDispatcher.MAX_CIGAR_OPS)
+        except native.HipswError:
+            return []
+        if (flags != 0).any():
+            return []
+        return [GlobalSWResult(int(score[k]), [int(x) for x in cig[k, :ncig[k]]])
+                for k in range(len(tasks))]

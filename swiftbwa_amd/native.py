@@ -56,6 +56,14 @@ _SIGS = {
     "hipsw_ext_run": (C.c_int, [_P, _P, C.c_int32, C.POINTER(HipswStats)]),
     "hipsw_ext_download": (C.c_int, [_P, _P, _P]),
     "hipsw_ext_free": (None, [_P, _P]),
+    "hipsw_global_nw": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P, _P, C.c_int32,
+                                  C.c_int32, C.POINTER(HipswScoring), C.c_int32, _P, _P, _P,
+                                  _P, _P, C.POINTER(HipswStats)]),
+    "hipsw_nw_upload": (_P, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P, _P, C.c_int32,
+                             C.c_int32, C.POINTER(HipswScoring)]),
+    "hipsw_nw_run": (C.c_int, [_P, _P, C.c_int32, C.POINTER(HipswStats)]),
+    "hipsw_nw_download": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    "hipsw_nw_free": (None, [_P, _P]),
 }
 
 _lib = None

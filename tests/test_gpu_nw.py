@@ -1,0 +1,111 @@
+"""GPU parity of the banded global NW + traceback kernel against the CPU oracle
+(GlobalAligner.align + traceback + CIGARGenerator's retry loop, restated in
+oracle/sw_oracle.c).  Bit-exact on score, CIGAR, op count and final band."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as orc
+from swiftbwa_amd import native, workload as wl
+from swiftbwa_amd.metalsw import GlobalSWDispatcher, GlobalSWTask, ScoringParameters
+
+pytestmark = pytest.mark.gpu
+MODES = [native.MODE_PLAIN, native.MODE_BATCHED]
+
+
+def oracle_sc(sp):
+    return orc.scoring(a=sp.matchScore, b=sp.mismatchPenalty, o_ins=sp.gapOpenPenalty,
+                       e_ins=sp.gapExtendPenalty, o_del=sp.gapOpenPenaltyDeletion,
+                       e_del=sp.gapExtendPenaltyDeletion, zdrop=sp.zDrop)
+
+
+def check(engine, js, sp=None, max_retries=3, cap=512):
+    sp = sp or ScoringParameters()
+    rs, rc, rn, cells = orc.global_batch(js, oracle_sc(sp), max_retries=max_retries, cap=cap,
+                                         nthreads=8)
+    for mode in MODES:
+        st = {}
+        s, c, n, fl, fw = engine.global_nw(js, sp, max_retries=max_retries, cap=cap, mode=mode,
+                                           stats=st)
+        bad = np.nonzero(s != rs)[0]
+        assert len(bad) == 0, "mode %d score mismatch on %d jobs, first %d: %d vs %d" % (
+            mode, len(bad), bad[0], s[bad[0]], rs[bad[0]])
+        assert np.array_equal(n, rn), mode
+        assert np.array_equal(fl, (rn > cap).astype(np.int32)), mode
+        for k in range(js["n"]):
+            m = min(n[k], cap)
+            assert np.array_equal(c[k, :m], rc[k, :m]), (mode, k)
+        assert st["cells"] >= cells  # retries re-run only the retried jobs, as the oracle does
+        assert st["cells"] == cells, (mode, st["cells"], cells)
+
+
+def test_known_answers(engine):
+    # AlignmentTests.swift:313-446
+    sp = ScoringParameters()
+    T = lambda q, t, w: GlobalSWTask(bytes(q), bytes(t), -(1 << 30), w, sp)
+    res = GlobalSWDispatcher.dispatchBatch([
+        T([0, 1, 2, 3, 0, 1, 2, 3, 0, 1], [0, 1, 2, 3, 0, 1, 2, 3, 0, 1], 10),
+        T([0, 1, 2, 3, 0, 1, 2, 3, 0, 1], [0, 1, 2, 3, 0, 3, 2, 3, 0, 1], 10),
+        T([0, 1, 2, 3, 2, 0, 1], [0, 1, 2, 3, 0, 1], 10),
+        T([0, 1, 2, 3, 0, 1], [0, 1, 2, 3, 2, 0, 1], 10),
+        T([0, 1, 2, 3] * 2, [0, 1, 2, 3] * 2, 1),
+        T([0, 1, 2, 3] * 2, [0, 1, 2, 3] * 2, 50)], engine)
+    assert res[0].score == 10 and res[0].cigar == [10 << 4]
+    assert res[1].score == 5 and res[1].cigar == [10 << 4]
+    ops = lambda cg: [(c & 0xF, c >> 4) for c in cg]
+    q_ins = sum(l for o, l in ops(res[2].cigar) if o in (0, 1))
+    t_ins = sum(l for o, l in ops(res[2].cigar) if o in (0, 2))
+    assert any(o == 1 for o, _ in ops(res[2].cigar)) and (q_ins, t_ins) == (7, 6)
+    q_d = sum(l for o, l in ops(res[3].cigar) if o in (0, 1))
+    t_d = sum(l for o, l in ops(res[3].cigar) if o in (0, 2))
+    assert any(o == 2 for o, _ in ops(res[3].cigar)) and (q_d, t_d) == (6, 7)
+    assert res[4].score == res[5].score and res[4].cigar == res[5].cigar
+
+
+def test_nw_subset(engine):
+    check(engine, wl.generate(wl.NW, 3, 20000))
+
+
+def test_forced_retries(engine):
+    js = wl.generate(wl.NW, 7, 3000)
+    js["w"] = np.maximum(1, js["w"] // 8).astype(np.int32)      # too narrow: retries fire
+    js["true_score"] = js["true_score"] + 2
+    check(engine, js)
+
+
+def _random(rng, n, qmax, tmax, n_rate=0.0):
+    qs, ts, ql, tl, w, tsc = [], [], [], [], [], []
+    for _ in range(n):
+        q = rng.integers(0, 4, rng.integers(1, qmax + 1)).astype(np.uint8)
+        t = q.copy()
+        t[rng.random(len(t)) < 0.05] = rng.integers(0, 4)
+        cut = int(rng.integers(0, max(1, len(t) // 4)))
+        t = np.concatenate([t[:len(t) // 2], rng.integers(0, 4, int(rng.integers(0, 6))).astype(np.uint8),
+                            t[len(t) // 2 + cut:]])[:tmax]
+        if len(t) == 0:
+            t = np.zeros(1, np.uint8)
+        if n_rate:
+            q[rng.random(len(q)) < n_rate] = 4
+            t[rng.random(len(t)) < n_rate] = 4
+        qs.append(q); ts.append(t); ql.append(len(q)); tl.append(len(t))
+        w.append(int(rng.choice([1, 2, 5, 20, 100])))
+        tsc.append(int(rng.integers(-50, len(q))))
+    qo = np.zeros(n, np.uint32); to = np.zeros(n, np.uint32)
+    qo[1:] = np.cumsum(ql)[:-1]; to[1:] = np.cumsum(tl)[:-1]
+    return {"n": n, "queries": np.concatenate(qs), "targets": np.concatenate(ts), "q_off": qo,
+            "t_off": to, "q_len": np.array(ql, np.uint16), "t_len": np.array(tl, np.uint16),
+            "w": np.array(w, np.int32), "true_score": np.array(tsc, np.int32)}
+
+
+def test_random_with_n(engine):
+    check(engine, _random(np.random.default_rng(21), 2000, 120, 140, n_rate=0.03))
+
+
+def test_long_and_int32_tier(engine):
+    # long pairs exceed the int16 bound and the LDS budget
+    sp = ScoringParameters(gapOpenPenalty=20, gapExtendPenalty=3, gapOpenPenaltyDeletion=20,
+                           gapExtendPenaltyDeletion=3, mismatchPenalty=9)
+    check(engine, _random(np.random.default_rng(22), 200, 900, 1000), sp)
+
+
+def test_small_cap_flags(engine):
+    check(engine, _random(np.random.default_rng(23), 500, 60, 70), cap=2)
