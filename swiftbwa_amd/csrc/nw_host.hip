@@ -182,7 +182,7 @@ static void plan_pass(const hipsw_engine *eng, const hipsw_nw_set *set,
         const int q = set->ql[k], t = set->tl[k];
         const int wp = std::max(std::min(w[k], 1 << 20), std::abs(q - t));
         ncol[a] = std::min(q, 2 * wp + 1);
-        S[a] = ncol[a] + 2;
+        S[a] = std::max(ncol[a] + 2, 10);   // the fill reads 8 distinct slots per block
         const int64_t bound = (int64_t)(q + t) * set->max_pen + 64;
         tier[a] = bound <= 8000 ? NW_I16 : NW_I32;
     }
@@ -280,7 +280,9 @@ extern "C" int hipsw_nw_run(hipsw_engine *eng, hipsw_nw_set *set, int32_t mode,
                 launches.push_back(NwLaunch{raw[r], raw[r + 1], raw[r + 2], raw[r + 3], raw[r + 4],
                                             raw[r + 5], raw[r + 6], raw[r + 7] != 0, raw[r + 8]});
         } else {
-            plan_pass(eng, set, active, w, mode, order, items, launches);
+            // Retry passes hold few tasks: one plain launch each.
+            plan_pass(eng, set, active, w, pass == 0 ? mode : HIPSW_MODE_PLAIN, order, items,
+                      launches);
             if (pass == 0) {
                 set->plan_order[mode] = order;
                 set->plan_items[mode] = items;
@@ -331,7 +333,8 @@ extern "C" int hipsw_nw_run(hipsw_engine *eng, hipsw_nw_set *set, int32_t mode,
             ok = eng->check(launch_nw(L.tier, L.global_state, set->dev, set->params, set->d_order,
                                       set->d_items + L.item_begin, L.item_count,
                                       eng->counters + k, L.S, zbase, L.zrows, ncw, L.grid,
-                                      mode == HIPSW_MODE_BATCHED, gstate, L.qw8, eng->stream),
+                                      pass == 0 && mode == HIPSW_MODE_BATCHED, gstate, L.qw8,
+                                      eng->stream),
                             "launch nw");
             ++k;
             ++launches_total;

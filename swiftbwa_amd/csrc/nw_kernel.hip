@@ -136,40 +136,53 @@ __device__ void nw_wave(const NwDev &d, const NwParams &p, const uint32_t *idx, 
             h_left = beg == 0 ? 0 - (p.oe_del + p.e_del * i) : kNegInf;
             col[sb * 64] = St::pack(h_left, St::e(v));
         }
-        int j = beg;
-        int s1 = sb + 1 == S ? 0 : sb + 1;                     // slot of column j+1
-        uint32_t zword = 0;
-        for (int c = 0; c < cmax; ++c) {                       // :108-141
-            // Every lane computes every step; lanes past their band end (or
-            // without a job) keep their LDS state and record no flags.
-            const bool inb = act && c < width;
-            const int jq = max(0, min(j, qlen - 1));
-            const int q = (int)((ql[(jq >> 3) * 64] >> ((jq & 7) * 4)) & 0xFu);
-            int h_cur = h_diag + (int)smat[t5 + q];
-            const Wd v = col[s1 * 64];
-            const int hd = St::h(v);
-            const int e_open = hd - p.oe_del;
-            const int e_ext = St::e(v) - p.e_del;
-            const uint32_t e_flag = e_open > e_ext ? 0u : 1u;
-            const int ev = e_open > e_ext ? e_open : e_ext;
-            const int f_open = h_left - p.oe_ins;
-            const int f_ext = f - p.e_ins;
-            const uint32_t f_flag = f_open > f_ext ? 0u : 2u;
-            f = f_open > f_ext ? f_open : f_ext;
-            const uint32_t from_e = ev > h_cur ? 4u : 0u;
-            h_cur = ev > h_cur ? ev : h_cur;
-            const uint32_t from_f = f > h_cur ? 8u : 0u;
-            h_cur = f > h_cur ? f : h_cur;
-            col[s1 * 64] = inb ? St::pack(h_cur, ev) : v;
-            h_diag = hd;
-            h_left = h_cur;
-            zword |= inb ? (e_flag | f_flag | from_e | from_f) << ((c & 7) * 4) : 0u;
-            ++j;
-            s1 = s1 + 1 == S ? 0 : s1 + 1;
-            if ((c & 7) == 7 || c == cmax - 1) {
-                if (i < zrows) zl[((size_t)i * ncw + (c >> 3)) * 64] = zword;
-                zword = 0;
+        // Cells go in blocks of 8 (one backpointer dword per lane): the 8 row
+        // state words and the 8 query codes of a block are read before any of
+        // the block's writes, so only the F/H chain is serial.  S >= 10 keeps
+        // the 8 slots of a block distinct.
+        int s1 = sb + 1 == S ? 0 : sb + 1;                     // slot of column beg+1
+        const int qlim = qw8 - 1;
+        for (int cb = 0; cb * 8 < cmax; ++cb) {                // :108-141
+            int sl[8];
+            Wd v[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                sl[k] = s1;
+                s1 = s1 + 1 == S ? 0 : s1 + 1;
             }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v[k] = col[sl[k] * 64];
+            const int j0 = beg + cb * 8;
+            const int wq = max(0, min(j0 >> 3, qlim));
+            const uint64_t win = (uint64_t)ql[wq * 64] | ((uint64_t)ql[min(wq + 1, qlim) * 64] << 32);
+            const uint32_t codes = (uint32_t)(win >> ((j0 & 7) * 4));
+            uint32_t zword = 0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                // Lanes past their band end (or without a job) keep their LDS
+                // state and record no flags.
+                const bool inb = act && cb * 8 + k < width;
+                const int q = (int)((codes >> (4 * k)) & 0xFu);
+                int h_cur = h_diag + (int)smat[t5 + q];
+                const int hd = St::h(v[k]);
+                const int e_open = hd - p.oe_del;
+                const int e_ext = St::e(v[k]) - p.e_del;
+                const uint32_t e_flag = e_open > e_ext ? 0u : 1u;
+                const int ev = e_open > e_ext ? e_open : e_ext;
+                const int f_open = h_left - p.oe_ins;
+                const int f_ext = f - p.e_ins;
+                const uint32_t f_flag = f_open > f_ext ? 0u : 2u;
+                f = f_open > f_ext ? f_open : f_ext;
+                const uint32_t from_e = ev > h_cur ? 4u : 0u;
+                h_cur = ev > h_cur ? ev : h_cur;
+                const uint32_t from_f = f > h_cur ? 8u : 0u;
+                h_cur = f > h_cur ? f : h_cur;
+                col[sl[k] * 64] = inb ? St::pack(h_cur, ev) : v[k];
+                h_diag = hd;
+                h_left = h_cur;
+                zword |= inb ? (e_flag | f_flag | from_e | from_f) << (4 * k) : 0u;
+            }
+            if (i < zrows) zl[((size_t)i * ncw + cb) * 64] = zword;
         }
         if (act) {
             // Column end+1 is read by the next row; in the full-width array it
