@@ -143,6 +143,21 @@ int  hipsw_nw_download(hipsw_engine *eng, hipsw_nw_set *set, int32_t *out_score,
                        int32_t *out_final_w);
 void hipsw_nw_free(hipsw_engine *eng, hipsw_nw_set *set);
 
+/* ---- mate-rescue local SW (replaces LocalSWDispatcher.dispatchBatch,
+ *      LocalSWDispatcher.swift:31-95) ----
+ * Per task LocalSWAligner.align (LocalSWAligner.swift:42-79): the 16-lane
+ * uint8 striped pass with its reverse pass, or the int32 scalar pass when
+ * qLen*matchScore > 250 or the striped pass overflows.  min_sub_score enables
+ * score2 (KSW_XSUBO; MateRescue passes minSeedLength*matchScore,
+ * MateRescue.swift:125).  out6 rows: {score, qb, qe (inclusive), tb, te
+ * (inclusive), score2}; ok[k] = 0 where the reference returns nil (the row is
+ * then all zeros).  Unlike MetalSW's LocalSWResult, score2 is returned. */
+int hipsw_local_sw(hipsw_engine *eng, int32_t n,
+                   const uint8_t *queries, const uint32_t *q_off, const uint16_t *q_len,
+                   const uint8_t *targets, const uint32_t *t_off, const uint16_t *t_len,
+                   int32_t min_sub_score, int32_t match_score, int32_t mismatch_penalty,
+                   const hipsw_scoring *sc, int32_t *out6, int32_t *ok, hipsw_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,351 @@
+// Mate-rescue local Smith-Waterman for gfx950, exact with LocalSWAligner.
+//
+// Semantics: LocalSWAligner.align, Sources/Alignment/LocalSWAligner.swift:42-79:
+// a 16-lane uint8 Farrar striped pass (:133-306) with a single-pass lazy-F
+// (:234-242), stripe-then-lane tie order (:250-262) and score2 peaks (:266-303);
+// a reverse pass on the reversed prefixes recovers the start (:98-124); any
+// overflow, an over-long query (qLen*A > 250) or a zero score falls back to the
+// int32 scalar pass (:327-492).
+//
+// Layout of the striped pass: a 16-lane DPP row of the wave is one task and
+// lane l of the row is SIMD byte-lane l of the reference's SIMD16<UInt8>, so
+// every vector op of the reference is one VALU op, `shiftRight` is a DPP
+// row_shr:1 with zero fill, and the per-stripe horizontal max is a 4-step
+// in-row reduction.  Four tasks share a wave.  The u8 saturating arithmetic is
+// done in 32-bit registers with explicit clamps.
+//
+// Layout of the scalar fallback: one task per lane, h/e rows in a per-wave HBM
+// region interleaved as [column][lane].
+#include "hipsw_internal.h"
+#include "local_internal.h"
+
+namespace hipsw {
+
+__device__ __forceinline__ int row_shr1(int v)
+{
+    // DPP row_shr:1 (0x111), bound_ctrl: lane 0 of each 16-lane row reads 0.
+    return __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, true);
+}
+
+__device__ __forceinline__ int row_max16(int v)
+{
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) v = max(v, __shfl_xor(v, o, 16));
+    return v;
+}
+
+__device__ __forceinline__ int satsub(int x, int c) { return max(x - c, 0); }
+
+// Base k of a 2-bit + N-mask sequence (k must be in range).
+__device__ __forceinline__ int base_at(const LocalDev &d, uint32_t w0, uint32_t n0, int k)
+{
+    int b = (int)((d.seq[w0 + (k >> 4)] >> ((k & 15) * 2)) & 3u);
+    if (n0 != kNoN && ((d.nmask[n0 + (k >> 5)] >> (k & 31)) & 1u)) b = 4;
+    return b;
+}
+
+// One striped pass (simdSWPass, :133-306) for the task of this lane's 16-lane
+// row.  `rev` reads the reversed prefixes query[0..qlen) / target[0..tlen).
+// Returns false on 8-bit overflow.  colmax[i] receives the column maxima when
+// peaks are tracked (min_sub > 0).
+template <int SNMAX>
+__device__ bool striped_pass(const LocalDev &d, const LocalParams &p, bool act, uint32_t qw0,
+                             uint32_t qn0, uint32_t tw0, uint32_t tn0, int qlen, int tlen,
+                             bool rev, int min_sub, uint8_t *colmax, int &o_score, int &o_qend,
+                             int &o_tend, int &o_score2)
+{
+    const int lane16 = threadIdx.x & 15;
+    const int grp = (threadIdx.x & 63) >> 4;
+    const int sn = act ? (qlen + 15) / 16 : 0;
+    const int bias = p.bias, goe = p.gap_oe, ge = p.gap_e;
+    // striped profile, :152-167: lane l of stripe s holds sat8(mat[k*5+q[s+l*sn]] + bias),
+    // 0 for padding.  Packed 4 stripes per register, one set per target code k.
+    constexpr int NR = (SNMAX + 3) / 4;
+    int qc[SNMAX];                                             // query code per stripe, -1 = padding
+#pragma unroll
+    for (int s = 0; s < SNMAX; ++s) {
+        const int j = s + lane16 * sn;
+        qc[s] = (act && s < sn && j < qlen) ? base_at(d, qw0, qn0, rev ? qlen - 1 - j : j) : -1;
+    }
+    // prof[k][r] of this lane lives in LDS at prof_lds[(k*NR + r)*64 + lane]:
+    // the row's target code selects a set with a per-lane address.
+    __shared__ uint32_t prof_lds[5 * 4 * 64];
+    uint32_t *prof = prof_lds + (threadIdx.x & 63);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            uint32_t x = 0;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int s = r * 4 + b;
+                if (s < SNMAX) {
+                    const int q = qc[s];
+                    const int v = q < 0 ? 0 : min(max((int)p.mat[k * 5 + max(q, 0)] + bias, 0), 255);
+                    x |= (uint32_t)v << (b * 8);
+                }
+            }
+            prof[(k * NR + r) * 64] = x;
+        }
+    }
+    int H[SNMAX], E[SNMAX];
+#pragma unroll
+    for (int s = 0; s < SNMAX; ++s) { H[s] = 0; E[s] = 0; }
+    int max_score = 0, max_i = -1, max_j = -1;
+    bool ok = true;
+    const int tmax_w = [&] {
+        int v = act ? tlen : 0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+        return v;
+    }();
+    for (int i = 0; i < tmax_w; ++i) {
+        const bool live = act && ok && i < tlen;
+        int tb = 0;
+        if (live) {
+            const int ti = rev ? tlen - 1 - i : i;             // reversed prefix of length tlen
+            tb = min(base_at(d, tw0, tn0, ti), 4);
+        }
+        uint32_t pr[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) pr[r] = prof[(tb * NR + r) * 64];
+        // vH = shiftRight(H[sn-1]), :206
+        int hlast = 0;
+#pragma unroll
+        for (int s = 0; s < SNMAX; ++s) hlast = s == sn - 1 ? H[s] : hlast;
+        int vH = row_shr1(hlast);
+        int f = 0;
+#pragma unroll
+        for (int s = 0; s < SNMAX; ++s) {                      // :209-231
+            if (s < sn && live) {
+                const int pv = (int)((pr[s >> 2] >> ((s & 3) * 8)) & 0xFFu);
+                int hn = (vH + pv) & 0xFF;                     // wrapping &+
+                hn = satsub(hn, bias);
+                const int hp = H[s];
+                vH = hp;
+                int ev = max(satsub(hp, goe), E[s]);
+                ev = satsub(ev, ge);
+                E[s] = ev;
+                hn = max(max(hn, ev), f);
+                H[s] = hn;
+                f = satsub(max(satsub(hn, goe), f), ge);
+            }
+        }
+        // lazy-F, one pass with an early break (:234-242)
+        int cf = row_shr1(f);
+        bool go = live;
+#pragma unroll
+        for (int s = 0; s < SNMAX; ++s) {
+            const bool in = go && s < sn;
+            const int nh = max(H[s], cf);
+            const uint64_t changed = __ballot(in && nh != H[s]);
+            const bool any = ((changed >> (grp * 16)) & 0xFFFFull) != 0;
+            go = in && any;
+            if (go) {
+                H[s] = nh;
+                cf = satsub(max(satsub(nh, goe), cf), ge);
+            }
+        }
+        // column max and best cell (:245-263)
+        int col_max = 0;
+#pragma unroll
+        for (int s = 0; s < SNMAX; ++s) {
+            const bool in = live && s < sn;
+            const int lm = row_max16(in ? H[s] : 0);
+            if (in) {
+                if (lm > 250) ok = false;
+                if (ok) {
+                    col_max = max(col_max, lm);
+                }
+            }
+            const bool better = in && ok && lm > max_score;
+            const int j = s + lane16 * sn;
+            const uint64_t hit = __ballot(better && H[s] == lm && j < qlen);
+            if (better) {
+                max_score = lm;
+                max_i = i;
+                const uint32_t m16 = (uint32_t)((hit >> (grp * 16)) & 0xFFFFull);
+                if (m16) max_j = s + (__ffs((int)m16) - 1) * sn;
+            }
+        }
+        if (live && ok && min_sub > 0 && lane16 == 0) colmax[i] = (uint8_t)col_max;
+    }
+    o_score = max_score;
+    o_qend = max_j;
+    o_tend = max_i;
+    o_score2 = 0;
+    // score2 from the peak list (:266-303), replayed from the column maxima
+    if (act && ok && min_sub > 0 && lane16 == 0 && max_score > 0 && p.max_base_u8 > 0) {
+        const int r = (max_score + p.max_base_u8 - 1) / p.max_base_u8;
+        const int lo = max_i - r, hi = max_i + r;
+        int s2 = 0;
+        int pk_col = -2, pk_sc = -1;
+        auto close = [&]() {
+            if (pk_sc >= 0 && (pk_col < lo || pk_col > hi) && pk_sc > s2) s2 = pk_sc;
+        };
+        for (int i = 0; i < tlen; ++i) {
+            const int cm = colmax[i];
+            if (cm < min_sub) continue;
+            if (pk_sc < 0 || pk_col + 1 != i) {
+                close();
+                pk_col = i;
+                pk_sc = cm;
+            } else if (pk_sc < cm) {
+                pk_col = i;
+                pk_sc = cm;
+            }
+        }
+        close();
+        o_score2 = s2;
+    }
+    o_score2 = __shfl(o_score2, (threadIdx.x & 63) & ~15, 64);
+    return ok;
+}
+
+template <int SNMAX>
+__global__ __launch_bounds__(64) void local_striped_kernel(LocalDev d, LocalParams p,
+                                                           const uint32_t *order, int n,
+                                                           uint8_t *colmax_base, int tcap)
+{
+    const int grp = threadIdx.x >> 4;
+    const int slot = blockIdx.x * 4 + grp;
+    const bool valid = slot < n;
+    const int job = valid ? (int)order[slot] : 0;
+    const int qlen = valid ? d.q_len[job] : 0;
+    const int tlen = valid ? d.t_len[job] : 0;
+    const uint32_t qw0 = valid ? d.q_word[job] : 0, tw0 = valid ? d.t_word[job] : 0;
+    const uint32_t qn0 = valid ? d.qn_word[job] : kNoN, tn0 = valid ? d.tn_word[job] : kNoN;
+    uint8_t *colmax = colmax_base + (size_t)slot * tcap;
+    const bool act = valid && qlen > 0 && tlen > 0 && qlen <= 16 * SNMAX;
+    int fs, fq, ft, f2;
+    bool ok = striped_pass<SNMAX>(d, p, act, qw0, qn0, tw0, tn0, qlen, tlen, false, p.min_sub,
+                                  colmax, fs, fq, ft, f2);
+    // reverse pass on the reversed prefixes (:100-115)
+    const bool act2 = act && ok && fs > 0 && fq >= 0;
+    int rs = 0, rq = 0, rt = 0, r2 = 0;
+    bool ok2 = striped_pass<SNMAX>(d, p, act2, qw0, qn0, tw0, tn0, act2 ? fq + 1 : 0,
+                                   act2 ? ft + 1 : 0, true, 0, colmax, rs, rq, rt, r2);
+    if (!valid || (threadIdx.x & 15) != 0) return;
+    int32_t *o = d.out + (size_t)job * 6;
+    if (act2 && ok2) {
+        o[0] = fs; o[1] = fq - rq; o[2] = fq; o[3] = ft - rt; o[4] = ft; o[5] = f2;   // :117-124
+        d.status[job] = LOCAL_OK;
+    } else {
+        d.status[job] = LOCAL_NEEDS_SCALAR;                    // :70-78
+    }
+}
+
+// scalarSWPass (:383-492): query-outer int32 DP, strict '>' ties; h/e rows of
+// tlen+1 in HBM as [column][lane]; peaks over query rows from rowmax[].
+__device__ void scalar_pass(const LocalDev &d, const LocalParams &p, bool act, uint32_t qw0,
+                            uint32_t qn0, uint32_t tw0, uint32_t tn0, int qcut, int tcut,
+                            bool rev, int min_sub, int *hcol, int *ecol, int *rowmax,
+                            int &o_score, int &o_qend, int &o_tend, int &o_score2)
+{
+    // qcut/tcut: lengths of the query/target prefixes aligned (reversed if rev)
+    int best = 0, bq = -1, bt = -1;
+    if (act) {
+        for (int j = 0; j <= tcut; ++j) { hcol[j * 64] = 0; ecol[j * 64] = 0; }
+        for (int i = 0; i < qcut; ++i) {
+            const int qi = rev ? qcut - 1 - i : i;
+            const int qb = min(base_at(d, qw0, qn0, qi), 4);
+            int f = 0, hprev = 0, row_max = 0;
+            for (int j = 0; j < tcut; ++j) {
+                const int tj = rev ? tcut - 1 - j : j;
+                const int tb = min(base_at(d, tw0, tn0, tj), 4);
+                int hc = hprev + (int)p.mat[qb * 5 + tb];
+                const int hj1 = hcol[(j + 1) * 64];
+                const int fo = hj1 - p.gap_oe, fx = f - p.gap_e;
+                f = fo > fx ? fo : fx;
+                const int eo = hcol[j * 64] - p.gap_oe, ex = ecol[(j + 1) * 64] - p.gap_e;
+                const int ev = eo > ex ? eo : ex;
+                ecol[(j + 1) * 64] = ev;
+                hc = hc > f ? hc : f;
+                hc = hc > ev ? hc : ev;
+                hc = hc > 0 ? hc : 0;
+                hprev = hj1;
+                hcol[(j + 1) * 64] = hc;
+                if (hc > row_max) row_max = hc;
+                if (hc > best) { best = hc; bq = i; bt = j; }
+            }
+            rowmax[i * 64] = row_max;
+        }
+    }
+    o_score = best; o_qend = bq; o_tend = bt; o_score2 = 0;
+    if (act && min_sub > 0 && best > 0 && p.max_base_i > 0) {
+        const int r = (best + p.max_base_i - 1) / p.max_base_i;
+        const int lo = bq - r, hi = bq + r;
+        int s2 = 0, pk_col = -2, pk_sc = -1;
+        auto close = [&]() {
+            if (pk_sc >= 0 && (pk_col < lo || pk_col > hi) && pk_sc > s2) s2 = pk_sc;
+        };
+        for (int i = 0; i < qcut; ++i) {
+            const int rm = rowmax[i * 64];
+            if (rm < min_sub) continue;
+            if (pk_sc < 0 || pk_col + 1 != i) { close(); pk_col = i; pk_sc = rm; }
+            else if (pk_sc < rm) { pk_col = i; pk_sc = rm; }
+        }
+        close();
+        o_score2 = s2;
+    }
+}
+
+__global__ __launch_bounds__(64) void local_scalar_kernel(LocalDev d, LocalParams p,
+                                                          const uint32_t *order, int n,
+                                                          int *scratch, int tcap, int qcap)
+{
+    const int lane = threadIdx.x;
+    const int slot = blockIdx.x * 64 + lane;
+    const bool valid = slot < n;
+    const int job = valid ? (int)order[slot] : 0;
+    const int qlen = valid ? d.q_len[job] : 0;
+    const int tlen = valid ? d.t_len[job] : 0;
+    const uint32_t qw0 = valid ? d.q_word[job] : 0, tw0 = valid ? d.t_word[job] : 0;
+    const uint32_t qn0 = valid ? d.qn_word[job] : kNoN, tn0 = valid ? d.tn_word[job] : kNoN;
+    const bool act = valid && qlen > 0 && tlen > 0;
+    int *base = scratch + (size_t)blockIdx.x * 64 * (2 * (tcap + 1) + qcap);
+    int *hcol = base + lane, *ecol = base + (tcap + 1) * 64 + lane;
+    int *rowmax = base + 2 * (tcap + 1) * 64 + lane;
+    int fs, fq, ft, f2;
+    scalar_pass(d, p, act, qw0, qn0, tw0, tn0, qlen, tlen, false, p.min_sub, hcol, ecol, rowmax,
+                fs, fq, ft, f2);
+    const bool act2 = act && fs > 0;
+    int rs, rq, rt, r2;
+    scalar_pass(d, p, act2, qw0, qn0, tw0, tn0, act2 ? fq + 1 : 0, act2 ? ft + 1 : 0, true, 0,
+                hcol, ecol, rowmax, rs, rq, rt, r2);
+    if (!valid) return;
+    int32_t *o = d.out + (size_t)job * 6;
+    if (act2) {
+        o[0] = fs; o[1] = fq - rq; o[2] = fq; o[3] = ft - rt; o[4] = ft; o[5] = f2;
+        d.status[job] = LOCAL_OK;
+    } else {
+        o[0] = o[1] = o[2] = o[3] = o[4] = o[5] = 0;
+        d.status[job] = LOCAL_NIL;                             // :348
+    }
+}
+
+hipError_t launch_local_striped(int snmax, const LocalDev &d, const LocalParams &p,
+                                const uint32_t *order, int n, uint8_t *colmax, int tcap,
+                                hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int grid = (n + 3) / 4;
+    switch (snmax) {
+    case 4: hipLaunchKernelGGL((local_striped_kernel<4>), dim3(grid), dim3(64), 0, s, d, p, order, n, colmax, tcap); break;
+    case 8: hipLaunchKernelGGL((local_striped_kernel<8>), dim3(grid), dim3(64), 0, s, d, p, order, n, colmax, tcap); break;
+    case 12: hipLaunchKernelGGL((local_striped_kernel<12>), dim3(grid), dim3(64), 0, s, d, p, order, n, colmax, tcap); break;
+    default: hipLaunchKernelGGL((local_striped_kernel<16>), dim3(grid), dim3(64), 0, s, d, p, order, n, colmax, tcap); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_local_scalar(const LocalDev &d, const LocalParams &p, const uint32_t *order,
+                               int n, int *scratch, int tcap, int qcap, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(local_scalar_kernel, dim3((n + 63) / 64), dim3(64), 0, s, d, p, order, n,
+                       scratch, tcap, qcap);
+    return hipGetLastError();
+}
+
+}  // namespace hipsw

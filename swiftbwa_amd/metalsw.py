@@ -161,13 +161,36 @@ class HipSWEngine:
         return score, cig, ncig, flags, fw
 
 
+    def local_sw(self, jobs, scoring: ScoringParameters, min_sub_score=0, stats=None):
+        """hipsw_local_sw on a job dict; returns (out[n, 6], ok[n])."""
+        n = int(jobs["n"])
+        out = np.zeros((n, 6), np.int32)
+        ok = np.zeros(n, np.int32)
+        if n == 0:
+            return out, ok
+        a = _ext_arrays(jobs)
+        sc = scoring.to_c()
+        st = native.HipswStats()
+        rc = self.lib.hipsw_local_sw(
+            self.handle, n, native.ptr(a["queries"]), native.ptr(a["q_off"]),
+            native.ptr(a["q_len"]), native.ptr(a["targets"]), native.ptr(a["t_off"]),
+            native.ptr(a["t_len"]), int(min_sub_score), int(scoring.matchScore),
+            int(scoring.mismatchPenalty), C.byref(sc), native.ptr(out), native.ptr(ok),
+            C.byref(st))
+        if rc != native.HIPSW_OK:
+            raise native.HipswError(rc, self.last_error())
+        if stats is not None:
+            stats.update(st.as_dict())
+        return out, ok
+
+
 def _ext_arrays(jobs):
     def c(name, dt):
         return np.ascontiguousarray(jobs[name], dtype=dt)
     a = {"queries": c("queries", np.uint8), "q_off": c("q_off", np.uint32),
          "q_len": c("q_len", np.uint16), "targets": c("targets", np.uint8),
          "t_off": c("t_off", np.uint32), "t_len": c("t_len", np.uint16),
-         "w": c("w", np.int32)}
+         "w": c("w", np.int32) if "w" in jobs else np.zeros(int(jobs["n"]), np.int32)}
     n = int(jobs["n"])
     for name in ("h0", "end_bonus"):
         a[name] = c(name, np.int32) if name in jobs else np.zeros(n, np.int32)
@@ -265,3 +288,42 @@ class GlobalSWDispatcher:
             return []
         return [GlobalSWResult(int(score[k]), [int(x) for x in cig[k, :ncig[k]]])
                 for k in range(len(tasks))]
+
+
+@dataclass
+class LocalSWTask:
+    """LocalSWDispatcher.swift:6-16."""
+    query: bytes
+    target: bytes
+    scoring: ScoringParameters = field(default_factory=ScoringParameters)
+
+
+@dataclass
+class LocalSWResult:
+    """LocalSWAligner.swift:4-23 (ends inclusive); score2 included."""
+    score: int
+    queryBegin: int
+    queryEnd: int
+    targetBegin: int
+    targetEnd: int
+    score2: int = 0
+
+
+class LocalSWDispatcher:
+    """LocalSWDispatcher.swift:31-95 with LocalSWAligner semantics: one result
+    per task, None where the CPU aligner returns nil.  Returns [] on failure."""
+
+    @staticmethod
+    def dispatchBatch(tasks: List[LocalSWTask], engine: HipSWEngine,
+                      minSubScore: int = 0) -> List[Optional[LocalSWResult]]:
+        if not tasks:
+            return []
+        sc = tasks[0].scoring
+        if any(t.scoring != sc for t in tasks):
+            return []
+        jobs = tasks_to_jobs([BandedSWTask(t.query, t.target, 0, 0, sc) for t in tasks])
+        try:
+            out, ok = engine.local_sw(jobs, sc, min_sub_score=minSubScore)
+        except native.HipswError:
+            return []
+        return [LocalSWResult(*map(int, out[k])) if ok[k] else None for k in range(len(tasks))]
