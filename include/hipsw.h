@@ -158,6 +158,62 @@ int hipsw_local_sw(hipsw_engine *eng, int32_t n,
                    int32_t min_sub_score, int32_t match_score, int32_t mismatch_penalty,
                    const hipsw_scoring *sc, int32_t *out6, int32_t *ok, hipsw_stats *stats);
 
+/* ---- serving loop (continuous batching over a resident reference) ----
+ * No MetalSW counterpart (MetalSW dispatches one synchronous batch per call,
+ * MetalDispatchHelper.swift:67-89).  A chunk carries the hot-path jobs of a
+ * batch of reads; every target is a window of the resident reference given as
+ * (BWT-space position, length), fetched on the device exactly as
+ * FMIndex.getReference does (FMIndex.swift:25-65), and reversed afterwards when
+ * ext_t_rev[k] is set (left extensions, ExtensionAligner.swift:219-225).
+ * Chunks are admitted FIFO onto n_slots streams while their device memory fits
+ * hbm_budget next to the running chunks, so chunks overlap on the device.
+ * Results are those of hipsw_banded_extend / hipsw_local_sw / hipsw_global_nw
+ * (max_retries 3) on the same jobs. */
+typedef struct hipsw_server hipsw_server;
+
+typedef struct {
+    int32_t n_ext;                  /* banded extensions */
+    const uint8_t *ext_q; const uint32_t *ext_q_off; const uint16_t *ext_q_len;
+    const int64_t *ext_t_pos; const uint16_t *ext_t_len; const uint8_t *ext_t_rev;
+    const int32_t *ext_h0, *ext_w, *ext_eb;
+    int32_t n_loc;                  /* mate-rescue local SW */
+    const uint8_t *loc_q; const uint32_t *loc_q_off; const uint16_t *loc_q_len;
+    const int64_t *loc_t_pos; const uint16_t *loc_t_len;
+    int32_t loc_min_sub;
+    int32_t n_nw;                   /* CIGAR NW */
+    const uint8_t *nw_q; const uint32_t *nw_q_off; const uint16_t *nw_q_len;
+    const int64_t *nw_t_pos; const uint16_t *nw_t_len;
+    const int32_t *nw_w, *nw_true_score;
+} hipsw_chunk;
+
+typedef struct {
+    int32_t *ext_out;               /* n_ext x 6 (nullable)                     */
+    int32_t *loc_out, *loc_ok;      /* n_loc x 6, n_loc                         */
+    int32_t *nw_score;              /* n_nw                                     */
+    uint32_t *nw_cigar;             /* n_nw x nw_cap                            */
+    int32_t *nw_n_cigar, *nw_flags; /* n_nw                                     */
+    double latency_ms;              /* submit -> results ready                  */
+    double queue_ms, run_ms;
+    int64_t cells;                  /* extension + NW fill cells                */
+} hipsw_chunk_result;
+
+hipsw_server *hipsw_server_create(hipsw_engine *eng, int32_t n_slots, int64_t hbm_budget,
+                                  const hipsw_scoring *sc, int32_t match_score,
+                                  int32_t mismatch_penalty, int32_t nw_cap);
+/* Uploads the 2-bit .pac (4 bases per byte, first base in the high bits). */
+int     hipsw_server_set_reference(hipsw_server *srv, const uint8_t *pac, int64_t n_bases);
+/* Copies the chunk and queues it; returns a ticket (> 0) or a negative status
+ * (HIPSW_E_BUDGET when the chunk alone exceeds the budget). */
+int64_t hipsw_server_submit(hipsw_server *srv, const hipsw_chunk *chunk);
+/* HIPSW_OK when the ticket's results are ready, HIPSW_E_PENDING otherwise. */
+int     hipsw_server_poll(hipsw_server *srv, int64_t ticket);
+/* Blocks until ready, copies results into `res` and retires the ticket. */
+int     hipsw_server_wait(hipsw_server *srv, int64_t ticket, hipsw_chunk_result *res);
+int     hipsw_server_stats(hipsw_server *srv, int64_t *in_use, int64_t *peak_in_use,
+                           int32_t *running, int32_t *queued);
+/* Stops the slots; call after every ticket has been waited for. */
+void    hipsw_server_destroy(hipsw_server *srv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -237,3 +237,48 @@ def local_batch(js, sc=None, min_sub_score=0, nthreads=1):
                                  to.ctypes.data, tl.ctypes.data, int(min_sub_score), C.byref(sc),
                                  out.ctypes.data, ok.ctypes.data, int(nthreads))
     return out, ok, cells
+
+
+def get_reference(ref, gl, pos, length):
+    """FMIndex.getReference (Sources/FMIndex/FMIndex.swift:25-65) over a forward
+    reference given as bytes 0..3: positions >= gl read the reverse complement."""
+    if pos >= gl:
+        fend = 2 * gl - pos
+        fstart = fend - length
+        s, e = max(0, fstart), min(fend, gl)
+        if e <= s:
+            return np.zeros(0, np.uint8)
+        return (3 - ref[s:e][::-1]).astype(np.uint8)
+    n = min(length, gl - pos)
+    if n <= 0 or pos < 0:
+        return np.zeros(0, np.uint8)
+    return ref[pos:pos + n].astype(np.uint8)
+
+
+def chunk_to_job_sets(ref, chunk):
+    """Materialises a serving chunk's windows (test side) into the job dicts the
+    batch oracles take: extension (with left-window reversal), local SW, NW."""
+    gl = len(ref)
+    out = {}
+    for kind in ("ext", "loc", "nw"):
+        j = chunk[kind]
+        n = j["n"]
+        ts, tl = [], []
+        for k in range(n):
+            t = get_reference(ref, gl, int(j["t_pos"][k]), int(j["t_len"][k]))
+            if kind == "ext" and j["t_rev"][k]:
+                t = t[::-1]
+            ts.append(t)
+            tl.append(len(t))
+        t_off = np.zeros(n, np.uint32)
+        if n:
+            t_off[1:] = np.cumsum(tl)[:-1]
+        d = {"n": n, "queries": j["q"], "q_off": j["q_off"], "q_len": j["q_len"],
+             "targets": np.concatenate(ts) if sum(tl) else np.zeros(1, np.uint8),
+             "t_off": t_off, "t_len": np.array(tl, np.uint16)}
+        if kind == "ext":
+            d.update(h0=j["a"], w=j["b"], end_bonus=j["c"])
+        if kind == "nw":
+            d.update(w=j["a"], true_score=j["b"])
+        out[kind] = d
+    return out

@@ -15,7 +15,7 @@ OBJ = os.path.join(HERE, "_build")
 INCLUDE = os.path.join(ROOT, "include")
 
 HIP_SOURCES = ["engine.hip", "ext_host.hip", "ext_kernel.hip", "nw_host.hip", "nw_kernel.hip",
-               "local_host.hip", "local_kernel.hip"]
+               "local_host.hip", "local_kernel.hip", "server.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
@@ -55,7 +55,7 @@ def build_workload():
     out = os.path.join(HERE, "libswwork.so")
     tmp = out + ".tmp"
     _run(["gcc", "-O3", "-std=c11", "-fPIC", "-shared", "-Wall", "-o", tmp,
-          os.path.join(CSRC, "workload.c")])
+          os.path.join(CSRC, "workload.c"), "-lm"])
     os.replace(tmp, out)
     return out
 

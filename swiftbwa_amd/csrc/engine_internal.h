@@ -72,6 +72,11 @@ struct ExtLaunch {
     size_t scratch_off;   // bytes into the engine scratch (global_state only)
 };
 
+size_t plan_ext_batched(const hipsw_engine *eng, int n, const int32_t *ql, const int32_t *tl,
+                        const int32_t *h0, const int32_t *w, int max_mat,
+                        std::vector<uint32_t> &order, std::vector<WorkItem> &items,
+                        std::vector<ExtLaunch> &launches);
+
 }  // namespace hipsw
 
 struct hipsw_ext_set {

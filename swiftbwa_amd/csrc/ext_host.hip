@@ -59,6 +59,56 @@ size_t span(size_t count)
 
 }  // namespace
 
+namespace hipsw {
+
+// Batched form: bin by (tier, query-length class); inside a bin order by
+// estimated DP area, largest first; cut into 64-job work items; one persistent
+// launch per bin.  Returns the global-state scratch bytes the launches need.
+size_t plan_ext_batched(const hipsw_engine *eng, int n, const int32_t *ql, const int32_t *tl,
+                        const int32_t *h0, const int32_t *w, int max_mat,
+                        std::vector<uint32_t> &order, std::vector<WorkItem> &items,
+                        std::vector<ExtLaunch> &launches)
+{
+    std::vector<uint32_t> keys(n), idx(n);
+    for (int k = 0; k < n; ++k) {
+        const uint32_t bin = (uint32_t)(tier_of(h0[k], ql[k], max_mat) * kClasses + class_of(ql[k]));
+        const uint64_t band = (uint64_t)std::min<int64_t>(ql[k], 2 * (int64_t)std::max(w[k], 0) + 1);
+        const uint64_t area = std::min<uint64_t>((uint64_t)tl[k] * band, (1ull << 28) - 1);
+        keys[k] = (bin << 28) | (uint32_t)(((1ull << 28) - 1) - area);   // bin asc, area desc
+        idx[k] = (uint32_t)k;
+    }
+    radix_sort_by_key(keys, idx);
+    order = idx;
+    items.clear();
+    launches.clear();
+    items.reserve((size_t)n / 64 + 16);
+    size_t scratch_off = 0;
+    for (int s = 0; s < n;) {
+        const uint32_t bin = keys[s] >> 28;
+        int e = s, cmax = 0;
+        while (e < n && (keys[e] >> 28) == bin) {
+            cmax = std::max(cmax, ql[idx[e]]);
+            ++e;
+        }
+        ExtLaunch L;
+        L.tier = (int)bin / kClasses;
+        L.cols = cmax + 1;
+        L.global_state = ext_lds_bytes(L.tier, false, L.cols) > (size_t)eng->lds_per_block;
+        L.item_begin = (int)items.size();
+        for (int g = s; g < e; g += 64) items.push_back(WorkItem{(uint32_t)g, (uint32_t)std::min(64, e - g)});
+        L.item_count = (int)items.size() - L.item_begin;
+        const int wpc = L.global_state ? 8 : waves_per_cu(eng, ext_lds_bytes(L.tier, false, L.cols));
+        L.grid = std::max(1, std::min(L.item_count, eng->n_cu * wpc));
+        L.scratch_off = scratch_off;
+        if (L.global_state) scratch_off += (size_t)L.grid * L.cols * 64 * ext_state_bytes(L.tier);
+        launches.push_back(L);
+        s = e;
+    }
+    return scratch_off;
+}
+
+}  // namespace hipsw
+
 extern "C" hipsw_ext_set *hipsw_ext_upload(hipsw_engine *eng, int32_t n,
                                            const uint8_t *queries, const uint32_t *q_off,
                                            const uint16_t *q_len, const uint8_t *targets,
@@ -136,12 +186,11 @@ extern "C" hipsw_ext_set *hipsw_ext_upload(hipsw_engine *eng, int32_t n,
     }
 
     // Exact score tier per job.
-    std::vector<uint8_t> tier(n);
     int plain_tier = TIER_U8, qmax = 0;
     for (int k = 0; k < n; ++k) {
-        tier[k] = (uint8_t)tier_of(vh0[k], ql[k], max_mat);
-        set->tier_jobs[tier[k]]++;
-        plain_tier = std::max(plain_tier, (int)tier[k]);
+        const int tr = tier_of(vh0[k], ql[k], max_mat);
+        set->tier_jobs[tr]++;
+        plain_tier = std::max(plain_tier, tr);
         qmax = std::max(qmax, ql[k]);
     }
 
@@ -158,48 +207,11 @@ extern "C" hipsw_ext_set *hipsw_ext_upload(hipsw_engine *eng, int32_t n,
         set->plain_grid = std::max(1, groups);
     }
 
-    // Batched form: bin by (tier, query-length class); inside a bin order by
-    // estimated DP area, largest first; cut into 64-job work items.
-    std::vector<uint64_t> keyed(n);
-    for (int k = 0; k < n; ++k) {
-        const int bin = tier[k] * kClasses + class_of(ql[k]);
-        const uint64_t band = (uint64_t)std::min<int64_t>(ql[k], 2 * (int64_t)std::max(vw[k], 0) + 1);
-        const uint64_t area = std::min<uint64_t>((uint64_t)tl[k] * band, (1ull << 31) - 1);
-        // bin ascending, area descending, index ascending (stable)
-        keyed[k] = ((uint64_t)bin << 59) | ((((1ull << 31) - 1) - area) << 27) | 0;
-        (void)area;
-    }
-    std::vector<uint32_t> order(n);
-    std::iota(order.begin(), order.end(), 0u);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](uint32_t a, uint32_t b) { return keyed[a] < keyed[b]; });
+    // Batched form (see plan_ext_batched).
+    std::vector<uint32_t> order;
     std::vector<WorkItem> items;
-    items.reserve((size_t)groups + 16);
-    size_t scratch_off = 0;
-    for (int s = 0; s < n;) {
-        const int bin = (int)(keyed[order[s]] >> 59);
-        int e = s;
-        int cmax = 0;
-        while (e < n && (int)(keyed[order[e]] >> 59) == bin) {
-            cmax = std::max(cmax, ql[order[e]]);
-            ++e;
-        }
-        ExtLaunch L;
-        L.tier = bin / kClasses;
-        L.cols = cmax + 1;
-        L.global_state = ext_lds_bytes(L.tier, false, L.cols) > (size_t)eng->lds_per_block;
-        L.item_begin = (int)items.size();
-        for (int g = s; g < e; g += 64) items.push_back(WorkItem{(uint32_t)g, (uint32_t)std::min(64, e - g)});
-        L.item_count = (int)items.size() - L.item_begin;
-        const int wpc = L.global_state ? 8 : waves_per_cu(eng, ext_lds_bytes(L.tier, false, L.cols));
-        L.grid = std::max(1, std::min(L.item_count, eng->n_cu * wpc));
-        L.scratch_off = scratch_off;
-        if (L.global_state) scratch_off += (size_t)L.grid * L.cols * 64 * ext_state_bytes(L.tier);
-        set->launches.push_back(L);
-        s = e;
-    }
-    set->batched_scratch = scratch_off;
-
+    set->batched_scratch = plan_ext_batched(eng, n, ql.data(), tl.data(), vh0.data(), vw.data(),
+                                            max_mat, order, items, set->launches);
     // One device allocation for the whole set.
     const size_t bytes = span<uint32_t>(words) + span<uint32_t>(nwords) + 4 * span<uint32_t>(n) +
                          6 * span<int32_t>(n) + span<int32_t>((size_t)n * 6) +

@@ -46,12 +46,6 @@ T *carve(char *&cur, size_t count)
 template <typename T>
 size_t span(size_t count) { return (count * sizeof(T) + 255) & ~(size_t)255; }
 
-struct NwLaunch {
-    int tier, S, ncol_max, zrows, item_begin, item_count, grid;
-    bool global_state;
-    int qw8;     // staged query words (8 bases each) per lane
-};
-
 int s_class(int S)
 {
     if (S <= 32) return 0;
@@ -170,20 +164,20 @@ extern "C" hipsw_nw_set *hipsw_nw_upload(hipsw_engine *eng, int32_t n, const uin
 }
 
 // Builds the launches of one pass over `active` (job ids) with bands w[].
-static void plan_pass(const hipsw_engine *eng, const hipsw_nw_set *set,
-                      const std::vector<uint32_t> &active, const std::vector<int32_t> &w,
-                      int mode, std::vector<uint32_t> &order, std::vector<WorkItem> &items,
-                      std::vector<NwLaunch> &launches)
+void hipsw::nw_plan_pass(const hipsw_engine *eng, const int32_t *qlv, const int32_t *tlv,
+                         int max_pen, const std::vector<uint32_t> &active,
+                         const std::vector<int32_t> &w, int mode, std::vector<uint32_t> &order,
+                         std::vector<WorkItem> &items, std::vector<NwLaunch> &launches)
 {
     const int m = (int)active.size();
     std::vector<int> ncol(m), S(m), tier(m);
     for (int a = 0; a < m; ++a) {
         const int k = (int)active[a];
-        const int q = set->ql[k], t = set->tl[k];
+        const int q = qlv[k], t = tlv[k];
         const int wp = std::max(std::min(w[k], 1 << 20), std::abs(q - t));
         ncol[a] = std::min(q, 2 * wp + 1);
         S[a] = std::max(ncol[a] + 2, 10);   // the fill reads 8 distinct slots per block
-        const int64_t bound = (int64_t)(q + t) * set->max_pen + 64;
+        const int64_t bound = (int64_t)(q + t) * max_pen + 64;
         tier[a] = bound <= 8000 ? NW_I16 : NW_I32;
     }
     order.clear();
@@ -211,8 +205,8 @@ static void plan_pass(const hipsw_engine *eng, const hipsw_nw_set *set,
             tr = std::max(tr, tier[a]);
             smax = std::max(smax, S[a]);
             cmax = std::max(cmax, ncol[a]);
-            zr = std::max(zr, set->tl[active[a]]);
-            qm = std::max(qm, set->ql[active[a]]);
+            zr = std::max(zr, tlv[active[a]]);
+            qm = std::max(qm, qlv[active[a]]);
             order.push_back(active[a]);
         }
         for (int g = 0; g < m; g += 64) items.push_back(WorkItem{(uint32_t)g, (uint32_t)std::min(64, m - g)});
@@ -225,7 +219,7 @@ static void plan_pass(const hipsw_engine *eng, const hipsw_nw_set *set,
     for (int a = 0; a < m; ++a) {
         const uint32_t bin = (uint32_t)(tier[a] * 8 + s_class(S[a]));
         const uint32_t nc = (uint32_t)std::min(ncol[a], 4095);
-        const uint32_t tl = (uint32_t)std::min(set->tl[active[a]], 65535);
+        const uint32_t tl = (uint32_t)std::min(tlv[active[a]], 65535);
         keys[a] = (bin << 28) | ((4095u - nc) << 16) | (65535u - tl);
         idx[a] = (uint32_t)a;
     }
@@ -237,8 +231,8 @@ static void plan_pass(const hipsw_engine *eng, const hipsw_nw_set *set,
         while (e < m && (keys[e] >> 28) == bin) {
             smax = std::max(smax, S[idx[e]]);
             cmax = std::max(cmax, ncol[idx[e]]);
-            zr = std::max(zr, set->tl[active[idx[e]]]);
-            qm = std::max(qm, set->ql[active[idx[e]]]);
+            zr = std::max(zr, tlv[active[idx[e]]]);
+            qm = std::max(qm, qlv[active[idx[e]]]);
             ++e;
         }
         const int ib = (int)items.size();
@@ -281,8 +275,8 @@ extern "C" int hipsw_nw_run(hipsw_engine *eng, hipsw_nw_set *set, int32_t mode,
                                             raw[r + 5], raw[r + 6], raw[r + 7] != 0, raw[r + 8]});
         } else {
             // Retry passes hold few tasks: one plain launch each.
-            plan_pass(eng, set, active, w, pass == 0 ? mode : HIPSW_MODE_PLAIN, order, items,
-                      launches);
+            nw_plan_pass(eng, set->ql.data(), set->tl.data(), set->max_pen, active, w,
+                         pass == 0 ? mode : HIPSW_MODE_PLAIN, order, items, launches);
             if (pass == 0) {
                 set->plan_order[mode] = order;
                 set->plan_items[mode] = items;
@@ -386,9 +380,16 @@ extern "C" int hipsw_nw_download(hipsw_engine *eng, hipsw_nw_set *set, int32_t *
     auto down = [&](void *dst, const void *src, size_t b) {
         if (ok && dst) ok = eng->check(hipMemcpy(dst, src, b, hipMemcpyDeviceToHost), "download");
     };
+    std::vector<int32_t> ncig(n);
     down(out_score, set->dev.score, (size_t)n * 4);
     down(out_cigar, set->dev.cigar, (size_t)n * set->cap * 4);
-    down(out_n_cigar, set->dev.n_cigar, (size_t)n * 4);
+    down(ncig.data(), set->dev.n_cigar, (size_t)n * 4);
+    if (ok && out_n_cigar) memcpy(out_n_cigar, ncig.data(), (size_t)n * 4);
+    if (ok && out_cigar)                                       // entries past the CIGAR are zero
+        for (int k = 0; k < n; ++k) {
+            const int used = std::min(std::max(ncig[k], 0), set->cap);
+            std::fill(out_cigar + (size_t)k * set->cap + used, out_cigar + (size_t)(k + 1) * set->cap, 0u);
+        }
     down(out_flags, set->dev.flags, (size_t)n * 4);
     if (out_final_w) memcpy(out_final_w, set->final_w.data(), (size_t)n * 4);
     return ok ? HIPSW_OK : HIPSW_E_DEVICE;

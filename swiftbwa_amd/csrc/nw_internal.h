@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "hipsw_internal.h"
 
 namespace hipsw {
@@ -44,4 +46,20 @@ hipError_t launch_nw(int tier, bool global_state, const NwDev &d, const NwParams
                      uint32_t *counter, int S, uint32_t *zbase, int zrows, int ncw, int grid,
                      bool persistent, void *gstate, int qw8, hipStream_t s);
 
+// One launch of an NW pass (host plan).
+struct NwLaunch {
+    int tier, S, ncol_max, zrows, item_begin, item_count, grid;
+    bool global_state;
+    int qw8;     // staged query words (8 bases each) per lane
+};
+
+}  // namespace hipsw
+
+struct hipsw_engine;
+namespace hipsw {
+// Plans one NW pass over the job ids in `active` with bands w[] (indexed by job).
+void nw_plan_pass(const hipsw_engine *eng, const int32_t *ql, const int32_t *tl, int max_pen,
+                  const std::vector<uint32_t> &active, const std::vector<int32_t> &w, int mode,
+                  std::vector<uint32_t> &order, std::vector<WorkItem> &items,
+                  std::vector<NwLaunch> &launches);
 }  // namespace hipsw

@@ -1,0 +1,609 @@
+// Serving loop: continuous batching of read chunks over a resident reference.
+//
+// A chunk is the hot-path work of a batch of read pairs: banded extensions,
+// mate-rescue local SW and CIGAR NW, whose targets are windows of the reference
+// `.pac` kept resident in HBM.  Chunks are submitted as they arrive and return
+// a ticket.  Each of the server's S slots owns a HIP stream and a worker thread;
+// a slot takes the chunk at the head of the FIFO as soon as the chunk fits the
+// HBM budget next to the chunks already running, so work of different chunks
+// overlaps on the device.  A chunk's device memory is stream-ordered
+// (hipMallocAsync/hipFreeAsync on its slot stream) and counted against the
+// budget from admission to completion.
+//
+// Window fetch follows FMIndex.getReference (Sources/FMIndex/FMIndex.swift:25-65):
+// positions in [0, gl) read the forward strand, positions in [gl, 2gl) the
+// reverse complement of [2gl-pos-len, 2gl-pos), both clipped at the strand
+// boundary; the .pac is 2 bits per base, high bits first
+// (Sources/FMIndex/PackedReference.swift:4-5, :34-38).  Left-extension windows
+// are then reversed, as ExtensionAligner does (ExtensionAligner.swift:219-225).
+#include "engine_internal.h"
+#include "local_internal.h"
+#include "nw_internal.h"
+
+#include <atomic>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <thread>
+
+using namespace hipsw;
+
+namespace hipsw {
+
+struct Window {
+    int64_t pos;        // BWT-space start
+    uint32_t word;      // destination word in the chunk's 2-bit arena
+    int32_t len;        // bases actually fetched (host-computed clip)
+    int32_t rev;        // reverse after fetching (left extensions)
+};
+
+// One thread per window: fetch, strand-complement, optionally reverse, and
+// pack 16 bases per word into the chunk arena.
+__global__ void fetch_windows_kernel(const uint8_t *pac, int64_t gl, const Window *win, int n,
+                                     uint32_t *seq)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const Window w = win[k];
+    if (w.len <= 0) return;
+    const bool rc = w.pos >= gl;
+    // forward-strand start of the fetched bases (FMIndex.swift:27-41)
+    const int64_t fstart = rc ? (2 * gl - w.pos) - w.len : w.pos;
+    uint32_t *dst = seq + w.word;
+    const int words = (w.len + 15) >> 4;
+    for (int wd = 0; wd < words; ++wd) {
+        uint32_t v = 0;
+        const int lo = wd * 16, hi = min(w.len, lo + 16);
+        for (int i = lo; i < hi; ++i) {
+            // index within the strand-oriented window, then optional reversal
+            const int o = w.rev ? w.len - 1 - i : i;
+            const int64_t fp = rc ? fstart + (w.len - 1 - o) : fstart + o;
+            uint32_t b = (pac[fp >> 2] >> ((3 - (int)(fp & 3)) << 1)) & 3u;
+            if (rc) b = 3u - b;
+            v |= b << ((i - lo) * 2);
+        }
+        dst[wd] = v;
+    }
+}
+
+}  // namespace hipsw
+
+struct ChunkReq {
+    int64_t ticket = 0;
+    double t_submit = 0, t_start = 0, t_done = 0;
+    size_t est_bytes = 0;
+    int status = HIPSW_OK;
+    bool done = false;
+    // inputs (owned copies)
+    int n_ext = 0, n_loc = 0, n_nw = 0;
+    std::vector<uint8_t> qbytes;                 // all queries, one arena
+    std::vector<uint32_t> q_off;                 // per job (ext, then loc, then nw)
+    std::vector<uint16_t> q_len;
+    std::vector<int64_t> t_pos;
+    std::vector<uint16_t> t_len_req;
+    std::vector<uint8_t> t_rev;
+    std::vector<int32_t> h0, w, eb, nw_w, nw_ts;
+    int32_t loc_min_sub = 0;
+    // outputs
+    std::vector<int32_t> ext_out, loc_out, loc_ok, nw_score, nw_ncig, nw_flags;
+    std::vector<uint32_t> nw_cig;
+    int64_t cells = 0;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    uint32_t *counters = nullptr;
+    std::thread th;
+};
+
+struct hipsw_server {
+    hipsw_engine *eng = nullptr;
+    int64_t budget = 0;
+    int nw_cap = 512;
+    hipsw_scoring sc;
+    int32_t match_score = 1, mismatch = 4;
+    uint8_t *d_pac = nullptr;
+    int64_t gl = 0;                              // genome length (one strand)
+    std::vector<std::unique_ptr<Slot>> slots;
+    std::mutex mu;
+    std::condition_variable cv, done_cv;
+    std::deque<std::shared_ptr<ChunkReq>> queue;
+    std::map<int64_t, std::shared_ptr<ChunkReq>> live;
+    int64_t next_ticket = 1;
+    int64_t in_use = 0;
+    int running = 0;
+    int64_t peak_in_use = 0;
+    bool stop = false;
+};
+
+namespace {
+
+template <typename T>
+T *carve(char *&cur, size_t count)
+{
+    T *p = reinterpret_cast<T *>(cur);
+    cur += (count * sizeof(T) + 255) & ~(size_t)255;
+    return p;
+}
+template <typename T>
+size_t span(size_t count) { return (count * sizeof(T) + 255) & ~(size_t)255; }
+
+// Bases getReference(pos, len) returns (FMIndex.swift:27-50).
+int fetched_len(int64_t pos, int len, int64_t gl)
+{
+    if (pos >= gl) {
+        const int64_t fend = 2 * gl - pos, fstart = fend - len;
+        const int64_t s = std::max<int64_t>(0, fstart), e = std::min<int64_t>(fend, gl);
+        return e > s ? (int)(e - s) : 0;
+    }
+    if (pos < 0) return 0;
+    return (int)std::max<int64_t>(0, std::min<int64_t>(len, gl - pos));
+}
+
+double now_ms_s() { return hipsw::now_ms(); }
+
+// Runs one chunk on one slot.  Returns a status.
+int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
+{
+    hipsw_engine *eng = srv->eng;
+    const int n = r.n_ext + r.n_loc + r.n_nw;
+    if (n == 0) return HIPSW_OK;
+    hipStream_t st = sl.stream;
+    // --- host packing: queries from the request, targets as fetch windows ---
+    std::vector<uint32_t> qw(n), tw(n), qnw(n, kNoN), tnw(n, kNoN);
+    std::vector<int32_t> ql(n), tl(n);
+    std::vector<Window> win(n);
+    uint64_t words = 0, nwords = 0;
+    for (int k = 0; k < n; ++k) {
+        ql[k] = r.q_len[k];
+        qw[k] = (uint32_t)words;
+        words += (ql[k] + 15) / 16;
+        if (has_n(r.qbytes.data() + r.q_off[k], ql[k])) {
+            qnw[k] = (uint32_t)nwords;
+            nwords += (ql[k] + 31) / 32;
+        }
+        tl[k] = fetched_len(r.t_pos[k], r.t_len_req[k], srv->gl);
+        tw[k] = (uint32_t)words;
+        win[k] = Window{r.t_pos[k], (uint32_t)words, tl[k], (int32_t)r.t_rev[k]};
+        words += (tl[k] + 15) / 16;
+    }
+    words += 64;
+    nwords += 64;
+    std::vector<uint32_t> seq(words, 0), nmask(nwords, 0);
+    for (int k = 0; k < n; ++k)
+        if (!pack2(r.qbytes.data() + r.q_off[k], ql[k], seq.data() + qw[k],
+                   qnw[k] != kNoN ? nmask.data() + qnw[k] : nullptr))
+            return HIPSW_E_INVALID;
+    // extension plan (jobs [0, n_ext))
+    const ExtParams ep = make_params(&srv->sc);
+    std::vector<uint32_t> ext_order;
+    std::vector<WorkItem> ext_items;
+    std::vector<ExtLaunch> ext_launch;
+    size_t ext_scratch = 0;
+    if (r.n_ext)
+        ext_scratch = plan_ext_batched(eng, r.n_ext, ql.data(), tl.data(), r.h0.data(), r.w.data(),
+                                       ep.max_mat, ext_order, ext_items, ext_launch);
+    // local SW split
+    std::vector<uint32_t> loc_s, loc_c;
+    int loc_qmax = 0, loc_tmax = 1;
+    for (int k = r.n_ext; k < r.n_ext + r.n_loc; ++k) {
+        if (ql[k] <= 0 || tl[k] <= 0) continue;
+        loc_tmax = std::max(loc_tmax, tl[k]);
+        if ((int64_t)ql[k] * srv->match_score <= 250 && ql[k] <= 256) {
+            loc_s.push_back((uint32_t)k);
+            loc_qmax = std::max(loc_qmax, ql[k]);
+        } else {
+            loc_c.push_back((uint32_t)k);
+        }
+    }
+    // NW first pass
+    const int nw0 = r.n_ext + r.n_loc;
+    std::vector<int32_t> wcur(n, 1);
+    for (int k = 0; k < r.n_nw; ++k) wcur[nw0 + k] = std::max(1, r.nw_w[k]);
+    int mat_abs = 0;
+    for (int k = 0; k < 25; ++k) mat_abs = std::max(mat_abs, std::abs((int)srv->sc.mat[k]));
+    const int max_pen = std::max(mat_abs, std::max(srv->sc.o_del + srv->sc.e_del, srv->sc.o_ins + srv->sc.e_ins));
+
+    // --- device arena ---
+    const size_t colmax_b = r.loc_min_sub > 0 ? loc_s.size() * (size_t)loc_tmax : 1;
+    const int cap = srv->nw_cap;
+    const size_t bytes = span<uint32_t>(words) + span<uint32_t>(nwords) + 4 * span<uint32_t>(n) +
+                         5 * span<int32_t>(n) + span<Window>(n) + span<int32_t>((size_t)n * 6) +
+                         span<int32_t>(n) + span<uint32_t>((size_t)r.n_nw * cap + 1) +
+                         3 * span<int32_t>(n) + 2 * span<uint32_t>(n) +
+                         span<WorkItem>(n / 64 + 64) + span<uint8_t>(colmax_b) +
+                         span<unsigned long long>(2);
+    void *dmem = nullptr;
+    if (hipMallocAsync(&dmem, bytes, st) != hipSuccess) return HIPSW_E_DEVICE;
+    char *cur = reinterpret_cast<char *>(dmem);
+    uint32_t *d_seq = carve<uint32_t>(cur, words), *d_nmask = carve<uint32_t>(cur, nwords);
+    uint32_t *d_qw = carve<uint32_t>(cur, n), *d_tw = carve<uint32_t>(cur, n);
+    uint32_t *d_qn = carve<uint32_t>(cur, n), *d_tn = carve<uint32_t>(cur, n);
+    int32_t *d_ql = carve<int32_t>(cur, n), *d_tl = carve<int32_t>(cur, n);
+    int32_t *d_h0 = carve<int32_t>(cur, n), *d_w = carve<int32_t>(cur, n), *d_eb = carve<int32_t>(cur, n);
+    Window *d_win = carve<Window>(cur, n);
+    int32_t *d_out6 = carve<int32_t>(cur, (size_t)n * 6);
+    int32_t *d_status = carve<int32_t>(cur, n);
+    uint32_t *d_cig = carve<uint32_t>(cur, (size_t)r.n_nw * cap + 1);
+    int32_t *d_score = carve<int32_t>(cur, n), *d_ncig = carve<int32_t>(cur, n), *d_flags = carve<int32_t>(cur, n);
+    uint32_t *d_order = carve<uint32_t>(cur, n), *d_order2 = carve<uint32_t>(cur, n);
+    WorkItem *d_items = carve<WorkItem>(cur, n / 64 + 64);
+    uint8_t *d_colmax = carve<uint8_t>(cur, colmax_b);
+    unsigned long long *d_cells = carve<unsigned long long>(cur, 2);
+    std::vector<int32_t> h0v(n, 0), wv(n, 0), ebv(n, 0);
+    for (int k = 0; k < r.n_ext; ++k) { h0v[k] = r.h0[k]; wv[k] = r.w[k]; ebv[k] = r.eb[k]; }
+    bool ok = true;
+    auto up = [&](void *dst, const void *src, size_t b) {
+        if (ok && b) ok = hipMemcpyAsync(dst, src, b, hipMemcpyHostToDevice, st) == hipSuccess;
+    };
+    up(d_seq, seq.data(), words * 4);
+    up(d_nmask, nmask.data(), nwords * 4);
+    up(d_qw, qw.data(), (size_t)n * 4); up(d_tw, tw.data(), (size_t)n * 4);
+    up(d_qn, qnw.data(), (size_t)n * 4); up(d_tn, tnw.data(), (size_t)n * 4);
+    up(d_ql, ql.data(), (size_t)n * 4); up(d_tl, tl.data(), (size_t)n * 4);
+    up(d_h0, h0v.data(), (size_t)n * 4); up(d_w, wv.data(), (size_t)n * 4); up(d_eb, ebv.data(), (size_t)n * 4);
+    up(d_win, win.data(), (size_t)n * sizeof(Window));
+    if (ok) ok = launch_zero_u32(reinterpret_cast<uint32_t *>(d_cells), 4, st) == hipSuccess;
+    if (ok) {
+        hipLaunchKernelGGL(fetch_windows_kernel, dim3((n + 127) / 128), dim3(128), 0, st,
+                           srv->d_pac, srv->gl, d_win, n, d_seq);
+        ok = hipGetLastError() == hipSuccess;
+    }
+    // --- extension ---
+    if (ok && r.n_ext) {
+        up(d_order, ext_order.data(), ext_order.size() * 4);
+        up(d_items, ext_items.data(), ext_items.size() * sizeof(WorkItem));
+        void *gs = nullptr;
+        if (ok && ext_scratch) ok = hipMallocAsync(&gs, ext_scratch, st) == hipSuccess;
+        if (ok) ok = launch_zero_u32(sl.counters, 64, st) == hipSuccess;
+        const ExtDev ed{d_seq, d_nmask, d_qw, d_tw, d_qn, d_tn, d_ql, d_tl, d_h0, d_w, d_eb, d_out6, d_cells};
+        int k = 0;
+        for (const ExtLaunch &L : ext_launch) {
+            if (!ok || k >= 64) { ok = false; break; }
+            ok = launch_ext_persistent(L.tier, L.global_state, ed, ep, d_order, d_items + L.item_begin,
+                                       L.item_count, sl.counters + k, L.cols, L.grid,
+                                       L.global_state ? (char *)gs + L.scratch_off : nullptr, st) == hipSuccess;
+            ++k;
+        }
+        if (gs) hipFreeAsync(gs, st);
+    }
+    // --- local SW ---
+    if (ok && r.n_loc) {
+        LocalParams lp;
+        memset(&lp, 0, sizeof lp);
+        int max_u8 = 0, max_i = 0;
+        for (int k = 0; k < 25; ++k) {
+            lp.mat[k] = srv->sc.mat[k];
+            max_u8 = std::max(max_u8, std::min(std::max((int)srv->sc.mat[k] + srv->mismatch, 0), 255));
+            max_i = std::max(max_i, (int)srv->sc.mat[k]);
+        }
+        lp.bias = srv->mismatch;
+        lp.gap_oe = srv->sc.o_ins + srv->sc.e_ins;
+        lp.gap_e = srv->sc.e_ins;
+        lp.min_sub = std::min(std::max(r.loc_min_sub, 0), 255);
+        lp.max_base_u8 = max_u8 > srv->mismatch ? max_u8 - srv->mismatch : 0;
+        lp.max_base_i = max_i;
+        const LocalDev ld{d_seq, d_nmask, d_qw, d_tw, d_qn, d_tn, d_ql, d_tl, d_out6, d_status};
+        if (ok) ok = hipMemsetAsync(d_status, 0, (size_t)n * 4, st) == hipSuccess;
+        if (ok && !loc_s.empty()) {
+            up(d_order2, loc_s.data(), loc_s.size() * 4);
+            const int snmax = loc_qmax <= 64 ? 4 : loc_qmax <= 128 ? 8 : loc_qmax <= 192 ? 12 : 16;
+            if (ok) ok = launch_local_striped(snmax, ld, lp, d_order2, (int)loc_s.size(), d_colmax,
+                                              lp.min_sub > 0 ? loc_tmax : 0, st) == hipSuccess;
+            std::vector<int32_t> stv(n);
+            if (ok) ok = hipMemcpyAsync(stv.data(), d_status, (size_t)n * 4, hipMemcpyDeviceToHost, st) == hipSuccess;
+            if (ok) ok = hipStreamSynchronize(st) == hipSuccess;
+            if (ok)
+                for (uint32_t k : loc_s)
+                    if (stv[k] == LOCAL_NEEDS_SCALAR) loc_c.push_back(k);
+        }
+        if (ok && !loc_c.empty()) {
+            int qc = 1, tc = 1;
+            for (uint32_t k : loc_c) { qc = std::max(qc, ql[k]); tc = std::max(tc, tl[k]); }
+            const int blocks = (int)((loc_c.size() + 63) / 64);
+            void *scr = nullptr;
+            up(d_order2, loc_c.data(), loc_c.size() * 4);
+            if (ok) ok = hipMallocAsync(&scr, (size_t)blocks * 64 * (2 * (tc + 1) + qc) * 4, st) == hipSuccess;
+            if (ok) ok = launch_local_scalar(ld, lp, d_order2, (int)loc_c.size(), (int *)scr, tc, qc, st) == hipSuccess;
+            if (scr) hipFreeAsync(scr, st);
+        }
+    }
+    // --- NW with the CPU retry loop ---
+    if (ok && r.n_nw) {
+        NwParams np;
+        memset(&np, 0, sizeof np);
+        for (int k = 0; k < 25; ++k) np.mat[k] = srv->sc.mat[k];
+        np.o_del = srv->sc.o_del; np.e_del = srv->sc.e_del;
+        np.o_ins = srv->sc.o_ins; np.e_ins = srv->sc.e_ins;
+        np.oe_del = np.o_del + np.e_del; np.oe_ins = np.o_ins + np.e_ins;
+        // NW outputs are indexed by job id; cigar rows by (job - nw0).
+        const NwDev nd{d_seq, d_nmask, d_qw, d_tw, d_qn, d_tn, d_ql, d_tl, d_w, d_score,
+                       d_cig - (size_t)nw0 * cap, d_ncig, d_flags, d_cells + 1, cap};
+        std::vector<uint32_t> active;
+        for (int k = 0; k < r.n_nw; ++k) active.push_back((uint32_t)(nw0 + k));
+        std::vector<int> tries(n, 0);
+        std::vector<int32_t> score(n);
+        std::vector<uint32_t> order;
+        std::vector<WorkItem> items;
+        std::vector<NwLaunch> launches;
+        for (int pass = 0; ok && !active.empty(); ++pass) {
+            nw_plan_pass(eng, ql.data(), tl.data(), max_pen, active, wcur,
+                         pass == 0 ? HIPSW_MODE_BATCHED : HIPSW_MODE_PLAIN, order, items, launches);
+            size_t zb = 0, gb = 0;
+            for (const NwLaunch &L : launches) {
+                zb = std::max(zb, (size_t)L.grid * L.zrows * ((L.ncol_max + 7) / 8) * 64 * 4);
+                if (L.global_state) gb = std::max(gb, (size_t)L.grid * L.S * 64 * (L.tier == NW_I16 ? 4 : 8));
+            }
+            void *scr = nullptr;
+            up(d_w, wcur.data(), (size_t)n * 4);
+            up(d_order, order.data(), order.size() * 4);
+            up(d_items, items.data(), items.size() * sizeof(WorkItem));
+            if (ok) ok = hipMallocAsync(&scr, zb + gb + 512, st) == hipSuccess;
+            if (ok) ok = launch_zero_u32(sl.counters, 64, st) == hipSuccess;
+            int k = 0;
+            for (const NwLaunch &L : launches) {
+                if (!ok || k >= 64) { ok = false; break; }
+                ok = launch_nw(L.tier, L.global_state, nd, np, d_order, d_items + L.item_begin,
+                               L.item_count, sl.counters + k, L.S, (uint32_t *)scr, L.zrows,
+                               (L.ncol_max + 7) / 8, L.grid, pass == 0, (char *)scr + ((zb + 255) & ~(size_t)255),
+                               L.qw8, st) == hipSuccess;
+                ++k;
+            }
+            if (scr) hipFreeAsync(scr, st);
+            if (ok) ok = hipMemcpyAsync(score.data(), d_score, (size_t)n * 4, hipMemcpyDeviceToHost, st) == hipSuccess;
+            if (ok) ok = hipStreamSynchronize(st) == hipSuccess;
+            std::vector<uint32_t> next;
+            for (uint32_t j : active) {
+                if (score[j] < r.nw_ts[j - nw0] && tries[j] < 3) {
+                    wcur[j] = wcur[j] < (1 << 20) ? wcur[j] * 2 : wcur[j];
+                    tries[j]++;
+                    next.push_back(j);
+                }
+            }
+            active.swap(next);
+        }
+    }
+    // --- results ---
+    std::vector<int32_t> out6((size_t)n * 6), stv(n), sc(n), nc(n), fl(n);
+    unsigned long long cells[2] = {0, 0};
+    auto down = [&](void *dst, const void *src, size_t b) {
+        if (ok && b) ok = hipMemcpyAsync(dst, src, b, hipMemcpyDeviceToHost, st) == hipSuccess;
+    };
+    down(out6.data(), d_out6, (size_t)n * 6 * 4);
+    down(stv.data(), d_status, (size_t)n * 4);
+    down(sc.data(), d_score, (size_t)n * 4);
+    down(nc.data(), d_ncig, (size_t)n * 4);
+    down(fl.data(), d_flags, (size_t)n * 4);
+    r.nw_cig.assign((size_t)r.n_nw * cap, 0);
+    down(r.nw_cig.data(), d_cig, (size_t)r.n_nw * cap * 4);
+    down(cells, d_cells, sizeof cells);
+    hipFreeAsync(dmem, st);
+    if (ok) ok = hipStreamSynchronize(st) == hipSuccess;
+    if (!ok) return HIPSW_E_DEVICE;
+    r.ext_out.assign(out6.begin(), out6.begin() + (size_t)r.n_ext * 6);
+    r.loc_out.assign((size_t)r.n_loc * 6, 0);
+    r.loc_ok.assign(r.n_loc, 0);
+    for (int k = 0; k < r.n_loc; ++k) {
+        const int j = r.n_ext + k;
+        r.loc_ok[k] = stv[j] == LOCAL_OK ? 1 : 0;
+        if (r.loc_ok[k]) memcpy(&r.loc_out[(size_t)k * 6], &out6[(size_t)j * 6], 6 * 4);
+    }
+    r.nw_score.assign(sc.begin() + nw0, sc.begin() + nw0 + r.n_nw);
+    r.nw_ncig.assign(nc.begin() + nw0, nc.begin() + nw0 + r.n_nw);
+    for (int k = 0; k < r.n_nw; ++k) {                       // entries past the CIGAR are zero
+        const int used = std::min(std::max(r.nw_ncig[k], 0), cap);
+        std::fill(r.nw_cig.begin() + (size_t)k * cap + used, r.nw_cig.begin() + (size_t)(k + 1) * cap, 0u);
+    }
+    r.nw_flags.assign(fl.begin() + nw0, fl.begin() + nw0 + r.n_nw);
+    r.cells = (int64_t)(cells[0] + cells[1]);
+    return HIPSW_OK;
+}
+
+size_t estimate_bytes(const ChunkReq &r, int cap)
+{
+    const size_t n = (size_t)(r.n_ext + r.n_loc + r.n_nw);
+    size_t words = 64;
+    for (size_t k = 0; k < n; ++k) words += (r.q_len[k] + 15) / 16 + (r.t_len_req[k] + 15) / 16;
+    // arena + outputs + generous scratch (NW backpointers, long-query state)
+    return words * 4 + n * 160 + (size_t)r.n_nw * cap * 4 + (size_t)r.n_nw * 300 * 32 + (64 << 20);
+}
+
+void slot_loop(hipsw_server *srv, Slot *sl)
+{
+    hipSetDevice(srv->eng->device);
+    for (;;) {
+        std::shared_ptr<ChunkReq> req;
+        {
+            std::unique_lock<std::mutex> lk(srv->mu);
+            srv->cv.wait(lk, [&] {
+                if (srv->stop) return true;
+                if (srv->queue.empty()) return false;
+                const auto &f = srv->queue.front();
+                return srv->running == 0 || srv->in_use + (int64_t)f->est_bytes <= srv->budget;
+            });
+            if (srv->stop) return;
+            req = srv->queue.front();
+            srv->queue.pop_front();
+            srv->in_use += (int64_t)req->est_bytes;
+            srv->peak_in_use = std::max(srv->peak_in_use, srv->in_use);
+            srv->running++;
+            req->t_start = now_ms_s();
+        }
+        const int rc = run_chunk(srv, *sl, *req);
+        {
+            std::lock_guard<std::mutex> lk(srv->mu);
+            req->status = rc;
+            req->t_done = now_ms_s();
+            req->done = true;
+            srv->in_use -= (int64_t)req->est_bytes;
+            srv->running--;
+        }
+        srv->done_cv.notify_all();
+        srv->cv.notify_all();
+    }
+}
+
+}  // namespace
+
+extern "C" hipsw_server *hipsw_server_create(hipsw_engine *eng, int32_t n_slots, int64_t hbm_budget,
+                                            const hipsw_scoring *sc, int32_t match_score,
+                                            int32_t mismatch_penalty, int32_t nw_cap)
+{
+    if (!eng || n_slots < 1 || n_slots > 16 || hbm_budget <= 0 || !scoring_valid(sc) || nw_cap < 1)
+        return nullptr;
+    hipSetDevice(eng->device);
+    auto *srv = new hipsw_server();
+    srv->eng = eng;
+    srv->budget = hbm_budget;
+    srv->sc = *sc;
+    srv->match_score = match_score;
+    srv->mismatch = mismatch_penalty;
+    srv->nw_cap = nw_cap;
+    for (int s = 0; s < n_slots; ++s) {
+        auto sl = std::make_unique<Slot>();
+        if (hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking) != hipSuccess ||
+            hipMalloc((void **)&sl->counters, 64 * sizeof(uint32_t)) != hipSuccess) {
+            delete srv;
+            return nullptr;
+        }
+        srv->slots.push_back(std::move(sl));
+    }
+    for (auto &sl : srv->slots) sl->th = std::thread(slot_loop, srv, sl.get());
+    return srv;
+}
+
+extern "C" int hipsw_server_set_reference(hipsw_server *srv, const uint8_t *pac, int64_t n_bases)
+{
+    if (!srv || !pac || n_bases <= 0) return HIPSW_E_INVALID;
+    std::unique_lock<std::mutex> lk(srv->mu);
+    if (srv->running || !srv->queue.empty()) return HIPSW_E_PENDING;
+    hipSetDevice(srv->eng->device);
+    if (srv->d_pac) hipFree(srv->d_pac);
+    srv->d_pac = nullptr;
+    const size_t bytes = (size_t)(n_bases + 3) / 4;
+    if (hipMalloc((void **)&srv->d_pac, bytes) != hipSuccess) return HIPSW_E_DEVICE;
+    if (hipMemcpy(srv->d_pac, pac, bytes, hipMemcpyHostToDevice) != hipSuccess) return HIPSW_E_DEVICE;
+    srv->gl = n_bases;
+    return HIPSW_OK;
+}
+
+extern "C" int64_t hipsw_server_submit(hipsw_server *srv, const hipsw_chunk *c)
+{
+    if (!srv || !c) return HIPSW_E_INVALID;
+    if (!srv->d_pac) return HIPSW_E_INVALID;
+    auto r = std::make_shared<ChunkReq>();
+    r->t_submit = now_ms_s();
+    r->n_ext = c->n_ext; r->n_loc = c->n_loc; r->n_nw = c->n_nw;
+    if (r->n_ext < 0 || r->n_loc < 0 || r->n_nw < 0) return HIPSW_E_INVALID;
+    r->loc_min_sub = c->loc_min_sub;
+    auto add = [&](int cnt, const uint8_t *q, const uint32_t *qo, const uint16_t *qlen,
+                   const int64_t *tp, const uint16_t *tlen, const uint8_t *trev) -> bool {
+        if (cnt == 0) return true;
+        if (!q || !qo || !qlen || !tp || !tlen) return false;
+        for (int k = 0; k < cnt; ++k) {
+            r->q_off.push_back((uint32_t)r->qbytes.size());
+            r->q_len.push_back(qlen[k]);
+            r->qbytes.insert(r->qbytes.end(), q + qo[k], q + qo[k] + qlen[k]);
+            r->t_pos.push_back(tp[k]);
+            r->t_len_req.push_back(tlen[k]);
+            r->t_rev.push_back(trev ? trev[k] : 0);
+        }
+        return true;
+    };
+    if (!add(c->n_ext, c->ext_q, c->ext_q_off, c->ext_q_len, c->ext_t_pos, c->ext_t_len, c->ext_t_rev) ||
+        !add(c->n_loc, c->loc_q, c->loc_q_off, c->loc_q_len, c->loc_t_pos, c->loc_t_len, nullptr) ||
+        !add(c->n_nw, c->nw_q, c->nw_q_off, c->nw_q_len, c->nw_t_pos, c->nw_t_len, nullptr))
+        return HIPSW_E_INVALID;
+    if (c->n_ext && (!c->ext_h0 || !c->ext_w || !c->ext_eb)) return HIPSW_E_INVALID;
+    if (c->n_nw && (!c->nw_w || !c->nw_true_score)) return HIPSW_E_INVALID;
+    r->h0.assign(c->ext_h0, c->ext_h0 + c->n_ext);
+    r->w.assign(c->ext_w, c->ext_w + c->n_ext);
+    r->eb.assign(c->ext_eb, c->ext_eb + c->n_ext);
+    r->nw_w.assign(c->nw_w, c->nw_w + c->n_nw);
+    r->nw_ts.assign(c->nw_true_score, c->nw_true_score + c->n_nw);
+    r->est_bytes = estimate_bytes(*r, srv->nw_cap);
+    std::lock_guard<std::mutex> lk(srv->mu);
+    if ((int64_t)r->est_bytes > srv->budget) return HIPSW_E_BUDGET;
+    r->ticket = srv->next_ticket++;
+    srv->live[r->ticket] = r;
+    srv->queue.push_back(r);
+    srv->cv.notify_all();
+    return r->ticket;
+}
+
+extern "C" int hipsw_server_poll(hipsw_server *srv, int64_t ticket)
+{
+    if (!srv) return HIPSW_E_INVALID;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    auto it = srv->live.find(ticket);
+    if (it == srv->live.end()) return HIPSW_E_INVALID;
+    return it->second->done ? HIPSW_OK : HIPSW_E_PENDING;
+}
+
+extern "C" int hipsw_server_wait(hipsw_server *srv, int64_t ticket, hipsw_chunk_result *res)
+{
+    if (!srv) return HIPSW_E_INVALID;
+    std::shared_ptr<ChunkReq> r;
+    {
+        std::unique_lock<std::mutex> lk(srv->mu);
+        auto it = srv->live.find(ticket);
+        if (it == srv->live.end()) return HIPSW_E_INVALID;
+        r = it->second;
+        srv->done_cv.wait(lk, [&] { return r->done; });
+        srv->live.erase(it);
+    }
+    if (res) {
+        res->latency_ms = r->t_done - r->t_submit;
+        res->queue_ms = r->t_start - r->t_submit;
+        res->run_ms = r->t_done - r->t_start;
+        res->cells = r->cells;
+        if (r->status == HIPSW_OK) {
+            const int cap = srv->nw_cap;
+            if (res->ext_out && r->n_ext) memcpy(res->ext_out, r->ext_out.data(), (size_t)r->n_ext * 24);
+            if (res->loc_out && r->n_loc) memcpy(res->loc_out, r->loc_out.data(), (size_t)r->n_loc * 24);
+            if (res->loc_ok && r->n_loc) memcpy(res->loc_ok, r->loc_ok.data(), (size_t)r->n_loc * 4);
+            if (res->nw_score && r->n_nw) memcpy(res->nw_score, r->nw_score.data(), (size_t)r->n_nw * 4);
+            if (res->nw_n_cigar && r->n_nw) memcpy(res->nw_n_cigar, r->nw_ncig.data(), (size_t)r->n_nw * 4);
+            if (res->nw_flags && r->n_nw) memcpy(res->nw_flags, r->nw_flags.data(), (size_t)r->n_nw * 4);
+            if (res->nw_cigar && r->n_nw) memcpy(res->nw_cigar, r->nw_cig.data(), (size_t)r->n_nw * cap * 4);
+        }
+    }
+    return r->status;
+}
+
+extern "C" int hipsw_server_stats(hipsw_server *srv, int64_t *in_use, int64_t *peak_in_use,
+                                  int32_t *running, int32_t *queued)
+{
+    if (!srv) return HIPSW_E_INVALID;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    if (in_use) *in_use = srv->in_use;
+    if (peak_in_use) *peak_in_use = srv->peak_in_use;
+    if (running) *running = srv->running;
+    if (queued) *queued = (int32_t)srv->queue.size();
+    return HIPSW_OK;
+}
+
+extern "C" void hipsw_server_destroy(hipsw_server *srv)
+{
+    if (!srv) return;
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->stop = true;
+    }
+    srv->cv.notify_all();
+    for (auto &sl : srv->slots)
+        if (sl->th.joinable()) sl->th.join();
+    hipSetDevice(srv->eng->device);
+    for (auto &sl : srv->slots) {
+        if (sl->stream) {
+            hipStreamSynchronize(sl->stream);
+            hipStreamDestroy(sl->stream);
+        }
+        if (sl->counters) hipFree(sl->counters);
+    }
+    if (srv->d_pac) hipFree(srv->d_pac);
+    delete srv;
+}

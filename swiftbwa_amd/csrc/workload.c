@@ -286,3 +286,188 @@ void wl_free(wl_set_t *s)
     free(s->h0); free(s->w); free(s->eb); free(s->true_score); free(s->dir);
     free(s);
 }
+
+/* ------------------------------------------------------------------------
+ * Serving workload: a random reference with planted repeat families, and
+ * chunks of read pairs turned into the hot-path jobs of each pair.
+ * ---------------------------------------------------------------------- */
+
+/* Random ACGT reference of n bases with two planted families: a 300-bp and a
+ * 6-kb element, `copies` copies each at `div` substitution divergence. */
+uint8_t *wl_reference(uint64_t seed, int64_t n, int copies, double div)
+{
+    if (n <= 0) return NULL;
+    rng_t r;
+    rng_seed(&r, seed * 0x9E3779B97F4A7C15ull + 11);
+    uint8_t *ref = (uint8_t *)malloc((size_t)n);
+    for (int64_t k = 0; k < n; ++k) ref[k] = (uint8_t)(next_u64(&r) & 3);
+    const int fam_len[2] = {300, 6000};
+    for (int f = 0; f < 2; ++f) {
+        int L = fam_len[f];
+        if (L * 4 > n) continue;
+        uint8_t *elem = (uint8_t *)malloc((size_t)L);
+        random_seq(&r, elem, L);
+        for (int c = 0; c < copies; ++c) {
+            int64_t pos = uni(&r, 0, n - L);
+            for (int k = 0; k < L; ++k)
+                ref[pos + k] = unif01(&r) < div ? (uint8_t)((elem[k] + 1 + uni(&r, 0, 2)) & 3) : elem[k];
+        }
+        free(elem);
+    }
+    return ref;
+}
+
+/* .pac layout: 4 bases per byte, first base in bits 7-6 (PackedReference.swift:4-5). */
+void wl_pack_pac(const uint8_t *ref, int64_t n, uint8_t *pac)
+{
+    int64_t nb = (n + 3) / 4;
+    memset(pac, 0, (size_t)nb);
+    for (int64_t k = 0; k < n; ++k) pac[k >> 2] |= (uint8_t)((ref[k] & 3) << ((3 - (k & 3)) << 1));
+}
+
+void wl_free_bytes(uint8_t *p) { free(p); }
+
+typedef struct {
+    int64_t n;                       /* jobs */
+    uint8_t *q; int64_t q_bytes;
+    uint32_t *q_off; uint16_t *q_len;
+    int64_t *t_pos; uint16_t *t_len; uint8_t *t_rev;
+    int32_t *a, *b, *c;              /* ext: h0, w, eb; nw: w, true_score */
+} wl_jobs_t;
+
+typedef struct {
+    int64_t pairs;
+    wl_jobs_t ext, loc, nw;
+} wl_chunk_t;
+
+static void jobs_alloc(wl_jobs_t *j, int64_t n, int qcap)
+{
+    j->n = 0;
+    j->q = (uint8_t *)malloc((size_t)n * qcap + 1);
+    j->q_bytes = 0;
+    j->q_off = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n + 1));
+    j->q_len = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)(n + 1));
+    j->t_pos = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n + 1));
+    j->t_len = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)(n + 1));
+    j->t_rev = (uint8_t *)calloc((size_t)n + 1, 1);
+    j->a = (int32_t *)calloc((size_t)n + 1, sizeof(int32_t));
+    j->b = (int32_t *)calloc((size_t)n + 1, sizeof(int32_t));
+    j->c = (int32_t *)calloc((size_t)n + 1, sizeof(int32_t));
+}
+
+static int64_t jobs_add(wl_jobs_t *j, const uint8_t *q, int ql, int64_t tpos, int tlen, int rev)
+{
+    int64_t k = j->n++;
+    j->q_off[k] = (uint32_t)j->q_bytes;
+    j->q_len[k] = (uint16_t)ql;
+    memcpy(j->q + j->q_bytes, q, (size_t)ql);
+    j->q_bytes += ql;
+    j->t_pos[k] = tpos;
+    j->t_len[k] = (uint16_t)tlen;
+    j->t_rev[k] = (uint8_t)rev;
+    return k;
+}
+
+/* Chunk of `pairs` read pairs (2 x 150 bp, insert ~ N(400, 50), 0.5% subs /
+ * 0.05% indels; mate 1 forward, mate 2 reverse strand) on a reference of gl
+ * bases.  Per pair: for each mate, a 30-bp seed at offset U[20,100] of the
+ * read and its left and right extension jobs (h0 = seed score, w = 100, end
+ * bonus 5; windows sized as ExtensionAligner sizes them for a one-seed chain);
+ * a CIGAR NW job per mate (w = 5); and a mate-rescue local-SW job of mate 2
+ * (forward-oriented) against a 600-bp window.  Reverse-strand windows use
+ * BWT-space positions in [gl, 2gl). */
+wl_chunk_t *wl_chunk(const uint8_t *ref, int64_t gl, uint64_t seed, int64_t pairs)
+{
+    if (pairs < 0 || gl < 10000) return NULL;
+    wl_chunk_t *c = (wl_chunk_t *)calloc(1, sizeof *c);
+    c->pairs = pairs;
+    jobs_alloc(&c->ext, pairs * 4, 150);
+    jobs_alloc(&c->nw, pairs * 2, 150);
+    jobs_alloc(&c->loc, pairs, 150);
+    rng_t r;
+    rng_seed(&r, seed * 0xD1342543DE82EF95ull + 29);
+    uint8_t m[2][320], tmp[320];
+    for (int64_t p = 0; p < pairs; ++p) {
+        double u1 = unif01(&r) + 1e-12, u2 = unif01(&r);
+        double z = __builtin_sqrt(-2.0 * __builtin_log(u1)) * __builtin_cos(6.283185307179586 * u2);
+        int ins = (int)(400.0 + 50.0 * z);
+        if (ins < 200) ins = 200;
+        if (ins > 600) ins = 600;
+        int64_t p1 = uni(&r, 1000, gl - 1000 - ins);
+        /* mate 1: forward segment; mate 2: reverse complement of [p1+ins-150, p1+ins) */
+        int l1 = mutate(&r, ref + p1, 150, 0.005, 0.0005, m[0], 300, NULL);
+        int l2m = mutate(&r, ref + p1 + ins - 150, 150, 0.005, 0.0005, tmp, 300, NULL);
+        if (l1 > 150) l1 = 150;
+        if (l2m > 150) l2m = 150;
+        for (int k = 0; k < l2m; ++k) m[1][k] = (uint8_t)(3 - tmp[l2m - 1 - k]);
+        int lens[2] = {l1, l2m};
+        /* locus of each mate in BWT space, read orientation */
+        int64_t locus[2] = {p1, 2 * gl - (p1 + ins)};
+        for (int mt = 0; mt < 2; ++mt) {
+            const uint8_t *rd = m[mt];
+            int rl = lens[mt];
+            int o = (int)uni(&r, 20, 100);
+            if (o + 30 > rl) o = rl - 30 > 0 ? rl - 30 : 0;
+            int64_t rbeg = locus[mt] + o;
+            uint8_t qrev[160];
+            if (o > 0) {
+                for (int k = 0; k < o; ++k) qrev[k] = rd[o - 1 - k];
+                int L = o + cal_max_gap(o, 100);
+                int64_t k = jobs_add(&c->ext, qrev, o, rbeg - L, L, 1);
+                c->ext.a[k] = 30; c->ext.b[k] = 100; c->ext.c[k] = 5;
+            }
+            int rem = rl - o - 30;
+            if (rem > 0) {
+                int R = rem + cal_max_gap(rem, 100);
+                int64_t k = jobs_add(&c->ext, rd + o + 30, rem, rbeg + 30, R, 0);
+                c->ext.a[k] = 30 + o; c->ext.b[k] = 100; c->ext.c[k] = 5;
+            }
+            int64_t k = jobs_add(&c->nw, rd, rl, locus[mt], 150, 0);
+            c->nw.a[k] = 5;
+            c->nw.b[k] = rl - 30;
+        }
+        /* rescue mate 2 (forward orientation) in a 600-bp window after mate 1 */
+        for (int k = 0; k < l2m; ++k) tmp[k] = (uint8_t)(3 - m[1][l2m - 1 - k]);
+        int64_t wstart = p1 + 400 - 150 - 225;
+        if (wstart < 0) wstart = 0;
+        if (wstart + 600 > gl) wstart = gl - 600;
+        jobs_add(&c->loc, tmp, l2m, wstart, 600, 0);
+    }
+    return c;
+}
+
+void *wl_chunk_field(wl_chunk_t *c, int kind, int which)
+{
+    wl_jobs_t *j = kind == 0 ? &c->ext : kind == 1 ? &c->loc : &c->nw;
+    switch (which) {
+    case 0: return j->q;
+    case 1: return j->q_off;
+    case 2: return j->q_len;
+    case 3: return j->t_pos;
+    case 4: return j->t_len;
+    case 5: return j->t_rev;
+    case 6: return j->a;
+    case 7: return j->b;
+    case 8: return j->c;
+    default: return NULL;
+    }
+}
+
+int64_t wl_chunk_count(wl_chunk_t *c, int kind, int which)
+{
+    wl_jobs_t *j = kind == 0 ? &c->ext : kind == 1 ? &c->loc : &c->nw;
+    return which == 0 ? j->q_bytes : j->n;
+}
+
+static void jobs_free(wl_jobs_t *j)
+{
+    free(j->q); free(j->q_off); free(j->q_len); free(j->t_pos); free(j->t_len); free(j->t_rev);
+    free(j->a); free(j->b); free(j->c);
+}
+
+void wl_chunk_free(wl_chunk_t *c)
+{
+    if (!c) return;
+    jobs_free(&c->ext); jobs_free(&c->loc); jobs_free(&c->nw);
+    free(c);
+}

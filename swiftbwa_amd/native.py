@@ -39,6 +39,25 @@ class HipswStats(C.Structure):
                 "tier_jobs": list(self.tier_jobs), "cells": self.cells}
 
 
+class HipswChunk(C.Structure):
+    _P = C.c_void_p
+    _fields_ = [("n_ext", C.c_int32), ("ext_q", _P), ("ext_q_off", _P), ("ext_q_len", _P),
+                ("ext_t_pos", _P), ("ext_t_len", _P), ("ext_t_rev", _P),
+                ("ext_h0", _P), ("ext_w", _P), ("ext_eb", _P),
+                ("n_loc", C.c_int32), ("loc_q", _P), ("loc_q_off", _P), ("loc_q_len", _P),
+                ("loc_t_pos", _P), ("loc_t_len", _P), ("loc_min_sub", C.c_int32),
+                ("n_nw", C.c_int32), ("nw_q", _P), ("nw_q_off", _P), ("nw_q_len", _P),
+                ("nw_t_pos", _P), ("nw_t_len", _P), ("nw_w", _P), ("nw_true_score", _P)]
+
+
+class HipswChunkResult(C.Structure):
+    _P = C.c_void_p
+    _fields_ = [("ext_out", _P), ("loc_out", _P), ("loc_ok", _P), ("nw_score", _P),
+                ("nw_cigar", _P), ("nw_n_cigar", _P), ("nw_flags", _P),
+                ("latency_ms", C.c_double), ("queue_ms", C.c_double), ("run_ms", C.c_double),
+                ("cells", C.c_int64)]
+
+
 # Every symbol include/hipsw.h declares, with (restype, argtypes).
 _P = C.c_void_p
 _I32P = C.POINTER(C.c_int32)
@@ -67,6 +86,14 @@ _SIGS = {
     "hipsw_local_sw": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, C.c_int32, C.c_int32,
                                  C.c_int32, C.POINTER(HipswScoring), _P, _P,
                                  C.POINTER(HipswStats)]),
+    "hipsw_server_create": (_P, [_P, C.c_int32, C.c_int64, C.POINTER(HipswScoring),
+                                 C.c_int32, C.c_int32, C.c_int32]),
+    "hipsw_server_set_reference": (C.c_int, [_P, _P, C.c_int64]),
+    "hipsw_server_submit": (C.c_int64, [_P, C.POINTER(HipswChunk)]),
+    "hipsw_server_poll": (C.c_int, [_P, C.c_int64]),
+    "hipsw_server_wait": (C.c_int, [_P, C.c_int64, C.POINTER(HipswChunkResult)]),
+    "hipsw_server_stats": (C.c_int, [_P, _P, _P, _P, _P]),
+    "hipsw_server_destroy": (None, [_P]),
 }
 
 _lib = None
