@@ -92,6 +92,50 @@ def bench_nw(eng, args):
     return res
 
 
+def bench_serving(eng, args):
+    """Serving loop: 250 Mbp reference with planted repeat families resident in
+    HBM; chunks of 2x150-bp read pairs (log-uniform 1k..200k pairs) carrying
+    their extension, mate-rescue and CIGAR jobs; Poisson arrivals at 25/50/75/90%
+    of the measured saturation throughput; p50/p99 of arrival -> results ready."""
+    from swiftbwa_amd import serving
+    t0 = time.perf_counter()
+    ref = serving.make_reference(5, args.serve_ref, copies=1000, divergence=0.10)
+    pac = serving.pack_pac(ref)
+    rng = np.random.default_rng(5)
+    sizes = np.exp(rng.uniform(np.log(1000), np.log(200000), args.serve_chunks)).astype(int)
+    pool = [serving.make_chunk(ref, 1000 + k, int(n)) for k, n in enumerate(sizes)]
+    log("serving: reference %d bp, %d chunks (%d pairs) generated in %.1f s" % (
+        len(ref), len(pool), int(sizes.sum()), time.perf_counter() - t0))
+    srv = serving.Server(eng, slots=args.serve_slots, hbm_budget=args.serve_budget_gb << 30)
+    try:
+        srv.set_reference(pac, len(ref))
+        srv.wait(srv.submit(pool[0]))                       # warm-up
+        t0 = time.perf_counter()
+        recs, wall = serving.run_poisson(srv, pool, 0.0)
+        sat = sizes.sum() / wall
+        log("serving saturation: %.0f pairs/s" % sat)
+        mean_pairs = sizes.mean()
+        loads = []
+        for rho in (0.25, 0.5, 0.75, 0.9):
+            rate = rho * sat / mean_pairs
+            recs, wall = serving.run_poisson(srv, pool, rate, seed=int(rho * 100))
+            lat = np.array([r["latency_ms"] for r in recs])
+            loads.append({"offered_load": rho, "chunks_per_s": round(rate, 3),
+                          "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                          "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                          "reads_per_s": round(2 * sizes.sum() / wall, 1),
+                          "mean_queue_ms": round(float(np.mean([r["queue_ms"] for r in recs])), 2)})
+            log("serving load %.2f: p50 %.1f ms p99 %.1f ms" % (rho, loads[-1]["p50_ms"], loads[-1]["p99_ms"]))
+        st = srv.stats()
+    finally:
+        srv.close()
+    return {"reference_bp": len(ref), "chunks": len(pool), "pairs_per_pass": int(sizes.sum()),
+            "slots": args.serve_slots, "hbm_budget_gb": args.serve_budget_gb,
+            "peak_admitted_bytes": int(st["peak_in_use"]),
+            "saturation_pairs_per_s": round(float(sat), 1),
+            "saturation_reads_per_s": round(float(2 * sat), 1), "loads": loads}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +145,11 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=600000)
     ap.add_argument("--nw-n", type=int, default=500000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-ref", type=int, default=250000000)
+    ap.add_argument("--serve-chunks", type=int, default=24)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-budget-gb", type=int, default=64)
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (each job set runs on one "
@@ -158,6 +207,7 @@ def main():
     lib.hipsw_ext_free(eng.handle, st)
 
     nw = bench_nw(eng, args) if args.nw_n > 0 else None
+    serve = None if args.no_serve else bench_serving(eng, args)
 
     gcups = cells / (b_wall * 1e-3) / 1e9
     achieved = cells * OPS_PER_CELL / (b_kms * 1e-3) / 1e12
@@ -187,6 +237,7 @@ def main():
             "upload_s_incl_pack": round(upload_s, 3),
             "device": dev_name, "n_cu": n_cu,
             "nw_cigar": nw,
+            "serving": serve,
         },
         "roofline": {
             "bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_PK16_TOPS,
