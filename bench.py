@@ -92,6 +92,47 @@ def bench_nw(eng, args):
     return res
 
 
+def bench_mixed(eng, args):
+    """Uneven-size batching: 4,000,000 extension jobs, half left (q U[0,150]) and
+    half right (q U[0,250]), w in {20,50,100}, 5% at 10% substitutions, 2% with
+    h0 U[61,250]; plain one-launch form vs the binned persistent form."""
+    import ctypes as C
+    from swiftbwa_amd import native, workload as wl
+    from swiftbwa_amd.metalsw import ScoringParameters, _ext_arrays
+    lib = eng.lib
+    js = wl.generate(wl.EXT_MIXED, 4, args.mixed_n)
+    a = _ext_arrays(js)
+    sc = ScoringParameters().to_c()
+    st = lib.hipsw_ext_upload(eng.handle, js["n"], native.ptr(a["queries"]), native.ptr(a["q_off"]),
+                              native.ptr(a["q_len"]), native.ptr(a["targets"]),
+                              native.ptr(a["t_off"]), native.ptr(a["t_len"]), native.ptr(a["h0"]),
+                              native.ptr(a["w"]), native.ptr(a["end_bonus"]), C.byref(sc))
+    if not st:
+        raise RuntimeError(eng.last_error())
+    res = {"jobs": js["n"]}
+    for name, mode in (("plain", native.MODE_PLAIN), ("batched", native.MODE_BATCHED)):
+        s = native.HipswStats()
+        for _ in range(args.warmup):
+            if lib.hipsw_ext_run(eng.handle, st, mode, C.byref(s)):
+                raise RuntimeError(eng.last_error())
+        kms = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            if lib.hipsw_ext_run(eng.handle, st, mode, C.byref(s)):
+                raise RuntimeError(eng.last_error())
+            kms.append(s.kernel_ms)
+        wall = (time.perf_counter() - t0) * 1000.0 / args.steps
+        res[name] = {"ms_per_step": round(wall, 3), "kernel_ms": round(float(np.mean(kms)), 3),
+                     "gcups": round(s.cells / (wall * 1e-3) / 1e9, 2),
+                     "ext_per_s": round(js["n"] / (wall * 1e-3), 1), "launches": int(s.launches)}
+        res["cells"] = int(s.cells)
+        res["tier_jobs"] = list(s.tier_jobs)
+        log("mixed %s: %.2f ms/step (kernel %.2f ms)" % (name, wall, np.mean(kms)))
+    lib.hipsw_ext_free(eng.handle, st)
+    res["batched_over_plain"] = round(res["plain"]["ms_per_step"] / res["batched"]["ms_per_step"], 3)
+    return res
+
+
 def bench_serving(eng, args):
     """Serving loop: 250 Mbp reference with planted repeat families resident in
     HBM; chunks of 2x150-bp read pairs (log-uniform 1k..200k pairs) carrying
@@ -150,6 +191,7 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-budget-gb", type=int, default=64)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--mixed-n", type=int, default=4000000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (each job set runs on one "
@@ -207,6 +249,7 @@ def main():
     lib.hipsw_ext_free(eng.handle, st)
 
     nw = bench_nw(eng, args) if args.nw_n > 0 else None
+    mixed = bench_mixed(eng, args) if args.mixed_n > 0 else None
     serve = None if args.no_serve else bench_serving(eng, args)
 
     gcups = cells / (b_wall * 1e-3) / 1e9
@@ -237,6 +280,7 @@ def main():
             "upload_s_incl_pack": round(upload_s, 3),
             "device": dev_name, "n_cu": n_cu,
             "nw_cigar": nw,
+            "uneven_batching": mixed,
             "serving": serve,
         },
         "roofline": {

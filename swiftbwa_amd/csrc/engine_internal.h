@@ -66,6 +66,7 @@ void radix_sort_by_key(std::vector<uint32_t> &keys, std::vector<uint32_t> &idx);
 struct ExtLaunch {
     int tier;
     bool global_state;
+    bool packed;          // two jobs per lane (8-bit tier), items of 128 jobs
     int cols;
     int item_begin, item_count;
     int grid;

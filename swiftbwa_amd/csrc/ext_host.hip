@@ -94,10 +94,15 @@ size_t plan_ext_batched(const hipsw_engine *eng, int n, const int32_t *ql, const
         L.tier = (int)bin / kClasses;
         L.cols = cmax + 1;
         L.global_state = ext_lds_bytes(L.tier, false, L.cols) > (size_t)eng->lds_per_block;
+        L.packed = L.tier == TIER_U8 && cmax <= 255 &&
+                   ext_pk_lds_bytes(L.cols) <= (size_t)eng->lds_per_block;
+        const int per_item = L.packed ? 128 : 64;
         L.item_begin = (int)items.size();
-        for (int g = s; g < e; g += 64) items.push_back(WorkItem{(uint32_t)g, (uint32_t)std::min(64, e - g)});
+        for (int g = s; g < e; g += per_item)
+            items.push_back(WorkItem{(uint32_t)g, (uint32_t)std::min(per_item, e - g)});
         L.item_count = (int)items.size() - L.item_begin;
-        const int wpc = L.global_state ? 8 : waves_per_cu(eng, ext_lds_bytes(L.tier, false, L.cols));
+        const size_t lds = L.packed ? ext_pk_lds_bytes(L.cols) : ext_lds_bytes(L.tier, false, L.cols);
+        const int wpc = L.global_state ? 8 : waves_per_cu(eng, lds);
         L.grid = std::max(1, std::min(L.item_count, eng->n_cu * wpc));
         L.scratch_off = scratch_off;
         if (L.global_state) scratch_off += (size_t)L.grid * L.cols * 64 * ext_state_bytes(L.tier);
@@ -306,12 +311,16 @@ extern "C" int hipsw_ext_run(hipsw_engine *eng, hipsw_ext_set *set, int32_t mode
             int k = 0;
             for (const ExtLaunch &L : set->launches) {
                 void *lgs = L.global_state ? (char *)gs + L.scratch_off : nullptr;
-                if (!eng->check(launch_ext_persistent(L.tier, L.global_state, set->dev, set->params,
-                                                      set->d_order, set->d_items + L.item_begin,
-                                                      L.item_count, eng->counters + k, L.cols,
-                                                      L.grid, lgs, eng->stream),
-                                "launch ext_persistent"))
-                    return HIPSW_E_DEVICE;
+                const hipError_t le =
+                    L.packed ? launch_ext_persistent_pk(set->dev, set->params, set->d_order,
+                                                        set->d_items + L.item_begin, L.item_count,
+                                                        eng->counters + k, L.cols, L.grid,
+                                                        eng->stream)
+                             : launch_ext_persistent(L.tier, L.global_state, set->dev, set->params,
+                                                     set->d_order, set->d_items + L.item_begin,
+                                                     L.item_count, eng->counters + k, L.cols,
+                                                     L.grid, lgs, eng->stream);
+                if (!eng->check(le, "launch ext_persistent")) return HIPSW_E_DEVICE;
                 ++k;
                 ++launches;
             }

@@ -550,4 +550,300 @@ hipError_t launch_ext_persistent(int tier, bool global_state, const ExtDev &d,
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Packed form: two jobs per lane, 8-bit tier.
+//
+// Lane l carries job idx[l] in the low 16-bit half and job idx[l+64] in the
+// high half of every 32-bit register, and each cell step is a packed 16-bit
+// op (v_pk_*) on both.  The column state word holds (hA, eA, hB, eB) as bytes
+// 0..3, so a column read yields both jobs' H and E and a job's own (h, e) pair
+// is one 16-bit half (its band-end store is one ds_write_b16).  All values of
+// the 8-bit tier are <= 255, so every packed intermediate (M + A, M * (A+1),
+// H << 8 | j) fits in 16 bits and the saturating 16-bit subtract is the
+// max(x - c, 0) of the scalar recurrence.
+// ---------------------------------------------------------------------------
+namespace {
+
+typedef unsigned short us2 __attribute__((ext_vector_type(2)));
+typedef short ss2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ us2 u2(uint32_t v) { return __builtin_bit_cast(us2, v); }
+__device__ __forceinline__ uint32_t w32(us2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ us2 splat(int v) { return us2{(unsigned short)v, (unsigned short)v}; }
+__device__ __forceinline__ us2 vmax(us2 a, us2 b) { return __builtin_elementwise_max(a, b); }
+__device__ __forceinline__ us2 vmin(us2 a, us2 b) { return __builtin_elementwise_min(a, b); }
+__device__ __forceinline__ us2 vsubs(us2 a, us2 b) { return __builtin_elementwise_sub_sat(a, b); }
+
+// 16 match/mismatch bits of a block, compacted: bit k = base k differs.
+__device__ __forceinline__ uint32_t mismatch16(uint32_t qw, uint32_t tpat)
+{
+    uint32_t x = qw ^ tpat;
+    x = (x | (x >> 1)) & 0x55555555u;
+    x = (x | (x >> 1)) & 0x33333333u;
+    x = (x | (x >> 2)) & 0x0F0F0F0Fu;
+    x = (x | (x >> 4)) & 0x00FF00FFu;
+    x = (x | (x >> 8)) & 0x0000FFFFu;
+    return x;
+}
+
+// Per-job row-level state of one half.
+struct PkJob {
+    bool valid, nonempty, alive;
+    int job, qlen, tlen, h0, w, beg, end, maxv, max_i, max_j, gsc, gtle, max_off;
+    uint32_t qw0, tw0, tw, tpat;
+    uint64_t ncells;
+};
+
+__device__ __forceinline__ void pk_setup(PkJob &J, const ExtDev &d, const ExtParams &p,
+                                         const uint32_t *idx, int slot, int count)
+{
+    J.valid = slot < count;
+    J.job = J.valid ? (int)idx[slot] : 0;
+    J.qlen = J.valid ? d.q_len[J.job] : 0;
+    J.tlen = J.valid ? d.t_len[J.job] : 0;
+    J.h0 = J.valid ? d.h0[J.job] : 0;
+    int w = J.valid ? d.w[J.job] : 0;
+    const int eb = J.valid ? d.eb[J.job] : 0;
+    J.qw0 = J.valid ? d.q_word[J.job] : 0;
+    J.tw0 = J.valid ? d.t_word[J.job] : 0;
+    J.nonempty = J.valid && J.qlen > 0 && J.tlen > 0;        // :33-35
+    J.alive = J.nonempty;
+    if (J.alive) {                                             // w cap, :52-62
+        int mi = (int)((double)(J.qlen * p.max_mat + eb - p.o_ins) / (double)p.e_ins + 1.0);
+        w = min(w, max(1, mi));
+        int md = (int)((double)(J.qlen * p.max_mat + eb - p.o_del) / (double)p.e_del + 1.0);
+        w = min(w, max(1, md));
+    }
+    J.w = w;
+    J.beg = 0; J.end = J.qlen;                                 // :79-87
+    J.maxv = J.h0; J.max_i = -1; J.max_j = -1;
+    J.gsc = -1; J.gtle = -1; J.max_off = 0;
+    J.tw = 0; J.tpat = 0; J.ncells = 0;
+}
+
+// Row start of one half (:95-107); returns h1.
+__device__ __forceinline__ int pk_row_start(PkJob &J, const ExtDev &d, const ExtParams &p, int i)
+{
+    if (J.alive && i >= J.tlen) J.alive = false;
+    if (!J.alive) return 0;
+    if ((i & 15) == 0) J.tw = d.seq[J.tw0 + (i >> 4)];
+    const uint32_t t = (J.tw >> ((i & 15) * 2)) & 3u;
+    J.tpat = t * 0x55555555u;
+    J.beg = max(J.beg, i - J.w);
+    J.end = min(J.end, i + J.w + 1);
+    J.end = min(J.end, J.qlen);
+    if (J.end > J.beg) J.ncells += (uint64_t)(J.end - J.beg);
+    return J.beg == 0 ? max(0, J.h0 - (p.o_del + p.e_del * (i + 1))) : 0;
+}
+
+// Row end of one half (:145-180).  `half` selects the 16-bit half of col[].
+__device__ __forceinline__ void pk_row_end(PkJob &J, const ExtParams &p, int i, int h1, int key,
+                                           uint32_t *col, int half)
+{
+    if (!J.alive) return;
+    const int row_max = key >> 8, row_max_j = key & 0xFF;
+    uint16_t *c16 = reinterpret_cast<uint16_t *>(col) + half;   // (h, e) of this job: u16 at col16[j*128]
+    c16[J.end * 128] = (uint16_t)h1;                           // h[end] = h1, e[end] = 0
+    if (J.end == J.qlen) {                                     // :149-152
+        J.gtle = J.gsc > h1 ? J.gtle : i;
+        J.gsc = max(J.gsc, h1);
+    }
+    if (row_max == 0) {                                        // :155
+        J.alive = false;
+    } else if (row_max > J.maxv) {                             // :158-163
+        J.maxv = row_max;
+        J.max_i = i;
+        J.max_j = row_max_j;
+        J.max_off = max(J.max_off, abs(row_max_j - i));
+    } else if (p.zdrop > 0) {                                  // :164-172
+        const int di = i - J.max_i, dj = row_max_j - J.max_j;
+        const int drop = di > dj ? J.maxv - row_max - (di - dj) * p.e_del
+                                 : J.maxv - row_max - (dj - di) * p.e_ins;
+        if (drop > p.zdrop) J.alive = false;
+    }
+    if (J.alive) {                                             // :175-180
+        int nb = J.beg;
+        while (nb < J.end && c16[nb * 128] == 0) ++nb;
+        J.beg = nb;
+        int ne = J.end;
+        while (ne >= J.beg && c16[ne * 128] == 0) --ne;
+        J.end = min(ne + 2, J.qlen);
+    }
+}
+
+__device__ __forceinline__ void pk_finish(const PkJob &J, const ExtDev &d)
+{
+    if (!J.valid) return;                                      // :183-190
+    int32_t *o = d.out + (size_t)J.job * 6;
+    if (J.nonempty) {
+        o[0] = J.maxv; o[1] = J.max_j + 1; o[2] = J.max_i + 1;
+        o[3] = J.gtle + 1; o[4] = J.gsc; o[5] = J.max_off;
+    } else {
+        o[0] = 0; o[1] = 0; o[2] = 0; o[3] = 0; o[4] = 0; o[5] = 0;
+    }
+}
+
+}  // namespace
+
+// Extends jobs idx[0..count) (count <= 128) with one wave, two per lane.
+// Requirements (checked by the caller): 8-bit tier for every job, qlen <= 256,
+// no N base, the standard A/-B matrix.
+__device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const uint32_t *idx, int count,
+                            uint32_t *col)
+{
+    const int lane = threadIdx.x & 63;
+    PkJob A, B;
+    pk_setup(A, d, p, idx, lane, count);
+    pk_setup(B, d, p, idx, lane + 64, count);
+    const int qmax = wave_max(max(A.alive ? A.qlen : 0, B.alive ? B.qlen : 0));
+    for (int j = 0; j <= qmax; ++j) {                          // first row, :68-77
+        const int ha = (A.alive && j <= A.qlen) ? (j == 0 ? A.h0 : max(0, A.h0 - p.o_ins - p.e_ins * j)) : 0;
+        const int hb = (B.alive && j <= B.qlen) ? (j == 0 ? B.h0 : max(0, B.h0 - p.o_ins - p.e_ins * j)) : 0;
+        col[j * 64] = (uint32_t)ha | ((uint32_t)hb << 16);
+    }
+    const us2 A2 = splat(p.match_sc), A1 = splat(p.match_sc + 1);
+    const ss2 Ssc = ss2{(short)p.match_sc, (short)p.match_sc};
+    const ss2 nAB = ss2{(short)(p.mis_sc - p.match_sc), (short)(p.mis_sc - p.match_sc)};
+    const us2 eD = splat(p.e_del), eI = splat(p.e_ins), oeD = splat(p.oe_del), oeI = splat(p.oe_ins);
+    const uint32_t LOW = 0x00FF00FFu;
+
+    for (int i = 0;; ++i) {
+        const int h1a = pk_row_start(A, d, p, i);
+        const int h1b = pk_row_start(B, d, p, i);
+        if (__ballot(A.alive || B.alive) == 0) break;
+        us2 h1 = us2{(unsigned short)h1a, (unsigned short)h1b};
+        us2 f = splat(0), key = splat(0);
+        const int BIG = 0x7fffffff;
+        const int jlo = wave_min(min(A.alive ? A.beg : BIG, B.alive ? B.beg : BIG));
+        const int jhi = wave_max(max(A.alive ? A.end : 0, B.alive ? B.end : 0));
+        // [mlo, mhi): columns inside the band of every live job of the wave
+        const int mlo = wave_max(max(A.alive ? A.beg : 0, B.alive ? B.beg : 0));
+        const int mhi = wave_min(min(A.alive ? A.end : BIG, B.alive ? B.end : BIG));
+
+        // one packed cell; ab = per-half mismatch flags (0 or 1 in each half)
+        auto cell = [&](uint32_t &word, uint32_t ab, uint32_t jj) {
+            const us2 M = u2(word & LOW);
+            const us2 E = u2(__builtin_amdgcn_perm(0u, word, 0x0C030C01u));   // bytes 1, 3
+            const ss2 s = __builtin_bit_cast(ss2, ab) * nAB + Ssc;   // A or -B per half
+            us2 diag = __builtin_bit_cast(us2, __builtin_elementwise_max(
+                           __builtin_bit_cast(ss2, M) + s, ss2{0, 0}));   // max(M + s, 0)
+            diag = vmin(diag, M * A1);                               // M == 0 -> 0 (:117)
+            const us2 H = vmax(vmax(diag, E), f);
+            key = vmax(key, (H << 8) | u2(jj * 0x00010001u));
+            const us2 ne = vmax(vsubs(E, eD), vsubs(diag, oeD));
+            f = vmax(vsubs(f, eI), vsubs(diag, oeI));
+            // bytes (h1.lo, ne.lo, h1.hi, ne.hi): a finished job's drifting
+            // values can never spill into the other half
+            word = __builtin_amdgcn_perm(w32(ne), w32(h1), 0x06020400u);
+            h1 = H;
+        };
+        (void)A2;
+        if (jlo < jhi) {
+            // query words of the current and the next 16-column block
+            int b = jlo >> 4;
+            const int bend = (jhi - 1) >> 4;
+            uint32_t qa = b * 16 < A.qlen ? d.seq[A.qw0 + b] : 0u;
+            uint32_t qb = b * 16 < B.qlen ? d.seq[B.qw0 + b] : 0u;
+            for (; b <= bend; ++b) {
+                const uint32_t na = (b + 1) * 16 < A.qlen ? d.seq[A.qw0 + b + 1] : 0u;
+                const uint32_t nb = (b + 1) * 16 < B.qlen ? d.seq[B.qw0 + b + 1] : 0u;
+                const uint32_t mm = mismatch16(qa, A.tpat) | (mismatch16(qb, B.tpat) << 16);
+                uint32_t *cb = col + b * 16 * 64;
+                const int c0 = b * 16;
+                if (c0 >= mlo && c0 + 16 <= mhi) {
+                    // whole block inside every live band: no masking
+                    uint32_t wv[16];
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) wv[k] = cb[k * 64];
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) {
+                        cell(wv[k], (mm >> k) & 0x00010001u, (uint32_t)(c0 + k));
+                        cb[k * 64] = wv[k];
+                    }
+                } else {
+                    // each half commits only inside its own band
+                    const int k0 = max(jlo, c0) - c0, k1 = min(jhi, c0 + 16) - c0;
+                    for (int k = k0; k < k1; ++k) {
+                        const int j = c0 + k;
+                        const bool ia = A.alive && (unsigned)(j - A.beg) < (unsigned)(A.end - A.beg);
+                        const bool ib = B.alive && (unsigned)(j - B.beg) < (unsigned)(B.end - B.beg);
+                        const uint32_t old = cb[k * 64];
+                        uint32_t word = old;
+                        const us2 f0 = f, k0v = key, h0v = h1;
+                        cell(word, (mm >> k) & 0x00010001u, (uint32_t)j);
+                        const uint32_t hm = (ia ? 0x0000FFFFu : 0u) | (ib ? 0xFFFF0000u : 0u);
+                        cb[k * 64] = (word & hm) | (old & ~hm);
+                        f = u2((w32(f) & hm) | (w32(f0) & ~hm));
+                        key = u2((w32(key) & hm) | (w32(k0v) & ~hm));
+                        h1 = u2((w32(h1) & hm) | (w32(h0v) & ~hm));
+                    }
+                }
+                qa = na;
+                qb = nb;
+            }
+        }
+
+        const uint32_t kw = w32(key), hw = w32(h1);
+        pk_row_end(A, p, i, (int)(hw & 0xFFFF), (int)(kw & 0xFFFF), col, 0);
+        pk_row_end(B, p, i, (int)(hw >> 16), (int)(kw >> 16), col, 1);
+    }
+    if (d.cells) {
+        unsigned long long c = A.ncells + B.ncells;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+        if (lane == 0) atomicAdd(d.cells, c);
+    }
+    pk_finish(A, d);
+    pk_finish(B, d);
+}
+
+// Persistent kernel of the packed form.  Items hold up to 128 jobs; an item
+// whose jobs include an N base runs as two generic 64-job halves instead.
+__global__ __launch_bounds__(64) void ext_persistent_pk_kernel(ExtDev d, ExtParams p,
+                                                               const uint32_t *order,
+                                                               const WorkItem *items, int n_items,
+                                                               uint32_t *counter, int cols)
+{
+    const int8_t *smat = load_smat(p);
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    uint32_t *col = reinterpret_cast<uint32_t *>(lds_raw + 32) + threadIdx.x;
+    __shared__ uint32_t idx[128];
+    __shared__ int item_sh;
+    for (;;) {
+        if (threadIdx.x == 0) item_sh = (int)atomicAdd(counter, 1u);
+        __syncthreads();
+        const int item = item_sh;
+        if (item >= n_items) break;
+        const WorkItem wi = items[item];
+        for (int k = threadIdx.x; k < (int)wi.count; k += 64) idx[k] = order[wi.start + k];
+        __syncthreads();
+        const int lane = threadIdx.x;
+        bool has_n = false;
+        for (int k = lane; k < (int)wi.count; k += 64)
+            has_n = has_n || d.qn_word[idx[k]] != kNoN || d.tn_word[idx[k]] != kNoN;
+        if (__ballot(has_n) == 0) {
+            ext_wave_pk(d, p, idx, (int)wi.count, col);
+        } else {
+            uint16_t *c16 = reinterpret_cast<uint16_t *>(lds_raw + 32) + threadIdx.x;
+            ext_wave_generic<TIER_U8>(d, p, idx, min(64, (int)wi.count), c16, smat);
+            if (wi.count > 64)
+                ext_wave_generic<TIER_U8>(d, p, idx + 64, (int)wi.count - 64, c16, smat);
+        }
+        __syncthreads();
+    }
+}
+
+size_t ext_pk_lds_bytes(int cols) { return 32 + (size_t)cols * 64 * 4; }
+
+hipError_t launch_ext_persistent_pk(const ExtDev &d, const ExtParams &p, const uint32_t *order,
+                                    const WorkItem *items, int n_items, uint32_t *counter,
+                                    int cols, int grid, hipStream_t s)
+{
+    if (n_items <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ext_persistent_pk_kernel, dim3(grid), dim3(64), ext_pk_lds_bytes(cols), s, d,
+                       p, order, items, n_items, counter, cols);
+    return hipGetLastError();
+}
+
 }  // namespace hipsw

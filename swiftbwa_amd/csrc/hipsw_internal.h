@@ -64,4 +64,9 @@ hipError_t launch_ext_persistent(int tier, bool global_state, const ExtDev &d,
 hipError_t launch_zero_u32(uint32_t *p, int n, hipStream_t s);
 size_t ext_state_bytes(int tier);   // bytes of h/e state per column per lane
 size_t ext_lds_bytes(int tier, bool global_state, int cols);   // dynamic LDS per block
+// Packed form (two jobs per lane, 8-bit tier, LDS state, items of <= 128 jobs).
+size_t ext_pk_lds_bytes(int cols);
+hipError_t launch_ext_persistent_pk(const ExtDev &d, const ExtParams &p, const uint32_t *order,
+                                    const WorkItem *items, int n_items, uint32_t *counter,
+                                    int cols, int grid, hipStream_t s);
 }  // namespace hipsw

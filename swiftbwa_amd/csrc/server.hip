@@ -263,9 +263,13 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
         int k = 0;
         for (const ExtLaunch &L : ext_launch) {
             if (!ok || k >= 64) { ok = false; break; }
-            ok = launch_ext_persistent(L.tier, L.global_state, ed, ep, d_order, d_items + L.item_begin,
-                                       L.item_count, sl.counters + k, L.cols, L.grid,
-                                       L.global_state ? (char *)gs + L.scratch_off : nullptr, st) == hipSuccess;
+            ok = (L.packed ? launch_ext_persistent_pk(ed, ep, d_order, d_items + L.item_begin,
+                                                      L.item_count, sl.counters + k, L.cols, L.grid, st)
+                           : launch_ext_persistent(L.tier, L.global_state, ed, ep, d_order,
+                                                   d_items + L.item_begin, L.item_count,
+                                                   sl.counters + k, L.cols, L.grid,
+                                                   L.global_state ? (char *)gs + L.scratch_off : nullptr,
+                                                   st)) == hipSuccess;
             ++k;
         }
         if (gs) hipFreeAsync(gs, st);
