@@ -16,8 +16,10 @@ using namespace hipsw;
 namespace {
 
 // Query-length classes of the batched form: each class is one persistent
-// launch whose LDS footprint is sized to the class' longest query.
-constexpr int kClassBounds[] = {64, 160, 256, 1 << 30};
+// launch whose LDS footprint is sized to the class' longest query.  The bounds
+// end one short of a 16-column multiple, so a class fills its packed-form LDS
+// exactly (columns 0..qlen): 150-bp queries take 160 columns, four waves a CU.
+constexpr int kClassBounds[] = {63, 159, 255, 1 << 30};
 constexpr int kClasses = 4;
 
 int class_of(int qlen)
@@ -101,8 +103,13 @@ size_t plan_ext_batched(const hipsw_engine *eng, int n, const int32_t *ql, const
         for (int g = s; g < e; g += per_item)
             items.push_back(WorkItem{(uint32_t)g, (uint32_t)std::min(per_item, e - g)});
         L.item_count = (int)items.size() - L.item_begin;
-        const size_t lds = L.packed ? ext_pk_lds_bytes(L.cols) : ext_lds_bytes(L.tier, false, L.cols);
-        const int wpc = L.global_state ? 8 : waves_per_cu(eng, lds);
+        int wpc;
+        if (L.packed) {   // no static shared memory: the CU holds whole multiples
+            const size_t per_cu = eng->lds_per_cu > 0 ? (size_t)eng->lds_per_cu : 65536;
+            wpc = (int)std::max<size_t>(1, std::min<size_t>(8, per_cu / ext_pk_lds_bytes(L.cols)));
+        } else {
+            wpc = L.global_state ? 8 : waves_per_cu(eng, ext_lds_bytes(L.tier, false, L.cols));
+        }
         L.grid = std::max(1, std::min(L.item_count, eng->n_cu * wpc));
         L.scratch_off = scratch_off;
         if (L.global_state) scratch_off += (size_t)L.grid * L.cols * 64 * ext_state_bytes(L.tier);

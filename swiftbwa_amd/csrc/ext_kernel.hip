@@ -559,9 +559,23 @@ hipError_t launch_ext_persistent(int tier, bool global_state, const ExtDev &d,
 // op (v_pk_*) on both.  The column state word holds (hA, eA, hB, eB) as bytes
 // 0..3, so a column read yields both jobs' H and E and a job's own (h, e) pair
 // is one 16-bit half (its band-end store is one ds_write_b16).  All values of
-// the 8-bit tier are <= 255, so every packed intermediate (M + A, M * (A+1),
-// H << 8 | j) fits in 16 bits and the saturating 16-bit subtract is the
-// max(x - c, 0) of the scalar recurrence.
+// the 8-bit tier are <= 255, so every packed intermediate (M + A, H * 256 + j)
+// fits in 16 bits and the saturating 16-bit subtract is the max(x - c, 0) of
+// the scalar recurrence.
+//
+// Band edges.  Each row sweeps whole 16-column blocks from the wave's lowest
+// band start to its highest band end.
+//  * Left of a job's band (j < beg) the job computes without masks: ksw_extend2
+//    never reads h/e entries below beg again (beg only grows), so the kernel
+//    keeps every such entry at zero -- the narrowing step (:175-177) leaves
+//    zeros behind, and the one entry the w-clamp (:96) drops per row is
+//    cleared -- and a cell with M = E = F = 0 and h1 = 0 (the row's h1 when
+//    beg > 0) yields H = 0 and writes zeros again.  Its (H = 0, j) row-max
+//    candidates never win: a row whose maximum is 0 ends the job (:155).
+//  * Right of a job's band (j >= end) entries are live (the band end may grow
+//    by up to two columns per row), so a block that crosses some job's end
+//    commits the state word, h1 and the row-max candidate under a per-half
+//    mask; blocks left of every live end run unmasked.
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -574,6 +588,31 @@ __device__ __forceinline__ us2 splat(int v) { return us2{(unsigned short)v, (uns
 __device__ __forceinline__ us2 vmax(us2 a, us2 b) { return __builtin_elementwise_max(a, b); }
 __device__ __forceinline__ us2 vmin(us2 a, us2 b) { return __builtin_elementwise_min(a, b); }
 __device__ __forceinline__ us2 vsubs(us2 a, us2 b) { return __builtin_elementwise_sub_sat(a, b); }
+// A constant the compiler cannot see through: keeps min(M, 1) a packed min
+// instead of per-half compares and selects.
+__device__ __forceinline__ uint32_t opaque(uint32_t v)
+{
+    asm volatile("" : "+s"(v));
+    return v;
+}
+
+// Wave-uniform min / max of a per-lane int (all 64 lanes active): DPP steps
+// reduce each 16-lane row, lane reads combine the four rows in scalar registers.
+__device__ __forceinline__ int row_reduce_min(int v)
+{
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));    // quad_perm 1,0,3,2
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));    // quad_perm 2,3,0,1
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false));   // row_half_mirror
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false));   // row_mirror
+    return v;
+}
+__device__ __forceinline__ int uniform_min(int v)
+{
+    v = row_reduce_min(v);
+    return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+               min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+__device__ __forceinline__ int uniform_max(int v) { return -uniform_min(-v); }
 
 // 16 match/mismatch bits of a block, compacted: bit k = base k differs.
 __device__ __forceinline__ uint32_t mismatch16(uint32_t qw, uint32_t tpat)
@@ -591,7 +630,7 @@ __device__ __forceinline__ uint32_t mismatch16(uint32_t qw, uint32_t tpat)
 struct PkJob {
     bool valid, nonempty, alive;
     int job, qlen, tlen, h0, w, beg, end, maxv, max_i, max_j, gsc, gtle, max_off;
-    uint32_t qw0, tw0, tw, tpat;
+    uint32_t qw0, tw0, tw, twn, tpat;
     uint64_t ncells;
 };
 
@@ -619,31 +658,43 @@ __device__ __forceinline__ void pk_setup(PkJob &J, const ExtDev &d, const ExtPar
     J.beg = 0; J.end = J.qlen;                                 // :79-87
     J.maxv = J.h0; J.max_i = -1; J.max_j = -1;
     J.gsc = -1; J.gtle = -1; J.max_off = 0;
-    J.tw = 0; J.tpat = 0; J.ncells = 0;
+    // target words of rows 0-15 and 16-31 (the second is a prefetch)
+    J.tw = J.alive ? d.seq[J.tw0] : 0u;
+    J.twn = (J.alive && J.tlen > 16) ? d.seq[J.tw0 + 1] : 0u;
+    J.tpat = 0; J.ncells = 0;
 }
 
-// Row start of one half (:95-107); returns h1.
-__device__ __forceinline__ int pk_row_start(PkJob &J, const ExtDev &d, const ExtParams &p, int i)
+// Row start of one half (:95-107); returns h1.  `c16` is this half's column
+// state (u16 (h, e) of column j at c16[j * 128]).
+__device__ __forceinline__ int pk_row_start(PkJob &J, const ExtDev &d, const ExtParams &p, int i,
+                                            uint16_t *c16)
 {
     if (J.alive && i >= J.tlen) J.alive = false;
     if (!J.alive) return 0;
-    if ((i & 15) == 0) J.tw = d.seq[J.tw0 + (i >> 4)];
+    if (i > 0 && (i & 15) == 0) {                              // next 16 target bases
+        J.tw = J.twn;
+        J.twn = i + 16 < J.tlen ? d.seq[J.tw0 + (i >> 4) + 1] : 0u;
+    }
     const uint32_t t = (J.tw >> ((i & 15) * 2)) & 3u;
     J.tpat = t * 0x55555555u;
-    J.beg = max(J.beg, i - J.w);
+    const int wb = i - J.w;
+    if (J.beg < wb) {                                          // beg = i - w drops column beg
+        for (int j = J.beg; j < wb; ++j) c16[j * 128] = 0;
+        J.beg = wb;
+    }
     J.end = min(J.end, i + J.w + 1);
     J.end = min(J.end, J.qlen);
     if (J.end > J.beg) J.ncells += (uint64_t)(J.end - J.beg);
     return J.beg == 0 ? max(0, J.h0 - (p.o_del + p.e_del * (i + 1))) : 0;
 }
 
-// Row end of one half (:145-180).  `half` selects the 16-bit half of col[].
+// Row end of one half (:145-172): band-end entry, global-score column, row
+// maximum and z-drop.
 __device__ __forceinline__ void pk_row_end(PkJob &J, const ExtParams &p, int i, int h1, int key,
-                                           uint32_t *col, int half)
+                                           uint16_t *c16)
 {
     if (!J.alive) return;
     const int row_max = key >> 8, row_max_j = key & 0xFF;
-    uint16_t *c16 = reinterpret_cast<uint16_t *>(col) + half;   // (h, e) of this job: u16 at col16[j*128]
     c16[J.end * 128] = (uint16_t)h1;                           // h[end] = h1, e[end] = 0
     if (J.end == J.qlen) {                                     // :149-152
         J.gtle = J.gsc > h1 ? J.gtle : i;
@@ -662,14 +713,43 @@ __device__ __forceinline__ void pk_row_end(PkJob &J, const ExtParams &p, int i, 
                                  : J.maxv - row_max - (dj - di) * p.e_ins;
         if (drop > p.zdrop) J.alive = false;
     }
-    if (J.alive) {                                             // :175-180
-        int nb = J.beg;
-        while (nb < J.end && c16[nb * 128] == 0) ++nb;
-        J.beg = nb;
-        int ne = J.end;
-        while (ne >= J.beg && c16[ne * 128] == 0) --ne;
-        J.end = min(ne + 2, J.qlen);
+}
+
+// Band narrowing of both halves (:175-180), one LDS round trip per step.
+__device__ __forceinline__ void pk_narrow(PkJob &A, PkJob &B, int h1a, int h1b, uint16_t *ca,
+                                          uint16_t *cb)
+{
+    int ja = A.beg, jb = B.beg;
+    bool ga = A.alive && ja < A.end, gb = B.alive && jb < B.end;
+    while (ga || gb) {
+        const uint32_t va = ga ? ca[ja * 128] : 1u;
+        const uint32_t vb = gb ? cb[jb * 128] : 1u;
+        if (ga) {
+            if (va == 0) { ++ja; ga = ja < A.end; } else ga = false;
+        }
+        if (gb) {
+            if (vb == 0) { ++jb; gb = jb < B.end; } else gb = false;
+        }
     }
+    if (A.alive) A.beg = ja;
+    if (B.alive) B.beg = jb;
+    // column end holds (h1, 0): only a zero h1 needs the backward scan
+    int ka = A.end, kb = B.end;
+    bool ea = A.alive && ka >= A.beg && h1a == 0, eb = B.alive && kb >= B.beg && h1b == 0;
+    if (ea) { --ka; ea = ka >= A.beg; }
+    if (eb) { --kb; eb = kb >= B.beg; }
+    while (ea || eb) {
+        const uint32_t va = ea ? ca[ka * 128] : 1u;
+        const uint32_t vb = eb ? cb[kb * 128] : 1u;
+        if (ea) {
+            if (va == 0) { --ka; ea = ka >= A.beg; } else ea = false;
+        }
+        if (eb) {
+            if (vb == 0) { --kb; eb = kb >= B.beg; } else eb = false;
+        }
+    }
+    if (A.alive) A.end = min(ka + 2, A.qlen);
+    if (B.alive) B.end = min(kb + 2, B.qlen);
 }
 
 __device__ __forceinline__ void pk_finish(const PkJob &J, const ExtDev &d)
@@ -687,8 +767,10 @@ __device__ __forceinline__ void pk_finish(const PkJob &J, const ExtDev &d)
 }  // namespace
 
 // Extends jobs idx[0..count) (count <= 128) with one wave, two per lane.
-// Requirements (checked by the caller): 8-bit tier for every job, qlen <= 256,
-// no N base, the standard A/-B matrix.
+// Requirements (checked by the caller): 8-bit tier for every job, qlen <= 255,
+// no N base, the standard A/-B matrix, LDS for 16 * ceil((qmax + 1) / 16)
+// columns.  SYM: o_del == o_ins and e_del == e_ins (one shared gap-open term).
+template <bool SYM>
 __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const uint32_t *idx, int count,
                             uint32_t *col)
 {
@@ -696,97 +778,116 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const uint32_t 
     PkJob A, B;
     pk_setup(A, d, p, idx, lane, count);
     pk_setup(B, d, p, idx, lane + 64, count);
-    const int qmax = wave_max(max(A.alive ? A.qlen : 0, B.alive ? B.qlen : 0));
+    uint16_t *ca = reinterpret_cast<uint16_t *>(col);
+    uint16_t *cbh = ca + 1;
+    // the query's 2-bit words live in registers for the whole item
+    uint32_t QA[16], QB[16];
+#pragma unroll
+    for (int b = 0; b < 16; ++b) {
+        QA[b] = b * 16 < A.qlen ? d.seq[A.qw0 + b] : 0u;
+        QB[b] = b * 16 < B.qlen ? d.seq[B.qw0 + b] : 0u;
+    }
+    const int qmax = uniform_max(max(A.alive ? A.qlen : 0, B.alive ? B.qlen : 0));
     for (int j = 0; j <= qmax; ++j) {                          // first row, :68-77
         const int ha = (A.alive && j <= A.qlen) ? (j == 0 ? A.h0 : max(0, A.h0 - p.o_ins - p.e_ins * j)) : 0;
         const int hb = (B.alive && j <= B.qlen) ? (j == 0 ? B.h0 : max(0, B.h0 - p.o_ins - p.e_ins * j)) : 0;
         col[j * 64] = (uint32_t)ha | ((uint32_t)hb << 16);
     }
-    const us2 A2 = splat(p.match_sc), A1 = splat(p.match_sc + 1);
     const ss2 Ssc = ss2{(short)p.match_sc, (short)p.match_sc};
     const ss2 nAB = ss2{(short)(p.mis_sc - p.match_sc), (short)(p.mis_sc - p.match_sc)};
     const us2 eD = splat(p.e_del), eI = splat(p.e_ins), oeD = splat(p.oe_del), oeI = splat(p.oe_ins);
     const uint32_t LOW = 0x00FF00FFu;
+    const us2 ONE = u2(opaque(0x00010001u));
+    const us2 C256 = u2(opaque(0x01000100u));   // H * 256 + k: one packed mad
+    const int BIG = 0x7fffffff;
 
     for (int i = 0;; ++i) {
-        const int h1a = pk_row_start(A, d, p, i);
-        const int h1b = pk_row_start(B, d, p, i);
+        const int h1a = pk_row_start(A, d, p, i, ca);
+        const int h1b = pk_row_start(B, d, p, i, cbh);
         if (__ballot(A.alive || B.alive) == 0) break;
         us2 h1 = us2{(unsigned short)h1a, (unsigned short)h1b};
         us2 f = splat(0), key = splat(0);
-        const int BIG = 0x7fffffff;
-        const int jlo = wave_min(min(A.alive ? A.beg : BIG, B.alive ? B.beg : BIG));
-        const int jhi = wave_max(max(A.alive ? A.end : 0, B.alive ? B.end : 0));
-        // [mlo, mhi): columns inside the band of every live job of the wave
-        const int mlo = wave_max(max(A.alive ? A.beg : 0, B.alive ? B.beg : 0));
-        const int mhi = wave_min(min(A.alive ? A.end : BIG, B.alive ? B.end : BIG));
+        const int jlo = uniform_min(min(A.alive ? A.beg : BIG, B.alive ? B.beg : BIG));
+        const int jhi = uniform_max(max(A.alive ? A.end : 0, B.alive ? B.end : 0));
+        const int mhi = uniform_min(min(A.alive ? A.end : BIG, B.alive ? B.end : BIG));
+        // per-half band end (columns >= it commit nothing); 0 for a finished job
+        const us2 L = us2{(unsigned short)(A.alive ? A.end : 0), (unsigned short)(B.alive ? B.end : 0)};
 
-        // one packed cell; ab = per-half mismatch flags (0 or 1 in each half)
-        auto cell = [&](uint32_t &word, uint32_t ab, uint32_t jj) {
-            const us2 M = u2(word & LOW);
-            const us2 E = u2(__builtin_amdgcn_perm(0u, word, 0x0C030C01u));   // bytes 1, 3
-            const ss2 s = __builtin_bit_cast(ss2, ab) * nAB + Ssc;   // A or -B per half
-            us2 diag = __builtin_bit_cast(us2, __builtin_elementwise_max(
-                           __builtin_bit_cast(ss2, M) + s, ss2{0, 0}));   // max(M + s, 0)
-            diag = vmin(diag, M * A1);                               // M == 0 -> 0 (:117)
-            const us2 H = vmax(vmax(diag, E), f);
-            key = vmax(key, (H << 8) | u2(jj * 0x00010001u));
-            const us2 ne = vmax(vsubs(E, eD), vsubs(diag, oeD));
-            f = vmax(vsubs(f, eI), vsubs(diag, oeI));
-            // bytes (h1.lo, ne.lo, h1.hi, ne.hi): a finished job's drifting
-            // values can never spill into the other half
-            word = __builtin_amdgcn_perm(w32(ne), w32(h1), 0x06020400u);
-            h1 = H;
-        };
-        (void)A2;
+        // A block runs in three passes so that the packed ops of one cell need
+        // not wait on each other: (1) everything a cell derives from its own
+        // state word and base pair -- diag (:117-119), max(diag, E) and the new
+        // E (:128-129); (2) the running F and h1 (:121-123, :130-131);
+        // (3) the row-maximum candidates H * 256 + k (:124-125).
         if (jlo < jhi) {
-            // query words of the current and the next 16-column block
-            int b = jlo >> 4;
-            const int bend = (jhi - 1) >> 4;
-            uint32_t qa = b * 16 < A.qlen ? d.seq[A.qw0 + b] : 0u;
-            uint32_t qb = b * 16 < B.qlen ? d.seq[B.qw0 + b] : 0u;
-            for (; b <= bend; ++b) {
-                const uint32_t na = (b + 1) * 16 < A.qlen ? d.seq[A.qw0 + b + 1] : 0u;
-                const uint32_t nb = (b + 1) * 16 < B.qlen ? d.seq[B.qw0 + b + 1] : 0u;
-                const uint32_t mm = mismatch16(qa, A.tpat) | (mismatch16(qb, B.tpat) << 16);
+            for (int b = jlo >> 4; b <= (jhi - 1) >> 4; ++b) {
+                const uint32_t mm = mismatch16(QA[b], A.tpat) | (mismatch16(QB[b], B.tpat) << 16);
                 uint32_t *cb = col + b * 16 * 64;
                 const int c0 = b * 16;
-                if (c0 >= mlo && c0 + 16 <= mhi) {
-                    // whole block inside every live band: no masking
-                    uint32_t wv[16];
+                uint32_t wv[16];
 #pragma unroll
-                    for (int k = 0; k < 16; ++k) wv[k] = cb[k * 64];
+                for (int k = 0; k < 16; ++k) wv[k] = cb[k * 64];
+                // pass 1, stage by stage over the 16 cells (independent chains)
+                us2 dgE[16], tf[16], ne[16], Hs[16], M[16], E[16], s[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const uint32_t ab = (mm >> k) & 0x00010001u;      // per-half mismatch flag
+                    s[k] = __builtin_bit_cast(us2, __builtin_bit_cast(ss2, ab) * nAB + Ssc);
+                    M[k] = u2(wv[k] & LOW);
+                    E[k] = u2(__builtin_amdgcn_perm(0u, wv[k], 0x0C030C01u));   // bytes 1, 3
+                }
+#pragma unroll
+                for (int k = 0; k < 16; ++k) s[k] = s[k] * vmin(M[k], ONE) + M[k];   // M ? M + s : 0
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const us2 dg = __builtin_bit_cast(
+                        us2, __builtin_elementwise_max(__builtin_bit_cast(ss2, s[k]), ss2{0, 0}));
+                    dgE[k] = vmax(dg, E[k]);
+                    const us2 t = vsubs(dg, oeD);
+                    ne[k] = vmax(vsubs(E[k], eD), t);
+                    tf[k] = SYM ? t : vsubs(dg, oeI);
+                }
+                if (c0 + 16 <= mhi) {
+                    // left of every live band end: no masking
 #pragma unroll
                     for (int k = 0; k < 16; ++k) {
-                        cell(wv[k], (mm >> k) & 0x00010001u, (uint32_t)(c0 + k));
-                        cb[k * 64] = wv[k];
+                        const us2 H = vmax(dgE[k], f);
+                        f = vmax(vsubs(f, eI), tf[k]);
+                        wv[k] = __builtin_amdgcn_perm(w32(ne[k]), w32(h1), 0x06020400u);
+                        h1 = H;
+                        Hs[k] = H * C256 + splat(k);
                     }
                 } else {
-                    // each half commits only inside its own band
-                    const int k0 = max(jlo, c0) - c0, k1 = min(jhi, c0 + 16) - c0;
-                    for (int k = k0; k < k1; ++k) {
-                        const int j = c0 + k;
-                        const bool ia = A.alive && (unsigned)(j - A.beg) < (unsigned)(A.end - A.beg);
-                        const bool ib = B.alive && (unsigned)(j - B.beg) < (unsigned)(B.end - B.beg);
-                        const uint32_t old = cb[k * 64];
-                        uint32_t word = old;
-                        const us2 f0 = f, k0v = key, h0v = h1;
-                        cell(word, (mm >> k) & 0x00010001u, (uint32_t)j);
-                        const uint32_t hm = (ia ? 0x0000FFFFu : 0u) | (ib ? 0xFFFF0000u : 0u);
-                        cb[k * 64] = (word & hm) | (old & ~hm);
-                        f = u2((w32(f) & hm) | (w32(f0) & ~hm));
-                        key = u2((w32(key) & hm) | (w32(k0v) & ~hm));
-                        h1 = u2((w32(h1) & hm) | (w32(h0v) & ~hm));
+                    // commit only left of each half's band end
+                    const us2 Lb = vsubs(L, splat(c0));
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) {
+                        uint32_t hm = w32(__builtin_bit_cast(
+                            us2, __builtin_bit_cast(ss2, splat(k) - Lb) >> ss2{15, 15}));
+                        asm volatile("" : "+v"(hm));   // keep the selects bitwise
+                        const us2 H = vmax(dgE[k], f);
+                        f = vmax(vsubs(f, eI), tf[k]);
+                        const uint32_t nw = __builtin_amdgcn_perm(w32(ne[k]), w32(h1), 0x06020400u);
+                        wv[k] = (nw & hm) | (wv[k] & ~hm);
+                        h1 = u2((w32(H) & hm) | (w32(h1) & ~hm));
+                        Hs[k] = u2(w32(H * C256 + splat(k)) & hm);
                     }
                 }
-                qa = na;
-                qb = nb;
+#pragma unroll
+                for (int k = 0; k < 16; ++k) cb[k * 64] = wv[k];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) Hs[k] = vmax(Hs[k], Hs[k + 8]);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) Hs[k] = vmax(Hs[k], Hs[k + 4]);
+                const us2 kb = vmax(vmax(Hs[0], Hs[2]), vmax(Hs[1], Hs[3]));
+                key = vmax(key, kb + splat(c0));
             }
         }
 
         const uint32_t kw = w32(key), hw = w32(h1);
-        pk_row_end(A, p, i, (int)(hw & 0xFFFF), (int)(kw & 0xFFFF), col, 0);
-        pk_row_end(B, p, i, (int)(hw >> 16), (int)(kw >> 16), col, 1);
+        const int fa = (int)(hw & 0xFFFF), fb = (int)(hw >> 16);
+        pk_row_end(A, p, i, fa, (int)(kw & 0xFFFF), ca);
+        pk_row_end(B, p, i, fb, (int)(kw >> 16), cbh);
+        pk_narrow(A, B, fa, fb, ca, cbh);
     }
     if (d.cells) {
         unsigned long long c = A.ncells + B.ncells;
@@ -799,33 +900,36 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const uint32_t 
 }
 
 // Persistent kernel of the packed form.  Items hold up to 128 jobs; an item
-// whose jobs include an N base runs as two generic 64-job halves instead.
+// whose jobs include an N base runs as two generic 64-job halves instead.  The
+// block's LDS is exactly the column state (no static shared memory), so four
+// waves of 150-bp queries fill a 160 KiB CU.
+template <bool SYM>
 __global__ __launch_bounds__(64) void ext_persistent_pk_kernel(ExtDev d, ExtParams p,
                                                                const uint32_t *order,
                                                                const WorkItem *items, int n_items,
                                                                uint32_t *counter, int cols)
 {
-    const int8_t *smat = load_smat(p);
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    uint32_t *col = reinterpret_cast<uint32_t *>(lds_raw + 32) + threadIdx.x;
-    __shared__ uint32_t idx[128];
-    __shared__ int item_sh;
+    uint32_t *col = reinterpret_cast<uint32_t *>(lds_raw) + threadIdx.x;
+    const int lane = threadIdx.x;
     for (;;) {
-        if (threadIdx.x == 0) item_sh = (int)atomicAdd(counter, 1u);
-        __syncthreads();
-        const int item = item_sh;
+        int item = 0;
+        if (lane == 0) item = (int)atomicAdd(counter, 1u);
+        item = __builtin_amdgcn_readfirstlane(item);
         if (item >= n_items) break;
         const WorkItem wi = items[item];
-        for (int k = threadIdx.x; k < (int)wi.count; k += 64) idx[k] = order[wi.start + k];
-        __syncthreads();
-        const int lane = threadIdx.x;
+        const uint32_t *idx = order + wi.start;
         bool has_n = false;
         for (int k = lane; k < (int)wi.count; k += 64)
             has_n = has_n || d.qn_word[idx[k]] != kNoN || d.tn_word[idx[k]] != kNoN;
         if (__ballot(has_n) == 0) {
-            ext_wave_pk(d, p, idx, (int)wi.count, col);
+            ext_wave_pk<SYM>(d, p, idx, (int)wi.count, col);
         } else {
-            uint16_t *c16 = reinterpret_cast<uint16_t *>(lds_raw + 32) + threadIdx.x;
+            // generic 8-bit form: u16 columns first, the score matrix after them
+            int8_t *smat = reinterpret_cast<int8_t *>(lds_raw) + (size_t)cols * 128;
+            if (lane < 32) smat[lane] = p.mat[lane];
+            __syncthreads();
+            uint16_t *c16 = reinterpret_cast<uint16_t *>(lds_raw) + lane;
             ext_wave_generic<TIER_U8>(d, p, idx, min(64, (int)wi.count), c16, smat);
             if (wi.count > 64)
                 ext_wave_generic<TIER_U8>(d, p, idx + 64, (int)wi.count - 64, c16, smat);
@@ -834,15 +938,21 @@ __global__ __launch_bounds__(64) void ext_persistent_pk_kernel(ExtDev d, ExtPara
     }
 }
 
-size_t ext_pk_lds_bytes(int cols) { return 32 + (size_t)cols * 64 * 4; }
+size_t ext_pk_lds_bytes(int cols) { return (size_t)((cols + 15) / 16) * 16 * 64 * 4; }
 
 hipError_t launch_ext_persistent_pk(const ExtDev &d, const ExtParams &p, const uint32_t *order,
                                     const WorkItem *items, int n_items, uint32_t *counter,
                                     int cols, int grid, hipStream_t s)
 {
     if (n_items <= 0) return hipSuccess;
-    hipLaunchKernelGGL(ext_persistent_pk_kernel, dim3(grid), dim3(64), ext_pk_lds_bytes(cols), s, d,
-                       p, order, items, n_items, counter, cols);
+    if (cols < 1 || cols > 256) return hipErrorInvalidValue;
+    const bool sym = p.o_del == p.o_ins && p.e_del == p.e_ins;
+    if (sym)
+        hipLaunchKernelGGL(ext_persistent_pk_kernel<true>, dim3(grid), dim3(64), ext_pk_lds_bytes(cols),
+                           s, d, p, order, items, n_items, counter, cols);
+    else
+        hipLaunchKernelGGL(ext_persistent_pk_kernel<false>, dim3(grid), dim3(64), ext_pk_lds_bytes(cols),
+                           s, d, p, order, items, n_items, counter, cols);
     return hipGetLastError();
 }
 
