@@ -11,6 +11,7 @@ one-launch form is timed beside it.  Cells are counted exactly as the CPU
 loop counts them (one per inner iteration of BandedSWScalar.swift:112).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+       python bench.py --profile-leg ext|copy   (one leg only, for rocprofv3 runs)
 """
 import argparse
 import json
@@ -177,6 +178,51 @@ def bench_serving(eng, args):
             "saturation_reads_per_s": round(float(2 * sat), 1), "loads": loads}
 
 
+def profile_leg(args):
+    """Runs one leg alone so a profiler sees only its kernels.
+
+    ext:  the headline job set, batched form, `warmup` + `steps` passes.
+    copy: a 256 MiB device-to-device copy, `steps` times -- the known-size
+          reference that scales the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    from swiftbwa_amd import native, workload as wl
+    from swiftbwa_amd.metalsw import HipSWEngine, ScoringParameters, _ext_arrays
+    eng = HipSWEngine(0)
+    lib = eng.lib
+    if args.profile_leg == "ext":
+        js = wl.generate(wl.EXT_UNIFORM, 2, args.n)
+        a = _ext_arrays(js)
+        sc = ScoringParameters().to_c()
+        st = lib.hipsw_ext_upload(eng.handle, js["n"], native.ptr(a["queries"]),
+                                  native.ptr(a["q_off"]), native.ptr(a["q_len"]),
+                                  native.ptr(a["targets"]), native.ptr(a["t_off"]),
+                                  native.ptr(a["t_len"]), native.ptr(a["h0"]), native.ptr(a["w"]),
+                                  native.ptr(a["end_bonus"]), C.byref(sc))
+        if not st:
+            raise RuntimeError(eng.last_error())
+        s = native.HipswStats()
+        for k in range(args.warmup + args.steps):
+            if lib.hipsw_ext_run(eng.handle, st, native.MODE_BATCHED, C.byref(s)):
+                raise RuntimeError(eng.last_error())
+            log("ext batched pass %d: kernel %.3f ms, %d cells" % (k, s.kernel_ms, s.cells))
+        lib.hipsw_ext_free(eng.handle, st)
+    else:
+        hip = C.CDLL("libamdhip64.so")
+        nb = 256 << 20
+        src, dst = C.c_void_p(), C.c_void_p()
+        if hip.hipMalloc(C.byref(src), C.c_size_t(nb)) or hip.hipMalloc(C.byref(dst), C.c_size_t(nb)):
+            raise RuntimeError("hipMalloc failed")
+        hip.hipMemset(src, 1, C.c_size_t(nb))
+        for k in range(args.steps):
+            if hip.hipMemcpy(dst, src, C.c_size_t(nb), 3):     # device to device
+                raise RuntimeError("hipMemcpy failed")
+        hip.hipDeviceSynchronize()
+        log("copy: %d x %d bytes" % (args.steps, nb))
+        hip.hipFree(src)
+        hip.hipFree(dst)
+    eng.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,11 +238,16 @@ def main():
     ap.add_argument("--serve-budget-gb", type=int, default=64)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--mixed-n", type=int, default=4000000)
+    ap.add_argument("--profile-leg", choices=["ext", "copy"], default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (each job set runs on one "
-              "device; see the sharding note in DESIGN.md). Use --gpus 1.", file=sys.stderr)
+              "device; a multi-GPU run would shard the job set, one engine per GPU). "
+              "Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.profile_leg:
+        profile_leg(args)
+        return
 
     import ctypes as C
     from swiftbwa_amd import native, workload as wl

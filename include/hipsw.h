@@ -87,7 +87,9 @@ int           hipsw_device_info(const hipsw_engine *eng, char *name, int name_ca
  * int32 in SWResult order {score, queryEnd, targetEnd, globalTargetEnd,
  * globalScore, maxOff} (SWResult.swift:2-25, BandedSWDispatcher.swift:92-99).
  * Scores are exact in every case: the engine picks an 8-, 16- or 32-bit score
- * tier per task from the bound h0 + qlen * max(mat). */
+ * tier per task from the bound h0 + qlen * max(mat).  Every w must be >= 0
+ * (HIPSW_E_INVALID otherwise): a negative band makes the scalar loop index its
+ * row below column 0 (BandedSWScalar.swift:97-99, :145). */
 int hipsw_banded_extend(hipsw_engine *eng, int32_t n,
                         const uint8_t *queries, const uint32_t *q_off, const uint16_t *q_len,
                         const uint8_t *targets, const uint32_t *t_off, const uint16_t *t_len,

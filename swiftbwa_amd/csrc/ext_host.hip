@@ -139,6 +139,13 @@ extern "C" hipsw_ext_set *hipsw_ext_upload(hipsw_engine *eng, int32_t n,
         eng->last_error = "hipsw_ext_upload: gap extension penalties must be > 0";
         return nullptr;
     }
+    for (int k = 0; k < n; ++k) {
+        // a negative band puts the row's end below column 0 (BandedSWScalar.swift:97-99)
+        if (w[k] < 0) {
+            eng->last_error = "hipsw_ext_upload: band width w must be >= 0";
+            return nullptr;
+        }
+    }
     hipSetDevice(eng->device);
     auto *set = new hipsw_ext_set();
     set->n = n;

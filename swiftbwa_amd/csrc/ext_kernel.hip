@@ -629,9 +629,8 @@ __device__ __forceinline__ uint32_t mismatch16(uint32_t qw, uint32_t tpat)
 // Per-job row-level state of one half.
 struct PkJob {
     bool valid, nonempty, alive;
-    int job, qlen, tlen, h0, w, beg, end, maxv, max_i, max_j, gsc, gtle, max_off;
-    uint32_t qw0, tw0, tw, twn, tpat;
-    uint64_t ncells;
+    int job, qlen, tlen, h0, w, beg, end, maxv, max_i, max_j, gsc, gtle, max_off, tnw;
+    uint32_t qw0, tw0, tw, twn, tpat, ncells;
 };
 
 __device__ __forceinline__ void pk_setup(PkJob &J, const ExtDev &d, const ExtParams &p,
@@ -658,95 +657,115 @@ __device__ __forceinline__ void pk_setup(PkJob &J, const ExtDev &d, const ExtPar
     J.beg = 0; J.end = J.qlen;                                 // :79-87
     J.maxv = J.h0; J.max_i = -1; J.max_j = -1;
     J.gsc = -1; J.gtle = -1; J.max_off = 0;
-    // target words of rows 0-15 and 16-31 (the second is a prefetch)
-    J.tw = J.alive ? d.seq[J.tw0] : 0u;
-    J.twn = (J.alive && J.tlen > 16) ? d.seq[J.tw0 + 1] : 0u;
+    // target words of rows 0-15 and 16-31 (the second is a prefetch); word
+    // indexes stop at the job's last word + 1, inside the padded arena
+    J.tnw = (J.tlen + 15) >> 4;
+    J.tw = d.seq[J.tw0];
+    J.twn = d.seq[J.tw0 + min(1, J.tnw)];
     J.tpat = 0; J.ncells = 0;
 }
 
-// Row start of one half (:95-107); returns h1.  `c16` is this half's column
-// state (u16 (h, e) of column j at c16[j * 128]).
-__device__ __forceinline__ int pk_row_start(PkJob &J, const ExtDev &d, const ExtParams &p, int i,
-                                            uint16_t *c16)
+// Row start of one half (:92-107); returns h1.  `c16` is this half's column
+// state (u16 (h, e) of column j at c16[j * 128]).  The target word of the row
+// is current (the caller refreshes tw / twn every 16 rows).
+__device__ __forceinline__ int pk_row_start(PkJob &J, const ExtParams &p, int i, uint16_t *c16)
 {
-    if (J.alive && i >= J.tlen) J.alive = false;
-    if (!J.alive) return 0;
-    if (i > 0 && (i & 15) == 0) {                              // next 16 target bases
-        J.tw = J.twn;
-        J.twn = i + 16 < J.tlen ? d.seq[J.tw0 + (i >> 4) + 1] : 0u;
-    }
-    const uint32_t t = (J.tw >> ((i & 15) * 2)) & 3u;
-    J.tpat = t * 0x55555555u;
+    J.alive = J.alive && i < J.tlen;
+    J.tpat = ((J.tw >> ((i & 15) * 2)) & 3u) * 0x55555555u;
     const int wb = i - J.w;
-    if (J.beg < wb) {                                          // beg = i - w drops column beg
-        for (int j = J.beg; j < wb; ++j) c16[j * 128] = 0;
+    if (J.alive && J.beg < wb) {
+        // beg = i - w (:96) drops column beg: with w >= 0, beg >= i - 1 - w
+        // after the previous row, so exactly one column per row; clear it
+        c16[J.beg * 128] = 0;
         J.beg = wb;
     }
-    J.end = min(J.end, i + J.w + 1);
-    J.end = min(J.end, J.qlen);
-    if (J.end > J.beg) J.ncells += (uint64_t)(J.end - J.beg);
-    return J.beg == 0 ? max(0, J.h0 - (p.o_del + p.e_del * (i + 1))) : 0;
+    J.end = min(min(J.end, i + J.w + 1), J.qlen);             // :97-98
+    J.ncells += (J.alive && J.end > J.beg) ? (uint32_t)(J.end - J.beg) : 0u;
+    return (J.alive && J.beg == 0) ? max(0, J.h0 - (p.o_del + p.e_del * (i + 1))) : 0;
 }
 
 // Row end of one half (:145-172): band-end entry, global-score column, row
-// maximum and z-drop.
+// maximum and z-drop.  A finished job's half is never read again, so its
+// band-end store needs no guard (end <= qlen keeps it in bounds).
 __device__ __forceinline__ void pk_row_end(PkJob &J, const ExtParams &p, int i, int h1, int key,
                                            uint16_t *c16)
 {
-    if (!J.alive) return;
-    const int row_max = key >> 8, row_max_j = key & 0xFF;
     c16[J.end * 128] = (uint16_t)h1;                           // h[end] = h1, e[end] = 0
-    if (J.end == J.qlen) {                                     // :149-152
-        J.gtle = J.gsc > h1 ? J.gtle : i;
-        J.gsc = max(J.gsc, h1);
-    }
-    if (row_max == 0) {                                        // :155
-        J.alive = false;
-    } else if (row_max > J.maxv) {                             // :158-163
-        J.maxv = row_max;
-        J.max_i = i;
-        J.max_j = row_max_j;
-        J.max_off = max(J.max_off, abs(row_max_j - i));
-    } else if (p.zdrop > 0) {                                  // :164-172
-        const int di = i - J.max_i, dj = row_max_j - J.max_j;
-        const int drop = di > dj ? J.maxv - row_max - (di - dj) * p.e_del
-                                 : J.maxv - row_max - (dj - di) * p.e_ins;
-        if (drop > p.zdrop) J.alive = false;
-    }
+    const int rm = key >> 8, rj = key & 0xFF;
+    const bool at_q = J.alive && J.end == J.qlen;              // :149-152
+    J.gtle = (at_q && !(J.gsc > h1)) ? i : J.gtle;
+    J.gsc = at_q ? max(J.gsc, h1) : J.gsc;
+    const bool up = J.alive && rm > J.maxv;                    // :158-163
+    const int di = i - J.max_i, dj = rj - J.max_j;             // :164-172
+    const int drop = di > dj ? J.maxv - rm - (di - dj) * p.e_del : J.maxv - rm - (dj - di) * p.e_ins;
+    const bool zd = p.zdrop > 0 && drop > p.zdrop;
+    J.max_off = up ? max(J.max_off, abs(rj - i)) : J.max_off;
+    J.max_i = up ? i : J.max_i;
+    J.max_j = up ? rj : J.max_j;
+    J.maxv = up ? rm : J.maxv;
+    J.alive = J.alive && rm != 0 && (up || !zd);               // :155, :171
 }
 
-// Band narrowing of both halves (:175-180), one LDS round trip per step.
+// Band narrowing of both halves (:175-180).  Each trip reads four columns per
+// half at once (one LDS round trip); more trips only while all four are zero.
 __device__ __forceinline__ void pk_narrow(PkJob &A, PkJob &B, int h1a, int h1b, uint16_t *ca,
                                           uint16_t *cb)
 {
+    // first column in [beg, end) whose (h, e) is non-zero, else end
     int ja = A.beg, jb = B.beg;
     bool ga = A.alive && ja < A.end, gb = B.alive && jb < B.end;
     while (ga || gb) {
-        const uint32_t va = ga ? ca[ja * 128] : 1u;
-        const uint32_t vb = gb ? cb[jb * 128] : 1u;
-        if (ga) {
-            if (va == 0) { ++ja; ga = ja < A.end; } else ga = false;
+        const int ea = A.end, eb = B.end;
+        uint32_t va[4], vb[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            va[k] = ca[min(ja + k, ea) * 128];
+            vb[k] = cb[min(jb + k, eb) * 128];
         }
-        if (gb) {
-            if (vb == 0) { ++jb; gb = jb < B.end; } else gb = false;
+        bool za = ga && va[0] == 0, zb = gb && vb[0] == 0;
+        int na = za, nb = zb;
+#pragma unroll
+        for (int k = 1; k < 4; ++k) {
+            za = za && ja + k < ea && va[k] == 0;
+            zb = zb && jb + k < eb && vb[k] == 0;
+            na += za;
+            nb += zb;
         }
+        ja += na;
+        jb += nb;
+        ga = za && ja < ea;
+        gb = zb && jb < eb;
     }
     if (A.alive) A.beg = ja;
     if (B.alive) B.beg = jb;
-    // column end holds (h1, 0): only a zero h1 needs the backward scan
+    // last column in [beg, end] whose (h, e) is non-zero; column end holds
+    // (h1, 0), so only a zero h1 starts the backward scan
     int ka = A.end, kb = B.end;
-    bool ea = A.alive && ka >= A.beg && h1a == 0, eb = B.alive && kb >= B.beg && h1b == 0;
-    if (ea) { --ka; ea = ka >= A.beg; }
-    if (eb) { --kb; eb = kb >= B.beg; }
-    while (ea || eb) {
-        const uint32_t va = ea ? ca[ka * 128] : 1u;
-        const uint32_t vb = eb ? cb[kb * 128] : 1u;
-        if (ea) {
-            if (va == 0) { --ka; ea = ka >= A.beg; } else ea = false;
+    bool sa = A.alive && ka >= A.beg && h1a == 0, sb = B.alive && kb >= B.beg && h1b == 0;
+    ka -= sa;
+    kb -= sb;
+    sa = sa && ka >= A.beg;
+    sb = sb && kb >= B.beg;
+    while (sa || sb) {
+        uint32_t va[4], vb[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            va[k] = ca[max(ka - k, 0) * 128];
+            vb[k] = cb[max(kb - k, 0) * 128];
         }
-        if (eb) {
-            if (vb == 0) { --kb; eb = kb >= B.beg; } else eb = false;
+        bool za = sa && va[0] == 0, zb = sb && vb[0] == 0;
+        int na = za, nb = zb;
+#pragma unroll
+        for (int k = 1; k < 4; ++k) {
+            za = za && ka - k >= A.beg && va[k] == 0;
+            zb = zb && kb - k >= B.beg && vb[k] == 0;
+            na += za;
+            nb += zb;
         }
+        ka -= na;
+        kb -= nb;
+        sa = za && ka >= A.beg;
+        sb = zb && kb >= B.beg;
     }
     if (A.alive) A.end = min(ka + 2, A.qlen);
     if (B.alive) B.end = min(kb + 2, B.qlen);
@@ -802,8 +821,14 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const uint32_t 
     const int BIG = 0x7fffffff;
 
     for (int i = 0;; ++i) {
-        const int h1a = pk_row_start(A, d, p, i, ca);
-        const int h1b = pk_row_start(B, d, p, i, cbh);
+        if (i > 0 && (i & 15) == 0) {                          // next 16 target bases
+            A.tw = A.twn;
+            B.tw = B.twn;
+            A.twn = d.seq[A.tw0 + min((i >> 4) + 1, A.tnw)];
+            B.twn = d.seq[B.tw0 + min((i >> 4) + 1, B.tnw)];
+        }
+        const int h1a = pk_row_start(A, p, i, ca);
+        const int h1b = pk_row_start(B, p, i, cbh);
         if (__ballot(A.alive || B.alive) == 0) break;
         us2 h1 = us2{(unsigned short)h1a, (unsigned short)h1b};
         us2 f = splat(0), key = splat(0);
@@ -813,73 +838,87 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const uint32_t 
         // per-half band end (columns >= it commit nothing); 0 for a finished job
         const us2 L = us2{(unsigned short)(A.alive ? A.end : 0), (unsigned short)(B.alive ? B.end : 0)};
 
-        // A block runs in three passes so that the packed ops of one cell need
-        // not wait on each other: (1) everything a cell derives from its own
-        // state word and base pair -- diag (:117-119), max(diag, E) and the new
-        // E (:128-129); (2) the running F and h1 (:121-123, :130-131);
-        // (3) the row-maximum candidates H * 256 + k (:124-125).
+        // One 16-column block, in three passes so that the packed ops of one
+        // cell need not wait on each other: (1) everything a cell derives from
+        // its own state word and base pair -- diag (:117-119), max(diag, E)
+        // and the new E (:128-129); (2) the running F and h1 (:121-123,
+        // :130-131); (3) the row-maximum candidates H * 256 + k (:124-125).
+        auto block = [&](uint32_t (&wv)[16], int b) {
+            const uint32_t mm = mismatch16(QA[b], A.tpat) | (mismatch16(QB[b], B.tpat) << 16);
+            const int c0 = b * 16;
+            us2 dgE[16], tf[16], ne[16], Hs[16], M[16], E[16], s[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const uint32_t ab = (mm >> k) & 0x00010001u;          // per-half mismatch flag
+                s[k] = __builtin_bit_cast(us2, __builtin_bit_cast(ss2, ab) * nAB + Ssc);
+                M[k] = u2(wv[k] & LOW);
+                E[k] = u2(__builtin_amdgcn_perm(0u, wv[k], 0x0C030C01u));   // bytes 1, 3
+            }
+#pragma unroll
+            for (int k = 0; k < 16; ++k) s[k] = s[k] * vmin(M[k], ONE) + M[k];   // M ? M + s : 0
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const us2 dg = __builtin_bit_cast(
+                    us2, __builtin_elementwise_max(__builtin_bit_cast(ss2, s[k]), ss2{0, 0}));
+                dgE[k] = vmax(dg, E[k]);
+                const us2 t = vsubs(dg, oeD);
+                ne[k] = vmax(vsubs(E[k], eD), t);
+                tf[k] = SYM ? t : vsubs(dg, oeI);
+            }
+            if (c0 + 16 <= mhi) {
+                // left of every live band end: no masking
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const us2 H = vmax(dgE[k], f);
+                    f = vmax(vsubs(f, eI), tf[k]);
+                    wv[k] = __builtin_amdgcn_perm(w32(ne[k]), w32(h1), 0x06020400u);
+                    h1 = H;
+                    Hs[k] = H * C256 + splat(k);
+                }
+            } else {
+                // commit only left of each half's band end
+                const us2 Lb = vsubs(L, splat(c0));
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    uint32_t hm = w32(__builtin_bit_cast(
+                        us2, __builtin_bit_cast(ss2, splat(k) - Lb) >> ss2{15, 15}));
+                    asm volatile("" : "+v"(hm));   // keep the selects bitwise
+                    const us2 H = vmax(dgE[k], f);
+                    f = vmax(vsubs(f, eI), tf[k]);
+                    const uint32_t nw = __builtin_amdgcn_perm(w32(ne[k]), w32(h1), 0x06020400u);
+                    wv[k] = (nw & hm) | (wv[k] & ~hm);
+                    h1 = u2((w32(H) & hm) | (w32(h1) & ~hm));
+                    Hs[k] = u2(w32(H * C256 + splat(k)) & hm);
+                }
+            }
+            uint32_t *cb = col + b * 16 * 64;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) cb[k * 64] = wv[k];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) Hs[k] = vmax(Hs[k], Hs[k + 8]);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) Hs[k] = vmax(Hs[k], Hs[k + 4]);
+            const us2 kb = vmax(vmax(Hs[0], Hs[2]), vmax(Hs[1], Hs[3]));
+            key = vmax(key, kb + splat(c0));
+        };
+        auto load = [&](uint32_t (&wv)[16], int b) {
+            const uint32_t *cb = col + b * 16 * 64;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) wv[k] = cb[k * 64];
+        };
         if (jlo < jhi) {
-            for (int b = jlo >> 4; b <= (jhi - 1) >> 4; ++b) {
-                const uint32_t mm = mismatch16(QA[b], A.tpat) | (mismatch16(QB[b], B.tpat) << 16);
-                uint32_t *cb = col + b * 16 * 64;
-                const int c0 = b * 16;
-                uint32_t wv[16];
-#pragma unroll
-                for (int k = 0; k < 16; ++k) wv[k] = cb[k * 64];
-                // pass 1, stage by stage over the 16 cells (independent chains)
-                us2 dgE[16], tf[16], ne[16], Hs[16], M[16], E[16], s[16];
-#pragma unroll
-                for (int k = 0; k < 16; ++k) {
-                    const uint32_t ab = (mm >> k) & 0x00010001u;      // per-half mismatch flag
-                    s[k] = __builtin_bit_cast(us2, __builtin_bit_cast(ss2, ab) * nAB + Ssc);
-                    M[k] = u2(wv[k] & LOW);
-                    E[k] = u2(__builtin_amdgcn_perm(0u, wv[k], 0x0C030C01u));   // bytes 1, 3
-                }
-#pragma unroll
-                for (int k = 0; k < 16; ++k) s[k] = s[k] * vmin(M[k], ONE) + M[k];   // M ? M + s : 0
-#pragma unroll
-                for (int k = 0; k < 16; ++k) {
-                    const us2 dg = __builtin_bit_cast(
-                        us2, __builtin_elementwise_max(__builtin_bit_cast(ss2, s[k]), ss2{0, 0}));
-                    dgE[k] = vmax(dg, E[k]);
-                    const us2 t = vsubs(dg, oeD);
-                    ne[k] = vmax(vsubs(E[k], eD), t);
-                    tf[k] = SYM ? t : vsubs(dg, oeI);
-                }
-                if (c0 + 16 <= mhi) {
-                    // left of every live band end: no masking
-#pragma unroll
-                    for (int k = 0; k < 16; ++k) {
-                        const us2 H = vmax(dgE[k], f);
-                        f = vmax(vsubs(f, eI), tf[k]);
-                        wv[k] = __builtin_amdgcn_perm(w32(ne[k]), w32(h1), 0x06020400u);
-                        h1 = H;
-                        Hs[k] = H * C256 + splat(k);
-                    }
-                } else {
-                    // commit only left of each half's band end
-                    const us2 Lb = vsubs(L, splat(c0));
-#pragma unroll
-                    for (int k = 0; k < 16; ++k) {
-                        uint32_t hm = w32(__builtin_bit_cast(
-                            us2, __builtin_bit_cast(ss2, splat(k) - Lb) >> ss2{15, 15}));
-                        asm volatile("" : "+v"(hm));   // keep the selects bitwise
-                        const us2 H = vmax(dgE[k], f);
-                        f = vmax(vsubs(f, eI), tf[k]);
-                        const uint32_t nw = __builtin_amdgcn_perm(w32(ne[k]), w32(h1), 0x06020400u);
-                        wv[k] = (nw & hm) | (wv[k] & ~hm);
-                        h1 = u2((w32(H) & hm) | (w32(h1) & ~hm));
-                        Hs[k] = u2(w32(H * C256 + splat(k)) & hm);
-                    }
-                }
-#pragma unroll
-                for (int k = 0; k < 16; ++k) cb[k * 64] = wv[k];
-#pragma unroll
-                for (int k = 0; k < 8; ++k) Hs[k] = vmax(Hs[k], Hs[k + 8]);
-#pragma unroll
-                for (int k = 0; k < 4; ++k) Hs[k] = vmax(Hs[k], Hs[k + 4]);
-                const us2 kb = vmax(vmax(Hs[0], Hs[2]), vmax(Hs[1], Hs[3]));
-                key = vmax(key, kb + splat(c0));
+            // two word sets in turn: the next block's reads are in flight
+            // while the current block computes
+            const int bb = jlo >> 4, be = (jhi - 1) >> 4;
+            uint32_t W0[16], W1[16];
+            load(W0, bb);
+            for (int b = bb;; b += 2) {
+                if (b + 1 <= be) load(W1, b + 1);
+                block(W0, b);
+                if (b + 1 > be) break;
+                if (b + 2 <= be) load(W0, b + 2);
+                block(W1, b + 1);
+                if (b + 2 > be) break;
             }
         }
 
@@ -890,7 +929,7 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const uint32_t 
         pk_narrow(A, B, fa, fb, ca, cbh);
     }
     if (d.cells) {
-        unsigned long long c = A.ncells + B.ncells;
+        unsigned long long c = (unsigned long long)A.ncells + B.ncells;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
         if (lane == 0) atomicAdd(d.cells, c);

@@ -522,6 +522,8 @@ extern "C" int64_t hipsw_server_submit(hipsw_server *srv, const hipsw_chunk *c)
         !add(c->n_nw, c->nw_q, c->nw_q_off, c->nw_q_len, c->nw_t_pos, c->nw_t_len, nullptr))
         return HIPSW_E_INVALID;
     if (c->n_ext && (!c->ext_h0 || !c->ext_w || !c->ext_eb)) return HIPSW_E_INVALID;
+    for (int k = 0; k < c->n_ext; ++k)
+        if (c->ext_w[k] < 0) return HIPSW_E_INVALID;   // band width must be >= 0
     if (c->n_nw && (!c->nw_w || !c->nw_true_score)) return HIPSW_E_INVALID;
     r->h0.assign(c->ext_h0, c->ext_h0 + c->n_ext);
     r->w.assign(c->ext_w, c->ext_w + c->n_ext);
