@@ -630,7 +630,7 @@ __device__ __forceinline__ uint32_t mismatch16(uint32_t qw, uint32_t tpat)
 struct PkJob {
     bool valid, nonempty, alive;
     int job, qlen, tlen, h0, w, beg, end, maxv, max_i, max_j, gsc, gtle, max_off, tnw;
-    uint32_t qw0, tw0, tw, twn, tpat, ncells;
+    uint32_t qw0, tw0, tw, twn, tb, ncells;
 };
 
 __device__ __forceinline__ void pk_setup(PkJob &J, const ExtDev &d, const ExtParams &p,
@@ -662,7 +662,7 @@ __device__ __forceinline__ void pk_setup(PkJob &J, const ExtDev &d, const ExtPar
     J.tnw = (J.tlen + 15) >> 4;
     J.tw = d.seq[J.tw0];
     J.twn = d.seq[J.tw0 + min(1, J.tnw)];
-    J.tpat = 0; J.ncells = 0;
+    J.tb = 0; J.ncells = 0;
 }
 
 // Row start of one half (:92-107); returns h1.  `c16` is this half's column
@@ -671,7 +671,7 @@ __device__ __forceinline__ void pk_setup(PkJob &J, const ExtDev &d, const ExtPar
 __device__ __forceinline__ int pk_row_start(PkJob &J, const ExtParams &p, int i, uint16_t *c16)
 {
     J.alive = J.alive && i < J.tlen;
-    J.tpat = ((J.tw >> ((i & 15) * 2)) & 3u) * 0x55555555u;
+    J.tb = (J.tw >> ((i & 15) * 2)) & 3u;                     // this row's target base
     const int wb = i - J.w;
     if (J.alive && J.beg < wb) {
         // beg = i - w (:96) drops column beg: with w >= 0, beg >= i - 1 - w
@@ -788,8 +788,9 @@ __device__ __forceinline__ void pk_finish(const PkJob &J, const ExtDev &d)
 // Extends jobs idx[0..count) (count <= 128) with one wave, two per lane.
 // Requirements (checked by the caller): 8-bit tier for every job, qlen <= 255,
 // no N base, the standard A/-B matrix, LDS for 16 * ceil((qmax + 1) / 16)
-// columns.  SYM: o_del == o_ins and e_del == e_ins (one shared gap-open term).
-template <bool SYM>
+// columns, qmax <= 16 * NQ - 1.  SYM: o_del == o_ins and e_del == e_ins (one
+// shared gap-open term).
+template <bool SYM, int NQ>
 __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const uint32_t *idx, int count,
                             uint32_t *col)
 {
@@ -799,12 +800,19 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const uint32_t 
     pk_setup(B, d, p, idx, lane + 64, count);
     uint16_t *ca = reinterpret_cast<uint16_t *>(col);
     uint16_t *cbh = ca + 1;
-    // the query's 2-bit words live in registers for the whole item
-    uint32_t QA[16], QB[16];
+    // Per 16-base query word, the "differs from base c" bit masks of every
+    // c, held in registers for the whole item: N01 = ne0 | ne1 << 16 and
+    // N23 = ne2 | ne3 << 16.  A row picks its target base's mask with one
+    // select and one shift per job and block.
+    uint32_t NA01[NQ], NA23[NQ], NB01[NQ], NB23[NQ];
 #pragma unroll
-    for (int b = 0; b < 16; ++b) {
-        QA[b] = b * 16 < A.qlen ? d.seq[A.qw0 + b] : 0u;
-        QB[b] = b * 16 < B.qlen ? d.seq[B.qw0 + b] : 0u;
+    for (int b = 0; b < NQ; ++b) {
+        const uint32_t qa = b * 16 < A.qlen ? d.seq[A.qw0 + b] : 0u;
+        const uint32_t qb = b * 16 < B.qlen ? d.seq[B.qw0 + b] : 0u;
+        NA01[b] = mismatch16(qa, 0u) | (mismatch16(qa, 0x55555555u) << 16);
+        NA23[b] = mismatch16(qa, 0xAAAAAAAAu) | (mismatch16(qa, 0xFFFFFFFFu) << 16);
+        NB01[b] = mismatch16(qb, 0u) | (mismatch16(qb, 0x55555555u) << 16);
+        NB23[b] = mismatch16(qb, 0xAAAAAAAAu) | (mismatch16(qb, 0xFFFFFFFFu) << 16);
     }
     const int qmax = uniform_max(max(A.alive ? A.qlen : 0, B.alive ? B.qlen : 0));
     for (int j = 0; j <= qmax; ++j) {                          // first row, :68-77
@@ -830,6 +838,9 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const uint32_t 
         const int h1a = pk_row_start(A, p, i, ca);
         const int h1b = pk_row_start(B, p, i, cbh);
         if (__ballot(A.alive || B.alive) == 0) break;
+        // this row's target base of each job: mask register (N23 for G/T) and shift
+        const bool hiA = (A.tb & 2u) != 0, hiB = (B.tb & 2u) != 0;
+        const uint32_t shA = (A.tb & 1u) * 16, shB = (B.tb & 1u) * 16;
         us2 h1 = us2{(unsigned short)h1a, (unsigned short)h1b};
         us2 f = splat(0), key = splat(0);
         const int jlo = uniform_min(min(A.alive ? A.beg : BIG, B.alive ? B.beg : BIG));
@@ -844,7 +855,10 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const uint32_t 
         // and the new E (:128-129); (2) the running F and h1 (:121-123,
         // :130-131); (3) the row-maximum candidates H * 256 + k (:124-125).
         auto block = [&](uint32_t (&wv)[16], int b) {
-            const uint32_t mm = mismatch16(QA[b], A.tpat) | (mismatch16(QB[b], B.tpat) << 16);
+            const uint32_t a01 = NA01[b], a23 = NA23[b], b01 = NB01[b], b23 = NB23[b];
+            const uint32_t ma = (hiA ? a23 : a01) >> shA;
+            const uint32_t mb = (hiB ? b23 : b01) >> shB;
+            const uint32_t mm = __builtin_amdgcn_perm(mb, ma, 0x05040100u);   // ma.lo16 | mb.lo16 << 16
             const int c0 = b * 16;
             us2 dgE[16], tf[16], ne[16], Hs[16], M[16], E[16], s[16];
 #pragma unroll
@@ -878,11 +892,16 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const uint32_t 
             } else {
                 // commit only left of each half's band end
                 const us2 Lb = vsubs(L, splat(c0));
+                uint32_t hms[16];
 #pragma unroll
                 for (int k = 0; k < 16; ++k) {
-                    uint32_t hm = w32(__builtin_bit_cast(
+                    hms[k] = w32(__builtin_bit_cast(
                         us2, __builtin_bit_cast(ss2, splat(k) - Lb) >> ss2{15, 15}));
-                    asm volatile("" : "+v"(hm));   // keep the selects bitwise
+                    asm volatile("" : "+v"(hms[k]));   // keep the selects bitwise
+                }
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const uint32_t hm = hms[k];
                     const us2 H = vmax(dgE[k], f);
                     f = vmax(vsubs(f, eI), tf[k]);
                     const uint32_t nw = __builtin_amdgcn_perm(w32(ne[k]), w32(h1), 0x06020400u);
@@ -909,14 +928,15 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const uint32_t 
         if (jlo < jhi) {
             // two word sets in turn: the next block's reads are in flight
             // while the current block computes
+            // (a read past the last block repeats the last block: in bounds, unused)
             const int bb = jlo >> 4, be = (jhi - 1) >> 4;
             uint32_t W0[16], W1[16];
             load(W0, bb);
             for (int b = bb;; b += 2) {
-                if (b + 1 <= be) load(W1, b + 1);
+                load(W1, min(b + 1, be));
                 block(W0, b);
                 if (b + 1 > be) break;
-                if (b + 2 <= be) load(W0, b + 2);
+                load(W0, min(b + 2, be));
                 block(W1, b + 1);
                 if (b + 2 > be) break;
             }
@@ -942,7 +962,7 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const uint32_t 
 // whose jobs include an N base runs as two generic 64-job halves instead.  The
 // block's LDS is exactly the column state (no static shared memory), so four
 // waves of 150-bp queries fill a 160 KiB CU.
-template <bool SYM>
+template <bool SYM, int NQ>
 __global__ __launch_bounds__(64) void ext_persistent_pk_kernel(ExtDev d, ExtParams p,
                                                                const uint32_t *order,
                                                                const WorkItem *items, int n_items,
@@ -962,7 +982,7 @@ __global__ __launch_bounds__(64) void ext_persistent_pk_kernel(ExtDev d, ExtPara
         for (int k = lane; k < (int)wi.count; k += 64)
             has_n = has_n || d.qn_word[idx[k]] != kNoN || d.tn_word[idx[k]] != kNoN;
         if (__ballot(has_n) == 0) {
-            ext_wave_pk<SYM>(d, p, idx, (int)wi.count, col);
+            ext_wave_pk<SYM, NQ>(d, p, idx, (int)wi.count, col);
         } else {
             // generic 8-bit form: u16 columns first, the score matrix after them
             int8_t *smat = reinterpret_cast<int8_t *>(lds_raw) + (size_t)cols * 128;
@@ -986,12 +1006,19 @@ hipError_t launch_ext_persistent_pk(const ExtDev &d, const ExtParams &p, const u
     if (n_items <= 0) return hipSuccess;
     if (cols < 1 || cols > 256) return hipErrorInvalidValue;
     const bool sym = p.o_del == p.o_ins && p.e_del == p.e_ins;
-    if (sym)
-        hipLaunchKernelGGL(ext_persistent_pk_kernel<true>, dim3(grid), dim3(64), ext_pk_lds_bytes(cols),
-                           s, d, p, order, items, n_items, counter, cols);
-    else
-        hipLaunchKernelGGL(ext_persistent_pk_kernel<false>, dim3(grid), dim3(64), ext_pk_lds_bytes(cols),
-                           s, d, p, order, items, n_items, counter, cols);
+    const int nq = (cols + 15) / 16;             // query words of the longest query
+    const size_t lds = ext_pk_lds_bytes(cols);
+#define HIPSW_PK_LAUNCH(S, Q)                                                                   \
+    hipLaunchKernelGGL((ext_persistent_pk_kernel<S, Q>), dim3(grid), dim3(64), lds, s, d, p,   \
+                       order, items, n_items, counter, cols)
+    if (nq <= 4) {
+        if (sym) HIPSW_PK_LAUNCH(true, 4); else HIPSW_PK_LAUNCH(false, 4);
+    } else if (nq <= 10) {
+        if (sym) HIPSW_PK_LAUNCH(true, 10); else HIPSW_PK_LAUNCH(false, 10);
+    } else {
+        if (sym) HIPSW_PK_LAUNCH(true, 16); else HIPSW_PK_LAUNCH(false, 16);
+    }
+#undef HIPSW_PK_LAUNCH
     return hipGetLastError();
 }
 
