@@ -64,8 +64,11 @@ size_t span(size_t count)
 namespace hipsw {
 
 // Batched form: bin by (tier, query-length class); inside a bin order by
-// estimated DP area, largest first; cut into 64-job work items; one persistent
-// launch per bin.  Returns the global-state scratch bytes the launches need.
+// estimated DP area, largest first, and within an area step of 256 cells by h0,
+// largest first (h0 sets how fast a band's left edge moves, so jobs sharing a
+// wave keep similar bands); cut into work items of 64 jobs (128 for the packed
+// form); one persistent launch per bin.  Returns the global-state scratch
+// bytes the launches need.
 size_t plan_ext_batched(const hipsw_engine *eng, int n, const int32_t *ql, const int32_t *tl,
                         const int32_t *h0, const int32_t *w, int max_mat,
                         std::vector<uint32_t> &order, std::vector<WorkItem> &items,
@@ -76,7 +79,8 @@ size_t plan_ext_batched(const hipsw_engine *eng, int n, const int32_t *ql, const
         const uint32_t bin = (uint32_t)(tier_of(h0[k], ql[k], max_mat) * kClasses + class_of(ql[k]));
         const uint64_t band = (uint64_t)std::min<int64_t>(ql[k], 2 * (int64_t)std::max(w[k], 0) + 1);
         const uint64_t area = std::min<uint64_t>((uint64_t)tl[k] * band, (1ull << 28) - 1);
-        keys[k] = (bin << 28) | (uint32_t)(((1ull << 28) - 1) - area);   // bin asc, area desc
+        const uint32_t h0c = (uint32_t)std::min(std::max(h0[k], 0), 255);
+        keys[k] = (bin << 28) | ((uint32_t)(((1ull << 28) - 1) - area) & ~0xFFu) | (255u - h0c);
         idx[k] = (uint32_t)k;
     }
     radix_sort_by_key(keys, idx);

@@ -626,11 +626,18 @@ __device__ __forceinline__ uint32_t mismatch16(uint32_t qw, uint32_t tpat)
     return x;
 }
 
+// Column state layout: groups of four columns, lane-major inside a group, so
+// one 128-bit LDS access moves four columns of a lane (a 16-column block is
+// four reads and four writes).  Word (j, lane) is u32 number
+// (j / 4) * 256 + lane * 4 + j % 4; cpos(j) is the u16 offset of column j from
+// a lane's half pointer.
+__device__ __forceinline__ int cpos(int j) { return (j >> 2) * 512 + (j & 3) * 2; }
+
 // Per-job row-level state of one half.
 struct PkJob {
     bool valid, nonempty, alive;
     int job, qlen, tlen, h0, w, beg, end, maxv, max_i, max_j, gsc, gtle, max_off, tnw;
-    uint32_t qw0, tw0, tw, twn, tb, ncells;
+    uint32_t qw0, tw0, twe, two, tb, ncells;   // twe / two: target words of even / odd 16-row blocks
 };
 
 __device__ __forceinline__ void pk_setup(PkJob &J, const ExtDev &d, const ExtParams &p,
@@ -660,23 +667,23 @@ __device__ __forceinline__ void pk_setup(PkJob &J, const ExtDev &d, const ExtPar
     // target words of rows 0-15 and 16-31 (the second is a prefetch); word
     // indexes stop at the job's last word + 1, inside the padded arena
     J.tnw = (J.tlen + 15) >> 4;
-    J.tw = d.seq[J.tw0];
-    J.twn = d.seq[J.tw0 + min(1, J.tnw)];
+    J.twe = d.seq[J.tw0];
+    J.two = d.seq[J.tw0 + min(1, J.tnw)];
     J.tb = 0; J.ncells = 0;
 }
 
 // Row start of one half (:92-107); returns h1.  `c16` is this half's column
-// state (u16 (h, e) of column j at c16[j * 128]).  The target word of the row
-// is current (the caller refreshes tw / twn every 16 rows).
+// state (u16 (h, e) of column j at c16[cpos(j)]).  The caller keeps the target
+// words of the current and the next 16-row block loaded.
 __device__ __forceinline__ int pk_row_start(PkJob &J, const ExtParams &p, int i, uint16_t *c16)
 {
     J.alive = J.alive && i < J.tlen;
-    J.tb = (J.tw >> ((i & 15) * 2)) & 3u;                     // this row's target base
+    J.tb = (((i >> 4) & 1 ? J.two : J.twe) >> ((i & 15) * 2)) & 3u;   // this row's target base
     const int wb = i - J.w;
     if (J.alive && J.beg < wb) {
         // beg = i - w (:96) drops column beg: with w >= 0, beg >= i - 1 - w
         // after the previous row, so exactly one column per row; clear it
-        c16[J.beg * 128] = 0;
+        c16[cpos(J.beg)] = 0;
         J.beg = wb;
     }
     J.end = min(min(J.end, i + J.w + 1), J.qlen);             // :97-98
@@ -690,7 +697,7 @@ __device__ __forceinline__ int pk_row_start(PkJob &J, const ExtParams &p, int i,
 __device__ __forceinline__ void pk_row_end(PkJob &J, const ExtParams &p, int i, int h1, int key,
                                            uint16_t *c16)
 {
-    c16[J.end * 128] = (uint16_t)h1;                           // h[end] = h1, e[end] = 0
+    c16[cpos(J.end)] = (uint16_t)h1;                           // h[end] = h1, e[end] = 0
     const int rm = key >> 8, rj = key & 0xFF;
     const bool at_q = J.alive && J.end == J.qlen;              // :149-152
     J.gtle = (at_q && !(J.gsc > h1)) ? i : J.gtle;
@@ -719,8 +726,8 @@ __device__ __forceinline__ void pk_narrow(PkJob &A, PkJob &B, int h1a, int h1b, 
         uint32_t va[4], vb[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            va[k] = ca[min(ja + k, ea) * 128];
-            vb[k] = cb[min(jb + k, eb) * 128];
+            va[k] = ca[cpos(min(ja + k, ea))];
+            vb[k] = cb[cpos(min(jb + k, eb))];
         }
         bool za = ga && va[0] == 0, zb = gb && vb[0] == 0;
         int na = za, nb = zb;
@@ -750,8 +757,8 @@ __device__ __forceinline__ void pk_narrow(PkJob &A, PkJob &B, int h1a, int h1b, 
         uint32_t va[4], vb[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            va[k] = ca[max(ka - k, 0) * 128];
-            vb[k] = cb[max(kb - k, 0) * 128];
+            va[k] = ca[cpos(max(ka - k, 0))];
+            vb[k] = cb[cpos(max(kb - k, 0))];
         }
         bool za = sa && va[0] == 0, zb = sb && vb[0] == 0;
         int na = za, nb = zb;
@@ -789,7 +796,8 @@ __device__ __forceinline__ void pk_finish(const PkJob &J, const ExtDev &d)
 // Requirements (checked by the caller): 8-bit tier for every job, qlen <= 255,
 // no N base, the standard A/-B matrix, LDS for 16 * ceil((qmax + 1) / 16)
 // columns, qmax <= 16 * NQ - 1.  SYM: o_del == o_ins and e_del == e_ins (one
-// shared gap-open term).
+// shared gap-open term).  `col` is this lane's word of column 0 (group layout,
+// see cpos).
 template <bool SYM, int NQ>
 __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const uint32_t *idx, int count,
                             uint32_t *col)
@@ -818,7 +826,7 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const uint32_t 
     for (int j = 0; j <= qmax; ++j) {                          // first row, :68-77
         const int ha = (A.alive && j <= A.qlen) ? (j == 0 ? A.h0 : max(0, A.h0 - p.o_ins - p.e_ins * j)) : 0;
         const int hb = (B.alive && j <= B.qlen) ? (j == 0 ? B.h0 : max(0, B.h0 - p.o_ins - p.e_ins * j)) : 0;
-        col[j * 64] = (uint32_t)ha | ((uint32_t)hb << 16);
+        col[(j >> 2) * 256 + (j & 3)] = (uint32_t)ha | ((uint32_t)hb << 16);
     }
     const ss2 Ssc = ss2{(short)p.match_sc, (short)p.match_sc};
     const ss2 nAB = ss2{(short)(p.mis_sc - p.match_sc), (short)(p.mis_sc - p.match_sc)};
@@ -829,11 +837,17 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const uint32_t 
     const int BIG = 0x7fffffff;
 
     for (int i = 0;; ++i) {
-        if (i > 0 && (i & 15) == 0) {                          // next 16 target bases
-            A.tw = A.twn;
-            B.tw = B.twn;
-            A.twn = d.seq[A.tw0 + min((i >> 4) + 1, A.tnw)];
-            B.twn = d.seq[B.tw0 + min((i >> 4) + 1, B.tnw)];
+        if (i > 0 && (i & 15) == 0) {
+            // 16-row block i/16 starts: fetch block i/16 + 1 into the register
+            // of block i/16 - 1 (it is first read 16 rows from now)
+            const int nb = (i >> 4) + 1;
+            if (nb & 1) {
+                A.two = d.seq[A.tw0 + min(nb, A.tnw)];
+                B.two = d.seq[B.tw0 + min(nb, B.tnw)];
+            } else {
+                A.twe = d.seq[A.tw0 + min(nb, A.tnw)];
+                B.twe = d.seq[B.tw0 + min(nb, B.tnw)];
+            }
         }
         const int h1a = pk_row_start(A, p, i, ca);
         const int h1b = pk_row_start(B, p, i, cbh);
@@ -910,9 +924,10 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const uint32_t 
                     Hs[k] = u2(w32(H * C256 + splat(k)) & hm);
                 }
             }
-            uint32_t *cb = col + b * 16 * 64;
+            uint4 *cb = reinterpret_cast<uint4 *>(col + b * 1024);
 #pragma unroll
-            for (int k = 0; k < 16; ++k) cb[k * 64] = wv[k];
+            for (int q = 0; q < 4; ++q)
+                cb[q * 64] = uint4{wv[4 * q], wv[4 * q + 1], wv[4 * q + 2], wv[4 * q + 3]};
 #pragma unroll
             for (int k = 0; k < 8; ++k) Hs[k] = vmax(Hs[k], Hs[k + 8]);
 #pragma unroll
@@ -921,9 +936,12 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const uint32_t 
             key = vmax(key, kb + splat(c0));
         };
         auto load = [&](uint32_t (&wv)[16], int b) {
-            const uint32_t *cb = col + b * 16 * 64;
+            const uint4 *cb = reinterpret_cast<const uint4 *>(col + b * 1024);
 #pragma unroll
-            for (int k = 0; k < 16; ++k) wv[k] = cb[k * 64];
+            for (int q = 0; q < 4; ++q) {
+                const uint4 v = cb[q * 64];
+                wv[4 * q] = v.x; wv[4 * q + 1] = v.y; wv[4 * q + 2] = v.z; wv[4 * q + 3] = v.w;
+            }
         };
         if (jlo < jhi) {
             // two word sets in turn: the next block's reads are in flight
@@ -969,7 +987,7 @@ __global__ __launch_bounds__(64) void ext_persistent_pk_kernel(ExtDev d, ExtPara
                                                                uint32_t *counter, int cols)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    uint32_t *col = reinterpret_cast<uint32_t *>(lds_raw) + threadIdx.x;
+    uint32_t *col = reinterpret_cast<uint32_t *>(lds_raw) + threadIdx.x * 4;
     const int lane = threadIdx.x;
     for (;;) {
         int item = 0;
