@@ -238,7 +238,7 @@ extern "C" hipsw_ext_set *hipsw_ext_upload(hipsw_engine *eng, int32_t n,
     // One device allocation for the whole set.
     const size_t bytes = span<uint32_t>(words) + span<uint32_t>(nwords) + 4 * span<uint32_t>(n) +
                          6 * span<int32_t>(n) + span<int32_t>((size_t)n * 6) +
-                         span<uint32_t>(n) + span<WorkItem>(items.size() + 1);
+                         span<uint32_t>(n) + span<WorkItem>(items.size() + 1) + span<PkMeta>(n);
     if (!eng->check(hipMalloc(&set->dmem, bytes), "hipMalloc(ext set)")) {
         delete set;
         return nullptr;
@@ -260,6 +260,10 @@ extern "C" hipsw_ext_set *hipsw_ext_upload(hipsw_engine *eng, int32_t n,
     int32_t *d_out = carve<int32_t>(cur, (size_t)n * 6);
     set->d_order = carve<uint32_t>(cur, n);
     set->d_items = carve<WorkItem>(cur, items.size() + 1);
+    PkMeta *d_pk = carve<PkMeta>(cur, n);
+    std::vector<PkMeta> pk(n);
+    build_pk_meta(n, order.data(), qw.data(), tw.data(), qnw.data(), tnw.data(), ql.data(), tl.data(),
+                  vh0.data(), vw.data(), veb.data(), pk.data());
     bool ok = true;
     auto up = [&](void *dst, const void *src, size_t b) {
         if (ok && b) ok = eng->check(hipMemcpyAsync(dst, src, b, hipMemcpyHostToDevice, eng->stream), "upload");
@@ -277,6 +281,7 @@ extern "C" hipsw_ext_set *hipsw_ext_upload(hipsw_engine *eng, int32_t n,
     up(d_eb, veb.data(), (size_t)n * 4);
     up(set->d_order, order.data(), (size_t)n * 4);
     up(set->d_items, items.data(), items.size() * sizeof(WorkItem));
+    up(d_pk, pk.data(), (size_t)n * sizeof(PkMeta));
     if (ok) ok = eng->check(hipStreamSynchronize(eng->stream), "upload sync");
     if (!ok) {
         hipFree(set->dmem);
@@ -284,7 +289,7 @@ extern "C" hipsw_ext_set *hipsw_ext_upload(hipsw_engine *eng, int32_t n,
         return nullptr;
     }
     set->dev = ExtDev{d_seq, d_nmask, d_qw, d_tw, d_qn, d_tn, d_ql, d_tl, d_h0, d_w, d_eb, d_out,
-                      reinterpret_cast<unsigned long long *>(d_unused)};
+                      reinterpret_cast<unsigned long long *>(d_unused), d_pk};
     return set;
 }
 

@@ -641,17 +641,18 @@ struct PkJob {
 };
 
 __device__ __forceinline__ void pk_setup(PkJob &J, const ExtDev &d, const ExtParams &p,
-                                         const uint32_t *idx, int slot, int count)
+                                         const PkMeta *meta, int slot, int count)
 {
     J.valid = slot < count;
-    J.job = J.valid ? (int)idx[slot] : 0;
-    J.qlen = J.valid ? d.q_len[J.job] : 0;
-    J.tlen = J.valid ? d.t_len[J.job] : 0;
-    J.h0 = J.valid ? d.h0[J.job] : 0;
-    int w = J.valid ? d.w[J.job] : 0;
-    const int eb = J.valid ? d.eb[J.job] : 0;
-    J.qw0 = J.valid ? d.q_word[J.job] : 0;
-    J.tw0 = J.valid ? d.t_word[J.job] : 0;
+    const PkMeta m = meta[J.valid ? slot : 0];
+    J.job = (int)m.job;
+    J.qlen = J.valid ? m.q_len : 0;
+    J.tlen = J.valid ? m.t_len : 0;
+    J.h0 = J.valid ? m.h0 : 0;
+    int w = J.valid ? m.w : 0;
+    const int eb = J.valid ? m.eb : 0;
+    J.qw0 = J.valid ? m.q_word : 0;
+    J.tw0 = J.valid ? m.t_word : 0;
     J.nonempty = J.valid && J.qlen > 0 && J.tlen > 0;        // :33-35
     J.alive = J.nonempty;
     if (J.alive) {                                             // w cap, :52-62
@@ -799,13 +800,13 @@ __device__ __forceinline__ void pk_finish(const PkJob &J, const ExtDev &d)
 // shared gap-open term).  `col` is this lane's word of column 0 (group layout,
 // see cpos).
 template <bool SYM, int NQ>
-__device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const uint32_t *idx, int count,
+__device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *meta, int count,
                             uint32_t *col)
 {
     const int lane = threadIdx.x & 63;
     PkJob A, B;
-    pk_setup(A, d, p, idx, lane, count);
-    pk_setup(B, d, p, idx, lane + 64, count);
+    pk_setup(A, d, p, meta, lane, count);
+    pk_setup(B, d, p, meta, lane + 64, count);
     uint16_t *ca = reinterpret_cast<uint16_t *>(col);
     uint16_t *cbh = ca + 1;
     // Per 16-base query word, the "differs from base c" bit masks of every
@@ -996,11 +997,11 @@ __global__ __launch_bounds__(64) void ext_persistent_pk_kernel(ExtDev d, ExtPara
         if (item >= n_items) break;
         const WorkItem wi = items[item];
         const uint32_t *idx = order + wi.start;
+        const PkMeta *meta = d.pk + wi.start;
         bool has_n = false;
-        for (int k = lane; k < (int)wi.count; k += 64)
-            has_n = has_n || d.qn_word[idx[k]] != kNoN || d.tn_word[idx[k]] != kNoN;
+        for (int k = lane; k < (int)wi.count; k += 64) has_n = has_n || meta[k].has_n != 0;
         if (__ballot(has_n) == 0) {
-            ext_wave_pk<SYM, NQ>(d, p, idx, (int)wi.count, col);
+            ext_wave_pk<SYM, NQ>(d, p, meta, (int)wi.count, col);
         } else {
             // generic 8-bit form: u16 columns first, the score matrix after them
             int8_t *smat = reinterpret_cast<int8_t *>(lds_raw) + (size_t)cols * 128;
@@ -1012,6 +1013,18 @@ __global__ __launch_bounds__(64) void ext_persistent_pk_kernel(ExtDev d, ExtPara
                 ext_wave_generic<TIER_U8>(d, p, idx + 64, (int)wi.count - 64, c16, smat);
         }
         __syncthreads();
+    }
+}
+
+void build_pk_meta(int n, const uint32_t *order, const uint32_t *q_word, const uint32_t *t_word,
+                   const uint32_t *qn_word, const uint32_t *tn_word, const int32_t *q_len,
+                   const int32_t *t_len, const int32_t *h0, const int32_t *w, const int32_t *eb,
+                   PkMeta *out)
+{
+    for (int s = 0; s < n; ++s) {
+        const uint32_t k = order[s];
+        out[s] = PkMeta{k, q_word[k], t_word[k], (uint16_t)q_len[k], (uint16_t)t_len[k],
+                        h0[k], w[k], eb[k], (uint32_t)(qn_word[k] != kNoN || tn_word[k] != kNoN)};
     }
 }
 

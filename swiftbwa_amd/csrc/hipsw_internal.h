@@ -17,6 +17,16 @@ constexpr uint32_t kNoN = 0xFFFFFFFFu;   // "sequence has no N" marker
 // Sequences are 2-bit codes, 16 per 32-bit word, base k of a word at bits
 // 2k..2k+1; every query and target starts on a word boundary.  N bases carry
 // code 0 in the 2-bit stream and a set bit in the N-mask stream (32 per word).
+// One job of the packed form, stored in batched order so that a work item's
+// descriptors are one contiguous read.
+struct PkMeta {
+    uint32_t job;             // index into the caller's job arrays / output rows
+    uint32_t q_word, t_word;  // first 2-bit word of query / target
+    uint16_t q_len, t_len;
+    int32_t  h0, w, eb;
+    uint32_t has_n;           // non-zero: query or target holds an N
+};
+
 struct ExtDev {
     const uint32_t *seq;      // 2-bit words
     const uint32_t *nmask;    // N-bit words (never dereferenced when no N)
@@ -31,6 +41,7 @@ struct ExtDev {
     const int32_t  *eb;       // end bonus
     int32_t        *out;      // 6 per job, SWResult order
     unsigned long long *cells; // DP cells evaluated (one atomic add per wave)
+    const PkMeta   *pk;       // packed-form descriptors in batched order (batched form only)
 };
 
 // Scoring constants broadcast to every lane.
@@ -66,6 +77,11 @@ size_t ext_state_bytes(int tier);   // bytes of h/e state per column per lane
 size_t ext_lds_bytes(int tier, bool global_state, int cols);   // dynamic LDS per block
 // Packed form (two jobs per lane, 8-bit tier, LDS state, items of <= 128 jobs).
 size_t ext_pk_lds_bytes(int cols);
+// Descriptors of the packed form, in `order`.
+void build_pk_meta(int n, const uint32_t *order, const uint32_t *q_word, const uint32_t *t_word,
+                   const uint32_t *qn_word, const uint32_t *tn_word, const int32_t *q_len,
+                   const int32_t *t_len, const int32_t *h0, const int32_t *w, const int32_t *eb,
+                   PkMeta *out);
 hipError_t launch_ext_persistent_pk(const ExtDev &d, const ExtParams &p, const uint32_t *order,
                                     const WorkItem *items, int n_items, uint32_t *counter,
                                     int cols, int grid, hipStream_t s);

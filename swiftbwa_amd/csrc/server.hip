@@ -215,6 +215,7 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
                          span<int32_t>(n) + span<uint32_t>((size_t)r.n_nw * cap + 1) +
                          3 * span<int32_t>(n) + 2 * span<uint32_t>(n) +
                          span<WorkItem>(n / 64 + 64) + span<uint8_t>(colmax_b) +
+                         span<PkMeta>(r.n_ext) +
                          span<unsigned long long>(2);
     void *dmem = nullptr;
     if (hipMallocAsync(&dmem, bytes, st) != hipSuccess) return HIPSW_E_DEVICE;
@@ -233,7 +234,9 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
     WorkItem *d_items = carve<WorkItem>(cur, n / 64 + 64);
     uint8_t *d_colmax = carve<uint8_t>(cur, colmax_b);
     unsigned long long *d_cells = carve<unsigned long long>(cur, 2);
+    PkMeta *d_pk = carve<PkMeta>(cur, r.n_ext);
     std::vector<int32_t> h0v(n, 0), wv(n, 0), ebv(n, 0);
+    std::vector<PkMeta> pk;   // host copy lives until the chunk's stream is synchronised
     for (int k = 0; k < r.n_ext; ++k) { h0v[k] = r.h0[k]; wv[k] = r.w[k]; ebv[k] = r.eb[k]; }
     bool ok = true;
     auto up = [&](void *dst, const void *src, size_t b) {
@@ -256,10 +259,15 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
     if (ok && r.n_ext) {
         up(d_order, ext_order.data(), ext_order.size() * 4);
         up(d_items, ext_items.data(), ext_items.size() * sizeof(WorkItem));
+        pk.resize(r.n_ext);
+        build_pk_meta(r.n_ext, ext_order.data(), qw.data(), tw.data(), qnw.data(), tnw.data(), ql.data(),
+                      tl.data(), h0v.data(), wv.data(), ebv.data(), pk.data());
+        up(d_pk, pk.data(), pk.size() * sizeof(PkMeta));
         void *gs = nullptr;
         if (ok && ext_scratch) ok = hipMallocAsync(&gs, ext_scratch, st) == hipSuccess;
         if (ok) ok = launch_zero_u32(sl.counters, 64, st) == hipSuccess;
-        const ExtDev ed{d_seq, d_nmask, d_qw, d_tw, d_qn, d_tn, d_ql, d_tl, d_h0, d_w, d_eb, d_out6, d_cells};
+        const ExtDev ed{d_seq, d_nmask, d_qw, d_tw, d_qn, d_tn, d_ql, d_tl, d_h0, d_w, d_eb, d_out6,
+                        d_cells, d_pk};
         int k = 0;
         for (const ExtLaunch &L : ext_launch) {
             if (!ok || k >= 64) { ok = false; break; }
