@@ -11,7 +11,7 @@ one-launch form is timed beside it.  Cells are counted exactly as the CPU
 loop counts them (one per inner iteration of BandedSWScalar.swift:112).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
-       python bench.py --profile-leg ext|copy   (one leg only, for rocprofv3 runs)
+       python bench.py --profile-leg ext|mixed|copy   (one leg only, for rocprofv3 runs)
 """
 import argparse
 import json
@@ -27,27 +27,53 @@ sys.path.insert(0, ROOT)
 OPS_PER_CELL = 11          # algorithmic VALU ops per extension cell
 PEAK_PK16_TOPS = 78.6      # 256 CU x 64 lanes x 2.4 GHz x 2 (packed int16)
 PEAK_I32_TOPS = 39.3       # 32-bit VALU lane-ops
+# HBM bytes per launch of the headline kernel (ext_persistent_pk_kernel<true, 10>
+# on the 1,000,000-pair set), from two rocprofv3 --pmc passes over
+# `bench.py --profile-leg ext`: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB, the
+# factors being what the same counters read for a 256 MiB device-to-device
+# copy (`bench.py --profile-leg copy`: FETCH_SIZE 131,087 KiB, WRITE_SIZE
+# 262,144 KiB).  Summary and commands: profiles/ext_uniform_1M_pmc.md.
+EXT_TRAFFIC_BYTES = 372535603
+# Algorithmic HBM bytes per job: 2-bit query and target words, one 32-byte
+# descriptor, one 24-byte result row.
 
 
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(js, sample):
-    """Oracle (C restatement of BandedSWScalar) on a bounded sample, all host cores."""
+    """Oracle (C restatement of BandedSWScalar) on bounded samples of the same
+    job set: all of this GPU's host-core share (16 threads), and one thread."""
     from oracle import pyoracle as orc
     from swiftbwa_amd import workload as wl
-    idx = np.arange(min(sample, js["n"]))
-    sub = wl.slice_jobs(js, idx)
     threads = max(1, min(16, os.cpu_count() or 1))
+    idx = np.arange(min(sample, js["n"]))
     t0 = time.perf_counter()
-    _, cells = orc.ksw_extend_batch(sub, nthreads=threads)
+    _, cells = orc.ksw_extend_batch(wl.slice_jobs(js, idx), nthreads=threads)
     dt = time.perf_counter() - t0
+    idx1 = np.arange(min(max(sample // 12, 1), js["n"]))
+    t1 = time.perf_counter()
+    _, cells1 = orc.ksw_extend_batch(wl.slice_jobs(js, idx1), nthreads=1)
+    dt1 = time.perf_counter() - t1
     return {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": threads,
             "kind": "port",
             "sample": "first %d jobs of the same 1,000,000-pair set (%d cells), C oracle "
-                      "-O3, %d threads, %.2f s" % (len(idx), cells, threads, dt),
-            "ext_per_s": round(len(idx) / dt, 1)}
+                      "-O3, %d threads, %.2f s; CPU %s" % (len(idx), cells, threads, dt, cpu_model()),
+            "ext_per_s": round(len(idx) / dt, 1),
+            "one_thread": {"value": round(cells1 / dt1 / 1e9, 4), "unit": "GCUPS", "cores": 1,
+                           "sample": "first %d jobs (%d cells), %.2f s" % (len(idx1), cells1, dt1)}}
 
 
 def bench_nw(eng, args):
@@ -182,6 +208,7 @@ def profile_leg(args):
     """Runs one leg alone so a profiler sees only its kernels.
 
     ext:  the headline job set, batched form, `warmup` + `steps` passes.
+    mixed: the uneven-size job set (4,000,000 jobs), batched form, likewise.
     copy: a 256 MiB device-to-device copy, `steps` times -- the known-size
           reference that scales the FETCH_SIZE / WRITE_SIZE counters."""
     import ctypes as C
@@ -189,8 +216,9 @@ def profile_leg(args):
     from swiftbwa_amd.metalsw import HipSWEngine, ScoringParameters, _ext_arrays
     eng = HipSWEngine(0)
     lib = eng.lib
-    if args.profile_leg == "ext":
-        js = wl.generate(wl.EXT_UNIFORM, 2, args.n)
+    if args.profile_leg in ("ext", "mixed"):
+        js = (wl.generate(wl.EXT_UNIFORM, 2, args.n) if args.profile_leg == "ext"
+              else wl.generate(wl.EXT_MIXED, 4, args.mixed_n))
         a = _ext_arrays(js)
         sc = ScoringParameters().to_c()
         st = lib.hipsw_ext_upload(eng.handle, js["n"], native.ptr(a["queries"]),
@@ -204,7 +232,8 @@ def profile_leg(args):
         for k in range(args.warmup + args.steps):
             if lib.hipsw_ext_run(eng.handle, st, native.MODE_BATCHED, C.byref(s)):
                 raise RuntimeError(eng.last_error())
-            log("ext batched pass %d: kernel %.3f ms, %d cells" % (k, s.kernel_ms, s.cells))
+            log("%s batched pass %d: kernel %.3f ms, %d cells" % (args.profile_leg, k, s.kernel_ms,
+                                                                  s.cells))
         lib.hipsw_ext_free(eng.handle, st)
     else:
         hip = C.CDLL("libamdhip64.so")
@@ -238,7 +267,7 @@ def main():
     ap.add_argument("--serve-budget-gb", type=int, default=64)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--mixed-n", type=int, default=4000000)
-    ap.add_argument("--profile-leg", choices=["ext", "copy"], default=None)
+    ap.add_argument("--profile-leg", choices=["ext", "mixed", "copy"], default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (each job set runs on one "
@@ -305,6 +334,9 @@ def main():
 
     gcups = cells / (b_wall * 1e-3) / 1e9
     achieved = cells * OPS_PER_CELL / (b_kms * 1e-3) / 1e12
+    qb = (js["q_len"].astype(np.int64) + 15) // 16 * 4
+    tb = (js["t_len"].astype(np.int64) + 15) // 16 * 4
+    algo_bytes = int(qb.sum() + tb.sum()) + js["n"] * (32 + 24)
     rec = {
         "metric": "banded-SW GCUPS (batched persistent form, 1 GPU)",
         "value": round(gcups, 2),
@@ -316,7 +348,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int32 VALU on u8-packed h/e state (exact score tier per job)",
+        "dtype": "u16x2 (two jobs per lane in packed 16-bit halves; 8-bit h/e state)",
         "data": "synthetic",
         "config": {
             "workload": "uniform banded extension, int16 range: 1,000,000 pairs, q=150 bp, "
@@ -336,10 +368,14 @@ def main():
         },
         "roofline": {
             "bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_PK16_TOPS,
-            "unit": "TOP/s", "frac": round(achieved / PEAK_PK16_TOPS, 4), "traffic": None,
+            "unit": "TOP/s", "frac": round(achieved / PEAK_PK16_TOPS, 4),
+            "traffic": EXT_TRAFFIC_BYTES, "algorithmic_bytes": int(algo_bytes),
+            "hbm_frac": round(algo_bytes / (b_kms * 1e-3) / 8.0e12, 6),
             "ops_per_cell": OPS_PER_CELL, "peak_int32": PEAK_I32_TOPS,
             "frac_int32": round(achieved / PEAK_I32_TOPS, 4),
-            "kernel": "ext_persistent_kernel<u8 tier>",
+            "kernel": "ext_persistent_pk_kernel<sym gaps, 10 query words>",
+            "note": "VALU-bound integer DP (cells x 11 algorithmic ops per cell over the packed "
+                    "int16 peak); HBM traffic is far below the 8 TB/s bound (hbm_frac)",
         },
     }
     if not args.no_cpu:
