@@ -131,6 +131,32 @@ int hipsw_global_nw(hipsw_engine *eng, int32_t n,
                     int32_t *out_score, uint32_t *out_cigar, int32_t *out_n_cigar,
                     int32_t *out_flags, int32_t *out_final_w, hipsw_stats *stats);
 
+/* ---- CIGAR generation (replaces CIGARGenerator.generate, CIGARGenerator.swift:
+ *      30-111, and its postProcess, :117-142, for a batch of regions) ----
+ * Region k: query segment qsegs[q_off[k] ..+ q_len[k]) (already reverse-
+ * complemented for reverse-strand regions) and reference segment
+ * rsegs[r_off[k] ..+ r_len[k]), one byte per base 0-4; qb/qe/read_len/
+ * is_reverse/true_score/initial_w/ref_pos as generate() takes them.
+ * match_score / mismatch_penalty are ScoringParameters.matchScore /
+ * mismatchPenalty (inferBandWidth and the fast path use them).  The band is
+ * max(inferBandWidth, initialW); equal-length regions whose score deficit is
+ * below the indel cost take the fast path (lenM, score = trueScore); the rest
+ * run NW with up to 3 doublings on the device.  Outputs per region: score, NM,
+ * pos (refPos plus squeezed leading deletions), the packed CIGAR with soft
+ * clips in out_cigar[k*cap ...] (out_n_cigar[k] = full op count), the MD tag
+ * NUL-terminated in out_md[k*md_cap ...].  out_flags[k]: bit 0 = more than cap
+ * ops (redo the region on the CPU), bit 1 = MD truncated to md_cap - 1 chars. */
+int hipsw_cigar_generate(hipsw_engine *eng, int32_t n,
+                         const uint8_t *qsegs, const uint32_t *q_off, const uint16_t *q_len,
+                         const uint8_t *rsegs, const uint32_t *r_off, const uint16_t *r_len,
+                         const int32_t *qb, const int32_t *qe, const int32_t *read_len,
+                         const uint8_t *is_reverse, const int32_t *true_score,
+                         const int32_t *initial_w, const int64_t *ref_pos,
+                         const hipsw_scoring *sc, int32_t match_score, int32_t mismatch_penalty,
+                         int32_t cap, int32_t md_cap, int32_t *out_score, int32_t *out_nm,
+                         int64_t *out_pos, uint32_t *out_cigar, int32_t *out_n_cigar,
+                         char *out_md, int32_t *out_flags, hipsw_stats *stats);
+
 /* Device-resident NW job sets (benchmarks, serving); see hipsw_ext_upload. */
 typedef struct hipsw_nw_set hipsw_nw_set;
 hipsw_nw_set *hipsw_nw_upload(hipsw_engine *eng, int32_t n,

@@ -327,3 +327,80 @@ class LocalSWDispatcher:
         except native.HipswError:
             return []
         return [LocalSWResult(*map(int, out[k])) if ok[k] else None for k in range(len(tasks))]
+
+
+@dataclass
+class CIGARRegion:
+    """The arguments of CIGARGenerator.generate (CIGARGenerator.swift:30-42)."""
+    querySegment: bytes
+    refSegment: bytes
+    qb: int
+    qe: int
+    readLength: int
+    isReverse: bool
+    trueScore: int
+    initialW: int
+    refPos: int = 0
+
+
+@dataclass
+class CIGARResult:
+    """CIGARResult: packed CIGAR (len << 4 | op, M0 I1 D2 S4), NM, MD, score, pos."""
+    cigar: List[int]
+    nm: int
+    md: str
+    score: int
+    pos: int
+
+
+class CIGARGenerator:
+    """CIGARGenerator.generate for a batch of regions through
+    hipsw_cigar_generate.  A region whose CIGAR or MD does not fit the caps
+    comes back as None (the caller generates it on the CPU); a failed call
+    returns []."""
+    MAX_CIGAR_OPS = 512
+    MAX_MD = 1024
+
+    @staticmethod
+    def generateBatch(regions: List[CIGARRegion], scoring: ScoringParameters,
+                      engine: HipSWEngine, stats=None) -> List[Optional[CIGARResult]]:
+        if not regions:
+            return []
+        n = len(regions)
+        jobs = tasks_to_jobs([BandedSWTask(r.querySegment, r.refSegment, 0, 0, scoring)
+                              for r in regions])
+        a = _ext_arrays(jobs)
+        i32 = lambda xs: np.ascontiguousarray(xs, np.int32)
+        qb, qe = i32([r.qb for r in regions]), i32([r.qe for r in regions])
+        rl, ts = i32([r.readLength for r in regions]), i32([r.trueScore for r in regions])
+        iw = i32([r.initialW for r in regions])
+        rev = np.ascontiguousarray([1 if r.isReverse else 0 for r in regions], np.uint8)
+        pos = np.ascontiguousarray([r.refPos for r in regions], np.int64)
+        cap, mcap = CIGARGenerator.MAX_CIGAR_OPS, CIGARGenerator.MAX_MD
+        o_score, o_nm, o_ncig, o_flags = (np.zeros(n, np.int32) for _ in range(4))
+        o_pos = np.zeros(n, np.int64)
+        o_cig = np.zeros((n, cap), np.uint32)
+        o_md = np.zeros((n, mcap), np.uint8)
+        sc = scoring.to_c()
+        st = native.HipswStats()
+        rc = engine.lib.hipsw_cigar_generate(
+            engine.handle, n, native.ptr(a["queries"]), native.ptr(a["q_off"]),
+            native.ptr(a["q_len"]), native.ptr(a["targets"]), native.ptr(a["t_off"]),
+            native.ptr(a["t_len"]), native.ptr(qb), native.ptr(qe), native.ptr(rl),
+            native.ptr(rev), native.ptr(ts), native.ptr(iw), native.ptr(pos), C.byref(sc),
+            scoring.matchScore, scoring.mismatchPenalty, cap, mcap, native.ptr(o_score),
+            native.ptr(o_nm), native.ptr(o_pos), native.ptr(o_cig), native.ptr(o_ncig),
+            native.ptr(o_md), native.ptr(o_flags), C.byref(st))
+        if rc != native.HIPSW_OK:
+            return []
+        if stats is not None:
+            stats.update(st.as_dict())
+        out = []
+        for k in range(n):
+            if o_flags[k]:
+                out.append(None)
+                continue
+            md = bytes(o_md[k]).split(b"\0", 1)[0].decode()
+            out.append(CIGARResult([int(x) for x in o_cig[k, :o_ncig[k]]], int(o_nm[k]), md,
+                                   int(o_score[k]), int(o_pos[k])))
+        return out

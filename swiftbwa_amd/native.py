@@ -78,6 +78,10 @@ _SIGS = {
     "hipsw_global_nw": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P, _P, C.c_int32,
                                   C.c_int32, C.POINTER(HipswScoring), C.c_int32, _P, _P, _P,
                                   _P, _P, C.POINTER(HipswStats)]),
+    "hipsw_cigar_generate": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                       _P, _P, _P, C.POINTER(HipswScoring), C.c_int32, C.c_int32,
+                                       C.c_int32, C.c_int32, _P, _P, _P, _P, _P, _P, _P,
+                                       C.POINTER(HipswStats)]),
     "hipsw_nw_upload": (_P, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P, _P, C.c_int32,
                              C.c_int32, C.POINTER(HipswScoring)]),
     "hipsw_nw_run": (C.c_int, [_P, _P, C.c_int32, C.POINTER(HipswStats)]),
