@@ -157,6 +157,46 @@ int hipsw_cigar_generate(hipsw_engine *eng, int32_t n,
                          int64_t *out_pos, uint32_t *out_cigar, int32_t *out_n_cigar,
                          char *out_md, int32_t *out_flags, hipsw_stats *stats);
 
+/* ---- chain extension (replaces ExtensionAligner.extend,
+ *      ExtensionAligner.swift:114-346, for a batch of chains) ----
+ * Chain c owns seeds [chain_seed_off[c], chain_seed_off[c+1]) (MemSeed fields
+ * rbeg / qbeg / len / score, Seed.swift:4-19) and read chain_read[c] (bases
+ * 0-4 at reads[read_off[r] ..+ read_len[r]]).  Reference windows come from
+ * get_ref(ref_ctx, pos, len, buf), the getReference closure of the Swift call
+ * (returns the bases written, <= len); hipsw_get_reference_pac is
+ * FMIndex.getReference over a .pac.  Results: the regions of chain c in the
+ * order the CPU loop appends them, at out[chain_seed_off[c] ...], out_n[c] of
+ * them.  Exact with the CPU loop (windows, stable score order, covered-seed
+ * skip, left then right with h0 chaining, two band tries, clip or extend,
+ * trueScore, seedCov, the sub threshold); rid / isAlt / fracRep are the
+ * chain's and stay with the caller. */
+typedef struct {
+    int64_t rb, re;
+    int32_t qb, qe, score, true_score, sub, sub_n, w, seed_cov, seed_len0, reserved;
+} hipsw_region;
+
+typedef struct {   /* ScoringParameters fields the driver reads */
+    int32_t match_score, mismatch_penalty, band_width, pen_clip5, pen_clip3, min_seed_len;
+} hipsw_ext_opts;
+
+typedef int32_t (*hipsw_get_reference_fn)(void *ctx, int64_t pos, int32_t len, uint8_t *buf);
+
+typedef struct {   /* context of hipsw_get_reference_pac */
+    const uint8_t *pac;     /* 2-bit forward strand, first base in bits 7-6 */
+    int64_t genome_len;     /* forward length; positions >= it read the reverse strand */
+} hipsw_pac_ref;
+
+int32_t hipsw_get_reference_pac(void *ctx, int64_t pos, int32_t len, uint8_t *buf);
+
+int hipsw_extend_chains(hipsw_engine *eng, int32_t n_chains, const uint32_t *chain_seed_off,
+                        const int64_t *seed_rbeg, const int32_t *seed_qbeg,
+                        const int32_t *seed_len, const int32_t *seed_score,
+                        const uint32_t *chain_read, const uint8_t *reads,
+                        const uint32_t *read_off, const int32_t *read_len,
+                        hipsw_get_reference_fn get_ref, void *ref_ctx, const hipsw_scoring *sc,
+                        const hipsw_ext_opts *opt, hipsw_region *out, int32_t *out_n,
+                        hipsw_stats *stats);
+
 /* Device-resident NW job sets (benchmarks, serving); see hipsw_ext_upload. */
 typedef struct hipsw_nw_set hipsw_nw_set;
 hipsw_nw_set *hipsw_nw_upload(hipsw_engine *eng, int32_t n,

@@ -15,7 +15,8 @@ OBJ = os.path.join(HERE, "_build")
 INCLUDE = os.path.join(ROOT, "include")
 
 HIP_SOURCES = ["engine.hip", "ext_host.hip", "ext_kernel.hip", "nw_host.hip", "nw_kernel.hip",
-               "local_host.hip", "local_kernel.hip", "server.hip", "cigar_host.hip"]
+               "local_host.hip", "local_kernel.hip", "server.hip", "cigar_host.hip",
+               "driver_host.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

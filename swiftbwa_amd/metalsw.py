@@ -404,3 +404,60 @@ class CIGARGenerator:
             out.append(CIGARResult([int(x) for x in o_cig[k, :o_ncig[k]]], int(o_nm[k]), md,
                                    int(o_score[k]), int(o_pos[k])))
         return out
+
+
+class ExtensionAligner:
+    """ExtensionAligner.extend (ExtensionAligner.swift:114-346) for a batch of
+    chains through hipsw_extend_chains, reference windows read from a resident
+    .pac with FMIndex.getReference semantics.  chains: list of (read index,
+    [(rbeg, qbeg, len, score), ...]); reads: list of base-code byte strings.
+    Returns, per chain, its regions as dicts in creation order, or [] when the
+    call fails (the caller keeps its CPU path)."""
+    KEYS = ("qb", "qe", "rb", "re", "score", "true_score", "sub", "sub_n", "w", "seed_cov",
+            "seed_len0")
+
+    @staticmethod
+    def extendChains(chains, reads, pac, genome_len, scoring: ScoringParameters,
+                     engine: HipSWEngine, stats=None):
+        nc = len(chains)
+        if nc == 0:
+            return []
+        lib = engine.lib
+        off = np.zeros(nc + 1, np.uint32)
+        off[1:] = np.cumsum([len(s) for _, s in chains])
+        ns = int(off[-1])
+        flat = [x for _, s in chains for x in s]
+        rbeg = np.ascontiguousarray([x[0] for x in flat], np.int64)
+        qbeg = np.ascontiguousarray([x[1] for x in flat], np.int32)
+        slen = np.ascontiguousarray([x[2] for x in flat], np.int32)
+        sscore = np.ascontiguousarray([x[3] for x in flat], np.int32)
+        cread = np.ascontiguousarray([r for r, _ in chains], np.uint32)
+        rl = np.ascontiguousarray([len(r) for r in reads], np.int32)
+        roff = np.zeros(len(reads), np.uint32)
+        if len(reads) > 1:
+            roff[1:] = np.cumsum(rl.astype(np.int64))[:-1]
+        rbytes = np.frombuffer(b"".join(bytes(r) for r in reads) or b"\0", np.uint8).copy()
+        pac = np.ascontiguousarray(pac, np.uint8)
+        ref = native.HipswPacRef(pac.ctypes.data, int(genome_len))
+        opt = native.HipswExtOpts(scoring.matchScore, scoring.mismatchPenalty, scoring.bandWidth,
+                                  scoring.penClip5, scoring.penClip3, scoring.minSeedLength)
+        out = (native.HipswRegion * max(ns, 1))()
+        out_n = np.zeros(nc, np.int32)
+        sc = scoring.to_c()
+        st = native.HipswStats()
+        fn = C.cast(lib.hipsw_get_reference_pac, C.c_void_p)
+        rc = lib.hipsw_extend_chains(engine.handle, nc, native.ptr(off), native.ptr(rbeg),
+                                     native.ptr(qbeg), native.ptr(slen), native.ptr(sscore),
+                                     native.ptr(cread), native.ptr(rbytes), native.ptr(roff),
+                                     native.ptr(rl), fn, C.byref(ref), C.byref(sc), C.byref(opt),
+                                     C.cast(out, C.c_void_p), native.ptr(out_n), C.byref(st))
+        if rc != native.HIPSW_OK:
+            return []
+        if stats is not None:
+            stats.update(st.as_dict())
+        res = []
+        for c in range(nc):
+            base = int(off[c])
+            res.append([{k: int(getattr(out[base + r], k)) for k in ExtensionAligner.KEYS}
+                        for r in range(int(out_n[c]))])
+        return res

@@ -39,6 +39,23 @@ class HipswStats(C.Structure):
                 "tier_jobs": list(self.tier_jobs), "cells": self.cells}
 
 
+class HipswRegion(C.Structure):
+    _fields_ = [("rb", C.c_int64), ("re", C.c_int64), ("qb", C.c_int32), ("qe", C.c_int32),
+                ("score", C.c_int32), ("true_score", C.c_int32), ("sub", C.c_int32),
+                ("sub_n", C.c_int32), ("w", C.c_int32), ("seed_cov", C.c_int32),
+                ("seed_len0", C.c_int32), ("reserved", C.c_int32)]
+
+
+class HipswExtOpts(C.Structure):
+    _fields_ = [("match_score", C.c_int32), ("mismatch_penalty", C.c_int32),
+                ("band_width", C.c_int32), ("pen_clip5", C.c_int32), ("pen_clip3", C.c_int32),
+                ("min_seed_len", C.c_int32)]
+
+
+class HipswPacRef(C.Structure):
+    _fields_ = [("pac", C.c_void_p), ("genome_len", C.c_int64)]
+
+
 class HipswChunk(C.Structure):
     _P = C.c_void_p
     _fields_ = [("n_ext", C.c_int32), ("ext_q", _P), ("ext_q_off", _P), ("ext_q_len", _P),
@@ -82,6 +99,10 @@ _SIGS = {
                                        _P, _P, _P, C.POINTER(HipswScoring), C.c_int32, C.c_int32,
                                        C.c_int32, C.c_int32, _P, _P, _P, _P, _P, _P, _P,
                                        C.POINTER(HipswStats)]),
+    "hipsw_get_reference_pac": (C.c_int32, [_P, C.c_int64, C.c_int32, _P]),
+    "hipsw_extend_chains": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                      _P, C.POINTER(HipswScoring), C.POINTER(HipswExtOpts), _P,
+                                      _P, C.POINTER(HipswStats)]),
     "hipsw_nw_upload": (_P, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P, _P, C.c_int32,
                              C.c_int32, C.POINTER(HipswScoring)]),
     "hipsw_nw_run": (C.c_int, [_P, _P, C.c_int32, C.POINTER(HipswStats)]),
