@@ -76,6 +76,26 @@ def cpu_baseline(js, sample):
                            "sample": "first %d jobs (%d cells), %.2f s" % (len(idx1), cells1, dt1)}}
 
 
+def autotune(eng, st, max_jobs):
+    """hipsw_ext_autotune on an uploaded set: per 8-bit bin, packed vs
+    one-job-per-lane layout timed on the bin's first max_jobs jobs."""
+    import ctypes as C
+    from swiftbwa_amd import native
+    rep = (native.HipswTuneEntry * 16)()
+    nrep = C.c_int32()
+    if eng.lib.hipsw_ext_autotune(eng.handle, st, int(max_jobs), rep, 16, C.byref(nrep)):
+        raise RuntimeError(eng.last_error())
+    kinds = {1: "packed (2 jobs/lane, int16x2)", 2: "one job/lane (int32)"}
+    out = [{"tier": "8-bit", "length_class": ["0-63", "64-159", "160-255", "256+"][e.length_class],
+            "jobs_timed": e.jobs, "packed_ms": round(e.ms_packed, 3),
+            "lane_ms": round(e.ms_lane, 3), "chosen": kinds.get(e.chosen, str(e.chosen))}
+           for e in rep[:nrep.value]]
+    for e in out:
+        log("autotune %s %s: packed %.3f ms, lane %.3f ms -> %s" % (
+            e["tier"], e["length_class"], e["packed_ms"], e["lane_ms"], e["chosen"]))
+    return out
+
+
 def bench_nw(eng, args):
     """Banded global NW + traceback: 500,000 pairs, qlen U[50,250], target =
     query +- a planted 0..20-bp indel, 1% subs / 0.2% indels; w = min(100,
@@ -136,7 +156,7 @@ def bench_mixed(eng, args):
                               native.ptr(a["w"]), native.ptr(a["end_bonus"]), C.byref(sc))
     if not st:
         raise RuntimeError(eng.last_error())
-    res = {"jobs": js["n"]}
+    res = {"jobs": js["n"], "autotune": autotune(eng, st, 100000)}
     for name, mode in (("plain", native.MODE_PLAIN), ("batched", native.MODE_BATCHED)):
         s = native.HipswStats()
         for _ in range(args.warmup):
@@ -299,6 +319,7 @@ def main():
     if not st:
         raise RuntimeError(eng.last_error())
     upload_s = time.perf_counter() - t0
+    tune = autotune(eng, st, 200000)
 
     def run(mode, steps, warmup):
         s = native.HipswStats()
@@ -360,6 +381,7 @@ def main():
             "plain_kernel_ms": round(p_kms, 3),
             "batched_over_plain": round(p_wall / b_wall, 3),
             "tier_jobs": list(b_st.tier_jobs),
+            "autotune": tune,
             "upload_s_incl_pack": round(upload_s, 3),
             "device": dev_name, "n_cu": n_cu,
             "nw_cigar": nw,

@@ -108,6 +108,21 @@ hipsw_ext_set *hipsw_ext_upload(hipsw_engine *eng, int32_t n,
                                 const int32_t *h0, const int32_t *w, const int32_t *end_bonus,
                                 const hipsw_scoring *sc);
 int  hipsw_ext_run(hipsw_engine *eng, hipsw_ext_set *set, int32_t mode, hipsw_stats *stats);
+
+/* Autotuner of the batched form (no reference counterpart).  For each bin of
+ * the set in the 8-bit score tier (one query-length class each) it times the
+ * packed layout (two matrices per lane, packed int16 arithmetic) against one
+ * matrix per lane (int32 arithmetic) on the bin's first max_jobs jobs, keeps
+ * the faster per class in the engine (later uploads and the server plan with
+ * it) and re-plans the set.  Results do not change, only speed.  report
+ * receives up to report_cap entries: tier (0 = 8-bit), query-length class,
+ * jobs timed, the kind chosen (1 packed, 2 one per lane) and both times. */
+typedef struct {
+    int32_t tier, length_class, jobs, chosen;
+    double ms_packed, ms_lane;
+} hipsw_tune_entry;
+int  hipsw_ext_autotune(hipsw_engine *eng, hipsw_ext_set *set, int32_t max_jobs,
+                        hipsw_tune_entry *report, int32_t report_cap, int32_t *n_report);
 int  hipsw_ext_download(hipsw_engine *eng, hipsw_ext_set *set, int32_t *out);
 void hipsw_ext_free(hipsw_engine *eng, hipsw_ext_set *set);
 

@@ -31,6 +31,10 @@ struct hipsw_engine {
     void *nw_dev = nullptr;
     size_t nw_dev_bytes = 0;
 
+    // Autotuner choice per query-length class of the 8-bit tier: 0 = default
+    // (packed form), otherwise an ExtKind (hipsw_ext_autotune).
+    int tuned_kind[4] = {0, 0, 0, 0};
+
     bool fail(const char *what, hipError_t e);
     bool check(hipError_t e, const char *what);
     void *scratch_get(size_t bytes);
@@ -63,10 +67,19 @@ bool pack_jobs(int n, const uint8_t *queries, const uint32_t *q_off, const uint1
 void radix_sort_by_key(std::vector<uint32_t> &keys, std::vector<uint32_t> &idx);
 
 // One persistent launch of the batched extension form.
+// Kernel families of a batched extension bin.
+enum ExtKind {
+    EXT_UNPACKED = 0,     // one job per lane, any matrix / N / tier (ext_persistent_kernel)
+    EXT_PACKED = 1,       // two jobs per lane, 8-bit tier, items of 128 jobs
+    EXT_LANE = 2,         // one job per lane, 8/16-bit tier, packed-form structure
+};
+
 struct ExtLaunch {
     int tier;
     bool global_state;
-    bool packed;          // two jobs per lane (8-bit tier), items of 128 jobs
+    int kind;             // ExtKind
+    int cls;              // query-length class
+    int job_begin, job_end;   // the bin's range in the batched order
     int cols;
     int item_begin, item_count;
     int grid;
@@ -74,9 +87,14 @@ struct ExtLaunch {
 };
 
 size_t plan_ext_batched(const hipsw_engine *eng, int n, const int32_t *ql, const int32_t *tl,
-                        const int32_t *h0, const int32_t *w, int max_mat,
-                        std::vector<uint32_t> &order, std::vector<WorkItem> &items,
+                        const int32_t *h0, const int32_t *w, const uint8_t *has_n, int max_mat,
+                        bool std_mat, std::vector<uint32_t> &order, std::vector<WorkItem> &items,
                         std::vector<ExtLaunch> &launches);
+// Work items and grid of a bin for its kind.
+void ext_bin_items(const hipsw_engine *eng, ExtLaunch &L, std::vector<WorkItem> &items);
+hipError_t launch_ext_bin(const ExtLaunch &L, const ExtDev &d, const ExtParams &p,
+                          const uint32_t *order, const WorkItem *items, uint32_t *counter,
+                          void *gscratch, hipStream_t s);
 
 }  // namespace hipsw
 
@@ -96,5 +114,6 @@ struct hipsw_ext_set {
     // batched form
     std::vector<hipsw::ExtLaunch> launches;
     size_t batched_scratch = 0;
+    size_t items_cap = 0;            // capacity of d_items (work items)
     int64_t tier_jobs[3] = {0, 0, 0};
 };

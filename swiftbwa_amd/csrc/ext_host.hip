@@ -63,24 +63,29 @@ size_t span(size_t count)
 
 namespace hipsw {
 
-// Batched form: bin by (tier, query-length class); inside a bin order by
-// estimated DP area, largest first, and within an area step of 256 cells by h0,
-// largest first (h0 sets how fast a band's left edge moves, so jobs sharing a
-// wave keep similar bands); cut into work items of 64 jobs (128 for the packed
-// form); one persistent launch per bin.  Returns the global-state scratch
-// bytes the launches need.
+// Batched form: bin by (tier, query-length class, N present); inside a bin
+// order by estimated DP area, largest first, and within an area step of 256
+// cells by h0, largest first (h0 sets how fast a band's left edge moves, so
+// jobs sharing a wave keep similar bands); cut into work items; one persistent
+// launch per bin.  Bins without N, with the standard matrix and queries below
+// 256 bases run the register-query kernels: the packed form for the 8-bit
+// tier (unless the engine's tuning table picks the one-job-per-lane form for
+// that class), the one-job-per-lane form for the 16-bit tier.  Returns the
+// global-state scratch bytes the launches need.
 size_t plan_ext_batched(const hipsw_engine *eng, int n, const int32_t *ql, const int32_t *tl,
-                        const int32_t *h0, const int32_t *w, int max_mat,
-                        std::vector<uint32_t> &order, std::vector<WorkItem> &items,
+                        const int32_t *h0, const int32_t *w, const uint8_t *has_n, int max_mat,
+                        bool std_mat, std::vector<uint32_t> &order, std::vector<WorkItem> &items,
                         std::vector<ExtLaunch> &launches)
 {
     std::vector<uint32_t> keys(n), idx(n);
+    const uint64_t amax = (1ull << 27) - 1;
     for (int k = 0; k < n; ++k) {
-        const uint32_t bin = (uint32_t)(tier_of(h0[k], ql[k], max_mat) * kClasses + class_of(ql[k]));
+        const uint32_t bin = (uint32_t)((tier_of(h0[k], ql[k], max_mat) * kClasses + class_of(ql[k])) * 2 +
+                                        (has_n && has_n[k] ? 1 : 0));
         const uint64_t band = (uint64_t)std::min<int64_t>(ql[k], 2 * (int64_t)std::max(w[k], 0) + 1);
-        const uint64_t area = std::min<uint64_t>((uint64_t)tl[k] * band, (1ull << 28) - 1);
+        const uint64_t area = std::min<uint64_t>((uint64_t)tl[k] * band, amax);
         const uint32_t h0c = (uint32_t)std::min(std::max(h0[k], 0), 255);
-        keys[k] = (bin << 28) | ((uint32_t)(((1ull << 28) - 1) - area) & ~0xFFu) | (255u - h0c);
+        keys[k] = (bin << 27) | ((uint32_t)(amax - area) & ~0xFFu) | (255u - h0c);
         idx[k] = (uint32_t)k;
     }
     radix_sort_by_key(keys, idx);
@@ -90,37 +95,68 @@ size_t plan_ext_batched(const hipsw_engine *eng, int n, const int32_t *ql, const
     items.reserve((size_t)n / 64 + 16);
     size_t scratch_off = 0;
     for (int s = 0; s < n;) {
-        const uint32_t bin = keys[s] >> 28;
+        const uint32_t bin = keys[s] >> 27;
         int e = s, cmax = 0;
-        while (e < n && (keys[e] >> 28) == bin) {
+        while (e < n && (keys[e] >> 27) == bin) {
             cmax = std::max(cmax, ql[idx[e]]);
             ++e;
         }
         ExtLaunch L;
-        L.tier = (int)bin / kClasses;
+        const bool with_n = (bin & 1) != 0;
+        L.tier = (int)(bin >> 1) / kClasses;
+        L.cls = (int)(bin >> 1) % kClasses;
         L.cols = cmax + 1;
-        L.global_state = ext_lds_bytes(L.tier, false, L.cols) > (size_t)eng->lds_per_block;
-        L.packed = L.tier == TIER_U8 && cmax <= 255 &&
-                   ext_pk_lds_bytes(L.cols) <= (size_t)eng->lds_per_block;
-        const int per_item = L.packed ? 128 : 64;
-        L.item_begin = (int)items.size();
-        for (int g = s; g < e; g += per_item)
-            items.push_back(WorkItem{(uint32_t)g, (uint32_t)std::min(per_item, e - g)});
-        L.item_count = (int)items.size() - L.item_begin;
-        int wpc;
-        if (L.packed) {   // no static shared memory: the CU holds whole multiples
-            const size_t per_cu = eng->lds_per_cu > 0 ? (size_t)eng->lds_per_cu : 65536;
-            wpc = (int)std::max<size_t>(1, std::min<size_t>(8, per_cu / ext_pk_lds_bytes(L.cols)));
-        } else {
-            wpc = L.global_state ? 8 : waves_per_cu(eng, ext_lds_bytes(L.tier, false, L.cols));
-        }
-        L.grid = std::max(1, std::min(L.item_count, eng->n_cu * wpc));
+        const bool fast = !with_n && std_mat && cmax <= 255 && L.tier != TIER_I32 &&
+                          ext_pk_lds_bytes(L.cols) <= (size_t)eng->lds_per_block;
+        L.kind = !fast ? EXT_UNPACKED
+                       : L.tier == TIER_U16 ? EXT_LANE
+                                            : eng->tuned_kind[L.cls] > 0 ? eng->tuned_kind[L.cls] : EXT_PACKED;
+        L.global_state = L.kind == EXT_UNPACKED &&
+                         ext_lds_bytes(L.tier, false, L.cols) > (size_t)eng->lds_per_block;
+        L.job_begin = s;
+        L.job_end = e;
+        ext_bin_items(eng, L, items);
         L.scratch_off = scratch_off;
         if (L.global_state) scratch_off += (size_t)L.grid * L.cols * 64 * ext_state_bytes(L.tier);
         launches.push_back(L);
         s = e;
     }
     return scratch_off;
+}
+
+void ext_bin_items(const hipsw_engine *eng, ExtLaunch &L, std::vector<WorkItem> &items)
+{
+    const int per_item = L.kind == EXT_PACKED ? 128 : 64;
+    L.item_begin = (int)items.size();
+    for (int g = L.job_begin; g < L.job_end; g += per_item)
+        items.push_back(WorkItem{(uint32_t)g, (uint32_t)std::min(per_item, L.job_end - g)});
+    L.item_count = (int)items.size() - L.item_begin;
+    const size_t per_cu = eng->lds_per_cu > 0 ? (size_t)eng->lds_per_cu : 65536;
+    int wpc;
+    if (L.kind != EXT_UNPACKED)   // no static shared memory: the CU holds whole multiples
+        wpc = (int)std::max<size_t>(1, std::min<size_t>(8, per_cu / ext_pk_lds_bytes(L.cols)));
+    else
+        wpc = L.global_state ? 8 : waves_per_cu(eng, ext_lds_bytes(L.tier, false, L.cols));
+    L.grid = std::max(1, std::min(L.item_count, eng->n_cu * wpc));
+}
+
+// One bin's launch.
+hipError_t launch_ext_bin(const ExtLaunch &L, const ExtDev &d, const ExtParams &p,
+                          const uint32_t *order, const WorkItem *items, uint32_t *counter,
+                          void *gscratch, hipStream_t s)
+{
+    switch (L.kind) {
+    case EXT_PACKED:
+        return launch_ext_persistent_pk(d, p, order, items + L.item_begin, L.item_count, counter,
+                                        L.cols, L.grid, s);
+    case EXT_LANE:
+        return launch_ext_persistent_ln(d, p, items + L.item_begin, L.item_count, counter, L.cols,
+                                        L.grid, s);
+    default:
+        return launch_ext_persistent(L.tier, L.global_state, d, p, order, items + L.item_begin,
+                                     L.item_count, counter, L.cols, L.grid,
+                                     L.global_state ? (char *)gscratch + L.scratch_off : nullptr, s);
+    }
 }
 
 }  // namespace hipsw
@@ -233,12 +269,18 @@ extern "C" hipsw_ext_set *hipsw_ext_upload(hipsw_engine *eng, int32_t n,
     // Batched form (see plan_ext_batched).
     std::vector<uint32_t> order;
     std::vector<WorkItem> items;
+    std::vector<uint8_t> any_n(n);
+    for (int k = 0; k < n; ++k) any_n[k] = qn_flag[k] | tn_flag[k];
     set->batched_scratch = plan_ext_batched(eng, n, ql.data(), tl.data(), vh0.data(), vw.data(),
-                                            max_mat, order, items, set->launches);
+                                            any_n.data(), max_mat, set->params.std_mat != 0,
+                                            order, items, set->launches);
+    // Work items of any later re-plan (hipsw_ext_autotune) fit: 64 jobs or more per item.
+    const size_t items_cap = (size_t)n / 64 + set->launches.size() + 1;
+    set->items_cap = items_cap;
     // One device allocation for the whole set.
     const size_t bytes = span<uint32_t>(words) + span<uint32_t>(nwords) + 4 * span<uint32_t>(n) +
                          6 * span<int32_t>(n) + span<int32_t>((size_t)n * 6) +
-                         span<uint32_t>(n) + span<WorkItem>(items.size() + 1) + span<PkMeta>(n);
+                         span<uint32_t>(n) + span<WorkItem>(items_cap) + span<PkMeta>(n);
     if (!eng->check(hipMalloc(&set->dmem, bytes), "hipMalloc(ext set)")) {
         delete set;
         return nullptr;
@@ -259,7 +301,7 @@ extern "C" hipsw_ext_set *hipsw_ext_upload(hipsw_engine *eng, int32_t n,
     int32_t *d_unused = carve<int32_t>(cur, std::max(n, 2));   // 8-byte cell counter
     int32_t *d_out = carve<int32_t>(cur, (size_t)n * 6);
     set->d_order = carve<uint32_t>(cur, n);
-    set->d_items = carve<WorkItem>(cur, items.size() + 1);
+    set->d_items = carve<WorkItem>(cur, items_cap);
     PkMeta *d_pk = carve<PkMeta>(cur, n);
     std::vector<PkMeta> pk(n);
     build_pk_meta(n, order.data(), qw.data(), tw.data(), qnw.data(), tnw.data(), ql.data(), tl.data(),
@@ -333,16 +375,9 @@ extern "C" int hipsw_ext_run(hipsw_engine *eng, hipsw_ext_set *set, int32_t mode
                 return HIPSW_E_DEVICE;
             int k = 0;
             for (const ExtLaunch &L : set->launches) {
-                void *lgs = L.global_state ? (char *)gs + L.scratch_off : nullptr;
-                const hipError_t le =
-                    L.packed ? launch_ext_persistent_pk(set->dev, set->params, set->d_order,
-                                                        set->d_items + L.item_begin, L.item_count,
-                                                        eng->counters + k, L.cols, L.grid,
-                                                        eng->stream)
-                             : launch_ext_persistent(L.tier, L.global_state, set->dev, set->params,
-                                                     set->d_order, set->d_items + L.item_begin,
-                                                     L.item_count, eng->counters + k, L.cols,
-                                                     L.grid, lgs, eng->stream);
+                const hipError_t le = launch_ext_bin(L, set->dev, set->params, set->d_order,
+                                                     set->d_items, eng->counters + k, gs,
+                                                     eng->stream);
                 if (!eng->check(le, "launch ext_persistent")) return HIPSW_E_DEVICE;
                 ++k;
                 ++launches;
@@ -416,4 +451,79 @@ extern "C" int hipsw_banded_extend(hipsw_engine *eng, int32_t n, const uint8_t *
     hipsw_ext_free(eng, set);
     if (stats) stats->total_ms = now_ms() - t0;
     return rc;
+}
+
+// Autotuner: for every 8-bit bin of the set that the register-query kernels
+// can run, time the packed form (two jobs per lane, packed 16-bit arithmetic)
+// against the one-job-per-lane form (32-bit arithmetic) on the bin's first
+// `max_jobs` jobs in batched order, keep the faster in the engine's table for
+// that query-length class (later uploads plan with it) and re-plan the set.
+extern "C" int hipsw_ext_autotune(hipsw_engine *eng, hipsw_ext_set *set, int32_t max_jobs,
+                                  hipsw_tune_entry *report, int32_t report_cap,
+                                  int32_t *n_report)
+{
+    if (!eng || !set || max_jobs < 1) return HIPSW_E_INVALID;
+    std::lock_guard<std::recursive_mutex> lock(eng->mu);
+    hipSetDevice(eng->device);
+    if (n_report) *n_report = 0;
+    int nrep = 0;
+    WorkItem *d_tmp = nullptr;
+    const size_t tmp_cap = (size_t)max_jobs / 64 + 2;
+    if (!eng->check(hipMalloc(&d_tmp, tmp_cap * sizeof(WorkItem)), "hipMalloc(tune items)"))
+        return HIPSW_E_DEVICE;
+    int rc = HIPSW_OK;
+    for (ExtLaunch &L : set->launches) {
+        if (L.tier != TIER_U8 || L.kind == EXT_UNPACKED) continue;
+        ExtLaunch S = L;
+        S.job_end = std::min(L.job_end, L.job_begin + max_jobs);
+        double best = 1e30, ms[2] = {0, 0};
+        int winner = L.kind;
+        const int kinds[2] = {EXT_PACKED, EXT_LANE};
+        for (int c = 0; c < 2 && rc == HIPSW_OK; ++c) {
+            ExtLaunch K = S;
+            K.kind = kinds[c];
+            std::vector<WorkItem> items;
+            ext_bin_items(eng, K, items);
+            K.item_begin = 0;
+            if (!eng->check(hipMemcpy(d_tmp, items.data(), items.size() * sizeof(WorkItem),
+                                      hipMemcpyHostToDevice), "tune items")) {
+                rc = HIPSW_E_DEVICE;
+                break;
+            }
+            float t = 1e30f;
+            for (int rep = 0; rep < 2; ++rep) {     // the first run warms caches and code
+                if (!eng->check(launch_zero_u32(eng->counters, 1, eng->stream), "zero") ||
+                    !eng->check(hipEventRecord(eng->ev0, eng->stream), "event") ||
+                    !eng->check(launch_ext_bin(K, set->dev, set->params, set->d_order, d_tmp,
+                                               eng->counters, nullptr, eng->stream), "tune launch") ||
+                    !eng->check(hipEventRecord(eng->ev1, eng->stream), "event") ||
+                    !eng->check(hipEventSynchronize(eng->ev1), "tune sync")) {
+                    rc = HIPSW_E_DEVICE;
+                    break;
+                }
+                float e = 0;
+                hipEventElapsedTime(&e, eng->ev0, eng->ev1);
+                t = std::min(t, e);
+            }
+            ms[c] = t;
+            if (t < best) { best = t; winner = kinds[c]; }
+        }
+        if (rc != HIPSW_OK) break;
+        L.kind = winner;
+        eng->tuned_kind[L.cls] = winner;
+        if (report && nrep < report_cap)
+            report[nrep] = hipsw_tune_entry{L.tier, L.cls, S.job_end - S.job_begin, winner, ms[0], ms[1]};
+        ++nrep;
+    }
+    hipFree(d_tmp);
+    if (rc != HIPSW_OK) return rc;
+    // re-plan the set's work items with the chosen kinds
+    std::vector<WorkItem> items;
+    for (ExtLaunch &L : set->launches) ext_bin_items(eng, L, items);
+    if (items.size() > set->items_cap) return HIPSW_E_UNSUPPORTED;
+    if (!eng->check(hipMemcpy(set->d_items, items.data(), items.size() * sizeof(WorkItem),
+                              hipMemcpyHostToDevice), "tune re-plan"))
+        return HIPSW_E_DEVICE;
+    if (n_report) *n_report = nrep;
+    return HIPSW_OK;
 }

@@ -629,9 +629,12 @@ __device__ __forceinline__ uint32_t mismatch16(uint32_t qw, uint32_t tpat)
 // Column state layout: groups of four columns, lane-major inside a group, so
 // one 128-bit LDS access moves four columns of a lane (a 16-column block is
 // four reads and four writes).  Word (j, lane) is u32 number
-// (j / 4) * 256 + lane * 4 + j % 4; cpos(j) is the u16 offset of column j from
-// a lane's half pointer.
-__device__ __forceinline__ int cpos(int j) { return (j >> 2) * 512 + (j & 3) * 2; }
+// (j / 4) * 256 + lane * 4 + j % 4.  A job's (h, e) entry of column j sits at
+// cpos<T>(j) elements from the job's pointer: T = u16 for the packed form (one
+// 16-bit half of the word), T = u32 for the one-job-per-lane form.
+template <typename T> __device__ __forceinline__ int cpos(int j);
+template <> __device__ __forceinline__ int cpos<uint16_t>(int j) { return (j >> 2) * 512 + (j & 3) * 2; }
+template <> __device__ __forceinline__ int cpos<uint32_t>(int j) { return (j >> 2) * 256 + (j & 3); }
 
 // Per-job row-level state of one half.
 struct PkJob {
@@ -673,10 +676,11 @@ __device__ __forceinline__ void pk_setup(PkJob &J, const ExtDev &d, const ExtPar
     J.tb = 0; J.ncells = 0;
 }
 
-// Row start of one half (:92-107); returns h1.  `c16` is this half's column
-// state (u16 (h, e) of column j at c16[cpos(j)]).  The caller keeps the target
+// Row start of one job (:92-107); returns h1.  `c16` is the job's column
+// state ((h, e) of column j at c16[cpos<T>(j)]).  The caller keeps the target
 // words of the current and the next 16-row block loaded.
-__device__ __forceinline__ int pk_row_start(PkJob &J, const ExtParams &p, int i, uint16_t *c16)
+template <typename T>
+__device__ __forceinline__ int pk_row_start(PkJob &J, const ExtParams &p, int i, T *c16)
 {
     J.alive = J.alive && i < J.tlen;
     J.tb = (((i >> 4) & 1 ? J.two : J.twe) >> ((i & 15) * 2)) & 3u;   // this row's target base
@@ -684,7 +688,7 @@ __device__ __forceinline__ int pk_row_start(PkJob &J, const ExtParams &p, int i,
     if (J.alive && J.beg < wb) {
         // beg = i - w (:96) drops column beg: with w >= 0, beg >= i - 1 - w
         // after the previous row, so exactly one column per row; clear it
-        c16[cpos(J.beg)] = 0;
+        c16[cpos<T>(J.beg)] = 0;
         J.beg = wb;
     }
     J.end = min(min(J.end, i + J.w + 1), J.qlen);             // :97-98
@@ -695,10 +699,11 @@ __device__ __forceinline__ int pk_row_start(PkJob &J, const ExtParams &p, int i,
 // Row end of one half (:145-172): band-end entry, global-score column, row
 // maximum and z-drop.  A finished job's half is never read again, so its
 // band-end store needs no guard (end <= qlen keeps it in bounds).
+template <typename T>
 __device__ __forceinline__ void pk_row_end(PkJob &J, const ExtParams &p, int i, int h1, int key,
-                                           uint16_t *c16)
+                                           T *c16)
 {
-    c16[cpos(J.end)] = (uint16_t)h1;                           // h[end] = h1, e[end] = 0
+    c16[cpos<T>(J.end)] = (T)h1;                               // h[end] = h1, e[end] = 0
     const int rm = key >> 8, rj = key & 0xFF;
     const bool at_q = J.alive && J.end == J.qlen;              // :149-152
     J.gtle = (at_q && !(J.gsc > h1)) ? i : J.gtle;
@@ -716,8 +721,8 @@ __device__ __forceinline__ void pk_row_end(PkJob &J, const ExtParams &p, int i, 
 
 // Band narrowing of both halves (:175-180).  Each trip reads four columns per
 // half at once (one LDS round trip); more trips only while all four are zero.
-__device__ __forceinline__ void pk_narrow(PkJob &A, PkJob &B, int h1a, int h1b, uint16_t *ca,
-                                          uint16_t *cb)
+template <typename T>
+__device__ __forceinline__ void pk_narrow(PkJob &A, PkJob &B, int h1a, int h1b, T *ca, T *cb)
 {
     // first column in [beg, end) whose (h, e) is non-zero, else end
     int ja = A.beg, jb = B.beg;
@@ -727,8 +732,8 @@ __device__ __forceinline__ void pk_narrow(PkJob &A, PkJob &B, int h1a, int h1b, 
         uint32_t va[4], vb[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            va[k] = ca[cpos(min(ja + k, ea))];
-            vb[k] = cb[cpos(min(jb + k, eb))];
+            va[k] = ca[cpos<T>(min(ja + k, ea))];
+            vb[k] = cb[cpos<T>(min(jb + k, eb))];
         }
         bool za = ga && va[0] == 0, zb = gb && vb[0] == 0;
         int na = za, nb = zb;
@@ -758,8 +763,8 @@ __device__ __forceinline__ void pk_narrow(PkJob &A, PkJob &B, int h1a, int h1b, 
         uint32_t va[4], vb[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            va[k] = ca[cpos(max(ka - k, 0))];
-            vb[k] = cb[cpos(max(kb - k, 0))];
+            va[k] = ca[cpos<T>(max(ka - k, 0))];
+            vb[k] = cb[cpos<T>(max(kb - k, 0))];
         }
         bool za = sa && va[0] == 0, zb = sb && vb[0] == 0;
         int na = za, nb = zb;
@@ -977,42 +982,193 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
     pk_finish(B, d);
 }
 
-// Persistent kernel of the packed form.  Items hold up to 128 jobs; an item
-// whose jobs include an N base runs as two generic 64-job halves instead.  The
-// block's LDS is exactly the column state (no static shared memory), so four
-// waves of 150-bp queries fill a 160 KiB CU.
+// Extends jobs meta[0..count) (count <= 64) with one wave, one per lane, for
+// the 8- and 16-bit tiers: the column word is (h | e << 16) in 16-bit fields
+// and the arithmetic 32-bit.  Same structure as the packed form (band edges,
+// query masks in registers, three passes per block, 4-column LDS groups, two
+// word sets in turn); requirements as there with qmax <= 16 * NQ - 1 and
+// scores below 65536.
 template <bool SYM, int NQ>
-__global__ __launch_bounds__(64) void ext_persistent_pk_kernel(ExtDev d, ExtParams p,
-                                                               const uint32_t *order,
+__device__ void ext_wave_ln(const ExtDev &d, const ExtParams &p, const PkMeta *meta, int count,
+                            uint32_t *col)
+{
+    const int lane = threadIdx.x & 63;
+    PkJob A, B;
+    pk_setup(A, d, p, meta, lane, count);
+    B.valid = B.nonempty = B.alive = false;                    // no second job: narrowing skips it
+    B.beg = B.end = B.qlen = 0;
+    uint32_t NA01[NQ], NA23[NQ];
+#pragma unroll
+    for (int b = 0; b < NQ; ++b) {
+        const uint32_t qa = b * 16 < A.qlen ? d.seq[A.qw0 + b] : 0u;
+        NA01[b] = mismatch16(qa, 0u) | (mismatch16(qa, 0x55555555u) << 16);
+        NA23[b] = mismatch16(qa, 0xAAAAAAAAu) | (mismatch16(qa, 0xFFFFFFFFu) << 16);
+    }
+    const int qmax = uniform_max(A.alive ? A.qlen : 0);
+    for (int j = 0; j <= qmax; ++j)                            // first row, :68-77
+        col[cpos<uint32_t>(j)] = (A.alive && j <= A.qlen)
+                                     ? (uint32_t)(j == 0 ? A.h0 : max(0, A.h0 - p.o_ins - p.e_ins * j))
+                                     : 0u;
+    const int sA = p.match_sc, nAB = p.mis_sc - p.match_sc;
+    const uint32_t eD = p.e_del, eI = p.e_ins, oeD = p.oe_del, oeI = p.oe_ins;
+    const uint32_t ONE = opaque(1u);
+    const int BIG = 0x7fffffff;
+
+    for (int i = 0;; ++i) {
+        if (i > 0 && (i & 15) == 0) {
+            const int nb = (i >> 4) + 1;
+            if (nb & 1) A.two = d.seq[A.tw0 + min(nb, A.tnw)];
+            else A.twe = d.seq[A.tw0 + min(nb, A.tnw)];
+        }
+        uint32_t h1 = (uint32_t)pk_row_start<uint32_t>(A, p, i, col);
+        if (__ballot(A.alive) == 0) break;
+        uint32_t f = 0, key = 0;
+        const int jlo = uniform_min(A.alive ? A.beg : BIG);
+        const int jhi = uniform_max(A.alive ? A.end : 0);
+        const int mhi = uniform_min(A.alive ? A.end : BIG);
+        const int L = A.alive ? A.end : 0;
+        const bool hiA = (A.tb & 2u) != 0;
+        const uint32_t shA = (A.tb & 1u) * 16;
+
+        auto block = [&](uint32_t (&wv)[16], int b) {
+            const uint32_t a01 = NA01[b], a23 = NA23[b];
+            const uint32_t mm = (hiA ? a23 : a01) >> shA;          // bit k: base c0 + k differs
+            const int c0 = b * 16;
+            uint32_t dgE[16], tf[16], ne[16], Hs[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const uint32_t M = wv[k] & 0xFFFFu, E = wv[k] >> 16;
+                const int s = (int)((mm >> k) & 1u) * nAB + sA;
+                const uint32_t dg = (uint32_t)max(s * (int)min(M, ONE) + (int)M, 0);   // :117-119
+                dgE[k] = max(dg, E);
+                const uint32_t t = __builtin_elementwise_sub_sat(dg, oeD);
+                ne[k] = max(__builtin_elementwise_sub_sat(E, eD), t);
+                tf[k] = SYM ? t : __builtin_elementwise_sub_sat(dg, oeI);
+            }
+            if (c0 + 16 <= mhi) {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const uint32_t H = max(dgE[k], f);
+                    f = max(__builtin_elementwise_sub_sat(f, eI), tf[k]);
+                    wv[k] = h1 | (ne[k] << 16);
+                    h1 = H;
+                    Hs[k] = (H << 8) | (uint32_t)k;
+                }
+            } else {
+                const int lb = L - c0;                           // cells k < lb commit
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const bool in = k < lb;
+                    const uint32_t H = max(dgE[k], f);
+                    f = max(__builtin_elementwise_sub_sat(f, eI), tf[k]);
+                    wv[k] = in ? (h1 | (ne[k] << 16)) : wv[k];
+                    h1 = in ? H : h1;
+                    Hs[k] = in ? ((H << 8) | (uint32_t)k) : 0u;
+                }
+            }
+            uint4 *cb = reinterpret_cast<uint4 *>(col + b * 1024);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                cb[q * 64] = uint4{wv[4 * q], wv[4 * q + 1], wv[4 * q + 2], wv[4 * q + 3]};
+#pragma unroll
+            for (int k = 0; k < 8; ++k) Hs[k] = max(Hs[k], Hs[k + 8]);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) Hs[k] = max(Hs[k], Hs[k + 4]);
+            key = max(key, max(max(Hs[0], Hs[2]), max(Hs[1], Hs[3])) + (uint32_t)c0);
+        };
+        auto load = [&](uint32_t (&wv)[16], int b) {
+            const uint4 *cb = reinterpret_cast<const uint4 *>(col + b * 1024);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint4 v = cb[q * 64];
+                wv[4 * q] = v.x; wv[4 * q + 1] = v.y; wv[4 * q + 2] = v.z; wv[4 * q + 3] = v.w;
+            }
+        };
+        if (jlo < jhi) {
+            const int bb = jlo >> 4, be = (jhi - 1) >> 4;
+            uint32_t W0[16], W1[16];
+            load(W0, bb);
+            for (int b = bb;; b += 2) {
+                load(W1, min(b + 1, be));
+                block(W0, b);
+                if (b + 1 > be) break;
+                load(W0, min(b + 2, be));
+                block(W1, b + 1);
+                if (b + 2 > be) break;
+            }
+        }
+        pk_row_end<uint32_t>(A, p, i, (int)h1, (int)key, col);
+        pk_narrow<uint32_t>(A, B, (int)h1, 1, col, col);
+    }
+    if (d.cells) {
+        unsigned long long c = A.ncells;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+        if (lane == 0) atomicAdd(d.cells, c);
+    }
+    pk_finish(A, d);
+}
+
+// Persistent kernel of the one-job-per-lane form: items of up to 64 jobs, no
+// N base (the planner bins N-holding jobs apart).
+template <bool SYM, int NQ>
+__global__ __launch_bounds__(64) void ext_persistent_ln_kernel(ExtDev d, ExtParams p,
                                                                const WorkItem *items, int n_items,
-                                                               uint32_t *counter, int cols)
+                                                               uint32_t *counter)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     uint32_t *col = reinterpret_cast<uint32_t *>(lds_raw) + threadIdx.x * 4;
-    const int lane = threadIdx.x;
     for (;;) {
         int item = 0;
-        if (lane == 0) item = (int)atomicAdd(counter, 1u);
+        if (threadIdx.x == 0) item = (int)atomicAdd(counter, 1u);
         item = __builtin_amdgcn_readfirstlane(item);
         if (item >= n_items) break;
         const WorkItem wi = items[item];
-        const uint32_t *idx = order + wi.start;
-        const PkMeta *meta = d.pk + wi.start;
-        bool has_n = false;
-        for (int k = lane; k < (int)wi.count; k += 64) has_n = has_n || meta[k].has_n != 0;
-        if (__ballot(has_n) == 0) {
-            ext_wave_pk<SYM, NQ>(d, p, meta, (int)wi.count, col);
-        } else {
-            // generic 8-bit form: u16 columns first, the score matrix after them
-            int8_t *smat = reinterpret_cast<int8_t *>(lds_raw) + (size_t)cols * 128;
-            if (lane < 32) smat[lane] = p.mat[lane];
-            __syncthreads();
-            uint16_t *c16 = reinterpret_cast<uint16_t *>(lds_raw) + lane;
-            ext_wave_generic<TIER_U8>(d, p, idx, min(64, (int)wi.count), c16, smat);
-            if (wi.count > 64)
-                ext_wave_generic<TIER_U8>(d, p, idx + 64, (int)wi.count - 64, c16, smat);
-        }
-        __syncthreads();
+        ext_wave_ln<SYM, NQ>(d, p, d.pk + wi.start, (int)wi.count, col);
+    }
+}
+
+hipError_t launch_ext_persistent_ln(const ExtDev &d, const ExtParams &p, const WorkItem *items,
+                                    int n_items, uint32_t *counter, int cols, int grid,
+                                    hipStream_t s)
+{
+    if (n_items <= 0) return hipSuccess;
+    if (cols < 1 || cols > 256) return hipErrorInvalidValue;
+    const bool sym = p.o_del == p.o_ins && p.e_del == p.e_ins;
+    const int nq = (cols + 15) / 16;
+    const size_t lds = ext_pk_lds_bytes(cols);
+#define HIPSW_LN_LAUNCH(S, Q)                                                                   \
+    hipLaunchKernelGGL((ext_persistent_ln_kernel<S, Q>), dim3(grid), dim3(64), lds, s, d, p,   \
+                       items, n_items, counter)
+    if (nq <= 4) {
+        if (sym) HIPSW_LN_LAUNCH(true, 4); else HIPSW_LN_LAUNCH(false, 4);
+    } else if (nq <= 10) {
+        if (sym) HIPSW_LN_LAUNCH(true, 10); else HIPSW_LN_LAUNCH(false, 10);
+    } else {
+        if (sym) HIPSW_LN_LAUNCH(true, 16); else HIPSW_LN_LAUNCH(false, 16);
+    }
+#undef HIPSW_LN_LAUNCH
+    return hipGetLastError();
+}
+
+// Persistent kernel of the packed form: items of up to 128 jobs, no N base
+// (the planner bins N-holding jobs apart).  The block's LDS is exactly the
+// column state (no static shared memory), so four waves of 150-bp queries fill
+// a 160 KiB CU.
+template <bool SYM, int NQ>
+__global__ __launch_bounds__(64) void ext_persistent_pk_kernel(ExtDev d, ExtParams p,
+                                                               const WorkItem *items, int n_items,
+                                                               uint32_t *counter)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    uint32_t *col = reinterpret_cast<uint32_t *>(lds_raw) + threadIdx.x * 4;
+    for (;;) {
+        int item = 0;
+        if (threadIdx.x == 0) item = (int)atomicAdd(counter, 1u);
+        item = __builtin_amdgcn_readfirstlane(item);
+        if (item >= n_items) break;
+        const WorkItem wi = items[item];
+        ext_wave_pk<SYM, NQ>(d, p, d.pk + wi.start, (int)wi.count, col);
     }
 }
 
@@ -1041,7 +1197,7 @@ hipError_t launch_ext_persistent_pk(const ExtDev &d, const ExtParams &p, const u
     const size_t lds = ext_pk_lds_bytes(cols);
 #define HIPSW_PK_LAUNCH(S, Q)                                                                   \
     hipLaunchKernelGGL((ext_persistent_pk_kernel<S, Q>), dim3(grid), dim3(64), lds, s, d, p,   \
-                       order, items, n_items, counter, cols)
+                       items, n_items, counter)
     if (nq <= 4) {
         if (sym) HIPSW_PK_LAUNCH(true, 4); else HIPSW_PK_LAUNCH(false, 4);
     } else if (nq <= 10) {

@@ -85,4 +85,9 @@ void build_pk_meta(int n, const uint32_t *order, const uint32_t *q_word, const u
 hipError_t launch_ext_persistent_pk(const ExtDev &d, const ExtParams &p, const uint32_t *order,
                                     const WorkItem *items, int n_items, uint32_t *counter,
                                     int cols, int grid, hipStream_t s);
+// One-job-per-lane form (8- and 16-bit tiers, no N, standard matrix, items of
+// <= 64 jobs, descriptors in d.pk); same LDS size as the packed form.
+hipError_t launch_ext_persistent_ln(const ExtDev &d, const ExtParams &p, const WorkItem *items,
+                                    int n_items, uint32_t *counter, int cols, int grid,
+                                    hipStream_t s);
 }  // namespace hipsw

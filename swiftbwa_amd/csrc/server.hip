@@ -183,9 +183,14 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
     std::vector<WorkItem> ext_items;
     std::vector<ExtLaunch> ext_launch;
     size_t ext_scratch = 0;
-    if (r.n_ext)
+    if (r.n_ext) {
+        // windows come from the .pac (no N); queries may hold N
+        std::vector<uint8_t> any_n(r.n_ext);
+        for (int k = 0; k < r.n_ext; ++k) any_n[k] = qnw[k] != kNoN || tnw[k] != kNoN;
         ext_scratch = plan_ext_batched(eng, r.n_ext, ql.data(), tl.data(), r.h0.data(), r.w.data(),
-                                       ep.max_mat, ext_order, ext_items, ext_launch);
+                                       any_n.data(), ep.max_mat, ep.std_mat != 0, ext_order,
+                                       ext_items, ext_launch);
+    }
     // local SW split
     std::vector<uint32_t> loc_s, loc_c;
     int loc_qmax = 0, loc_tmax = 1;
@@ -271,13 +276,7 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
         int k = 0;
         for (const ExtLaunch &L : ext_launch) {
             if (!ok || k >= 64) { ok = false; break; }
-            ok = (L.packed ? launch_ext_persistent_pk(ed, ep, d_order, d_items + L.item_begin,
-                                                      L.item_count, sl.counters + k, L.cols, L.grid, st)
-                           : launch_ext_persistent(L.tier, L.global_state, ed, ep, d_order,
-                                                   d_items + L.item_begin, L.item_count,
-                                                   sl.counters + k, L.cols, L.grid,
-                                                   L.global_state ? (char *)gs + L.scratch_off : nullptr,
-                                                   st)) == hipSuccess;
+            ok = launch_ext_bin(L, ed, ep, d_order, d_items, sl.counters + k, gs, st) == hipSuccess;
             ++k;
         }
         if (gs) hipFreeAsync(gs, st);

@@ -56,6 +56,11 @@ class HipswPacRef(C.Structure):
     _fields_ = [("pac", C.c_void_p), ("genome_len", C.c_int64)]
 
 
+class HipswTuneEntry(C.Structure):
+    _fields_ = [("tier", C.c_int32), ("length_class", C.c_int32), ("jobs", C.c_int32),
+                ("chosen", C.c_int32), ("ms_packed", C.c_double), ("ms_lane", C.c_double)]
+
+
 class HipswChunk(C.Structure):
     _P = C.c_void_p
     _fields_ = [("n_ext", C.c_int32), ("ext_q", _P), ("ext_q_off", _P), ("ext_q_len", _P),
@@ -103,6 +108,7 @@ _SIGS = {
     "hipsw_extend_chains": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                       _P, C.POINTER(HipswScoring), C.POINTER(HipswExtOpts), _P,
                                       _P, C.POINTER(HipswStats)]),
+    "hipsw_ext_autotune": (C.c_int, [_P, _P, C.c_int32, _P, C.c_int32, _P]),
     "hipsw_nw_upload": (_P, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P, _P, C.c_int32,
                              C.c_int32, C.POINTER(HipswScoring)]),
     "hipsw_nw_run": (C.c_int, [_P, _P, C.c_int32, C.POINTER(HipswStats)]),

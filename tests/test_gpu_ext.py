@@ -141,3 +141,44 @@ def test_full_size_uniform_properties(engine):
     sub = wl.slice_jobs(js, np.sort(idx))
     ref, _ = orc.ksw_extend_batch(sub, nthreads=8)
     assert np.array_equal(a[np.sort(idx)], ref)
+
+
+def test_autotune_both_layouts_exact(engine):
+    """The autotuner times the packed and the one-job-per-lane layouts on each
+    8-bit bin; the lane layout runs last, so the sampled jobs' results on the
+    device right after tuning are its results.  Both must equal the oracle, and
+    the re-planned set must give the same results again."""
+    import ctypes as C
+    from swiftbwa_amd.metalsw import _ext_arrays
+    js = wl.generate(wl.EXT_MIXED, 7, 40000)
+    sp = ScoringParameters()
+    ref, _ = orc.ksw_extend_batch(js, oracle_sc(sp), nthreads=8)
+    lib = engine.lib
+    a = _ext_arrays(js)
+    sc = sp.to_c()
+    st = lib.hipsw_ext_upload(engine.handle, js["n"], native.ptr(a["queries"]),
+                              native.ptr(a["q_off"]), native.ptr(a["q_len"]),
+                              native.ptr(a["targets"]), native.ptr(a["t_off"]),
+                              native.ptr(a["t_len"]), native.ptr(a["h0"]), native.ptr(a["w"]),
+                              native.ptr(a["end_bonus"]), C.byref(sc))
+    assert st
+    try:
+        s = native.HipswStats()
+        out = np.zeros((js["n"], 6), np.int32)
+        assert lib.hipsw_ext_run(engine.handle, st, native.MODE_BATCHED, C.byref(s)) == 0
+        assert lib.hipsw_ext_download(engine.handle, st, native.ptr(out)) == 0
+        assert np.array_equal(out, ref)
+        rep = (native.HipswTuneEntry * 8)()
+        nrep = C.c_int32()
+        assert lib.hipsw_ext_autotune(engine.handle, st, 6000, rep, 8, C.byref(nrep)) == 0
+        assert 1 <= nrep.value <= 3
+        for e in rep[:nrep.value]:
+            assert e.tier == 0 and e.chosen in (1, 2) and e.jobs > 0
+            assert e.ms_packed > 0 and e.ms_lane > 0
+        assert lib.hipsw_ext_download(engine.handle, st, native.ptr(out)) == 0
+        assert np.array_equal(out, ref)                       # lane-layout results
+        assert lib.hipsw_ext_run(engine.handle, st, native.MODE_BATCHED, C.byref(s)) == 0
+        assert lib.hipsw_ext_download(engine.handle, st, native.ptr(out)) == 0
+        assert np.array_equal(out, ref)                       # re-planned set
+    finally:
+        lib.hipsw_ext_free(engine.handle, st)
