@@ -83,7 +83,8 @@ struct ExtLaunch {
     int cols;
     int item_begin, item_count;
     int grid;
-    size_t scratch_off;   // bytes into the engine scratch (global_state only)
+    size_t scratch_off;   // bytes into the engine scratch
+    size_t scratch_bytes; // HBM DP state of the launch (0: all in LDS)
 };
 
 size_t plan_ext_batched(const hipsw_engine *eng, int n, const int32_t *ql, const int32_t *tl,

@@ -117,7 +117,15 @@ size_t plan_ext_batched(const hipsw_engine *eng, int n, const int32_t *ql, const
         L.job_end = e;
         ext_bin_items(eng, L, items);
         L.scratch_off = scratch_off;
-        if (L.global_state) scratch_off += (size_t)L.grid * L.cols * 64 * ext_state_bytes(L.tier);
+        if (L.kind == EXT_UNPACKED) {
+            L.scratch_bytes = L.global_state ? (size_t)L.grid * L.cols * 64 * ext_state_bytes(L.tier) : 0;
+        } else {
+            // every wave the CU can hold gets a region, so a re-plan's grid fits too
+            const size_t per_cu = eng->lds_per_cu > 0 ? (size_t)eng->lds_per_cu : 65536;
+            const size_t wpc = std::max<size_t>(1, std::min<size_t>(8, per_cu / ext_pk_lds_bytes(L.cols)));
+            L.scratch_bytes = (size_t)eng->n_cu * wpc * ext_pk_hbm_bytes_per_wave(L.cols);
+        }
+        scratch_off += (L.scratch_bytes + 255) & ~(size_t)255;
         launches.push_back(L);
         s = e;
     }
@@ -145,17 +153,17 @@ hipError_t launch_ext_bin(const ExtLaunch &L, const ExtDev &d, const ExtParams &
                           const uint32_t *order, const WorkItem *items, uint32_t *counter,
                           void *gscratch, hipStream_t s)
 {
+    void *gs = L.scratch_bytes ? (char *)gscratch + L.scratch_off : nullptr;
     switch (L.kind) {
     case EXT_PACKED:
-        return launch_ext_persistent_pk(d, p, order, items + L.item_begin, L.item_count, counter,
-                                        L.cols, L.grid, s);
+        return launch_ext_persistent_pk(d, p, items + L.item_begin, L.item_count, counter, L.cols,
+                                        L.grid, gs, s);
     case EXT_LANE:
         return launch_ext_persistent_ln(d, p, items + L.item_begin, L.item_count, counter, L.cols,
-                                        L.grid, s);
+                                        L.grid, gs, s);
     default:
         return launch_ext_persistent(L.tier, L.global_state, d, p, order, items + L.item_begin,
-                                     L.item_count, counter, L.cols, L.grid,
-                                     L.global_state ? (char *)gscratch + L.scratch_off : nullptr, s);
+                                     L.item_count, counter, L.cols, L.grid, gs, s);
     }
 }
 
@@ -472,6 +480,11 @@ extern "C" int hipsw_ext_autotune(hipsw_engine *eng, hipsw_ext_set *set, int32_t
     if (!eng->check(hipMalloc(&d_tmp, tmp_cap * sizeof(WorkItem)), "hipMalloc(tune items)"))
         return HIPSW_E_DEVICE;
     int rc = HIPSW_OK;
+    void *gs = nullptr;
+    if (set->batched_scratch && !(gs = eng->scratch_get(set->batched_scratch))) {
+        hipFree(d_tmp);
+        return HIPSW_E_DEVICE;
+    }
     for (ExtLaunch &L : set->launches) {
         if (L.tier != TIER_U8 || L.kind == EXT_UNPACKED) continue;
         ExtLaunch S = L;
@@ -495,7 +508,7 @@ extern "C" int hipsw_ext_autotune(hipsw_engine *eng, hipsw_ext_set *set, int32_t
                 if (!eng->check(launch_zero_u32(eng->counters, 1, eng->stream), "zero") ||
                     !eng->check(hipEventRecord(eng->ev0, eng->stream), "event") ||
                     !eng->check(launch_ext_bin(K, set->dev, set->params, set->d_order, d_tmp,
-                                               eng->counters, nullptr, eng->stream), "tune launch") ||
+                                               eng->counters, gs, eng->stream), "tune launch") ||
                     !eng->check(hipEventRecord(eng->ev1, eng->stream), "event") ||
                     !eng->check(hipEventSynchronize(eng->ev1), "tune sync")) {
                     rc = HIPSW_E_DEVICE;

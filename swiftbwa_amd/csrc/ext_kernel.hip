@@ -636,6 +636,22 @@ template <typename T> __device__ __forceinline__ int cpos(int j);
 template <> __device__ __forceinline__ int cpos<uint16_t>(int j) { return (j >> 2) * 512 + (j & 3) * 2; }
 template <> __device__ __forceinline__ int cpos<uint32_t>(int j) { return (j >> 2) * 256 + (j & 3); }
 
+// Queries longer than 159 bases (HYB): columns below kLdsCols stay in LDS and
+// the rest live in the wave's own HBM region (same group layout, L2-resident:
+// 24 KiB per wave), so such a wave still takes 40 KiB of LDS and four fill a
+// CU.  ColRef is a job's (h, e) entry of column j, wherever it lives.
+constexpr int kLdsCols = 160;
+// u32 words of one wave's HBM region: columns kLdsCols..255, 64 lanes
+__host__ __device__ constexpr size_t ext_hyb_words() { return (size_t)(256 - kLdsCols) * 64; }
+template <typename T, bool HYB> struct ColRef {
+    T *l, *g;
+    __device__ __forceinline__ T &operator[](int j) const
+    {
+        if (HYB && j >= kLdsCols) return g[cpos<T>(j - kLdsCols)];
+        return l[cpos<T>(j)];
+    }
+};
+
 // Per-job row-level state of one half.
 struct PkJob {
     bool valid, nonempty, alive;
@@ -679,8 +695,9 @@ __device__ __forceinline__ void pk_setup(PkJob &J, const ExtDev &d, const ExtPar
 // Row start of one job (:92-107); returns h1.  `c16` is the job's column
 // state ((h, e) of column j at c16[cpos<T>(j)]).  The caller keeps the target
 // words of the current and the next 16-row block loaded.
-template <typename T>
-__device__ __forceinline__ int pk_row_start(PkJob &J, const ExtParams &p, int i, T *c16)
+template <typename T, bool HYB>
+__device__ __forceinline__ int pk_row_start(PkJob &J, const ExtParams &p, int i,
+                                            const ColRef<T, HYB> &c16)
 {
     J.alive = J.alive && i < J.tlen;
     J.tb = (((i >> 4) & 1 ? J.two : J.twe) >> ((i & 15) * 2)) & 3u;   // this row's target base
@@ -688,7 +705,7 @@ __device__ __forceinline__ int pk_row_start(PkJob &J, const ExtParams &p, int i,
     if (J.alive && J.beg < wb) {
         // beg = i - w (:96) drops column beg: with w >= 0, beg >= i - 1 - w
         // after the previous row, so exactly one column per row; clear it
-        c16[cpos<T>(J.beg)] = 0;
+        c16[J.beg] = 0;
         J.beg = wb;
     }
     J.end = min(min(J.end, i + J.w + 1), J.qlen);             // :97-98
@@ -699,11 +716,11 @@ __device__ __forceinline__ int pk_row_start(PkJob &J, const ExtParams &p, int i,
 // Row end of one half (:145-172): band-end entry, global-score column, row
 // maximum and z-drop.  A finished job's half is never read again, so its
 // band-end store needs no guard (end <= qlen keeps it in bounds).
-template <typename T>
+template <typename T, bool HYB>
 __device__ __forceinline__ void pk_row_end(PkJob &J, const ExtParams &p, int i, int h1, int key,
-                                           T *c16)
+                                           const ColRef<T, HYB> &c16)
 {
-    c16[cpos<T>(J.end)] = (T)h1;                               // h[end] = h1, e[end] = 0
+    c16[J.end] = (T)h1;                                        // h[end] = h1, e[end] = 0
     const int rm = key >> 8, rj = key & 0xFF;
     const bool at_q = J.alive && J.end == J.qlen;              // :149-152
     J.gtle = (at_q && !(J.gsc > h1)) ? i : J.gtle;
@@ -721,8 +738,9 @@ __device__ __forceinline__ void pk_row_end(PkJob &J, const ExtParams &p, int i, 
 
 // Band narrowing of both halves (:175-180).  Each trip reads four columns per
 // half at once (one LDS round trip); more trips only while all four are zero.
-template <typename T>
-__device__ __forceinline__ void pk_narrow(PkJob &A, PkJob &B, int h1a, int h1b, T *ca, T *cb)
+template <typename T, bool HYB>
+__device__ __forceinline__ void pk_narrow(PkJob &A, PkJob &B, int h1a, int h1b,
+                                          const ColRef<T, HYB> &ca, const ColRef<T, HYB> &cb)
 {
     // first column in [beg, end) whose (h, e) is non-zero, else end
     int ja = A.beg, jb = B.beg;
@@ -732,8 +750,8 @@ __device__ __forceinline__ void pk_narrow(PkJob &A, PkJob &B, int h1a, int h1b, 
         uint32_t va[4], vb[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            va[k] = ca[cpos<T>(min(ja + k, ea))];
-            vb[k] = cb[cpos<T>(min(jb + k, eb))];
+            va[k] = ca[min(ja + k, ea)];
+            vb[k] = cb[min(jb + k, eb)];
         }
         bool za = ga && va[0] == 0, zb = gb && vb[0] == 0;
         int na = za, nb = zb;
@@ -763,8 +781,8 @@ __device__ __forceinline__ void pk_narrow(PkJob &A, PkJob &B, int h1a, int h1b, 
         uint32_t va[4], vb[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            va[k] = ca[cpos<T>(max(ka - k, 0))];
-            vb[k] = cb[cpos<T>(max(kb - k, 0))];
+            va[k] = ca[max(ka - k, 0)];
+            vb[k] = cb[max(kb - k, 0)];
         }
         bool za = sa && va[0] == 0, zb = sb && vb[0] == 0;
         int na = za, nb = zb;
@@ -782,6 +800,52 @@ __device__ __forceinline__ void pk_narrow(PkJob &A, PkJob &B, int h1a, int h1b, 
     }
     if (A.alive) A.end = min(ka + 2, A.qlen);
     if (B.alive) B.end = min(kb + 2, B.qlen);
+}
+
+// The 16 column words of block b of this lane: four 128-bit accesses to LDS,
+// or to the wave's HBM region for blocks from kLdsCols on (HYB).  The pointers
+// carry their address space so that each path keeps its own instructions
+// (ds_*_b128 / global_*_dwordx4) instead of generic flat accesses.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) u32x4 lds_u4;
+typedef __attribute__((address_space(1))) u32x4 glb_u4;
+
+template <bool HYB, int LB>
+__device__ __forceinline__ void load_block(const uint32_t *col, const uint32_t *gcol, int b,
+                                           uint32_t (&wv)[16])
+{
+    if (!HYB || b < LB) {
+        const lds_u4 *cb = (const lds_u4 *)(col + b * 1024);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const u32x4 v = cb[q * 64];
+            wv[4 * q] = v.x; wv[4 * q + 1] = v.y; wv[4 * q + 2] = v.z; wv[4 * q + 3] = v.w;
+        }
+    } else {
+        const glb_u4 *cb = (const glb_u4 *)(gcol + (b - LB) * 1024);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const u32x4 v = cb[q * 64];
+            wv[4 * q] = v.x; wv[4 * q + 1] = v.y; wv[4 * q + 2] = v.z; wv[4 * q + 3] = v.w;
+        }
+    }
+}
+
+template <bool HYB, int LB>
+__device__ __forceinline__ void store_block(uint32_t *col, uint32_t *gcol, int b,
+                                            const uint32_t (&wv)[16])
+{
+    if (!HYB || b < LB) {
+        lds_u4 *cb = (lds_u4 *)(col + b * 1024);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            cb[q * 64] = u32x4{wv[4 * q], wv[4 * q + 1], wv[4 * q + 2], wv[4 * q + 3]};
+    } else {
+        glb_u4 *cb = (glb_u4 *)(gcol + (b - LB) * 1024);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            cb[q * 64] = u32x4{wv[4 * q], wv[4 * q + 1], wv[4 * q + 2], wv[4 * q + 3]};
+    }
 }
 
 __device__ __forceinline__ void pk_finish(const PkJob &J, const ExtDev &d)
@@ -803,17 +867,21 @@ __device__ __forceinline__ void pk_finish(const PkJob &J, const ExtDev &d)
 // no N base, the standard A/-B matrix, LDS for 16 * ceil((qmax + 1) / 16)
 // columns, qmax <= 16 * NQ - 1.  SYM: o_del == o_ins and e_del == e_ins (one
 // shared gap-open term).  `col` is this lane's word of column 0 (group layout,
-// see cpos).
+// see cpos) and `gcol` its word of column kLdsCols in the wave's HBM region
+// (used when NQ > 10).
 template <bool SYM, int NQ>
 __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *meta, int count,
-                            uint32_t *col)
+                            uint32_t *col, uint32_t *gcol)
 {
+    constexpr bool HYB = NQ * 16 > kLdsCols;
+    constexpr int LB = kLdsCols / 16;                          // blocks held in LDS
     const int lane = threadIdx.x & 63;
     PkJob A, B;
     pk_setup(A, d, p, meta, lane, count);
     pk_setup(B, d, p, meta, lane + 64, count);
-    uint16_t *ca = reinterpret_cast<uint16_t *>(col);
-    uint16_t *cbh = ca + 1;
+    const ColRef<uint32_t, HYB> cw{col, gcol};
+    const ColRef<uint16_t, HYB> ca{reinterpret_cast<uint16_t *>(col), reinterpret_cast<uint16_t *>(gcol)};
+    const ColRef<uint16_t, HYB> cbh{ca.l + 1, ca.g + 1};
     // Per 16-base query word, the "differs from base c" bit masks of every
     // c, held in registers for the whole item: N01 = ne0 | ne1 << 16 and
     // N23 = ne2 | ne3 << 16.  A row picks its target base's mask with one
@@ -832,7 +900,7 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
     for (int j = 0; j <= qmax; ++j) {                          // first row, :68-77
         const int ha = (A.alive && j <= A.qlen) ? (j == 0 ? A.h0 : max(0, A.h0 - p.o_ins - p.e_ins * j)) : 0;
         const int hb = (B.alive && j <= B.qlen) ? (j == 0 ? B.h0 : max(0, B.h0 - p.o_ins - p.e_ins * j)) : 0;
-        col[(j >> 2) * 256 + (j & 3)] = (uint32_t)ha | ((uint32_t)hb << 16);
+        cw[j] = (uint32_t)ha | ((uint32_t)hb << 16);
     }
     const ss2 Ssc = ss2{(short)p.match_sc, (short)p.match_sc};
     const ss2 nAB = ss2{(short)(p.mis_sc - p.match_sc), (short)(p.mis_sc - p.match_sc)};
@@ -930,10 +998,7 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
                     Hs[k] = u2(w32(H * C256 + splat(k)) & hm);
                 }
             }
-            uint4 *cb = reinterpret_cast<uint4 *>(col + b * 1024);
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                cb[q * 64] = uint4{wv[4 * q], wv[4 * q + 1], wv[4 * q + 2], wv[4 * q + 3]};
+            store_block<HYB, LB>(col, gcol, b, wv);
 #pragma unroll
             for (int k = 0; k < 8; ++k) Hs[k] = vmax(Hs[k], Hs[k + 8]);
 #pragma unroll
@@ -941,14 +1006,7 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
             const us2 kb = vmax(vmax(Hs[0], Hs[2]), vmax(Hs[1], Hs[3]));
             key = vmax(key, kb + splat(c0));
         };
-        auto load = [&](uint32_t (&wv)[16], int b) {
-            const uint4 *cb = reinterpret_cast<const uint4 *>(col + b * 1024);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint4 v = cb[q * 64];
-                wv[4 * q] = v.x; wv[4 * q + 1] = v.y; wv[4 * q + 2] = v.z; wv[4 * q + 3] = v.w;
-            }
-        };
+        auto load = [&](uint32_t (&wv)[16], int b) { load_block<HYB, LB>(col, gcol, b, wv); };
         if (jlo < jhi) {
             // two word sets in turn: the next block's reads are in flight
             // while the current block computes
@@ -990,8 +1048,11 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
 // scores below 65536.
 template <bool SYM, int NQ>
 __device__ void ext_wave_ln(const ExtDev &d, const ExtParams &p, const PkMeta *meta, int count,
-                            uint32_t *col)
+                            uint32_t *col, uint32_t *gcol)
 {
+    constexpr bool HYB = NQ * 16 > kLdsCols;
+    constexpr int LB = kLdsCols / 16;
+    const ColRef<uint32_t, HYB> cw{col, gcol};
     const int lane = threadIdx.x & 63;
     PkJob A, B;
     pk_setup(A, d, p, meta, lane, count);
@@ -1006,7 +1067,7 @@ __device__ void ext_wave_ln(const ExtDev &d, const ExtParams &p, const PkMeta *m
     }
     const int qmax = uniform_max(A.alive ? A.qlen : 0);
     for (int j = 0; j <= qmax; ++j)                            // first row, :68-77
-        col[cpos<uint32_t>(j)] = (A.alive && j <= A.qlen)
+        cw[j] = (A.alive && j <= A.qlen)
                                      ? (uint32_t)(j == 0 ? A.h0 : max(0, A.h0 - p.o_ins - p.e_ins * j))
                                      : 0u;
     const int sA = p.match_sc, nAB = p.mis_sc - p.match_sc;
@@ -1020,7 +1081,7 @@ __device__ void ext_wave_ln(const ExtDev &d, const ExtParams &p, const PkMeta *m
             if (nb & 1) A.two = d.seq[A.tw0 + min(nb, A.tnw)];
             else A.twe = d.seq[A.tw0 + min(nb, A.tnw)];
         }
-        uint32_t h1 = (uint32_t)pk_row_start<uint32_t>(A, p, i, col);
+        uint32_t h1 = (uint32_t)pk_row_start(A, p, i, cw);
         if (__ballot(A.alive) == 0) break;
         uint32_t f = 0, key = 0;
         const int jlo = uniform_min(A.alive ? A.beg : BIG);
@@ -1066,24 +1127,14 @@ __device__ void ext_wave_ln(const ExtDev &d, const ExtParams &p, const PkMeta *m
                     Hs[k] = in ? ((H << 8) | (uint32_t)k) : 0u;
                 }
             }
-            uint4 *cb = reinterpret_cast<uint4 *>(col + b * 1024);
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                cb[q * 64] = uint4{wv[4 * q], wv[4 * q + 1], wv[4 * q + 2], wv[4 * q + 3]};
+            store_block<HYB, LB>(col, gcol, b, wv);
 #pragma unroll
             for (int k = 0; k < 8; ++k) Hs[k] = max(Hs[k], Hs[k + 8]);
 #pragma unroll
             for (int k = 0; k < 4; ++k) Hs[k] = max(Hs[k], Hs[k + 4]);
             key = max(key, max(max(Hs[0], Hs[2]), max(Hs[1], Hs[3])) + (uint32_t)c0);
         };
-        auto load = [&](uint32_t (&wv)[16], int b) {
-            const uint4 *cb = reinterpret_cast<const uint4 *>(col + b * 1024);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint4 v = cb[q * 64];
-                wv[4 * q] = v.x; wv[4 * q + 1] = v.y; wv[4 * q + 2] = v.z; wv[4 * q + 3] = v.w;
-            }
-        };
+        auto load = [&](uint32_t (&wv)[16], int b) { load_block<HYB, LB>(col, gcol, b, wv); };
         if (jlo < jhi) {
             const int bb = jlo >> 4, be = (jhi - 1) >> 4;
             uint32_t W0[16], W1[16];
@@ -1097,8 +1148,8 @@ __device__ void ext_wave_ln(const ExtDev &d, const ExtParams &p, const PkMeta *m
                 if (b + 2 > be) break;
             }
         }
-        pk_row_end<uint32_t>(A, p, i, (int)h1, (int)key, col);
-        pk_narrow<uint32_t>(A, B, (int)h1, 1, col, col);
+        pk_row_end(A, p, i, (int)h1, (int)key, cw);
+        pk_narrow(A, B, (int)h1, 1, cw, cw);
     }
     if (d.cells) {
         unsigned long long c = A.ncells;
@@ -1114,32 +1165,36 @@ __device__ void ext_wave_ln(const ExtDev &d, const ExtParams &p, const PkMeta *m
 template <bool SYM, int NQ>
 __global__ __launch_bounds__(64) void ext_persistent_ln_kernel(ExtDev d, ExtParams p,
                                                                const WorkItem *items, int n_items,
-                                                               uint32_t *counter)
+                                                               uint32_t *counter, uint32_t *gscratch)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     uint32_t *col = reinterpret_cast<uint32_t *>(lds_raw) + threadIdx.x * 4;
+    uint32_t *gcol = gscratch ? gscratch + (size_t)blockIdx.x * ext_hyb_words() + threadIdx.x * 4
+                              : nullptr;
     for (;;) {
         int item = 0;
         if (threadIdx.x == 0) item = (int)atomicAdd(counter, 1u);
         item = __builtin_amdgcn_readfirstlane(item);
         if (item >= n_items) break;
         const WorkItem wi = items[item];
-        ext_wave_ln<SYM, NQ>(d, p, d.pk + wi.start, (int)wi.count, col);
+        ext_wave_ln<SYM, NQ>(d, p, d.pk + wi.start, (int)wi.count, col, gcol);
     }
 }
 
 hipError_t launch_ext_persistent_ln(const ExtDev &d, const ExtParams &p, const WorkItem *items,
                                     int n_items, uint32_t *counter, int cols, int grid,
-                                    hipStream_t s)
+                                    void *gscratch, hipStream_t s)
 {
     if (n_items <= 0) return hipSuccess;
     if (cols < 1 || cols > 256) return hipErrorInvalidValue;
+    if (cols > kLdsCols && !gscratch) return hipErrorInvalidValue;
     const bool sym = p.o_del == p.o_ins && p.e_del == p.e_ins;
     const int nq = (cols + 15) / 16;
     const size_t lds = ext_pk_lds_bytes(cols);
+    uint32_t *gs = static_cast<uint32_t *>(gscratch);
 #define HIPSW_LN_LAUNCH(S, Q)                                                                   \
     hipLaunchKernelGGL((ext_persistent_ln_kernel<S, Q>), dim3(grid), dim3(64), lds, s, d, p,   \
-                       items, n_items, counter)
+                       items, n_items, counter, gs)
     if (nq <= 4) {
         if (sym) HIPSW_LN_LAUNCH(true, 4); else HIPSW_LN_LAUNCH(false, 4);
     } else if (nq <= 10) {
@@ -1158,17 +1213,19 @@ hipError_t launch_ext_persistent_ln(const ExtDev &d, const ExtParams &p, const W
 template <bool SYM, int NQ>
 __global__ __launch_bounds__(64) void ext_persistent_pk_kernel(ExtDev d, ExtParams p,
                                                                const WorkItem *items, int n_items,
-                                                               uint32_t *counter)
+                                                               uint32_t *counter, uint32_t *gscratch)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     uint32_t *col = reinterpret_cast<uint32_t *>(lds_raw) + threadIdx.x * 4;
+    uint32_t *gcol = gscratch ? gscratch + (size_t)blockIdx.x * ext_hyb_words() + threadIdx.x * 4
+                              : nullptr;
     for (;;) {
         int item = 0;
         if (threadIdx.x == 0) item = (int)atomicAdd(counter, 1u);
         item = __builtin_amdgcn_readfirstlane(item);
         if (item >= n_items) break;
         const WorkItem wi = items[item];
-        ext_wave_pk<SYM, NQ>(d, p, d.pk + wi.start, (int)wi.count, col);
+        ext_wave_pk<SYM, NQ>(d, p, d.pk + wi.start, (int)wi.count, col, gcol);
     }
 }
 
@@ -1184,20 +1241,30 @@ void build_pk_meta(int n, const uint32_t *order, const uint32_t *q_word, const u
     }
 }
 
-size_t ext_pk_lds_bytes(int cols) { return (size_t)((cols + 15) / 16) * 16 * 64 * 4; }
+size_t ext_pk_lds_bytes(int cols)
+{
+    return (size_t)std::min((cols + 15) / 16 * 16, kLdsCols) * 64 * 4;
+}
 
-hipError_t launch_ext_persistent_pk(const ExtDev &d, const ExtParams &p, const uint32_t *order,
-                                    const WorkItem *items, int n_items, uint32_t *counter,
-                                    int cols, int grid, hipStream_t s)
+size_t ext_pk_hbm_bytes_per_wave(int cols)
+{
+    return cols > kLdsCols ? (size_t)ext_hyb_words() * 4 : 0;
+}
+
+hipError_t launch_ext_persistent_pk(const ExtDev &d, const ExtParams &p, const WorkItem *items,
+                                    int n_items, uint32_t *counter, int cols, int grid,
+                                    void *gscratch, hipStream_t s)
 {
     if (n_items <= 0) return hipSuccess;
     if (cols < 1 || cols > 256) return hipErrorInvalidValue;
+    if (cols > kLdsCols && !gscratch) return hipErrorInvalidValue;
+    uint32_t *gs = static_cast<uint32_t *>(gscratch);
     const bool sym = p.o_del == p.o_ins && p.e_del == p.e_ins;
     const int nq = (cols + 15) / 16;             // query words of the longest query
     const size_t lds = ext_pk_lds_bytes(cols);
 #define HIPSW_PK_LAUNCH(S, Q)                                                                   \
     hipLaunchKernelGGL((ext_persistent_pk_kernel<S, Q>), dim3(grid), dim3(64), lds, s, d, p,   \
-                       items, n_items, counter)
+                       items, n_items, counter, gs)
     if (nq <= 4) {
         if (sym) HIPSW_PK_LAUNCH(true, 4); else HIPSW_PK_LAUNCH(false, 4);
     } else if (nq <= 10) {

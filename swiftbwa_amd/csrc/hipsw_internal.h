@@ -77,17 +77,20 @@ size_t ext_state_bytes(int tier);   // bytes of h/e state per column per lane
 size_t ext_lds_bytes(int tier, bool global_state, int cols);   // dynamic LDS per block
 // Packed form (two jobs per lane, 8-bit tier, LDS state, items of <= 128 jobs).
 size_t ext_pk_lds_bytes(int cols);
+// HBM bytes per resident wave of the packed / one-per-lane forms for queries
+// longer than the LDS holds (0 otherwise).
+size_t ext_pk_hbm_bytes_per_wave(int cols);
 // Descriptors of the packed form, in `order`.
 void build_pk_meta(int n, const uint32_t *order, const uint32_t *q_word, const uint32_t *t_word,
                    const uint32_t *qn_word, const uint32_t *tn_word, const int32_t *q_len,
                    const int32_t *t_len, const int32_t *h0, const int32_t *w, const int32_t *eb,
                    PkMeta *out);
-hipError_t launch_ext_persistent_pk(const ExtDev &d, const ExtParams &p, const uint32_t *order,
-                                    const WorkItem *items, int n_items, uint32_t *counter,
-                                    int cols, int grid, hipStream_t s);
+hipError_t launch_ext_persistent_pk(const ExtDev &d, const ExtParams &p, const WorkItem *items,
+                                    int n_items, uint32_t *counter, int cols, int grid,
+                                    void *gscratch, hipStream_t s);
 // One-job-per-lane form (8- and 16-bit tiers, no N, standard matrix, items of
 // <= 64 jobs, descriptors in d.pk); same LDS size as the packed form.
 hipError_t launch_ext_persistent_ln(const ExtDev &d, const ExtParams &p, const WorkItem *items,
                                     int n_items, uint32_t *counter, int cols, int grid,
-                                    hipStream_t s);
+                                    void *gscratch, hipStream_t s);
 }  // namespace hipsw
