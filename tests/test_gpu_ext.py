@@ -182,3 +182,37 @@ def test_autotune_both_layouts_exact(engine):
         assert np.array_equal(out, ref)                       # re-planned set
     finally:
         lib.hipsw_ext_free(engine.handle, st)
+
+
+def test_size_and_tier_boundaries(engine):
+    """Query lengths at every block / LDS / class edge (15..17, 63/64, 159..161,
+    255/256), targets up to the driver's 10,000-base window, bands 0..1000, and
+    h0 placed exactly at the 8/16-bit score-tier bounds."""
+    rng = np.random.default_rng(21)
+    qls = [0, 1, 15, 16, 17, 63, 64, 65, 159, 160, 161, 191, 254, 255, 256, 300]
+    qs, ts, h0, w, eb = [], [], [], [], []
+    for ql in qls:
+        for rep in range(24):
+            q = rng.integers(0, 4, ql).astype(np.uint8)
+            tl = int(rng.choice([0, 1, ql, ql + 50, 2 * ql + 200, 10000 if rep == 0 else 600]))
+            t = rng.integers(0, 4, tl).astype(np.uint8)
+            m = min(ql, tl)
+            if m and rep % 3:                                   # similar prefix: long alignments
+                t[:m] = q[:m]
+                flip = rng.random(m) < 0.03
+                t[:m][flip] = (t[:m][flip] + 1) % 4
+            qs.append(q); ts.append(t)
+            bound = [255 - ql, 256 - ql, 65535 - ql, 65536 - ql, 20, 0][rep % 6]
+            h0.append(max(0, bound))
+            w.append(int([0, 1, 2, 100, 1000][rep % 5]))
+            eb.append(int(rep % 7))
+    n = len(qs)
+    ql = np.array([len(x) for x in qs], np.uint16); tl = np.array([len(x) for x in ts], np.uint16)
+    qo = np.zeros(n, np.uint32); to = np.zeros(n, np.uint32)
+    qo[1:] = np.cumsum(ql.astype(np.int64))[:-1]; to[1:] = np.cumsum(tl.astype(np.int64))[:-1]
+    js = {"n": n, "queries": np.concatenate(qs), "q_off": qo, "q_len": ql,
+          "targets": np.concatenate(ts), "t_off": to, "t_len": tl,
+          "h0": np.array(h0, np.int32), "w": np.array(w, np.int32),
+          "end_bonus": np.array(eb, np.int32)}
+    check(engine, js)
+    check(engine, js, ScoringParameters(gapOpenPenalty=4, gapExtendPenalty=2, zDrop=40))
