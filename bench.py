@@ -77,22 +77,33 @@ def cpu_baseline(js, sample):
 
 
 def autotune(eng, st, max_jobs):
-    """hipsw_ext_autotune on an uploaded set: per 8-bit bin, packed vs
-    one-job-per-lane layout timed on the bin's first max_jobs jobs."""
+    """hipsw_ext_autotune on an uploaded set: per 8-bit bin, the packed layout
+    at each LDS/HBM split of the column state and the one-job-per-lane layout,
+    timed on the bin's first max_jobs jobs."""
     import ctypes as C
     from swiftbwa_amd import native
-    rep = (native.HipswTuneEntry * 16)()
+    rep = (native.HipswTuneEntry * 32)()
     nrep = C.c_int32()
-    if eng.lib.hipsw_ext_autotune(eng.handle, st, int(max_jobs), rep, 16, C.byref(nrep)):
+    if eng.lib.hipsw_ext_autotune(eng.handle, st, int(max_jobs), rep, 32, C.byref(nrep)):
         raise RuntimeError(eng.last_error())
+    classes = ["0-63", "64-159", "160-255", "256+"]
     kinds = {1: "packed (2 jobs/lane, int16x2)", 2: "one job/lane (int32)"}
-    out = [{"tier": "8-bit", "length_class": ["0-63", "64-159", "160-255", "256+"][e.length_class],
-            "jobs_timed": e.jobs, "packed_ms": round(e.ms_packed, 3),
-            "lane_ms": round(e.ms_lane, 3), "chosen": kinds.get(e.chosen, str(e.chosen))}
-           for e in rep[:nrep.value]]
-    for e in out:
-        log("autotune %s %s: packed %.3f ms, lane %.3f ms -> %s" % (
-            e["tier"], e["length_class"], e["packed_ms"], e["lane_ms"], e["chosen"]))
+    out = []
+    for e in rep[:nrep.value]:
+        if not out or out[-1]["length_class"] != classes[e.length_class]:
+            out.append({"tier": "8-bit", "length_class": classes[e.length_class],
+                        "jobs_timed": e.jobs, "candidates": []})
+        c = {"kind": kinds.get(e.kind, str(e.kind)), "lds_columns": 16 * e.lds_blocks,
+             "ms": round(e.ms, 3)}
+        out[-1]["candidates"].append(c)
+        if e.chosen:
+            out[-1]["chosen"] = c
+    for b in out:
+        log("autotune %s %s: %s -> %s, %d LDS columns" % (
+            b["tier"], b["length_class"],
+            ", ".join("%s/%d %.3f ms" % (c["kind"].split(" ")[0], c["lds_columns"], c["ms"])
+                      for c in b["candidates"]),
+            b["chosen"]["kind"], b["chosen"]["lds_columns"]))
     return out
 
 

@@ -111,18 +111,31 @@ int  hipsw_ext_run(hipsw_engine *eng, hipsw_ext_set *set, int32_t mode, hipsw_st
 
 /* Autotuner of the batched form (no reference counterpart).  For each bin of
  * the set in the 8-bit score tier (one query-length class each) it times the
- * packed layout (two matrices per lane, packed int16 arithmetic) against one
- * matrix per lane (int32 arithmetic) on the bin's first max_jobs jobs, keeps
- * the faster per class in the engine (later uploads and the server plan with
- * it) and re-plans the set.  Results do not change, only speed.  report
- * receives up to report_cap entries: tier (0 = 8-bit), query-length class,
- * jobs timed, the kind chosen (1 packed, 2 one per lane) and both times. */
+ * candidate kernels on the bin's first max_jobs jobs -- the packed layout (two
+ * matrices per lane, packed int16 arithmetic) with each built split of the
+ * column state between LDS and HBM (lds_blocks: 16-column blocks in LDS; fewer
+ * blocks, more waves per CU), then one matrix per lane (int32 arithmetic) --
+ * keeps the fastest per class in the engine (later uploads, the server and the
+ * chain driver plan with it) and re-plans the set.  Results do not change, only
+ * speed.  report receives up to report_cap entries, one per (bin, candidate):
+ * tier (0 = 8-bit), query-length class (0: 0-63, 1: 64-159, 2: 160-255),
+ * jobs timed, kind (1 packed, 2 one per lane), lds_blocks, chosen (1 for the
+ * winner of its bin) and the best of two timed runs in ms. */
 typedef struct {
-    int32_t tier, length_class, jobs, chosen;
-    double ms_packed, ms_lane;
+    int32_t tier, length_class, jobs, kind, lds_blocks, chosen;
+    double ms;
 } hipsw_tune_entry;
 int  hipsw_ext_autotune(hipsw_engine *eng, hipsw_ext_set *set, int32_t max_jobs,
                         hipsw_tune_entry *report, int32_t report_cap, int32_t *n_report);
+/* Sets the engine's layout for one query-length class of the 8-bit tier, as
+ * the autotuner would (no reference counterpart): lets a host restore tuned
+ * choices without timing again.  kind 0 restores the default (packed, column
+ * state in LDS up to 160 columns); kind 1 (packed) or 2 (one job per lane)
+ * with lds_blocks among the splits built for the class (class 0: 4; class 1:
+ * 10, 6, 5; class 2: 10, 8, 6; one job per lane: 4 / 10 / 10).  Applies to
+ * plans made afterwards.  HIPSW_E_INVALID for a layout not built. */
+int  hipsw_ext_set_layout(hipsw_engine *eng, int32_t length_class, int32_t kind,
+                          int32_t lds_blocks);
 int  hipsw_ext_download(hipsw_engine *eng, hipsw_ext_set *set, int32_t *out);
 void hipsw_ext_free(hipsw_engine *eng, hipsw_ext_set *set);
 

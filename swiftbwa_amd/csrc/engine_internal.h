@@ -32,8 +32,10 @@ struct hipsw_engine {
     size_t nw_dev_bytes = 0;
 
     // Autotuner choice per query-length class of the 8-bit tier: 0 = default
-    // (packed form), otherwise an ExtKind (hipsw_ext_autotune).
+    // (packed form), otherwise an ExtKind, and the 16-column blocks of state
+    // held in LDS (0 = default) (hipsw_ext_autotune).
     int tuned_kind[4] = {0, 0, 0, 0};
+    int tuned_lb[4] = {0, 0, 0, 0};
 
     bool fail(const char *what, hipError_t e);
     bool check(hipError_t e, const char *what);
@@ -67,13 +69,6 @@ bool pack_jobs(int n, const uint8_t *queries, const uint32_t *q_off, const uint1
 void radix_sort_by_key(std::vector<uint32_t> &keys, std::vector<uint32_t> &idx);
 
 // One persistent launch of the batched extension form.
-// Kernel families of a batched extension bin.
-enum ExtKind {
-    EXT_UNPACKED = 0,     // one job per lane, any matrix / N / tier (ext_persistent_kernel)
-    EXT_PACKED = 1,       // two jobs per lane, 8-bit tier, items of 128 jobs
-    EXT_LANE = 2,         // one job per lane, 8/16-bit tier, packed-form structure
-};
-
 struct ExtLaunch {
     int tier;
     bool global_state;
@@ -81,6 +76,7 @@ struct ExtLaunch {
     int cls;              // query-length class
     int job_begin, job_end;   // the bin's range in the batched order
     int cols;
+    int lb;               // packed / one-per-lane: 16-column blocks of state in LDS
     int item_begin, item_count;
     int grid;
     size_t scratch_off;   // bytes into the engine scratch
@@ -93,6 +89,10 @@ size_t plan_ext_batched(const hipsw_engine *eng, int n, const int32_t *ql, const
                         std::vector<ExtLaunch> &launches);
 // Work items and grid of a bin for its kind.
 void ext_bin_items(const hipsw_engine *eng, ExtLaunch &L, std::vector<WorkItem> &items);
+// Work items, grids and scratch regions of every bin (kinds fixed); returns
+// the scratch bytes the launches need.
+size_t ext_layout_launches(const hipsw_engine *eng, std::vector<ExtLaunch> &launches,
+                           std::vector<WorkItem> &items);
 hipError_t launch_ext_bin(const ExtLaunch &L, const ExtDev &d, const ExtParams &p,
                           const uint32_t *order, const WorkItem *items, uint32_t *counter,
                           void *gscratch, hipStream_t s);

@@ -90,10 +90,7 @@ size_t plan_ext_batched(const hipsw_engine *eng, int n, const int32_t *ql, const
     }
     radix_sort_by_key(keys, idx);
     order = idx;
-    items.clear();
     launches.clear();
-    items.reserve((size_t)n / 64 + 16);
-    size_t scratch_off = 0;
     for (int s = 0; s < n;) {
         const uint32_t bin = keys[s] >> 27;
         int e = s, cmax = 0;
@@ -107,27 +104,50 @@ size_t plan_ext_batched(const hipsw_engine *eng, int n, const int32_t *ql, const
         L.cls = (int)(bin >> 1) % kClasses;
         L.cols = cmax + 1;
         const bool fast = !with_n && std_mat && cmax <= 255 && L.tier != TIER_I32 &&
-                          ext_pk_lds_bytes(L.cols) <= (size_t)eng->lds_per_block;
+                          ext_pk_lds_bytes(L.cols, ext_pk_default_lb(L.cols)) <=
+                              (size_t)eng->lds_per_block;
         L.kind = !fast ? EXT_UNPACKED
                        : L.tier == TIER_U16 ? EXT_LANE
                                             : eng->tuned_kind[L.cls] > 0 ? eng->tuned_kind[L.cls] : EXT_PACKED;
+        L.lb = 0;
+        if (L.kind != EXT_UNPACKED) {
+            const int t = L.tier == TIER_U8 ? eng->tuned_lb[L.cls] : 0;
+            L.lb = t > 0 && ext_pk_lb_supported(L.kind, L.cols, t) ? t : ext_pk_default_lb(L.cols);
+        }
         L.global_state = L.kind == EXT_UNPACKED &&
                          ext_lds_bytes(L.tier, false, L.cols) > (size_t)eng->lds_per_block;
         L.job_begin = s;
         L.job_end = e;
+        launches.push_back(L);
+        s = e;
+    }
+    return ext_layout_launches(eng, launches, items);
+}
+
+// Waves of a packed / one-per-lane bin that one CU holds (whole multiples of
+// the wave's LDS share, at most 8).
+static int pk_waves_per_cu(const hipsw_engine *eng, const ExtLaunch &L)
+{
+    const size_t per_cu = eng->lds_per_cu > 0 ? (size_t)eng->lds_per_cu : 65536;
+    return (int)std::max<size_t>(1, std::min<size_t>(8, per_cu / ext_pk_lds_bytes(L.cols, L.lb)));
+}
+
+size_t ext_layout_launches(const hipsw_engine *eng, std::vector<ExtLaunch> &launches,
+                           std::vector<WorkItem> &items)
+{
+    items.clear();
+    size_t scratch_off = 0;
+    for (ExtLaunch &L : launches) {
         ext_bin_items(eng, L, items);
         L.scratch_off = scratch_off;
         if (L.kind == EXT_UNPACKED) {
             L.scratch_bytes = L.global_state ? (size_t)L.grid * L.cols * 64 * ext_state_bytes(L.tier) : 0;
         } else {
-            // every wave the CU can hold gets a region, so a re-plan's grid fits too
-            const size_t per_cu = eng->lds_per_cu > 0 ? (size_t)eng->lds_per_cu : 65536;
-            const size_t wpc = std::max<size_t>(1, std::min<size_t>(8, per_cu / ext_pk_lds_bytes(L.cols)));
-            L.scratch_bytes = (size_t)eng->n_cu * wpc * ext_pk_hbm_bytes_per_wave(L.cols);
+            // every wave the CU can hold gets a region
+            L.scratch_bytes = (size_t)eng->n_cu * pk_waves_per_cu(eng, L) *
+                              ext_pk_hbm_bytes_per_wave(L.cols, L.lb);
         }
         scratch_off += (L.scratch_bytes + 255) & ~(size_t)255;
-        launches.push_back(L);
-        s = e;
     }
     return scratch_off;
 }
@@ -139,10 +159,9 @@ void ext_bin_items(const hipsw_engine *eng, ExtLaunch &L, std::vector<WorkItem> 
     for (int g = L.job_begin; g < L.job_end; g += per_item)
         items.push_back(WorkItem{(uint32_t)g, (uint32_t)std::min(per_item, L.job_end - g)});
     L.item_count = (int)items.size() - L.item_begin;
-    const size_t per_cu = eng->lds_per_cu > 0 ? (size_t)eng->lds_per_cu : 65536;
     int wpc;
     if (L.kind != EXT_UNPACKED)   // no static shared memory: the CU holds whole multiples
-        wpc = (int)std::max<size_t>(1, std::min<size_t>(8, per_cu / ext_pk_lds_bytes(L.cols)));
+        wpc = pk_waves_per_cu(eng, L);
     else
         wpc = L.global_state ? 8 : waves_per_cu(eng, ext_lds_bytes(L.tier, false, L.cols));
     L.grid = std::max(1, std::min(L.item_count, eng->n_cu * wpc));
@@ -157,10 +176,10 @@ hipError_t launch_ext_bin(const ExtLaunch &L, const ExtDev &d, const ExtParams &
     switch (L.kind) {
     case EXT_PACKED:
         return launch_ext_persistent_pk(d, p, items + L.item_begin, L.item_count, counter, L.cols,
-                                        L.grid, gs, s);
+                                        L.lb, L.grid, gs, s);
     case EXT_LANE:
         return launch_ext_persistent_ln(d, p, items + L.item_begin, L.item_count, counter, L.cols,
-                                        L.grid, gs, s);
+                                        L.lb, L.grid, gs, s);
     default:
         return launch_ext_persistent(L.tier, L.global_state, d, p, order, items + L.item_begin,
                                      L.item_count, counter, L.cols, L.grid, gs, s);
@@ -466,6 +485,26 @@ extern "C" int hipsw_banded_extend(hipsw_engine *eng, int32_t n, const uint8_t *
 // against the one-job-per-lane form (32-bit arithmetic) on the bin's first
 // `max_jobs` jobs in batched order, keep the faster in the engine's table for
 // that query-length class (later uploads plan with it) and re-plan the set.
+extern "C" int hipsw_ext_set_layout(hipsw_engine *eng, int32_t length_class, int32_t kind,
+                                   int32_t lds_blocks)
+{
+    if (!eng || length_class < 0 || length_class > 2) return HIPSW_E_INVALID;
+    std::lock_guard<std::recursive_mutex> lock(eng->mu);
+    if (kind == 0) {
+        eng->tuned_kind[length_class] = 0;
+        eng->tuned_lb[length_class] = 0;
+        return HIPSW_OK;
+    }
+    const int cols = length_class == 0 ? 64 : length_class == 1 ? 160 : 256;   // class's widest
+    if ((kind != EXT_PACKED && kind != EXT_LANE) || !ext_pk_lb_supported(kind, cols, lds_blocks)) {
+        eng->last_error = "hipsw_ext_set_layout: layout not built for this length class";
+        return HIPSW_E_INVALID;
+    }
+    eng->tuned_kind[length_class] = kind;
+    eng->tuned_lb[length_class] = lds_blocks;
+    return HIPSW_OK;
+}
+
 extern "C" int hipsw_ext_autotune(hipsw_engine *eng, hipsw_ext_set *set, int32_t max_jobs,
                                   hipsw_tune_entry *report, int32_t report_cap,
                                   int32_t *n_report)
@@ -480,25 +519,37 @@ extern "C" int hipsw_ext_autotune(hipsw_engine *eng, hipsw_ext_set *set, int32_t
     if (!eng->check(hipMalloc(&d_tmp, tmp_cap * sizeof(WorkItem)), "hipMalloc(tune items)"))
         return HIPSW_E_DEVICE;
     int rc = HIPSW_OK;
-    void *gs = nullptr;
-    if (set->batched_scratch && !(gs = eng->scratch_get(set->batched_scratch))) {
-        hipFree(d_tmp);
-        return HIPSW_E_DEVICE;
-    }
     for (ExtLaunch &L : set->launches) {
         if (L.tier != TIER_U8 || L.kind == EXT_UNPACKED) continue;
+        // candidates: the packed form at each LDS share built for this query
+        // length, then one job per lane (last: its results stay on the device)
+        struct Cand { int kind, lb; };
+        std::vector<Cand> cands;
+        int lbs[5];
+        const int nlb = ext_pk_lb_candidates(EXT_PACKED, L.cols, lbs);
+        for (int c = 0; c < nlb; ++c) cands.push_back(Cand{EXT_PACKED, lbs[c]});
+        cands.push_back(Cand{EXT_LANE, ext_pk_default_lb(L.cols)});
         ExtLaunch S = L;
         S.job_end = std::min(L.job_end, L.job_begin + max_jobs);
-        double best = 1e30, ms[2] = {0, 0};
-        int winner = L.kind;
-        const int kinds[2] = {EXT_PACKED, EXT_LANE};
-        for (int c = 0; c < 2 && rc == HIPSW_OK; ++c) {
+        double best = 1e30;
+        Cand win{L.kind, L.lb};
+        const int first_rep = nrep;
+        for (const Cand &c : cands) {
             ExtLaunch K = S;
-            K.kind = kinds[c];
+            K.kind = c.kind;
+            K.lb = c.lb;
             std::vector<WorkItem> items;
-            ext_bin_items(eng, K, items);
-            K.item_begin = 0;
-            if (!eng->check(hipMemcpy(d_tmp, items.data(), items.size() * sizeof(WorkItem),
+            std::vector<ExtLaunch> one{K};
+            const size_t need = ext_layout_launches(eng, one, items);
+            K = one[0];
+            void *gs = nullptr;
+            if (need && !(gs = eng->scratch_get(need))) {
+                eng->last_error = "hipsw_ext_autotune: scratch allocation failed";
+                rc = HIPSW_E_DEVICE;
+                break;
+            }
+            if (items.size() > tmp_cap ||
+                !eng->check(hipMemcpy(d_tmp, items.data(), items.size() * sizeof(WorkItem),
                                       hipMemcpyHostToDevice), "tune items")) {
                 rc = HIPSW_E_DEVICE;
                 break;
@@ -518,21 +569,26 @@ extern "C" int hipsw_ext_autotune(hipsw_engine *eng, hipsw_ext_set *set, int32_t
                 hipEventElapsedTime(&e, eng->ev0, eng->ev1);
                 t = std::min(t, e);
             }
-            ms[c] = t;
-            if (t < best) { best = t; winner = kinds[c]; }
+            if (rc != HIPSW_OK) break;
+            if (report && nrep < report_cap)
+                report[nrep] = hipsw_tune_entry{L.tier, L.cls, S.job_end - S.job_begin, c.kind, c.lb, 0,
+                                                (double)t};
+            ++nrep;
+            if (t < best) { best = t; win = c; }
         }
         if (rc != HIPSW_OK) break;
-        L.kind = winner;
-        eng->tuned_kind[L.cls] = winner;
-        if (report && nrep < report_cap)
-            report[nrep] = hipsw_tune_entry{L.tier, L.cls, S.job_end - S.job_begin, winner, ms[0], ms[1]};
-        ++nrep;
+        for (int r = first_rep; r < nrep && r < report_cap; ++r)
+            if (report && report[r].kind == win.kind && report[r].lds_blocks == win.lb) report[r].chosen = 1;
+        L.kind = win.kind;
+        L.lb = win.lb;
+        eng->tuned_kind[L.cls] = win.kind;
+        eng->tuned_lb[L.cls] = win.lb;
     }
     hipFree(d_tmp);
     if (rc != HIPSW_OK) return rc;
-    // re-plan the set's work items with the chosen kinds
+    // re-plan the set's work items, grids and scratch regions with the choices
     std::vector<WorkItem> items;
-    for (ExtLaunch &L : set->launches) ext_bin_items(eng, L, items);
+    set->batched_scratch = ext_layout_launches(eng, set->launches, items);
     if (items.size() > set->items_cap) return HIPSW_E_UNSUPPORTED;
     if (!eng->check(hipMemcpy(set->d_items, items.data(), items.size() * sizeof(WorkItem),
                               hipMemcpyHostToDevice), "tune re-plan"))

@@ -636,18 +636,19 @@ template <typename T> __device__ __forceinline__ int cpos(int j);
 template <> __device__ __forceinline__ int cpos<uint16_t>(int j) { return (j >> 2) * 512 + (j & 3) * 2; }
 template <> __device__ __forceinline__ int cpos<uint32_t>(int j) { return (j >> 2) * 256 + (j & 3); }
 
-// Queries longer than 159 bases (HYB): columns below kLdsCols stay in LDS and
-// the rest live in the wave's own HBM region (same group layout, L2-resident:
-// 24 KiB per wave), so such a wave still takes 40 KiB of LDS and four fill a
-// CU.  ColRef is a job's (h, e) entry of column j, wherever it lives.
-constexpr int kLdsCols = 160;
-// u32 words of one wave's HBM region: columns kLdsCols..255, 64 lanes
-__host__ __device__ constexpr size_t ext_hyb_words() { return (size_t)(256 - kLdsCols) * 64; }
-template <typename T, bool HYB> struct ColRef {
+// Hybrid column state: a kernel instance keeps its first LB 16-column blocks
+// (LC = 16 * LB columns) in LDS and the rest in the wave's own HBM region (same
+// group layout, L2-resident), so that a wave's LDS share, and with it the waves
+// a CU holds, does not grow with the query: 150-bp queries held whole take
+// 40 KiB (four waves per CU), held as 96 + 64 columns 24 KiB (six).  ColRef is a
+// job's (h, e) entry of column j, wherever it lives; LC = 0: all in LDS.
+// u32 words of one wave's HBM region (blocks LB..NQ-1, 64 lanes)
+__host__ __device__ constexpr size_t ext_hyb_words(int nq, int lb) { return nq > lb ? (size_t)(nq - lb) * 1024 : 0; }
+template <typename T, int LC> struct ColRef {
     T *l, *g;
     __device__ __forceinline__ T &operator[](int j) const
     {
-        if (HYB && j >= kLdsCols) return g[cpos<T>(j - kLdsCols)];
+        if (LC > 0 && j >= LC) return g[cpos<T>(j - LC)];
         return l[cpos<T>(j)];
     }
 };
@@ -695,9 +696,9 @@ __device__ __forceinline__ void pk_setup(PkJob &J, const ExtDev &d, const ExtPar
 // Row start of one job (:92-107); returns h1.  `c16` is the job's column
 // state ((h, e) of column j at c16[cpos<T>(j)]).  The caller keeps the target
 // words of the current and the next 16-row block loaded.
-template <typename T, bool HYB>
+template <typename T, int LC>
 __device__ __forceinline__ int pk_row_start(PkJob &J, const ExtParams &p, int i,
-                                            const ColRef<T, HYB> &c16)
+                                            const ColRef<T, LC> &c16)
 {
     J.alive = J.alive && i < J.tlen;
     J.tb = (((i >> 4) & 1 ? J.two : J.twe) >> ((i & 15) * 2)) & 3u;   // this row's target base
@@ -716,9 +717,9 @@ __device__ __forceinline__ int pk_row_start(PkJob &J, const ExtParams &p, int i,
 // Row end of one half (:145-172): band-end entry, global-score column, row
 // maximum and z-drop.  A finished job's half is never read again, so its
 // band-end store needs no guard (end <= qlen keeps it in bounds).
-template <typename T, bool HYB>
+template <typename T, int LC>
 __device__ __forceinline__ void pk_row_end(PkJob &J, const ExtParams &p, int i, int h1, int key,
-                                           const ColRef<T, HYB> &c16)
+                                           const ColRef<T, LC> &c16)
 {
     c16[J.end] = (T)h1;                                        // h[end] = h1, e[end] = 0
     const int rm = key >> 8, rj = key & 0xFF;
@@ -738,9 +739,9 @@ __device__ __forceinline__ void pk_row_end(PkJob &J, const ExtParams &p, int i, 
 
 // Band narrowing of both halves (:175-180).  Each trip reads four columns per
 // half at once (one LDS round trip); more trips only while all four are zero.
-template <typename T, bool HYB>
+template <typename T, int LC>
 __device__ __forceinline__ void pk_narrow(PkJob &A, PkJob &B, int h1a, int h1b,
-                                          const ColRef<T, HYB> &ca, const ColRef<T, HYB> &cb)
+                                          const ColRef<T, LC> &ca, const ColRef<T, LC> &cb)
 {
     // first column in [beg, end) whose (h, e) is non-zero, else end
     int ja = A.beg, jb = B.beg;
@@ -803,7 +804,7 @@ __device__ __forceinline__ void pk_narrow(PkJob &A, PkJob &B, int h1a, int h1b,
 }
 
 // The 16 column words of block b of this lane: four 128-bit accesses to LDS,
-// or to the wave's HBM region for blocks from kLdsCols on (HYB).  The pointers
+// or to the wave's HBM region for blocks from LB on (HYB).  The pointers
 // carry their address space so that each path keeps its own instructions
 // (ds_*_b128 / global_*_dwordx4) instead of generic flat accesses.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -865,23 +866,23 @@ __device__ __forceinline__ void pk_finish(const PkJob &J, const ExtDev &d)
 // Extends jobs idx[0..count) (count <= 128) with one wave, two per lane.
 // Requirements (checked by the caller): 8-bit tier for every job, qlen <= 255,
 // no N base, the standard A/-B matrix, LDS for 16 * ceil((qmax + 1) / 16)
-// columns, qmax <= 16 * NQ - 1.  SYM: o_del == o_ins and e_del == e_ins (one
-// shared gap-open term).  `col` is this lane's word of column 0 (group layout,
-// see cpos) and `gcol` its word of column kLdsCols in the wave's HBM region
-// (used when NQ > 10).
-template <bool SYM, int NQ>
+// columns (at most LB blocks), qmax <= 16 * NQ - 1.  SYM: o_del == o_ins and
+// e_del == e_ins (one shared gap-open term).  `col` is this lane's word of
+// column 0 (group layout, see cpos) and `gcol` its word of column 16 * LB in
+// the wave's HBM region (used when LB < NQ).
+template <bool SYM, int NQ, int LB>
 __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *meta, int count,
                             uint32_t *col, uint32_t *gcol)
 {
-    constexpr bool HYB = NQ * 16 > kLdsCols;
-    constexpr int LB = kLdsCols / 16;                          // blocks held in LDS
+    constexpr bool HYB = LB < NQ;
+    constexpr int LC = HYB ? LB * 16 : 0;                      // ColRef: columns in LDS
     const int lane = threadIdx.x & 63;
     PkJob A, B;
     pk_setup(A, d, p, meta, lane, count);
     pk_setup(B, d, p, meta, lane + 64, count);
-    const ColRef<uint32_t, HYB> cw{col, gcol};
-    const ColRef<uint16_t, HYB> ca{reinterpret_cast<uint16_t *>(col), reinterpret_cast<uint16_t *>(gcol)};
-    const ColRef<uint16_t, HYB> cbh{ca.l + 1, ca.g + 1};
+    const ColRef<uint32_t, LC> cw{col, gcol};
+    const ColRef<uint16_t, LC> ca{reinterpret_cast<uint16_t *>(col), reinterpret_cast<uint16_t *>(gcol)};
+    const ColRef<uint16_t, LC> cbh{ca.l + 1, ca.g + 1};
     // Per 16-base query word, the "differs from base c" bit masks of every
     // c, held in registers for the whole item: N01 = ne0 | ne1 << 16 and
     // N23 = ne2 | ne3 << 16.  A row picks its target base's mask with one
@@ -1046,13 +1047,13 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
 // query masks in registers, three passes per block, 4-column LDS groups, two
 // word sets in turn); requirements as there with qmax <= 16 * NQ - 1 and
 // scores below 65536.
-template <bool SYM, int NQ>
+template <bool SYM, int NQ, int LB>
 __device__ void ext_wave_ln(const ExtDev &d, const ExtParams &p, const PkMeta *meta, int count,
                             uint32_t *col, uint32_t *gcol)
 {
-    constexpr bool HYB = NQ * 16 > kLdsCols;
-    constexpr int LB = kLdsCols / 16;
-    const ColRef<uint32_t, HYB> cw{col, gcol};
+    constexpr bool HYB = LB < NQ;
+    constexpr int LC = HYB ? LB * 16 : 0;
+    const ColRef<uint32_t, LC> cw{col, gcol};
     const int lane = threadIdx.x & 63;
     PkJob A, B;
     pk_setup(A, d, p, meta, lane, count);
@@ -1162,14 +1163,14 @@ __device__ void ext_wave_ln(const ExtDev &d, const ExtParams &p, const PkMeta *m
 
 // Persistent kernel of the one-job-per-lane form: items of up to 64 jobs, no
 // N base (the planner bins N-holding jobs apart).
-template <bool SYM, int NQ>
+template <bool SYM, int NQ, int LB>
 __global__ __launch_bounds__(64) void ext_persistent_ln_kernel(ExtDev d, ExtParams p,
                                                                const WorkItem *items, int n_items,
                                                                uint32_t *counter, uint32_t *gscratch)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     uint32_t *col = reinterpret_cast<uint32_t *>(lds_raw) + threadIdx.x * 4;
-    uint32_t *gcol = gscratch ? gscratch + (size_t)blockIdx.x * ext_hyb_words() + threadIdx.x * 4
+    uint32_t *gcol = gscratch ? gscratch + (size_t)blockIdx.x * ext_hyb_words(NQ, LB) + threadIdx.x * 4
                               : nullptr;
     for (;;) {
         int item = 0;
@@ -1177,47 +1178,22 @@ __global__ __launch_bounds__(64) void ext_persistent_ln_kernel(ExtDev d, ExtPara
         item = __builtin_amdgcn_readfirstlane(item);
         if (item >= n_items) break;
         const WorkItem wi = items[item];
-        ext_wave_ln<SYM, NQ>(d, p, d.pk + wi.start, (int)wi.count, col, gcol);
+        ext_wave_ln<SYM, NQ, LB>(d, p, d.pk + wi.start, (int)wi.count, col, gcol);
     }
-}
-
-hipError_t launch_ext_persistent_ln(const ExtDev &d, const ExtParams &p, const WorkItem *items,
-                                    int n_items, uint32_t *counter, int cols, int grid,
-                                    void *gscratch, hipStream_t s)
-{
-    if (n_items <= 0) return hipSuccess;
-    if (cols < 1 || cols > 256) return hipErrorInvalidValue;
-    if (cols > kLdsCols && !gscratch) return hipErrorInvalidValue;
-    const bool sym = p.o_del == p.o_ins && p.e_del == p.e_ins;
-    const int nq = (cols + 15) / 16;
-    const size_t lds = ext_pk_lds_bytes(cols);
-    uint32_t *gs = static_cast<uint32_t *>(gscratch);
-#define HIPSW_LN_LAUNCH(S, Q)                                                                   \
-    hipLaunchKernelGGL((ext_persistent_ln_kernel<S, Q>), dim3(grid), dim3(64), lds, s, d, p,   \
-                       items, n_items, counter, gs)
-    if (nq <= 4) {
-        if (sym) HIPSW_LN_LAUNCH(true, 4); else HIPSW_LN_LAUNCH(false, 4);
-    } else if (nq <= 10) {
-        if (sym) HIPSW_LN_LAUNCH(true, 10); else HIPSW_LN_LAUNCH(false, 10);
-    } else {
-        if (sym) HIPSW_LN_LAUNCH(true, 16); else HIPSW_LN_LAUNCH(false, 16);
-    }
-#undef HIPSW_LN_LAUNCH
-    return hipGetLastError();
 }
 
 // Persistent kernel of the packed form: items of up to 128 jobs, no N base
 // (the planner bins N-holding jobs apart).  The block's LDS is exactly the
-// column state (no static shared memory), so four waves of 150-bp queries fill
-// a 160 KiB CU.
-template <bool SYM, int NQ>
+// column state (no static shared memory), so a CU holds as many waves as their
+// LDS shares allow (150-bp queries: four held whole, six as 96 + 64 columns).
+template <bool SYM, int NQ, int LB>
 __global__ __launch_bounds__(64) void ext_persistent_pk_kernel(ExtDev d, ExtParams p,
                                                                const WorkItem *items, int n_items,
                                                                uint32_t *counter, uint32_t *gscratch)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     uint32_t *col = reinterpret_cast<uint32_t *>(lds_raw) + threadIdx.x * 4;
-    uint32_t *gcol = gscratch ? gscratch + (size_t)blockIdx.x * ext_hyb_words() + threadIdx.x * 4
+    uint32_t *gcol = gscratch ? gscratch + (size_t)blockIdx.x * ext_hyb_words(NQ, LB) + threadIdx.x * 4
                               : nullptr;
     for (;;) {
         int item = 0;
@@ -1225,7 +1201,7 @@ __global__ __launch_bounds__(64) void ext_persistent_pk_kernel(ExtDev d, ExtPara
         item = __builtin_amdgcn_readfirstlane(item);
         if (item >= n_items) break;
         const WorkItem wi = items[item];
-        ext_wave_pk<SYM, NQ>(d, p, d.pk + wi.start, (int)wi.count, col, gcol);
+        ext_wave_pk<SYM, NQ, LB>(d, p, d.pk + wi.start, (int)wi.count, col, gcol);
     }
 }
 
@@ -1241,39 +1217,91 @@ void build_pk_meta(int n, const uint32_t *order, const uint32_t *q_word, const u
     }
 }
 
-size_t ext_pk_lds_bytes(int cols)
+// Query words of the kernel instance that runs a bin whose longest query needs
+// `cols` columns: 4, 10 or 16.
+int ext_pk_nq(int cols)
 {
-    return (size_t)std::min((cols + 15) / 16 * 16, kLdsCols) * 64 * 4;
+    const int nq = (cols + 15) / 16;
+    return nq <= 4 ? 4 : nq <= 10 ? 10 : 16;
 }
 
-size_t ext_pk_hbm_bytes_per_wave(int cols)
+int ext_pk_default_lb(int cols) { return std::min(ext_pk_nq(cols), 10); }
+
+// Instances built: blocks held in LDS per query-word class.
+bool ext_pk_lb_supported(int kind, int cols, int lb)
 {
-    return cols > kLdsCols ? (size_t)ext_hyb_words() * 4 : 0;
+    if (cols < 1 || cols > 256) return false;
+    const int nq = ext_pk_nq(cols);
+    if (lb == ext_pk_default_lb(cols)) return true;
+    if (kind != EXT_PACKED) return false;
+    return (nq == 10 && (lb == 5 || lb == 6)) || (nq == 16 && (lb == 6 || lb == 8));
 }
 
-hipError_t launch_ext_persistent_pk(const ExtDev &d, const ExtParams &p, const WorkItem *items,
-                                    int n_items, uint32_t *counter, int cols, int grid,
+int ext_pk_lb_candidates(int kind, int cols, int *out)
+{
+    static const int cand[] = {10, 8, 6, 5, 4};
+    int n = 0;
+    for (int lb : cand)
+        if (ext_pk_lb_supported(kind, cols, lb)) out[n++] = lb;
+    return n;
+}
+
+size_t ext_pk_lds_bytes(int cols, int lb)
+{
+    return (size_t)std::min((cols + 15) / 16 * 16, 16 * lb) * 64 * 4;
+}
+
+size_t ext_pk_hbm_bytes_per_wave(int cols, int lb)
+{
+    return (cols + 15) / 16 > lb ? ext_hyb_words(ext_pk_nq(cols), lb) * 4 : 0;
+}
+
+#define HIPSW_EXT_LAUNCH(KER, S, Q, B)                                                          \
+    hipLaunchKernelGGL((KER<S, Q, B>), dim3(grid), dim3(64), lds, s, d, p, items, n_items,      \
+                       counter, gs)
+#define HIPSW_EXT_SYM(KER, Q, B)                                                                \
+    do {                                                                                        \
+        if (sym) HIPSW_EXT_LAUNCH(KER, true, Q, B); else HIPSW_EXT_LAUNCH(KER, false, Q, B);   \
+    } while (0)
+
+hipError_t launch_ext_persistent_ln(const ExtDev &d, const ExtParams &p, const WorkItem *items,
+                                    int n_items, uint32_t *counter, int cols, int lb, int grid,
                                     void *gscratch, hipStream_t s)
 {
     if (n_items <= 0) return hipSuccess;
-    if (cols < 1 || cols > 256) return hipErrorInvalidValue;
-    if (cols > kLdsCols && !gscratch) return hipErrorInvalidValue;
-    uint32_t *gs = static_cast<uint32_t *>(gscratch);
+    if (!ext_pk_lb_supported(EXT_LANE, cols, lb)) return hipErrorInvalidValue;
+    if (ext_pk_hbm_bytes_per_wave(cols, lb) && !gscratch) return hipErrorInvalidValue;
     const bool sym = p.o_del == p.o_ins && p.e_del == p.e_ins;
-    const int nq = (cols + 15) / 16;             // query words of the longest query
-    const size_t lds = ext_pk_lds_bytes(cols);
-#define HIPSW_PK_LAUNCH(S, Q)                                                                   \
-    hipLaunchKernelGGL((ext_persistent_pk_kernel<S, Q>), dim3(grid), dim3(64), lds, s, d, p,   \
-                       items, n_items, counter, gs)
-    if (nq <= 4) {
-        if (sym) HIPSW_PK_LAUNCH(true, 4); else HIPSW_PK_LAUNCH(false, 4);
-    } else if (nq <= 10) {
-        if (sym) HIPSW_PK_LAUNCH(true, 10); else HIPSW_PK_LAUNCH(false, 10);
-    } else {
-        if (sym) HIPSW_PK_LAUNCH(true, 16); else HIPSW_PK_LAUNCH(false, 16);
-    }
-#undef HIPSW_PK_LAUNCH
+    const int nq = ext_pk_nq(cols);
+    const size_t lds = ext_pk_lds_bytes(cols, lb);
+    uint32_t *gs = static_cast<uint32_t *>(gscratch);
+    if (nq == 4) HIPSW_EXT_SYM(ext_persistent_ln_kernel, 4, 4);
+    else if (nq == 10) HIPSW_EXT_SYM(ext_persistent_ln_kernel, 10, 10);
+    else HIPSW_EXT_SYM(ext_persistent_ln_kernel, 16, 10);
     return hipGetLastError();
 }
+
+hipError_t launch_ext_persistent_pk(const ExtDev &d, const ExtParams &p, const WorkItem *items,
+                                    int n_items, uint32_t *counter, int cols, int lb, int grid,
+                                    void *gscratch, hipStream_t s)
+{
+    if (n_items <= 0) return hipSuccess;
+    if (!ext_pk_lb_supported(EXT_PACKED, cols, lb)) return hipErrorInvalidValue;
+    if (ext_pk_hbm_bytes_per_wave(cols, lb) && !gscratch) return hipErrorInvalidValue;
+    uint32_t *gs = static_cast<uint32_t *>(gscratch);
+    const bool sym = p.o_del == p.o_ins && p.e_del == p.e_ins;
+    const int nq = ext_pk_nq(cols);             // query words of the longest query
+    const size_t lds = ext_pk_lds_bytes(cols, lb);
+    if (nq == 4) HIPSW_EXT_SYM(ext_persistent_pk_kernel, 4, 4);
+    else if (nq == 10 && lb == 10) HIPSW_EXT_SYM(ext_persistent_pk_kernel, 10, 10);
+    else if (nq == 10 && lb == 6) HIPSW_EXT_SYM(ext_persistent_pk_kernel, 10, 6);
+    else if (nq == 10) HIPSW_EXT_SYM(ext_persistent_pk_kernel, 10, 5);
+    else if (lb == 10) HIPSW_EXT_SYM(ext_persistent_pk_kernel, 16, 10);
+    else if (lb == 8) HIPSW_EXT_SYM(ext_persistent_pk_kernel, 16, 8);
+    else HIPSW_EXT_SYM(ext_persistent_pk_kernel, 16, 6);
+    return hipGetLastError();
+}
+#undef HIPSW_EXT_SYM
+#undef HIPSW_EXT_LAUNCH
 
 }  // namespace hipsw

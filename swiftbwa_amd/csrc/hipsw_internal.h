@@ -75,22 +75,32 @@ hipError_t launch_ext_persistent(int tier, bool global_state, const ExtDev &d,
 hipError_t launch_zero_u32(uint32_t *p, int n, hipStream_t s);
 size_t ext_state_bytes(int tier);   // bytes of h/e state per column per lane
 size_t ext_lds_bytes(int tier, bool global_state, int cols);   // dynamic LDS per block
-// Packed form (two jobs per lane, 8-bit tier, LDS state, items of <= 128 jobs).
-size_t ext_pk_lds_bytes(int cols);
-// HBM bytes per resident wave of the packed / one-per-lane forms for queries
-// longer than the LDS holds (0 otherwise).
-size_t ext_pk_hbm_bytes_per_wave(int cols);
+// Kernel families of a batched extension bin.
+enum ExtKind {
+    EXT_UNPACKED = 0,     // one job per lane, any matrix / N / tier (ext_persistent_kernel)
+    EXT_PACKED = 1,       // two jobs per lane, 8-bit tier, items of 128 jobs
+    EXT_LANE = 2,         // one job per lane, 8/16-bit tier, packed-form structure
+};
+// Packed and one-per-lane forms: a bin whose longest query needs `cols`
+// columns runs the instance for ext_pk_nq(cols) query words with `lb`
+// 16-column blocks of its state in LDS and the rest in HBM.
+int ext_pk_nq(int cols);
+int ext_pk_default_lb(int cols);
+bool ext_pk_lb_supported(int kind, int cols, int lb);
+int ext_pk_lb_candidates(int kind, int cols, int *out);   // out: up to 5 entries
+size_t ext_pk_lds_bytes(int cols, int lb);                // LDS bytes per wave
+size_t ext_pk_hbm_bytes_per_wave(int cols, int lb);       // HBM state bytes per resident wave
 // Descriptors of the packed form, in `order`.
 void build_pk_meta(int n, const uint32_t *order, const uint32_t *q_word, const uint32_t *t_word,
                    const uint32_t *qn_word, const uint32_t *tn_word, const int32_t *q_len,
                    const int32_t *t_len, const int32_t *h0, const int32_t *w, const int32_t *eb,
                    PkMeta *out);
 hipError_t launch_ext_persistent_pk(const ExtDev &d, const ExtParams &p, const WorkItem *items,
-                                    int n_items, uint32_t *counter, int cols, int grid,
+                                    int n_items, uint32_t *counter, int cols, int lb, int grid,
                                     void *gscratch, hipStream_t s);
 // One-job-per-lane form (8- and 16-bit tiers, no N, standard matrix, items of
 // <= 64 jobs, descriptors in d.pk); same LDS size as the packed form.
 hipError_t launch_ext_persistent_ln(const ExtDev &d, const ExtParams &p, const WorkItem *items,
-                                    int n_items, uint32_t *counter, int cols, int grid,
+                                    int n_items, uint32_t *counter, int cols, int lb, int grid,
                                     void *gscratch, hipStream_t s);
 }  // namespace hipsw

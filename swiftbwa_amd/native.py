@@ -58,7 +58,8 @@ class HipswPacRef(C.Structure):
 
 class HipswTuneEntry(C.Structure):
     _fields_ = [("tier", C.c_int32), ("length_class", C.c_int32), ("jobs", C.c_int32),
-                ("chosen", C.c_int32), ("ms_packed", C.c_double), ("ms_lane", C.c_double)]
+                ("kind", C.c_int32), ("lds_blocks", C.c_int32), ("chosen", C.c_int32),
+                ("ms", C.c_double)]
 
 
 class HipswChunk(C.Structure):
@@ -109,6 +110,7 @@ _SIGS = {
                                       _P, C.POINTER(HipswScoring), C.POINTER(HipswExtOpts), _P,
                                       _P, C.POINTER(HipswStats)]),
     "hipsw_ext_autotune": (C.c_int, [_P, _P, C.c_int32, _P, C.c_int32, _P]),
+    "hipsw_ext_set_layout": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32]),
     "hipsw_nw_upload": (_P, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P, _P, C.c_int32,
                              C.c_int32, C.POINTER(HipswScoring)]),
     "hipsw_nw_run": (C.c_int, [_P, _P, C.c_int32, C.POINTER(HipswStats)]),

@@ -168,13 +168,22 @@ def test_autotune_both_layouts_exact(engine):
         assert lib.hipsw_ext_run(engine.handle, st, native.MODE_BATCHED, C.byref(s)) == 0
         assert lib.hipsw_ext_download(engine.handle, st, native.ptr(out)) == 0
         assert np.array_equal(out, ref)
-        rep = (native.HipswTuneEntry * 8)()
+        rep = (native.HipswTuneEntry * 32)()
         nrep = C.c_int32()
-        assert lib.hipsw_ext_autotune(engine.handle, st, 6000, rep, 8, C.byref(nrep)) == 0
-        assert 1 <= nrep.value <= 3
-        for e in rep[:nrep.value]:
-            assert e.tier == 0 and e.chosen in (1, 2) and e.jobs > 0
-            assert e.ms_packed > 0 and e.ms_lane > 0
+        assert lib.hipsw_ext_autotune(engine.handle, st, 6000, rep, 32, C.byref(nrep)) == 0
+        ents = rep[:nrep.value]
+        classes = sorted({e.length_class for e in ents})
+        assert 1 <= len(classes) <= 3
+        for c in classes:
+            es = [e for e in ents if e.length_class == c]
+            assert sum(e.chosen for e in es) == 1
+            assert es[-1].kind == 2                          # one job per lane runs last
+            assert {e.kind for e in es} == {1, 2}
+            for e in es:
+                assert e.tier == 0 and e.jobs > 0 and e.ms > 0 and 4 <= e.lds_blocks <= 10
+        # every LDS/HBM split of the 64-159 and 160-255 classes was timed
+        assert {e.lds_blocks for e in ents if e.length_class == 1 and e.kind == 1} == {10, 6, 5}
+        assert {e.lds_blocks for e in ents if e.length_class == 2 and e.kind == 1} == {10, 8, 6}
         assert lib.hipsw_ext_download(engine.handle, st, native.ptr(out)) == 0
         assert np.array_equal(out, ref)                       # lane-layout results
         assert lib.hipsw_ext_run(engine.handle, st, native.MODE_BATCHED, C.byref(s)) == 0
@@ -182,6 +191,51 @@ def test_autotune_both_layouts_exact(engine):
         assert np.array_equal(out, ref)                       # re-planned set
     finally:
         lib.hipsw_ext_free(engine.handle, st)
+
+
+def test_every_layout_exact(engine):
+    """Every built layout of the register-query kernels -- the packed form at
+    each LDS/HBM split of the column state and one job per lane -- pinned per
+    length class through hipsw_ext_set_layout, on a set that fills every class
+    (queries at the split edges 79..81 and 95..97 included)."""
+    js = wl.generate(wl.EXT_MIXED, 11, 12000)
+    rng = np.random.default_rng(3)
+    extra = []
+    for ql in (79, 80, 81, 95, 96, 97, 127, 128, 129, 175, 176, 177, 250):
+        for rep in range(40):
+            q = rng.integers(0, 4, ql).astype(np.uint8)
+            t = q.copy()
+            flip = rng.random(ql) < 0.03
+            t[flip] = (t[flip] + 1) % 4
+            extra.append((q, np.concatenate([t, rng.integers(0, 4, rep * 3).astype(np.uint8)])))
+    qs = [js["queries"][js["q_off"][k]:js["q_off"][k] + js["q_len"][k]] for k in range(js["n"])]
+    ts = [js["targets"][js["t_off"][k]:js["t_off"][k] + js["t_len"][k]] for k in range(js["n"])]
+    qs += [q for q, _ in extra]
+    ts += [t for _, t in extra]
+    n = len(qs)
+    ql = np.array([len(x) for x in qs], np.uint16); tl = np.array([len(x) for x in ts], np.uint16)
+    qo = np.zeros(n, np.uint32); to = np.zeros(n, np.uint32)
+    qo[1:] = np.cumsum(ql.astype(np.int64))[:-1]; to[1:] = np.cumsum(tl.astype(np.int64))[:-1]
+    h0 = np.concatenate([js["h0"], rng.integers(1, 60, len(extra)).astype(np.int32)])
+    w = np.concatenate([js["w"], np.full(len(extra), 100, np.int32)])
+    eb = np.concatenate([js["end_bonus"], np.full(len(extra), 5, np.int32)])
+    big = {"n": n, "queries": np.concatenate(qs), "q_off": qo, "q_len": ql,
+           "targets": np.concatenate(ts), "t_off": to, "t_len": tl, "h0": h0, "w": w,
+           "end_bonus": eb}
+    lib = engine.lib
+    assert lib.hipsw_ext_set_layout(engine.handle, 1, 1, 7) == native.HIPSW_E_INVALID
+    assert lib.hipsw_ext_set_layout(engine.handle, 1, 2, 5) == native.HIPSW_E_INVALID
+    assert lib.hipsw_ext_set_layout(engine.handle, 3, 1, 10) == native.HIPSW_E_INVALID
+    combos = [((1, 4), (1, 10), (1, 10)), ((2, 4), (1, 6), (1, 8)),
+              ((1, 4), (1, 5), (1, 6)), ((2, 4), (2, 10), (2, 10))]
+    try:
+        for combo in combos:
+            for cls, (kind, lb) in enumerate(combo):
+                assert lib.hipsw_ext_set_layout(engine.handle, cls, kind, lb) == 0
+            check(engine, big, modes=(native.MODE_BATCHED,))
+    finally:
+        for cls in range(3):
+            lib.hipsw_ext_set_layout(engine.handle, cls, 0, 0)
 
 
 def test_size_and_tier_boundaries(engine):
