@@ -14,6 +14,8 @@
 // touches 64 consecutive words (conflict-free).  Scores are held in the
 // narrowest exact tier: two u8 (16-bit word), two u16 (32-bit word) or two i32.
 // Queries and targets are read as 2-bit words from HBM (16 bases per load).
+#include <type_traits>
+
 #include "hipsw_internal.h"
 
 namespace hipsw {
@@ -849,6 +851,65 @@ __device__ __forceinline__ void store_block(uint32_t *col, uint32_t *gcol, int b
     }
 }
 
+// Groups g .. g + 3 of this lane (4-column groups, 128-bit accesses); each
+// group index is clamped to glast (a read past the row's last group repeats it:
+// in bounds, unused; g <= glast).  Groups below GL live in LDS, the rest in the
+// wave's HBM region (GL = 0: all in LDS).  At most one block per row straddles
+// the two; it takes a uniform branch per group.
+template <int GL>
+__device__ __forceinline__ void load_groups(const uint32_t *col, const uint32_t *gcol, int g,
+                                            int glast, uint32_t (&wv)[16])
+{
+    const lds_u4 *lc = (const lds_u4 *)col;
+    const glb_u4 *gc = (const glb_u4 *)gcol;
+    int gq[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) gq[q] = min(g + q, glast);
+    u32x4 v[4];
+    if (GL == 0 || gq[3] < GL) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = lc[gq[q] * 64];
+    } else if (g >= GL) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = gc[(gq[q] - GL) * 64];
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (gq[q] < GL) v[q] = lc[gq[q] * 64];
+            else v[q] = gc[(gq[q] - GL) * 64];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        wv[4 * q] = v[q].x; wv[4 * q + 1] = v[q].y; wv[4 * q + 2] = v[q].z; wv[4 * q + 3] = v[q].w;
+    }
+}
+
+// Stores groups g .. g + NG - 1 (all inside the row's sweep).
+template <int GL, int NG>
+__device__ __forceinline__ void store_groups(uint32_t *col, uint32_t *gcol, int g,
+                                             const uint32_t (&wv)[16])
+{
+    lds_u4 *lc = (lds_u4 *)col;
+    glb_u4 *gc = (glb_u4 *)gcol;
+    if (GL == 0 || g + NG - 1 < GL) {
+#pragma unroll
+        for (int q = 0; q < NG; ++q)
+            lc[(g + q) * 64] = u32x4{wv[4 * q], wv[4 * q + 1], wv[4 * q + 2], wv[4 * q + 3]};
+    } else if (g >= GL) {
+#pragma unroll
+        for (int q = 0; q < NG; ++q)
+            gc[(g + q - GL) * 64] = u32x4{wv[4 * q], wv[4 * q + 1], wv[4 * q + 2], wv[4 * q + 3]};
+    } else {
+#pragma unroll
+        for (int q = 0; q < NG; ++q) {
+            const u32x4 v{wv[4 * q], wv[4 * q + 1], wv[4 * q + 2], wv[4 * q + 3]};
+            if (g + q < GL) lc[(g + q) * 64] = v;
+            else gc[(g + q - GL) * 64] = v;
+        }
+    }
+}
+
 __device__ __forceinline__ void pk_finish(const PkJob &J, const ExtDev &d)
 {
     if (!J.valid) return;                                      // :183-190
@@ -876,6 +937,7 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
 {
     constexpr bool HYB = LB < NQ;
     constexpr int LC = HYB ? LB * 16 : 0;                      // ColRef: columns in LDS
+    constexpr int GL = LC / 4;                                 // groups in LDS (0: all)
     const int lane = threadIdx.x & 63;
     PkJob A, B;
     pk_setup(A, d, p, meta, lane, count);
@@ -938,29 +1000,38 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
         // per-half band end (columns >= it commit nothing); 0 for a finished job
         const us2 L = us2{(unsigned short)(A.alive ? A.end : 0), (unsigned short)(B.alive ? B.end : 0)};
 
-        // One 16-column block, in three passes so that the packed ops of one
-        // cell need not wait on each other: (1) everything a cell derives from
-        // its own state word and base pair -- diag (:117-119), max(diag, E)
-        // and the new E (:128-129); (2) the running F and h1 (:121-123,
-        // :130-131); (3) the row-maximum candidates H * 256 + k (:124-125).
-        auto block = [&](uint32_t (&wv)[16], int b) {
+        // Each row sweeps the 4-column groups from the wave's lowest band start
+        // to its highest band end in blocks of 16 columns (the last one 4, 8
+        // or 12 wide).  A block starts at column 16 q + o of query word q, o in
+        // {0, 4, 8, 12} the same for the whole row, so its 32 mismatch bits
+        // (16 per job) come from the masks of words q and q + 1.
+        auto qmask = [&](int b) {   // packed "differs" bits of query word b, this row's target bases
             const uint32_t a01 = NA01[b], a23 = NA23[b], b01 = NB01[b], b23 = NB23[b];
             const uint32_t ma = (hiA ? a23 : a01) >> shA;
             const uint32_t mb = (hiB ? b23 : b01) >> shB;
-            const uint32_t mm = __builtin_amdgcn_perm(mb, ma, 0x05040100u);   // ma.lo16 | mb.lo16 << 16
-            const int c0 = b * 16;
+            return __builtin_amdgcn_perm(mb, ma, 0x05040100u);   // ma.lo16 | mb.lo16 << 16
+        };
+        // One block of N columns from c0, in three passes so that the packed
+        // ops of one cell need not wait on each other: (1) everything a cell
+        // derives from its own state word and base pair -- diag (:117-119),
+        // max(diag, E) and the new E (:128-129); (2) the running F and h1
+        // (:121-123, :130-131); (3) the row-maximum candidates H * 256 + k
+        // (:124-125).  MASKED: commit only left of each half's band end.
+        auto block = [&](auto NC, auto MK, uint32_t (&wv)[16], int c0, uint32_t mm) {
+            constexpr int N = decltype(NC)::value;
+            constexpr bool MASKED = decltype(MK)::value;
             us2 dgE[16], tf[16], ne[16], Hs[16], M[16], E[16], s[16];
 #pragma unroll
-            for (int k = 0; k < 16; ++k) {
+            for (int k = 0; k < N; ++k) {
                 const uint32_t ab = (mm >> k) & 0x00010001u;          // per-half mismatch flag
                 s[k] = __builtin_bit_cast(us2, __builtin_bit_cast(ss2, ab) * nAB + Ssc);
                 M[k] = u2(wv[k] & LOW);
                 E[k] = u2(__builtin_amdgcn_perm(0u, wv[k], 0x0C030C01u));   // bytes 1, 3
             }
 #pragma unroll
-            for (int k = 0; k < 16; ++k) s[k] = s[k] * vmin(M[k], ONE) + M[k];   // M ? M + s : 0
+            for (int k = 0; k < N; ++k) s[k] = s[k] * vmin(M[k], ONE) + M[k];   // M ? M + s : 0
 #pragma unroll
-            for (int k = 0; k < 16; ++k) {
+            for (int k = 0; k < N; ++k) {
                 const us2 dg = __builtin_bit_cast(
                     us2, __builtin_elementwise_max(__builtin_bit_cast(ss2, s[k]), ss2{0, 0}));
                 dgE[k] = vmax(dg, E[k]);
@@ -968,10 +1039,9 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
                 ne[k] = vmax(vsubs(E[k], eD), t);
                 tf[k] = SYM ? t : vsubs(dg, oeI);
             }
-            if (c0 + 16 <= mhi) {
-                // left of every live band end: no masking
+            if (!MASKED) {
 #pragma unroll
-                for (int k = 0; k < 16; ++k) {
+                for (int k = 0; k < N; ++k) {
                     const us2 H = vmax(dgE[k], f);
                     f = vmax(vsubs(f, eI), tf[k]);
                     wv[k] = __builtin_amdgcn_perm(w32(ne[k]), w32(h1), 0x06020400u);
@@ -979,17 +1049,16 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
                     Hs[k] = H * C256 + splat(k);
                 }
             } else {
-                // commit only left of each half's band end
                 const us2 Lb = vsubs(L, splat(c0));
                 uint32_t hms[16];
 #pragma unroll
-                for (int k = 0; k < 16; ++k) {
+                for (int k = 0; k < N; ++k) {
                     hms[k] = w32(__builtin_bit_cast(
                         us2, __builtin_bit_cast(ss2, splat(k) - Lb) >> ss2{15, 15}));
                     asm volatile("" : "+v"(hms[k]));   // keep the selects bitwise
                 }
 #pragma unroll
-                for (int k = 0; k < 16; ++k) {
+                for (int k = 0; k < N; ++k) {
                     const uint32_t hm = hms[k];
                     const us2 H = vmax(dgE[k], f);
                     f = vmax(vsubs(f, eI), tf[k]);
@@ -999,29 +1068,65 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
                     Hs[k] = u2(w32(H * C256 + splat(k)) & hm);
                 }
             }
-            store_block<HYB, LB>(col, gcol, b, wv);
+            store_groups<GL, N / 4>(col, gcol, c0 >> 2, wv);
 #pragma unroll
-            for (int k = 0; k < 8; ++k) Hs[k] = vmax(Hs[k], Hs[k + 8]);
+            for (int st = 8; st >= 1; st >>= 1)
 #pragma unroll
-            for (int k = 0; k < 4; ++k) Hs[k] = vmax(Hs[k], Hs[k + 4]);
-            const us2 kb = vmax(vmax(Hs[0], Hs[2]), vmax(Hs[1], Hs[3]));
-            key = vmax(key, kb + splat(c0));
+                for (int k = 0; k < st; ++k)
+                    if (k + st < N) Hs[k] = vmax(Hs[k], Hs[k + st]);
+            key = vmax(key, Hs[0] + splat(c0));
         };
-        auto load = [&](uint32_t (&wv)[16], int b) { load_block<HYB, LB>(col, gcol, b, wv); };
+        using N16 = std::integral_constant<int, 16>;
+        using Unmasked = std::integral_constant<bool, false>;
+        using Masked = std::integral_constant<bool, true>;
+        auto full = [&](uint32_t (&wv)[16], int c0, uint32_t mm) {
+            if (c0 + 16 <= mhi) block(N16{}, Unmasked{}, wv, c0, mm);   // left of every live band end
+            else block(N16{}, Masked{}, wv, c0, mm);
+        };
         if (jlo < jhi) {
+            const int g0 = jlo >> 2, gend = (jhi + 3) >> 2, glast = gend - 1;
+            const int nfull = (gend - g0) >> 2, tail = (gend - g0) & 3;
+            const int o = (g0 & 3) * 4;
+            const us2 osh = splat(o), ish = splat((16 - o) & 15);
+            const uint32_t ohi = o ? 0xFFFFFFFFu : 0u;
+            int q = g0 >> 2;
+            uint32_t mcur = qmask(q);
+            // mismatch bits of the block at column 16 q + o; moves to the next word
+            auto next_mm = [&]() {
+                const uint32_t mnext = qmask(min(q + 1, NQ - 1));
+                const uint32_t m = w32(u2(mcur) >> osh) | (w32(u2(mnext) << ish) & ohi);
+                mcur = mnext;
+                ++q;
+                return m;
+            };
             // two word sets in turn: the next block's reads are in flight
             // while the current block computes
-            // (a read past the last block repeats the last block: in bounds, unused)
-            const int bb = jlo >> 4, be = (jhi - 1) >> 4;
             uint32_t W0[16], W1[16];
-            load(W0, bb);
-            for (int b = bb;; b += 2) {
-                load(W1, min(b + 1, be));
-                block(W0, b);
-                if (b + 1 > be) break;
-                load(W0, min(b + 2, be));
-                block(W1, b + 1);
-                if (b + 2 > be) break;
+            int g = g0;
+            bool in_w1 = false;
+            load_groups<GL>(col, gcol, g, glast, W0);
+            for (int t = 0; t < nfull;) {
+                load_groups<GL>(col, gcol, min(g + 4, glast), glast, W1);
+                full(W0, 4 * g, next_mm());
+                g += 4;
+                if (++t >= nfull) {
+                    in_w1 = true;
+                    break;
+                }
+                load_groups<GL>(col, gcol, min(g + 4, glast), glast, W0);
+                full(W1, 4 * g, next_mm());
+                g += 4;
+                ++t;
+            }
+            if (tail) {
+                if (in_w1) {
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) W0[k] = W1[k];
+                }
+                const uint32_t m = next_mm();
+                if (tail == 1) block(std::integral_constant<int, 4>{}, Masked{}, W0, 4 * g, m);
+                else if (tail == 2) block(std::integral_constant<int, 8>{}, Masked{}, W0, 4 * g, m);
+                else block(std::integral_constant<int, 12>{}, Masked{}, W0, 4 * g, m);
             }
         }
 
