@@ -247,6 +247,11 @@ def profile_leg(args):
     from swiftbwa_amd.metalsw import HipSWEngine, ScoringParameters, _ext_arrays
     eng = HipSWEngine(0)
     lib = eng.lib
+    if args.pk_lds_blocks:
+        # pin the packed layout of the 64-159 class (what the autotuner picks
+        # for the 150-bp set) so the profiled kernel is the benchmarked one
+        if lib.hipsw_ext_set_layout(eng.handle, 1, 1, args.pk_lds_blocks):
+            raise RuntimeError(eng.last_error())
     if args.profile_leg in ("ext", "mixed"):
         js = (wl.generate(wl.EXT_UNIFORM, 2, args.n) if args.profile_leg == "ext"
               else wl.generate(wl.EXT_MIXED, 4, args.mixed_n))
@@ -299,6 +304,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--mixed-n", type=int, default=4000000)
     ap.add_argument("--profile-leg", choices=["ext", "mixed", "copy"], default=None)
+    ap.add_argument("--pk-lds-blocks", type=int, default=0,
+                    help="profile leg: 16-column blocks of packed column state in LDS for "
+                         "64-159-base queries (10, 6 or 5; 0 = default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (each job set runs on one "

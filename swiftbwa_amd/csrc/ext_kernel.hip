@@ -922,6 +922,164 @@ __device__ __forceinline__ void pk_finish(const PkJob &J, const ExtDev &d)
     }
 }
 
+// ---- row-level helpers of the packed form ---------------------------------
+
+// Wave-uniform minima of three per-lane u32 values (all 64 lanes active).
+// Passing the identity as the DPP "old" value lets the compiler fuse each
+// step into one v_min_u32_dpp; the three chains are interleaved so no step
+// waits on the previous one.  Rows are reduced by quad_perm / mirror steps,
+// then row_bcast:15 and row_bcast:31 carry rows 0-2 into row 3; lane 63
+// holds the result.
+__device__ __forceinline__ void wave_min3(uint32_t &a, uint32_t &b, uint32_t &c)
+{
+#define HIPSW_MIN3_STEP(CTRL, RM)                                                             \
+    a = min(a, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)a, CTRL, RM, 0xF, false));      \
+    b = min(b, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)b, CTRL, RM, 0xF, false));      \
+    c = min(c, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)c, CTRL, RM, 0xF, false));
+    HIPSW_MIN3_STEP(0xB1, 0xF)      // quad_perm 1,0,3,2
+    HIPSW_MIN3_STEP(0x4E, 0xF)      // quad_perm 2,3,0,1
+    HIPSW_MIN3_STEP(0x141, 0xF)     // row_half_mirror
+    HIPSW_MIN3_STEP(0x140, 0xF)     // row_mirror
+    HIPSW_MIN3_STEP(0x142, 0xA)     // row_bcast:15 into rows 1, 3
+    HIPSW_MIN3_STEP(0x143, 0xC)     // row_bcast:31 into rows 2, 3
+#undef HIPSW_MIN3_STEP
+    a = __builtin_amdgcn_readlane(a, 63);
+    b = __builtin_amdgcn_readlane(b, 63);
+    c = __builtin_amdgcn_readlane(c, 63);
+}
+
+// The 16-bit (h, e) entry of column j in half `hs` (0: job A, 1: job B) of
+// this lane, in LDS for j < LC and in the wave's HBM region above (LC = 0:
+// all LDS).  With LC > 0 the pointer is generic (flat access), which avoids
+// a divergent branch for the few per-row accesses that use it.
+template <int LC>
+__device__ __forceinline__ uint16_t *col16(uint32_t *col, uint32_t *gcol, int j, int hs)
+{
+    uint16_t *l = reinterpret_cast<uint16_t *>(col);
+    if (LC == 0) return l + cpos<uint16_t>(j) + hs;
+    uint16_t *g = reinterpret_cast<uint16_t *>(gcol);
+    return j < LC ? l + cpos<uint16_t>(j) + hs : g + cpos<uint16_t>(j - LC) + hs;
+}
+
+// Group g (columns 4g..4g+3, both halves) of this lane's column state.
+template <int GL>
+__device__ __forceinline__ u32x4 group_rd(const uint32_t *col, const uint32_t *gcol, int g)
+{
+    if (GL == 0) return *(const lds_u4 *)(col + g * 256);
+    const u32x4 *p = g < GL ? reinterpret_cast<const u32x4 *>(col + g * 256)
+                            : reinterpret_cast<const u32x4 *>(gcol + (g - GL) * 256);
+    return *p;
+}
+
+// Non-zero columns of two groups: bit c (c < 4) = column c of va's low half
+// (job A) is non-zero, bit 16 + c = column c of vb's high half (job B).
+__device__ __forceinline__ uint32_t nz_bits(const u32x4 &va, const u32x4 &vb)
+{
+    const us2 one = u2(opaque(0x00010001u));   // keeps min(x, 1) one packed op
+    const uint32_t m0 = (va.x & 0xFFFFu) | (vb.x & 0xFFFF0000u);
+    const uint32_t m1 = (va.y & 0xFFFFu) | (vb.y & 0xFFFF0000u);
+    const uint32_t m2 = (va.z & 0xFFFFu) | (vb.z & 0xFFFF0000u);
+    const uint32_t m3 = (va.w & 0xFFFFu) | (vb.w & 0xFFFF0000u);
+    return w32(vmin(u2(m0), one)) | (w32(vmin(u2(m1), one)) << 1) |
+           (w32(vmin(u2(m2), one)) << 2) | (w32(vmin(u2(m3), one)) << 3);
+}
+
+// Row start of one half (:92-98, the h1 of :99-107 is made by the caller):
+// band clamp -- the column the w-clamp drops is cleared, which keeps every
+// entry below beg at zero -- band end, cell count.
+template <int LC>
+__device__ __forceinline__ void pk2_row_start(PkJob &J, int i, uint32_t *col, uint32_t *gcol, int hs)
+{
+    J.tb = (((i >> 4) & 1 ? J.two : J.twe) >> ((i & 15) * 2)) & 3u;   // this row's target base
+    const int wb = i - J.w;
+    if (J.alive && J.beg < wb) {
+        // beg = i - w (:96) drops column beg: with w >= 0, beg >= i - 1 - w
+        // after the previous row, so exactly one column per row
+        *col16<LC>(col, gcol, J.beg, hs) = 0;
+        J.beg = wb;
+    }
+    J.end = min(J.end, i + J.w + 1);                           // :97-98 (end <= qlen holds)
+    J.ncells += (J.alive && J.end > J.beg) ? (uint32_t)(J.end - J.beg) : 0u;
+}
+
+// Row end of one half (:145-172): band-end entry, global-score column, row
+// maximum, z-drop, and the last-row test of the next row (:90).
+template <int LC>
+__device__ __forceinline__ void pk2_row_end(PkJob &J, const ExtParams &p, int i, int h1, int key,
+                                            uint32_t *col, uint32_t *gcol, int hs)
+{
+    *col16<LC>(col, gcol, J.end, hs) = (uint16_t)h1;          // h[end] = h1, e[end] = 0
+    const int rm = key >> 8, rj = key & 0xFF;
+    const bool at_q = J.alive && J.end == J.qlen;              // :149-152
+    J.gtle = (at_q && !(J.gsc > h1)) ? i : J.gtle;
+    J.gsc = at_q ? max(J.gsc, h1) : J.gsc;
+    const bool up = J.alive && rm > J.maxv;                    // :158-163
+    const int di = i - J.max_i, dj = rj - J.max_j;             // :164-172
+    const int drop = di > dj ? J.maxv - rm - (di - dj) * p.e_del : J.maxv - rm - (dj - di) * p.e_ins;
+    const bool zd = p.zdrop > 0 && drop > p.zdrop;
+    J.max_off = up ? max(J.max_off, abs(rj - i)) : J.max_off;
+    J.max_i = up ? i : J.max_i;
+    J.max_j = up ? rj : J.max_j;
+    J.maxv = up ? rm : J.maxv;
+    J.alive = J.alive && rm != 0 && (up || !zd) && i + 1 < J.tlen;   // :155, :171, :90
+}
+
+// Band narrowing of both halves (:175-180), a 4-column group per trip and
+// half.  Forward: first column in [beg, end) whose (h, e) is non-zero, else
+// end.  Backward: last column in [beg, end] whose (h, e) is non-zero --
+// column end holds (h1, 0), so only a zero h1 searches -- then
+// end = min(that + 2, qlen); none found: beg - 1.
+template <int LC>
+__device__ __forceinline__ void pk2_narrow(PkJob &A, PkJob &B, int h1a, int h1b,
+                                           const uint32_t *col, const uint32_t *gcol)
+{
+    constexpr int GL = LC / 4;
+    bool fa = A.alive && A.beg < A.end, fb = B.alive && B.beg < B.end;
+    int ga = A.beg >> 2, gb = B.beg >> 2;
+    uint32_t ma = 0xFu << (A.beg & 3), mb = 0xFu << (B.beg & 3);   // columns >= beg
+    while (__builtin_amdgcn_ballot_w64(fa || fb)) {
+        // a lane done searching re-reads group 0 (its own index may have run
+        // past the state)
+        const uint32_t z = nz_bits(group_rd<GL>(col, gcol, fa ? ga : 0), group_rd<GL>(col, gcol, fb ? gb : 0));
+        const uint32_t za = z & ma, zb = (z >> 16) & mb;
+        if (fa) {
+            if (za) { A.beg = min(4 * ga + (int)__builtin_ctz(za), A.end); fa = false; }
+            else { ++ga; fa = 4 * ga < A.end; if (!fa) A.beg = A.end; }
+        }
+        if (fb) {
+            if (zb) { B.beg = min(4 * gb + (int)__builtin_ctz(zb), B.end); fb = false; }
+            else { ++gb; fb = 4 * gb < B.end; if (!fb) B.beg = B.end; }
+        }
+        ma = mb = 0xFu;
+    }
+    // backward: search from column end - 1 down to beg when h1 == 0
+    int ka = A.end, kb = B.end;                                 // h1 != 0: k = end
+    bool sa = A.alive && h1a == 0, sb = B.alive && h1b == 0;
+    ka -= sa;
+    kb -= sb;
+    sa = sa && ka >= A.beg;
+    sb = sb && kb >= B.beg;
+    int gka = max(ka, 0) >> 2, gkb = max(kb, 0) >> 2;
+    while (__builtin_amdgcn_ballot_w64(sa || sb)) {
+        const uint32_t z = nz_bits(group_rd<GL>(col, gcol, sa ? gka : 0), group_rd<GL>(col, gcol, sb ? gkb : 0));
+        // columns of the group in [beg, k]
+        const int la = A.beg - 4 * gka, lb = B.beg - 4 * gkb;
+        const uint32_t ra = (0xFu >> (3 - min(ka - 4 * gka, 3))) & (0xFu << max(min(la, 4), 0));
+        const uint32_t rb = (0xFu >> (3 - min(kb - 4 * gkb, 3))) & (0xFu << max(min(lb, 4), 0));
+        const uint32_t za = z & ra & 0xFu, zb = (z >> 16) & rb & 0xFu;
+        if (sa) {
+            if (za) { ka = 4 * gka + 31 - (int)__builtin_clz(za); sa = false; }
+            else { --gka; ka = 4 * gka + 3; sa = ka >= A.beg; if (!sa) ka = A.beg - 1; }
+        }
+        if (sb) {
+            if (zb) { kb = 4 * gkb + 31 - (int)__builtin_clz(zb); sb = false; }
+            else { --gkb; kb = 4 * gkb + 3; sb = kb >= B.beg; if (!sb) kb = B.beg - 1; }
+        }
+    }
+    if (A.alive) A.end = min(ka + 2, A.qlen);
+    if (B.alive) B.end = min(kb + 2, B.qlen);
+}
+
 }  // namespace
 
 // Extends jobs idx[0..count) (count <= 128) with one wave, two per lane.
@@ -943,8 +1101,6 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
     pk_setup(A, d, p, meta, lane, count);
     pk_setup(B, d, p, meta, lane + 64, count);
     const ColRef<uint32_t, LC> cw{col, gcol};
-    const ColRef<uint16_t, LC> ca{reinterpret_cast<uint16_t *>(col), reinterpret_cast<uint16_t *>(gcol)};
-    const ColRef<uint16_t, LC> cbh{ca.l + 1, ca.g + 1};
     // Per 16-base query word, the "differs from base c" bit masks of every
     // c, held in registers for the whole item: N01 = ne0 | ne1 << 16 and
     // N23 = ne2 | ne3 << 16.  A row picks its target base's mask with one
@@ -971,7 +1127,16 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
     const uint32_t LOW = 0x00FF00FFu;
     const us2 ONE = u2(opaque(0x00010001u));
     const us2 C256 = u2(opaque(0x01000100u));   // H * 256 + k: one packed mad
-    const int BIG = 0x7fffffff;
+    // Rows whose band-start h1 (:101-104) can be non-zero for some job of the
+    // wave: h0 - o_del - e_del * (i + 1) > 0 needs i < ceil((h0 - o_del) / e_del).
+    int h1rows;
+    {
+        const int ra = A.alive ? max(0, (A.h0 - p.o_del + p.e_del - 1) / p.e_del) : 0;
+        const int rb = B.alive ? max(0, (B.h0 - p.o_del + p.e_del - 1) / p.e_del) : 0;
+        uint32_t x = ~(uint32_t)max(ra, rb), y = ~0u, z = ~0u;
+        wave_min3(x, y, z);
+        h1rows = (int)~x;
+    }
 
     for (int i = 0;; ++i) {
         if (i > 0 && (i & 15) == 0) {
@@ -986,17 +1151,26 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
                 B.twe = d.seq[B.tw0 + min(nb, B.tnw)];
             }
         }
-        const int h1a = pk_row_start(A, p, i, ca);
-        const int h1b = pk_row_start(B, p, i, cbh);
-        if (__ballot(A.alive || B.alive) == 0) break;
+        pk2_row_start<LC>(A, i, col, gcol, 0);
+        pk2_row_start<LC>(B, i, col, gcol, 1);
+        // wave band: lowest live start, highest and lowest live end; no live
+        // job leaves the first at ~0
+        uint32_t rlo = min(A.alive ? (uint32_t)A.beg : ~0u, B.alive ? (uint32_t)B.beg : ~0u);
+        uint32_t rhi = min(A.alive ? ~(uint32_t)A.end : ~0u, B.alive ? ~(uint32_t)B.end : ~0u);
+        uint32_t rmh = min(A.alive ? (uint32_t)A.end : ~0u, B.alive ? (uint32_t)B.end : ~0u);
+        wave_min3(rlo, rhi, rmh);
+        if (rlo == ~0u) break;                                 // every job has ended
+        const int jlo = (int)rlo, jhi = (int)~rhi, mhi = (int)rmh;
         // this row's target base of each job: mask register (N23 for G/T) and shift
         const bool hiA = (A.tb & 2u) != 0, hiB = (B.tb & 2u) != 0;
         const uint32_t shA = (A.tb & 1u) * 16, shB = (B.tb & 1u) * 16;
-        us2 h1 = us2{(unsigned short)h1a, (unsigned short)h1b};
+        us2 h1 = splat(0);
+        if (i < h1rows) {                                      // :99-107
+            const int ha = (A.alive && A.beg == 0) ? max(0, A.h0 - (p.o_del + p.e_del * (i + 1))) : 0;
+            const int hb = (B.alive && B.beg == 0) ? max(0, B.h0 - (p.o_del + p.e_del * (i + 1))) : 0;
+            h1 = us2{(unsigned short)ha, (unsigned short)hb};
+        }
         us2 f = splat(0), key = splat(0);
-        const int jlo = uniform_min(min(A.alive ? A.beg : BIG, B.alive ? B.beg : BIG));
-        const int jhi = uniform_max(max(A.alive ? A.end : 0, B.alive ? B.end : 0));
-        const int mhi = uniform_min(min(A.alive ? A.end : BIG, B.alive ? B.end : BIG));
         // per-half band end (columns >= it commit nothing); 0 for a finished job
         const us2 L = us2{(unsigned short)(A.alive ? A.end : 0), (unsigned short)(B.alive ? B.end : 0)};
 
@@ -1132,9 +1306,9 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
 
         const uint32_t kw = w32(key), hw = w32(h1);
         const int fa = (int)(hw & 0xFFFF), fb = (int)(hw >> 16);
-        pk_row_end(A, p, i, fa, (int)(kw & 0xFFFF), ca);
-        pk_row_end(B, p, i, fb, (int)(kw >> 16), cbh);
-        pk_narrow(A, B, fa, fb, ca, cbh);
+        pk2_row_end<LC>(A, p, i, fa, (int)(kw & 0xFFFF), col, gcol, 0);
+        pk2_row_end<LC>(B, p, i, fb, (int)(kw >> 16), col, gcol, 1);
+        pk2_narrow<LC>(A, B, fa, fb, col, gcol);
     }
     if (d.cells) {
         unsigned long long c = (unsigned long long)A.ncells + B.ncells;
