@@ -590,6 +590,11 @@ __device__ __forceinline__ us2 splat(int v) { return us2{(unsigned short)v, (uns
 __device__ __forceinline__ us2 vmax(us2 a, us2 b) { return __builtin_elementwise_max(a, b); }
 __device__ __forceinline__ us2 vmin(us2 a, us2 b) { return __builtin_elementwise_min(a, b); }
 __device__ __forceinline__ us2 vsubs(us2 a, us2 b) { return __builtin_elementwise_sub_sat(a, b); }
+// signed 16-bit max of a possibly negative value and a non-negative one
+__device__ __forceinline__ us2 smax(ss2 a, us2 b)
+{
+    return __builtin_bit_cast(us2, __builtin_elementwise_max(a, __builtin_bit_cast(ss2, b)));
+}
 // A constant the compiler cannot see through: keeps min(M, 1) a packed min
 // instead of per-half compares and selects.
 __device__ __forceinline__ uint32_t opaque(uint32_t v)
@@ -1123,7 +1128,8 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
     }
     const ss2 Ssc = ss2{(short)p.match_sc, (short)p.match_sc};
     const ss2 nAB = ss2{(short)(p.mis_sc - p.match_sc), (short)(p.mis_sc - p.match_sc)};
-    const us2 eD = splat(p.e_del), eI = splat(p.e_ins), oeD = splat(p.oe_del), oeI = splat(p.oe_ins);
+    const us2 eD = splat(p.e_del), eI = splat(p.e_ins);
+    const ss2 soeD = ss2{(short)p.oe_del, (short)p.oe_del}, soeI = ss2{(short)p.oe_ins, (short)p.oe_ins};
     const uint32_t LOW = 0x00FF00FFu;
     const us2 ONE = u2(opaque(0x00010001u));
     const us2 C256 = u2(opaque(0x01000100u));   // H * 256 + k: one packed mad
@@ -1206,18 +1212,23 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
             for (int k = 0; k < N; ++k) s[k] = s[k] * vmin(M[k], ONE) + M[k];   // M ? M + s : 0
 #pragma unroll
             for (int k = 0; k < N; ++k) {
-                const us2 dg = __builtin_bit_cast(
-                    us2, __builtin_elementwise_max(__builtin_bit_cast(ss2, s[k]), ss2{0, 0}));
-                dgE[k] = vmax(dg, E[k]);
-                const us2 t = vsubs(dg, oeD);
-                ne[k] = vmax(vsubs(E[k], eD), t);
-                tf[k] = SYM ? t : vsubs(dg, oeI);
+                // M' = M ? M + s : 0 may be negative here (a mismatch after a
+                // small M); ksw_extend2's max(M', 0) is left out because every
+                // use of M' is a max with a non-negative term: H = max(M', E, F),
+                // E' = max(E - e_del, M' - oe_del, 0) with E - e_del saturating
+                // at 0, likewise F'.  So M' - oe stays signed and the maxes are
+                // signed 16-bit ones.
+                const ss2 dg = __builtin_bit_cast(ss2, s[k]);
+                dgE[k] = smax(dg, E[k]);
+                const ss2 t = dg - soeD;
+                ne[k] = smax(t, vsubs(E[k], eD));
+                tf[k] = __builtin_bit_cast(us2, SYM ? t : dg - soeI);
             }
             if (!MASKED) {
 #pragma unroll
                 for (int k = 0; k < N; ++k) {
                     const us2 H = vmax(dgE[k], f);
-                    f = vmax(vsubs(f, eI), tf[k]);
+                    f = smax(__builtin_bit_cast(ss2, tf[k]), vsubs(f, eI));
                     wv[k] = __builtin_amdgcn_perm(w32(ne[k]), w32(h1), 0x06020400u);
                     h1 = H;
                     Hs[k] = H * C256 + splat(k);
@@ -1235,7 +1246,7 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
                 for (int k = 0; k < N; ++k) {
                     const uint32_t hm = hms[k];
                     const us2 H = vmax(dgE[k], f);
-                    f = vmax(vsubs(f, eI), tf[k]);
+                    f = smax(__builtin_bit_cast(ss2, tf[k]), vsubs(f, eI));
                     const uint32_t nw = __builtin_amdgcn_perm(w32(ne[k]), w32(h1), 0x06020400u);
                     wv[k] = (nw & hm) | (wv[k] & ~hm);
                     h1 = u2((w32(H) & hm) | (w32(h1) & ~hm));
