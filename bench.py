@@ -11,7 +11,7 @@ one-launch form is timed beside it.  Cells are counted exactly as the CPU
 loop counts them (one per inner iteration of BandedSWScalar.swift:112).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
-       python bench.py --profile-leg ext|mixed|copy   (one leg only, for rocprofv3 runs)
+       python bench.py --profile-leg ext|mixed|nw|copy   (one leg only, for rocprofv3 runs)
 """
 import argparse
 import json
@@ -107,7 +107,7 @@ def autotune(eng, st, max_jobs):
     return out
 
 
-def bench_nw(eng, args):
+def bench_nw(eng, args, modes=None):
     """Banded global NW + traceback: 500,000 pairs, qlen U[50,250], target =
     query +- a planted 0..20-bp indel, 1% subs / 0.2% indels; w = min(100,
     2|dlen|+3); score + CIGAR per pair, retries while score < planted score."""
@@ -126,7 +126,7 @@ def bench_nw(eng, args):
     if not st:
         raise RuntimeError(eng.last_error())
     res = {}
-    for name, mode in (("plain", native.MODE_PLAIN), ("batched", native.MODE_BATCHED)):
+    for name, mode in modes or (("plain", native.MODE_PLAIN), ("batched", native.MODE_BATCHED)):
         s = native.HipswStats()
         for _ in range(args.warmup):
             if lib.hipsw_nw_run(eng.handle, st, mode, C.byref(s)):
@@ -146,7 +146,8 @@ def bench_nw(eng, args):
         log("nw %s: %.2f ms/step (kernel %.2f ms)" % (name, wall, np.mean(kms)))
     lib.hipsw_nw_free(eng.handle, st)
     res["pairs"] = js["n"]
-    res["batched_over_plain"] = round(res["plain"]["ms_per_step"] / res["batched"]["ms_per_step"], 3)
+    if "plain" in res:
+        res["batched_over_plain"] = round(res["plain"]["ms_per_step"] / res["batched"]["ms_per_step"], 3)
     return res
 
 
@@ -240,6 +241,7 @@ def profile_leg(args):
 
     ext:  the headline job set, batched form, `warmup` + `steps` passes.
     mixed: the uneven-size job set (4,000,000 jobs), batched form, likewise.
+    nw:   the NW + CIGAR set (500,000 pairs), batched form, likewise.
     copy: a 256 MiB device-to-device copy, `steps` times -- the known-size
           reference that scales the FETCH_SIZE / WRITE_SIZE counters."""
     import ctypes as C
@@ -252,7 +254,10 @@ def profile_leg(args):
         # for the 150-bp set) so the profiled kernel is the benchmarked one
         if lib.hipsw_ext_set_layout(eng.handle, 1, 1, args.pk_lds_blocks):
             raise RuntimeError(eng.last_error())
-    if args.profile_leg in ("ext", "mixed"):
+    if args.profile_leg == "nw":
+        nw_args = argparse.Namespace(warmup=args.warmup, steps=args.steps, nw_n=args.nw_n)
+        bench_nw(eng, nw_args, modes=(("batched", native.MODE_BATCHED),))
+    elif args.profile_leg in ("ext", "mixed"):
         js = (wl.generate(wl.EXT_UNIFORM, 2, args.n) if args.profile_leg == "ext"
               else wl.generate(wl.EXT_MIXED, 4, args.mixed_n))
         a = _ext_arrays(js)
@@ -303,7 +308,7 @@ def main():
     ap.add_argument("--serve-budget-gb", type=int, default=64)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--mixed-n", type=int, default=4000000)
-    ap.add_argument("--profile-leg", choices=["ext", "mixed", "copy"], default=None)
+    ap.add_argument("--profile-leg", choices=["ext", "mixed", "nw", "copy"], default=None)
     ap.add_argument("--pk-lds-blocks", type=int, default=0,
                     help="profile leg: 16-column blocks of packed column state in LDS for "
                          "64-159-base queries (10, 6 or 5; 0 = default)")

@@ -31,7 +31,7 @@ struct hipsw_nw_set {
     bool plan_ready[2] = {false, false};
     std::vector<uint32_t> plan_order[2];
     std::vector<hipsw::WorkItem> plan_items[2];
-    std::vector<int> plan_launch_raw[2];
+    std::vector<hipsw::NwLaunch> plan_launches[2];
 };
 
 namespace {
@@ -167,7 +167,7 @@ extern "C" hipsw_nw_set *hipsw_nw_upload(hipsw_engine *eng, int32_t n, const uin
 void hipsw::nw_plan_pass(const hipsw_engine *eng, const int32_t *qlv, const int32_t *tlv,
                          int max_pen, const std::vector<uint32_t> &active,
                          const std::vector<int32_t> &w, int mode, std::vector<uint32_t> &order,
-                         std::vector<WorkItem> &items, std::vector<NwLaunch> &launches)
+                         std::vector<WorkItem> &items, std::vector<NwLaunch> &launches, int wide)
 {
     const int m = (int)active.size();
     std::vector<int> ncol(m), S(m), tier(m);
@@ -183,8 +183,13 @@ void hipsw::nw_plan_pass(const hipsw_engine *eng, const int32_t *qlv, const int3
     order.clear();
     items.clear();
     launches.clear();
-    auto add_launch = [&](int tr, int s_max, int c_max, int z_rows, int ib, int ic, int q_max) {
+    auto add_launch = [&](int tr, int s_max, int c_max, int z_rows, int ib, int ic, int q_max,
+                          size_t z_max) {
         NwLaunch L;
+        L.qmax = q_max;
+        L.zmax = z_max;
+        L.wide = 0;
+        if (wide == 2 || (wide == 1 && ic < eng->n_cu)) L.wide = nw_wide_kc(q_max) > 0 ? 1 : 0;
         L.qw8 = std::max(1, (q_max + 7) / 8);
         L.tier = tr;
         L.S = s_max;
@@ -197,20 +202,30 @@ void hipsw::nw_plan_pass(const hipsw_engine *eng, const int32_t *qlv, const int3
         const size_t lds_used = nw_lds_bytes(tr, L.global_state, s_max, L.qw8);
         const int wpc = std::max(1, std::min(16, (int)(eng->lds_per_cu / (lds_used + 512))));
         L.grid = std::max(1, std::min(ic, eng->n_cu * wpc));
+        if (L.wide) {
+            // one wave per job; flag matrices in LDS when they fit
+            int jobs = 0;
+            for (int t = ib; t < ib + ic; ++t) jobs += (int)items[t].count;
+            const bool zl = z_max <= 49152;
+            const int per_cu = zl ? std::max(1, std::min(8, (int)(eng->lds_per_cu / (z_max + 1024)))) : 8;
+            L.grid = std::max(1, std::min(jobs, eng->n_cu * per_cu));
+        }
         launches.push_back(L);
     };
     if (mode == HIPSW_MODE_PLAIN) {
         int tr = NW_I16, smax = 3, cmax = 1, zr = 1, qm = 1;
+        size_t zm = 1;
         for (int a = 0; a < m; ++a) {
             tr = std::max(tr, tier[a]);
             smax = std::max(smax, S[a]);
             cmax = std::max(cmax, ncol[a]);
             zr = std::max(zr, tlv[active[a]]);
             qm = std::max(qm, qlv[active[a]]);
+            zm = std::max(zm, (size_t)tlv[active[a]] * ncol[a]);
             order.push_back(active[a]);
         }
         for (int g = 0; g < m; g += 64) items.push_back(WorkItem{(uint32_t)g, (uint32_t)std::min(64, m - g)});
-        add_launch(tr, smax, cmax, zr, 0, (int)items.size(), qm);
+        add_launch(tr, smax, cmax, zr, 0, (int)items.size(), qm, zm);
         return;
     }
     // Batched: bins by (tier, S class); inside a bin lanes of one wave share
@@ -228,16 +243,18 @@ void hipsw::nw_plan_pass(const hipsw_engine *eng, const int32_t *qlv, const int3
     for (int s = 0; s < m;) {
         const uint32_t bin = keys[s] >> 28;
         int e = s, smax = 3, cmax = 1, zr = 1, qm = 1;
+        size_t zm = 1;
         while (e < m && (keys[e] >> 28) == bin) {
             smax = std::max(smax, S[idx[e]]);
             cmax = std::max(cmax, ncol[idx[e]]);
             zr = std::max(zr, tlv[active[idx[e]]]);
             qm = std::max(qm, qlv[active[idx[e]]]);
+            zm = std::max(zm, (size_t)tlv[active[idx[e]]] * ncol[idx[e]]);
             ++e;
         }
         const int ib = (int)items.size();
         for (int g = s; g < e; g += 64) items.push_back(WorkItem{(uint32_t)g, (uint32_t)std::min(64, e - g)});
-        add_launch((int)bin / 8, smax, cmax, zr, ib, (int)items.size() - ib, qm);
+        add_launch((int)bin / 8, smax, cmax, zr, ib, (int)items.size() - ib, qm, zm);
         s = e;
     }
 }
@@ -268,25 +285,19 @@ extern "C" int hipsw_nw_run(hipsw_engine *eng, hipsw_nw_set *set, int32_t mode,
         if (pass == 0 && set->plan_ready[mode]) {
             order = set->plan_order[mode];
             items = set->plan_items[mode];
-            launches.clear();
-            const std::vector<int> &raw = set->plan_launch_raw[mode];
-            for (size_t r = 0; r + 9 <= raw.size(); r += 9)
-                launches.push_back(NwLaunch{raw[r], raw[r + 1], raw[r + 2], raw[r + 3], raw[r + 4],
-                                            raw[r + 5], raw[r + 6], raw[r + 7] != 0, raw[r + 8]});
+            launches = set->plan_launches[mode];
         } else {
-            // Retry passes hold few tasks: one plain launch each.
+            // Retry passes hold few tasks: one launch each.  The batched form
+            // runs its small bins and its retry passes one wave per job (the
+            // F recurrence needs o_ins >= 0); the plain form keeps one lane
+            // per job throughout.
+            const int wide = mode == HIPSW_MODE_BATCHED && set->params.o_ins >= 0 ? (pass == 0 ? 1 : 2) : 0;
             nw_plan_pass(eng, set->ql.data(), set->tl.data(), set->max_pen, active, w,
-                         pass == 0 ? mode : HIPSW_MODE_PLAIN, order, items, launches);
+                         pass == 0 ? mode : HIPSW_MODE_PLAIN, order, items, launches, wide);
             if (pass == 0) {
                 set->plan_order[mode] = order;
                 set->plan_items[mode] = items;
-                std::vector<int> raw;
-                for (const NwLaunch &L : launches) {
-                    const int v[9] = {L.tier, L.S, L.ncol_max, L.zrows, L.item_begin, L.item_count,
-                                      L.grid, L.global_state ? 1 : 0, L.qw8};
-                    raw.insert(raw.end(), v, v + 9);
-                }
-                set->plan_launch_raw[mode] = raw;
+                set->plan_launches[mode] = launches;
                 set->plan_ready[mode] = true;
             }
         }
@@ -299,6 +310,10 @@ extern "C" int hipsw_nw_run(hipsw_engine *eng, hipsw_nw_set *set, int32_t mode,
         // order on one stream, so they share one region.
         size_t zbytes = 0, gbytes = 0;
         for (const NwLaunch &L : launches) {
+            if (L.wide) {
+                if (L.zmax > 49152) zbytes = std::max(zbytes, (size_t)L.grid * L.zmax);
+                continue;
+            }
             const size_t ncw = (size_t)(L.ncol_max + 7) / 8;
             zbytes = std::max(zbytes, (size_t)L.grid * L.zrows * ncw * 64 * 4);
             if (L.global_state)
@@ -323,6 +338,20 @@ extern "C" int hipsw_nw_run(hipsw_engine *eng, hipsw_nw_set *set, int32_t mode,
         int k = 0;
         for (const NwLaunch &L : launches) {
             if (!ok) break;
+            if (L.wide) {
+                // the bin's jobs are order[items[ib].start ..) contiguously
+                const WorkItem &first = items[L.item_begin];
+                const WorkItem &last = items[L.item_begin + L.item_count - 1];
+                const int nj = (int)(last.start + last.count - first.start);
+                ok = eng->check(launch_nw_wide(set->dev, set->params, set->d_order + first.start, nj,
+                                               eng->counters + k, L.qmax,
+                                               reinterpret_cast<uint8_t *>(zbase), L.zmax,
+                                               L.zmax <= 49152, L.grid, eng->stream),
+                                "launch nw wide");
+                ++k;
+                ++launches_total;
+                continue;
+            }
             const int ncw = (L.ncol_max + 7) / 8;
             ok = eng->check(launch_nw(L.tier, L.global_state, set->dev, set->params, set->d_order,
                                       set->d_items + L.item_begin, L.item_count,

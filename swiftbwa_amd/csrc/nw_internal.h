@@ -46,11 +46,23 @@ hipError_t launch_nw(int tier, bool global_state, const NwDev &d, const NwParams
                      uint32_t *counter, int S, uint32_t *zbase, int zrows, int ncw, int grid,
                      bool persistent, void *gstate, int qw8, hipStream_t s);
 
+// Wave-per-job form (nw_kernel.hip): jobs[0..n), query lengths <= 512 (the
+// register chunks of the instance chosen by nw_wide_kc(qmax); 0 = none fits),
+// o_ins >= 0.  Flag matrices of at most zmax bytes each, in LDS (zlds) or at
+// zg + block * zmax.
+int nw_wide_kc(int qmax);
+hipError_t launch_nw_wide(const NwDev &d, const NwParams &p, const uint32_t *jobs, int n,
+                          uint32_t *counter, int qmax, uint8_t *zg, size_t zmax, bool zlds,
+                          int grid, hipStream_t s);
+
 // One launch of an NW pass (host plan).
 struct NwLaunch {
     int tier, S, ncol_max, zrows, item_begin, item_count, grid;
     bool global_state;
     int qw8;     // staged query words (8 bases each) per lane
+    int wide;    // 1: run the bin's jobs with the wave-per-job form
+    int qmax;    // longest query of the bin
+    size_t zmax; // wave-per-job form: largest tlen * ncol of the bin
 };
 
 }  // namespace hipsw
@@ -58,8 +70,11 @@ struct NwLaunch {
 struct hipsw_engine;
 namespace hipsw {
 // Plans one NW pass over the job ids in `active` with bands w[] (indexed by job).
+// wide: 0 = lane-per-job launches only; 1 = bins too small to give every CU a
+// wave go to the wave-per-job form; 2 = the whole pass does (if every query
+// fits it).
 void nw_plan_pass(const hipsw_engine *eng, const int32_t *ql, const int32_t *tl, int max_pen,
                   const std::vector<uint32_t> &active, const std::vector<int32_t> &w, int mode,
                   std::vector<uint32_t> &order, std::vector<WorkItem> &items,
-                  std::vector<NwLaunch> &launches);
+                  std::vector<NwLaunch> &launches, int wide = 0);
 }  // namespace hipsw

@@ -49,6 +49,76 @@ __device__ __forceinline__ int wave_max_nw(int v)
     return v;
 }
 
+// Traceback (GlobalAligner.swift:160-232) of one job by one lane, with
+// CIGARBuilder merging (CIGARBuilder.swift:4-38); ops come out last-first.
+// They go to a ring of `cap` slots, so when the CIGAR has more than cap ops
+// the row still ends up holding the first cap ops of the final CIGAR.
+// zf(i, c) returns the 4 flag bits of row i, band column c = j - max(0, i - w).
+template <class ZF>
+__device__ __forceinline__ void nw_traceback(const NwDev &d, int job, int qlen, int tlen, int w,
+                                             int ncol, ZF zf)
+{
+    uint32_t *cig = d.cigar + (size_t)job * d.cap;
+    int i = tlen - 1, jj = qlen - 1, state = 0;
+    int nops = 0;
+    uint32_t cur_op = 0xFu, cur_len = 0;
+    const int cap = d.cap;
+    int slot = 0;
+    auto push = [&]() {
+        cig[slot] = (cur_len << 4) | cur_op;
+        slot = slot + 1 == cap ? 0 : slot + 1;
+        ++nops;
+    };
+    auto emit = [&](uint32_t op) {
+        if (op == cur_op) {
+            ++cur_len;
+        } else {
+            if (cur_len) push();
+            cur_op = op;
+            cur_len = 1;
+        }
+    };
+    while (i >= 0 || jj >= 0) {
+        if (i < 0) { emit(1u); --jj; continue; }
+        if (jj < 0) { emit(2u); --i; continue; }
+        const int beg = max(0, i - w);
+        const int c = jj - beg;
+        if (!(c >= 0 && c < ncol)) break;
+        const uint32_t dd = zf(i, c);
+        if (state == 0) {
+            const uint32_t hdir = (dd >> 2) & 3u;
+            if (hdir == 0) { emit(0u); --i; --jj; }
+            else if (hdir == 1) state = 1;
+            else state = 2;
+        } else if (state == 1) {
+            emit(2u);
+            if ((dd & 1u) == 0) state = 0;
+            --i;
+        } else {
+            emit(1u);
+            if ((dd & 2u) == 0) state = 0;
+            --jj;
+        }
+    }
+    if (cur_len) push();
+    // Final order = reverse of emission order (:230).  With the ring, slots
+    // [0, p) and [p, kept) are each reversed in place, p = nops mod cap.
+    const int kept = min(nops, cap);
+    const int p_split = nops > cap ? nops % cap : kept;
+    for (int a = 0, b = p_split - 1; a < b; ++a, --b) {
+        const uint32_t tmp = cig[a];
+        cig[a] = cig[b];
+        cig[b] = tmp;
+    }
+    for (int a = p_split, b = kept - 1; a < b; ++a, --b) {
+        const uint32_t tmp = cig[a];
+        cig[a] = cig[b];
+        cig[b] = tmp;
+    }
+    d.n_cigar[job] = nops;
+    d.flags[job] = nops > d.cap ? NW_FLAG_TRUNCATED : 0;
+}
+
 // Fill + traceback for the (<= 64) jobs idx[0..count).
 template <int T>
 __device__ void nw_wave(const NwDev &d, const NwParams &p, const uint32_t *idx, int count,
@@ -201,7 +271,6 @@ __device__ void nw_wave(const NwDev &d, const NwParams &p, const uint32_t *idx, 
 
     if (!valid) return;
     int32_t *o_score = d.score + job;
-    uint32_t *cig = d.cigar + (size_t)job * d.cap;
     if (!nonempty) {
         *o_score = 0;
         d.n_cigar[job] = 0;
@@ -214,68 +283,9 @@ __device__ void nw_wave(const NwDev &d, const NwParams &p, const uint32_t *idx, 
         const int s = (sb + (qlen - beg_last)) % S;
         *o_score = St::h(col[s * 64]);
     }
-    // Traceback (:160-232) with CIGARBuilder merging; ops come out last-first.
-    // They go to a ring of `cap` slots, so when the CIGAR has more than cap ops
-    // the row still ends up holding the first cap ops of the final CIGAR.
-    int i = tlen - 1, jj = qlen - 1, state = 0;
-    int nops = 0;
-    uint32_t cur_op = 0xFu, cur_len = 0;
-    const int cap = d.cap;
-    int slot = 0;
-    auto push = [&]() {
-        cig[slot] = (cur_len << 4) | cur_op;
-        slot = slot + 1 == cap ? 0 : slot + 1;
-        ++nops;
-    };
-    auto emit = [&](uint32_t op) {
-        if (op == cur_op) {
-            ++cur_len;
-        } else {
-            if (cur_len) push();
-            cur_op = op;
-            cur_len = 1;
-        }
-    };
-    while (i >= 0 || jj >= 0) {
-        if (i < 0) { emit(1u); --jj; continue; }
-        if (jj < 0) { emit(2u); --i; continue; }
-        const int beg = max(0, i - w);
-        const int c = jj - beg;
-        if (!(c >= 0 && c < ncol)) break;
-        const uint32_t word = zl[((size_t)i * ncw + (c >> 3)) * 64];
-        const uint32_t dd = (word >> ((c & 7) * 4)) & 0xFu;
-        if (state == 0) {
-            const uint32_t hdir = (dd >> 2) & 3u;
-            if (hdir == 0) { emit(0u); --i; --jj; }
-            else if (hdir == 1) state = 1;
-            else state = 2;
-        } else if (state == 1) {
-            emit(2u);
-            if ((dd & 1u) == 0) state = 0;
-            --i;
-        } else {
-            emit(1u);
-            if ((dd & 2u) == 0) state = 0;
-            --jj;
-        }
-    }
-    if (cur_len) push();
-    // Final order = reverse of emission order (:230).  With the ring, slots
-    // [0, p) and [p, kept) are each reversed in place, p = nops mod cap.
-    const int kept = min(nops, cap);
-    const int p_split = nops > cap ? nops % cap : kept;
-    for (int a = 0, b = p_split - 1; a < b; ++a, --b) {
-        const uint32_t tmp = cig[a];
-        cig[a] = cig[b];
-        cig[b] = tmp;
-    }
-    for (int a = p_split, b = kept - 1; a < b; ++a, --b) {
-        const uint32_t tmp = cig[a];
-        cig[a] = cig[b];
-        cig[b] = tmp;
-    }
-    d.n_cigar[job] = nops;
-    d.flags[job] = nops > d.cap ? NW_FLAG_TRUNCATED : 0;
+    nw_traceback(d, job, qlen, tlen, w, ncol, [&](int i, int c) {
+        return (zl[((size_t)i * ncw + (c >> 3)) * 64] >> ((c & 7) * 4)) & 0xFu;
+    });
 }
 
 __device__ __forceinline__ const int8_t *nw_load_smat(const NwParams &p)
@@ -321,6 +331,182 @@ __global__ __launch_bounds__(64) void nw_kernel(NwDev d, NwParams p, const uint3
         __syncthreads();
         if (!persistent) it += gridDim.x;      // plain form: static round-robin
     }
+}
+
+// ---------------------------------------------------------------------------
+// One wave per job ("wide" form), for jobs whose band is too wide to leave to
+// one lane: the bins holding few of them and the band-doubling retry passes,
+// whose time a lane-per-job kernel spends on its single slowest job.
+//
+// Lane l holds the columns j = 64k + l (chunk k < KC, query length <= 64 KC)
+// of the row state H(i, j), E(i, j) in registers, int32 with the reference's
+// Int32.min/2 sentinel (GlobalAligner.swift:52).  A row runs the chunks that
+// meet its band [beg, end) (:90-91).  The diagonal input of column j is
+// column j - 1 of the previous row (a one-lane DPP shift, the chunk's lane 0
+// taking the previous chunk's lane 63); E is lane-local.  F, the row's only
+// left-to-right dependence, is the exclusive prefix maximum
+//   F(j) = max(C0 - e_ins (j - 1), max_{beg <= k < j} (G(k) - oe_ins - e_ins (j - 1 - k)))
+// with G = max(H_diag + s, E): since o_ins >= 0, H(j-1) - oe_ins <= F(j-1) - e_ins
+// whenever F(j-1) > G(j-1), so the F-free G carries the recurrence (:121-127);
+// C0 folds the row-start h[beg] and the initial f = Int32.min/2.  It is one
+// max-scan of G(k) - oe_ins + e_ins k (fused v_max_i32_dpp steps).  The four
+// flag bits of every cell are the serial loop's (:114-136); they go to LDS
+// (bytes, [row][j - beg] as the reference's z) when tlen * ncol fits, else to
+// the wave's HBM region, and lane 0 walks the traceback from there.
+template <int KC>
+__device__ void nw_wide_job(const NwDev &d, const NwParams &p, int job, uint8_t *z)
+{
+    constexpr int NEG = INT32_MIN / 2;                         // :52
+    constexpr int LOW = NEG - (1 << 28);                       // below every scanned term
+    const int lane = threadIdx.x & 63;
+    const int qlen = d.q_len[job], tlen = d.t_len[job];
+    if (qlen <= 0 || tlen <= 0) {                              // :29-31
+        if (lane == 0) { d.score[job] = 0; d.n_cigar[job] = 0; d.flags[job] = 0; }
+        return;
+    }
+    const int w = max(min(d.w[job], 1 << 20), abs(qlen - tlen));   // :47
+    const int ncol = min(qlen, 2 * w + 1);                     // :48
+    const uint32_t qw0 = d.q_word[job], tw0 = d.t_word[job];
+    const uint32_t qn0 = d.qn_word[job], tn0 = d.tn_word[job];
+    // Per chunk: this lane's query code (N = 4) and the first row (:75-84):
+    // h[j+1] = -(o_ins + e_ins (j+1)) for j + 1 <= min(w, qlen), e = sentinel.
+    // Scores of this lane's query base against target bases 0-3 (signed
+    // bytes of spk) and N (s4): mat[t * 5 + q] (GlobalAligner.swift:63-70).
+    uint32_t colw[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+        colw[q] = (uint32_t)(uint8_t)p.mat[q] | ((uint32_t)(uint8_t)p.mat[5 + q] << 8) |
+                  ((uint32_t)(uint8_t)p.mat[10 + q] << 16) | ((uint32_t)(uint8_t)p.mat[15 + q] << 24);
+    int spk[KC], s4[KC], Hc[KC], Ec[KC];
+    const int init_end = min(w, qlen);
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+        const int j = 64 * k + lane;
+        int c = 0;
+        if (j < qlen) {
+            c = (int)((d.seq[qw0 + (j >> 4)] >> ((j & 15) * 2)) & 3u);
+            if (qn0 != kNoN && ((d.nmask[qn0 + (j >> 5)] >> (j & 31)) & 1u)) c = 4;
+        }
+        spk[k] = (int)(c == 0 ? colw[0] : c == 1 ? colw[1] : c == 2 ? colw[2] : c == 3 ? colw[3] : colw[4]);
+        s4[k] = c == 0 ? p.mat[20] : c == 1 ? p.mat[21] : c == 2 ? p.mat[22] : c == 3 ? p.mat[23] : p.mat[24];
+        Hc[k] = j + 1 <= init_end ? -(p.o_ins + p.e_ins * (j + 1)) : NEG;
+        Ec[k] = NEG;
+    }
+    int hvirt = 0;                                             // h[0] (:81)
+    unsigned long long ncells = 0;
+    uint32_t tw = 0, tn = 0;
+    for (int i = 0; i < tlen; ++i) {                           // :89
+        if ((i & 15) == 0) tw = d.seq[tw0 + (i >> 4)];
+        int t = (int)((tw >> ((i & 15) * 2)) & 3u);
+        if (tn0 != kNoN) {
+            if ((i & 31) == 0) tn = d.nmask[tn0 + (i >> 5)];
+            if ((tn >> (i & 31)) & 1u) t = 4;
+        }
+        const int beg = max(0, i - w), end = min(qlen, i + w + 1);   // :90-91
+        ncells += (unsigned long long)(end - beg);
+        const int hb = beg == 0 ? 0 - (p.oe_del + p.e_del * i) : NEG;   // :97-103
+        int Cin = max(hb - p.oe_ins + p.e_ins * (beg - 1), NEG + p.e_ins * (beg - 2));
+        int hcarry = hb, fcarry = NEG;                        // H, F of the column left of a chunk
+        const int k0 = beg >> 6, k1 = (end - 1) >> 6;
+        int prev63 = hvirt;                                    // old H of column 64 k0 - 1
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+            if (k < k0 || k > k1) continue;                    // wave-uniform
+            if (k > 0 && k == k0) prev63 = __builtin_amdgcn_readlane(Hc[k - 1], 63);
+            const int j = 64 * k + lane;
+            const bool act = j >= beg && j < end;
+            const int hold = Hc[k], eold = Ec[k];
+            // H(i-1, j-1): lane l - 1, lane 0 from the chunk to the left
+            int dsrc = __builtin_amdgcn_update_dpp(0, hold, 0x138, 0xF, 0xF, false);   // wave_shr:1
+            if (lane == 0) dsrc = prev63;
+            prev63 = __builtin_amdgcn_readlane(hold, 63);
+            const int sc = t < 4 ? __builtin_amdgcn_sbfe(spk[k], 8 * t, 8) : s4[k];
+            const int hcur = dsrc + sc;                           // :110
+            const int e_open = hold - p.oe_del, e_ext = eold - p.e_del;   // :116-120
+            const uint32_t e_flag = e_open > e_ext ? 0u : 1u;
+            const int ev = e_open > e_ext ? e_open : e_ext;
+            const int g = max(hcur, ev);
+            // exclusive prefix max of G(k) - oe_ins + e_ins k over the band
+            int x = act ? g - p.oe_ins + p.e_ins * j : LOW;
+            x = max(x, __builtin_amdgcn_update_dpp(LOW, x, 0x111, 0xF, 0xF, false));   // row_shr:1
+            x = max(x, __builtin_amdgcn_update_dpp(LOW, x, 0x112, 0xF, 0xF, false));   // row_shr:2
+            x = max(x, __builtin_amdgcn_update_dpp(LOW, x, 0x114, 0xF, 0xF, false));   // row_shr:4
+            x = max(x, __builtin_amdgcn_update_dpp(LOW, x, 0x118, 0xF, 0xF, false));   // row_shr:8
+            x = max(x, __builtin_amdgcn_update_dpp(LOW, x, 0x142, 0xA, 0xF, false));   // row_bcast:15
+            x = max(x, __builtin_amdgcn_update_dpp(LOW, x, 0x143, 0xC, 0xF, false));   // row_bcast:31
+            int px = __builtin_amdgcn_update_dpp(LOW, x, 0x138, 0xF, 0xF, false);      // wave_shr:1
+            px = max(px, Cin);
+            Cin = max(Cin, __builtin_amdgcn_readlane(x, 63));
+            const int fv = px - p.e_ins * (j - 1);                 // F(i, j)
+            const uint32_t from_e = ev > hcur ? 4u : 0u;           // :129-134
+            const int h1 = ev > hcur ? ev : hcur;
+            const uint32_t from_f = fv > h1 ? 8u : 0u;
+            const int hv = fv > h1 ? fv : h1;
+            // f flag (:122-126): H(i, j-1) - oe_ins against F(i, j-1) - e_ins
+            int hl = __builtin_amdgcn_update_dpp(0, hv, 0x138, 0xF, 0xF, false);
+            int fl = __builtin_amdgcn_update_dpp(0, fv, 0x138, 0xF, 0xF, false);
+            if (lane == 0) { hl = hcarry; fl = fcarry; }
+            if (j == beg) { hl = hb; fl = NEG; }
+            const uint32_t f_flag = (hl - p.oe_ins) > (fl - p.e_ins) ? 0u : 2u;
+            hcarry = __builtin_amdgcn_readlane(hv, 63);
+            fcarry = __builtin_amdgcn_readlane(fv, 63);
+            if (act) {
+                Hc[k] = hv;                                    // :138
+                Ec[k] = ev;
+                z[(size_t)i * ncol + (j - beg)] = (uint8_t)(e_flag | f_flag | from_e | from_f);
+            }
+        }
+        if (beg == 0) hvirt = hb;
+    }
+    if (d.cells && lane == 0) atomicAdd(d.cells, ncells);
+    // score = h[qlen] (:144): column qlen - 1 of the last row
+    const int kq = (qlen - 1) >> 6;
+    int score = 0;
+#pragma unroll
+    for (int k = 0; k < KC; ++k)
+        if (k == kq) score = __builtin_amdgcn_readlane(Hc[k], (qlen - 1) & 63);
+    __syncthreads();                                           // flags visible to lane 0
+    if (lane != 0) return;
+    d.score[job] = score;
+    nw_traceback(d, job, qlen, tlen, w, ncol, [&](int i, int c) { return (uint32_t)z[(size_t)i * ncol + c]; });
+}
+
+// Jobs jobs[0..n) of one pass, one wave each, taken from an atomic counter.
+// zmax: bytes of one job's flag matrix at most (tlen * ncol); in LDS when
+// zlds, else at zg + blockIdx.x * zmax.
+template <int KC>
+__global__ __launch_bounds__(64) void nw_wide_kernel(NwDev d, NwParams p, const uint32_t *jobs,
+                                                     int n, uint32_t *counter, uint8_t *zg,
+                                                     size_t zmax, int zlds)
+{
+    extern __shared__ __align__(16) unsigned char nw_lds[];
+    uint8_t *z = zlds ? reinterpret_cast<uint8_t *>(nw_lds) : zg + (size_t)blockIdx.x * zmax;
+    __shared__ int item_sh;
+    for (;;) {
+        if (threadIdx.x == 0) item_sh = (int)atomicAdd(counter, 1u);
+        __syncthreads();
+        const int it = item_sh;
+        __syncthreads();
+        if (it >= n) break;
+        nw_wide_job<KC>(d, p, (int)jobs[it], z);
+        __syncthreads();
+    }
+}
+
+int nw_wide_kc(int qmax) { return qmax <= 128 ? 2 : qmax <= 256 ? 4 : qmax <= 512 ? 8 : 0; }
+
+hipError_t launch_nw_wide(const NwDev &d, const NwParams &p, const uint32_t *jobs, int n,
+                          uint32_t *counter, int qmax, uint8_t *zg, size_t zmax, bool zlds,
+                          int grid, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int kc = nw_wide_kc(qmax);
+    const size_t lds = zlds ? zmax : 0;
+    if (kc == 2) hipLaunchKernelGGL((nw_wide_kernel<2>), dim3(grid), dim3(64), lds, s, d, p, jobs, n, counter, zg, zmax, zlds ? 1 : 0);
+    else if (kc == 4) hipLaunchKernelGGL((nw_wide_kernel<4>), dim3(grid), dim3(64), lds, s, d, p, jobs, n, counter, zg, zmax, zlds ? 1 : 0);
+    else if (kc == 8) hipLaunchKernelGGL((nw_wide_kernel<8>), dim3(grid), dim3(64), lds, s, d, p, jobs, n, counter, zg, zmax, zlds ? 1 : 0);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
 }
 
 size_t nw_lds_bytes(int tier, bool global_state, int S, int qw8)

@@ -109,3 +109,36 @@ def test_long_and_int32_tier(engine):
 
 def test_small_cap_flags(engine):
     check(engine, _random(np.random.default_rng(23), 500, 60, 70), cap=2)
+
+
+def _edges(rng):
+    """Pairs whose query lengths sit at the 64-column chunk edges of the
+    wave-per-job form (64, 65, 128, 129, 256, 257, 512, 513), with bands from
+    1 to wider than the query, targets shorter and longer than the query, and
+    N bases; few jobs per bin, so the batched form runs them one wave each."""
+    qs, ts, ql, tl, w, tsc = [], [], [], [], [], []
+    for q_len in (1, 2, 63, 64, 65, 127, 128, 129, 255, 256, 257, 511, 512, 513):
+        for t_off in (-30, -1, 0, 1, 40):
+            t_len = max(1, q_len + t_off)
+            for band in (1, 7, 64, 300):
+                q = rng.integers(0, 4, q_len).astype(np.uint8)
+                t = np.resize(q, t_len).copy()
+                t[rng.random(t_len) < 0.05] = rng.integers(0, 4)
+                if rng.random() < 0.3:
+                    q[rng.random(q_len) < 0.02] = 4
+                    t[rng.random(t_len) < 0.02] = 4
+                qs.append(q); ts.append(t); ql.append(q_len); tl.append(t_len)
+                w.append(band); tsc.append(int(rng.integers(-20, q_len + 1)))
+    n = len(ql)
+    qo = np.zeros(n, np.uint32); to = np.zeros(n, np.uint32)
+    qo[1:] = np.cumsum(ql)[:-1]; to[1:] = np.cumsum(tl)[:-1]
+    return {"n": n, "queries": np.concatenate(qs), "targets": np.concatenate(ts), "q_off": qo,
+            "t_off": to, "q_len": np.array(ql, np.uint16), "t_len": np.array(tl, np.uint16),
+            "w": np.array(w, np.int32), "true_score": np.array(tsc, np.int32)}
+
+
+def test_wave_per_job_edges(engine):
+    check(engine, _edges(np.random.default_rng(24)))
+    sp = ScoringParameters(gapOpenPenalty=0, gapExtendPenalty=2, gapOpenPenaltyDeletion=3,
+                           gapExtendPenaltyDeletion=1, mismatchPenalty=3)
+    check(engine, _edges(np.random.default_rng(25)), sp)
