@@ -341,10 +341,17 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
         std::vector<WorkItem> items;
         std::vector<NwLaunch> launches;
         for (int pass = 0; ok && !active.empty(); ++pass) {
+            // small bins and the retry passes one wave per job, as the batched
+            // form of hipsw_nw_run does
             nw_plan_pass(eng, ql.data(), tl.data(), max_pen, active, wcur,
-                         pass == 0 ? HIPSW_MODE_BATCHED : HIPSW_MODE_PLAIN, order, items, launches);
+                         pass == 0 ? HIPSW_MODE_BATCHED : HIPSW_MODE_PLAIN, order, items, launches,
+                         np.o_ins >= 0 ? (pass == 0 ? 1 : 2) : 0);
             size_t zb = 0, gb = 0;
             for (const NwLaunch &L : launches) {
+                if (L.wide) {
+                    if (L.zmax > 49152) zb = std::max(zb, (size_t)L.grid * L.zmax);
+                    continue;
+                }
                 zb = std::max(zb, (size_t)L.grid * L.zrows * ((L.ncol_max + 7) / 8) * 64 * 4);
                 if (L.global_state) gb = std::max(gb, (size_t)L.grid * L.S * 64 * (L.tier == NW_I16 ? 4 : 8));
             }
@@ -357,6 +364,16 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
             int k = 0;
             for (const NwLaunch &L : launches) {
                 if (!ok || k >= 64) { ok = false; break; }
+                if (L.wide) {
+                    const WorkItem &first = items[L.item_begin];
+                    const WorkItem &last = items[L.item_begin + L.item_count - 1];
+                    ok = launch_nw_wide(nd, np, d_order + first.start,
+                                        (int)(last.start + last.count - first.start), sl.counters + k,
+                                        L.qmax, (uint8_t *)scr, L.zmax, L.zmax <= 49152, L.grid,
+                                        st) == hipSuccess;
+                    ++k;
+                    continue;
+                }
                 ok = launch_nw(L.tier, L.global_state, nd, np, d_order, d_items + L.item_begin,
                                L.item_count, sl.counters + k, L.S, (uint32_t *)scr, L.zrows,
                                (L.ncol_max + 7) / 8, L.grid, pass == 0, (char *)scr + ((zb + 255) & ~(size_t)255),
