@@ -27,13 +27,14 @@ sys.path.insert(0, ROOT)
 OPS_PER_CELL = 11          # algorithmic VALU ops per extension cell
 PEAK_PK16_TOPS = 78.6      # 256 CU x 64 lanes x 2.4 GHz x 2 (packed int16)
 PEAK_I32_TOPS = 39.3       # 32-bit VALU lane-ops
-# HBM bytes per launch of the headline kernel (ext_persistent_pk_kernel<true, 10>
-# on the 1,000,000-pair set), from two rocprofv3 --pmc passes over
-# `bench.py --profile-leg ext`: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB, the
-# factors being what the same counters read for a 256 MiB device-to-device
+# HBM bytes per launch of the headline kernel (ext_persistent_pk_kernel<true, 10, 5>
+# on the 1,000,000-pair set: the autotuned layout, column state of columns 80+
+# in HBM), from two rocprofv3 --pmc passes over
+# `bench.py --profile-leg ext --pk-lds-blocks 5`: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB,
+# the factors being what the same counters read for a 256 MiB device-to-device
 # copy (`bench.py --profile-leg copy`: FETCH_SIZE 131,087 KiB, WRITE_SIZE
 # 262,144 KiB).  Summary and commands: profiles/ext_uniform_1M_pmc.md.
-EXT_TRAFFIC_BYTES = 372535603
+EXT_TRAFFIC_BYTES = 28491421713
 # Algorithmic HBM bytes per job: 2-bit query and target words, one 32-byte
 # descriptor, one 24-byte result row.
 
@@ -419,7 +420,7 @@ def main():
             "hbm_frac": round(algo_bytes / (b_kms * 1e-3) / 8.0e12, 6),
             "ops_per_cell": OPS_PER_CELL, "peak_int32": PEAK_I32_TOPS,
             "frac_int32": round(achieved / PEAK_I32_TOPS, 4),
-            "kernel": "ext_persistent_pk_kernel<sym gaps, 10 query words>",
+            "kernel": "ext_persistent_pk_kernel<sym gaps, 10 query words, 5 LDS blocks>",
             "note": "VALU-bound integer DP (cells x 11 algorithmic ops per cell over the packed "
                     "int16 peak); HBM traffic is far below the 8 TB/s bound (hbm_frac)",
         },
