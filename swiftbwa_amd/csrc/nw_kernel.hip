@@ -226,14 +226,19 @@ __device__ void nw_wave(const NwDev &d, const NwParams &p, const uint32_t *idx, 
             const int wq = max(0, min(j0 >> 3, qlim));
             const uint64_t win = (uint64_t)ql[wq * 64] | ((uint64_t)ql[min(wq + 1, qlim) * 64] << 32);
             const uint32_t codes = (uint32_t)(win >> ((j0 & 7) * 4));
+            // the block's eight scores are read before any of its writes too:
+            // a read after a write to the same memory would wait for it, once
+            // per cell of the serial chain
+            int sc[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) sc[k] = smat[t5 + (int)((codes >> (4 * k)) & 0xFu)];
             uint32_t zword = 0;
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 // Lanes past their band end (or without a job) keep their LDS
                 // state and record no flags.
                 const bool inb = act && cb * 8 + k < width;
-                const int q = (int)((codes >> (4 * k)) & 0xFu);
-                int h_cur = h_diag + (int)smat[t5 + q];
+                int h_cur = h_diag + sc[k];
                 const int hd = St::h(v[k]);
                 const int e_open = hd - p.oe_del;
                 const int e_ext = St::e(v[k]) - p.e_del;
