@@ -53,7 +53,10 @@ __device__ __forceinline__ int wave_max_nw(int v)
 // CIGARBuilder merging (CIGARBuilder.swift:4-38); ops come out last-first.
 // They go to a ring of `cap` slots, so when the CIGAR has more than cap ops
 // the row still ends up holding the first cap ops of the final CIGAR.
-// zf(i, c) returns the 4 flag bits of row i, band column c = j - max(0, i - w).
+// zf(i, c) returns the 4 flag bits of row i, band column c = j - max(0, i - w):
+// bit 0 E opened from H (e_open > e_ext), bit 1 F opened from H
+// (f_open > f_ext), bit 2 H taken from E, bit 3 H taken from F -- the
+// reference's z byte with bits 0 and 1 negated (:114-136).
 template <class ZF>
 __device__ __forceinline__ void nw_traceback(const NwDev &d, int job, int qlen, int tlen, int w,
                                              int ncol, ZF zf)
@@ -92,11 +95,11 @@ __device__ __forceinline__ void nw_traceback(const NwDev &d, int job, int qlen, 
             else state = 2;
         } else if (state == 1) {
             emit(2u);
-            if ((dd & 1u) == 0) state = 0;
+            if (dd & 1u) state = 0;                            // E was opened from H
             --i;
         } else {
             emit(1u);
-            if ((dd & 2u) == 0) state = 0;
+            if (dd & 2u) state = 0;                            // F was opened from H
             --jj;
         }
     }
@@ -232,30 +235,39 @@ __device__ void nw_wave(const NwDev &d, const NwParams &p, const uint32_t *idx, 
             int sc[8];
 #pragma unroll
             for (int k = 0; k < 8; ++k) sc[k] = smat[t5 + (int)((codes >> (4 * k)) & 0xFu)];
+            // Flag bits (the serial loop's, :114-136, with bits 0 and 1
+            // stored as "opened": E opened = e_open > e_ext, F opened =
+            // f_open > f_ext; bit 2 H from E, bit 3 H from F) are the sign
+            // bits of differences, shifted into zword with one alignbit each:
+            // cell k's nibble ends up at bits 28 - 4k .. 31 - 4k.
             uint32_t zword = 0;
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 // Lanes past their band end (or without a job) keep their LDS
                 // state and record no flags.
                 const bool inb = act && cb * 8 + k < width;
-                int h_cur = h_diag + sc[k];
+                const int h_cur = h_diag + sc[k];
                 const int hd = St::h(v[k]);
                 const int e_open = hd - p.oe_del;
                 const int e_ext = St::e(v[k]) - p.e_del;
-                const uint32_t e_flag = e_open > e_ext ? 0u : 1u;
-                const int ev = e_open > e_ext ? e_open : e_ext;
+                const int ev = max(e_open, e_ext);
                 const int f_open = h_left - p.oe_ins;
                 const int f_ext = f - p.e_ins;
-                const uint32_t f_flag = f_open > f_ext ? 0u : 2u;
-                f = f_open > f_ext ? f_open : f_ext;
-                const uint32_t from_e = ev > h_cur ? 4u : 0u;
-                h_cur = ev > h_cur ? ev : h_cur;
-                const uint32_t from_f = f > h_cur ? 8u : 0u;
-                h_cur = f > h_cur ? f : h_cur;
-                col[sl[k] * 64] = inb ? St::pack(h_cur, ev) : v[k];
+                f = max(f_open, f_ext);
+                const int h1 = max(ev, h_cur);
+                const int hn = max(f, h1);
+                zword = __builtin_amdgcn_alignbit(zword, (uint32_t)(h1 - f), 31);        // f > h1
+                zword = __builtin_amdgcn_alignbit(zword, (uint32_t)(h_cur - ev), 31);    // ev > h_cur
+                zword = __builtin_amdgcn_alignbit(zword, (uint32_t)(f_ext - f_open), 31);
+                zword = __builtin_amdgcn_alignbit(zword, (uint32_t)(e_ext - e_open), 31);
+                col[sl[k] * 64] = inb ? St::pack(hn, ev) : v[k];
                 h_diag = hd;
-                h_left = h_cur;
-                zword |= inb ? (e_flag | f_flag | from_e | from_f) << (4 * k) : 0u;
+                h_left = hn;
+            }
+            {
+                // cells past the band end record no flags
+                const int nv = act ? min(max(width - cb * 8, 0), 8) : 0;
+                zword &= nv >= 8 ? 0xFFFFFFFFu : ~(0xFFFFFFFFu >> (4 * nv)) & (nv ? 0xFFFFFFFFu : 0u);
             }
             if (i < zrows) zl[((size_t)i * ncw + cb) * 64] = zword;
         }
@@ -289,7 +301,7 @@ __device__ void nw_wave(const NwDev &d, const NwParams &p, const uint32_t *idx, 
         *o_score = St::h(col[s * 64]);
     }
     nw_traceback(d, job, qlen, tlen, w, ncol, [&](int i, int c) {
-        return (zl[((size_t)i * ncw + (c >> 3)) * 64] >> ((c & 7) * 4)) & 0xFu;
+        return (zl[((size_t)i * ncw + (c >> 3)) * 64] >> ((7 - (c & 7)) * 4)) & 0xFu;
     });
 }
 
@@ -355,7 +367,8 @@ __global__ __launch_bounds__(64) void nw_kernel(NwDev d, NwParams p, const uint3
 // whenever F(j-1) > G(j-1), so the F-free G carries the recurrence (:121-127);
 // C0 folds the row-start h[beg] and the initial f = Int32.min/2.  It is one
 // max-scan of G(k) - oe_ins + e_ins k (fused v_max_i32_dpp steps).  The four
-// flag bits of every cell are the serial loop's (:114-136); they go to LDS
+// flag bits of every cell are the serial loop's (:114-136, in nw_traceback's
+// encoding); they go to LDS
 // (bytes, [row][j - beg] as the reference's z) when tlen * ncol fits, else to
 // the wave's HBM region, and lane 0 walks the traceback from there.
 template <int KC>
@@ -428,7 +441,7 @@ __device__ void nw_wide_job(const NwDev &d, const NwParams &p, int job, uint8_t 
             const int sc = t < 4 ? __builtin_amdgcn_sbfe(spk[k], 8 * t, 8) : s4[k];
             const int hcur = dsrc + sc;                           // :110
             const int e_open = hold - p.oe_del, e_ext = eold - p.e_del;   // :116-120
-            const uint32_t e_flag = e_open > e_ext ? 0u : 1u;
+            const uint32_t e_flag = e_open > e_ext ? 1u : 0u;          // E opened
             const int ev = e_open > e_ext ? e_open : e_ext;
             const int g = max(hcur, ev);
             // exclusive prefix max of G(k) - oe_ins + e_ins k over the band
@@ -452,7 +465,7 @@ __device__ void nw_wide_job(const NwDev &d, const NwParams &p, int job, uint8_t 
             int fl = __builtin_amdgcn_update_dpp(0, fv, 0x138, 0xF, 0xF, false);
             if (lane == 0) { hl = hcarry; fl = fcarry; }
             if (j == beg) { hl = hb; fl = NEG; }
-            const uint32_t f_flag = (hl - p.oe_ins) > (fl - p.e_ins) ? 0u : 2u;
+            const uint32_t f_flag = (hl - p.oe_ins) > (fl - p.e_ins) ? 2u : 0u;   // F opened
             hcarry = __builtin_amdgcn_readlane(hv, 63);
             fcarry = __builtin_amdgcn_readlane(fv, 63);
             if (act) {
