@@ -117,4 +117,8 @@ struct hipsw_ext_set {
     size_t batched_scratch = 0;
     size_t items_cap = 0;            // capacity of d_items (work items)
     int64_t tier_jobs[3] = {0, 0, 0};
+    // batched form as one graph (counter resets, then the bins); rebuilt when
+    // the plan or the scratch region changes
+    hipGraphExec_t graph = nullptr;
+    void *graph_scratch = nullptr;
 };

@@ -270,3 +270,51 @@ def test_size_and_tier_boundaries(engine):
           "end_bonus": np.array(eb, np.int32)}
     check(engine, js)
     check(engine, js, ScoringParameters(gapOpenPenalty=4, gapExtendPenalty=2, zDrop=40))
+
+
+def test_batched_graph_replays(engine):
+    """The batched form runs as a captured graph (counter resets, then the
+    bins).  Replays interleaved with eager work on the same
+    engine -- the plain form of the same set, and another set whose larger
+    scratch need moves the engine's scratch region (which forces a re-capture)
+    -- each give the oracle's results and cell count."""
+    import ctypes as C
+    from swiftbwa_amd.metalsw import _ext_arrays
+    sp = ScoringParameters()
+    sc = sp.to_c()
+    lib = engine.lib
+
+    def upload(js):
+        a = _ext_arrays(js)
+        st = lib.hipsw_ext_upload(engine.handle, js["n"], native.ptr(a["queries"]),
+                                  native.ptr(a["q_off"]), native.ptr(a["q_len"]),
+                                  native.ptr(a["targets"]), native.ptr(a["t_off"]),
+                                  native.ptr(a["t_len"]), native.ptr(a["h0"]), native.ptr(a["w"]),
+                                  native.ptr(a["end_bonus"]), C.byref(sc))
+        assert st
+        return st
+
+    js1 = wl.generate(wl.EXT_MIXED, 11, 30000)
+    js2 = wl.generate(wl.EXT_MIXED, 12, 60000)
+    ref1, cells1 = orc.ksw_extend_batch(js1, oracle_sc(sp), nthreads=8)
+    ref2, cells2 = orc.ksw_extend_batch(js2, oracle_sc(sp), nthreads=8)
+    s1, s2 = upload(js1), upload(js2)
+    try:
+        out1 = np.zeros((js1["n"], 6), np.int32)
+        out2 = np.zeros((js2["n"], 6), np.int32)
+
+        def run(st, mode, out, ref, cells):
+            s = native.HipswStats()
+            assert lib.hipsw_ext_run(engine.handle, st, mode, C.byref(s)) == 0
+            assert lib.hipsw_ext_download(engine.handle, st, native.ptr(out)) == 0
+            assert np.array_equal(out, ref)
+            assert s.cells == cells
+
+        for rep in range(4):
+            run(s1, native.MODE_BATCHED, out1, ref1, cells1)
+            run(s1, native.MODE_PLAIN, out1, ref1, cells1)
+            run(s2, native.MODE_BATCHED, out2, ref2, cells2)
+        run(s1, native.MODE_BATCHED, out1, ref1, cells1)
+    finally:
+        lib.hipsw_ext_free(engine.handle, s1)
+        lib.hipsw_ext_free(engine.handle, s2)
