@@ -291,6 +291,8 @@ typedef struct {
     double latency_ms;              /* submit -> results ready                  */
     double queue_ms, run_ms;
     int64_t cells;                  /* extension + NW fill cells                */
+    double host_ms;                 /* part of run_ms spent packing and planning
+                                       the chunk on the host                     */
 } hipsw_chunk_result;
 
 hipsw_server *hipsw_server_create(hipsw_engine *eng, int32_t n_slots, int64_t hbm_budget,

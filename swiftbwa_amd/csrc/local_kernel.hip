@@ -27,10 +27,15 @@ __device__ __forceinline__ int row_shr1(int v)
     return __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, true);
 }
 
+// Maximum over each 16-lane row, in every lane of the row (values >= 0): four
+// DPP steps, each fused into one v_max_i32_dpp by passing the identity as the
+// "old" value.
 __device__ __forceinline__ int row_max16(int v)
 {
-#pragma unroll
-    for (int o = 1; o < 16; o <<= 1) v = max(v, __shfl_xor(v, o, 16));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false));    // quad_perm 1,0,3,2
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false));    // quad_perm 2,3,0,1
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false));   // row_half_mirror
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false));   // row_mirror
     return v;
 }
 
@@ -136,6 +141,7 @@ __device__ bool striped_pass(const LocalDev &d, const LocalParams &p, bool act, 
         bool go = live;
 #pragma unroll
         for (int s = 0; s < SNMAX; ++s) {
+            if (__builtin_amdgcn_ballot_w64(go && s < sn) == 0) break;   // no task still correcting
             const bool in = go && s < sn;
             const int nh = max(H[s], cf);
             const uint64_t changed = __ballot(in && nh != H[s]);
@@ -146,26 +152,36 @@ __device__ bool striped_pass(const LocalDev &d, const LocalParams &p, bool act, 
                 cf = satsub(max(satsub(nh, goe), cf), ge);
             }
         }
-        // column max and best cell (:245-263)
-        int col_max = 0;
+        // column max and best cell (:245-263).  The stripe-by-stripe walk is
+        // needed only in a row whose column maximum beats the task's best or
+        // overflows; otherwise the column maximum alone (for the peaks) is the
+        // same, so it is taken over all stripes at once.
+        int lmax = 0;
 #pragma unroll
-        for (int s = 0; s < SNMAX; ++s) {
-            const bool in = live && s < sn;
-            const int lm = row_max16(in ? H[s] : 0);
-            if (in) {
-                if (lm > 250) ok = false;
-                if (ok) {
-                    col_max = max(col_max, lm);
+        for (int s = 0; s < SNMAX; ++s) lmax = (live && s < sn) ? max(lmax, H[s]) : lmax;
+        const int cmax = row_max16(lmax);
+        int col_max = cmax;
+        if (__builtin_amdgcn_ballot_w64(live && ok && (cmax > max_score || cmax > 250)) != 0) {
+            col_max = 0;
+#pragma unroll
+            for (int s = 0; s < SNMAX; ++s) {
+                const bool in = live && s < sn;
+                const int lm = row_max16(in ? H[s] : 0);
+                if (in) {
+                    if (lm > 250) ok = false;
+                    if (ok) {
+                        col_max = max(col_max, lm);
+                    }
                 }
-            }
-            const bool better = in && ok && lm > max_score;
-            const int j = s + lane16 * sn;
-            const uint64_t hit = __ballot(better && H[s] == lm && j < qlen);
-            if (better) {
-                max_score = lm;
-                max_i = i;
-                const uint32_t m16 = (uint32_t)((hit >> (grp * 16)) & 0xFFFFull);
-                if (m16) max_j = s + (__ffs((int)m16) - 1) * sn;
+                const bool better = in && ok && lm > max_score;
+                const int j = s + lane16 * sn;
+                const uint64_t hit = __ballot(better && H[s] == lm && j < qlen);
+                if (better) {
+                    max_score = lm;
+                    max_i = i;
+                    const uint32_t m16 = (uint32_t)((hit >> (grp * 16)) & 0xFFFFull);
+                    if (m16) max_j = s + (__ffs((int)m16) - 1) * sn;
+                }
             }
         }
         if (live && ok && min_sub > 0 && lane16 == 0) colmax[i] = (uint8_t)col_max;

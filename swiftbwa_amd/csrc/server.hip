@@ -88,8 +88,10 @@ struct ChunkReq {
     int32_t loc_min_sub = 0;
     // outputs
     std::vector<int32_t> ext_out, loc_out, loc_ok, nw_score, nw_ncig, nw_flags;
-    std::vector<uint32_t> nw_cig;
+    std::vector<uint32_t> nw_cig;                // the used ops of every CIGAR, back to back
+    std::vector<uint32_t> nw_cig_off;            // n_nw + 1 offsets into nw_cig
     int64_t cells = 0;
+    double host_ms = 0;                          // host packing and planning of the chunk
 };
 
 struct Slot {
@@ -145,6 +147,18 @@ int fetched_len(int64_t pos, int len, int64_t gl)
 
 double now_ms_s() { return hipsw::now_ms(); }
 
+// Copies the used ops of each job's CIGAR row (cap ops per row) back to back:
+// job k's n[k] ops go to dst[off[k] ..).  Only these bytes cross PCIe.
+__global__ void compact_cigars_kernel(const uint32_t *src, int cap, const uint32_t *off, int n,
+                                      uint32_t *dst)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const uint32_t o = off[k], used = off[k + 1] - o;
+        const uint32_t *row = src + (size_t)k * cap;
+        for (uint32_t j = 0; j < used; ++j) dst[o + j] = row[j];
+    }
+}
+
 // Runs one chunk on one slot.  Returns a status.
 int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
 {
@@ -152,16 +166,21 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
     const int n = r.n_ext + r.n_loc + r.n_nw;
     if (n == 0) return HIPSW_OK;
     hipStream_t st = sl.stream;
+    const double t_host0 = now_ms_s();
     // --- host packing: queries from the request, targets as fetch windows ---
     std::vector<uint32_t> qw(n), tw(n), qnw(n, kNoN), tnw(n, kNoN);
     std::vector<int32_t> ql(n), tl(n);
     std::vector<Window> win(n);
     uint64_t words = 0, nwords = 0;
+    std::vector<uint8_t> qhas_n(n);
+    parallel_for(n, host_threads(), [&](int64_t lo, int64_t hi) {
+        for (int64_t k = lo; k < hi; ++k) qhas_n[k] = has_n(r.qbytes.data() + r.q_off[k], r.q_len[k]);
+    });
     for (int k = 0; k < n; ++k) {
         ql[k] = r.q_len[k];
         qw[k] = (uint32_t)words;
         words += (ql[k] + 15) / 16;
-        if (has_n(r.qbytes.data() + r.q_off[k], ql[k])) {
+        if (qhas_n[k]) {
             qnw[k] = (uint32_t)nwords;
             nwords += (ql[k] + 31) / 32;
         }
@@ -173,10 +192,14 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
     words += 64;
     nwords += 64;
     std::vector<uint32_t> seq(words, 0), nmask(nwords, 0);
-    for (int k = 0; k < n; ++k)
-        if (!pack2(r.qbytes.data() + r.q_off[k], ql[k], seq.data() + qw[k],
-                   qnw[k] != kNoN ? nmask.data() + qnw[k] : nullptr))
-            return HIPSW_E_INVALID;
+    std::atomic<bool> packed_ok{true};
+    parallel_for(n, host_threads(), [&](int64_t lo, int64_t hi) {
+        for (int64_t k = lo; k < hi; ++k)
+            if (!pack2(r.qbytes.data() + r.q_off[k], ql[k], seq.data() + qw[k],
+                       qnw[k] != kNoN ? nmask.data() + qnw[k] : nullptr))
+                packed_ok = false;
+    });
+    if (!packed_ok) return HIPSW_E_INVALID;
     // extension plan (jobs [0, n_ext))
     const ExtParams ep = make_params(&srv->sc);
     std::vector<uint32_t> ext_order;
@@ -222,6 +245,7 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
                          span<WorkItem>(n / 64 + 64) + span<uint8_t>(colmax_b) +
                          span<PkMeta>(r.n_ext) +
                          span<unsigned long long>(2);
+    r.host_ms = now_ms_s() - t_host0;
     void *dmem = nullptr;
     if (hipMallocAsync(&dmem, bytes, st) != hipSuccess) return HIPSW_E_DEVICE;
     char *cur = reinterpret_cast<char *>(dmem);
@@ -405,9 +429,29 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
     down(sc.data(), d_score, (size_t)n * 4);
     down(nc.data(), d_ncig, (size_t)n * 4);
     down(fl.data(), d_flags, (size_t)n * 4);
-    r.nw_cig.assign((size_t)r.n_nw * cap, 0);
-    down(r.nw_cig.data(), d_cig, (size_t)r.n_nw * cap * 4);
     down(cells, d_cells, sizeof cells);
+    if (ok) ok = hipStreamSynchronize(st) == hipSuccess;
+    // CIGARs: only the used ops of each row come back (rows hold cap ops)
+    r.nw_cig_off.assign((size_t)r.n_nw + 1, 0);
+    for (int k = 0; k < r.n_nw; ++k)
+        r.nw_cig_off[k + 1] = r.nw_cig_off[k] + (uint32_t)std::min(std::max(nc[nw0 + k], 0), cap);
+    r.nw_cig.assign(r.n_nw ? (size_t)r.nw_cig_off[r.n_nw] : 0, 0u);
+    if (ok && r.n_nw && !r.nw_cig.empty()) {
+        void *cm = nullptr;
+        const size_t off_b = ((size_t)(r.n_nw + 1) * 4 + 255) & ~(size_t)255;
+        ok = hipMallocAsync(&cm, off_b + r.nw_cig.size() * 4, st) == hipSuccess;
+        uint32_t *d_off = static_cast<uint32_t *>(cm);
+        uint32_t *d_dense = reinterpret_cast<uint32_t *>(static_cast<char *>(cm) + off_b);
+        if (ok) ok = hipMemcpyAsync(d_off, r.nw_cig_off.data(), (size_t)(r.n_nw + 1) * 4,
+                                    hipMemcpyHostToDevice, st) == hipSuccess;
+        if (ok) {
+            hipLaunchKernelGGL(compact_cigars_kernel, dim3(std::min((r.n_nw + 255) / 256, 4096)),
+                               dim3(256), 0, st, d_cig, cap, d_off, r.n_nw, d_dense);
+            ok = hipGetLastError() == hipSuccess;
+        }
+        down(r.nw_cig.data(), d_dense, r.nw_cig.size() * 4);
+        if (cm) hipFreeAsync(cm, st);
+    }
     hipFreeAsync(dmem, st);
     if (ok) ok = hipStreamSynchronize(st) == hipSuccess;
     if (!ok) return HIPSW_E_DEVICE;
@@ -421,10 +465,6 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
     }
     r.nw_score.assign(sc.begin() + nw0, sc.begin() + nw0 + r.n_nw);
     r.nw_ncig.assign(nc.begin() + nw0, nc.begin() + nw0 + r.n_nw);
-    for (int k = 0; k < r.n_nw; ++k) {                       // entries past the CIGAR are zero
-        const int used = std::min(std::max(r.nw_ncig[k], 0), cap);
-        std::fill(r.nw_cig.begin() + (size_t)k * cap + used, r.nw_cig.begin() + (size_t)(k + 1) * cap, 0u);
-    }
     r.nw_flags.assign(fl.begin() + nw0, fl.begin() + nw0 + r.n_nw);
     r.cells = (int64_t)(cells[0] + cells[1]);
     return HIPSW_OK;
@@ -589,6 +629,7 @@ extern "C" int hipsw_server_wait(hipsw_server *srv, int64_t ticket, hipsw_chunk_
         res->latency_ms = r->t_done - r->t_submit;
         res->queue_ms = r->t_start - r->t_submit;
         res->run_ms = r->t_done - r->t_start;
+        res->host_ms = r->host_ms;
         res->cells = r->cells;
         if (r->status == HIPSW_OK) {
             const int cap = srv->nw_cap;
@@ -598,7 +639,18 @@ extern "C" int hipsw_server_wait(hipsw_server *srv, int64_t ticket, hipsw_chunk_
             if (res->nw_score && r->n_nw) memcpy(res->nw_score, r->nw_score.data(), (size_t)r->n_nw * 4);
             if (res->nw_n_cigar && r->n_nw) memcpy(res->nw_n_cigar, r->nw_ncig.data(), (size_t)r->n_nw * 4);
             if (res->nw_flags && r->n_nw) memcpy(res->nw_flags, r->nw_flags.data(), (size_t)r->n_nw * 4);
-            if (res->nw_cigar && r->n_nw) memcpy(res->nw_cigar, r->nw_cig.data(), (size_t)r->n_nw * cap * 4);
+            if (res->nw_cigar && r->n_nw) {
+                // rows of cap ops: the used ones, then zeros
+                const ChunkReq *rq = r.get();
+                parallel_for(rq->n_nw, host_threads(), [&](int64_t lo, int64_t hi) {
+                    for (int64_t k = lo; k < hi; ++k) {
+                        const uint32_t o = rq->nw_cig_off[k], used = rq->nw_cig_off[k + 1] - o;
+                        uint32_t *row = res->nw_cigar + (size_t)k * cap;
+                        if (used) memcpy(row, rq->nw_cig.data() + o, (size_t)used * 4);
+                        memset(row + used, 0, (size_t)(cap - used) * 4);
+                    }
+                });
+            }
         }
     }
     return r->status;

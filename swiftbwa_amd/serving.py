@@ -150,7 +150,8 @@ class Server:
             raise native.HipswError(rc, "chunk failed")
         return {"ext": ext, "loc": loc, "loc_ok": lok, "nw_score": nws, "nw_cigar": nwc,
                 "nw_n_cigar": nwn, "nw_flags": nwf, "latency_ms": r.latency_ms,
-                "queue_ms": r.queue_ms, "run_ms": r.run_ms, "cells": r.cells}
+                "queue_ms": r.queue_ms, "run_ms": r.run_ms, "cells": r.cells,
+                "host_ms": r.host_ms}
 
     def stats(self):
         a, b = C.c_int64(), C.c_int64()
@@ -190,7 +191,8 @@ def run_poisson(server, chunks, rate_per_s, seed=0):
         ready.acquire()
         r = server.wait(tickets[k])
         out.append({"pairs": chunks[k]["pairs"], "latency_ms": r["latency_ms"],
-                    "queue_ms": r["queue_ms"], "run_ms": r["run_ms"], "cells": r["cells"]})
+                    "queue_ms": r["queue_ms"], "run_ms": r["run_ms"], "cells": r["cells"],
+                    "host_ms": r["host_ms"]})
     th.join()
     wall = time.perf_counter() - t0
     return out, wall
