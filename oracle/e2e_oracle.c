@@ -1,0 +1,1265 @@
+/*
+ * End-to-end single-end harness of the CPU oracle -- TEST INFRASTRUCTURE.
+ *
+ * Only tests/ load this code, as the checker: it restates swiftbwa's single-
+ * end `mem` path (file:line citations at each function) so that the GPU
+ * library's extension and CIGAR results can be taken through the same
+ * seeding, chaining, dedup, primary-marking and SAM steps as the CPU path.
+ *
+ * Parity pins (tests/test_oracle_e2e.py):
+ *   index + SMEMs    brute force on the text: every SMEM's SA interval equals
+ *                    the set of exact-match positions of its substring in
+ *                    fwd + revcomp, it cannot be extended either way, and
+ *                    every maximal match >= min_seed_len is reported; SA is
+ *                    sorted; occ / count identities.
+ *   whole pipeline   the reference's lambda fixtures
+ *                    (TestData/e2e_expected_se.sam via IntegrationTests.swift:
+ *                    319-358: POS, FLAG, CIGAR, MAPQ within 5, NM, MD).
+ *   chainer, filter, dedup, markSecondary, MAPQ, SAM fields: pinned only
+ *                    through the lambda fixtures (parity partially unpinned:
+ *                    the fixtures exercise unique loci, one chain per read).
+ */
+#define _GNU_SOURCE
+#include "e2e_oracle.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---------------------------------------------------------------------- */
+/* options (ScoringParameters.swift:3-42 defaults)                         */
+/* ---------------------------------------------------------------------- */
+enum {
+    MAX_OCC = 500, MIN_OUT_SCORE = 30, MAX_CHAIN_GAP = 10000, SPLIT_WIDTH = 10,
+    MAX_XA_HITS = 5, MIN_CHAIN_WEIGHT = 0
+};
+static const float SEED_SPLIT_RATIO = 1.5f, MASK_LEVEL = 0.50f, MASK_LEVEL_REDUN = 0.95f,
+                   CHAIN_DROP_RATIO = 0.50f;
+
+/* ---------------------------------------------------------------------- */
+/* stable merge sort (Swift's sort keeps equal elements in order)          */
+/* ---------------------------------------------------------------------- */
+typedef int (*before_fn)(const void *a, const void *b, void *ctx);
+
+static void msort(char *a, char *tmp, size_t n, size_t sz, before_fn before, void *ctx)
+{
+    if (n < 2) return;
+    size_t h = n / 2;
+    msort(a, tmp, h, sz, before, ctx);
+    msort(a + h * sz, tmp, n - h, sz, before, ctx);
+    size_t i = 0, j = h, o = 0;
+    while (i < h && j < n) {
+        if (before(a + j * sz, a + i * sz, ctx)) memcpy(tmp + (o++) * sz, a + (j++) * sz, sz);
+        else memcpy(tmp + (o++) * sz, a + (i++) * sz, sz);
+    }
+    while (i < h) memcpy(tmp + (o++) * sz, a + (i++) * sz, sz);
+    while (j < n) memcpy(tmp + (o++) * sz, a + (j++) * sz, sz);
+    memcpy(a, tmp, n * sz);
+}
+
+static void stable_sort(void *base, size_t n, size_t sz, before_fn before, void *ctx)
+{
+    if (n < 2) return;
+    char *tmp = (char *)malloc(n * sz);
+    msort((char *)base, tmp, n, sz, before, ctx);
+    free(tmp);
+}
+
+/* ---------------------------------------------------------------------- */
+/* types                                                                   */
+/* ---------------------------------------------------------------------- */
+typedef struct { int64_t k, l; int32_t qb, qe; } smem_t;          /* SMEM.swift:5-25 */
+typedef struct { int64_t k, l, s; int32_t m, n; } bidi_t;          /* SMEMFinder.swift:62-68 */
+typedef struct { int64_t rbeg; int32_t qbeg, len, score; } seed_t; /* Seed.swift */
+
+typedef struct {                                                   /* Chain.swift:4-22 */
+    seed_t *seeds;
+    int n, cap;
+    int32_t weight, rid, kept, first;
+    int64_t pos;
+    float frac_rep;
+} chain_t;
+
+typedef struct {                                                   /* AlignmentRegion.swift */
+    int64_t rb, re;
+    int32_t qb, qe, rid, score, true_score, sub, csub, sub_n, w, seed_cov, secondary;
+    uint64_t hash;
+    float frac_rep;
+    int64_t cig;                                                   /* CIGAR job, -1: dummy */
+} region_t;
+
+typedef struct { smem_t *v; int n, cap; } smem_vec;
+typedef struct { chain_t *v; int n, cap; } chain_vec;
+typedef struct { region_t *v; int n, cap; } region_vec;
+
+typedef struct {                                                   /* one CIGAR job */
+    int read;
+    int64_t q_off, r_off;
+    int32_t q_len, r_len, qb, qe, read_len, true_score, w;
+    uint8_t rev;
+    int64_t ref_pos;
+    /* result */
+    uint32_t *cigar;
+    int n_cigar;
+    int32_t nm;
+    char *md;
+    int64_t pos;
+    int done;
+} cjob_t;
+
+typedef struct {                                                   /* one extension call */
+    int64_t q_off, t_off;
+    int32_t q_len, t_len, h0, w, eb;
+    or_swres_t r;
+} call_t;
+
+struct or_e2e {
+    or_scoring_t sc;
+    /* index */
+    int64_t gl, n;                  /* n = 2 gl (text without '$') */
+    uint8_t *fwd, *text, *bwt;
+    int64_t *sa;                    /* n + 1 rows, sa[0] = n */
+    int64_t cnt[5];
+    int64_t sentinel;
+    int32_t *occ;                   /* (n + 2) x 4 prefix counts */
+    char name[256];
+    /* batch */
+    int n_reads;
+    uint8_t *reads;
+    int64_t *read_off;
+    int32_t *read_len;
+    chain_vec chains;
+    int32_t *chain_read;            /* per chain */
+    region_vec *regs;               /* per read */
+    cjob_t *jobs;
+    int64_t n_jobs, jobs_cap;
+    uint8_t *cq, *cr;               /* CIGAR segment arenas */
+    int64_t cq_n, cq_cap, cr_n, cr_cap;
+    call_t *calls;
+    int64_t n_calls, calls_cap;
+    uint8_t *lq, *lt;
+    int64_t lq_n, lq_cap, lt_n, lt_cap;
+};
+
+#define VEC_PUSH(vec, x)                                                              \
+    do {                                                                              \
+        if ((vec).n == (vec).cap) {                                                   \
+            (vec).cap = (vec).cap ? 2 * (vec).cap : 16;                               \
+            (vec).v = realloc((vec).v, (size_t)(vec).cap * sizeof(*(vec).v));         \
+        }                                                                             \
+        (vec).v[(vec).n++] = (x);                                                     \
+    } while (0)
+
+static void *grow(void *p, int64_t *cap, int64_t need, size_t sz)
+{
+    if (need <= *cap) return p;
+    int64_t c = *cap ? *cap : 1024;
+    while (c < need) c *= 2;
+    *cap = c;
+    return realloc(p, (size_t)c * sz);
+}
+
+/* ---------------------------------------------------------------------- */
+/* index: IndexBuilder.swift:20-110, FMIndexWriter.swift:18-60             */
+/* ---------------------------------------------------------------------- */
+static int suffix_cmp(const void *a, const void *b, void *ctx)
+{
+    const or_e2e_t *e = (const or_e2e_t *)ctx;
+    int64_t x = *(const int64_t *)a, y = *(const int64_t *)b;
+    const uint8_t *t = e->text;
+    while (x < e->n && y < e->n && t[x] == t[y]) { ++x; ++y; }
+    if (x == e->n) return y == e->n ? 0 : -1;                  /* '$' sorts first */
+    if (y == e->n) return 1;
+    return (int)t[x] - (int)t[y];
+}
+
+or_e2e_t *or_e2e_create(const uint8_t *fwd, int64_t gl, const char *name, const or_scoring_t *sc)
+{
+    if (gl <= 0) return NULL;
+    or_e2e_t *e = (or_e2e_t *)calloc(1, sizeof *e);
+    e->sc = *sc;
+    e->gl = gl;
+    e->n = 2 * gl;
+    snprintf(e->name, sizeof e->name, "%s", name ? name : "ref");
+    e->fwd = (uint8_t *)malloc((size_t)gl);
+    memcpy(e->fwd, fwd, (size_t)gl);
+    e->text = (uint8_t *)malloc((size_t)e->n);
+    for (int64_t i = 0; i < gl; ++i) e->text[i] = fwd[i] & 3;
+    for (int64_t i = 0; i < gl; ++i) e->text[gl + i] = (uint8_t)(3 - (fwd[gl - 1 - i] & 3));
+    e->sa = (int64_t *)malloc(sizeof(int64_t) * (size_t)(e->n + 1));
+    e->sa[0] = e->n;                                           /* the '$' suffix */
+    for (int64_t i = 0; i < e->n; ++i) e->sa[i + 1] = i;
+    qsort_r(e->sa + 1, (size_t)e->n, sizeof(int64_t), suffix_cmp, e);
+    e->bwt = (uint8_t *)malloc((size_t)(e->n + 1));
+    e->sentinel = -1;
+    for (int64_t i = 0; i <= e->n; ++i) {
+        if (e->sa[i] > 0) e->bwt[i] = e->text[e->sa[i] - 1];
+        else { e->bwt[i] = 4; e->sentinel = i; }
+    }
+    e->occ = (int32_t *)calloc((size_t)(e->n + 2) * 4, sizeof(int32_t));
+    for (int64_t i = 0; i <= e->n; ++i)
+        for (int c = 0; c < 4; ++c)
+            e->occ[(i + 1) * 4 + c] = e->occ[i * 4 + c] + (e->bwt[i] == c);
+    int64_t f[4] = {0, 0, 0, 0};
+    for (int64_t i = 0; i < e->n; ++i) f[e->text[i]]++;
+    e->cnt[0] = 1;                                             /* count[c] + 1 (BWT.swift:10-11) */
+    for (int c = 0; c < 4; ++c) e->cnt[c + 1] = e->cnt[c] + f[c];
+    return e;
+}
+
+static void batch_clear(or_e2e_t *e)
+{
+    for (int i = 0; i < e->chains.n; ++i) free(e->chains.v[i].seeds);
+    free(e->chains.v);
+    memset(&e->chains, 0, sizeof e->chains);
+    free(e->chain_read); e->chain_read = NULL;
+    if (e->regs) {
+        for (int i = 0; i < e->n_reads; ++i) free(e->regs[i].v);
+        free(e->regs); e->regs = NULL;
+    }
+    for (int64_t j = 0; j < e->n_jobs; ++j) { free(e->jobs[j].cigar); free(e->jobs[j].md); }
+    e->n_jobs = 0;
+    e->cq_n = e->cr_n = 0;
+    e->n_calls = 0; e->lq_n = e->lt_n = 0;
+    free(e->reads); e->reads = NULL;
+    free(e->read_off); e->read_off = NULL;
+    free(e->read_len); e->read_len = NULL;
+    e->n_reads = 0;
+}
+
+void or_e2e_free(or_e2e_t *e)
+{
+    if (!e) return;
+    batch_clear(e);
+    free(e->jobs); free(e->cq); free(e->cr); free(e->calls); free(e->lq); free(e->lt);
+    free(e->fwd); free(e->text); free(e->bwt); free(e->sa); free(e->occ);
+    free(e);
+}
+
+int64_t or_e2e_sa(const or_e2e_t *e, int64_t i) { return e->sa[i]; }
+int or_e2e_bwt(const or_e2e_t *e, int64_t i) { return e->bwt[i]; }
+void or_e2e_counts(const or_e2e_t *e, int64_t *cnt5) { memcpy(cnt5, e->cnt, sizeof e->cnt); }
+
+/* BWT.occ (BWT.swift:58-77): occurrences of c in bwt[0, pos) */
+int64_t or_e2e_occ(const or_e2e_t *e, int64_t pos, int c) { return e->occ[pos * 4 + c]; }
+
+/* BackwardSearch.backwardExt (BackwardSearch.swift:22-71) */
+static bidi_t backward_ext(const or_e2e_t *e, int64_t k, int64_t l, int64_t s, int a)
+{
+    bidi_t r = {0, 0, 0, 0, 0};
+    if (a >= 4) return r;
+    const int64_t sp = k, ep = k + s;
+    int64_t kk[4], ss[4], ll[4];
+    for (int c = 0; c < 4; ++c) {
+        int64_t o0 = or_e2e_occ(e, sp, c), o1 = or_e2e_occ(e, ep, c);
+        kk[c] = e->cnt[c] + o0;
+        ss[c] = o1 - o0;
+    }
+    int64_t sent = (sp <= e->sentinel && ep > e->sentinel) ? 1 : 0;
+    ll[3] = l + sent;
+    ll[2] = ll[3] + ss[3];
+    ll[1] = ll[2] + ss[2];
+    ll[0] = ll[1] + ss[1];
+    r.k = kk[a]; r.l = ll[a]; r.s = ss[a];
+    return r;
+}
+
+/* ---------------------------------------------------------------------- */
+/* SMEMFinder.findSMEMsAtPosition (SMEMFinder.swift:90-270)                */
+/* ---------------------------------------------------------------------- */
+static int smems_at(const or_e2e_t *e, const uint8_t *q, int qlen, int start, int32_t min_seed,
+                    int64_t min_intv, smem_vec *out)
+{
+    int next = start + 1;
+    const int a = q[start];
+    if (a >= 4) return next;                                   /* :104-108 */
+    bidi_t smem = {e->cnt[a], e->cnt[3 - a], e->cnt[a + 1] - e->cnt[a], start, start};
+    bidi_t *prev = (bidi_t *)malloc(sizeof(bidi_t) * (size_t)(qlen + 1));
+    int n_prev = 0;
+    for (int j = start + 1; j < qlen; ++j) {                   /* forward phase :124-157 */
+        const int b = q[j];
+        next = j + 1;
+        if (b >= 4) break;
+        bidi_t ext = backward_ext(e, smem.l, smem.k, smem.s, 3 - b);
+        bidi_t ns = {ext.l, ext.k, ext.s, smem.m, j};
+        if (ns.s != smem.s) prev[n_prev++] = smem;
+        if (ns.s < min_intv) { next = j; break; }
+        smem = ns;
+    }
+    if (smem.s >= min_intv) prev[n_prev++] = smem;             /* :160-163 */
+    for (int lo = 0, hi = n_prev - 1; lo < hi; ++lo, --hi) {   /* :166 reverse */
+        bidi_t t = prev[lo]; prev[lo] = prev[hi]; prev[hi] = t;
+    }
+    for (int bj = start - 1; bj >= 0; --bj) {                  /* backward phase :169-232 */
+        const int b = q[bj];
+        if (b >= 4) break;
+        int n_curr = 0;
+        int64_t curr_s = -1;
+        int p = 0;
+        for (; p < n_prev; ++p) {
+            bidi_t iv = prev[p];
+            bidi_t ext = backward_ext(e, iv.k, iv.l, iv.s, b);
+            bidi_t ni = {ext.k, ext.l, ext.s, bj, iv.n};
+            if (ni.s < min_intv && (iv.n - iv.m + 1) >= min_seed) {
+                smem_t sm = {iv.k, iv.k + iv.s - 1, iv.m, iv.n + 1};
+                VEC_PUSH(*out, sm);
+                break;
+            }
+            if (ni.s >= min_intv && ni.s != curr_s) {
+                curr_s = ni.s;
+                prev[n_curr++] = ni;
+                break;
+            }
+        }
+        for (++p; p < n_prev; ++p) {
+            bidi_t iv = prev[p];
+            bidi_t ext = backward_ext(e, iv.k, iv.l, iv.s, b);
+            bidi_t ni = {ext.k, ext.l, ext.s, bj, iv.n};
+            if (ni.s >= min_intv && ni.s != curr_s) {
+                curr_s = ni.s;
+                prev[n_curr++] = ni;
+            }
+        }
+        n_prev = n_curr;
+        if (n_curr == 0) break;
+    }
+    if (n_prev != 0) {                                         /* :234-244 */
+        bidi_t iv = prev[0];
+        if ((iv.n - iv.m + 1) >= min_seed) {
+            smem_t sm = {iv.k, iv.k + iv.s - 1, iv.m, iv.n + 1};
+            VEC_PUSH(*out, sm);
+        }
+    }
+    free(prev);
+    return next;
+}
+
+/* order of SMEMFinder.swift:53-58: query begin ascending, then length descending */
+static int smem_before(const void *a, const void *b, void *ctx)
+{
+    const smem_t *x = (const smem_t *)a, *y = (const smem_t *)b;
+    if (x->qb != y->qb) return x->qb < y->qb;
+    return (x->qe - x->qb) > (y->qe - y->qb);
+}
+
+/* SMEMFinder.findAllSMEMs (SMEMFinder.swift:28-60, arena form :407-444) */
+static void find_all_smems(const or_e2e_t *e, const uint8_t *q, int qlen, int32_t min_seed,
+                           int64_t min_intv, smem_vec *out)
+{
+    int pos = 0;
+    while (pos < qlen) pos = smems_at(e, q, qlen, pos, min_seed, min_intv, out);
+    stable_sort(out->v, (size_t)out->n, sizeof(smem_t), smem_before, NULL);
+}
+
+int or_e2e_smems(const or_e2e_t *e, const uint8_t *q, int qlen, int32_t min_seed_len,
+                 int64_t min_intv, int64_t *rows, int cap)
+{
+    smem_vec v = {0};
+    if (qlen > 0) find_all_smems(e, q, qlen, min_seed_len, min_intv, &v);
+    for (int i = 0; i < v.n && i < cap; ++i) {
+        rows[i * 4 + 0] = v.v[i].k; rows[i * 4 + 1] = v.v[i].l;
+        rows[i * 4 + 2] = v.v[i].qb; rows[i * 4 + 3] = v.v[i].qe;
+    }
+    int n = v.n;
+    free(v.v);
+    return n;
+}
+
+/* ---------------------------------------------------------------------- */
+/* phases 1-3: BWAMemAligner.alignReadPhase1to3 (:130-205)                 */
+/* ---------------------------------------------------------------------- */
+
+/* internalReseed, arena form (BWAMemAligner.swift:260-312) */
+static void internal_reseed(const or_e2e_t *e, const uint8_t *q, int qlen, smem_vec *sm)
+{
+    const int split_len = (int)((float)e->sc.min_seed_len * SEED_SPLIT_RATIO + 0.499f);
+    const int orig = sm->n;
+    smem_vec extra = {0};
+    for (int i = 0; i < orig; ++i) {
+        smem_t s = sm->v[i];
+        int len = s.qe - s.qb;
+        int64_t count = s.l - s.k + 1;
+        if (len < split_len || count > SPLIT_WIDTH) continue;
+        int mid = (s.qb + s.qe) / 2;
+        smems_at(e, q, qlen, mid, e->sc.min_seed_len, count + 1, &extra);
+    }
+    if (extra.n == 0) { free(extra.v); return; }
+    for (int i = 0; i < extra.n; ++i) VEC_PUSH(*sm, extra.v[i]);
+    free(extra.v);
+    stable_sort(sm->v, (size_t)sm->n, sizeof(smem_t), smem_before, NULL);
+}
+
+/* computeFracRep (BWAMemAligner.swift:208-229) */
+static float frac_rep(const smem_vec *sm, int read_len)
+{
+    int32_t l_rep = 0, rep_b = 0, rep_e = 0;
+    for (int i = 0; i < sm->n; ++i) {
+        const smem_t *s = &sm->v[i];
+        if (s->l - s->k + 1 <= MAX_OCC) continue;
+        if (s->qb > rep_e) { l_rep += rep_e - rep_b; rep_b = s->qb; rep_e = s->qe; }
+        else if (s->qe > rep_e) rep_e = s->qe;
+    }
+    l_rep += rep_e - rep_b;
+    return read_len > 0 ? (float)l_rep / (float)read_len : 0.0f;
+}
+
+static void chain_push(chain_t *c, seed_t s)
+{
+    if (c->n == c->cap) {
+        c->cap = c->cap ? 2 * c->cap : 8;
+        c->seeds = (seed_t *)realloc(c->seeds, sizeof(seed_t) * (size_t)c->cap);
+    }
+    c->seeds[c->n++] = s;
+}
+
+static int seed_before(const void *a, const void *b, void *ctx)
+{
+    return ((const seed_t *)a)->rbeg < ((const seed_t *)b)->rbeg;
+}
+
+/* SeedChainer.chain (SeedChainer.swift:16-107); one reference sequence, so
+ * every seed's rid is 0 and no contig is ALT */
+static void seed_chain(const or_e2e_t *e, const smem_vec *sm, chain_vec *out)
+{
+    for (int i = 0; i < sm->n; ++i) {
+        const smem_t *s = &sm->v[i];
+        int32_t seed_len = s->qe - s->qb;
+        if (seed_len < e->sc.min_seed_len) continue;
+        int64_t occs = s->l - s->k + 1;
+        int64_t step = occs > MAX_OCC ? occs / MAX_OCC : 1;
+        int64_t count = 0;
+        for (int64_t sa_idx = 0; sa_idx < occs && count < MAX_OCC; sa_idx += step, ++count) {
+            int64_t ref_pos = e->sa[s->k + sa_idx];
+            const int32_t rid = 0;
+            seed_t seed = {ref_pos, s->qb, seed_len, seed_len * e->sc.a};
+            int merged = 0;
+            for (int ci = out->n - 1; ci >= 0; --ci) {
+                chain_t *c = &out->v[ci];
+                if (c->rid != rid || c->n == 0) continue;
+                const seed_t *last = &c->seeds[c->n - 1];
+                int64_t rgap = llabs(ref_pos - last->rbeg - (int64_t)last->len);
+                int64_t qgap = llabs((int64_t)seed.qbeg - last->qbeg - (int64_t)last->len);
+                int64_t gap = rgap > qgap ? rgap : qgap;
+                if (gap < MAX_CHAIN_GAP) {
+                    chain_push(c, seed);
+                    c->weight += seed_len;
+                    merged = 1;
+                    break;
+                }
+            }
+            if (!merged) {
+                chain_t c = {0};
+                chain_push(&c, seed);
+                c.weight = seed_len;
+                c.rid = rid;
+                c.pos = ref_pos;
+                c.kept = 3;
+                c.first = -1;
+                VEC_PUSH(*out, c);
+            }
+        }
+    }
+    for (int i = 0; i < out->n; ++i) {                         /* :97-103 */
+        chain_t *c = &out->v[i];
+        stable_sort(c->seeds, (size_t)c->n, sizeof(seed_t), seed_before, NULL);
+        if (c->n) c->pos = c->seeds[0].rbeg;
+    }
+}
+
+/* ChainFilter.chainWeight */
+static int32_t chain_weight(const chain_t *c)
+{
+    int32_t w = 0;
+    int64_t end = 0;
+    for (int i = 0; i < c->n; ++i) {
+        int64_t sb = c->seeds[i].qbeg, se = sb + c->seeds[i].len;
+        if (sb >= end) w += c->seeds[i].len;
+        else if (se > end) w += (int32_t)(se - end);
+        if (se > end) end = se;
+    }
+    int32_t tmp = w;
+    w = 0;
+    end = 0;
+    for (int i = 0; i < c->n; ++i) {
+        int64_t sb = c->seeds[i].rbeg, se = sb + c->seeds[i].len;
+        if (sb >= end) w += c->seeds[i].len;
+        else if (se > end) w += (int32_t)(se - end);
+        if (se > end) end = se;
+    }
+    w = w < tmp ? w : tmp;
+    return w < (1 << 30) ? w : (1 << 30) - 1;
+}
+
+static int chain_heavier(const void *a, const void *b, void *ctx)
+{
+    return ((const chain_t *)a)->weight > ((const chain_t *)b)->weight;
+}
+
+/* ChainFilter.filter */
+static void chain_filter(const or_e2e_t *e, chain_vec *cv)
+{
+    if (cv->n == 0) return;
+    const int32_t min_w = MIN_CHAIN_WEIGHT > 0 ? MIN_CHAIN_WEIGHT : e->sc.min_seed_len;
+    for (int i = 0; i < cv->n; ++i) cv->v[i].weight = chain_weight(&cv->v[i]);
+    int m = 0;
+    for (int i = 0; i < cv->n; ++i) {
+        if (cv->v[i].weight < min_w) free(cv->v[i].seeds);
+        else cv->v[m++] = cv->v[i];
+    }
+    cv->n = m;
+    if (m == 0) return;
+    stable_sort(cv->v, (size_t)m, sizeof(chain_t), chain_heavier, NULL);
+    for (int i = 0; i < m; ++i) { cv->v[i].kept = 0; cv->v[i].first = -1; }
+    cv->v[0].kept = 3;
+    int *kept = (int *)malloc(sizeof(int) * (size_t)m);
+    int nk = 0;
+    kept[nk++] = 0;
+    for (int i = 1; i < m; ++i) {
+        chain_t *ci = &cv->v[i];
+        int32_t ib = ci->n ? ci->seeds[0].qbeg : 0;
+        int32_t ie = ci->n ? ci->seeds[ci->n - 1].qbeg + ci->seeds[ci->n - 1].len : 0;
+        int32_t il = ie - ib;
+        int large = 0, suppressed = 0;
+        for (int x = 0; x < nk; ++x) {
+            chain_t *ck = &cv->v[kept[x]];
+            int32_t kb = ck->n ? ck->seeds[0].qbeg : 0;
+            int32_t ke = ck->n ? ck->seeds[ck->n - 1].qbeg + ck->seeds[ck->n - 1].len : 0;
+            int32_t kl = ke - kb;
+            int32_t ob = ib > kb ? ib : kb, oe = ie < ke ? ie : ke;
+            if (ob < oe) {                                     /* no ALT chains here */
+                int32_t ov = oe - ob, ml = il < kl ? il : kl;
+                if (ml > 0 && (float)ov >= (float)ml * MASK_LEVEL && ml < MAX_CHAIN_GAP) {
+                    large = 1;
+                    if (ck->first < 0) ck->first = i;
+                    if ((float)ci->weight < (float)ck->weight * CHAIN_DROP_RATIO &&
+                        ck->weight - ci->weight >= (e->sc.min_seed_len << 1)) {
+                        suppressed = 1;
+                        break;
+                    }
+                }
+            }
+        }
+        if (!suppressed) {
+            kept[nk++] = i;
+            ci->kept = large ? 2 : 3;
+        }
+    }
+    for (int x = 0; x < nk; ++x) {
+        int f = cv->v[kept[x]].first;
+        if (f >= 0 && f < m) cv->v[f].kept = 1;
+    }
+    free(kept);
+    int n2 = 0;
+    for (int i = 0; i < m; ++i) {
+        if (cv->v[i].kept == 0) free(cv->v[i].seeds);
+        else cv->v[n2++] = cv->v[i];
+    }
+    cv->n = n2;
+}
+
+int64_t or_e2e_seed(or_e2e_t *e, int n_reads, const uint8_t *bases, const int64_t *off,
+                    const int32_t *len)
+{
+    batch_clear(e);
+    e->n_reads = n_reads;
+    int64_t total = 0;
+    for (int i = 0; i < n_reads; ++i) total += len[i];
+    e->reads = (uint8_t *)malloc((size_t)(total ? total : 1));
+    e->read_off = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n_reads + 1));
+    e->read_len = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n_reads + 1));
+    int64_t o = 0;
+    for (int i = 0; i < n_reads; ++i) {
+        memcpy(e->reads + o, bases + off[i], (size_t)len[i]);
+        e->read_off[i] = o;
+        e->read_len[i] = len[i];
+        o += len[i];
+    }
+    e->regs = (region_vec *)calloc((size_t)(n_reads > 0 ? n_reads : 1), sizeof(region_vec));
+    int32_t *cr = NULL;
+    int64_t cr_cap = 0;
+    for (int r = 0; r < n_reads; ++r) {
+        const uint8_t *q = e->reads + e->read_off[r];
+        const int qlen = e->read_len[r];
+        smem_vec sm = {0};
+        if (qlen > 0) find_all_smems(e, q, qlen, e->sc.min_seed_len, 1, &sm);
+        if (sm.n == 0) { free(sm.v); continue; }               /* :140 */
+        internal_reseed(e, q, qlen, &sm);                      /* :143 */
+        /* -y reseeding (:149-171) is off at the default reseedLength 0 */
+        stable_sort(sm.v, (size_t)sm.n, sizeof(smem_t), smem_before, NULL);   /* :174-177 */
+        float fr = frac_rep(&sm, qlen);                        /* :181 */
+        chain_vec cv = {0};
+        seed_chain(e, &sm, &cv);                               /* :184-192 */
+        for (int i = 0; i < cv.n; ++i) cv.v[i].frac_rep = fr;  /* :195-197 */
+        chain_filter(e, &cv);                                  /* :200 */
+        for (int i = 0; i < cv.n; ++i) {
+            VEC_PUSH(e->chains, cv.v[i]);
+            cr = (int32_t *)grow(cr, &cr_cap, e->chains.n, sizeof(int32_t));
+            cr[e->chains.n - 1] = r;
+        }
+        free(cv.v);
+        free(sm.v);
+    }
+    e->chain_read = cr;
+    return e->chains.n;
+}
+
+int64_t or_e2e_n_seeds(const or_e2e_t *e)
+{
+    int64_t n = 0;
+    for (int i = 0; i < e->chains.n; ++i) n += e->chains.v[i].n;
+    return n;
+}
+
+void or_e2e_get_chains(const or_e2e_t *e, int32_t *chain_read, int64_t *chain_off,
+                       int64_t *rbeg, int32_t *qbeg, int32_t *slen, int32_t *sscore)
+{
+    int64_t o = 0;
+    for (int i = 0; i < e->chains.n; ++i) {
+        const chain_t *c = &e->chains.v[i];
+        chain_read[i] = e->chain_read[i];
+        chain_off[i] = o;
+        for (int k = 0; k < c->n; ++k, ++o) {
+            rbeg[o] = c->seeds[k].rbeg; qbeg[o] = c->seeds[k].qbeg;
+            slen[o] = c->seeds[k].len; sscore[o] = c->seeds[k].score;
+        }
+    }
+    chain_off[e->chains.n] = o;
+}
+
+/* ---------------------------------------------------------------------- */
+/* phase 4: extension (ExtensionAligner.extend via or_extend_chain_ex)     */
+/* ---------------------------------------------------------------------- */
+static void log_call(void *ctx, const uint8_t *q, int qlen, const uint8_t *t, int tlen,
+                     int32_t h0, int32_t w, int32_t eb, const or_swres_t *r)
+{
+    or_e2e_t *e = (or_e2e_t *)ctx;
+    e->calls = (call_t *)grow(e->calls, &e->calls_cap, e->n_calls + 1, sizeof(call_t));
+    e->lq = (uint8_t *)grow(e->lq, &e->lq_cap, e->lq_n + qlen + 1, 1);
+    e->lt = (uint8_t *)grow(e->lt, &e->lt_cap, e->lt_n + tlen + 1, 1);
+    call_t *c = &e->calls[e->n_calls++];
+    c->q_off = e->lq_n; c->t_off = e->lt_n;
+    c->q_len = qlen; c->t_len = tlen; c->h0 = h0; c->w = w; c->eb = eb; c->r = *r;
+    memcpy(e->lq + e->lq_n, q, (size_t)qlen); e->lq_n += qlen;
+    memcpy(e->lt + e->lt_n, t, (size_t)tlen); e->lt_n += tlen;
+}
+
+static void add_regions(or_e2e_t *e, int ci, const int64_t *rows, int n)
+{
+    const chain_t *c = &e->chains.v[ci];
+    region_vec *rv = &e->regs[e->chain_read[ci]];
+    for (int k = 0; k < n; ++k) {
+        const int64_t *o = rows + (size_t)k * 10;
+        region_t g;
+        memset(&g, 0, sizeof g);
+        g.qb = (int32_t)o[0]; g.qe = (int32_t)o[1]; g.rb = o[2]; g.re = o[3];
+        g.score = (int32_t)o[4]; g.true_score = (int32_t)o[5]; g.sub = (int32_t)o[6];
+        g.sub_n = (int32_t)o[7]; g.w = (int32_t)o[8]; g.seed_cov = (int32_t)o[9];
+        g.rid = c->rid; g.frac_rep = c->frac_rep;               /* ExtensionAligner.swift:191-195 */
+        g.secondary = -1;
+        g.cig = -1;
+        VEC_PUSH(*rv, g);
+    }
+}
+
+int64_t or_e2e_extend_cpu(or_e2e_t *e)
+{
+    e->n_calls = 0; e->lq_n = e->lt_n = 0;
+    int64_t total = 0;
+    for (int ci = 0; ci < e->chains.n; ++ci) {
+        const chain_t *c = &e->chains.v[ci];
+        int r = e->chain_read[ci];
+        int32_t *sq = (int32_t *)malloc(sizeof(int32_t) * (size_t)c->n * 3);
+        int32_t *sl = sq + c->n, *ss = sq + 2 * c->n;
+        int64_t *sr = (int64_t *)malloc(sizeof(int64_t) * (size_t)c->n);
+        for (int k = 0; k < c->n; ++k) {
+            sq[k] = c->seeds[k].qbeg; sl[k] = c->seeds[k].len;
+            ss[k] = c->seeds[k].score; sr[k] = c->seeds[k].rbeg;
+        }
+        int64_t *rows = (int64_t *)malloc(sizeof(int64_t) * 10 * (size_t)c->n);
+        int nreg = or_extend_chain_ex(c->n, sq, sl, sr, ss, e->reads + e->read_off[r],
+                                      e->read_len[r], e->fwd, e->gl, e->gl, &e->sc, rows, c->n,
+                                      NULL, NULL, log_call, e);
+        add_regions(e, ci, rows, nreg);
+        total += nreg;
+        free(sq); free(sr); free(rows);
+    }
+    return total;
+}
+
+void or_e2e_set_regions(or_e2e_t *e, const int32_t *n_per_chain, const int64_t *rows)
+{
+    for (int i = 0; i < e->n_reads; ++i) e->regs[i].n = 0;
+    int64_t o = 0;
+    for (int ci = 0; ci < e->chains.n; ++ci) {
+        add_regions(e, ci, rows + o * 10, n_per_chain[ci]);
+        o += n_per_chain[ci];
+    }
+}
+
+int64_t or_e2e_n_calls(const or_e2e_t *e, int64_t *q_bytes, int64_t *t_bytes)
+{
+    if (q_bytes) *q_bytes = e->lq_n;
+    if (t_bytes) *t_bytes = e->lt_n;
+    return e->n_calls;
+}
+
+void or_e2e_get_calls(const or_e2e_t *e, uint8_t *q, int64_t *q_off, int32_t *q_len,
+                      uint8_t *t, int64_t *t_off, int32_t *t_len, int32_t *h0, int32_t *w,
+                      int32_t *end_bonus, int32_t *out6)
+{
+    if (e->lq_n) memcpy(q, e->lq, (size_t)e->lq_n);
+    if (e->lt_n) memcpy(t, e->lt, (size_t)e->lt_n);
+    for (int64_t i = 0; i < e->n_calls; ++i) {
+        const call_t *c = &e->calls[i];
+        q_off[i] = c->q_off; q_len[i] = c->q_len; t_off[i] = c->t_off; t_len[i] = c->t_len;
+        h0[i] = c->h0; w[i] = c->w; end_bonus[i] = c->eb;
+        memcpy(out6 + i * 6, &c->r, sizeof(or_swres_t));
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* phases 4.5-5: RegionDedup.sortDedupPatch, ChainFilter.markSecondary     */
+/* ---------------------------------------------------------------------- */
+
+/* FMIndex.getReference, array form (FMIndex.swift:52-59) */
+static int get_reference(const or_e2e_t *e, int64_t pos, int len, uint8_t *buf)
+{
+    const int64_t gl = e->gl;
+    if (pos >= gl) {
+        int64_t fe = 2 * gl - pos, fs = fe - len;
+        int64_t s0 = fs > 0 ? fs : 0, e0 = fe < gl ? fe : gl;
+        int n = e0 > s0 ? (int)(e0 - s0) : 0;
+        for (int k = 0; k < n; ++k) buf[k] = (uint8_t)(3 - e->fwd[e0 - 1 - k]);
+        return n;
+    }
+    int64_t n = (int64_t)len < gl - pos ? (int64_t)len : gl - pos;
+    if (n <= 0 || pos < 0) return 0;
+    memcpy(buf, e->fwd + pos, (size_t)n);
+    return (int)n;
+}
+
+/* RegionDedup.patchRegion */
+static int32_t patch_region(const or_e2e_t *e, const region_t *a, const region_t *b,
+                            const uint8_t *query, int32_t w)
+{
+    const int64_t gl = e->gl;
+    if ((a->rb >= gl) != (b->rb >= gl)) return 0;
+    if (!(a->qb < b->qb && a->qe < b->qe && a->re < b->re)) return 0;
+    int64_t ref_gap = b->rb - a->re;
+    int64_t q_gap = (int64_t)b->qb - (int64_t)a->qe;
+    int32_t bw = abs((int32_t)(ref_gap - q_gap));
+    float rel;
+    if (a->re > b->rb) rel = (float)bw / (float)(a->re - b->rb);
+    else rel = (float)bw / (float)(b->rb - a->re);
+    if (a->re <= b->rb) {
+        if (bw > w << 1 || rel >= 0.05f) return 0;
+    } else {
+        if (bw > w << 2 || rel >= 0.05f * 2) return 0;
+    }
+    int32_t mqb = a->qb, mqe = b->qe;
+    int64_t mrb = a->rb, mre = b->re;
+    if (!(mqe > mqb && mre > mrb)) return 0;
+    int qn = mqe - mqb;
+    int rl = (int)(mre - mrb);
+    uint8_t *rbuf = (uint8_t *)malloc((size_t)rl + 1);
+    int rn = get_reference(e, mrb, rl, rbuf);
+    if (qn <= 0 || rn <= 0) { free(rbuf); return 0; }
+    int32_t mw = w > bw ? w : bw;
+    int32_t score = 0;
+    int64_t cells = 0;
+    or_ksw_global(qn, query + mqb, rn, rbuf, &e->sc, mw, &score, NULL, 0, &cells);
+    free(rbuf);
+    int32_t ql = mqe - mqb, rlen = (int32_t)(mre - mrb);
+    int32_t d = abs(ql - rlen);
+    int32_t ex_q = e->sc.a * ql - e->sc.o_ins - e->sc.e_ins * d;
+    int32_t ex_r = e->sc.a * rlen - e->sc.o_ins - e->sc.e_ins * d;
+    int32_t ex = ex_q > ex_r ? ex_q : ex_r;
+    if (ex <= 0) return 0;
+    if ((float)score / (float)ex >= 0.90f) return score;
+    return 0;
+}
+
+static int reg_by_rid_re(const void *a, const void *b, void *ctx)
+{
+    const region_t *x = (const region_t *)a, *y = (const region_t *)b;
+    if (x->rid != y->rid) return x->rid < y->rid;
+    return x->re < y->re;
+}
+
+static int reg_by_score(const void *a, const void *b, void *ctx)
+{
+    return ((const region_t *)a)->score > ((const region_t *)b)->score;
+}
+
+static void remove_deleted(region_vec *rv)
+{
+    int m = 0;
+    for (int i = 0; i < rv->n; ++i)
+        if (rv->v[i].qe != rv->v[i].qb) rv->v[m++] = rv->v[i];
+    rv->n = m;
+}
+
+/* RegionDedup.sortDedupPatch */
+static void sort_dedup_patch(const or_e2e_t *e, region_vec *rv, const uint8_t *query)
+{
+    if (rv->n <= 1) return;
+    region_t *R = rv->v;
+    stable_sort(R, (size_t)rv->n, sizeof(region_t), reg_by_rid_re, NULL);
+    const int n = rv->n;
+    for (int i = 0; i < n; ++i) {
+        if (!(R[i].qe > R[i].qb)) continue;
+        for (int j = i - 1; j >= 0; --j) {
+            if (R[j].rid != R[i].rid) break;
+            if (!(R[j].rb < R[i].re + (int64_t)MAX_CHAIN_GAP)) continue;
+            if (!(R[j].qe > R[j].qb)) continue;
+            int64_t rob = R[i].rb > R[j].rb ? R[i].rb : R[j].rb;
+            int64_t roe = R[i].re < R[j].re ? R[i].re : R[j].re;
+            int32_t qob = R[i].qb > R[j].qb ? R[i].qb : R[j].qb;
+            int32_t qoe = R[i].qe < R[j].qe ? R[i].qe : R[j].qe;
+            if (rob < roe && qob < qoe) {
+                float ro = (float)(roe - rob), qo = (float)(qoe - qob);
+                int64_t rli = R[i].re - R[i].rb, rlj = R[j].re - R[j].rb;
+                int32_t qli = R[i].qe - R[i].qb, qlj = R[j].qe - R[j].qb;
+                float mr = (float)(rli < rlj ? rli : rlj), mq = (float)(qli < qlj ? qli : qlj);
+                if (mr > 0 && mq > 0 && ro > MASK_LEVEL_REDUN * mr && qo > MASK_LEVEL_REDUN * mq) {
+                    if (R[i].score < R[j].score) { R[i].qe = R[i].qb; break; }
+                    R[j].qe = R[j].qb;
+                    continue;
+                }
+            }
+            int p, qx;
+            if (R[j].rb < R[i].rb) { p = i; qx = j; } else { p = j; qx = i; }
+            if (R[qx].qb < R[p].qb) {
+                int32_t ww = R[qx].w > R[p].w ? R[qx].w : R[p].w;
+                int32_t ms = patch_region(e, &R[qx], &R[p], query, ww);
+                if (ms > 0) {
+                    R[p].rb = R[p].rb < R[qx].rb ? R[p].rb : R[qx].rb;
+                    R[p].re = R[p].re > R[qx].re ? R[p].re : R[qx].re;
+                    R[p].qb = R[p].qb < R[qx].qb ? R[p].qb : R[qx].qb;
+                    R[p].qe = R[p].qe > R[qx].qe ? R[p].qe : R[qx].qe;
+                    R[p].score = ms;
+                    R[p].true_score = ms;
+                    R[p].w = R[p].w > R[qx].w ? R[p].w : R[qx].w;
+                    R[qx].qe = R[qx].qb;
+                    if (qx == i) break;
+                }
+            }
+        }
+    }
+    remove_deleted(rv);
+    stable_sort(rv->v, (size_t)rv->n, sizeof(region_t), reg_by_score, NULL);
+    if (rv->n > 1) {
+        R = rv->v;
+        for (int i = 1; i < rv->n; ++i)
+            if (R[i].score == R[i - 1].score && R[i].rb == R[i - 1].rb && R[i].qb == R[i - 1].qb)
+                R[i].qe = R[i].qb;
+        remove_deleted(rv);
+    }
+}
+
+/* hash64 (AlignmentRegion.swift:3-14), wrapping 64-bit arithmetic */
+static uint64_t hash64(uint64_t k)
+{
+    k += ~(k << 32);
+    k ^= (k >> 22);
+    k += ~(k << 13);
+    k ^= (k >> 8);
+    k += (k << 3);
+    k ^= (k >> 15);
+    k += ~(k << 27);
+    k ^= (k >> 31);
+    return k;
+}
+
+static int reg_by_score_hash(const void *a, const void *b, void *ctx)
+{
+    const region_t *x = (const region_t *)a, *y = (const region_t *)b;
+    if (x->score != y->score) return x->score > y->score;
+    return x->hash < y->hash;
+}
+
+/* ChainFilter.markSecondary + markSecondaryCore (no ALT regions) */
+static void mark_secondary(const or_e2e_t *e, region_vec *rv, uint64_t read_id)
+{
+    if (rv->n <= 1) return;
+    region_t *R = rv->v;
+    for (int i = 0; i < rv->n; ++i) {
+        R[i].sub = 0; R[i].sub_n = 0; R[i].secondary = -1;
+        R[i].hash = hash64(read_id + (uint64_t)i);
+    }
+    stable_sort(R, (size_t)rv->n, sizeof(region_t), reg_by_score_hash, NULL);
+    const or_scoring_t *sc = &e->sc;
+    int32_t tmp = sc->a + sc->b;
+    int32_t g1 = sc->o_ins + sc->e_ins, g2 = sc->o_del + sc->e_del;
+    if (g1 > tmp) tmp = g1;
+    if (g2 > tmp) tmp = g2;
+    int *pri = (int *)malloc(sizeof(int) * (size_t)rv->n);
+    int np = 0;
+    pri[np++] = 0;
+    for (int i = 1; i < rv->n; ++i) {
+        int found = 0;
+        for (int x = 0; x < np; ++x) {
+            int pi = pri[x];
+            int32_t ob = R[pi].qb > R[i].qb ? R[pi].qb : R[i].qb;
+            int32_t oe = R[pi].qe < R[i].qe ? R[pi].qe : R[i].qe;
+            if (ob < oe) {
+                float ov = (float)(oe - ob);
+                int32_t lp = R[pi].qe - R[pi].qb, li = R[i].qe - R[i].qb;
+                float ml = (float)(lp < li ? lp : li);
+                if (ml > 0 && ov / ml > MASK_LEVEL) {
+                    if (R[pi].sub == 0) R[pi].sub = R[i].score;
+                    if (R[pi].score - R[i].score <= tmp) R[pi].sub_n += 1;   /* no ALT */
+                    R[i].secondary = pi;
+                    found = 1;
+                    break;
+                }
+            }
+        }
+        if (!found) pri[np++] = i;
+    }
+    free(pri);
+}
+
+/* generateCIGAR's segment preparation (BWAMemAligner.swift:841-889) */
+static int64_t add_cigar_job(or_e2e_t *e, int r, const region_t *g)
+{
+    const int64_t gl = e->gl;
+    const uint8_t *read = e->reads + e->read_off[r];
+    const int rl = e->read_len[r];
+    const int rev = g->rb >= gl;
+    cjob_t j;
+    memset(&j, 0, sizeof j);
+    j.read = r;
+    j.q_len = g->qe - g->qb;
+    e->cq = (uint8_t *)grow(e->cq, &e->cq_cap, e->cq_n + j.q_len + 1, 1);
+    j.q_off = e->cq_n;
+    if (rev) {
+        int rq0 = rl - g->qe;                                 /* rc(read)[rl-qe ..< rl-qb] */
+        for (int k = 0; k < j.q_len; ++k) {
+            uint8_t b = read[rl - 1 - (rq0 + k)];
+            e->cq[e->cq_n + k] = b < 4 ? (uint8_t)(3 - b) : 4;
+        }
+    } else {
+        memcpy(e->cq + e->cq_n, read + g->qb, (size_t)j.q_len);
+    }
+    e->cq_n += j.q_len;
+    int64_t rb = g->rb, re = g->re;
+    if (rev) { int64_t frb = 2 * gl - re, fre = 2 * gl - rb; rb = frb; re = fre; }
+    if (re > gl) re = gl;                                      /* chromosome end (one sequence) */
+    int64_t ref_len = re - rb;
+    int64_t safe = ref_len < gl - rb ? ref_len : gl - rb;
+    j.r_off = e->cr_n;
+    if (safe > 0 && rb >= 0) {
+        e->cr = (uint8_t *)grow(e->cr, &e->cr_cap, e->cr_n + safe + 1, 1);
+        memcpy(e->cr + e->cr_n, e->fwd + rb, (size_t)safe);
+        j.r_len = (int32_t)safe;
+        e->cr_n += safe;
+    }
+    j.qb = g->qb; j.qe = g->qe; j.read_len = rl; j.rev = (uint8_t)rev;
+    j.true_score = g->true_score; j.w = g->w; j.ref_pos = rb;
+    e->jobs = (cjob_t *)grow(e->jobs, &e->jobs_cap, e->n_jobs + 1, sizeof(cjob_t));
+    e->jobs[e->n_jobs] = j;
+    return e->n_jobs++;
+}
+
+int64_t or_e2e_finish(or_e2e_t *e)
+{
+    for (int64_t j = 0; j < e->n_jobs; ++j) { free(e->jobs[j].cigar); free(e->jobs[j].md); }
+    e->n_jobs = 0; e->cq_n = e->cr_n = 0;
+    for (int r = 0; r < e->n_reads; ++r) {
+        region_vec *rv = &e->regs[r];
+        if (rv->n == 0) continue;
+        const uint8_t *q = e->reads + e->read_off[r];
+        sort_dedup_patch(e, rv, q);                            /* BWAMemAligner.swift:402-409 */
+        mark_secondary(e, rv, (uint64_t)r);                    /* :412-421, readId = batch index */
+        /* generateFilteredCIGARs (:434-476), default flags (no -a) */
+        region_t *R = rv->v;
+        int sec = 0;
+        for (int i = 0; i < rv->n; ++i) {
+            if (i > 0 && R[i].score < MIN_OUT_SCORE) continue;
+            if (R[i].secondary >= 0) {
+                int pi = R[i].secondary;
+                if (pi < rv->n && R[i].score < R[pi].score / 2) continue;
+                sec++;
+            }
+        }
+        const int skip_all = sec > MAX_XA_HITS;
+        for (int i = 0; i < rv->n; ++i) {
+            R[i].cig = -1;
+            if (i > 0 && R[i].score < MIN_OUT_SCORE) continue;
+            if (R[i].secondary >= 0) {
+                int pi = R[i].secondary;
+                if (pi < rv->n && R[i].score < R[pi].score / 2) continue;
+                if (i > 0 && skip_all) continue;
+            }
+            R[i].cig = add_cigar_job(e, r, &R[i]);
+        }
+    }
+    return e->n_jobs;
+}
+
+void or_e2e_cigar_sizes(const or_e2e_t *e, int64_t *q_bytes, int64_t *r_bytes)
+{
+    *q_bytes = e->cq_n;
+    *r_bytes = e->cr_n;
+}
+
+void or_e2e_get_cigar_jobs(const or_e2e_t *e, uint8_t *qseg, int64_t *q_off, int32_t *q_len,
+                           uint8_t *rseg, int64_t *r_off, int32_t *r_len, int32_t *qb,
+                           int32_t *qe, int32_t *read_len, uint8_t *rev, int32_t *true_score,
+                           int32_t *w, int64_t *ref_pos)
+{
+    if (e->cq_n) memcpy(qseg, e->cq, (size_t)e->cq_n);
+    if (e->cr_n) memcpy(rseg, e->cr, (size_t)e->cr_n);
+    for (int64_t i = 0; i < e->n_jobs; ++i) {
+        const cjob_t *j = &e->jobs[i];
+        q_off[i] = j->q_off; q_len[i] = j->q_len; r_off[i] = j->r_off; r_len[i] = j->r_len;
+        qb[i] = j->qb; qe[i] = j->qe; read_len[i] = j->read_len; rev[i] = j->rev;
+        true_score[i] = j->true_score; w[i] = j->w; ref_pos[i] = j->ref_pos;
+    }
+}
+
+void or_e2e_set_cigar(or_e2e_t *e, int64_t jx, const uint32_t *cigar, int n_cigar, int32_t nm,
+                      const char *md, int64_t pos)
+{
+    cjob_t *j = &e->jobs[jx];
+    free(j->cigar); free(j->md);
+    j->cigar = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n_cigar ? n_cigar : 1));
+    memcpy(j->cigar, cigar, sizeof(uint32_t) * (size_t)n_cigar);
+    j->n_cigar = n_cigar;
+    j->nm = nm;
+    j->md = strdup(md ? md : "");
+    j->pos = pos;
+    j->done = 1;
+}
+
+void or_e2e_cigar_cpu(or_e2e_t *e)
+{
+    enum { CAP = 4096, MDCAP = 8192 };
+    uint32_t *cig = (uint32_t *)malloc(sizeof(uint32_t) * CAP);
+    char *md = (char *)malloc(MDCAP);
+    for (int64_t i = 0; i < e->n_jobs; ++i) {
+        const cjob_t *j = &e->jobs[i];
+        or_cigar_res_t res;
+        int n = or_cigar_generate(e->cq + j->q_off, j->q_len, e->cr + j->r_off, j->r_len, j->qb,
+                                  j->qe, j->read_len, j->rev, j->true_score, j->w, &e->sc,
+                                  j->ref_pos, cig, CAP, md, MDCAP, &res);
+        or_e2e_set_cigar(e, i, cig, n < CAP ? n : CAP, res.nm, md, res.pos);
+    }
+    free(cig); free(md);
+}
+
+/* ---------------------------------------------------------------------- */
+/* classification, MAPQ and SAM text                                       */
+/* ---------------------------------------------------------------------- */
+
+/* MappingQuality.compute */
+static int mapq_of(const or_e2e_t *e, const region_t *g)
+{
+    if (g->score <= 0) return 0;
+    const or_scoring_t *sc = &e->sc;
+    int32_t sub = g->sub > 0 ? g->sub : sc->min_seed_len * sc->a;
+    if (g->csub > sub) sub = g->csub;
+    if (sub >= g->score) return 0;
+    int32_t ql = g->qe - g->qb, rl = (int32_t)(g->re - g->rb);
+    int32_t l = ql > rl ? ql : rl;
+    double a = sc->a, b = sc->b;
+    double identity = 1.0 - ((double)l * a - (double)g->score) / (a + b) / (double)l;
+    int32_t mapq;
+    {
+        double dl = l;
+        double tmp = dl < 50.0 ? 1.0 : 3.0 / log(dl);
+        tmp *= identity * identity;
+        mapq = (int32_t)(6.02 * (double)(g->score - sub) / a * tmp * tmp + 0.499);
+    }
+    if (g->sub_n > 0) mapq -= (int32_t)(4.343 * log((double)(g->sub_n + 1)) + 0.499);
+    if (mapq > 60) mapq = 60;
+    if (mapq < 0) mapq = 0;
+    mapq = (int32_t)((double)mapq * (1.0 - (double)g->frac_rep) + 0.499);
+    if (mapq < 0) mapq = 0;
+    if (mapq > 255) mapq = 255;
+    return mapq;
+}
+
+typedef struct { char *s; int64_t n, cap; } sbuf_t;
+
+static void sb_put(sbuf_t *b, const char *s, int64_t n)
+{
+    if (b->n + n + 1 > b->cap) {
+        int64_t c = b->cap ? b->cap : 4096;
+        while (c < b->n + n + 1) c *= 2;
+        b->s = (char *)realloc(b->s, (size_t)c);
+        b->cap = c;
+    }
+    memcpy(b->s + b->n, s, (size_t)n);
+    b->n += n;
+    b->s[b->n] = 0;
+}
+
+static void sb_printf(sbuf_t *b, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+static void sb_printf(sbuf_t *b, const char *fmt, ...)
+{
+    char tmp[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    int n = vsnprintf(tmp, sizeof tmp, fmt, ap);
+    va_end(ap);
+    if (n < (int)sizeof tmp) { sb_put(b, tmp, n); return; }
+    char *big = (char *)malloc((size_t)n + 1);
+    va_start(ap, fmt);
+    vsnprintf(big, (size_t)n + 1, fmt, ap);
+    va_end(ap);
+    sb_put(b, big, n);
+    free(big);
+}
+
+static void cigar_str(sbuf_t *b, const uint32_t *c, int n, int hard)
+{
+    static const char OPS[] = "MIDNSHP=X";
+    if (n == 0) { sb_put(b, "*", 1); return; }
+    for (int k = 0; k < n; ++k) {
+        uint32_t op = c[k] & 0xF, len = c[k] >> 4;
+        if (hard && op == 4 && (k == 0 || (k == n - 1 && n > 1))) op = 5;   /* convertToHardClip */
+        sb_printf(b, "%u%c", len, op < 9 ? OPS[op] : '?');
+    }
+}
+
+static const char BASES[] = "ACGTN";
+
+static void seq_qual(sbuf_t *b, const uint8_t *read, const uint8_t *qual, int rl, int rev,
+                     int t0, int t1)
+{
+    for (int k = t0; k < rl - t1; ++k) {
+        uint8_t x = rev ? read[rl - 1 - k] : read[k];
+        char c = rev ? (x < 4 ? BASES[3 - x] : 'N') : BASES[x < 4 ? x : 4];
+        sb_put(b, &c, 1);
+    }
+    sb_put(b, "\t", 1);
+    for (int k = t0; k < rl - t1; ++k) {
+        char c = (char)(rev ? qual[rl - 1 - k] : qual[k]);
+        sb_put(b, &c, 1);
+    }
+}
+
+typedef struct {
+    int reg;
+    const cjob_t *cj;
+    int mapq, primary, supp, secondary;
+    int64_t local_pos;
+} seg_t;
+
+int64_t or_e2e_sam(const or_e2e_t *e, const char *names, const uint8_t *quals, char *out,
+                   int64_t cap)
+{
+    sbuf_t b = {0};
+    const char *nm = names;
+    for (int r = 0; r < e->n_reads; ++r) {
+        const char *name = nm;
+        nm += strlen(nm) + 1;
+        const uint8_t *read = e->reads + e->read_off[r];
+        const uint8_t *qual = quals + e->read_off[r];
+        const int rl = e->read_len[r];
+        const region_vec *rv = &e->regs[r];
+        /* classifyAlignments (BWAMemAligner.swift:923-1099), SE, flag 0 */
+        seg_t *segs = (seg_t *)malloc(sizeof(seg_t) * (size_t)(rv->n + 1));
+        int ns = 0, nonsec = 0;
+        const region_t *xa_src[64];
+        int nxa = 0, xa_over = 0;
+        for (int i = 0; i < rv->n; ++i) {
+            const region_t *g = &rv->v[i];
+            if (g->score < MIN_OUT_SCORE) continue;
+            const cjob_t *cj = g->cig >= 0 ? &e->jobs[g->cig] : NULL;
+            if (g->secondary >= 0) {
+                int pi = g->secondary;
+                if (pi < rv->n && g->score < rv->v[pi].score / 2) continue;
+                if (nxa < 64) xa_src[nxa++] = g; else xa_over = 1;
+                continue;
+            }
+            seg_t s;
+            s.reg = i; s.cj = cj; s.mapq = mapq_of(e, g);
+            s.primary = nonsec == 0; s.supp = nonsec > 0; s.secondary = 0;
+            s.local_pos = cj ? cj->pos : 0;                    /* one sequence: offset 0 */
+            segs[ns++] = s;
+            nonsec++;
+        }
+        if (ns == 0) {                                         /* buildUnmappedRecord */
+            sb_printf(&b, "%s\t4\t*\t0\t0\t*\t*\t0\t0\t", name);
+            seq_qual(&b, read, qual, rl, 0, 0, 0);
+            sb_put(&b, "\n", 1);
+            free(segs);
+            continue;
+        }
+        for (int k = 1; k < ns; ++k)                           /* supplementary MAPQ cap */
+            if (segs[k].mapq > segs[0].mapq) segs[k].mapq = segs[0].mapq;
+        /* XA (buildXATag), omitted when more than maxXAHits secondaries */
+        sbuf_t xa = {0};
+        int have_xa = 0;
+        if (!xa_over && nxa <= MAX_XA_HITS) {
+            have_xa = 1;
+            sb_put(&xa, "", 0);
+            for (int k = 0; k < nxa; ++k) {
+                const region_t *g = xa_src[k];
+                const cjob_t *cj = g->cig >= 0 ? &e->jobs[g->cig] : NULL;
+                sb_printf(&xa, "%s,%c%lld,", e->name, g->rb >= e->gl ? '-' : '+',
+                          (long long)((cj ? cj->pos : 0) + 1));
+                if (cj) cigar_str(&xa, cj->cigar, cj->n_cigar, 0);
+                else sb_put(&xa, "*", 1);
+                sb_printf(&xa, ",%d;", cj ? cj->nm : 0);
+            }
+        }
+        for (int k = 0; k < ns; ++k) {
+            const seg_t *s = &segs[k];
+            const region_t *g = &rv->v[s->reg];
+            const cjob_t *cj = s->cj;
+            const int rev = g->rb >= e->gl;
+            int flag = rev ? 16 : 0;
+            if (s->supp) flag |= 0x800;
+            else if (!s->primary) flag |= 0x100;
+            int t0 = 0, t1 = 0;
+            if (s->supp && cj && cj->n_cigar) {                /* hard clips for supplementary */
+                if ((cj->cigar[0] & 0xF) == 4) t0 = (int)(cj->cigar[0] >> 4);
+                if (cj->n_cigar > 1 && (cj->cigar[cj->n_cigar - 1] & 0xF) == 4)
+                    t1 = (int)(cj->cigar[cj->n_cigar - 1] >> 4);
+                if (!(t0 < rl - t1)) { t0 = 0; t1 = 0; }
+            }
+            sb_printf(&b, "%s\t%d\t%s\t%lld\t%d\t", name, flag, e->name,
+                      (long long)(s->local_pos + 1), s->mapq);
+            if (cj) cigar_str(&b, cj->cigar, cj->n_cigar, s->supp);
+            else sb_put(&b, "*", 1);
+            sb_put(&b, "\t*\t0\t0\t", 7);
+            seq_qual(&b, read, qual, rl, rev, t0, t1);
+            int32_t xs = g->sub > g->csub ? g->sub : g->csub;
+            sb_printf(&b, "\tAS:i:%d\tXS:i:%d\tNM:i:%d\tMD:Z:%s", g->score, xs, cj ? cj->nm : 0,
+                      cj && cj->md ? cj->md : "");
+            if (ns > 1) {                                      /* buildSATag */
+                sb_put(&b, "\tSA:Z:", 6);
+                for (int x = 0; x < ns; ++x) {
+                    if (x == k) continue;
+                    const seg_t *o = &segs[x];
+                    const region_t *og = &rv->v[o->reg];
+                    sb_printf(&b, "%s,%lld,%c,", e->name, (long long)(o->local_pos + 1),
+                              og->rb >= e->gl ? '-' : '+');
+                    if (o->cj) cigar_str(&b, o->cj->cigar, o->cj->n_cigar, 0);
+                    else sb_put(&b, "*", 1);
+                    sb_printf(&b, ",%d,%d;", o->mapq, o->cj ? o->cj->nm : 0);
+                }
+            }
+            /* buildXATag returns "" without secondaries, and buildRecord writes
+             * any non-nil tag (SAMOutputBuilder.swift:262-264) */
+            if (s->primary && have_xa) sb_printf(&b, "\tXA:Z:%s", xa.s ? xa.s : "");
+            sb_put(&b, "\n", 1);
+        }
+        free(xa.s);
+        free(segs);
+    }
+    int64_t n = b.n;
+    if (out && cap > 0) {
+        int64_t m = n < cap - 1 ? n : cap - 1;
+        if (b.s) memcpy(out, b.s, (size_t)m);
+        out[m] = 0;
+    }
+    free(b.s);
+    return n;
+}

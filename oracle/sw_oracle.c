@@ -780,10 +780,27 @@ static int cal_max_gap(int qlen, const or_scoring_t *sc)       /* :352-363 */
     return l < cap ? l : cap;
 }
 
-/* getReference(pos, len) of the reference tests (AlignmentTests.swift:806-816):
- * start = max(0,pos), end = min(count, start+len). */
-static int get_ref(const uint8_t *ref, int64_t ref_len, int64_t pos, int len, uint8_t *buf)
+/* Two getReference semantics.  gl == 0: the reference tests' closure
+ * (AlignmentTests.swift:806-816), start = max(0,pos), end = min(count,
+ * start+len) over a flat array.  gl > 0: FMIndex.getReference
+ * (FMIndex.swift:25-50) over the forward genome `ref` of length gl, where
+ * positions >= gl address the reverse complement: a window is clipped at the
+ * strand boundary and at both genome ends. */
+static int get_ref(const uint8_t *ref, int64_t ref_len, int64_t gl, int64_t pos, int len, uint8_t *buf)
 {
+    if (gl > 0) {
+        if (pos >= gl) {
+            int64_t fwd_end = 2 * gl - pos, fwd_start = fwd_end - len;
+            int64_t s0 = fwd_start > 0 ? fwd_start : 0, e0 = fwd_end < gl ? fwd_end : gl;
+            int n = e0 > s0 ? (int)(e0 - s0) : 0;
+            for (int k = 0; k < n; ++k) buf[k] = (uint8_t)(3 - ref[e0 - 1 - k]);
+            return n;
+        }
+        int64_t n64 = (int64_t)len < gl - pos ? (int64_t)len : gl - pos;
+        if (n64 <= 0 || pos < 0) return 0;
+        memcpy(buf, ref + pos, (size_t)n64);
+        return (int)n64;
+    }
     int64_t start = pos > 0 ? pos : 0;
     int64_t end = start + len < ref_len ? start + len : ref_len;
     int n = end > start ? (int)(end - start) : 0;
@@ -797,6 +814,16 @@ int or_extend_chain(int n_seeds, const int32_t *sq, const int32_t *sl, const int
                     const int32_t *ss, const uint8_t *read, int read_len,
                     const uint8_t *ref, int64_t ref_len, const or_scoring_t *sc,
                     int64_t *regions_out, int max_regions, int64_t *ext_calls, int64_t *cells)
+{
+    return or_extend_chain_ex(n_seeds, sq, sl, sr, ss, read, read_len, ref, ref_len, 0, sc,
+                              regions_out, max_regions, ext_calls, cells, NULL, NULL);
+}
+
+int or_extend_chain_ex(int n_seeds, const int32_t *sq, const int32_t *sl, const int64_t *sr,
+                       const int32_t *ss, const uint8_t *read, int read_len,
+                       const uint8_t *ref, int64_t ref_len, int64_t gl, const or_scoring_t *sc,
+                       int64_t *regions_out, int max_regions, int64_t *ext_calls, int64_t *cells,
+                       or_ext_log_fn log, void *log_ctx)
 {
     if (n_seeds <= 0) return 0;
     const int32_t band = sc->w;
@@ -850,13 +877,14 @@ int or_extend_chain(int n_seeds, const int32_t *sq, const int32_t *sl, const int
             int64_t tl64 = rbeg - rmax0 < 10000 ? rbeg - rmax0 : 10000;
             int tl = (int)tl64;
             if (tl > 0) {
-                int at = get_ref(ref, ref_len, rbeg - tl, tl, tbuf);
+                int at = get_ref(ref, ref_len, gl, rbeg - tl, tl, tbuf);
                 for (int lo = 0, hi = at - 1; lo < hi; ++lo, --hi) { uint8_t t = tbuf[lo]; tbuf[lo] = tbuf[hi]; tbuf[hi] = t; }
                 int32_t w = band, prev = sscore;
                 or_swres_t r = {0};
                 for (int tr = 0; tr < 2; ++tr) {
                     nc += or_ksw_extend(lq, qbuf, at, tbuf, sc, w, sscore, sc->pen_clip5, &r);
                     calls++;
+                    if (log) log(log_ctx, qbuf, lq, tbuf, at, sscore, w, sc->pen_clip5, &r);
                     if (r.score == prev || r.max_off < (w >> 1) + (w >> 2) || tr == 1) break;
                     prev = r.score;
                     w = w << 1;
@@ -877,12 +905,13 @@ int or_extend_chain(int n_seeds, const int32_t *sq, const int32_t *sl, const int
             int64_t tr64 = rmax1 - rend < 10000 ? rmax1 - rend : 10000;
             int tr_len = (int)tr64;
             if (tr_len > 0) {
-                int at = get_ref(ref, ref_len, rend, tr_len, tbuf);
+                int at = get_ref(ref, ref_len, gl, rend, tr_len, tbuf);
                 int32_t w = band, prev = acc_h0;
                 or_swres_t r = {0};
                 for (int tr = 0; tr < 2; ++tr) {
                     nc += or_ksw_extend(rq, qbuf, at, tbuf, sc, w, acc_h0, sc->pen_clip3, &r);
                     calls++;
+                    if (log) log(log_ctx, qbuf, rq, tbuf, at, acc_h0, w, sc->pen_clip3, &r);
                     if (r.score == prev || r.max_off < (w >> 1) + (w >> 2) || tr == 1) break;
                     prev = r.score;
                     w = w << 1;

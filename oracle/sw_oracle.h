@@ -109,6 +109,20 @@ int or_extend_chain(int n_seeds, const int32_t *seed_qbeg, const int32_t *seed_l
                     const or_scoring_t *sc, int64_t *regions_out, int max_regions,
                     int64_t *ext_calls, int64_t *cells);
 
+/* or_extend_chain with a choice of getReference semantics (gl > 0: the
+ * FMIndex form over the forward genome `ref` of length gl, positions >= gl on
+ * the reverse strand; gl == 0: the flat form above) and an optional sink that
+ * sees every BandedSWScalar.align call: query, target window, h0, w, end
+ * bonus and the result. */
+typedef void (*or_ext_log_fn)(void *ctx, const uint8_t *q, int qlen, const uint8_t *t, int tlen,
+                              int32_t h0, int32_t w, int32_t end_bonus, const or_swres_t *r);
+int or_extend_chain_ex(int n_seeds, const int32_t *seed_qbeg, const int32_t *seed_len,
+                       const int64_t *seed_rbeg, const int32_t *seed_score,
+                       const uint8_t *read, int read_len,
+                       const uint8_t *ref, int64_t ref_len, int64_t gl, const or_scoring_t *sc,
+                       int64_t *regions_out, int max_regions, int64_t *ext_calls, int64_t *cells,
+                       or_ext_log_fn log, void *log_ctx);
+
 /* Batch entry points used by the tests and the CPU baseline.  Sequences are
  * one byte per base (0..4) in one arena; offsets are byte offsets. */
 int64_t or_ksw_extend_batch(int n, const uint8_t *seqs, const uint64_t *q_off,

@@ -327,6 +327,37 @@ void wl_pack_pac(const uint8_t *ref, int64_t n, uint8_t *pac)
 
 void wl_free_bytes(uint8_t *p) { free(p); }
 
+/* Single-end reads of `len` bases sampled uniformly from both strands of ref
+ * (one sequence of gl bases): the source window starts at a uniform position,
+ * is mutated at p_sub / p_indel (the shared model), cut to `len` bases, and
+ * reverse-complemented for reverse-strand reads.  bases: n x len codes;
+ * pos: 0-based source start; rev: 1 for reverse strand. */
+int wl_se_reads(const uint8_t *ref, int64_t gl, uint64_t seed, int64_t n, int len,
+                double p_sub, double p_indel, uint8_t *bases, int64_t *pos, uint8_t *rev)
+{
+    const int slack = 64;
+    if (gl < len + slack || len <= 0) return -1;
+    rng_t r;
+    rng_seed(&r, seed * 0x9E3779B97F4A7C15ull + 17);
+    uint8_t *buf = (uint8_t *)malloc((size_t)(len + 2 * slack));
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t p;
+        int m;
+        do {
+            p = uni(&r, 0, gl - len - slack);
+            m = mutate(&r, ref + p, len + slack, p_sub, p_indel, buf, len + 2 * slack, NULL);
+        } while (m < len);
+        const int rv = (int)(next_u64(&r) & 1);
+        uint8_t *o = bases + i * len;
+        if (rv) for (int k = 0; k < len; ++k) o[k] = (uint8_t)(3 - buf[len - 1 - k]);
+        else memcpy(o, buf, (size_t)len);
+        pos[i] = p;
+        rev[i] = (uint8_t)rv;
+    }
+    free(buf);
+    return 0;
+}
+
 typedef struct {
     int64_t n;                       /* jobs */
     uint8_t *q; int64_t q_bytes;

@@ -79,3 +79,23 @@ def slice_jobs(js, idx):
     for name in ("h0", "w", "end_bonus", "true_score", "direction"):
         res[name] = js[name][idx].copy()
     return res
+
+
+def se_reads(ref, seed, n, length=150, p_sub=0.005, p_indel=0.0005):
+    """Single-end reads of `length` bases from both strands of `ref` (bytes
+    0..3): (bases n x length, 0-based source starts, reverse flags)."""
+    lib = _load()
+    if not hasattr(lib, "_se_bound"):
+        lib.wl_se_reads.restype = C.c_int
+        lib.wl_se_reads.argtypes = [C.c_void_p, C.c_int64, C.c_uint64, C.c_int64, C.c_int,
+                                    C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib._se_bound = True
+    ref = np.ascontiguousarray(ref, np.uint8)
+    bases = np.zeros((n, length), np.uint8)
+    pos = np.zeros(n, np.int64)
+    rev = np.zeros(n, np.uint8)
+    rc = lib.wl_se_reads(ref.ctypes.data, len(ref), int(seed), int(n), int(length), float(p_sub),
+                         float(p_indel), bases.ctypes.data, pos.ctypes.data, rev.ctypes.data)
+    if rc != 0:
+        raise ValueError("reference too short for the read length")
+    return bases, pos, rev
