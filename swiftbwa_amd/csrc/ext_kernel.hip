@@ -915,6 +915,43 @@ __device__ __forceinline__ void store_groups(uint32_t *col, uint32_t *gcol, int 
     }
 }
 
+// Groups g .. g + 3 (clamped to gmax) of one memory only: KIND 0 = LDS
+// (groups below GL), KIND 2 = the wave's HBM region (groups GL and up).  A
+// sweep phase that touches one memory only lets the compiler wait for exactly
+// the block it is about to use while the next block's reads stay in flight;
+// behind a per-block LDS/HBM branch it has to wait for every outstanding load
+// and store.
+template <int KIND, int GL>
+__device__ __forceinline__ void load_groups_k(const uint32_t *col, const uint32_t *gcol, int g,
+                                              int gmax, uint32_t (&wv)[16])
+{
+    static_assert(KIND == 0 || KIND == 2, "one memory per phase");
+    u32x4 v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int gq = min(g + q, gmax);
+        if (KIND == 0) v[q] = ((const lds_u4 *)col)[gq * 64];
+        else v[q] = ((const glb_u4 *)gcol)[(gq - GL) * 64];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        wv[4 * q] = v[q].x; wv[4 * q + 1] = v[q].y; wv[4 * q + 2] = v[q].z; wv[4 * q + 3] = v[q].w;
+    }
+}
+
+template <int KIND, int GL, int NG>
+__device__ __forceinline__ void store_groups_k(uint32_t *col, uint32_t *gcol, int g,
+                                               const uint32_t (&wv)[16])
+{
+    static_assert(KIND == 0 || KIND == 2, "one memory per phase");
+#pragma unroll
+    for (int q = 0; q < NG; ++q) {
+        const u32x4 v{wv[4 * q], wv[4 * q + 1], wv[4 * q + 2], wv[4 * q + 3]};
+        if (KIND == 0) ((lds_u4 *)col)[(g + q) * 64] = v;
+        else ((glb_u4 *)gcol)[(g + q - GL) * 64] = v;
+    }
+}
+
 __device__ __forceinline__ void pk_finish(const PkJob &J, const ExtDev &d)
 {
     if (!J.valid) return;                                      // :183-190
@@ -966,14 +1003,26 @@ __device__ __forceinline__ uint16_t *col16(uint32_t *col, uint32_t *gcol, int j,
     return j < LC ? l + cpos<uint16_t>(j) + hs : g + cpos<uint16_t>(j - LC) + hs;
 }
 
+// Stores the 16-bit entry of column j, half hs, with an LDS or a global store
+// chosen per lane.  A generic (flat) store would leave a flat access pending
+// into the next sweep, and with one pending the compiler waits for every LDS
+// access at each block.
+template <int LC>
+__device__ __forceinline__ void store16(uint32_t *col, uint32_t *gcol, int j, int hs, uint16_t v)
+{
+    typedef __attribute__((address_space(3))) uint16_t lds_u16;
+    typedef __attribute__((address_space(1))) uint16_t glb_u16;
+    if (LC == 0 || j < LC) ((lds_u16 *)col)[cpos<uint16_t>(j) + hs] = v;
+    else ((glb_u16 *)gcol)[cpos<uint16_t>(j - LC) + hs] = v;
+}
+
 // Group g (columns 4g..4g+3, both halves) of this lane's column state.
+// An LDS or a global read chosen per lane (no generic access: see store16).
 template <int GL>
 __device__ __forceinline__ u32x4 group_rd(const uint32_t *col, const uint32_t *gcol, int g)
 {
-    if (GL == 0) return *(const lds_u4 *)(col + g * 256);
-    const u32x4 *p = g < GL ? reinterpret_cast<const u32x4 *>(col + g * 256)
-                            : reinterpret_cast<const u32x4 *>(gcol + (g - GL) * 256);
-    return *p;
+    if (GL == 0 || g < GL) return *(const lds_u4 *)(col + g * 256);
+    return *(const glb_u4 *)(gcol + (g - GL) * 256);
 }
 
 // Non-zero columns of two groups: bit c (c < 4) = column c of va's low half
@@ -1000,7 +1049,7 @@ __device__ __forceinline__ void pk2_row_start(PkJob &J, int i, uint32_t *col, ui
     if (J.alive && J.beg < wb) {
         // beg = i - w (:96) drops column beg: with w >= 0, beg >= i - 1 - w
         // after the previous row, so exactly one column per row
-        *col16<LC>(col, gcol, J.beg, hs) = 0;
+        store16<LC>(col, gcol, J.beg, hs, 0);
         J.beg = wb;
     }
     J.end = min(J.end, i + J.w + 1);                           // :97-98 (end <= qlen holds)
@@ -1013,7 +1062,7 @@ template <int LC>
 __device__ __forceinline__ void pk2_row_end(PkJob &J, const ExtParams &p, int i, int h1, int key,
                                             uint32_t *col, uint32_t *gcol, int hs)
 {
-    *col16<LC>(col, gcol, J.end, hs) = (uint16_t)h1;          // h[end] = h1, e[end] = 0
+    store16<LC>(col, gcol, J.end, hs, (uint16_t)h1);          // h[end] = h1, e[end] = 0
     const int rm = key >> 8, rj = key & 0xFF;
     const bool at_q = J.alive && J.end == J.qlen;              // :149-152
     J.gtle = (at_q && !(J.gsc > h1)) ? i : J.gtle;
@@ -1105,7 +1154,6 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
     PkJob A, B;
     pk_setup(A, d, p, meta, lane, count);
     pk_setup(B, d, p, meta, lane + 64, count);
-    const ColRef<uint32_t, LC> cw{col, gcol};
     // Per 16-base query word, the "differs from base c" bit masks of every
     // c, held in registers for the whole item: N01 = ne0 | ne1 << 16 and
     // N23 = ne2 | ne3 << 16.  A row picks its target base's mask with one
@@ -1124,7 +1172,10 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
     for (int j = 0; j <= qmax; ++j) {                          // first row, :68-77
         const int ha = (A.alive && j <= A.qlen) ? (j == 0 ? A.h0 : max(0, A.h0 - p.o_ins - p.e_ins * j)) : 0;
         const int hb = (B.alive && j <= B.qlen) ? (j == 0 ? B.h0 : max(0, B.h0 - p.o_ins - p.e_ins * j)) : 0;
-        cw[j] = (uint32_t)ha | ((uint32_t)hb << 16);
+        const uint32_t v = (uint32_t)ha | ((uint32_t)hb << 16);
+        // j is wave-uniform: one LDS or one global store (no generic access)
+        if (!HYB || j < LC) ((__attribute__((address_space(3))) uint32_t *)col)[cpos<uint32_t>(j)] = v;
+        else ((__attribute__((address_space(1))) uint32_t *)gcol)[cpos<uint32_t>(j - LC)] = v;
     }
     const ss2 Ssc = ss2{(short)p.match_sc, (short)p.match_sc};
     const ss2 nAB = ss2{(short)(p.mis_sc - p.match_sc), (short)(p.mis_sc - p.match_sc)};
@@ -1197,9 +1248,10 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
         // max(diag, E) and the new E (:128-129); (2) the running F and h1
         // (:121-123, :130-131); (3) the row-maximum candidates H * 256 + k
         // (:124-125).  MASKED: commit only left of each half's band end.
-        auto block = [&](auto NC, auto MK, uint32_t (&wv)[16], int c0, uint32_t mm) {
+        auto block = [&](auto NC, auto MK, auto KD, uint32_t (&wv)[16], int c0, uint32_t mm) {
             constexpr int N = decltype(NC)::value;
             constexpr bool MASKED = decltype(MK)::value;
+            constexpr int KIND = decltype(KD)::value;   // 0 LDS, 2 HBM, 1 either (per group)
             us2 dgE[16], tf[16], ne[16], Hs[16], M[16], E[16], s[16];
 #pragma unroll
             for (int k = 0; k < N; ++k) {
@@ -1253,7 +1305,8 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
                     Hs[k] = u2(w32(H * C256 + splat(k)) & hm);
                 }
             }
-            store_groups<GL, N / 4>(col, gcol, c0 >> 2, wv);
+            if constexpr (KIND == 1) store_groups<GL, N / 4>(col, gcol, c0 >> 2, wv);
+            else store_groups_k<KIND, GL, N / 4>(col, gcol, c0 >> 2, wv);
 #pragma unroll
             for (int st = 8; st >= 1; st >>= 1)
 #pragma unroll
@@ -1264,16 +1317,22 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
         using N16 = std::integral_constant<int, 16>;
         using Unmasked = std::integral_constant<bool, false>;
         using Masked = std::integral_constant<bool, true>;
-        auto full = [&](uint32_t (&wv)[16], int c0, uint32_t mm) {
-            if (c0 + 16 <= mhi) block(N16{}, Unmasked{}, wv, c0, mm);   // left of every live band end
-            else block(N16{}, Masked{}, wv, c0, mm);
+        using KAny = std::integral_constant<int, 1>;
+        auto full = [&](auto KD, uint32_t (&wv)[16], int c0, uint32_t mm) {
+            if (c0 + 16 <= mhi) block(N16{}, Unmasked{}, KD, wv, c0, mm);   // left of every live band end
+            else block(N16{}, Masked{}, KD, wv, c0, mm);
+        };
+        // partial block of r = 1..3 groups (always at a band end or a memory boundary)
+        auto part = [&](auto KD, int r, uint32_t (&wv)[16], int c0, uint32_t mm) {
+            if (r == 1) block(std::integral_constant<int, 4>{}, Masked{}, KD, wv, c0, mm);
+            else if (r == 2) block(std::integral_constant<int, 8>{}, Masked{}, KD, wv, c0, mm);
+            else block(std::integral_constant<int, 12>{}, Masked{}, KD, wv, c0, mm);
         };
         if (jlo < jhi) {
-            const int g0 = jlo >> 2, gend = (jhi + 3) >> 2, glast = gend - 1;
-            const int nfull = (gend - g0) >> 2, tail = (gend - g0) & 3;
-            const int o = (g0 & 3) * 4;
-            const us2 osh = splat(o), ish = splat((16 - o) & 15);
-            const uint32_t ohi = o ? 0xFFFFFFFFu : 0u;
+            const int g0 = jlo >> 2, gend = (jhi + 3) >> 2;
+            int o = (g0 & 3) * 4;
+            us2 osh = splat(o), ish = splat((16 - o) & 15);
+            uint32_t ohi = o ? 0xFFFFFFFFu : 0u;
             int q = g0 >> 2;
             uint32_t mcur = qmask(q);
             // mismatch bits of the block at column 16 q + o; moves to the next word
@@ -1284,34 +1343,81 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
                 ++q;
                 return m;
             };
-            // two word sets in turn: the next block's reads are in flight
-            // while the current block computes
+            // Blocks of four groups from g to gstop in one memory: two word
+            // sets in turn, the next block's reads in flight while the current
+            // one computes (each read clamped to gstop - 1, so a read past the
+            // phase repeats its last group: same memory, unused); then the
+            // partial block of the remaining 1-3 groups, whose words the last
+            // trip has already read.  W0 holds the first block on entry.
+            auto phase = [&](auto KD, int &g, int gstop, uint32_t (&W0)[16], uint32_t (&W1)[16]) {
+                constexpr int KIND = decltype(KD)::value;
+                const int gl = gstop - 1;
+                const int nf = (gstop - g) >> 2, r = (gstop - g) & 3;
+                bool in_w1 = false;
+                for (int t = 0; t < nf;) {
+                    if constexpr (KIND == 1) load_groups<GL>(col, gcol, min(g + 4, gl), gl, W1);
+                    else load_groups_k<KIND, GL>(col, gcol, min(g + 4, gl), gl, W1);
+                    full(KD, W0, 4 * g, next_mm());
+                    g += 4;
+                    if (++t >= nf) {
+                        in_w1 = true;
+                        break;
+                    }
+                    if constexpr (KIND == 1) load_groups<GL>(col, gcol, min(g + 4, gl), gl, W0);
+                    else load_groups_k<KIND, GL>(col, gcol, min(g + 4, gl), gl, W0);
+                    full(KD, W1, 4 * g, next_mm());
+                    g += 4;
+                    ++t;
+                }
+                if (r) {
+                    if (in_w1) {
+#pragma unroll
+                        for (int k = 0; k < 16; ++k) W0[k] = W1[k];
+                    }
+                    part(KD, r, W0, 4 * g, next_mm());
+                    g += r;
+                }
+            };
             uint32_t W0[16], W1[16];
             int g = g0;
-            bool in_w1 = false;
-            load_groups<GL>(col, gcol, g, glast, W0);
-            for (int t = 0; t < nfull;) {
-                load_groups<GL>(col, gcol, min(g + 4, glast), glast, W1);
-                full(W0, 4 * g, next_mm());
-                g += 4;
-                if (++t >= nfull) {
-                    in_w1 = true;
-                    break;
+            if constexpr (!HYB) {
+                load_groups<GL>(col, gcol, g, gend - 1, W0);
+                phase(KAny{}, g, gend, W0, W1);
+            } else {
+                // LDS groups [g0, min(GL, gend)) first, then HBM groups
+                // [max(g0, GL), gend); the HBM phase's first block is read
+                // into W2 before the LDS phase so its latency hides there.
+                using KL = std::integral_constant<int, 0>;
+                using KH = std::integral_constant<int, 2>;
+                uint32_t W2[16];
+                const int gh = max(g0, GL);
+                // Retire the previous row's global accesses (its last HBM trip
+                // read a block it did not use) so that W2's reads are the only
+                // ones in flight; otherwise the LDS phase's first overwrite of
+                // such a register waits for W2 as well (vmcnt counts in order).
+                // The row-end narrowing has waited on its own reads already,
+                // so this wait is normally free.  0x0F70: vmcnt(0), others
+                // unchanged.
+                __builtin_amdgcn_s_waitcnt(0x0F70);
+                if (gend > GL) load_groups_k<2, GL>(col, gcol, gh, gend - 1, W2);
+                if (g0 < GL) {
+                    const int gs = min(GL, gend);
+                    load_groups_k<0, GL>(col, gcol, g, gs - 1, W0);
+                    phase(KL{}, g, gs, W0, W1);
                 }
-                load_groups<GL>(col, gcol, min(g + 4, glast), glast, W0);
-                full(W1, 4 * g, next_mm());
-                g += 4;
-                ++t;
-            }
-            if (tail) {
-                if (in_w1) {
+                if (gend > GL) {
+                    if (g0 < GL) {
+                        // g == GL: 16-column aligned, the mask offset is 0
+                        o = 0;
+                        osh = splat(0); ish = splat(0);
+                        ohi = 0u;
+                        q = GL >> 2;
+                        mcur = qmask(q);
+                    }
 #pragma unroll
-                    for (int k = 0; k < 16; ++k) W0[k] = W1[k];
+                    for (int k = 0; k < 16; ++k) W0[k] = W2[k];
+                    phase(KH{}, g, gend, W0, W1);
                 }
-                const uint32_t m = next_mm();
-                if (tail == 1) block(std::integral_constant<int, 4>{}, Masked{}, W0, 4 * g, m);
-                else if (tail == 2) block(std::integral_constant<int, 8>{}, Masked{}, W0, 4 * g, m);
-                else block(std::integral_constant<int, 12>{}, Masked{}, W0, 4 * g, m);
             }
         }
 
