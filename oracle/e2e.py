@@ -48,6 +48,17 @@ def _lib():
         "or_e2e_cigar_cpu": (None, [P]),
         "or_e2e_set_cigar": (None, [P, I64, P, C.c_int, I32, C.c_char_p, I64]),
         "or_e2e_sam": (I64, [P, P, P, P, I64]),
+        "or_e2e_pe_finish": (I64, [P, C.c_int]),
+        "or_e2e_pe_dist": (None, [P, P, P]),
+        "or_e2e_pe_rescue_jobs": (I64, [P, C.c_int]),
+        "or_e2e_pe_rescue_sizes": (None, [P, P, P]),
+        "or_e2e_pe_get_rescue_jobs": (None, [P] * 7),
+        "or_e2e_pe_rescue_cpu": (None, [P]),
+        "or_e2e_pe_set_rescue": (None, [P, P, P]),
+        "or_e2e_pe_apply": (I64, [P]),
+        "or_e2e_pe_remark": (None, [P]),
+        "or_e2e_pe_cigar_plan": (I64, [P]),
+        "or_e2e_pe_sam": (I64, [P, P, P, P, I64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -206,9 +217,16 @@ class E2E:
 
     def finish(self):
         """Phases 4.5-5; returns the CIGAR jobs as a list of dicts."""
-        nj = self.L.or_e2e_finish(self.h)
+        self.n_jobs = self.L.or_e2e_finish(self.h)
+        return self._jobs()
+
+    def cigar_cpu(self):
+        self.L.or_e2e_cigar_cpu(self.h)
+
+    def _jobs(self):
         qb_, rb_ = C.c_int64(), C.c_int64()
         self.L.or_e2e_cigar_sizes(self.h, C.byref(qb_), C.byref(rb_))
+        nj = self.n_jobs
         qs = np.zeros(max(qb_.value, 1), np.uint8)
         rs = np.zeros(max(rb_.value, 1), np.uint8)
         m = max(nj, 1)
@@ -226,8 +244,80 @@ class E2E:
                       "ref_pos": int(pos[j])} for j in range(nj)]
         return self.jobs
 
-    def cigar_cpu(self):
-        self.L.or_e2e_cigar_cpu(self.h)
+    # -- paired end -------------------------------------------------------
+    def seed_pairs(self, reads1, reads2):
+        """Phases 1-3 of pairs: read 1 of pair i is read i, read 2 is n + i."""
+        assert len(reads1) == len(reads2)
+        self.n_pairs = len(reads1)
+        return self.seed(list(reads1) + list(reads2))
+
+    def pe_finish(self):
+        """Phases 4.5-5 and the insert-size distribution; True when rescue runs."""
+        rc = self.L.or_e2e_pe_finish(self.h, self.n_pairs)
+        assert rc >= 0
+        return bool(rc)
+
+    def pe_dist(self):
+        out = np.zeros((4, 8), np.float64)
+        pri = C.c_int32()
+        self.L.or_e2e_pe_dist(self.h, out.ctypes.data, C.byref(pri))
+        return out, pri.value
+
+    def rescue_jobs(self, direction):
+        """Local-SW jobs of one rescue direction as a job dict (queries = mate
+        sequences, targets = reference windows)."""
+        n = self.L.or_e2e_pe_rescue_jobs(self.h, int(direction))
+        qb_, tb_ = C.c_int64(), C.c_int64()
+        self.L.or_e2e_pe_rescue_sizes(self.h, C.byref(qb_), C.byref(tb_))
+        q = np.zeros(max(qb_.value, 1), np.uint8)
+        t = np.zeros(max(tb_.value, 1), np.uint8)
+        qo, to = np.zeros(max(n, 1), np.int64), np.zeros(max(n, 1), np.int64)
+        ql, tl = np.zeros(max(n, 1), np.int32), np.zeros(max(n, 1), np.int32)
+        self.L.or_e2e_pe_get_rescue_jobs(self.h, q.ctypes.data, qo.ctypes.data, ql.ctypes.data,
+                                         t.ctypes.data, to.ctypes.data, tl.ctypes.data)
+        return {"n": int(n), "queries": q, "q_off": qo[:n].astype(np.uint32),
+                "q_len": ql[:n].astype(np.uint16), "targets": t,
+                "t_off": to[:n].astype(np.uint32), "t_len": tl[:n].astype(np.uint16)}
+
+    def rescue_cpu(self):
+        self.L.or_e2e_pe_rescue_cpu(self.h)
+
+    def set_rescue(self, out6, ok):
+        out6 = np.ascontiguousarray(out6, np.int32).reshape(-1, 6)
+        ok = np.ascontiguousarray(ok, np.int32)
+        if len(ok):
+            self.L.or_e2e_pe_set_rescue(self.h, out6.ctypes.data, ok.ctypes.data)
+
+    def rescue_apply(self):
+        return self.L.or_e2e_pe_apply(self.h)
+
+    def pe_cigar_plan(self):
+        """remark + generateFilteredCIGARs over both mates; returns the jobs"""
+        self.L.or_e2e_pe_remark(self.h)
+        self.n_jobs = self.L.or_e2e_pe_cigar_plan(self.h)
+        return self._jobs()
+
+    def pe_sam(self, names, quals):
+        nb = b"".join(n.encode() + b"\0" for n in names) or b"\0"
+        qa = np.frombuffer(b"".join(quals) or b"\0", np.uint8).copy()
+        nbuf = C.create_string_buffer(nb, len(nb))
+        n = self.L.or_e2e_pe_sam(self.h, nbuf, qa.ctypes.data, None, 0)
+        out = C.create_string_buffer(n + 1)
+        self.L.or_e2e_pe_sam(self.h, nbuf, qa.ctypes.data, out, n + 1)
+        return out.value.decode()
+
+    def run_pe_cpu(self, names, reads1, reads2, quals1, quals2):
+        """The whole CPU paired-end path: SAM text."""
+        self.seed_pairs(reads1, reads2)
+        self.extend_cpu()
+        if self.pe_finish():
+            for d in (1, 2):
+                self.rescue_jobs(d)
+                self.rescue_cpu()
+                self.rescue_apply()
+        self.pe_cigar_plan()
+        self.cigar_cpu()
+        return self.pe_sam(names, list(quals1) + list(quals2))
 
     def set_cigar(self, j, cigar, nm, md, pos):
         c = np.ascontiguousarray(cigar if len(cigar) else [0], np.uint32)

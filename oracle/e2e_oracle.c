@@ -15,9 +15,16 @@
  *   whole pipeline   the reference's lambda fixtures
  *                    (TestData/e2e_expected_se.sam via IntegrationTests.swift:
  *                    319-358: POS, FLAG, CIGAR, MAPQ within 5, NM, MD).
- *   chainer, filter, dedup, markSecondary, MAPQ, SAM fields: pinned only
- *                    through the lambda fixtures (parity partially unpinned:
- *                    the fixtures exercise unique loci, one chain per read).
+ *   paired end       the reference's lambda PE fixture
+ *                    (TestData/e2e_expected_pe.sam via IntegrationTests.swift:
+ *                    430-479); the insert-size estimate against the simulated
+ *                    fragment distribution; rescue and proper pairing on a
+ *                    simulated set (tests/test_oracle_e2e.py).
+ *   chainer, filter, dedup, markSecondary, MAPQ, pair resolution, MAPQ
+ *                    adjustment, SAM fields: pinned only through the lambda
+ *                    fixtures (parity partially unpinned: the fixtures
+ *                    exercise unique loci, one chain per read, and four pairs,
+ *                    too few for an insert-size estimate, so no rescue).
  */
 #define _GNU_SOURCE
 #include "e2e_oracle.h"
@@ -109,6 +116,21 @@ typedef struct {                                                   /* one CIGAR 
     int done;
 } cjob_t;
 
+typedef struct {                                   /* OrientationStats (InsertSizeEstimator) */
+    int64_t low, high, proper_low, proper_high;
+    double mean, stddev;
+    int count, failed;
+} ori_stats_t;
+
+typedef struct {                                   /* MateRescue.RescueCandidate */
+    int pair, mate_read, is_rev, rid, mate_len;
+    int64_t clamped;
+    int64_t q_off, t_off;
+    int32_t q_len, t_len;
+    int32_t out6[6];
+    int ok;
+} rcand_t;
+
 typedef struct {                                                   /* one extension call */
     int64_t q_off, t_off;
     int32_t q_len, t_len, h0, w, eb;
@@ -141,6 +163,13 @@ struct or_e2e {
     int64_t n_calls, calls_cap;
     uint8_t *lq, *lt;
     int64_t lq_n, lq_cap, lt_n, lt_cap;
+    /* paired end: reads [0, n_pairs) are read 1, [n_pairs, 2 n_pairs) read 2 */
+    int n_pairs, do_rescue, primary_ori;
+    ori_stats_t dist[4];
+    rcand_t *resc;
+    int64_t n_resc, resc_cap;
+    uint8_t *rq, *rt;
+    int64_t rq_n, rq_cap, rt_n, rt_cap;
 };
 
 #define VEC_PUSH(vec, x)                                                              \
@@ -223,6 +252,7 @@ static void batch_clear(or_e2e_t *e)
     e->n_jobs = 0;
     e->cq_n = e->cr_n = 0;
     e->n_calls = 0; e->lq_n = e->lt_n = 0;
+    e->n_resc = 0; e->rq_n = e->rt_n = 0; e->n_pairs = 0; e->do_rescue = 0;
     free(e->reads); e->reads = NULL;
     free(e->read_off); e->read_off = NULL;
     free(e->read_len); e->read_len = NULL;
@@ -234,6 +264,7 @@ void or_e2e_free(or_e2e_t *e)
     if (!e) return;
     batch_clear(e);
     free(e->jobs); free(e->cq); free(e->cr); free(e->calls); free(e->lq); free(e->lt);
+    free(e->resc); free(e->rq); free(e->rt);
     free(e->fwd); free(e->text); free(e->bwt); free(e->sa); free(e->occ);
     free(e);
 }
@@ -964,38 +995,50 @@ static int64_t add_cigar_job(or_e2e_t *e, int r, const region_t *g)
     return e->n_jobs++;
 }
 
+/* phases 4.5-5 of one read (BWAMemAligner.swift:402-421) */
+static void dedup_read(or_e2e_t *e, int r, uint64_t read_id)
+{
+    region_vec *rv = &e->regs[r];
+    if (rv->n == 0) return;
+    sort_dedup_patch(e, rv, e->reads + e->read_off[r]);
+    mark_secondary(e, rv, read_id);
+}
+
+/* generateFilteredCIGARs (BWAMemAligner.swift:434-476), default flags (no -a):
+ * the regions that get a CIGAR become jobs, the rest keep the dummy */
+static void cigar_plan_read(or_e2e_t *e, int r)
+{
+    region_vec *rv = &e->regs[r];
+    region_t *R = rv->v;
+    int sec = 0;
+    for (int i = 0; i < rv->n; ++i) {
+        if (i > 0 && R[i].score < MIN_OUT_SCORE) continue;
+        if (R[i].secondary >= 0) {
+            int pi = R[i].secondary;
+            if (pi < rv->n && R[i].score < R[pi].score / 2) continue;
+            sec++;
+        }
+    }
+    const int skip_all = sec > MAX_XA_HITS;
+    for (int i = 0; i < rv->n; ++i) {
+        R[i].cig = -1;
+        if (i > 0 && R[i].score < MIN_OUT_SCORE) continue;
+        if (R[i].secondary >= 0) {
+            int pi = R[i].secondary;
+            if (pi < rv->n && R[i].score < R[pi].score / 2) continue;
+            if (i > 0 && skip_all) continue;
+        }
+        R[i].cig = add_cigar_job(e, r, &R[i]);
+    }
+}
+
 int64_t or_e2e_finish(or_e2e_t *e)
 {
     for (int64_t j = 0; j < e->n_jobs; ++j) { free(e->jobs[j].cigar); free(e->jobs[j].md); }
     e->n_jobs = 0; e->cq_n = e->cr_n = 0;
     for (int r = 0; r < e->n_reads; ++r) {
-        region_vec *rv = &e->regs[r];
-        if (rv->n == 0) continue;
-        const uint8_t *q = e->reads + e->read_off[r];
-        sort_dedup_patch(e, rv, q);                            /* BWAMemAligner.swift:402-409 */
-        mark_secondary(e, rv, (uint64_t)r);                    /* :412-421, readId = batch index */
-        /* generateFilteredCIGARs (:434-476), default flags (no -a) */
-        region_t *R = rv->v;
-        int sec = 0;
-        for (int i = 0; i < rv->n; ++i) {
-            if (i > 0 && R[i].score < MIN_OUT_SCORE) continue;
-            if (R[i].secondary >= 0) {
-                int pi = R[i].secondary;
-                if (pi < rv->n && R[i].score < R[pi].score / 2) continue;
-                sec++;
-            }
-        }
-        const int skip_all = sec > MAX_XA_HITS;
-        for (int i = 0; i < rv->n; ++i) {
-            R[i].cig = -1;
-            if (i > 0 && R[i].score < MIN_OUT_SCORE) continue;
-            if (R[i].secondary >= 0) {
-                int pi = R[i].secondary;
-                if (pi < rv->n && R[i].score < R[pi].score / 2) continue;
-                if (i > 0 && skip_all) continue;
-            }
-            R[i].cig = add_cigar_job(e, r, &R[i]);
-        }
+        dedup_read(e, r, (uint64_t)r);                         /* SE: readId = batch index */
+        cigar_plan_read(e, r);
     }
     return e->n_jobs;
 }
@@ -1143,12 +1186,200 @@ static void seq_qual(sbuf_t *b, const uint8_t *read, const uint8_t *qual, int rl
     }
 }
 
+
+/* CIGARInfo (BWAMemAligner.swift:817-839, the dummy :425-429) */
 typedef struct {
-    int reg;
-    const cjob_t *cj;
+    const uint32_t *cig;
+    int n;
+    int32_t nm;
+    const char *md;
+    int64_t pos;
+    int rev;
+    int64_t ref_len;                   /* reference consumed */
+    int dummy;                         /* cigarString "*" */
+} cinfo_t;
+
+static cinfo_t cinfo_of(const or_e2e_t *e, const region_t *g)
+{
+    cinfo_t c;
+    memset(&c, 0, sizeof c);
+    c.md = "";
+    if (g->cig < 0) { c.dummy = 1; return c; }
+    const cjob_t *j = &e->jobs[g->cig];
+    c.cig = j->cigar; c.n = j->n_cigar; c.nm = j->nm; c.md = j->md ? j->md : "";
+    c.pos = j->pos; c.rev = j->rev;
+    for (int k = 0; k < c.n; ++k) {
+        uint32_t op = c.cig[k] & 0xF;
+        if (op == 0 || op == 2 || op == 3 || op == 7 || op == 8) c.ref_len += c.cig[k] >> 4;
+    }
+    return c;
+}
+
+static void cinfo_str(sbuf_t *b, const cinfo_t *c, int hard)
+{
+    if (c->dummy) sb_put(b, "*", 1);
+    else cigar_str(b, c->cig, c->n, hard);
+}
+
+/* PairedEndInfo (SAMOutputBuilder.swift:5-31) */
+typedef struct {
+    int on, is_r1, proper, mate_tid, mate_rev, mate_unmapped;
+    int64_t mate_pos, tlen;
+    int has_mc;
+    cinfo_t mate_cig;
+} pe_t;
+
+/* RecordSpec */
+typedef struct {
+    int r1, reg;                       /* reg -1: unmapped record */
+    cinfo_t ci;
     int mapq, primary, supp, secondary;
-    int64_t local_pos;
-} seg_t;
+    pe_t pe;
+    char *sa, *xa;                     /* owned; NULL: no tag */
+} spec_t;
+
+typedef struct { spec_t *v; int n, cap; } spec_vec;
+
+static void specs_free(spec_vec *sv)
+{
+    for (int i = 0; i < sv->n; ++i) { free(sv->v[i].sa); free(sv->v[i].xa); }
+    free(sv->v);
+    memset(sv, 0, sizeof *sv);
+}
+
+static char *sb_take(sbuf_t *b)
+{
+    if (!b->s) { sb_put(b, "", 0); }
+    return b->s;
+}
+
+/* classifyAlignments (BWAMemAligner.swift:923-1099) for default flags: the
+ * regions that pass minOutputScore and are not secondaries become the primary
+ * and supplementaries; secondaries not below half their parent go to XA. */
+static void classify(const or_e2e_t *e, const region_t *R, int nreg, int is_r1, const pe_t *pe,
+                     spec_vec *out)
+{
+    int base = out->n, nonsec = 0;
+    const region_t *xa_src[64];
+    int nxa = 0, xa_over = 0;
+    for (int i = 0; i < nreg; ++i) {
+        const region_t *g = &R[i];
+        if (g->score < MIN_OUT_SCORE) continue;
+        if (g->secondary >= 0) {
+            int pi = g->secondary;
+            if (pi < nreg && g->score < R[pi].score / 2) continue;
+            if (nxa < 64) xa_src[nxa++] = g; else xa_over = 1;
+            continue;
+        }
+        spec_t s;
+        memset(&s, 0, sizeof s);
+        s.r1 = is_r1; s.reg = i; s.ci = cinfo_of(e, g); s.mapq = mapq_of(e, g);
+        s.primary = nonsec == 0; s.supp = nonsec > 0;
+        if (pe) s.pe = *pe;
+        VEC_PUSH(*out, s);
+        nonsec++;
+    }
+    if (nonsec == 0) {                                         /* unmapped record */
+        spec_t s;
+        memset(&s, 0, sizeof s);
+        s.r1 = is_r1; s.reg = -1;
+        if (pe) s.pe = *pe;
+        VEC_PUSH(*out, s);
+        return;
+    }
+    spec_t *S = out->v + base;
+    for (int k = 1; k < nonsec; ++k)                           /* supplementary MAPQ cap */
+        if (S[k].mapq > S[0].mapq) S[k].mapq = S[0].mapq;
+    if (!xa_over && nxa <= MAX_XA_HITS) {                      /* buildXATag */
+        sbuf_t xa = {0};
+        for (int k = 0; k < nxa; ++k) {
+            const region_t *g = xa_src[k];
+            cinfo_t c = cinfo_of(e, g);
+            sb_printf(&xa, "%s,%c%lld,", e->name, c.rev ? '-' : '+', (long long)(c.pos + 1));
+            cinfo_str(&xa, &c, 0);
+            sb_printf(&xa, ",%d;", c.nm);
+        }
+        S[0].xa = sb_take(&xa);
+    }
+    if (nonsec > 1) {                                          /* buildSATag */
+        for (int k = 0; k < nonsec; ++k) {
+            sbuf_t sa = {0};
+            for (int x = 0; x < nonsec; ++x) {
+                if (x == k) continue;
+                sb_printf(&sa, "%s,%lld,%c,", e->name, (long long)(S[x].ci.pos + 1),
+                          S[x].ci.rev ? '-' : '+');
+                cinfo_str(&sa, &S[x].ci, 0);
+                sb_printf(&sa, ",%d,%d;", S[x].mapq, S[x].ci.nm);
+            }
+            S[k].sa = sb_take(&sa);
+        }
+    }
+}
+
+/* buildRecord / buildUnmappedRecord (SAMOutputBuilder.swift:103-296, 370-438)
+ * rendered as SAM text; one reference sequence (tid 0). */
+static void write_record(sbuf_t *b, const or_e2e_t *e, const char *name, const uint8_t *read,
+                         const uint8_t *qual, int rl, const spec_t *s, const region_t *R)
+{
+    const pe_t *pe = s->pe.on ? &s->pe : NULL;
+    int flag = 0;
+    if (pe) {
+        flag |= 0x1 | (pe->is_r1 ? 0x40 : 0x80);
+        if (pe->mate_rev) flag |= 0x20;
+        if (pe->mate_unmapped) flag |= 0x8;
+    }
+    const int mtid = pe && !pe->mate_unmapped ? pe->mate_tid : -1;
+    const int64_t mpos = pe && !pe->mate_unmapped ? pe->mate_pos : -1;
+    if (s->reg < 0) {
+        flag |= 0x4;
+        sb_printf(b, "%s\t%d\t*\t0\t0\t*\t%s\t%lld\t0\t", name, flag, mtid < 0 ? "*" : e->name,
+                  (long long)(mpos + 1));
+        seq_qual(b, read, qual, rl, 0, 0, 0);
+        sb_put(b, "\n", 1);
+        return;
+    }
+    const region_t *g = &R[s->reg];
+    const cinfo_t *c = &s->ci;
+    const int rev = g->rb >= e->gl;
+    if (pe && pe->proper) flag |= 0x2;
+    if (rev) flag |= 0x10;
+    if (s->supp) flag |= 0x800;
+    else if (!s->primary) flag |= 0x100;
+    int t0 = 0, t1 = 0;
+    if (s->supp && !c->dummy && c->n) {                        /* convertToHardClip */
+        if ((c->cig[0] & 0xF) == 4) t0 = (int)(c->cig[0] >> 4);
+        if (c->n > 1 && (c->cig[c->n - 1] & 0xF) == 4) t1 = (int)(c->cig[c->n - 1] >> 4);
+        if (!(t0 < rl - t1)) { t0 = 0; t1 = 0; }
+    }
+    sb_printf(b, "%s\t%d\t%s\t%lld\t%d\t", name, flag, e->name, (long long)(c->pos + 1), s->mapq);
+    cinfo_str(b, c, s->supp);
+    sb_printf(b, "\t%s\t%lld\t%lld\t", mtid < 0 ? "*" : "=", (long long)(mpos + 1),
+              (long long)(pe ? pe->tlen : 0));
+    seq_qual(b, read, qual, rl, rev, t0, t1);
+    int32_t xs = g->sub > g->csub ? g->sub : g->csub;
+    sb_printf(b, "\tAS:i:%d\tXS:i:%d\tNM:i:%d\tMD:Z:%s", g->score, xs, c->nm, c->md);
+    if (pe && pe->has_mc) {
+        sb_put(b, "\tMC:Z:", 6);
+        cinfo_str(b, &pe->mate_cig, 0);
+    }
+    if (s->sa) sb_printf(b, "\tSA:Z:%s", s->sa);
+    /* buildXATag returns "" without secondaries, and buildRecord writes any
+     * non-nil tag (SAMOutputBuilder.swift:262-264) */
+    if (s->xa) sb_printf(b, "\tXA:Z:%s", s->xa);
+    sb_put(b, "\n", 1);
+}
+
+static int64_t emit(sbuf_t *b, char *out, int64_t cap)
+{
+    int64_t n = b->n;
+    if (out && cap > 0) {
+        int64_t m = n < cap - 1 ? n : cap - 1;
+        if (b->s) memcpy(out, b->s, (size_t)m);
+        out[m] = 0;
+    }
+    free(b->s);
+    return n;
+}
 
 int64_t or_e2e_sam(const or_e2e_t *e, const char *names, const uint8_t *quals, char *out,
                    int64_t cap)
@@ -1158,108 +1389,613 @@ int64_t or_e2e_sam(const or_e2e_t *e, const char *names, const uint8_t *quals, c
     for (int r = 0; r < e->n_reads; ++r) {
         const char *name = nm;
         nm += strlen(nm) + 1;
-        const uint8_t *read = e->reads + e->read_off[r];
-        const uint8_t *qual = quals + e->read_off[r];
-        const int rl = e->read_len[r];
         const region_vec *rv = &e->regs[r];
-        /* classifyAlignments (BWAMemAligner.swift:923-1099), SE, flag 0 */
-        seg_t *segs = (seg_t *)malloc(sizeof(seg_t) * (size_t)(rv->n + 1));
-        int ns = 0, nonsec = 0;
-        const region_t *xa_src[64];
-        int nxa = 0, xa_over = 0;
-        for (int i = 0; i < rv->n; ++i) {
-            const region_t *g = &rv->v[i];
-            if (g->score < MIN_OUT_SCORE) continue;
-            const cjob_t *cj = g->cig >= 0 ? &e->jobs[g->cig] : NULL;
-            if (g->secondary >= 0) {
-                int pi = g->secondary;
-                if (pi < rv->n && g->score < rv->v[pi].score / 2) continue;
-                if (nxa < 64) xa_src[nxa++] = g; else xa_over = 1;
-                continue;
-            }
-            seg_t s;
-            s.reg = i; s.cj = cj; s.mapq = mapq_of(e, g);
-            s.primary = nonsec == 0; s.supp = nonsec > 0; s.secondary = 0;
-            s.local_pos = cj ? cj->pos : 0;                    /* one sequence: offset 0 */
-            segs[ns++] = s;
-            nonsec++;
-        }
-        if (ns == 0) {                                         /* buildUnmappedRecord */
-            sb_printf(&b, "%s\t4\t*\t0\t0\t*\t*\t0\t0\t", name);
-            seq_qual(&b, read, qual, rl, 0, 0, 0);
-            sb_put(&b, "\n", 1);
-            free(segs);
-            continue;
-        }
-        for (int k = 1; k < ns; ++k)                           /* supplementary MAPQ cap */
-            if (segs[k].mapq > segs[0].mapq) segs[k].mapq = segs[0].mapq;
-        /* XA (buildXATag), omitted when more than maxXAHits secondaries */
-        sbuf_t xa = {0};
-        int have_xa = 0;
-        if (!xa_over && nxa <= MAX_XA_HITS) {
-            have_xa = 1;
-            sb_put(&xa, "", 0);
-            for (int k = 0; k < nxa; ++k) {
-                const region_t *g = xa_src[k];
-                const cjob_t *cj = g->cig >= 0 ? &e->jobs[g->cig] : NULL;
-                sb_printf(&xa, "%s,%c%lld,", e->name, g->rb >= e->gl ? '-' : '+',
-                          (long long)((cj ? cj->pos : 0) + 1));
-                if (cj) cigar_str(&xa, cj->cigar, cj->n_cigar, 0);
-                else sb_put(&xa, "*", 1);
-                sb_printf(&xa, ",%d;", cj ? cj->nm : 0);
-            }
-        }
-        for (int k = 0; k < ns; ++k) {
-            const seg_t *s = &segs[k];
-            const region_t *g = &rv->v[s->reg];
-            const cjob_t *cj = s->cj;
-            const int rev = g->rb >= e->gl;
-            int flag = rev ? 16 : 0;
-            if (s->supp) flag |= 0x800;
-            else if (!s->primary) flag |= 0x100;
-            int t0 = 0, t1 = 0;
-            if (s->supp && cj && cj->n_cigar) {                /* hard clips for supplementary */
-                if ((cj->cigar[0] & 0xF) == 4) t0 = (int)(cj->cigar[0] >> 4);
-                if (cj->n_cigar > 1 && (cj->cigar[cj->n_cigar - 1] & 0xF) == 4)
-                    t1 = (int)(cj->cigar[cj->n_cigar - 1] >> 4);
-                if (!(t0 < rl - t1)) { t0 = 0; t1 = 0; }
-            }
-            sb_printf(&b, "%s\t%d\t%s\t%lld\t%d\t", name, flag, e->name,
-                      (long long)(s->local_pos + 1), s->mapq);
-            if (cj) cigar_str(&b, cj->cigar, cj->n_cigar, s->supp);
-            else sb_put(&b, "*", 1);
-            sb_put(&b, "\t*\t0\t0\t", 7);
-            seq_qual(&b, read, qual, rl, rev, t0, t1);
-            int32_t xs = g->sub > g->csub ? g->sub : g->csub;
-            sb_printf(&b, "\tAS:i:%d\tXS:i:%d\tNM:i:%d\tMD:Z:%s", g->score, xs, cj ? cj->nm : 0,
-                      cj && cj->md ? cj->md : "");
-            if (ns > 1) {                                      /* buildSATag */
-                sb_put(&b, "\tSA:Z:", 6);
-                for (int x = 0; x < ns; ++x) {
-                    if (x == k) continue;
-                    const seg_t *o = &segs[x];
-                    const region_t *og = &rv->v[o->reg];
-                    sb_printf(&b, "%s,%lld,%c,", e->name, (long long)(o->local_pos + 1),
-                              og->rb >= e->gl ? '-' : '+');
-                    if (o->cj) cigar_str(&b, o->cj->cigar, o->cj->n_cigar, 0);
-                    else sb_put(&b, "*", 1);
-                    sb_printf(&b, ",%d,%d;", o->mapq, o->cj ? o->cj->nm : 0);
-                }
-            }
-            /* buildXATag returns "" without secondaries, and buildRecord writes
-             * any non-nil tag (SAMOutputBuilder.swift:262-264) */
-            if (s->primary && have_xa) sb_printf(&b, "\tXA:Z:%s", xa.s ? xa.s : "");
-            sb_put(&b, "\n", 1);
-        }
-        free(xa.s);
-        free(segs);
+        spec_vec sv = {0};
+        classify(e, rv->v, rv->n, 1, NULL, &sv);
+        for (int k = 0; k < sv.n; ++k)
+            write_record(&b, e, name, e->reads + e->read_off[r], quals + e->read_off[r],
+                         e->read_len[r], &sv.v[k], rv->v);
+        specs_free(&sv);
     }
-    int64_t n = b.n;
-    if (out && cap > 0) {
-        int64_t m = n < cap - 1 ? n : cap - 1;
-        if (b.s) memcpy(out, b.s, (size_t)m);
-        out[m] = 0;
+    return emit(&b, out, cap);
+}
+
+/* ---------------------------------------------------------------------- */
+/* paired end: BWAMemAligner.alignPairedBatch / processFullBatch (CPU      */
+/* path, :1348-1543, :1857-2128), InsertSizeEstimator, MateRescue,         */
+/* PairedEndResolver, preparePairOutput (:1584-1856)                       */
+/* ---------------------------------------------------------------------- */
+enum { UNPAIRED_PENALTY = 17, MAX_MATESW = 50, PE_DIST_PAIRS = 4096 };
+
+/* InsertSizeEstimator.inferOrientation; returns 0 when the rids differ */
+static int infer_orientation(const or_e2e_t *e, const region_t *r1, const region_t *r2, int *ori,
+                             int64_t *dist)
+{
+    if (r1->rid != r2->rid) return 0;
+    const int64_t gl = e->gl;
+    int same = (r1->rb >= gl) == (r2->rb >= gl);
+    int64_t p2 = same ? r2->rb : (2 * gl - 1 - r2->rb);
+    *dist = llabs(p2 - r1->rb);
+    *ori = (same ? 0 : 1) ^ (p2 > r1->rb ? 0 : 3);
+    return 1;
+}
+
+static int cmp_i64(const void *a, const void *b)
+{
+    int64_t x = *(const int64_t *)a, y = *(const int64_t *)b;
+    return x < y ? -1 : x > y;
+}
+
+static const region_t *first_nonsec(const region_vec *rv)
+{
+    for (int i = 0; i < rv->n; ++i)
+        if (rv->v[i].secondary < 0) return &rv->v[i];
+    return NULL;
+}
+
+/* InsertSizeEstimator.estimate over the first min(4096, n) pairs: the first
+ * sub-batch pair that alignPairedBatch receives (:1465-1472) */
+static void estimate_dist(or_e2e_t *e)
+{
+    const int np = e->n_pairs < PE_DIST_PAIRS ? e->n_pairs : PE_DIST_PAIRS;
+    int64_t *smp[4];
+    int ns[4] = {0, 0, 0, 0};
+    for (int d = 0; d < 4; ++d) smp[d] = (int64_t *)malloc(sizeof(int64_t) * (size_t)(np + 1));
+    for (int i = 0; i < np; ++i) {
+        const region_t *r1 = first_nonsec(&e->regs[i]), *r2 = first_nonsec(&e->regs[e->n_pairs + i]);
+        if (!r1 || !r2) continue;
+        if (mapq_of(e, r1) < 20 || mapq_of(e, r2) < 20) continue;   /* default scoring */
+        int ori;
+        int64_t d;
+        if (!infer_orientation(e, r1, r2, &ori, &d)) continue;
+        smp[ori][ns[ori]++] = d;
     }
-    free(b.s);
+    memset(e->dist, 0, sizeof e->dist);
+    int max_count = 0, primary = 0;
+    for (int d = 0; d < 4; ++d) {
+        ori_stats_t *st = &e->dist[d];
+        st->failed = 1;
+        int n = ns[d];
+        if (n < 10) continue;
+        int64_t *sz = smp[d];
+        qsort(sz, (size_t)n, sizeof(int64_t), cmp_i64);
+        int64_t q25 = sz[n / 4], q75 = sz[(n * 3) / 4], iqr = q75 - q25;
+        int64_t low = q25 - 2 * iqr, high = q75 + 2 * iqr;
+        int nf = 0;
+        double sum = 0.0;
+        for (int k = 0; k < n; ++k)
+            if (sz[k] >= low && sz[k] <= high) { sum += (double)sz[k]; nf++; }
+        if (nf == 0) continue;
+        double mean = sum / (double)nf, var = 0.0;
+        for (int k = 0; k < n; ++k)
+            if (sz[k] >= low && sz[k] <= high) var += ((double)sz[k] - mean) * ((double)sz[k] - mean);
+        var /= (double)(nf - 1 > 1 ? nf - 1 : 1);
+        double sd = sqrt(var);
+        st->low = low; st->high = high; st->mean = mean; st->stddev = sd; st->count = nf;
+        st->failed = 0;
+        int64_t plow = q25 - 3 * iqr, phigh = q75 + 3 * iqr;
+        int64_t a = (int64_t)(mean - 4.0 * sd + 0.5), b = (int64_t)(mean + 4.0 * sd + 0.5);
+        st->proper_low = plow < a ? plow : a;
+        if (st->proper_low < 1) st->proper_low = 1;
+        st->proper_high = phigh > b ? phigh : b;
+        if (nf > max_count) { max_count = nf; primary = d; }
+    }
+    e->primary_ori = primary;
+    if (max_count > 0)
+        for (int d = 0; d < 4; ++d)
+            if (!e->dist[d].failed && (double)e->dist[d].count < (double)max_count * 0.05)
+                e->dist[d].failed = 1;
+    for (int d = 0; d < 4; ++d) free(smp[d]);
+}
+
+void or_e2e_pe_dist(const or_e2e_t *e, double *out8x4, int32_t *primary)
+{
+    for (int d = 0; d < 4; ++d) {
+        const ori_stats_t *st = &e->dist[d];
+        double *o = out8x4 + d * 8;
+        o[0] = st->failed; o[1] = (double)st->low; o[2] = (double)st->high; o[3] = st->mean;
+        o[4] = st->stddev; o[5] = (double)st->proper_low; o[6] = (double)st->proper_high;
+        o[7] = st->count;
+    }
+    *primary = e->primary_ori;
+}
+
+int64_t or_e2e_pe_finish(or_e2e_t *e, int n_pairs)
+{
+    for (int64_t j = 0; j < e->n_jobs; ++j) { free(e->jobs[j].cigar); free(e->jobs[j].md); }
+    e->n_jobs = 0; e->cq_n = e->cr_n = 0;
+    if (2 * n_pairs != e->n_reads) return -1;
+    e->n_pairs = n_pairs;
+    for (int r = 0; r < e->n_reads; ++r) {                     /* phase 1: readId = gi << 1 | mate */
+        uint64_t gi = (uint64_t)(r % n_pairs), mate = (uint64_t)(r / n_pairs);
+        dedup_read(e, r, (gi << 1) | mate);
+    }
+    estimate_dist(e);
+    e->do_rescue = !e->dist[e->primary_ori].failed;
+    e->n_resc = 0;
+    return e->do_rescue;
+}
+
+/* MateRescue.hasMateHit */
+static int has_mate_hit(const or_e2e_t *e, const region_vec *mate, const region_t *a, int dir,
+                        const ori_stats_t *st)
+{
+    const int64_t gl = e->gl;
+    for (int i = 0; i < mate->n; ++i) {
+        const region_t *m = &mate->v[i];
+        if (m->rid != a->rid) continue;
+        int same = (a->rb >= gl) == (m->rb >= gl);
+        int64_t p2 = same ? m->rb : (2 * gl - 1 - m->rb);
+        int d = (same ? 0 : 1) ^ (p2 > a->rb ? 0 : 3);
+        if (d != dir) continue;
+        int64_t dist = llabs(p2 - a->rb);
+        if (dist >= st->proper_low && dist <= st->proper_high) return 1;
+    }
+    return 0;
+}
+
+/* BWAMemAligner.selectRescueCandidates (:2143-2159) */
+static int rescue_templates(const region_vec *rv, const region_t **out)
+{
+    const region_t *best = first_nonsec(rv);
+    if (!best) return 0;
+    int32_t thr = best->score - UNPAIRED_PENALTY;
+    int n = 0;
+    for (int i = 0; i < rv->n; ++i) {
+        if (!(rv->v[i].secondary < 0 && rv->v[i].score >= thr)) continue;
+        out[n++] = &rv->v[i];
+        if (n >= MAX_MATESW) break;
+    }
     return n;
+}
+
+/* MateRescue.collectCandidates for pair i, direction dir (1: read 2 from
+ * read 1's regions, 2: read 1 from read 2's), with PackedReference.
+ * fetchSequence windows (PackedReference.swift:72-121). */
+static void collect_rescue(or_e2e_t *e, int i, int dir)
+{
+    const int tr = dir == 1 ? i : e->n_pairs + i;              /* template read */
+    const int mr = dir == 1 ? e->n_pairs + i : i;              /* mate read */
+    const region_vec *tv = &e->regs[tr], *mv = &e->regs[mr];
+    const region_t *tp[MAX_MATESW];
+    const int nt = rescue_templates(tv, tp);
+    const int64_t gl = e->gl;
+    const int mlen = e->read_len[mr];
+    const uint8_t *mread = e->reads + e->read_off[mr];
+    for (int t = 0; t < nt; ++t) {
+        const region_t *a = tp[t];
+        int all_sat = 1;
+        for (int r = 0; r < 4; ++r) {
+            const ori_stats_t *st = &e->dist[r];
+            if (st->failed) continue;
+            if (has_mate_hit(e, mv, a, r, st)) continue;
+            all_sat = 0;
+            const int is_rev = (r >> 1) != (r & 1), larger = (r >> 1) == 0;
+            int64_t rb, re;
+            if (!is_rev) {
+                if (larger) { rb = a->rb + st->proper_low; re = a->rb + st->proper_high + mlen; }
+                else { rb = a->rb - st->proper_high - mlen; re = a->rb - st->proper_low; }
+            } else {
+                if (larger) { rb = a->rb + st->proper_low - mlen; re = a->rb + st->proper_high; }
+                else { rb = a->rb - st->proper_high; re = a->rb - st->proper_low + mlen; }
+            }
+            if (rb < 0) rb = 0;
+            if (re > 2 * gl) re = 2 * gl;
+            if (!(rb < re)) continue;
+            int64_t cb, ce, clamped;
+            const int fwd = rb < gl;
+            if (fwd) { cb = rb; ce = re < gl ? re : gl; }
+            else {
+                int64_t fb = 2 * gl - re, fe = 2 * gl - rb;
+                cb = fb > 0 ? fb : 0; ce = fe < gl ? fe : gl;
+            }
+            if (!(cb < ce)) continue;
+            clamped = fwd ? cb : 2 * gl - ce;
+            /* one sequence: rid 0 == a->rid */
+            e->resc = (rcand_t *)grow(e->resc, &e->resc_cap, e->n_resc + 1, sizeof(rcand_t));
+            rcand_t *c = &e->resc[e->n_resc++];
+            memset(c, 0, sizeof *c);
+            c->pair = i; c->mate_read = mr; c->is_rev = is_rev; c->rid = a->rid;
+            c->mate_len = mlen; c->clamped = clamped;
+            e->rq = (uint8_t *)grow(e->rq, &e->rq_cap, e->rq_n + mlen + 1, 1);
+            e->rt = (uint8_t *)grow(e->rt, &e->rt_cap, e->rt_n + (ce - cb) + 1, 1);
+            c->q_off = e->rq_n; c->q_len = mlen;
+            for (int k = 0; k < mlen; ++k) {
+                uint8_t x = is_rev ? mread[mlen - 1 - k] : mread[k];
+                e->rq[e->rq_n + k] = is_rev ? (x < 4 ? (uint8_t)(3 - x) : 4) : x;
+            }
+            e->rq_n += mlen;
+            c->t_off = e->rt_n; c->t_len = (int32_t)(ce - cb);
+            for (int64_t k = 0; k < ce - cb; ++k)
+                e->rt[e->rt_n + k] = fwd ? e->fwd[cb + k] : (uint8_t)(3 - e->fwd[ce - 1 - k]);
+            e->rt_n += ce - cb;
+        }
+        if (all_sat) break;
+    }
+}
+
+int64_t or_e2e_pe_rescue_jobs(or_e2e_t *e, int dir)
+{
+    e->n_resc = 0; e->rq_n = e->rt_n = 0;
+    if (!e->do_rescue) return 0;
+    for (int i = 0; i < e->n_pairs; ++i) collect_rescue(e, i, dir);
+    return e->n_resc;
+}
+
+void or_e2e_pe_rescue_sizes(const or_e2e_t *e, int64_t *q_bytes, int64_t *t_bytes)
+{
+    *q_bytes = e->rq_n;
+    *t_bytes = e->rt_n;
+}
+
+void or_e2e_pe_get_rescue_jobs(const or_e2e_t *e, uint8_t *q, int64_t *q_off, int32_t *q_len,
+                               uint8_t *t, int64_t *t_off, int32_t *t_len)
+{
+    if (e->rq_n) memcpy(q, e->rq, (size_t)e->rq_n);
+    if (e->rt_n) memcpy(t, e->rt, (size_t)e->rt_n);
+    for (int64_t i = 0; i < e->n_resc; ++i) {
+        q_off[i] = e->resc[i].q_off; q_len[i] = e->resc[i].q_len;
+        t_off[i] = e->resc[i].t_off; t_len[i] = e->resc[i].t_len;
+    }
+}
+
+void or_e2e_pe_set_rescue(or_e2e_t *e, const int32_t *out6, const int32_t *ok)
+{
+    for (int64_t i = 0; i < e->n_resc; ++i) {
+        memcpy(e->resc[i].out6, out6 + i * 6, sizeof(int32_t) * 6);
+        e->resc[i].ok = ok[i];
+    }
+}
+
+/* LocalSWAligner.align of every pending candidate (minSubScore =
+ * minSeedLength * matchScore clamped to u8, MateRescue.swift:125) */
+void or_e2e_pe_rescue_cpu(or_e2e_t *e)
+{
+    int ms = e->sc.min_seed_len * e->sc.a;
+    ms = ms < 0 ? 0 : ms > 255 ? 255 : ms;
+    for (int64_t i = 0; i < e->n_resc; ++i) {
+        rcand_t *c = &e->resc[i];
+        int64_t cells = 0;
+        c->ok = or_local_sw(e->rq + c->q_off, c->q_len, e->rt + c->t_off, c->t_len, &e->sc, ms,
+                            c->out6, &cells);
+    }
+}
+
+/* MateRescue.applyResult, appended in candidate order (the CPU path's
+ * rescue() appends per template and direction in the same order) */
+int64_t or_e2e_pe_apply(or_e2e_t *e)
+{
+    const int32_t min_score = e->sc.min_seed_len * e->sc.a;
+    int64_t added = 0;
+    for (int64_t i = 0; i < e->n_resc; ++i) {
+        const rcand_t *c = &e->resc[i];
+        if (!c->ok) continue;
+        const int32_t score = c->out6[0], qb = c->out6[1], qe = c->out6[2], tb = c->out6[3],
+                      te = c->out6[4], sc2 = c->out6[5];
+        if (score < min_score) continue;
+        region_t g;
+        memset(&g, 0, sizeof g);
+        g.score = score; g.true_score = score; g.csub = sc2; g.rid = c->rid; g.secondary = -1;
+        g.cig = -1;
+        if (!c->is_rev) {
+            g.rb = c->clamped + tb; g.re = c->clamped + te + 1; g.qb = qb; g.qe = qe + 1;
+        } else {
+            g.rb = 2 * e->gl - (c->clamped + te + 1); g.re = 2 * e->gl - (c->clamped + tb);
+            g.qb = c->mate_len - (qe + 1); g.qe = c->mate_len - qb;
+        }
+        int64_t rs = (int64_t)te - tb + 1, qs = (int64_t)(qe - qb + 1);
+        g.seed_cov = (int32_t)((rs < qs ? rs : qs) / 2);
+        VEC_PUSH(e->regs[c->mate_read], g);
+        added++;
+    }
+    return added;
+}
+
+/* re-marking after rescue (processFullBatch's CPU path, rescuePair :2104-2123) */
+void or_e2e_pe_remark(or_e2e_t *e)
+{
+    if (!e->do_rescue) return;
+    for (int r = 0; r < e->n_reads; ++r) {
+        uint64_t gi = (uint64_t)(r % e->n_pairs), mate = (uint64_t)(r / e->n_pairs);
+        if (e->regs[r].n > 1) mark_secondary(e, &e->regs[r], (gi << 1) | mate);
+    }
+}
+
+int64_t or_e2e_pe_cigar_plan(or_e2e_t *e)
+{
+    for (int64_t j = 0; j < e->n_jobs; ++j) { free(e->jobs[j].cigar); free(e->jobs[j].md); }
+    e->n_jobs = 0; e->cq_n = e->cr_n = 0;
+    for (int r = 0; r < e->n_reads; ++r) cigar_plan_read(e, r);
+    return e->n_jobs;
+}
+
+/* PairedEndResolver.computeZPenalty */
+static int z_penalty(double z, int32_t a)
+{
+    double ec = erfc(z / sqrt(2.0));
+    if (!(ec > 0)) return (int)a * 10;
+    double pen = 0.721 * log(2.0 * ec) * (double)a + 0.499;
+    int v = (int)floor(-pen);
+    return v > 0 ? v : 0;
+}
+
+typedef struct { int score, idx1, idx2, proper; int64_t ins; int ori; } phit_t;
+
+static int phit_before(const void *a, const void *b, void *ctx)
+{
+    return ((const phit_t *)a)->score > ((const phit_t *)b)->score;
+}
+
+typedef struct { int idx1, idx2, score, subo, n_sub, proper; } pdec_t;
+
+/* PairedEndResolver.resolve; returns 0 for nil */
+static int resolve_pair(const or_e2e_t *e, const region_vec *v1, const region_vec *v2, pdec_t *d)
+{
+    if (v1->n == 0 || v2->n == 0) return 0;
+    int c1[4096], c2[4096], n1 = 0, n2 = 0;
+    const region_vec *vs[2] = {v1, v2};
+    int *cs[2] = {c1, c2}, *ns[2] = {&n1, &n2};
+    for (int s = 0; s < 2; ++s) {                              /* topCandidates */
+        const region_t *best = first_nonsec(vs[s]);
+        for (int i = 0; i < vs[s]->n && *ns[s] < 4096; ++i)
+            if (!best || vs[s]->v[i].score >= best->score - UNPAIRED_PENALTY) cs[s][(*ns[s])++] = i;
+    }
+    phit_t *hits = (phit_t *)malloc(sizeof(phit_t) * (size_t)(n1 * n2 + 1));
+    int nh = 0;
+    for (int a = 0; a < n1; ++a)
+        for (int b = 0; b < n2; ++b) {
+            const region_t *r1 = &v1->v[c1[a]], *r2 = &v2->v[c2[b]];
+            int ori;
+            int64_t ins;
+            if (!infer_orientation(e, r1, r2, &ori, &ins)) continue;
+            const ori_stats_t *st = &e->dist[ori];
+            if (st->failed || !(st->stddev > 0) || ins < st->proper_low || ins > st->proper_high)
+                continue;
+            double z = fabs((double)ins - st->mean) / st->stddev;
+            phit_t h = {r1->score + r2->score - z_penalty(z, e->sc.a), c1[a], c2[b],
+                        ori == e->primary_ori, ins, ori};
+            hits[nh++] = h;
+        }
+    if (nh == 0) { free(hits); return 0; }
+    stable_sort(hits, (size_t)nh, sizeof(phit_t), phit_before, NULL);
+    d->idx1 = hits[0].idx1; d->idx2 = hits[0].idx2; d->score = hits[0].score;
+    d->proper = hits[0].proper;
+    d->subo = nh > 1 ? hits[1].score : 0;
+    const int tmp = e->sc.a + e->sc.b;
+    d->n_sub = 0;
+    for (int i = 1; i < nh; ++i)
+        if (d->subo - hits[i].score <= tmp) d->n_sub++;
+    free(hits);
+    return 1;
+}
+
+/* PairedEndResolver.computeTLEN */
+static void compute_tlen(int64_t p1, int rev1, int64_t l1, int64_t p2, int rev2, int64_t l2,
+                         int64_t *t1, int64_t *t2)
+{
+    (void)rev2;
+    int64_t e1 = p1 + l1, e2 = p2 + l2;
+    int64_t lo = p1 < p2 ? p1 : p2, hi = e1 > e2 ? e1 : e2, span = hi - lo;
+    if (p1 < p2 || (p1 == p2 && !rev1)) { *t1 = span; *t2 = -span; }
+    else { *t1 = -span; *t2 = span; }
+}
+
+/* PairedEndResolver.adjustMAPQ */
+static void adjust_mapq(const or_e2e_t *e, int *q1p, int *q2p, const region_t *g1,
+                        const region_t *g2, const pdec_t *d)
+{
+    const int o = d->score;
+    int subo = d->subo;
+    const int score_un = g1->score + g2->score - UNPAIRED_PENALTY;
+    if (subo < score_un) subo = score_un;
+    if (*q1p == 0 && *q2p == 0) {
+        if (g1->sub > 0) { int s = o - (g1->score - g1->sub); if (s > subo) subo = s; }
+        if (g2->sub > 0) { int s = o - (g2->score - g2->sub); if (s > subo) subo = s; }
+    }
+    const double a = e->sc.a;
+    int qpe = (int)(6.02 * (double)(o - subo) / a + 0.499);
+    if (d->n_sub > 0) qpe -= (int)(4.343 * log((double)(d->n_sub + 1)) + 0.499);
+    if (qpe < 0) qpe = 0;
+    if (qpe > 60) qpe = 60;
+    qpe = (int)((double)qpe * (1.0 - 0.5 * ((double)g1->frac_rep + (double)g2->frac_rep)) + 0.499);
+    int q1 = *q1p, q2 = *q2p;
+    if (o > score_un) {
+        q1 = q1 > qpe ? q1 : (qpe < q1 + 40 ? qpe : q1 + 40);
+        q2 = q2 > qpe ? q2 : (qpe < q2 + 40 ? qpe : q2 + 40);
+        if (g1->csub > 0) {
+            int cap = (int)(6.02 * (double)(g1->score - g1->csub) / a + 0.499);
+            if (q1 > cap) q1 = cap;
+        }
+        if (g2->csub > 0) {
+            int cap = (int)(6.02 * (double)(g2->score - g2->csub) / a + 0.499);
+            if (q2 > cap) q2 = cap;
+        }
+    }
+    *q1p = q1 < 0 ? 0 : q1 > 60 ? 60 : q1;
+    *q2p = q2 < 0 ? 0 : q2 > 60 ? 60 : q2;
+}
+
+/* BWAMemAligner.promotePairedRegion (:1549-1583) */
+static void promote(region_t *R, int n, int idx)
+{
+    if (idx < 0 || idx >= n) return;
+    if (R[idx].secondary >= 0 && R[idx].secondary < n) R[idx].sub = R[R[idx].secondary].score;
+    R[idx].secondary = -1;
+    if (idx > 0) {
+        region_t t = R[0]; R[0] = R[idx]; R[idx] = t;
+        for (int i = 0; i < n; ++i) {
+            if (R[i].secondary == idx) R[i].secondary = 0;
+            else if (R[i].secondary == 0) R[i].secondary = idx;
+        }
+    }
+    R[0].secondary = -1;
+}
+
+static int is_multi(const region_vec *rv)
+{
+    int c = 0;
+    for (int i = 0; i < rv->n; ++i)
+        if (rv->v[i].secondary < 0 && rv->v[i].score >= MIN_OUT_SCORE && ++c > 1) return 1;
+    return 0;
+}
+
+static void pe_pair(pe_t *pe, int is_r1, int proper, const cinfo_t *mate, int64_t tlen)
+{
+    memset(pe, 0, sizeof *pe);
+    pe->on = 1; pe->is_r1 = is_r1; pe->proper = proper;
+    pe->mate_tid = 0; pe->mate_pos = mate->pos; pe->mate_rev = mate->rev; pe->mate_unmapped = 0;
+    pe->tlen = tlen; pe->has_mc = 1; pe->mate_cig = *mate;
+}
+
+/* preparePairOutput (:1584-1799) and prepareMappedUnmappedPlan (:1800-1856)
+ * of pair i, then the records in plan order */
+static void pair_records(or_e2e_t *e, int i, const char *name, const uint8_t *quals, sbuf_t *b)
+{
+    const int r1 = i, r2 = e->n_pairs + i;
+    region_vec *v1 = &e->regs[r1], *v2 = &e->regs[r2];
+    spec_vec sv = {0};
+    region_t *R1 = v1->v, *R2 = v2->v;
+    region_t *P1 = NULL, *P2 = NULL;                           /* paired copies */
+    const int e1 = v1->n == 0, e2 = v2->n == 0;
+    if (e1 && e2) {
+        for (int s = 0; s < 2; ++s) {
+            spec_t sp;
+            memset(&sp, 0, sizeof sp);
+            sp.r1 = s == 0; sp.reg = -1;
+            sp.pe.on = 1; sp.pe.is_r1 = s == 0; sp.pe.mate_tid = -1; sp.pe.mate_pos = -1;
+            sp.pe.mate_unmapped = 1;
+            VEC_PUSH(sv, sp);
+        }
+    } else {
+        pdec_t d;
+        const int has_dec = resolve_pair(e, v1, v2, &d);
+        if (has_dec && !is_multi(v1) && !is_multi(v2)) {
+            P1 = (region_t *)malloc(sizeof(region_t) * (size_t)v1->n);
+            P2 = (region_t *)malloc(sizeof(region_t) * (size_t)v2->n);
+            memcpy(P1, v1->v, sizeof(region_t) * (size_t)v1->n);
+            memcpy(P2, v2->v, sizeof(region_t) * (size_t)v2->n);
+            promote(P1, v1->n, d.idx1);
+            promote(P2, v2->n, d.idx2);
+            for (int k = 1; k < v1->n; ++k) if (P1[k].secondary < 0) P1[k].secondary = 0;
+            for (int k = 1; k < v2->n; ++k) if (P2[k].secondary < 0) P2[k].secondary = 0;
+            /* a promoted region whose CIGAR was skipped gets one now (CPU) */
+            region_t *pp[2] = {&P1[0], &P2[0]};
+            const int rr[2] = {r1, r2};
+            for (int s = 0; s < 2; ++s) {
+                if (pp[s]->cig >= 0) continue;
+                int64_t jx = add_cigar_job(e, rr[s], pp[s]);
+                cjob_t *j = &e->jobs[jx];
+                uint32_t cig[4096];
+                char md[8192];
+                or_cigar_res_t res;
+                int n = or_cigar_generate(e->cq + j->q_off, j->q_len, e->cr + j->r_off, j->r_len,
+                                          j->qb, j->qe, j->read_len, j->rev, j->true_score, j->w,
+                                          &e->sc, j->ref_pos, cig, 4096, md, 8192, &res);
+                or_e2e_set_cigar(e, jx, cig, n < 4096 ? n : 4096, res.nm, md, res.pos);
+                pp[s]->cig = jx;
+            }
+            cinfo_t c1 = cinfo_of(e, &P1[0]), c2 = cinfo_of(e, &P2[0]);
+            int64_t t1, t2;
+            compute_tlen(c1.pos, c1.rev, c1.ref_len, c2.pos, c2.rev, c2.ref_len, &t1, &t2);
+            pe_t pe1, pe2;
+            pe_pair(&pe1, 1, d.proper, &c2, t1);
+            pe_pair(&pe2, 0, d.proper, &c1, t2);
+            classify(e, P1, v1->n, 1, &pe1, &sv);
+            int n_r1 = sv.n;
+            classify(e, P2, v2->n, 0, &pe2, &sv);
+            int i1 = -1, i2 = -1;
+            for (int k = 0; k < sv.n; ++k) {
+                if (sv.v[k].primary && sv.v[k].r1 && i1 < 0) i1 = k;
+                if (sv.v[k].primary && !sv.v[k].r1 && i2 < 0) i2 = k;
+            }
+            (void)n_r1;
+            if (i1 >= 0 && i2 >= 0) {
+                int q1 = sv.v[i1].mapq, q2 = sv.v[i2].mapq;
+                adjust_mapq(e, &q1, &q2, &P1[0], &P2[0], &d);
+                sv.v[i1].mapq = q1; sv.v[i2].mapq = q2;
+                for (int k = 0; k < sv.n; ++k)
+                    if (sv.v[k].supp) {
+                        int cap = sv.v[k].r1 ? q1 : q2;
+                        if (sv.v[k].mapq > cap) sv.v[k].mapq = cap;
+                    }
+            }
+            R1 = P1; R2 = P2;
+        } else if (!e1 && e2) {
+            goto mapped_unmapped;
+        } else if (e1 && !e2) {
+            goto mapped_unmapped;
+        } else {
+            /* both mapped, no pairing: default primaries */
+            cinfo_t c1 = cinfo_of(e, &v1->v[0]), c2 = cinfo_of(e, &v2->v[0]);
+            int proper = 0;
+            int64_t t1 = 0, t2 = 0;
+            int ori;
+            int64_t ins;
+            if (infer_orientation(e, &v1->v[0], &v2->v[0], &ori, &ins)) {
+                const ori_stats_t *st = &e->dist[ori];
+                if (!st->failed && ins >= st->proper_low && ins <= st->proper_high) proper = 1;
+            }
+            compute_tlen(c1.pos, c1.rev, c1.ref_len, c2.pos, c2.rev, c2.ref_len, &t1, &t2);
+            pe_t pe1, pe2;
+            pe_pair(&pe1, 1, proper, &c2, t1);
+            pe_pair(&pe2, 0, proper, &c1, t2);
+            classify(e, v1->v, v1->n, 1, &pe1, &sv);
+            classify(e, v2->v, v2->n, 0, &pe2, &sv);
+        }
+    }
+    goto write;
+mapped_unmapped: {
+        const int m_is_r1 = !e1;
+        region_vec *mv = m_is_r1 ? v1 : v2;
+        const int mr = m_is_r1 ? r1 : r2;
+        region_t *g = &mv->v[0];
+        if (g->cig < 0) {                                      /* cigarCache.first ?? generateCIGAR */
+            int64_t jx = add_cigar_job(e, mr, g);
+            cjob_t *j = &e->jobs[jx];
+            uint32_t cig[4096];
+            char md[8192];
+            or_cigar_res_t res;
+            int n = or_cigar_generate(e->cq + j->q_off, j->q_len, e->cr + j->r_off, j->r_len, j->qb,
+                                      j->qe, j->read_len, j->rev, j->true_score, j->w, &e->sc,
+                                      j->ref_pos, cig, 4096, md, 8192, &res);
+            or_e2e_set_cigar(e, jx, cig, n < 4096 ? n : 4096, res.nm, md, res.pos);
+            g->cig = jx;
+        }
+        cinfo_t c = cinfo_of(e, g);
+        spec_t ms, us;
+        memset(&ms, 0, sizeof ms);
+        memset(&us, 0, sizeof us);
+        ms.r1 = m_is_r1; ms.reg = 0; ms.ci = c; ms.mapq = mapq_of(e, g); ms.primary = 1;
+        ms.pe.on = 1; ms.pe.is_r1 = m_is_r1; ms.pe.mate_tid = 0; ms.pe.mate_pos = c.pos;
+        ms.pe.mate_unmapped = 1;
+        us.r1 = !m_is_r1; us.reg = -1;
+        us.pe.on = 1; us.pe.is_r1 = !m_is_r1; us.pe.mate_tid = 0; us.pe.mate_pos = c.pos;
+        us.pe.mate_rev = c.rev; us.pe.mate_unmapped = 0;
+        if (m_is_r1) { VEC_PUSH(sv, ms); VEC_PUSH(sv, us); }
+        else { VEC_PUSH(sv, us); VEC_PUSH(sv, ms); }
+        /* the plan keeps the mapped read's regions only */
+        if (m_is_r1) R1 = v1->v; else R2 = v2->v;
+    }
+write:
+    for (int k = 0; k < sv.n; ++k) {
+        const spec_t *s = &sv.v[k];
+        const int rd = s->r1 ? r1 : r2;
+        write_record(b, e, name, e->reads + e->read_off[rd], quals + e->read_off[rd],
+                     e->read_len[rd], s, s->r1 ? R1 : R2);
+    }
+    specs_free(&sv);
+    free(P1);
+    free(P2);
+}
+
+int64_t or_e2e_pe_sam(or_e2e_t *e, const char *names, const uint8_t *quals, char *out, int64_t cap)
+{
+    sbuf_t b = {0};
+    const char *nm = names;
+    for (int i = 0; i < e->n_pairs; ++i) {
+        const char *name = nm;
+        nm += strlen(nm) + 1;
+        pair_records(e, i, name, quals, &b);
+    }
+    return emit(&b, out, cap);
 }

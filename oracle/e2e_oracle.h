@@ -26,10 +26,24 @@
  *   Sources/BWAMem/SAMOutputBuilder.swift:103-438     buildRecord, hard clips, SA/XA tags,
  *                                                     buildUnmappedRecord
  *   Sources/BWACore/AlignmentRegion.swift:3-14        hash64
+ *   paired end:
+ *   Sources/BWAMem/BWAMemAligner.swift:1348-1543, 1857-2159
+ *                                                     alignPairedBatch (pair ids, distribution
+ *                                                     from the first 4096 pairs), processFullBatch
+ *                                                     and rescuePair (CPU rescue order),
+ *                                                     selectRescueCandidates, hasMultipleNonSecondary
+ *   Sources/BWAMem/BWAMemAligner.swift:1549-1856      promotePairedRegion, preparePairOutput,
+ *                                                     prepareMappedUnmappedPlan
+ *   Sources/BWAMem/InsertSizeEstimator.swift          estimate, inferOrientation
+ *   Sources/BWAMem/MateRescue.swift                   collectCandidates, applyResult, hasMateHit
+ *   Sources/FMIndex/PackedReference.swift:72-121      fetchSequence
+ *   Sources/BWAMem/PairedEndResolver.swift            resolve, computeTLEN, adjustMAPQ, z penalty
+ *   Sources/BWAMem/SAMOutputBuilder.swift:5-57        PairedEndInfo, makePair
  *
  * Scope: one reference sequence without ambiguous bases (the config-1 random
- * reference and the lambda fixture genome), default options (flag 0: no -a,
- * -M, -Y, -5, -q; no -y reseeding; no ALT contigs).
+ * reference, the lambda fixture genome, the simulated paired-end genome),
+ * default options (flag 0: no -a, -M, -Y, -5, -q, -P, -S; no -y reseeding; no
+ * ALT contigs; no manual insert size).
  */
 #ifndef E2E_ORACLE_H
 #define E2E_ORACLE_H
@@ -103,6 +117,37 @@ void or_e2e_set_cigar(or_e2e_t *e, int64_t j, const uint32_t *cigar, int n_cigar
  * fits). */
 int64_t or_e2e_sam(const or_e2e_t *e, const char *names, const uint8_t *quals, char *out,
                    int64_t cap);
+
+/* ---- paired end (reads [0, n) are read 1 of pairs 0..n-1, [n, 2n) read 2) ---- */
+
+/* Phases 4.5-5 per read with readId = pair << 1 | mate, then the insert-size
+ * distribution of the first min(4096, n) pairs.  Returns 1 when rescue runs
+ * (the primary orientation's estimate did not fail), 0 when not, -1 when the
+ * batch does not hold 2n reads. */
+int64_t or_e2e_pe_finish(or_e2e_t *e, int n_pairs);
+/* per orientation FF FR RF RR: failed, low, high, mean, stddev, properLow,
+ * properHigh, count; and the primary orientation */
+void or_e2e_pe_dist(const or_e2e_t *e, double *out8x4, int32_t *primary);
+/* Rescue candidates of direction 1 (read 2 from read 1's regions) or 2 (read 1
+ * from read 2's regions, rescued ones included): local-SW jobs (mate sequence,
+ * reference window).  Returns the count. */
+int64_t or_e2e_pe_rescue_jobs(or_e2e_t *e, int dir);
+void or_e2e_pe_rescue_sizes(const or_e2e_t *e, int64_t *q_bytes, int64_t *t_bytes);
+void or_e2e_pe_get_rescue_jobs(const or_e2e_t *e, uint8_t *q, int64_t *q_off, int32_t *q_len,
+                               uint8_t *t, int64_t *t_off, int32_t *t_len);
+/* LocalSWAligner.align of the pending candidates on the CPU, or imported
+ * results (n x {score, qb, qe, tb, te, score2} and the success flags). */
+void or_e2e_pe_rescue_cpu(or_e2e_t *e);
+void or_e2e_pe_set_rescue(or_e2e_t *e, const int32_t *out6, const int32_t *ok);
+/* MateRescue.applyResult of the pending candidates; returns regions added. */
+int64_t or_e2e_pe_apply(or_e2e_t *e);
+/* markSecondary after rescue (only when rescue runs). */
+void or_e2e_pe_remark(or_e2e_t *e);
+/* CIGAR job list over all 2n reads (generateFilteredCIGARs). */
+int64_t or_e2e_pe_cigar_plan(or_e2e_t *e);
+/* SAM records of the pairs in pair order (one name per pair). */
+int64_t or_e2e_pe_sam(or_e2e_t *e, const char *names, const uint8_t *quals, char *out,
+                      int64_t cap);
 
 #ifdef __cplusplus
 }

@@ -26,6 +26,7 @@
  *   WL_LOCAL        mate-rescue local SW: 150-bp mate (either strand) against
  *                   a 600-bp window holding its true locus (0.5% / 0.05%).
  */
+#include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -326,6 +327,51 @@ void wl_pack_pac(const uint8_t *ref, int64_t n, uint8_t *pac)
 }
 
 void wl_free_bytes(uint8_t *p) { free(p); }
+
+/* Read pairs of `len` bases in FR orientation: a fragment of length
+ * round(N(ins_mean, ins_sd)) (at least len) starts at a uniform position of
+ * ref (one sequence of gl bases); its source window is mutated with the shared
+ * model; read 1 is the fragment's first len bases and read 2 the reverse
+ * complement of its last len bases, or the other way round (probability 1/2:
+ * a fragment from the reverse strand).  A fraction `hard` of pairs get a
+ * substitution in read 2 every 15 bases (no exact 19-mer: seeding finds
+ * nothing, local alignment still scores ~100).  pos: fragment start; rev: 1
+ * when read 1 is the reverse-strand read. */
+int wl_pe_reads(const uint8_t *ref, int64_t gl, uint64_t seed, int64_t n, int len,
+                double ins_mean, double ins_sd, double p_sub, double p_indel, double hard,
+                uint8_t *bases1, uint8_t *bases2, int64_t *pos, uint8_t *rev)
+{
+    const int slack = 64;
+    const int max_frag = (int)(ins_mean + 8 * ins_sd) + len;
+    if (gl < max_frag + slack + 1 || len <= 0) return -1;
+    rng_t r;
+    rng_seed(&r, seed * 0x9E3779B97F4A7C15ull + 23);
+    uint8_t *buf = (uint8_t *)malloc((size_t)(max_frag + 2 * slack));
+    for (int64_t i = 0; i < n; ++i) {
+        int L, m;
+        int64_t p;
+        do {
+            double u1 = unif01(&r), u2 = unif01(&r);
+            double z = sqrt(-2.0 * log(u1 > 1e-300 ? u1 : 1e-300)) * cos(6.283185307179586 * u2);
+            L = (int)floor(ins_mean + ins_sd * z + 0.5);
+            if (L < len) L = len;
+            if (L > max_frag) L = max_frag;
+            p = uni(&r, 0, gl - L - slack);
+            m = mutate(&r, ref + p, L + slack, p_sub, p_indel, buf, max_frag + 2 * slack, NULL);
+        } while (m < L);
+        const int rv = (int)(next_u64(&r) & 1);
+        uint8_t *f = bases1 + i * len, *b = bases2 + i * len;
+        uint8_t *left = rv ? b : f, *right = rv ? f : b;
+        memcpy(left, buf, (size_t)len);
+        for (int k = 0; k < len; ++k) right[k] = (uint8_t)(3 - buf[L - 1 - k]);
+        if (unif01(&r) < hard)
+            for (int k = 7; k < len; k += 15) b[k] = (uint8_t)((b[k] + 1 + uni(&r, 0, 2)) & 3);
+        pos[i] = p;
+        rev[i] = (uint8_t)rv;
+    }
+    free(buf);
+    return 0;
+}
 
 /* Single-end reads of `len` bases sampled uniformly from both strands of ref
  * (one sequence of gl bases): the source window starts at a uniform position,

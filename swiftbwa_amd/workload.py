@@ -99,3 +99,28 @@ def se_reads(ref, seed, n, length=150, p_sub=0.005, p_indel=0.0005):
     if rc != 0:
         raise ValueError("reference too short for the read length")
     return bases, pos, rev
+
+
+def pe_reads(ref, seed, n, length=150, ins_mean=400.0, ins_sd=50.0, p_sub=0.005, p_indel=0.0005,
+             hard=0.0):
+    """FR read pairs from `ref` (bytes 0..3): (read 1 bases n x length, read 2
+    bases, fragment starts, read-1-reverse flags); a fraction `hard` of pairs
+    carry a substitution in read 2 every 15 bases."""
+    lib = _load()
+    if not hasattr(lib, "_pe_bound"):
+        lib.wl_pe_reads.restype = C.c_int
+        lib.wl_pe_reads.argtypes = [C.c_void_p, C.c_int64, C.c_uint64, C.c_int64, C.c_int,
+                                    C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib._pe_bound = True
+    ref = np.ascontiguousarray(ref, np.uint8)
+    b1 = np.zeros((n, length), np.uint8)
+    b2 = np.zeros((n, length), np.uint8)
+    pos = np.zeros(n, np.int64)
+    rev = np.zeros(n, np.uint8)
+    rc = lib.wl_pe_reads(ref.ctypes.data, len(ref), int(seed), int(n), int(length),
+                         float(ins_mean), float(ins_sd), float(p_sub), float(p_indel), float(hard),
+                         b1.ctypes.data, b2.ctypes.data, pos.ctypes.data, rev.ctypes.data)
+    if rc != 0:
+        raise ValueError("reference too short for the fragment lengths")
+    return b1, b2, pos, rev

@@ -11,7 +11,12 @@ on the CPU:
   IntegrationTests.swift:319-358 does: POS, FLAG bits, CIGAR, MAPQ within 5,
   NM, MD);
 * the config-1 read set: every read maps, at its simulated locus in >= 99.5%
-  of cases, and the call set is non-empty.
+  of cases, and the call set is non-empty;
+* paired end: the reference's lambda PE fixture (TestData/e2e_expected_pe.sam,
+  IntegrationTests.swift:430-479: POS and CIGAR of mapped reads, FLAG bits,
+  MAPQ, NM, MD; four pairs, so the insert-size estimate fails and no rescue
+  runs), and a simulated set whose FR distribution, rescue and proper pairs
+  follow from how it was made.
 """
 import os
 
@@ -157,3 +162,86 @@ def test_config1_cpu():
     assert hits >= 0.995 * len(names)
     js, out = E.calls()
     assert js["n"] > 0 and np.all(out[:, 0] >= js["h0"])
+
+
+# ---- paired end ---------------------------------------------------------------
+
+def _lambda_pe():
+    name, fwd = e2e.read_fasta(os.path.join(GOLD, "lambda.fa"))
+    r1 = e2e.read_fastq(os.path.join(GOLD, "e2e_reads_r1.fq"))
+    r2 = e2e.read_fastq(os.path.join(GOLD, "e2e_reads_r2.fq"))
+    with open(os.path.join(GOLD, "e2e_expected_pe.sam")) as f:
+        exp = e2e.parse_sam(f.read())
+    names = [a[0] for a in r1]
+    assert names == [b[0] for b in r2]
+    return name, fwd, names, r1, r2, exp
+
+
+def check_lambda_pe_sam(sam, exp):
+    """IntegrationTests.swift:430-479: POS and CIGAR for mapped reads, FLAG bits,
+    MAPQ exact when 60 (else within 5), NM, MD; records keyed by name and mate."""
+    got = e2e.parse_sam(sam)
+    n_checked = 0
+    for name, recs in exp.items():
+        for er in recs:
+            mate = er["flag"] & 0xC0
+            ar = [r for r in got[name] if r["flag"] & 0xC0 == mate]
+            assert len(ar) == 1, (name, mate)
+            ar = ar[0]
+            if not er["flag"] & 4:
+                assert ar["pos"] == er["pos"], name
+                assert ar["cigar"] == er["cigar"], name
+            assert ar["flag"] & FLAG_MASK == er["flag"] & FLAG_MASK, (name, ar["flag"], er["flag"])
+            if er["mapq"] == 60:
+                assert ar["mapq"] == 60, name
+            elif er["mapq"] > 0:
+                assert abs(ar["mapq"] - er["mapq"]) <= 5, name
+            for t in ("NM", "MD"):
+                if t in er["tags"]:
+                    assert ar["tags"][t] == er["tags"][t], (name, t)
+            n_checked += 1
+    assert n_checked == 8
+
+
+def test_lambda_pe_fixture():
+    name, fwd, names, r1, r2, exp = _lambda_pe()
+    E = e2e.E2E(fwd, name)
+    sam = E.run_pe_cpu(names, [a[1] for a in r1], [b[1] for b in r2], [a[2] for a in r1],
+                       [b[2] for b in r2])
+    check_lambda_pe_sam(sam, exp)
+    d, _ = E.pe_dist()
+    assert np.all(d[:, 0] == 1)           # four pairs: too few for the insert-size estimate
+
+
+def pe_set(n=10000):
+    """Paired-end set: 1-Mbp reference with planted 300-bp / 6-kb families (20
+    copies each at 2% divergence), FR pairs of 2 x 150 bp with insert ~ N(400, 50),
+    0.5% substitutions and 0.05% indels; 5% of pairs have a read 2 that seeding
+    cannot place (a substitution every 15 bases), so mate rescue has to."""
+    ref = serving.make_reference(5, 1_000_000, copies=20, divergence=0.02)
+    b1, b2, pos, rev = wl.pe_reads(ref, 5, n, hard=0.05)
+    names = ["p%d" % i for i in range(n)]
+    q = [b"I" * b1.shape[1]] * n
+    return ref, names, list(b1), list(b2), q, pos, rev
+
+
+def test_pe_cpu():
+    ref, names, r1, r2, q, pos, rev = pe_set(3000)
+    E = e2e.E2E(ref, "chr5")
+    E.seed_pairs(r1, r2)
+    E.extend_cpu()
+    assert E.pe_finish()
+    d, pri = E.pe_dist()
+    assert pri == 1 and d[1, 0] == 0       # FR estimated
+    assert abs(d[1, 3] - 400) < 10 and abs(d[1, 4] - 50) < 10
+    added = 0
+    for direction in (1, 2):
+        E.rescue_jobs(direction)
+        E.rescue_cpu()
+        added += E.rescue_apply()
+    assert added > 0.03 * len(names)
+    E.pe_cigar_plan()
+    E.cigar_cpu()
+    recs = e2e.parse_sam(E.pe_sam(names, q + q))
+    proper = sum(all(r["flag"] & 0x2 for r in recs[n]) for n in names)
+    assert proper >= 0.98 * len(names)
