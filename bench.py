@@ -34,7 +34,7 @@ PEAK_I32_TOPS = 39.3       # 32-bit VALU lane-ops
 # the factors being what the same counters read for a 256 MiB device-to-device
 # copy (`bench.py --profile-leg copy`: FETCH_SIZE 131,087 KiB, WRITE_SIZE
 # 262,144 KiB).  Summary and commands: profiles/ext_uniform_1M_pmc.md.
-EXT_TRAFFIC_BYTES = 28491421713
+EXT_TRAFFIC_BYTES = 27230386272
 # Algorithmic HBM bytes per job: 2-bit query and target words, one 32-byte
 # descriptor, one 24-byte result row.
 
