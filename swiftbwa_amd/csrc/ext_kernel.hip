@@ -990,19 +990,6 @@ __device__ __forceinline__ void wave_min3(uint32_t &a, uint32_t &b, uint32_t &c)
     c = __builtin_amdgcn_readlane(c, 63);
 }
 
-// The 16-bit (h, e) entry of column j in half `hs` (0: job A, 1: job B) of
-// this lane, in LDS for j < LC and in the wave's HBM region above (LC = 0:
-// all LDS).  With LC > 0 the pointer is generic (flat access), which avoids
-// a divergent branch for the few per-row accesses that use it.
-template <int LC>
-__device__ __forceinline__ uint16_t *col16(uint32_t *col, uint32_t *gcol, int j, int hs)
-{
-    uint16_t *l = reinterpret_cast<uint16_t *>(col);
-    if (LC == 0) return l + cpos<uint16_t>(j) + hs;
-    uint16_t *g = reinterpret_cast<uint16_t *>(gcol);
-    return j < LC ? l + cpos<uint16_t>(j) + hs : g + cpos<uint16_t>(j - LC) + hs;
-}
-
 // Stores the 16-bit entry of column j, half hs, with an LDS or a global store
 // chosen per lane.  A generic (flat) store would leave a flat access pending
 // into the next sweep, and with one pending the compiler waits for every LDS
