@@ -98,8 +98,43 @@ struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     uint32_t *counters = nullptr;
+    // pinned host staging of the slot's chunk (query words, extension
+    // descriptors, result rows), grown on demand and reused: copies from it
+    // run at link speed instead of through the driver's pageable staging.
+    // A chunk's copies finish before run_chunk returns, so the next chunk of
+    // the slot may overwrite it.
+    char *pin = nullptr;
+    size_t pin_cap = 0;
     std::thread th;
 };
+
+// A per-job host array without value-initialisation: a chunk's arrays are
+// filled in parallel, so zeroing them first on one thread would cost more
+// than the fill (tens of MB per large chunk).
+template <typename T> struct RawVec {
+    std::unique_ptr<T[]> p;
+    explicit RawVec(size_t n) : p(new T[n > 0 ? n : 1]) {}
+    T *data() { return p.get(); }
+    const T *data() const { return p.get(); }
+    T &operator[](size_t i) { return p[i]; }
+    const T &operator[](size_t i) const { return p[i]; }
+};
+
+// Grows a slot's pinned staging to at least `bytes` (contents not kept).
+static bool pin_reserve(Slot &sl, size_t bytes)
+{
+    if (bytes <= sl.pin_cap) return true;
+    if (sl.pin) (void)hipHostFree(sl.pin);
+    sl.pin = nullptr;
+    sl.pin_cap = 0;
+    const size_t cap = std::max(bytes + bytes / 4, (size_t)1 << 20);
+    if (hipHostMalloc((void **)&sl.pin, cap, hipHostMallocDefault) != hipSuccess) {
+        sl.pin = nullptr;
+        return false;
+    }
+    sl.pin_cap = cap;
+    return true;
+}
 
 struct hipsw_server {
     hipsw_engine *eng = nullptr;
@@ -168,34 +203,71 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
     hipStream_t st = sl.stream;
     const double t_host0 = now_ms_s();
     // --- host packing: queries from the request, targets as fetch windows ---
-    std::vector<uint32_t> qw(n), tw(n), qnw(n, kNoN), tnw(n, kNoN);
-    std::vector<int32_t> ql(n), tl(n);
-    std::vector<Window> win(n);
-    uint64_t words = 0, nwords = 0;
-    std::vector<uint8_t> qhas_n(n);
-    parallel_for(n, host_threads(), [&](int64_t lo, int64_t hi) {
-        for (int64_t k = lo; k < hi; ++k) qhas_n[k] = has_n(r.qbytes.data() + r.q_off[k], r.q_len[k]);
-    });
-    for (int k = 0; k < n; ++k) {
-        ql[k] = r.q_len[k];
-        qw[k] = (uint32_t)words;
-        words += (ql[k] + 15) / 16;
-        if (qhas_n[k]) {
-            qnw[k] = (uint32_t)nwords;
-            nwords += (ql[k] + 31) / 32;
+    // Sequence arena: every query's words, then every target's words (written
+    // by the window fetch on the device), then 64 zero words of padding, so
+    // the host packs and uploads the queries only.
+    RawVec<uint32_t> qw(n), tw(n), qnw(n), tnw(n);
+    RawVec<int32_t> ql(n), tl(n);
+    RawVec<Window> win(n);
+    RawVec<uint8_t> qhas_n(n);
+    // word offsets: per-block sums in parallel, a scan over the blocks, then
+    // each block's offsets in parallel.  The blocks are parallel_for's own
+    // chunks of the jobs (one thread each; one call covers them all when the
+    // chunk is too small to split).
+    const int nth = host_threads();
+    const int64_t bs = std::max<int64_t>(1, (n + nth - 1) / nth), nb = (n + bs - 1) / bs;
+    auto blocks = [&](const std::function<void(int64_t, int64_t, int64_t)> &body) {
+        parallel_for(n, nth, [&](int64_t lo, int64_t hi) {
+            for (int64_t b = lo / bs; b * bs < hi; ++b)
+                body(b, std::max(lo, b * bs), std::min(hi, (b + 1) * bs));
+        });
+    };
+    std::vector<uint64_t> bq(nb + 1, 0), bt(nb + 1, 0), bn(nb + 1, 0);
+    blocks([&](int64_t b, int64_t k0, int64_t k1) {
+        uint64_t sq = 0, stt = 0, sn = 0;
+        for (int64_t k = k0; k < k1; ++k) {
+            qhas_n[k] = has_n(r.qbytes.data() + r.q_off[k], r.q_len[k]);
+            ql[k] = r.q_len[k];
+            tl[k] = fetched_len(r.t_pos[k], r.t_len_req[k], srv->gl);
+            sq += (ql[k] + 15) / 16;
+            stt += (tl[k] + 15) / 16;
+            if (qhas_n[k]) sn += (ql[k] + 31) / 32;
         }
-        tl[k] = fetched_len(r.t_pos[k], r.t_len_req[k], srv->gl);
-        tw[k] = (uint32_t)words;
-        win[k] = Window{r.t_pos[k], (uint32_t)words, tl[k], (int32_t)r.t_rev[k]};
-        words += (tl[k] + 15) / 16;
-    }
-    words += 64;
+        bq[b + 1] = sq; bt[b + 1] = stt; bn[b + 1] = sn;
+    });
+    for (int64_t b = 0; b < nb; ++b) { bq[b + 1] += bq[b]; bt[b + 1] += bt[b]; bn[b + 1] += bn[b]; }
+    const uint64_t qwords = bq[nb], twords = bt[nb];
+    uint64_t nwords = bn[nb];
+    const uint64_t words = qwords + twords + 64;
+    if (words > 0xFFFFFFF0ull) return HIPSW_E_INVALID;
     nwords += 64;
-    std::vector<uint32_t> seq(words, 0), nmask(nwords, 0);
+    blocks([&](int64_t b, int64_t k0, int64_t k1) {
+        uint64_t q = bq[b], t = qwords + bt[b], m = bn[b];
+        for (int64_t k = k0; k < k1; ++k) {
+            qw[k] = (uint32_t)q;
+            q += (ql[k] + 15) / 16;
+            tw[k] = (uint32_t)t;
+            t += (tl[k] + 15) / 16;
+            qnw[k] = qhas_n[k] ? (uint32_t)m : kNoN;
+            if (qhas_n[k]) m += (ql[k] + 31) / 32;
+            tnw[k] = kNoN;                                // windows come from the .pac
+            win[k] = Window{r.t_pos[k], tw[k], tl[k], (int32_t)r.t_rev[k]};
+        }
+    });
+    // pinned staging: query words | extension descriptors | result rows
+    const size_t pin_q = ((size_t)qwords * 4 + 255) & ~(size_t)255;
+    const size_t pin_pk = ((size_t)r.n_ext * sizeof(PkMeta) + 255) & ~(size_t)255;
+    const size_t pin_out = (size_t)n * 6 * 4 + 4 * (size_t)n * 4;
+    if (!pin_reserve(sl, pin_q + pin_pk + pin_out)) return HIPSW_E_DEVICE;
+    uint32_t *hq = reinterpret_cast<uint32_t *>(sl.pin);
+    PkMeta *hpk = reinterpret_cast<PkMeta *>(sl.pin + pin_q);
+    int32_t *hout6 = reinterpret_cast<int32_t *>(sl.pin + pin_q + pin_pk);
+    int32_t *hstv = hout6 + (size_t)n * 6, *hsc = hstv + n, *hnc = hsc + n, *hfl = hnc + n;
+    std::vector<uint32_t> nmask(nwords, 0);
     std::atomic<bool> packed_ok{true};
     parallel_for(n, host_threads(), [&](int64_t lo, int64_t hi) {
         for (int64_t k = lo; k < hi; ++k)
-            if (!pack2(r.qbytes.data() + r.q_off[k], ql[k], seq.data() + qw[k],
+            if (!pack2(r.qbytes.data() + r.q_off[k], ql[k], hq + qw[k],
                        qnw[k] != kNoN ? nmask.data() + qnw[k] : nullptr))
                 packed_ok = false;
     });
@@ -229,8 +301,10 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
     }
     // NW first pass
     const int nw0 = r.n_ext + r.n_loc;
-    std::vector<int32_t> wcur(n, 1);
-    for (int k = 0; k < r.n_nw; ++k) wcur[nw0 + k] = std::max(1, r.nw_w[k]);
+    std::vector<int32_t> wcur(n);
+    parallel_for(n, host_threads(), [&](int64_t lo, int64_t hi) {
+        for (int64_t k = lo; k < hi; ++k) wcur[k] = k >= nw0 ? std::max(1, r.nw_w[k - nw0]) : 1;
+    });
     int mat_abs = 0;
     for (int k = 0; k < 25; ++k) mat_abs = std::max(mat_abs, std::abs((int)srv->sc.mat[k]));
     const int max_pen = std::max(mat_abs, std::max(srv->sc.o_del + srv->sc.e_del, srv->sc.o_ins + srv->sc.e_ins));
@@ -264,14 +338,19 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
     uint8_t *d_colmax = carve<uint8_t>(cur, colmax_b);
     unsigned long long *d_cells = carve<unsigned long long>(cur, 2);
     PkMeta *d_pk = carve<PkMeta>(cur, r.n_ext);
-    std::vector<int32_t> h0v(n, 0), wv(n, 0), ebv(n, 0);
-    std::vector<PkMeta> pk;   // host copy lives until the chunk's stream is synchronised
-    for (int k = 0; k < r.n_ext; ++k) { h0v[k] = r.h0[k]; wv[k] = r.w[k]; ebv[k] = r.eb[k]; }
+    RawVec<int32_t> h0v(n), wv(n), ebv(n);
+    parallel_for(n, host_threads(), [&](int64_t lo, int64_t hi) {
+        for (int64_t k = lo; k < hi; ++k) {
+            const bool e = k < r.n_ext;
+            h0v[k] = e ? r.h0[k] : 0; wv[k] = e ? r.w[k] : 0; ebv[k] = e ? r.eb[k] : 0;
+        }
+    });
     bool ok = true;
     auto up = [&](void *dst, const void *src, size_t b) {
         if (ok && b) ok = hipMemcpyAsync(dst, src, b, hipMemcpyHostToDevice, st) == hipSuccess;
     };
-    up(d_seq, seq.data(), words * 4);
+    up(d_seq, hq, (size_t)qwords * 4);
+    if (ok) ok = launch_zero_u32(d_seq + qwords + twords, 64, st) == hipSuccess;
     up(d_nmask, nmask.data(), nwords * 4);
     up(d_qw, qw.data(), (size_t)n * 4); up(d_tw, tw.data(), (size_t)n * 4);
     up(d_qn, qnw.data(), (size_t)n * 4); up(d_tn, tnw.data(), (size_t)n * 4);
@@ -288,10 +367,12 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
     if (ok && r.n_ext) {
         up(d_order, ext_order.data(), ext_order.size() * 4);
         up(d_items, ext_items.data(), ext_items.size() * sizeof(WorkItem));
-        pk.resize(r.n_ext);
-        build_pk_meta(r.n_ext, ext_order.data(), qw.data(), tw.data(), qnw.data(), tnw.data(), ql.data(),
-                      tl.data(), h0v.data(), wv.data(), ebv.data(), pk.data());
-        up(d_pk, pk.data(), pk.size() * sizeof(PkMeta));
+        parallel_for(r.n_ext, host_threads(), [&](int64_t lo, int64_t hi) {
+            build_pk_meta((int)(hi - lo), ext_order.data() + lo, qw.data(), tw.data(), qnw.data(),
+                          tnw.data(), ql.data(), tl.data(), h0v.data(), wv.data(), ebv.data(),
+                          hpk + lo);
+        });
+        up(d_pk, hpk, (size_t)r.n_ext * sizeof(PkMeta));
         void *gs = nullptr;
         if (ok && ext_scratch) ok = hipMallocAsync(&gs, ext_scratch, st) == hipSuccess;
         if (ok) ok = launch_zero_u32(sl.counters, 64, st) == hipSuccess;
@@ -328,12 +409,11 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
             const int snmax = loc_qmax <= 64 ? 4 : loc_qmax <= 128 ? 8 : loc_qmax <= 192 ? 12 : 16;
             if (ok) ok = launch_local_striped(snmax, ld, lp, d_order2, (int)loc_s.size(), d_colmax,
                                               lp.min_sub > 0 ? loc_tmax : 0, st) == hipSuccess;
-            std::vector<int32_t> stv(n);
-            if (ok) ok = hipMemcpyAsync(stv.data(), d_status, (size_t)n * 4, hipMemcpyDeviceToHost, st) == hipSuccess;
+            if (ok) ok = hipMemcpyAsync(hstv, d_status, (size_t)n * 4, hipMemcpyDeviceToHost, st) == hipSuccess;
             if (ok) ok = hipStreamSynchronize(st) == hipSuccess;
             if (ok)
                 for (uint32_t k : loc_s)
-                    if (stv[k] == LOCAL_NEEDS_SCALAR) loc_c.push_back(k);
+                    if (hstv[k] == LOCAL_NEEDS_SCALAR) loc_c.push_back(k);
         }
         if (ok && !loc_c.empty()) {
             int qc = 1, tc = 1;
@@ -419,16 +499,16 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
         }
     }
     // --- results ---
-    std::vector<int32_t> out6((size_t)n * 6), stv(n), sc(n), nc(n), fl(n);
+    int32_t *out6 = hout6, *stv = hstv, *sc = hsc, *nc = hnc, *fl = hfl;
     unsigned long long cells[2] = {0, 0};
     auto down = [&](void *dst, const void *src, size_t b) {
         if (ok && b) ok = hipMemcpyAsync(dst, src, b, hipMemcpyDeviceToHost, st) == hipSuccess;
     };
-    down(out6.data(), d_out6, (size_t)n * 6 * 4);
-    down(stv.data(), d_status, (size_t)n * 4);
-    down(sc.data(), d_score, (size_t)n * 4);
-    down(nc.data(), d_ncig, (size_t)n * 4);
-    down(fl.data(), d_flags, (size_t)n * 4);
+    down(out6, d_out6, (size_t)n * 6 * 4);
+    down(stv, d_status, (size_t)n * 4);
+    down(sc, d_score, (size_t)n * 4);
+    down(nc, d_ncig, (size_t)n * 4);
+    down(fl, d_flags, (size_t)n * 4);
     down(cells, d_cells, sizeof cells);
     if (ok) ok = hipStreamSynchronize(st) == hipSuccess;
     // CIGARs: only the used ops of each row come back (rows hold cap ops)
@@ -455,7 +535,7 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
     hipFreeAsync(dmem, st);
     if (ok) ok = hipStreamSynchronize(st) == hipSuccess;
     if (!ok) return HIPSW_E_DEVICE;
-    r.ext_out.assign(out6.begin(), out6.begin() + (size_t)r.n_ext * 6);
+    r.ext_out.assign(out6, out6 + (size_t)r.n_ext * 6);
     r.loc_out.assign((size_t)r.n_loc * 6, 0);
     r.loc_ok.assign(r.n_loc, 0);
     for (int k = 0; k < r.n_loc; ++k) {
@@ -463,9 +543,9 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
         r.loc_ok[k] = stv[j] == LOCAL_OK ? 1 : 0;
         if (r.loc_ok[k]) memcpy(&r.loc_out[(size_t)k * 6], &out6[(size_t)j * 6], 6 * 4);
     }
-    r.nw_score.assign(sc.begin() + nw0, sc.begin() + nw0 + r.n_nw);
-    r.nw_ncig.assign(nc.begin() + nw0, nc.begin() + nw0 + r.n_nw);
-    r.nw_flags.assign(fl.begin() + nw0, fl.begin() + nw0 + r.n_nw);
+    r.nw_score.assign(sc + nw0, sc + nw0 + r.n_nw);
+    r.nw_ncig.assign(nc + nw0, nc + nw0 + r.n_nw);
+    r.nw_flags.assign(fl + nw0, fl + nw0 + r.n_nw);
     r.cells = (int64_t)(cells[0] + cells[1]);
     return HIPSW_OK;
 }
@@ -685,6 +765,7 @@ extern "C" void hipsw_server_destroy(hipsw_server *srv)
             hipStreamDestroy(sl->stream);
         }
         if (sl->counters) hipFree(sl->counters);
+        if (sl->pin) (void)hipHostFree(sl->pin);
     }
     if (srv->d_pac) hipFree(srv->d_pac);
     delete srv;
