@@ -70,14 +70,13 @@ extern "C" int hipsw_local_sw(hipsw_engine *eng, int32_t n, const uint8_t *queri
     p.max_base_i = max_i;
 
     std::vector<uint32_t> striped, scalar;
-    int qmax_s = 0, tmax = 0;
+    int tmax = 0;
     for (int k = 0; k < n; ++k) {
         const int q = pj.ql[k], t = pj.tl[k];
         if (q <= 0 || t <= 0) continue;                        // :48, nil
         tmax = std::max(tmax, t);
         if ((int64_t)q * match_score <= 250 && q <= 256) {
             striped.push_back((uint32_t)k);
-            qmax_s = std::max(qmax_s, q);
         } else {
             scalar.push_back((uint32_t)k);
         }
@@ -111,17 +110,19 @@ extern "C" int hipsw_local_sw(hipsw_engine *eng, int32_t n, const uint8_t *queri
     up(d_tn, pj.tnw.data(), (size_t)n * 4);
     up(d_ql, pj.ql.data(), (size_t)n * 4);
     up(d_tl, pj.tl.data(), (size_t)n * 4);
-    up(d_ord_s, striped.data(), striped.size() * 4);
+    std::vector<uint32_t> grouped(striped.size());
+    int32_t grp_off[17];
+    group_by_stripes(striped.data(), striped.size(), pj.ql.data(), grouped.data(), grp_off);
+    up(d_ord_s, grouped.data(), grouped.size() * 4);
     const LocalDev dev{d_seq, d_nmask, d_qw, d_tw, d_qn, d_tn, d_ql, d_tl, d_out, d_status};
     double kernel_ms = 0;
     int64_t launches = 0;
     std::vector<int32_t> status(n, LOCAL_NIL);
     if (good) good = eng->check(hipMemsetAsync(d_status, 0, (size_t)n * 4, eng->stream), "memset");
     if (good && !striped.empty()) {
-        const int snmax = qmax_s <= 64 ? 4 : qmax_s <= 128 ? 8 : qmax_s <= 192 ? 12 : 16;
         good = eng->check(hipEventRecord(eng->ev0, eng->stream), "event") &&
-               eng->check(launch_local_striped(snmax, dev, p, d_ord_s, (int)striped.size(),
-                                               d_colmax, p.min_sub > 0 ? tmax : 0, eng->stream),
+               eng->check(launch_local_striped(dev, p, d_ord_s, grp_off, d_colmax,
+                                               p.min_sub > 0 ? tmax : 0, eng->stream),
                           "launch local_striped") &&
                eng->check(hipEventRecord(eng->ev1, eng->stream), "event") &&
                eng->check(hipEventSynchronize(eng->ev1), "local_striped");

@@ -28,9 +28,16 @@ struct LocalParams {
     int32_t max_base_i;  // :406-412
 };
 
-hipError_t launch_local_striped(int snmax, const LocalDev &d, const LocalParams &p,
-                                const uint32_t *order, int n, uint8_t *colmax, int tcap,
-                                hipStream_t s);
+// Stable counting sort of the task ids ids[0..n) by stripe count
+// sn = ceil(q_len/16), 1..16 (every q_len in 1..256): out[off[sn-1]..off[sn])
+// is group sn.
+void group_by_stripes(const uint32_t *ids, size_t n, const int32_t *q_len, uint32_t *out,
+                      int32_t off[17]);
+// Striped launches over the grouped id list `order` on the device, one per
+// group of at least a thousand tasks (smaller groups join the next larger
+// one); colmax holds tcap bytes per position of order.
+hipError_t launch_local_striped(const LocalDev &d, const LocalParams &p, const uint32_t *order,
+                                const int32_t off[17], uint8_t *colmax, int tcap, hipStream_t s);
 hipError_t launch_local_scalar(const LocalDev &d, const LocalParams &p, const uint32_t *order,
                                int n, int *scratch, int tcap, int qcap, hipStream_t s);
 

@@ -39,7 +39,11 @@ __device__ __forceinline__ int row_max16(int v)
     return v;
 }
 
-__device__ __forceinline__ int satsub(int x, int c) { return max(x - c, 0); }
+// u8 `pmax(x, c) &- c` for x, c in [0, 255]: a clamped unsigned subtract.
+__device__ __forceinline__ uint32_t usat(uint32_t x, uint32_t c)
+{
+    return __builtin_elementwise_sub_sat(x, c);
+}
 
 // Base k of a 2-bit + N-mask sequence (k must be in range).
 __device__ __forceinline__ int base_at(const LocalDev &d, uint32_t w0, uint32_t n0, int k)
@@ -52,8 +56,12 @@ __device__ __forceinline__ int base_at(const LocalDev &d, uint32_t w0, uint32_t 
 // One striped pass (simdSWPass, :133-306) for the task of this lane's 16-lane
 // row.  `rev` reads the reversed prefixes query[0..qlen) / target[0..tlen).
 // Returns false on 8-bit overflow.  colmax[i] receives the column maxima when
-// peaks are tracked (min_sub > 0).
-template <int SNMAX>
+// peaks are tracked (min_sub > 0).  Every stripe 0..SNMAX-1 is computed in
+// every row; with EXACT the caller guarantees that every active task of the
+// wave has sn == SNMAX, otherwise stripes s >= sn of a task are padding whose
+// cells are never read (the next row's diagonal, the lazy-F carry-in and the
+// column maxima take stripes < sn only).
+template <int SNMAX, bool EXACT>
 __device__ bool striped_pass(const LocalDev &d, const LocalParams &p, bool act, uint32_t qw0,
                              uint32_t qn0, uint32_t tw0, uint32_t tn0, int qlen, int tlen,
                              bool rev, int min_sub, uint8_t *colmax, int &o_score, int &o_qend,
@@ -93,17 +101,23 @@ __device__ bool striped_pass(const LocalDev &d, const LocalParams &p, bool act, 
             prof[(k * NR + r) * 64] = x;
         }
     }
-    int H[SNMAX], E[SNMAX];
+    // H and E are non-negative and at most 255 (u8 lanes held in 32-bit
+    // registers); `usat` is the u8 `pmax(x, c) &- c` of the reference, one
+    // clamped v_sub_u32.
+    uint32_t H[SNMAX], E[SNMAX];
 #pragma unroll
     for (int s = 0; s < SNMAX; ++s) { H[s] = 0; E[s] = 0; }
     int max_score = 0, max_i = -1, max_j = -1;
     bool ok = true;
-    const int tmax_w = [&] {
-        int v = act ? tlen : 0;
+    auto in_s = [&](int s) { return EXACT || s < sn; };            // stripe s is the task's
+    auto last_s = [&](int s) { return EXACT ? s == SNMAX - 1 : s == sn - 1; };
+    // wave-wide target length, as a scalar: the row loop is uniform
+    int tmax_w = act ? tlen : 0;
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-        return v;
-    }();
+    for (int o = 32; o > 0; o >>= 1) tmax_w = max(tmax_w, __shfl_xor(tmax_w, o, 64));
+    tmax_w = __builtin_amdgcn_readfirstlane(tmax_w);
+    const uint32_t ubias = (uint32_t)bias, ugoe = (uint32_t)goe, uge = (uint32_t)ge;
+    const uint64_t rowmask = 0xFFFFull << (grp * 16);
     for (int i = 0; i < tmax_w; ++i) {
         const bool live = act && ok && i < tlen;
         int tb = 0;
@@ -115,58 +129,53 @@ __device__ bool striped_pass(const LocalDev &d, const LocalParams &p, bool act, 
 #pragma unroll
         for (int r = 0; r < NR; ++r) pr[r] = prof[(tb * NR + r) * 64];
         // vH = shiftRight(H[sn-1]), :206
-        int hlast = 0;
+        uint32_t hlast = 0;
 #pragma unroll
-        for (int s = 0; s < SNMAX; ++s) hlast = s == sn - 1 ? H[s] : hlast;
-        int vH = row_shr1(hlast);
-        int f = 0;
+        for (int s = 0; s < SNMAX; ++s) hlast = last_s(s) ? H[s] : hlast;
+        uint32_t vH = (uint32_t)row_shr1((int)hlast);
+        uint32_t f = 0, fend = 0;
 #pragma unroll
         for (int s = 0; s < SNMAX; ++s) {                      // :209-231
-            if (s < sn && live) {
-                const int pv = (int)((pr[s >> 2] >> ((s & 3) * 8)) & 0xFFu);
-                int hn = (vH + pv) & 0xFF;                     // wrapping &+
-                hn = satsub(hn, bias);
-                const int hp = H[s];
-                vH = hp;
-                int ev = max(satsub(hp, goe), E[s]);
-                ev = satsub(ev, ge);
-                E[s] = ev;
-                hn = max(max(hn, ev), f);
-                H[s] = hn;
-                f = satsub(max(satsub(hn, goe), f), ge);
-            }
+            // wrapping u8 &+ of the diagonal and the profile byte
+            const uint32_t hd = (vH + (pr[s >> 2] >> ((s & 3) * 8))) & 0xFFu;
+            const uint32_t hp = H[s];
+            vH = hp;
+            const uint32_t ev = usat(max(usat(hp, ugoe), E[s]), uge);
+            E[s] = ev;
+            const uint32_t hn = max(max(usat(hd, ubias), ev), f);
+            H[s] = hn;
+            f = usat(max(usat(hn, ugoe), f), uge);
+            fend = last_s(s) ? f : fend;
         }
-        // lazy-F, one pass with an early break (:234-242)
-        int cf = row_shr1(f);
+        // lazy-F, one pass with an early break (:234-242): a task goes on to
+        // the next stripe while some lane of its 16-lane row changed
+        uint32_t cf = (uint32_t)row_shr1((int)fend);
         bool go = live;
 #pragma unroll
         for (int s = 0; s < SNMAX; ++s) {
-            if (__builtin_amdgcn_ballot_w64(go && s < sn) == 0) break;   // no task still correcting
-            const bool in = go && s < sn;
-            const int nh = max(H[s], cf);
-            const uint64_t changed = __ballot(in && nh != H[s]);
-            const bool any = ((changed >> (grp * 16)) & 0xFFFFull) != 0;
-            go = in && any;
-            if (go) {
-                H[s] = nh;
-                cf = satsub(max(satsub(nh, goe), cf), ge);
-            }
+            if (__builtin_amdgcn_ballot_w64(go) == 0) break;
+            const bool in = go && in_s(s);
+            const uint32_t nh = max(H[s], cf);
+            const uint64_t changed = __builtin_amdgcn_ballot_w64(in && nh != H[s]);
+            go = (changed & rowmask) != 0;
+            H[s] = go ? nh : H[s];
+            cf = usat(max(usat(nh, ugoe), cf), uge);
         }
         // column max and best cell (:245-263).  The stripe-by-stripe walk is
         // needed only in a row whose column maximum beats the task's best or
         // overflows; otherwise the column maximum alone (for the peaks) is the
         // same, so it is taken over all stripes at once.
-        int lmax = 0;
+        uint32_t lmax = 0;
 #pragma unroll
-        for (int s = 0; s < SNMAX; ++s) lmax = (live && s < sn) ? max(lmax, H[s]) : lmax;
-        const int cmax = row_max16(lmax);
+        for (int s = 0; s < SNMAX; ++s) lmax = in_s(s) ? max(lmax, H[s]) : lmax;
+        const int cmax = row_max16((int)lmax);
         int col_max = cmax;
         if (__builtin_amdgcn_ballot_w64(live && ok && (cmax > max_score || cmax > 250)) != 0) {
             col_max = 0;
 #pragma unroll
             for (int s = 0; s < SNMAX; ++s) {
-                const bool in = live && s < sn;
-                const int lm = row_max16(in ? H[s] : 0);
+                const bool in = live && in_s(s);
+                const int lm = row_max16(in ? (int)H[s] : 0);
                 if (in) {
                     if (lm > 250) ok = false;
                     if (ok) {
@@ -175,7 +184,7 @@ __device__ bool striped_pass(const LocalDev &d, const LocalParams &p, bool act, 
                 }
                 const bool better = in && ok && lm > max_score;
                 const int j = s + lane16 * sn;
-                const uint64_t hit = __ballot(better && H[s] == lm && j < qlen);
+                const uint64_t hit = __ballot(better && (int)H[s] == lm && j < qlen);
                 if (better) {
                     max_score = lm;
                     max_i = i;
@@ -218,7 +227,11 @@ __device__ bool striped_pass(const LocalDev &d, const LocalParams &p, bool act, 
     return ok;
 }
 
-template <int SNMAX>
+// Tasks of one launch have at most SN stripes (sn = ceil(qlen/16); the host
+// sorts them by sn, so most waves are uniform).  Each pass runs the exact form
+// when every task of the wave has sn == SN and the padded form otherwise; the
+// reverse pass, whose query prefix may need fewer stripes, decides again.
+template <int SN>
 __global__ __launch_bounds__(64) void local_striped_kernel(LocalDev d, LocalParams p,
                                                            const uint32_t *order, int n,
                                                            uint8_t *colmax_base, int tcap)
@@ -232,15 +245,35 @@ __global__ __launch_bounds__(64) void local_striped_kernel(LocalDev d, LocalPara
     const uint32_t qw0 = valid ? d.q_word[job] : 0, tw0 = valid ? d.t_word[job] : 0;
     const uint32_t qn0 = valid ? d.qn_word[job] : kNoN, tn0 = valid ? d.tn_word[job] : kNoN;
     uint8_t *colmax = colmax_base + (size_t)slot * tcap;
-    const bool act = valid && qlen > 0 && tlen > 0 && qlen <= 16 * SNMAX;
-    int fs, fq, ft, f2;
-    bool ok = striped_pass<SNMAX>(d, p, act, qw0, qn0, tw0, tn0, qlen, tlen, false, p.min_sub,
-                                  colmax, fs, fq, ft, f2);
-    // reverse pass on the reversed prefixes (:100-115)
-    const bool act2 = act && ok && fs > 0 && fq >= 0;
-    int rs = 0, rq = 0, rt = 0, r2 = 0;
-    bool ok2 = striped_pass<SNMAX>(d, p, act2, qw0, qn0, tw0, tn0, act2 ? fq + 1 : 0,
-                                   act2 ? ft + 1 : 0, true, 0, colmax, rs, rq, rt, r2);
+    // a task with more stripes than the launch's is left to the scalar pass
+    const bool act = valid && qlen > 0 && tlen > 0 && (qlen + 15) / 16 <= SN;
+    // pass 0: forward; pass 1: reverse on the reversed prefixes (:100-115)
+    int fs = 0, fq = 0, ft = 0, f2 = 0, rq = 0, rt = 0;
+    bool ok = false, ok2 = false;
+    bool pact = act;
+    int pql = qlen, ptl = tlen;
+#pragma nounroll
+    for (int pass = 0; pass < 2; ++pass) {
+        const bool rev = pass == 1;
+        const int msub = rev ? 0 : p.min_sub;
+        int sc, qe, te, s2;
+        bool r;
+        if (__builtin_amdgcn_ballot_w64(pact && (pql + 15) / 16 != SN) == 0)
+            r = striped_pass<SN, true>(d, p, pact, qw0, qn0, tw0, tn0, pql, ptl, rev, msub, colmax,
+                                       sc, qe, te, s2);
+        else
+            r = striped_pass<SN, false>(d, p, pact, qw0, qn0, tw0, tn0, pql, ptl, rev, msub,
+                                        colmax, sc, qe, te, s2);
+        if (!rev) {
+            fs = sc; fq = qe; ft = te; f2 = s2; ok = r;
+            pact = act && ok && fs > 0 && fq >= 0;
+            pql = pact ? fq + 1 : 0;
+            ptl = pact ? ft + 1 : 0;
+        } else {
+            rq = qe; rt = te; ok2 = r;
+        }
+    }
+    const bool act2 = pact;
     if (!valid || (threadIdx.x & 15) != 0) return;
     int32_t *o = d.out + (size_t)job * 6;
     if (act2 && ok2) {
@@ -340,19 +373,50 @@ __global__ __launch_bounds__(64) void local_scalar_kernel(LocalDev d, LocalParam
     }
 }
 
-hipError_t launch_local_striped(int snmax, const LocalDev &d, const LocalParams &p,
-                                const uint32_t *order, int n, uint8_t *colmax, int tcap,
-                                hipStream_t s)
+void group_by_stripes(const uint32_t *ids, size_t n, const int32_t *q_len, uint32_t *out,
+                      int32_t off[17])
 {
-    if (n <= 0) return hipSuccess;
-    const int grid = (n + 3) / 4;
-    switch (snmax) {
-    case 4: hipLaunchKernelGGL((local_striped_kernel<4>), dim3(grid), dim3(64), 0, s, d, p, order, n, colmax, tcap); break;
-    case 8: hipLaunchKernelGGL((local_striped_kernel<8>), dim3(grid), dim3(64), 0, s, d, p, order, n, colmax, tcap); break;
-    case 12: hipLaunchKernelGGL((local_striped_kernel<12>), dim3(grid), dim3(64), 0, s, d, p, order, n, colmax, tcap); break;
-    default: hipLaunchKernelGGL((local_striped_kernel<16>), dim3(grid), dim3(64), 0, s, d, p, order, n, colmax, tcap); break;
+    int32_t cnt[17] = {0};
+    for (size_t k = 0; k < n; ++k) ++cnt[(q_len[ids[k]] + 15) / 16];
+    off[0] = 0;
+    for (int sn = 1; sn <= 16; ++sn) off[sn] = off[sn - 1] + cnt[sn];
+    int32_t pos[17];
+    for (int sn = 1; sn <= 16; ++sn) pos[sn] = off[sn - 1];
+    for (size_t k = 0; k < n; ++k) out[pos[(q_len[ids[k]] + 15) / 16]++] = ids[k];
+}
+
+// Groups smaller than this many tasks are merged into the next larger group
+// (and run padded there): a launch of a few waves costs a whole wave's
+// latency, which a few hundred waves hide.
+static constexpr int kMinLocalLaunch = 1024;
+
+hipError_t launch_local_striped(const LocalDev &d, const LocalParams &p, const uint32_t *order,
+                                const int32_t off[17], uint8_t *colmax, int tcap, hipStream_t s)
+{
+    int top = 0;                                               // largest non-empty group
+    for (int sn = 1; sn <= 16; ++sn)
+        if (off[sn] > off[sn - 1]) top = sn;
+    int lo = 0;
+    for (int sn = 1; sn <= top; ++sn) {
+        const int n = off[sn] - lo;
+        if (off[sn] == off[sn - 1] || (n < kMinLocalLaunch && sn < top)) continue;
+        const uint32_t *o = order + lo;
+        uint8_t *cm = colmax + (size_t)lo * tcap;
+        const dim3 grid((n + 3) / 4), block(64);
+#define HIPSW_LOCAL_SN(K) \
+        case K: hipLaunchKernelGGL((local_striped_kernel<K>), grid, block, 0, s, d, p, o, n, cm, tcap); break;
+        switch (sn) {
+            HIPSW_LOCAL_SN(1) HIPSW_LOCAL_SN(2) HIPSW_LOCAL_SN(3) HIPSW_LOCAL_SN(4)
+            HIPSW_LOCAL_SN(5) HIPSW_LOCAL_SN(6) HIPSW_LOCAL_SN(7) HIPSW_LOCAL_SN(8)
+            HIPSW_LOCAL_SN(9) HIPSW_LOCAL_SN(10) HIPSW_LOCAL_SN(11) HIPSW_LOCAL_SN(12)
+            HIPSW_LOCAL_SN(13) HIPSW_LOCAL_SN(14) HIPSW_LOCAL_SN(15) HIPSW_LOCAL_SN(16)
+        }
+#undef HIPSW_LOCAL_SN
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        lo = off[sn];
     }
-    return hipGetLastError();
+    return hipSuccess;
 }
 
 hipError_t launch_local_scalar(const LocalDev &d, const LocalParams &p, const uint32_t *order,

@@ -288,13 +288,12 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
     }
     // local SW split
     std::vector<uint32_t> loc_s, loc_c;
-    int loc_qmax = 0, loc_tmax = 1;
+    int loc_tmax = 1;
     for (int k = r.n_ext; k < r.n_ext + r.n_loc; ++k) {
         if (ql[k] <= 0 || tl[k] <= 0) continue;
         loc_tmax = std::max(loc_tmax, tl[k]);
         if ((int64_t)ql[k] * srv->match_score <= 250 && ql[k] <= 256) {
             loc_s.push_back((uint32_t)k);
-            loc_qmax = std::max(loc_qmax, ql[k]);
         } else {
             loc_c.push_back((uint32_t)k);
         }
@@ -405,9 +404,11 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
         const LocalDev ld{d_seq, d_nmask, d_qw, d_tw, d_qn, d_tn, d_ql, d_tl, d_out6, d_status};
         if (ok) ok = hipMemsetAsync(d_status, 0, (size_t)n * 4, st) == hipSuccess;
         if (ok && !loc_s.empty()) {
-            up(d_order2, loc_s.data(), loc_s.size() * 4);
-            const int snmax = loc_qmax <= 64 ? 4 : loc_qmax <= 128 ? 8 : loc_qmax <= 192 ? 12 : 16;
-            if (ok) ok = launch_local_striped(snmax, ld, lp, d_order2, (int)loc_s.size(), d_colmax,
+            std::vector<uint32_t> grouped(loc_s.size());
+            int32_t grp_off[17];
+            group_by_stripes(loc_s.data(), loc_s.size(), ql.data(), grouped.data(), grp_off);
+            up(d_order2, grouped.data(), grouped.size() * 4);
+            if (ok) ok = launch_local_striped(ld, lp, d_order2, grp_off, d_colmax,
                                               lp.min_sub > 0 ? loc_tmax : 0, st) == hipSuccess;
             if (ok) ok = hipMemcpyAsync(hstv, d_status, (size_t)n * 4, hipMemcpyDeviceToHost, st) == hipSuccess;
             if (ok) ok = hipStreamSynchronize(st) == hipSuccess;
