@@ -64,7 +64,8 @@ __device__ __forceinline__ int base_at(const LocalDev &d, uint32_t w0, uint32_t 
 template <int SNMAX, bool EXACT>
 __device__ bool striped_pass(const LocalDev &d, const LocalParams &p, bool act, uint32_t qw0,
                              uint32_t qn0, uint32_t tw0, uint32_t tn0, int qlen, int tlen,
-                             bool rev, int min_sub, uint8_t *colmax, int &o_score, int &o_qend,
+                             bool rev, int min_sub, uint8_t *colmax, uint32_t *prof_lds,
+                             int &o_score, int &o_qend,
                              int &o_tend, int &o_score2)
 {
     const int lane16 = threadIdx.x & 15;
@@ -82,7 +83,6 @@ __device__ bool striped_pass(const LocalDev &d, const LocalParams &p, bool act, 
     }
     // prof[k][r] of this lane lives in LDS at prof_lds[(k*NR + r)*64 + lane]:
     // the row's target code selects a set with a per-lane address.
-    __shared__ uint32_t prof_lds[5 * 4 * 64];
     uint32_t *prof = prof_lds + (threadIdx.x & 63);
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
@@ -118,13 +118,16 @@ __device__ bool striped_pass(const LocalDev &d, const LocalParams &p, bool act, 
     tmax_w = __builtin_amdgcn_readfirstlane(tmax_w);
     const uint32_t ubias = (uint32_t)bias, ugoe = (uint32_t)goe, uge = (uint32_t)ge;
     const uint64_t rowmask = 0xFFFFull << (grp * 16);
+    // the target base is fetched one row ahead, so the global load overlaps
+    // a row of work; ti = tlen - 1 - i reads the reversed prefix
+    auto tbase = [&](int i) {
+        return (act && i < tlen) ? min(base_at(d, tw0, tn0, rev ? tlen - 1 - i : i), 4) : 0;
+    };
+    int tbn = tbase(0);
     for (int i = 0; i < tmax_w; ++i) {
         const bool live = act && ok && i < tlen;
-        int tb = 0;
-        if (live) {
-            const int ti = rev ? tlen - 1 - i : i;             // reversed prefix of length tlen
-            tb = min(base_at(d, tw0, tn0, ti), 4);
-        }
+        const int tb = live ? tbn : 0;
+        tbn = tbase(i + 1);
         uint32_t pr[NR];
 #pragma unroll
         for (int r = 0; r < NR; ++r) pr[r] = prof[(tb * NR + r) * 64];
@@ -161,35 +164,57 @@ __device__ bool striped_pass(const LocalDev &d, const LocalParams &p, bool act, 
             H[s] = go ? nh : H[s];
             cf = usat(max(usat(nh, ugoe), cf), uge);
         }
-        // column max and best cell (:245-263).  The stripe-by-stripe walk is
-        // needed only in a row whose column maximum beats the task's best or
-        // overflows; otherwise the column maximum alone (for the peaks) is the
-        // same, so it is taken over all stripes at once.
+        // column max and best cell (:245-263).  The column maximum (for the
+        // peaks) and the overflow test need only the row maximum cmax over
+        // all stripes.  The best cell changes only in a row where cmax beats
+        // the task's best; the reference's stripe walk then ends at the first
+        // stripe whose maximum is cmax, at its first lane holding cmax.  Such
+        // a cell is a real query position (a padding cell is below both the
+        // previous best and the row's real maximum: it takes a diagonal minus
+        // the bias, or a gap from an earlier row or an earlier real cell), so
+        // that lane is found directly; if it were not, the wave replays the
+        // walk itself.
         uint32_t lmax = 0;
 #pragma unroll
         for (int s = 0; s < SNMAX; ++s) lmax = in_s(s) ? max(lmax, H[s]) : lmax;
         const int cmax = row_max16((int)lmax);
         int col_max = cmax;
         if (__builtin_amdgcn_ballot_w64(live && ok && (cmax > max_score || cmax > 250)) != 0) {
-            col_max = 0;
+            // first stripe holding cmax, per lane, then over the row
+            int fs = 255;
 #pragma unroll
-            for (int s = 0; s < SNMAX; ++s) {
-                const bool in = live && in_s(s);
-                const int lm = row_max16(in ? (int)H[s] : 0);
-                if (in) {
-                    if (lm > 250) ok = false;
-                    if (ok) {
-                        col_max = max(col_max, lm);
-                    }
-                }
-                const bool better = in && ok && lm > max_score;
-                const int j = s + lane16 * sn;
-                const uint64_t hit = __ballot(better && (int)H[s] == lm && j < qlen);
-                if (better) {
-                    max_score = lm;
+            for (int s = SNMAX - 1; s >= 0; --s) fs = (in_s(s) && (int)H[s] == cmax) ? s : fs;
+            const int sst = 255 - row_max16(255 - fs);
+            const bool up = live && ok && cmax > max_score && cmax <= 250;
+            const uint64_t hit = __ballot(up && fs == sst && sst + lane16 * sn < qlen);
+            const uint32_t m16 = (uint32_t)((hit >> (grp * 16)) & 0xFFFFull);
+            if (__builtin_amdgcn_ballot_w64(up && m16 == 0) == 0) {
+                if (live && cmax > 250) ok = false;
+                if (up) {
+                    max_score = cmax;
                     max_i = i;
-                    const uint32_t m16 = (uint32_t)((hit >> (grp * 16)) & 0xFFFFull);
-                    if (m16) max_j = s + (__ffs((int)m16) - 1) * sn;
+                    max_j = sst + (__ffs((int)m16) - 1) * sn;
+                }
+            } else {
+                // the stripe walk itself (:245-263)
+                col_max = 0;
+#pragma unroll
+                for (int s = 0; s < SNMAX; ++s) {
+                    const bool in = live && in_s(s);
+                    const int lm = row_max16(in ? (int)H[s] : 0);
+                    if (in) {
+                        if (lm > 250) ok = false;
+                        if (ok) col_max = max(col_max, lm);
+                    }
+                    const bool better = in && ok && lm > max_score;
+                    const int j = s + lane16 * sn;
+                    const uint64_t hw = __ballot(better && (int)H[s] == lm && j < qlen);
+                    if (better) {
+                        max_score = lm;
+                        max_i = i;
+                        const uint32_t w16 = (uint32_t)((hw >> (grp * 16)) & 0xFFFFull);
+                        if (w16) max_j = s + (__ffs((int)w16) - 1) * sn;
+                    }
                 }
             }
         }
@@ -245,6 +270,9 @@ __global__ __launch_bounds__(64) void local_striped_kernel(LocalDev d, LocalPara
     const uint32_t qw0 = valid ? d.q_word[job] : 0, tw0 = valid ? d.t_word[job] : 0;
     const uint32_t qn0 = valid ? d.qn_word[job] : kNoN, tn0 = valid ? d.tn_word[job] : kNoN;
     uint8_t *colmax = colmax_base + (size_t)slot * tcap;
+    // striped profile (prof_lds[(k * NR + r) * 64 + lane]), one buffer for
+    // both forms of the pass
+    __shared__ uint32_t prof_lds[5 * 4 * 64];
     // a task with more stripes than the launch's is left to the scalar pass
     const bool act = valid && qlen > 0 && tlen > 0 && (qlen + 15) / 16 <= SN;
     // pass 0: forward; pass 1: reverse on the reversed prefixes (:100-115)
@@ -259,11 +287,11 @@ __global__ __launch_bounds__(64) void local_striped_kernel(LocalDev d, LocalPara
         int sc, qe, te, s2;
         bool r;
         if (__builtin_amdgcn_ballot_w64(pact && (pql + 15) / 16 != SN) == 0)
-            r = striped_pass<SN, true>(d, p, pact, qw0, qn0, tw0, tn0, pql, ptl, rev, msub, colmax,
+            r = striped_pass<SN, true>(d, p, pact, qw0, qn0, tw0, tn0, pql, ptl, rev, msub, colmax, prof_lds,
                                        sc, qe, te, s2);
         else
             r = striped_pass<SN, false>(d, p, pact, qw0, qn0, tw0, tn0, pql, ptl, rev, msub,
-                                        colmax, sc, qe, te, s2);
+                                        colmax, prof_lds, sc, qe, te, s2);
         if (!rev) {
             fs = sc; fq = qe; ft = te; f2 = s2; ok = r;
             pact = act && ok && fs > 0 && fq >= 0;

@@ -89,3 +89,10 @@ def test_scoring_variants(engine):
                ScoringParameters(matchScore=1, mismatchPenalty=1, gapOpenPenalty=1,
                                  gapExtendPenalty=1)):
         check(engine, js, sp, min_sub=15)
+
+
+def test_all_stripe_counts(engine):
+    # queries of 1..250 bases: every stripe count 1..16, small groups merged
+    # into larger ones (padded waves) and full groups (exact waves)
+    check(engine, _random(np.random.default_rng(35), 6000, 250, 320), min_sub=15)
+
