@@ -315,7 +315,7 @@ def main():
                          "64-159-base queries (10, 6 or 5; 0 = default)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (each job set runs on one "
+        print("bench.py: measures one GPU only (each job set runs on one "
               "device; a multi-GPU run would shard the job set, one engine per GPU). "
               "Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
