@@ -39,12 +39,13 @@ struct Window {
     int32_t rev;        // reverse after fetching (left extensions)
 };
 
-// One thread per window: fetch, strand-complement, optionally reverse, and
-// pack 16 bases per word into the chunk arena.
+// Sixteen lanes per window, one 16-base output word per lane in turn: fetch,
+// strand-complement, optionally reverse, and pack 16 bases per word into the
+// chunk arena.
 __global__ void fetch_windows_kernel(const uint8_t *pac, int64_t gl, const Window *win, int n,
                                      uint32_t *seq)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.x * (blockDim.x >> 4) + (threadIdx.x >> 4);
     if (k >= n) return;
     const Window w = win[k];
     if (w.len <= 0) return;
@@ -53,7 +54,7 @@ __global__ void fetch_windows_kernel(const uint8_t *pac, int64_t gl, const Windo
     const int64_t fstart = rc ? (2 * gl - w.pos) - w.len : w.pos;
     uint32_t *dst = seq + w.word;
     const int words = (w.len + 15) >> 4;
-    for (int wd = 0; wd < words; ++wd) {
+    for (int wd = threadIdx.x & 15; wd < words; wd += 16) {
         uint32_t v = 0;
         const int lo = wd * 16, hi = min(w.len, lo + 16);
         for (int i = lo; i < hi; ++i) {
@@ -358,8 +359,8 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
     up(d_win, win.data(), (size_t)n * sizeof(Window));
     if (ok) ok = launch_zero_u32(reinterpret_cast<uint32_t *>(d_cells), 4, st) == hipSuccess;
     if (ok) {
-        hipLaunchKernelGGL(fetch_windows_kernel, dim3((n + 127) / 128), dim3(128), 0, st,
-                           srv->d_pac, srv->gl, d_win, n, d_seq);
+        hipLaunchKernelGGL(fetch_windows_kernel, dim3((n + 7) / 8), dim3(128), 0, st,
+                           srv->d_pac, srv->gl, d_win, n, d_seq);   // 8 windows per block
         ok = hipGetLastError() == hipSuccess;
     }
     // --- extension ---
