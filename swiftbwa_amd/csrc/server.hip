@@ -411,7 +411,8 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
             up(d_order2, grouped.data(), grouped.size() * 4);
             if (ok) ok = launch_local_striped(ld, lp, d_order2, grp_off, d_colmax,
                                               lp.min_sub > 0 ? loc_tmax : 0, st) == hipSuccess;
-            if (ok) ok = hipMemcpyAsync(hstv, d_status, (size_t)n * 4, hipMemcpyDeviceToHost, st) == hipSuccess;
+            if (ok) ok = hipMemcpyAsync(hstv + r.n_ext, d_status + r.n_ext, (size_t)r.n_loc * 4,
+                                        hipMemcpyDeviceToHost, st) == hipSuccess;   // local rows
             if (ok) ok = hipStreamSynchronize(st) == hipSuccess;
             if (ok)
                 for (uint32_t k : loc_s)
@@ -441,8 +442,7 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
                        d_cig - (size_t)nw0 * cap, d_ncig, d_flags, d_cells + 1, cap};
         std::vector<uint32_t> active;
         for (int k = 0; k < r.n_nw; ++k) active.push_back((uint32_t)(nw0 + k));
-        std::vector<int> tries(n, 0);
-        std::vector<int32_t> score(n);
+        std::vector<int> tries(r.n_nw, 0);                     // indexed by job - nw0
         std::vector<uint32_t> order;
         std::vector<WorkItem> items;
         std::vector<NwLaunch> launches;
@@ -487,13 +487,15 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
                 ++k;
             }
             if (scr) hipFreeAsync(scr, st);
-            if (ok) ok = hipMemcpyAsync(score.data(), d_score, (size_t)n * 4, hipMemcpyDeviceToHost, st) == hipSuccess;
+            // the NW scores only, into the slot's pinned score row
+            if (ok) ok = hipMemcpyAsync(hsc + nw0, d_score + nw0, (size_t)r.n_nw * 4,
+                                        hipMemcpyDeviceToHost, st) == hipSuccess;
             if (ok) ok = hipStreamSynchronize(st) == hipSuccess;
             std::vector<uint32_t> next;
             for (uint32_t j : active) {
-                if (score[j] < r.nw_ts[j - nw0] && tries[j] < 3) {
+                if (hsc[j] < r.nw_ts[j - nw0] && tries[j - nw0] < 3) {
                     wcur[j] = wcur[j] < (1 << 20) ? wcur[j] * 2 : wcur[j];
-                    tries[j]++;
+                    tries[j - nw0]++;
                     next.push_back(j);
                 }
             }
@@ -506,11 +508,13 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
     auto down = [&](void *dst, const void *src, size_t b) {
         if (ok && b) ok = hipMemcpyAsync(dst, src, b, hipMemcpyDeviceToHost, st) == hipSuccess;
     };
-    down(out6, d_out6, (size_t)n * 6 * 4);
-    down(stv, d_status, (size_t)n * 4);
-    down(sc, d_score, (size_t)n * 4);
-    down(nc, d_ncig, (size_t)n * 4);
-    down(fl, d_flags, (size_t)n * 4);
+    // only the rows each kind reads: extension and local results, local
+    // status, NW score / op count / flags
+    down(out6, d_out6, (size_t)nw0 * 6 * 4);
+    down(stv + r.n_ext, d_status + r.n_ext, (size_t)r.n_loc * 4);
+    down(sc + nw0, d_score + nw0, (size_t)r.n_nw * 4);
+    down(nc + nw0, d_ncig + nw0, (size_t)r.n_nw * 4);
+    down(fl + nw0, d_flags + nw0, (size_t)r.n_nw * 4);
     down(cells, d_cells, sizeof cells);
     if (ok) ok = hipStreamSynchronize(st) == hipSuccess;
     // CIGARs: only the used ops of each row come back (rows hold cap ops)
