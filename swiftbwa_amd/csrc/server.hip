@@ -121,6 +121,13 @@ template <typename T> struct RawVec {
     const T &operator[](size_t i) const { return p[i]; }
 };
 
+// A view of an array inside a slot's pinned staging.
+template <typename T> struct PinArr {
+    T *p;
+    T *data() const { return p; }
+    T &operator[](size_t i) const { return p[i]; }
+};
+
 // Grows a slot's pinned staging to at least `bytes` (contents not kept).
 static bool pin_reserve(Slot &sl, size_t bytes)
 {
@@ -207,9 +214,6 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
     // Sequence arena: every query's words, then every target's words (written
     // by the window fetch on the device), then 64 zero words of padding, so
     // the host packs and uploads the queries only.
-    RawVec<uint32_t> qw(n), tw(n), qnw(n), tnw(n);
-    RawVec<int32_t> ql(n), tl(n);
-    RawVec<Window> win(n);
     RawVec<uint8_t> qhas_n(n);
     // word offsets: per-block sums in parallel, a scan over the blocks, then
     // each block's offsets in parallel.  The blocks are parallel_for's own
@@ -228,11 +232,10 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
         uint64_t sq = 0, stt = 0, sn = 0;
         for (int64_t k = k0; k < k1; ++k) {
             qhas_n[k] = has_n(r.qbytes.data() + r.q_off[k], r.q_len[k]);
-            ql[k] = r.q_len[k];
-            tl[k] = fetched_len(r.t_pos[k], r.t_len_req[k], srv->gl);
-            sq += (ql[k] + 15) / 16;
-            stt += (tl[k] + 15) / 16;
-            if (qhas_n[k]) sn += (ql[k] + 31) / 32;
+            const int q = r.q_len[k], t = fetched_len(r.t_pos[k], r.t_len_req[k], srv->gl);
+            sq += (q + 15) / 16;
+            stt += (t + 15) / 16;
+            if (qhas_n[k]) sn += (q + 31) / 32;
         }
         bq[b + 1] = sq; bt[b + 1] = stt; bn[b + 1] = sn;
     });
@@ -242,9 +245,29 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
     const uint64_t words = qwords + twords + 64;
     if (words > 0xFFFFFFF0ull) return HIPSW_E_INVALID;
     nwords += 64;
+    // Pinned staging of the chunk: the per-job arrays in the device arena's
+    // layout (one copy moves them all) | extension descriptors | result rows |
+    // query words.
+    const size_t meta_b = 4 * span<uint32_t>(n) + 5 * span<int32_t>(n) + span<Window>(n);
+    const size_t pin_pk = span<PkMeta>((size_t)r.n_ext);
+    const size_t pin_out = span<int32_t>((size_t)n * 6 + 4 * (size_t)n);
+    const size_t pin_q = span<uint32_t>(qwords);
+    if (!pin_reserve(sl, meta_b + pin_pk + pin_out + pin_q)) return HIPSW_E_DEVICE;
+    char *pc = sl.pin;
+    const PinArr<uint32_t> qw{carve<uint32_t>(pc, n)}, tw{carve<uint32_t>(pc, n)};
+    const PinArr<uint32_t> qnw{carve<uint32_t>(pc, n)}, tnw{carve<uint32_t>(pc, n)};
+    const PinArr<int32_t> ql{carve<int32_t>(pc, n)}, tl{carve<int32_t>(pc, n)};
+    const PinArr<int32_t> h0v{carve<int32_t>(pc, n)}, wv{carve<int32_t>(pc, n)}, ebv{carve<int32_t>(pc, n)};
+    const PinArr<Window> win{carve<Window>(pc, n)};
+    PkMeta *hpk = carve<PkMeta>(pc, (size_t)r.n_ext);
+    int32_t *hout6 = carve<int32_t>(pc, (size_t)n * 6 + 4 * (size_t)n);
+    uint32_t *hq = carve<uint32_t>(pc, qwords);
+    int32_t *hstv = hout6 + (size_t)n * 6, *hsc = hstv + n, *hnc = hsc + n, *hfl = hnc + n;
     blocks([&](int64_t b, int64_t k0, int64_t k1) {
         uint64_t q = bq[b], t = qwords + bt[b], m = bn[b];
         for (int64_t k = k0; k < k1; ++k) {
+            ql[k] = r.q_len[k];
+            tl[k] = fetched_len(r.t_pos[k], r.t_len_req[k], srv->gl);
             qw[k] = (uint32_t)q;
             q += (ql[k] + 15) / 16;
             tw[k] = (uint32_t)t;
@@ -255,15 +278,6 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
             win[k] = Window{r.t_pos[k], tw[k], tl[k], (int32_t)r.t_rev[k]};
         }
     });
-    // pinned staging: query words | extension descriptors | result rows
-    const size_t pin_q = ((size_t)qwords * 4 + 255) & ~(size_t)255;
-    const size_t pin_pk = ((size_t)r.n_ext * sizeof(PkMeta) + 255) & ~(size_t)255;
-    const size_t pin_out = (size_t)n * 6 * 4 + 4 * (size_t)n * 4;
-    if (!pin_reserve(sl, pin_q + pin_pk + pin_out)) return HIPSW_E_DEVICE;
-    uint32_t *hq = reinterpret_cast<uint32_t *>(sl.pin);
-    PkMeta *hpk = reinterpret_cast<PkMeta *>(sl.pin + pin_q);
-    int32_t *hout6 = reinterpret_cast<int32_t *>(sl.pin + pin_q + pin_pk);
-    int32_t *hstv = hout6 + (size_t)n * 6, *hsc = hstv + n, *hnc = hsc + n, *hfl = hnc + n;
     std::vector<uint32_t> nmask(nwords, 0);
     std::atomic<bool> packed_ok{true};
     parallel_for(n, host_threads(), [&](int64_t lo, int64_t hi) {
@@ -338,7 +352,6 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
     uint8_t *d_colmax = carve<uint8_t>(cur, colmax_b);
     unsigned long long *d_cells = carve<unsigned long long>(cur, 2);
     PkMeta *d_pk = carve<PkMeta>(cur, r.n_ext);
-    RawVec<int32_t> h0v(n), wv(n), ebv(n);
     parallel_for(n, host_threads(), [&](int64_t lo, int64_t hi) {
         for (int64_t k = lo; k < hi; ++k) {
             const bool e = k < r.n_ext;
@@ -352,11 +365,9 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
     up(d_seq, hq, (size_t)qwords * 4);
     if (ok) ok = launch_zero_u32(d_seq + qwords + twords, 64, st) == hipSuccess;
     up(d_nmask, nmask.data(), nwords * 4);
-    up(d_qw, qw.data(), (size_t)n * 4); up(d_tw, tw.data(), (size_t)n * 4);
-    up(d_qn, qnw.data(), (size_t)n * 4); up(d_tn, tnw.data(), (size_t)n * 4);
-    up(d_ql, ql.data(), (size_t)n * 4); up(d_tl, tl.data(), (size_t)n * 4);
-    up(d_h0, h0v.data(), (size_t)n * 4); up(d_w, wv.data(), (size_t)n * 4); up(d_eb, ebv.data(), (size_t)n * 4);
-    up(d_win, win.data(), (size_t)n * sizeof(Window));
+    // qw .. win: one copy of the pinned block (same carve order and spans as
+    // the arena, d_qw .. d_win)
+    up(d_qw, qw.data(), meta_b);
     if (ok) ok = launch_zero_u32(reinterpret_cast<uint32_t *>(d_cells), 4, st) == hipSuccess;
     if (ok) {
         hipLaunchKernelGGL(fetch_windows_kernel, dim3((n + 7) / 8), dim3(128), 0, st,
@@ -462,7 +473,7 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
                 if (L.global_state) gb = std::max(gb, (size_t)L.grid * L.S * 64 * (L.tier == NW_I16 ? 4 : 8));
             }
             void *scr = nullptr;
-            up(d_w, wcur.data(), (size_t)n * 4);
+            up(d_w + nw0, wcur.data() + nw0, (size_t)r.n_nw * 4);   // NW rows only
             up(d_order, order.data(), order.size() * 4);
             up(d_items, items.data(), items.size() * sizeof(WorkItem));
             if (ok) ok = hipMallocAsync(&scr, zb + gb + 512, st) == hipSuccess;
