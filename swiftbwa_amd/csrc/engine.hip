@@ -193,18 +193,39 @@ bool pack_jobs(int n, const uint8_t *queries, const uint32_t *q_off, const uint1
 
 void radix_sort_by_key(std::vector<uint32_t> &keys, std::vector<uint32_t> &idx)
 {
-    const size_t n = keys.size();
+    // Stable LSD sort, four 8-bit digits.  The jobs are cut into the chunks
+    // parallel_for hands its threads; per pass each chunk counts its digits,
+    // the per-chunk offsets follow digit by digit and chunk by chunk, and each
+    // chunk scatters its own elements in order, so the result is the serial
+    // stable sort's.
+    const int64_t n = (int64_t)keys.size();
+    const int threads = host_threads();
+    const bool par = threads > 1 && n >= 4096;
+    const int64_t chunk = par ? (n + threads - 1) / threads : std::max<int64_t>(n, 1);
+    const int nc = (int)((n + chunk - 1) / chunk);
     std::vector<uint32_t> k2(n), i2(n);
-    for (int shift = 0; shift < 32; shift += 16) {
-        std::vector<uint32_t> cnt(65537, 0);
-        for (size_t a = 0; a < n; ++a) cnt[((keys[a] >> shift) & 0xFFFFu) + 1]++;
-        for (size_t b = 1; b < cnt.size(); ++b) cnt[b] += cnt[b - 1];
-        for (size_t a = 0; a < n; ++a) {
-            const uint32_t d = (keys[a] >> shift) & 0xFFFFu;
-            const uint32_t pos = cnt[d]++;
-            k2[pos] = keys[a];
-            i2[pos] = idx[a];
-        }
+    std::vector<uint32_t> hist((size_t)std::max(nc, 1) * 256);
+    for (int shift = 0; shift < 32; shift += 8) {
+        std::fill(hist.begin(), hist.end(), 0u);
+        parallel_for(n, threads, [&](int64_t lo, int64_t hi) {
+            uint32_t *h = hist.data() + (size_t)(lo / chunk) * 256;
+            for (int64_t a = lo; a < hi; ++a) h[(keys[a] >> shift) & 0xFFu]++;
+        });
+        uint32_t run = 0;
+        for (int d = 0; d < 256; ++d)
+            for (int c = 0; c < nc; ++c) {
+                const uint32_t v = hist[(size_t)c * 256 + d];
+                hist[(size_t)c * 256 + d] = run;
+                run += v;
+            }
+        parallel_for(n, threads, [&](int64_t lo, int64_t hi) {
+            uint32_t *h = hist.data() + (size_t)(lo / chunk) * 256;
+            for (int64_t a = lo; a < hi; ++a) {
+                const uint32_t pos = h[(keys[a] >> shift) & 0xFFu]++;
+                k2[pos] = keys[a];
+                i2[pos] = idx[a];
+            }
+        });
         keys.swap(k2);
         idx.swap(i2);
     }

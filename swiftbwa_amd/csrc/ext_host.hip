@@ -79,15 +79,17 @@ size_t plan_ext_batched(const hipsw_engine *eng, int n, const int32_t *ql, const
 {
     std::vector<uint32_t> keys(n), idx(n);
     const uint64_t amax = (1ull << 27) - 1;
-    for (int k = 0; k < n; ++k) {
-        const uint32_t bin = (uint32_t)((tier_of(h0[k], ql[k], max_mat) * kClasses + class_of(ql[k])) * 2 +
-                                        (has_n && has_n[k] ? 1 : 0));
-        const uint64_t band = (uint64_t)std::min<int64_t>(ql[k], 2 * (int64_t)std::max(w[k], 0) + 1);
-        const uint64_t area = std::min<uint64_t>((uint64_t)tl[k] * band, amax);
-        const uint32_t h0c = (uint32_t)std::min(std::max(h0[k], 0), 255);
-        keys[k] = (bin << 27) | ((uint32_t)(amax - area) & ~0xFFu) | (255u - h0c);
-        idx[k] = (uint32_t)k;
-    }
+    parallel_for(n, host_threads(), [&](int64_t lo, int64_t hi) {
+        for (int64_t k = lo; k < hi; ++k) {
+            const uint32_t bin = (uint32_t)((tier_of(h0[k], ql[k], max_mat) * kClasses + class_of(ql[k])) * 2 +
+                                            (has_n && has_n[k] ? 1 : 0));
+            const uint64_t band = (uint64_t)std::min<int64_t>(ql[k], 2 * (int64_t)std::max(w[k], 0) + 1);
+            const uint64_t area = std::min<uint64_t>((uint64_t)tl[k] * band, amax);
+            const uint32_t h0c = (uint32_t)std::min(std::max(h0[k], 0), 255);
+            keys[k] = (bin << 27) | ((uint32_t)(amax - area) & ~0xFFu) | (255u - h0c);
+            idx[k] = (uint32_t)k;
+        }
+    });
     radix_sort_by_key(keys, idx);
     order = idx;
     launches.clear();
