@@ -293,6 +293,12 @@ typedef struct {
     int64_t cells;                  /* extension + NW fill cells                */
     double host_ms;                 /* part of run_ms spent packing and planning
                                        the chunk on the host                     */
+    uint32_t *nw_cigar_off;         /* NULL: nw_cigar receives rows of nw_cap ops
+                                       (the used ones, then zeros).  Otherwise
+                                       n_nw + 1 offsets: nw_cigar (same size)
+                                       receives the used ops back to back, job
+                                       k's at [off[k], off[k + 1]), and only
+                                       that prefix is written                    */
 } hipsw_chunk_result;
 
 hipsw_server *hipsw_server_create(hipsw_engine *eng, int32_t n_slots, int64_t hbm_budget,

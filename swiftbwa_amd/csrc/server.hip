@@ -552,14 +552,20 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
     hipFreeAsync(dmem, st);
     if (ok) ok = hipStreamSynchronize(st) == hipSuccess;
     if (!ok) return HIPSW_E_DEVICE;
-    r.ext_out.assign(out6, out6 + (size_t)r.n_ext * 6);
+    // result rows out of the pinned staging, in parallel
+    r.ext_out.resize((size_t)r.n_ext * 6);
+    parallel_for(r.n_ext, host_threads(), [&](int64_t lo, int64_t hi) {
+        memcpy(r.ext_out.data() + lo * 6, out6 + lo * 6, (size_t)(hi - lo) * 6 * 4);
+    });
     r.loc_out.assign((size_t)r.n_loc * 6, 0);
     r.loc_ok.assign(r.n_loc, 0);
-    for (int k = 0; k < r.n_loc; ++k) {
-        const int j = r.n_ext + k;
-        r.loc_ok[k] = stv[j] == LOCAL_OK ? 1 : 0;
-        if (r.loc_ok[k]) memcpy(&r.loc_out[(size_t)k * 6], &out6[(size_t)j * 6], 6 * 4);
-    }
+    parallel_for(r.n_loc, host_threads(), [&](int64_t lo, int64_t hi) {
+        for (int64_t k = lo; k < hi; ++k) {
+            const int64_t j = r.n_ext + k;
+            r.loc_ok[k] = stv[j] == LOCAL_OK ? 1 : 0;
+            if (r.loc_ok[k]) memcpy(&r.loc_out[(size_t)k * 6], &out6[(size_t)j * 6], 6 * 4);
+        }
+    });
     r.nw_score.assign(sc + nw0, sc + nw0 + r.n_nw);
     r.nw_ncig.assign(nc + nw0, nc + nw0 + r.n_nw);
     r.nw_flags.assign(fl + nw0, fl + nw0 + r.n_nw);
@@ -736,7 +742,14 @@ extern "C" int hipsw_server_wait(hipsw_server *srv, int64_t ticket, hipsw_chunk_
             if (res->nw_score && r->n_nw) memcpy(res->nw_score, r->nw_score.data(), (size_t)r->n_nw * 4);
             if (res->nw_n_cigar && r->n_nw) memcpy(res->nw_n_cigar, r->nw_ncig.data(), (size_t)r->n_nw * 4);
             if (res->nw_flags && r->n_nw) memcpy(res->nw_flags, r->nw_flags.data(), (size_t)r->n_nw * 4);
-            if (res->nw_cigar && r->n_nw) {
+            if (res->nw_cigar && r->n_nw && res->nw_cigar_off) {
+                // compact: the used ops back to back, with their offsets
+                memcpy(res->nw_cigar_off, r->nw_cig_off.data(), ((size_t)r->n_nw + 1) * 4);
+                const ChunkReq *rq = r.get();
+                parallel_for((int64_t)rq->nw_cig.size(), host_threads(), [&](int64_t lo, int64_t hi) {
+                    memcpy(res->nw_cigar + lo, rq->nw_cig.data() + lo, (size_t)(hi - lo) * 4);
+                });
+            } else if (res->nw_cigar && r->n_nw) {
                 // rows of cap ops: the used ones, then zeros
                 const ChunkReq *rq = r.get();
                 parallel_for(rq->n_nw, host_threads(), [&](int64_t lo, int64_t hi) {

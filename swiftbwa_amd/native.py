@@ -78,7 +78,7 @@ class HipswChunkResult(C.Structure):
     _fields_ = [("ext_out", _P), ("loc_out", _P), ("loc_ok", _P), ("nw_score", _P),
                 ("nw_cigar", _P), ("nw_n_cigar", _P), ("nw_flags", _P),
                 ("latency_ms", C.c_double), ("queue_ms", C.c_double), ("run_ms", C.c_double),
-                ("cells", C.c_int64), ("host_ms", C.c_double)]
+                ("cells", C.c_int64), ("host_ms", C.c_double), ("nw_cigar_off", _P)]
 
 
 # Every symbol include/hipsw.h declares, with (restype, argtypes).

@@ -134,24 +134,37 @@ class Server:
     def poll(self, ticket):
         return self.lib.hipsw_server_poll(self.h, int(ticket)) == native.HIPSW_OK
 
-    def wait(self, ticket):
+    def wait(self, ticket, compact=False):
+        """Results of a ticket.  CIGARs as rows of nw_cap ops ("nw_cigar"), or
+        with compact=True as the used ops back to back ("nw_cigar_ops", job k's
+        at nw_cigar_off[k] .. nw_cigar_off[k + 1])."""
         ne, nl, nn = self._keep.pop(int(ticket))
         ext = np.zeros((ne, 6), np.int32)
         loc = np.zeros((nl, 6), np.int32)
         lok = np.zeros(nl, np.int32)
         nws = np.zeros(nn, np.int32)
-        nwc = np.zeros((nn, self.nw_cap), np.uint32)
+        # compact: the buffer keeps its n_nw x nw_cap bound, but only the used
+        # prefix is written (the rest is never touched)
+        nwc = np.empty(nn * self.nw_cap, np.uint32) if compact else np.zeros((nn, self.nw_cap), np.uint32)
+        off = np.zeros(nn + 1, np.uint32) if compact else None
         nwn = np.zeros(nn, np.int32)
         nwf = np.zeros(nn, np.int32)
         r = HipswChunkResult(ext.ctypes.data, loc.ctypes.data, lok.ctypes.data, nws.ctypes.data,
-                             nwc.ctypes.data, nwn.ctypes.data, nwf.ctypes.data, 0, 0, 0, 0)
+                             nwc.ctypes.data, nwn.ctypes.data, nwf.ctypes.data, 0, 0, 0, 0, 0,
+                             off.ctypes.data if compact else None)
         rc = self.lib.hipsw_server_wait(self.h, int(ticket), C.byref(r))
         if rc:
             raise native.HipswError(rc, "chunk failed")
-        return {"ext": ext, "loc": loc, "loc_ok": lok, "nw_score": nws, "nw_cigar": nwc,
-                "nw_n_cigar": nwn, "nw_flags": nwf, "latency_ms": r.latency_ms,
-                "queue_ms": r.queue_ms, "run_ms": r.run_ms, "cells": r.cells,
-                "host_ms": r.host_ms}
+        out = {"ext": ext, "loc": loc, "loc_ok": lok, "nw_score": nws,
+               "nw_n_cigar": nwn, "nw_flags": nwf, "latency_ms": r.latency_ms,
+               "queue_ms": r.queue_ms, "run_ms": r.run_ms, "cells": r.cells,
+               "host_ms": r.host_ms}
+        if compact:
+            out["nw_cigar_ops"] = nwc[:int(off[-1])]
+            out["nw_cigar_off"] = off
+        else:
+            out["nw_cigar"] = nwc
+        return out
 
     def stats(self):
         a, b = C.c_int64(), C.c_int64()
@@ -189,7 +202,7 @@ def run_poisson(server, chunks, rate_per_s, seed=0):
     out = []
     for k in range(len(chunks)):
         ready.acquire()
-        r = server.wait(tickets[k])
+        r = server.wait(tickets[k], compact=True)
         out.append({"pairs": chunks[k]["pairs"], "latency_ms": r["latency_ms"],
                     "queue_ms": r["queue_ms"], "run_ms": r["run_ms"], "cells": r["cells"],
                     "host_ms": r["host_ms"]})

@@ -42,6 +42,16 @@ def test_alone_vs_mix_and_oracle(engine, ref):
         for k in range(sets["nw"]["n"]):
             m = min(rn[k], 512)
             assert np.array_equal(alone["nw_cigar"][k, :m], rc[k, :m]), k
+        # the compact CIGAR form: the same used ops back to back, with offsets
+        comp = srv.wait(srv.submit(target), compact=True)
+        for k in KEYS[:-1]:
+            assert np.array_equal(comp[k], alone[k]), k
+        off = comp["nw_cigar_off"]
+        assert off[0] == 0 and len(comp["nw_cigar_ops"]) == off[-1]
+        for k in range(sets["nw"]["n"]):
+            m = min(rn[k], 512)
+            assert off[k + 1] - off[k] == m, k
+            assert np.array_equal(comp["nw_cigar_ops"][off[k]:off[k + 1]], rc[k, :m]), k
         # every chunk of the mix also matches its own oracle on extension
         for o, r in zip(others[:3] + [target] + others[3:], res):
             s = orc.chunk_to_job_sets(ref, o)
