@@ -11,8 +11,12 @@
 // lane l of the row is SIMD byte-lane l of the reference's SIMD16<UInt8>, so
 // every vector op of the reference is one VALU op, `shiftRight` is a DPP
 // row_shr:1 with zero fill, and the per-stripe horizontal max is a 4-step
-// in-row reduction.  Four tasks share a wave.  The u8 saturating arithmetic is
-// done in 32-bit registers with explicit clamps.
+// in-row reduction.  Four tasks share a wave.  The u8 state lives in 32-bit
+// registers; `pmax(x, c) &- c` is one clamped unsigned subtract.  Launches are
+// per stripe count (the host sorts the tasks by sn = ceil(qlen/16)), so the
+// stripe loops are unrolled to the exact count; a wave that mixes counts runs
+// the padded form.  The best cell of a row is found from the row maximum
+// directly, with the reference's stripe walk as the fallback.
 //
 // Layout of the scalar fallback: one task per lane, h/e rows in a per-wave HBM
 // region interleaved as [column][lane].
