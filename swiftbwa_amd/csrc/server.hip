@@ -79,7 +79,7 @@ struct ChunkReq {
     bool done = false;
     // inputs (owned copies)
     int n_ext = 0, n_loc = 0, n_nw = 0;
-    std::vector<uint8_t> qbytes;                 // all queries, one arena
+    std::unique_ptr<uint8_t[]> qbytes;           // all queries, one arena (q_off / q_len)
     std::vector<uint32_t> q_off;                 // per job (ext, then loc, then nw)
     std::vector<uint16_t> q_len;
     std::vector<int64_t> t_pos;
@@ -231,7 +231,7 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
     blocks([&](int64_t b, int64_t k0, int64_t k1) {
         uint64_t sq = 0, stt = 0, sn = 0;
         for (int64_t k = k0; k < k1; ++k) {
-            qhas_n[k] = has_n(r.qbytes.data() + r.q_off[k], r.q_len[k]);
+            qhas_n[k] = has_n(r.qbytes.get() + r.q_off[k], r.q_len[k]);
             const int q = r.q_len[k], t = fetched_len(r.t_pos[k], r.t_len_req[k], srv->gl);
             sq += (q + 15) / 16;
             stt += (t + 15) / 16;
@@ -282,7 +282,7 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
     std::atomic<bool> packed_ok{true};
     parallel_for(n, host_threads(), [&](int64_t lo, int64_t hi) {
         for (int64_t k = lo; k < hi; ++k)
-            if (!pack2(r.qbytes.data() + r.q_off[k], ql[k], hq + qw[k],
+            if (!pack2(r.qbytes.get() + r.q_off[k], ql[k], hq + qw[k],
                        qnw[k] != kNoN ? nmask.data() + qnw[k] : nullptr))
                 packed_ok = false;
     });
@@ -670,24 +670,47 @@ extern "C" int64_t hipsw_server_submit(hipsw_server *srv, const hipsw_chunk *c)
     r->n_ext = c->n_ext; r->n_loc = c->n_loc; r->n_nw = c->n_nw;
     if (r->n_ext < 0 || r->n_loc < 0 || r->n_nw < 0) return HIPSW_E_INVALID;
     r->loc_min_sub = c->loc_min_sub;
-    auto add = [&](int cnt, const uint8_t *q, const uint32_t *qo, const uint16_t *qlen,
-                   const int64_t *tp, const uint16_t *tlen, const uint8_t *trev) -> bool {
-        if (cnt == 0) return true;
-        if (!q || !qo || !qlen || !tp || !tlen) return false;
-        for (int k = 0; k < cnt; ++k) {
-            r->q_off.push_back((uint32_t)r->qbytes.size());
-            r->q_len.push_back(qlen[k]);
-            r->qbytes.insert(r->qbytes.end(), q + qo[k], q + qo[k] + qlen[k]);
-            r->t_pos.push_back(tp[k]);
-            r->t_len_req.push_back(tlen[k]);
-            r->t_rev.push_back(trev ? trev[k] : 0);
-        }
-        return true;
+    // Jobs in order: extensions, local SW, NW.  The query offsets come from
+    // one scan over the lengths; every job's fields and query bytes are then
+    // copied in parallel (the byte arena is not zero-filled first).
+    struct Src {
+        int cnt;
+        const uint8_t *q; const uint32_t *qo; const uint16_t *ql;
+        const int64_t *tp; const uint16_t *tl; const uint8_t *tr;
     };
-    if (!add(c->n_ext, c->ext_q, c->ext_q_off, c->ext_q_len, c->ext_t_pos, c->ext_t_len, c->ext_t_rev) ||
-        !add(c->n_loc, c->loc_q, c->loc_q_off, c->loc_q_len, c->loc_t_pos, c->loc_t_len, nullptr) ||
-        !add(c->n_nw, c->nw_q, c->nw_q_off, c->nw_q_len, c->nw_t_pos, c->nw_t_len, nullptr))
-        return HIPSW_E_INVALID;
+    const Src src[3] = {
+        {c->n_ext, c->ext_q, c->ext_q_off, c->ext_q_len, c->ext_t_pos, c->ext_t_len, c->ext_t_rev},
+        {c->n_loc, c->loc_q, c->loc_q_off, c->loc_q_len, c->loc_t_pos, c->loc_t_len, nullptr},
+        {c->n_nw, c->nw_q, c->nw_q_off, c->nw_q_len, c->nw_t_pos, c->nw_t_len, nullptr}};
+    for (const Src &sr : src)
+        if (sr.cnt && (!sr.q || !sr.qo || !sr.ql || !sr.tp || !sr.tl)) return HIPSW_E_INVALID;
+    const int64_t n = (int64_t)r->n_ext + r->n_loc + r->n_nw;
+    r->q_off.resize(n);
+    r->q_len.resize(n);
+    r->t_pos.resize(n);
+    r->t_len_req.resize(n);
+    r->t_rev.resize(n);
+    uint64_t tot = 0;
+    for (int kind = 0, j = 0; kind < 3; ++kind)
+        for (int k = 0; k < src[kind].cnt; ++k, ++j) {
+            r->q_off[j] = (uint32_t)tot;
+            tot += src[kind].ql[k];
+        }
+    if (tot > 0xFFFFFFFFull) return HIPSW_E_INVALID;
+    r->qbytes.reset(new uint8_t[tot > 0 ? tot : 1]);
+    const int64_t b1 = r->n_ext, b2 = b1 + r->n_loc;
+    parallel_for(n, host_threads(), [&](int64_t lo, int64_t hi) {
+        for (int64_t j = lo; j < hi; ++j) {
+            const int kind = j < b1 ? 0 : j < b2 ? 1 : 2;
+            const int64_t k = j - (kind == 0 ? 0 : kind == 1 ? b1 : b2);
+            const Src &sr = src[kind];
+            r->q_len[j] = sr.ql[k];
+            r->t_pos[j] = sr.tp[k];
+            r->t_len_req[j] = sr.tl[k];
+            r->t_rev[j] = sr.tr ? sr.tr[k] : 0;
+            memcpy(r->qbytes.get() + r->q_off[j], sr.q + sr.qo[k], sr.ql[k]);
+        }
+    });
     if (c->n_ext && (!c->ext_h0 || !c->ext_w || !c->ext_eb)) return HIPSW_E_INVALID;
     for (int k = 0; k < c->n_ext; ++k)
         if (c->ext_w[k] < 0) return HIPSW_E_INVALID;   // band width must be >= 0
