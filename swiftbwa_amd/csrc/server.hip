@@ -633,6 +633,18 @@ extern "C" hipsw_server *hipsw_server_create(hipsw_engine *eng, int32_t n_slots,
     srv->match_score = match_score;
     srv->mismatch = mismatch_penalty;
     srv->nw_cap = nw_cap;
+    // Chunk arenas come from the device's default stream-ordered pool: keep
+    // up to the budget cached there, so that a chunk reuses the memory its
+    // predecessors freed instead of mapping it again.
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceGetDefaultMemPool(&pool, eng->device) == hipSuccess && pool) {
+        uint64_t keep = 0;
+        if (hipMemPoolGetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) == hipSuccess &&
+            keep < (uint64_t)hbm_budget) {
+            keep = (uint64_t)hbm_budget;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
+    }
     for (int s = 0; s < n_slots; ++s) {
         auto sl = std::make_unique<Slot>();
         if (hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking) != hipSuccess ||
