@@ -79,3 +79,32 @@ def test_budget_admission(engine, ref):
             srv.submit(serving.make_chunk(ref, 999, 20000))
     finally:
         srv.close()
+
+
+def test_queries_with_n(engine, ref):
+    # ambiguous query bases: the chunk's N masks, the extension kernels' N
+    # path and the masked local-SW / NW reads, against the oracle
+    srv = serving.Server(engine, slots=2, hbm_budget=4 << 30)
+    try:
+        srv.set_reference(serving.pack_pac(ref), len(ref))
+        ch = serving.make_chunk(ref, 400, 2000)
+        rng = np.random.default_rng(4)
+        for kind in ("ext", "loc", "nw"):
+            q = ch[kind]["q"]
+            q[rng.random(len(q)) < 0.01] = 4
+        res = srv.wait(srv.submit(ch))
+        sets = orc.chunk_to_job_sets(ref, ch)
+        ext_ref, _ = orc.ksw_extend_batch(sets["ext"], nthreads=8)
+        assert np.array_equal(res["ext"], ext_ref)
+        loc_ref, lok, _ = orc.local_batch(sets["loc"], min_sub_score=19, nthreads=8)
+        loc_ref[lok == 0] = 0
+        assert np.array_equal(res["loc_ok"], lok)
+        assert np.array_equal(res["loc"], loc_ref)
+        rs, rc, rn, _ = orc.global_batch(sets["nw"], max_retries=3, cap=512, nthreads=8)
+        assert np.array_equal(res["nw_score"], rs)
+        assert np.array_equal(res["nw_n_cigar"], rn)
+        for k in range(sets["nw"]["n"]):
+            m = min(rn[k], 512)
+            assert np.array_equal(res["nw_cigar"][k, :m], rc[k, :m]), k
+    finally:
+        srv.close()
