@@ -173,11 +173,11 @@ __device__ bool striped_pass(const LocalDev &d, const LocalParams &p, bool act, 
         // all stripes.  The best cell changes only in a row where cmax beats
         // the task's best; the reference's stripe walk then ends at the first
         // stripe whose maximum is cmax, at its first lane holding cmax.  Such
-        // a cell is a real query position (a padding cell is below both the
-        // previous best and the row's real maximum: it takes a diagonal minus
-        // the bias, or a gap from an earlier row or an earlier real cell), so
-        // that lane is found directly; if it were not, the wave replays the
-        // walk itself.
+        // a cell is a real query position when gap_oe >= 1 (a padding cell is
+        // below both the previous best and the row's real maximum: it takes a
+        // diagonal minus the bias, or a gap from an earlier row or an earlier
+        // real cell), so that lane is found directly; otherwise (free gaps)
+        // the search can fail and the wave replays the walk itself.
         uint32_t lmax = 0;
 #pragma unroll
         for (int s = 0; s < SNMAX; ++s) lmax = in_s(s) ? max(lmax, H[s]) : lmax;
