@@ -79,8 +79,9 @@ def cpu_baseline(js, sample):
 
 def autotune(eng, st, max_jobs):
     """hipsw_ext_autotune on an uploaded set: per 8-bit bin, the packed layout
-    at each LDS/HBM split of the column state and the one-job-per-lane layout,
-    timed on the bin's first max_jobs jobs."""
+    at each LDS/HBM split of the column state and the one-job-per-lane layout;
+    per 16-bit bin, the one-job-per-lane layout at each split; timed on the
+    bin's first max_jobs jobs."""
     import ctypes as C
     from swiftbwa_amd import native
     rep = (native.HipswTuneEntry * 32)()
@@ -90,9 +91,11 @@ def autotune(eng, st, max_jobs):
     classes = ["0-63", "64-159", "160-255", "256+"]
     kinds = {1: "packed (2 jobs/lane, int16x2)", 2: "one job/lane (int32)"}
     out = []
+    tiers = {0: "8-bit", 1: "16-bit", 2: "32-bit"}
     for e in rep[:nrep.value]:
-        if not out or out[-1]["length_class"] != classes[e.length_class]:
-            out.append({"tier": "8-bit", "length_class": classes[e.length_class],
+        tier = tiers.get(e.tier, str(e.tier))
+        if not out or out[-1]["length_class"] != classes[e.length_class] or out[-1]["tier"] != tier:
+            out.append({"tier": tier, "length_class": classes[e.length_class],
                         "jobs_timed": e.jobs, "candidates": []})
         c = {"kind": kinds.get(e.kind, str(e.kind)), "lds_columns": 16 * e.lds_blocks,
              "ms": round(e.ms, 3)}

@@ -115,10 +115,12 @@ int  hipsw_ext_run(hipsw_engine *eng, hipsw_ext_set *set, int32_t mode, hipsw_st
  * matrices per lane, packed int16 arithmetic) with each built split of the
  * column state between LDS and HBM (lds_blocks: 16-column blocks in LDS; fewer
  * blocks, more waves per CU), then one matrix per lane (int32 arithmetic) --
- * keeps the fastest per class in the engine (later uploads, the server and the
- * chain driver plan with it) and re-plans the set.  Results do not change, only
- * speed.  report receives up to report_cap entries, one per (bin, candidate):
- * tier (0 = 8-bit), query-length class (0: 0-63, 1: 64-159, 2: 160-255),
+ * and for each bin of the 16-bit tier the one-matrix-per-lane kernel at each
+ * built split; keeps the fastest per (tier, class) in the engine (later
+ * uploads, the server and the chain driver plan with it) and re-plans the set.
+ * Results do not change, only speed.  report receives up to report_cap
+ * entries, one per (bin, candidate): tier (0 = 8-bit, 1 = 16-bit),
+ * query-length class (0: 0-63, 1: 64-159, 2: 160-255),
  * jobs timed, kind (1 packed, 2 one per lane), lds_blocks, chosen (1 for the
  * winner of its bin) and the best of two timed runs in ms. */
 typedef struct {
@@ -132,10 +134,16 @@ int  hipsw_ext_autotune(hipsw_engine *eng, hipsw_ext_set *set, int32_t max_jobs,
  * choices without timing again.  kind 0 restores the default (packed, column
  * state in LDS up to 160 columns); kind 1 (packed) or 2 (one job per lane)
  * with lds_blocks among the splits built for the class (class 0: 4; class 1:
- * 10, 6, 5; class 2: 10, 8, 6; one job per lane: 4 / 10 / 10).  Applies to
+ * 10, 6, 5; class 2: 10, 8, 6; one job per lane the same).  Applies to
  * plans made afterwards.  HIPSW_E_INVALID for a layout not built. */
 int  hipsw_ext_set_layout(hipsw_engine *eng, int32_t length_class, int32_t kind,
                           int32_t lds_blocks);
+/* The same for the 16-bit tier, whose bins run the one-job-per-lane kernel
+ * (no reference counterpart): lds_blocks among the splits built for the class
+ * (class 0: 4; class 1: 10, 6, 5; class 2: 10, 8, 6), 0 for the default (all
+ * column state in LDS up to 160 columns).  Applies to plans made afterwards.
+ * HIPSW_E_INVALID for a layout not built. */
+int  hipsw_ext_set_layout16(hipsw_engine *eng, int32_t length_class, int32_t lds_blocks);
 int  hipsw_ext_download(hipsw_engine *eng, hipsw_ext_set *set, int32_t *out);
 void hipsw_ext_free(hipsw_engine *eng, hipsw_ext_set *set);
 

@@ -36,6 +36,7 @@ struct hipsw_engine {
     // held in LDS (0 = default) (hipsw_ext_autotune).
     int tuned_kind[4] = {0, 0, 0, 0};
     int tuned_lb[4] = {0, 0, 0, 0};
+    int tuned_lb16[4] = {0, 0, 0, 0};   // 16-bit tier: LDS blocks of the lane kernel
 
     bool fail(const char *what, hipError_t e);
     bool check(hipError_t e, const char *what);

@@ -113,7 +113,9 @@ size_t plan_ext_batched(const hipsw_engine *eng, int n, const int32_t *ql, const
                                             : eng->tuned_kind[L.cls] > 0 ? eng->tuned_kind[L.cls] : EXT_PACKED;
         L.lb = 0;
         if (L.kind != EXT_UNPACKED) {
-            const int t = L.tier == TIER_U8 ? eng->tuned_lb[L.cls] : 0;
+            const int t = L.tier == TIER_U8    ? eng->tuned_lb[L.cls]
+                          : L.tier == TIER_U16 ? eng->tuned_lb16[L.cls]
+                                               : 0;
             L.lb = t > 0 && ext_pk_lb_supported(L.kind, L.cols, t) ? t : ext_pk_default_lb(L.cols);
         }
         L.global_state = L.kind == EXT_UNPACKED &&
@@ -536,6 +538,19 @@ extern "C" int hipsw_ext_set_layout(hipsw_engine *eng, int32_t length_class, int
     return HIPSW_OK;
 }
 
+extern "C" int hipsw_ext_set_layout16(hipsw_engine *eng, int32_t length_class, int32_t lds_blocks)
+{
+    if (!eng || length_class < 0 || length_class > 2) return HIPSW_E_INVALID;
+    std::lock_guard<std::recursive_mutex> lock(eng->mu);
+    const int cols = length_class == 0 ? 64 : length_class == 1 ? 160 : 256;   // class's widest
+    if (lds_blocks != 0 && !ext_pk_lb_supported(EXT_LANE, cols, lds_blocks)) {
+        eng->last_error = "hipsw_ext_set_layout16: layout not built for this length class";
+        return HIPSW_E_INVALID;
+    }
+    eng->tuned_lb16[length_class] = lds_blocks;
+    return HIPSW_OK;
+}
+
 extern "C" int hipsw_ext_autotune(hipsw_engine *eng, hipsw_ext_set *set, int32_t max_jobs,
                                   hipsw_tune_entry *report, int32_t report_cap,
                                   int32_t *n_report)
@@ -551,15 +566,21 @@ extern "C" int hipsw_ext_autotune(hipsw_engine *eng, hipsw_ext_set *set, int32_t
         return HIPSW_E_DEVICE;
     int rc = HIPSW_OK;
     for (ExtLaunch &L : set->launches) {
-        if (L.tier != TIER_U8 || L.kind == EXT_UNPACKED) continue;
-        // candidates: the packed form at each LDS share built for this query
-        // length, then one job per lane (last: its results stay on the device)
+        if ((L.tier != TIER_U8 && L.tier != TIER_U16) || L.kind == EXT_UNPACKED) continue;
+        // candidates, 8-bit tier: the packed form at each LDS share built for
+        // this query length, then one job per lane (last: its results stay on
+        // the device); 16-bit tier: one job per lane at each LDS share
         struct Cand { int kind, lb; };
         std::vector<Cand> cands;
         int lbs[5];
-        const int nlb = ext_pk_lb_candidates(EXT_PACKED, L.cols, lbs);
-        for (int c = 0; c < nlb; ++c) cands.push_back(Cand{EXT_PACKED, lbs[c]});
-        cands.push_back(Cand{EXT_LANE, ext_pk_default_lb(L.cols)});
+        if (L.tier == TIER_U8) {
+            const int nlb = ext_pk_lb_candidates(EXT_PACKED, L.cols, lbs);
+            for (int c = 0; c < nlb; ++c) cands.push_back(Cand{EXT_PACKED, lbs[c]});
+            cands.push_back(Cand{EXT_LANE, ext_pk_default_lb(L.cols)});
+        } else {
+            const int nlb = ext_pk_lb_candidates(EXT_LANE, L.cols, lbs);
+            for (int c = 0; c < nlb; ++c) cands.push_back(Cand{EXT_LANE, lbs[c]});
+        }
         ExtLaunch S = L;
         S.job_end = std::min(L.job_end, L.job_begin + max_jobs);
         double best = 1e30;
@@ -612,8 +633,12 @@ extern "C" int hipsw_ext_autotune(hipsw_engine *eng, hipsw_ext_set *set, int32_t
             if (report && report[r].kind == win.kind && report[r].lds_blocks == win.lb) report[r].chosen = 1;
         L.kind = win.kind;
         L.lb = win.lb;
-        eng->tuned_kind[L.cls] = win.kind;
-        eng->tuned_lb[L.cls] = win.lb;
+        if (L.tier == TIER_U8) {
+            eng->tuned_kind[L.cls] = win.kind;
+            eng->tuned_lb[L.cls] = win.lb;
+        } else {
+            eng->tuned_lb16[L.cls] = win.lb;
+        }
     }
     hipFree(d_tmp);
     if (rc != HIPSW_OK) return rc;

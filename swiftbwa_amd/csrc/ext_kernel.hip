@@ -1616,7 +1616,7 @@ bool ext_pk_lb_supported(int kind, int cols, int lb)
     if (cols < 1 || cols > 256) return false;
     const int nq = ext_pk_nq(cols);
     if (lb == ext_pk_default_lb(cols)) return true;
-    if (kind != EXT_PACKED) return false;
+    if (kind != EXT_PACKED && kind != EXT_LANE) return false;
     return (nq == 10 && (lb == 5 || lb == 6)) || (nq == 16 && (lb == 6 || lb == 8));
 }
 
@@ -1659,8 +1659,12 @@ hipError_t launch_ext_persistent_ln(const ExtDev &d, const ExtParams &p, const W
     const size_t lds = ext_pk_lds_bytes(cols, lb);
     uint32_t *gs = static_cast<uint32_t *>(gscratch);
     if (nq == 4) HIPSW_EXT_SYM(ext_persistent_ln_kernel, 4, 4);
-    else if (nq == 10) HIPSW_EXT_SYM(ext_persistent_ln_kernel, 10, 10);
-    else HIPSW_EXT_SYM(ext_persistent_ln_kernel, 16, 10);
+    else if (nq == 10 && lb == 10) HIPSW_EXT_SYM(ext_persistent_ln_kernel, 10, 10);
+    else if (nq == 10 && lb == 6) HIPSW_EXT_SYM(ext_persistent_ln_kernel, 10, 6);
+    else if (nq == 10) HIPSW_EXT_SYM(ext_persistent_ln_kernel, 10, 5);
+    else if (lb == 10) HIPSW_EXT_SYM(ext_persistent_ln_kernel, 16, 10);
+    else if (lb == 8) HIPSW_EXT_SYM(ext_persistent_ln_kernel, 16, 8);
+    else HIPSW_EXT_SYM(ext_persistent_ln_kernel, 16, 6);
     return hipGetLastError();
 }
 

@@ -172,18 +172,22 @@ def test_autotune_both_layouts_exact(engine):
         nrep = C.c_int32()
         assert lib.hipsw_ext_autotune(engine.handle, st, 6000, rep, 32, C.byref(nrep)) == 0
         ents = rep[:nrep.value]
-        classes = sorted({e.length_class for e in ents})
-        assert 1 <= len(classes) <= 3
-        for c in classes:
-            es = [e for e in ents if e.length_class == c]
+        bins = sorted({(e.tier, e.length_class) for e in ents})   # (tier, class)
+        assert 1 <= len([b for b in bins if b[0] == 0]) <= 3
+        for tier, c in bins:
+            es = [e for e in ents if e.tier == tier and e.length_class == c]
             assert sum(e.chosen for e in es) == 1
-            assert es[-1].kind == 2                          # one job per lane runs last
-            assert {e.kind for e in es} == {1, 2}
             for e in es:
-                assert e.tier == 0 and e.jobs > 0 and e.ms > 0 and 4 <= e.lds_blocks <= 10
+                assert e.jobs > 0 and e.ms > 0 and 4 <= e.lds_blocks <= 10
+            if tier == 0:
+                assert es[-1].kind == 2                      # one job per lane runs last
+                assert {e.kind for e in es} == {1, 2}
+            else:                                            # 16-bit tier: one job per lane
+                assert tier == 1 and {e.kind for e in es} == {2}
         # every LDS/HBM split of the 64-159 and 160-255 classes was timed
-        assert {e.lds_blocks for e in ents if e.length_class == 1 and e.kind == 1} == {10, 6, 5}
-        assert {e.lds_blocks for e in ents if e.length_class == 2 and e.kind == 1} == {10, 8, 6}
+        t8 = [e for e in ents if e.tier == 0 and e.kind == 1]
+        assert {e.lds_blocks for e in t8 if e.length_class == 1} == {10, 6, 5}
+        assert {e.lds_blocks for e in t8 if e.length_class == 2} == {10, 8, 6}
         assert lib.hipsw_ext_download(engine.handle, st, native.ptr(out)) == 0
         assert np.array_equal(out, ref)                       # lane-layout results
         assert lib.hipsw_ext_run(engine.handle, st, native.MODE_BATCHED, C.byref(s)) == 0
@@ -224,10 +228,11 @@ def test_every_layout_exact(engine):
            "end_bonus": eb}
     lib = engine.lib
     assert lib.hipsw_ext_set_layout(engine.handle, 1, 1, 7) == native.HIPSW_E_INVALID
-    assert lib.hipsw_ext_set_layout(engine.handle, 1, 2, 5) == native.HIPSW_E_INVALID
+    assert lib.hipsw_ext_set_layout(engine.handle, 1, 2, 7) == native.HIPSW_E_INVALID
     assert lib.hipsw_ext_set_layout(engine.handle, 3, 1, 10) == native.HIPSW_E_INVALID
     combos = [((1, 4), (1, 10), (1, 10)), ((2, 4), (1, 6), (1, 8)),
-              ((1, 4), (1, 5), (1, 6)), ((2, 4), (2, 10), (2, 10))]
+              ((1, 4), (1, 5), (1, 6)), ((2, 4), (2, 10), (2, 10)),
+              ((2, 4), (2, 5), (2, 6)), ((2, 4), (2, 6), (2, 8))]
     try:
         for combo in combos:
             for cls, (kind, lb) in enumerate(combo):
@@ -236,6 +241,41 @@ def test_every_layout_exact(engine):
     finally:
         for cls in range(3):
             lib.hipsw_ext_set_layout(engine.handle, cls, 0, 0)
+
+
+def test_every_layout16_exact(engine):
+    """Every built layout of the 16-bit tier's one-job-per-lane kernel, pinned
+    per length class through hipsw_ext_set_layout16, on jobs whose seed score
+    puts them in the 16-bit tier in every class (split edges included)."""
+    rng = np.random.default_rng(23)
+    qs, ts = [], []
+    for ql in (5, 40, 63, 64, 79, 80, 81, 95, 96, 97, 120, 159, 160, 175, 176, 177, 200, 250):
+        for rep in range(30):
+            q = rng.integers(0, 4, ql).astype(np.uint8)
+            t = q.copy()
+            flip = rng.random(ql) < 0.03
+            t[flip] = (t[flip] + 1) % 4
+            qs.append(q)
+            ts.append(np.concatenate([t, rng.integers(0, 4, rep * 4).astype(np.uint8)]))
+    n = len(qs)
+    ql = np.array([len(x) for x in qs], np.uint16); tl = np.array([len(x) for x in ts], np.uint16)
+    qo = np.zeros(n, np.uint32); to = np.zeros(n, np.uint32)
+    qo[1:] = np.cumsum(ql.astype(np.int64))[:-1]; to[1:] = np.cumsum(tl.astype(np.int64))[:-1]
+    js = {"n": n, "queries": np.concatenate(qs), "q_off": qo, "q_len": ql,
+          "targets": np.concatenate(ts), "t_off": to, "t_len": tl,
+          "h0": rng.integers(200, 1000, n).astype(np.int32), "w": np.full(n, 100, np.int32),
+          "end_bonus": np.full(n, 5, np.int32)}
+    lib = engine.lib
+    assert lib.hipsw_ext_set_layout16(engine.handle, 1, 7) == native.HIPSW_E_INVALID
+    assert lib.hipsw_ext_set_layout16(engine.handle, 3, 10) == native.HIPSW_E_INVALID
+    try:
+        for combo in ((4, 10, 10), (4, 6, 8), (4, 5, 6)):
+            for cls, lb in enumerate(combo):
+                assert lib.hipsw_ext_set_layout16(engine.handle, cls, lb) == 0
+            check(engine, js, modes=(native.MODE_BATCHED,))
+    finally:
+        for cls in range(3):
+            lib.hipsw_ext_set_layout16(engine.handle, cls, 0)
 
 
 def test_size_and_tier_boundaries(engine):
