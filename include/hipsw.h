@@ -16,8 +16,15 @@
  *    QueryTargetPacker packs them (MetalDispatchHelper.swift:30-62): one byte
  *    arena each, u32 offsets, u16 lengths.  The engine packs to 2 bits + an N
  *    mask before the upload.
- *  - Every call is thread-safe; concurrent calls on one engine serialise on its
- *    stream slots.
+ *  - Every call is thread-safe.  The one-shot entry points (hipsw_banded_extend,
+ *    hipsw_global_nw, hipsw_local_sw, hipsw_cigar_generate, hipsw_extend_chains)
+ *    each lease one of the engine's stream contexts (stream, events, counters,
+ *    grow-only device buffers) for the call, so calls from different host threads
+ *    pack, upload, run and copy back concurrently; a call waits only when every
+ *    context is leased.  Calls on device-resident sets (hipsw_ext_* / hipsw_nw_*)
+ *    and the autotuner run on the engine's own stream under its mutex.
+ *  - hipsw_last_error reports the calling thread's last failure on the engine
+ *    (errno-like), so concurrent callers do not see each other's messages.
  */
 #ifndef HIPSW_H
 #define HIPSW_H
@@ -71,7 +78,8 @@ typedef struct {
  * gives (MetalSWEngine.swift:8-17); callers then stay on the CPU. */
 hipsw_engine *hipsw_engine_create(int device);
 void          hipsw_engine_destroy(hipsw_engine *eng);
-/* Text of the last error on this engine (valid until the next call on it). */
+/* Text of the calling thread's last error on this engine (valid until that
+ * thread's next call on it). */
 const char   *hipsw_last_error(const hipsw_engine *eng);
 /* MetalSWEngine.minBatchSize (MetalSWEngine.swift:39). */
 int32_t       hipsw_min_batch_size(void);
@@ -144,6 +152,17 @@ int  hipsw_ext_set_layout(hipsw_engine *eng, int32_t length_class, int32_t kind,
  * column state in LDS up to 160 columns).  Applies to plans made afterwards.
  * HIPSW_E_INVALID for a layout not built. */
 int  hipsw_ext_set_layout16(hipsw_engine *eng, int32_t length_class, int32_t lds_blocks);
+/* The batched plan of a set, one entry per bin (no reference counterpart):
+ * tier (0 8-bit, 1 16-bit, 2 32-bit), query-length class, has_n, jobs, kind
+ * (0 one job per lane, generic; 1 packed; 2 one job per lane, register query),
+ * lds_blocks (16-column blocks of column state in LDS, 0 for the generic
+ * kind), work items and grid.  Lets a host check which layout a pinned or
+ * tuned choice gave each bin. */
+typedef struct {
+    int32_t tier, length_class, has_n, jobs, kind, lds_blocks, items, grid;
+} hipsw_plan_entry;
+int  hipsw_ext_plan(hipsw_engine *eng, const hipsw_ext_set *set, hipsw_plan_entry *entries,
+                    int32_t cap, int32_t *n_entries);
 int  hipsw_ext_download(hipsw_engine *eng, hipsw_ext_set *set, int32_t *out);
 void hipsw_ext_free(hipsw_engine *eng, hipsw_ext_set *set);
 
@@ -307,6 +326,9 @@ typedef struct {
                                        receives the used ops back to back, job
                                        k's at [off[k], off[k + 1]), and only
                                        that prefix is written                    */
+    int64_t est_bytes;              /* device bytes the chunk was admitted with  */
+    int64_t dev_peak_bytes;         /* largest device footprint of its run; never
+                                       above est_bytes                           */
 } hipsw_chunk_result;
 
 hipsw_server *hipsw_server_create(hipsw_engine *eng, int32_t n_slots, int64_t hbm_budget,

@@ -2,8 +2,9 @@
 //
 // Replaces MetalSW's engine and dispatchers (Sources/MetalSW/MetalSWEngine.swift,
 // BandedSWDispatcher.swift, GlobalSWDispatcher.swift, LocalSWDispatcher.swift)
-// with a HIP engine: one stream per engine, grow-only device arenas, 2-bit
-// sequence packing on the host, and length-binned persistent launches.
+// with a HIP engine: a pool of stream contexts for concurrent one-shot calls,
+// grow-only device arenas, 2-bit sequence packing on the host, and
+// length-binned persistent launches.
 #include "hipsw.h"
 #include "hipsw_internal.h"
 #include "engine_internal.h"
@@ -15,34 +16,106 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 using namespace hipsw;
 
-bool hipsw_engine::fail(const char *what, hipError_t e)
+namespace {
+// Error text per (thread, engine): concurrent calls on one engine from
+// different threads each keep their own message, as errno does.
+thread_local std::unordered_map<const hipsw_engine *, std::string> tl_errors;
+}  // namespace
+
+void hipsw_engine::set_error(const std::string &msg) const { tl_errors[this] = msg; }
+
+bool hipsw_engine::fail(const char *what, hipError_t e) const
 {
     char buf[256];
     snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
-    last_error = buf;
+    set_error(buf);
     return false;
 }
 
-bool hipsw_engine::check(hipError_t e, const char *what)
+bool hipsw_engine::check(hipError_t e, const char *what) const
 {
     if (e == hipSuccess) return true;
     return fail(what, e);
 }
 
-void *hipsw_engine::scratch_get(size_t bytes)
+TuneTable hipsw_engine::tune_snapshot() const
 {
-    if (bytes <= scratch_bytes) return scratch;
-    if (scratch) hipFree(scratch);
-    scratch = nullptr;
-    scratch_bytes = 0;
-    if (hipMalloc(&scratch, bytes) != hipSuccess) return nullptr;
-    scratch_bytes = bytes;
-    return scratch;
+    std::lock_guard<std::mutex> lk(tune_mu);
+    return tune;
 }
+
+namespace hipsw {
+
+const char *thread_error(const hipsw_engine *eng)
+{
+    auto it = tl_errors.find(eng);
+    return it == tl_errors.end() ? "" : it->second.c_str();
+}
+
+void *DevBuf::get(size_t bytes)
+{
+    if (bytes <= cap && p) return p;
+    release();
+    if (hipMalloc(&p, std::max<size_t>(bytes, 256)) != hipSuccess) {
+        p = nullptr;
+        return nullptr;
+    }
+    cap = std::max<size_t>(bytes, 256);
+    return p;
+}
+
+void DevBuf::release()
+{
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+bool ExecCtx::init()
+{
+    return hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) == hipSuccess &&
+           hipEventCreate(&ev0) == hipSuccess && hipEventCreate(&ev1) == hipSuccess &&
+           hipMalloc((void **)&counters, 64 * sizeof(uint32_t)) == hipSuccess;
+}
+
+void ExecCtx::destroy()
+{
+    if (stream) hipStreamSynchronize(stream);
+    arena.release();
+    scratch.release();
+    nw.release();
+    if (counters) hipFree(counters);
+    if (ev0) hipEventDestroy(ev0);
+    if (ev1) hipEventDestroy(ev1);
+    if (stream) hipStreamDestroy(stream);
+    counters = nullptr;
+    ev0 = ev1 = nullptr;
+    stream = nullptr;
+}
+
+SlotLease::SlotLease(hipsw_engine *eng) : eng_(eng), idx_(-1)
+{
+    std::unique_lock<std::mutex> lk(eng->slot_mu);
+    eng->slot_cv.wait(lk, [&] { return !eng->free_slots.empty(); });
+    idx_ = eng->free_slots.back();
+    eng->free_slots.pop_back();
+}
+
+SlotLease::~SlotLease()
+{
+    {
+        std::lock_guard<std::mutex> lk(eng_->slot_mu);
+        eng_->free_slots.push_back(idx_);
+    }
+    eng_->slot_cv.notify_one();
+}
+
+}  // namespace hipsw
 
 namespace hipsw {
 
@@ -253,24 +326,19 @@ extern "C" hipsw_engine *hipsw_engine_create(int device)
     eng->lds_per_block = (int)std::min<size_t>(prop.sharedMemPerBlock, 65536);
     eng->name = prop.name;
     eng->arch = prop.gcnArchName;
-    if (hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (!eng->main.init()) {
         delete eng;
         return nullptr;
     }
-    for (auto &s : eng->side) {
-        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+    for (int k = 0; k < kEngineSlots; ++k) {
+        auto c = std::make_unique<ExecCtx>();
+        const bool ok = c->init();
+        eng->slots.push_back(std::move(c));
+        if (!ok) {
             delete eng;
             return nullptr;
         }
-    }
-    if (hipEventCreate(&eng->ev0) != hipSuccess || hipEventCreate(&eng->ev1) != hipSuccess) {
-        delete eng;
-        return nullptr;
-    }
-    for (auto &e : eng->side_ev) hipEventCreateWithFlags(&e, hipEventDisableTiming);
-    if (hipMalloc((void **)&eng->counters, 64 * sizeof(uint32_t)) != hipSuccess) {
-        delete eng;
-        return nullptr;
+        eng->free_slots.push_back(k);
     }
     return eng;
 }
@@ -278,24 +346,15 @@ extern "C" hipsw_engine *hipsw_engine_create(int device)
 hipsw_engine::~hipsw_engine()
 {
     hipSetDevice(device);
-    if (stream) hipStreamSynchronize(stream);
-    for (auto s : side)
-        if (s) hipStreamDestroy(s);
-    if (stream) hipStreamDestroy(stream);
-    if (ev0) hipEventDestroy(ev0);
-    if (ev1) hipEventDestroy(ev1);
-    for (auto e : side_ev)
-        if (e) hipEventDestroy(e);
-    if (counters) hipFree(counters);
-    if (scratch) hipFree(scratch);
-    free_nw_buffers();
+    for (auto &c : slots) c->destroy();
+    main.destroy();
 }
 
 extern "C" void hipsw_engine_destroy(hipsw_engine *eng) { delete eng; }
 
 extern "C" const char *hipsw_last_error(const hipsw_engine *eng)
 {
-    return eng ? eng->last_error.c_str() : "no engine";
+    return eng ? thread_error(eng) : "no engine";
 }
 
 extern "C" int32_t hipsw_min_batch_size(void) { return 32; }
@@ -309,11 +368,4 @@ extern "C" int hipsw_device_info(const hipsw_engine *eng, char *name, int name_c
     }
     if (n_cu) *n_cu = eng->n_cu;
     return HIPSW_OK;
-}
-
-void hipsw_engine::free_nw_buffers()
-{
-    if (nw_dev) hipFree(nw_dev);
-    nw_dev = nullptr;
-    nw_dev_bytes = 0;
 }

@@ -4,7 +4,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <condition_variable>
 #include <functional>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -12,40 +14,91 @@
 #include "hipsw.h"
 #include "hipsw_internal.h"
 
+namespace hipsw {
+
+// A grow-only device buffer (hipMalloc only when a request outgrows it).
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    void *get(size_t bytes);   // nullptr when the allocation fails
+    void release();
+};
+
+// The device resources one call runs on: a stream, two timing events, the
+// persistent kernels' work-queue counters and grow-only buffers for the call's
+// job arrays, HBM DP state and NW backpointers.  The engine owns one context
+// for its device-resident sets and the autotuner (used under the engine
+// mutex) and a pool of them for the one-shot entry points, so that calls from
+// several host threads run concurrently, each on its own stream.
+struct ExecCtx {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint32_t *counters = nullptr;   // 64 work-queue counters
+    DevBuf arena;                   // job arrays of a one-shot call
+    DevBuf scratch;                 // HBM DP state (extension hybrid layouts, local SW scalar)
+    DevBuf nw;                      // NW backpointers and HBM row state
+    bool init();
+    void destroy();
+};
+
+// The autotuner's layout choice per query-length class: kind[c] 0 = default
+// (packed form) or an ExtKind for the 8-bit tier, and the 16-column blocks of
+// column state held in LDS for the 8-bit (lb) and 16-bit (lb16) tiers, 0 =
+// default.  Planners read a copy taken under the table's own mutex.
+struct TuneTable {
+    int kind[4] = {0, 0, 0, 0};
+    int lb[4] = {0, 0, 0, 0};
+    int lb16[4] = {0, 0, 0, 0};
+};
+
+}  // namespace hipsw
+
 struct hipsw_engine {
     int device = 0;
     int n_cu = 0;
     int lds_per_cu = 0;
     int lds_per_block = 0;
-    std::string name, arch, last_error;
-    hipStream_t stream = nullptr;
-    hipStream_t side[3] = {nullptr, nullptr, nullptr};
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    hipEvent_t side_ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    uint32_t *counters = nullptr;     // 64 work-queue counters
-    void *scratch = nullptr;          // global DP state for very long queries
-    size_t scratch_bytes = 0;
+    std::string name, arch;
+    // device-resident sets (hipsw_ext_* / hipsw_nw_*) and the autotuner run on
+    // `main` while holding `mu`
+    hipsw::ExecCtx main;
     std::recursive_mutex mu;
+    // one-shot entry points lease a context from this pool; the lock covers
+    // only taking and returning a context
+    std::vector<std::unique_ptr<hipsw::ExecCtx>> slots;
+    std::vector<int> free_slots;
+    std::mutex slot_mu;
+    std::condition_variable slot_cv;
+    // autotuner table, read by every planner as a copy
+    mutable std::mutex tune_mu;
+    hipsw::TuneTable tune;
 
-    // NW arenas (grow-only), see nw_host.hip.
-    void *nw_dev = nullptr;
-    size_t nw_dev_bytes = 0;
-
-    // Autotuner choice per query-length class of the 8-bit tier: 0 = default
-    // (packed form), otherwise an ExtKind, and the 16-column blocks of state
-    // held in LDS (0 = default) (hipsw_ext_autotune).
-    int tuned_kind[4] = {0, 0, 0, 0};
-    int tuned_lb[4] = {0, 0, 0, 0};
-    int tuned_lb16[4] = {0, 0, 0, 0};   // 16-bit tier: LDS blocks of the lane kernel
-
-    bool fail(const char *what, hipError_t e);
-    bool check(hipError_t e, const char *what);
-    void *scratch_get(size_t bytes);
-    void free_nw_buffers();
+    hipsw::TuneTable tune_snapshot() const;
+    // Error text of the calling thread's last failed call on this engine.
+    void set_error(const std::string &msg) const;
+    bool fail(const char *what, hipError_t e) const;
+    bool check(hipError_t e, const char *what) const;
     ~hipsw_engine();
 };
 
 namespace hipsw {
+
+// A context leased from the engine's pool for the duration of one call.
+class SlotLease {
+public:
+    explicit SlotLease(hipsw_engine *eng);
+    ~SlotLease();
+    ExecCtx &ctx() { return *eng_->slots[idx_]; }
+    SlotLease(const SlotLease &) = delete;
+    SlotLease &operator=(const SlotLease &) = delete;
+
+private:
+    hipsw_engine *eng_;
+    int idx_;
+};
+
+constexpr int kEngineSlots = 4;
+
 
 double now_ms();
 void parallel_for(int64_t n, int threads, const std::function<void(int64_t, int64_t)> &fn);
@@ -75,25 +128,33 @@ struct ExtLaunch {
     bool global_state;
     int kind;             // ExtKind
     int cls;              // query-length class
+    int has_n;            // the bin's jobs hold an N base
     int job_begin, job_end;   // the bin's range in the batched order
     int cols;
     int lb;               // packed / one-per-lane: 16-column blocks of state in LDS
     int item_begin, item_count;
     int grid;
-    size_t scratch_off;   // bytes into the engine scratch
-    size_t scratch_bytes; // HBM DP state of the launch (0: all in LDS)
+    size_t scratch_bytes; // HBM DP state of the launch (0: all in LDS); the
+                          // bins run one after another on one stream, so all
+                          // of them use the same region from its start
 };
 
-size_t plan_ext_batched(const hipsw_engine *eng, int n, const int32_t *ql, const int32_t *tl,
-                        const int32_t *h0, const int32_t *w, const uint8_t *has_n, int max_mat,
-                        bool std_mat, std::vector<uint32_t> &order, std::vector<WorkItem> &items,
+size_t plan_ext_batched(const hipsw_engine *eng, const TuneTable &tune, int n, const int32_t *ql,
+                        const int32_t *tl, const int32_t *h0, const int32_t *w,
+                        const uint8_t *has_n, int max_mat, bool std_mat,
+                        std::vector<uint32_t> &order, std::vector<WorkItem> &items,
                         std::vector<ExtLaunch> &launches);
 // Work items and grid of a bin for its kind.
 void ext_bin_items(const hipsw_engine *eng, ExtLaunch &L, std::vector<WorkItem> &items);
-// Work items, grids and scratch regions of every bin (kinds fixed); returns
-// the scratch bytes the launches need.
+// Work items, grids and scratch sizes of every bin (kinds fixed); returns the
+// scratch bytes the launches need (the largest bin's: bins share the region).
 size_t ext_layout_launches(const hipsw_engine *eng, std::vector<ExtLaunch> &launches,
                            std::vector<WorkItem> &items);
+// Upper bound of the scratch bytes plan_ext_batched can return for jobs with
+// these query lengths and h0, whatever the tuning table says (every built
+// LDS/HBM split is considered); used for admission before planning.
+size_t ext_scratch_bound(const hipsw_engine *eng, int n, const int32_t *ql, const int32_t *h0,
+                         const uint8_t *has_n, int max_mat, bool std_mat);
 hipError_t launch_ext_bin(const ExtLaunch &L, const ExtDev &d, const ExtParams &p,
                           const uint32_t *order, const WorkItem *items, uint32_t *counter,
                           void *gscratch, hipStream_t s);
@@ -103,7 +164,8 @@ hipError_t launch_ext_bin(const ExtLaunch &L, const ExtDev &d, const ExtParams &
 struct hipsw_ext_set {
     int n = 0;
     hipsw::ExtParams params;
-    void *dmem = nullptr;            // one allocation holding every array below
+    void *dmem = nullptr;            // the arrays below; owned when own_dmem
+    bool own_dmem = false;
     size_t dmem_bytes = 0;
     hipsw::ExtDev dev;
     uint32_t *d_order = nullptr;     // batched order
@@ -118,8 +180,52 @@ struct hipsw_ext_set {
     size_t batched_scratch = 0;
     size_t items_cap = 0;            // capacity of d_items (work items)
     int64_t tier_jobs[3] = {0, 0, 0};
-    // batched form as one graph (counter resets, then the bins); rebuilt when
-    // the plan or the scratch region changes
+    // batched form as one graph (counter resets, then the bins), for sets
+    // that are run many times; rebuilt when the plan or the scratch region
+    // changes
     hipGraphExec_t graph = nullptr;
     void *graph_scratch = nullptr;
 };
+
+namespace hipsw {
+
+// Host half of an extension set: packed sequences, per-job arrays, tiers,
+// both plans and the packed-form descriptors, with the device arena's size.
+// Built without touching the device.
+struct ExtPrep {
+    int n = 0;
+    ExtParams params;
+    std::vector<uint32_t> seq, nmask, qw, tw, qnw, tnw, order;
+    std::vector<int32_t> ql, tl, h0, w, eb;
+    std::vector<WorkItem> items;
+    std::vector<PkMeta> pk;
+    hipsw_ext_set shape;             // tiers, plans and scratch sizes (no device pointers)
+    size_t arena_bytes = 0;          // device bytes of the set's arrays
+};
+// Validates, packs and plans n jobs.  Returns HIPSW_OK or a negative status
+// with the reason in err.
+int ext_prepare(const hipsw_engine *eng, const TuneTable &tune, int n, const uint8_t *queries,
+                const uint32_t *q_off, const uint16_t *q_len, const uint8_t *targets,
+                const uint32_t *t_off, const uint16_t *t_len, const int32_t *h0,
+                const int32_t *w, const int32_t *end_bonus, const hipsw_scoring *sc,
+                ExtPrep &P, std::string &err);
+// Points `set` at an arena of P.arena_bytes at dmem and queues the uploads of
+// P's arrays on stream s.
+hipError_t ext_bind_upload(const ExtPrep &P, void *dmem, hipsw_ext_set *set, hipStream_t s);
+// Queues one run of a bound set on stream s: the cell counter reset, event e0,
+// the launches (plain: one; batched: one per bin) with work-queue counters
+// `counters` (64) and HBM state at gs, event e1.
+hipError_t ext_enqueue(const hipsw_ext_set *set, int mode, void *gs, uint32_t *counters,
+                       hipStream_t s, hipEvent_t e0, hipEvent_t e1);
+
+// Error text of the calling thread's last failed call on eng.
+const char *thread_error(const hipsw_engine *eng);
+
+// One-shot run of a prepared set on context c (hipsw_banded_extend): the
+// job arrays go to `arena` (the context's grow-only arena when null), HBM
+// state to gs_fixed (the context's scratch when null), the work-queue
+// counters to `counters` (the context's when null); results into out.
+int ext_run_once(hipsw_engine *eng, ExecCtx &c, const ExtPrep &P, int mode, void *arena,
+                 void *gs_fixed, uint32_t *counters, int32_t *out, hipsw_stats *stats, double t0);
+
+}  // namespace hipsw

@@ -7,6 +7,7 @@
 #include "engine_internal.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -72,9 +73,10 @@ namespace hipsw {
 // tier (unless the engine's tuning table picks the one-job-per-lane form for
 // that class), the one-job-per-lane form for the 16-bit tier.  Returns the
 // global-state scratch bytes the launches need.
-size_t plan_ext_batched(const hipsw_engine *eng, int n, const int32_t *ql, const int32_t *tl,
-                        const int32_t *h0, const int32_t *w, const uint8_t *has_n, int max_mat,
-                        bool std_mat, std::vector<uint32_t> &order, std::vector<WorkItem> &items,
+size_t plan_ext_batched(const hipsw_engine *eng, const TuneTable &tune, int n, const int32_t *ql,
+                        const int32_t *tl, const int32_t *h0, const int32_t *w,
+                        const uint8_t *has_n, int max_mat, bool std_mat,
+                        std::vector<uint32_t> &order, std::vector<WorkItem> &items,
                         std::vector<ExtLaunch> &launches)
 {
     std::vector<uint32_t> keys(n), idx(n);
@@ -104,17 +106,18 @@ size_t plan_ext_batched(const hipsw_engine *eng, int n, const int32_t *ql, const
         const bool with_n = (bin & 1) != 0;
         L.tier = (int)(bin >> 1) / kClasses;
         L.cls = (int)(bin >> 1) % kClasses;
+        L.has_n = with_n ? 1 : 0;
         L.cols = cmax + 1;
         const bool fast = !with_n && std_mat && cmax <= 255 && L.tier != TIER_I32 &&
                           ext_pk_lds_bytes(L.cols, ext_pk_default_lb(L.cols)) <=
                               (size_t)eng->lds_per_block;
         L.kind = !fast ? EXT_UNPACKED
                        : L.tier == TIER_U16 ? EXT_LANE
-                                            : eng->tuned_kind[L.cls] > 0 ? eng->tuned_kind[L.cls] : EXT_PACKED;
+                                            : tune.kind[L.cls] > 0 ? tune.kind[L.cls] : EXT_PACKED;
         L.lb = 0;
         if (L.kind != EXT_UNPACKED) {
-            const int t = L.tier == TIER_U8    ? eng->tuned_lb[L.cls]
-                          : L.tier == TIER_U16 ? eng->tuned_lb16[L.cls]
+            const int t = L.tier == TIER_U8    ? tune.lb[L.cls]
+                          : L.tier == TIER_U16 ? tune.lb16[L.cls]
                                                : 0;
             L.lb = t > 0 && ext_pk_lb_supported(L.kind, L.cols, t) ? t : ext_pk_default_lb(L.cols);
         }
@@ -140,20 +143,72 @@ size_t ext_layout_launches(const hipsw_engine *eng, std::vector<ExtLaunch> &laun
                            std::vector<WorkItem> &items)
 {
     items.clear();
-    size_t scratch_off = 0;
+    size_t need = 0;
     for (ExtLaunch &L : launches) {
         ext_bin_items(eng, L, items);
-        L.scratch_off = scratch_off;
-        if (L.kind == EXT_UNPACKED) {
+        // one region per resident block of the launch (the kernels index it
+        // by blockIdx.x); the bins run in turn on one stream and share it
+        if (L.kind == EXT_UNPACKED)
             L.scratch_bytes = L.global_state ? (size_t)L.grid * L.cols * 64 * ext_state_bytes(L.tier) : 0;
-        } else {
-            // every wave the CU can hold gets a region
-            L.scratch_bytes = (size_t)eng->n_cu * pk_waves_per_cu(eng, L) *
-                              ext_pk_hbm_bytes_per_wave(L.cols, L.lb);
-        }
-        scratch_off += (L.scratch_bytes + 255) & ~(size_t)255;
+        else
+            L.scratch_bytes = (size_t)L.grid * ext_pk_hbm_bytes_per_wave(L.cols, L.lb);
+        need = std::max(need, (L.scratch_bytes + 255) & ~(size_t)255);
     }
-    return scratch_off;
+    return need;
+}
+
+size_t ext_scratch_bound(const hipsw_engine *eng, int n, const int32_t *ql, const int32_t *h0,
+                         const uint8_t *has_n, int max_mat, bool std_mat)
+{
+    // per (tier, class, N) bin: its job count and longest query
+    constexpr int kBins = 3 * kClasses * 2;
+    int64_t cnt[kBins] = {0};
+    int cmax[kBins] = {0};
+    for (int k = 0; k < n; ++k) {
+        const int b = (tier_of(h0[k], ql[k], max_mat) * kClasses + class_of(ql[k])) * 2 +
+                      (has_n && has_n[k] ? 1 : 0);
+        cnt[b]++;
+        cmax[b] = std::max(cmax[b], ql[k]);
+    }
+    size_t bound = 0;
+    for (int b = 0; b < kBins; ++b) {
+        if (!cnt[b]) continue;
+        ExtLaunch L{};
+        L.tier = (b >> 1) / kClasses;
+        L.cls = (b >> 1) % kClasses;
+        L.has_n = b & 1;
+        L.cols = cmax[b] + 1;
+        L.job_begin = 0;
+        L.job_end = (int)cnt[b];
+        const bool fast = !(b & 1) && std_mat && cmax[b] <= 255 && L.tier != TIER_I32 &&
+                          ext_pk_lds_bytes(L.cols, ext_pk_default_lb(L.cols)) <=
+                              (size_t)eng->lds_per_block;
+        std::vector<ExtLaunch> cand;
+        if (!fast) {
+            L.kind = EXT_UNPACKED;
+            L.lb = 0;
+            L.global_state = ext_lds_bytes(L.tier, false, L.cols) > (size_t)eng->lds_per_block;
+            cand.push_back(L);
+        } else {
+            for (int kind : {EXT_PACKED, EXT_LANE}) {
+                int lbs[5];
+                const int nlb = ext_pk_lb_candidates(kind, L.cols, lbs);
+                for (int c = 0; c < nlb; ++c) {
+                    ExtLaunch K = L;
+                    K.kind = kind;
+                    K.lb = lbs[c];
+                    K.global_state = false;
+                    cand.push_back(K);
+                }
+            }
+        }
+        for (ExtLaunch &K : cand) {
+            std::vector<WorkItem> items;
+            std::vector<ExtLaunch> one{K};
+            bound = std::max(bound, ext_layout_launches(eng, one, items));
+        }
+    }
+    return bound;
 }
 
 void ext_bin_items(const hipsw_engine *eng, ExtLaunch &L, std::vector<WorkItem> &items)
@@ -176,7 +231,7 @@ hipError_t launch_ext_bin(const ExtLaunch &L, const ExtDev &d, const ExtParams &
                           const uint32_t *order, const WorkItem *items, uint32_t *counter,
                           void *gscratch, hipStream_t s)
 {
-    void *gs = L.scratch_bytes ? (char *)gscratch + L.scratch_off : nullptr;
+    void *gs = L.scratch_bytes ? gscratch : nullptr;   // every bin starts at the region's base
     switch (L.kind) {
     case EXT_PACKED:
         return launch_ext_persistent_pk(d, p, items + L.item_begin, L.item_count, counter, L.cols,
@@ -192,40 +247,38 @@ hipError_t launch_ext_bin(const ExtLaunch &L, const ExtDev &d, const ExtParams &
 
 }  // namespace hipsw
 
-extern "C" hipsw_ext_set *hipsw_ext_upload(hipsw_engine *eng, int32_t n,
-                                           const uint8_t *queries, const uint32_t *q_off,
-                                           const uint16_t *q_len, const uint8_t *targets,
-                                           const uint32_t *t_off, const uint16_t *t_len,
-                                           const int32_t *h0, const int32_t *w,
-                                           const int32_t *end_bonus, const hipsw_scoring *sc)
+namespace hipsw {
+
+int ext_prepare(const hipsw_engine *eng, const TuneTable &tune, int n, const uint8_t *queries,
+                const uint32_t *q_off, const uint16_t *q_len, const uint8_t *targets,
+                const uint32_t *t_off, const uint16_t *t_len, const int32_t *h0,
+                const int32_t *w, const int32_t *end_bonus, const hipsw_scoring *sc,
+                ExtPrep &P, std::string &err)
 {
-    if (!eng) return nullptr;
-    std::lock_guard<std::recursive_mutex> lock(eng->mu);
     if (n < 0 || (n > 0 && (!queries || !q_off || !q_len || !targets || !t_off || !t_len ||
                             !h0 || !w || !end_bonus))) {
-        eng->last_error = "hipsw_ext_upload: null argument";
-        return nullptr;
+        err = "null argument";
+        return HIPSW_E_INVALID;
     }
     if (!scoring_valid(sc)) {
-        eng->last_error = "hipsw_ext_upload: gap extension penalties must be > 0";
-        return nullptr;
+        err = "gap extension penalties must be > 0";
+        return HIPSW_E_INVALID;
     }
     for (int k = 0; k < n; ++k) {
         // a negative band puts the row's end below column 0 (BandedSWScalar.swift:97-99)
         if (w[k] < 0) {
-            eng->last_error = "hipsw_ext_upload: band width w must be >= 0";
-            return nullptr;
+            err = "band width w must be >= 0";
+            return HIPSW_E_INVALID;
         }
     }
-    hipSetDevice(eng->device);
-    auto *set = new hipsw_ext_set();
-    set->n = n;
-    set->params = make_params(sc);
-    const int max_mat = set->params.max_mat;
+    hipsw_ext_set &S = P.shape;
+    P.n = S.n = n;
+    P.params = S.params = make_params(sc);
+    const int max_mat = P.params.max_mat;
     const int threads = host_threads();
 
     // Word layout: every query and target starts on a 16-base word boundary.
-    std::vector<uint32_t> qw(n), tw(n), qnw(n), tnw(n);
+    P.qw.assign(n, 0); P.tw.assign(n, 0); P.qnw.assign(n, 0); P.tnw.assign(n, 0);
     std::vector<uint8_t> qn_flag(n), tn_flag(n);
     parallel_for(n, threads, [&](int64_t lo, int64_t hi) {
         for (int64_t k = lo; k < hi; ++k) {
@@ -235,91 +288,104 @@ extern "C" hipsw_ext_set *hipsw_ext_upload(hipsw_engine *eng, int32_t n,
     });
     uint64_t words = 0, nwords = 0;
     for (int k = 0; k < n; ++k) {
-        qw[k] = (uint32_t)words;
+        P.qw[k] = (uint32_t)words;
         words += (q_len[k] + 15) / 16;
-        tw[k] = (uint32_t)words;
+        P.tw[k] = (uint32_t)words;
         words += (t_len[k] + 15) / 16;
-        qnw[k] = qn_flag[k] ? (uint32_t)nwords : kNoN;
+        P.qnw[k] = qn_flag[k] ? (uint32_t)nwords : kNoN;
         if (qn_flag[k]) nwords += (q_len[k] + 31) / 32;
-        tnw[k] = tn_flag[k] ? (uint32_t)nwords : kNoN;
+        P.tnw[k] = tn_flag[k] ? (uint32_t)nwords : kNoN;
         if (tn_flag[k]) nwords += (t_len[k] + 31) / 32;
     }
     if (words + 64 > 0xFFFFFFF0ull) {
-        eng->last_error = "hipsw_ext_upload: job set exceeds 2^32 sequence words";
-        delete set;
-        return nullptr;
+        err = "job set exceeds 2^32 sequence words";
+        return HIPSW_E_UNSUPPORTED;
     }
     words += 64;    // tail padding: whole-word reads past the last sequence stay in bounds
     nwords += 64;
-    std::vector<uint32_t> seq(words, 0), nmask(nwords, 0);
-    std::vector<int32_t> ql(n), tl(n), vh0(n), vw(n), veb(n);
-    bool bad = false;
+    P.seq.assign(words, 0);
+    P.nmask.assign(nwords, 0);
+    P.ql.resize(n); P.tl.resize(n); P.h0.resize(n); P.w.resize(n); P.eb.resize(n);
+    std::atomic<bool> bad{false};
     parallel_for(n, threads, [&](int64_t lo, int64_t hi) {
         for (int64_t k = lo; k < hi; ++k) {
-            ql[k] = q_len[k];
-            tl[k] = t_len[k];
-            vh0[k] = h0[k];
-            vw[k] = w[k];
-            veb[k] = end_bonus[k];
-            if (!pack2(queries + q_off[k], q_len[k], seq.data() + qw[k],
-                       qn_flag[k] ? nmask.data() + qnw[k] : nullptr))
+            P.ql[k] = q_len[k];
+            P.tl[k] = t_len[k];
+            P.h0[k] = h0[k];
+            P.w[k] = w[k];
+            P.eb[k] = end_bonus[k];
+            if (!pack2(queries + q_off[k], q_len[k], P.seq.data() + P.qw[k],
+                       qn_flag[k] ? P.nmask.data() + P.qnw[k] : nullptr))
                 bad = true;
-            if (!pack2(targets + t_off[k], t_len[k], seq.data() + tw[k],
-                       tn_flag[k] ? nmask.data() + tnw[k] : nullptr))
+            if (!pack2(targets + t_off[k], t_len[k], P.seq.data() + P.tw[k],
+                       tn_flag[k] ? P.nmask.data() + P.tnw[k] : nullptr))
                 bad = true;
         }
     });
     if (bad) {
-        eng->last_error = "hipsw_ext_upload: base code > 4";
-        delete set;
-        return nullptr;
+        err = "base code > 4";
+        return HIPSW_E_INVALID;
     }
 
     // Exact score tier per job.
     int plain_tier = TIER_U8, qmax = 0;
     for (int k = 0; k < n; ++k) {
-        const int tr = tier_of(vh0[k], ql[k], max_mat);
-        set->tier_jobs[tr]++;
+        const int tr = tier_of(P.h0[k], P.ql[k], max_mat);
+        S.tier_jobs[tr]++;
         plain_tier = std::max(plain_tier, tr);
-        qmax = std::max(qmax, ql[k]);
+        qmax = std::max(qmax, P.ql[k]);
     }
 
     // Plain form: input order, one tier and one LDS size for the whole launch.
-    set->plain_tier = plain_tier;
-    set->plain_cols = qmax + 1;
-    const size_t plain_lds = ext_lds_bytes(plain_tier, false, set->plain_cols);
-    set->plain_global = plain_lds > (size_t)eng->lds_per_block;
+    S.plain_tier = plain_tier;
+    S.plain_cols = qmax + 1;
+    const size_t plain_lds = ext_lds_bytes(plain_tier, false, S.plain_cols);
+    S.plain_global = plain_lds > (size_t)eng->lds_per_block;
     const int groups = (n + 63) / 64;
-    if (set->plain_global) {
-        set->plain_grid = std::max(1, std::min(groups, eng->n_cu * 8));
-        set->plain_scratch = (size_t)set->plain_grid * set->plain_cols * 64 * ext_state_bytes(plain_tier);
+    if (S.plain_global) {
+        S.plain_grid = std::max(1, std::min(groups, eng->n_cu * 8));
+        S.plain_scratch = (size_t)S.plain_grid * S.plain_cols * 64 * ext_state_bytes(plain_tier);
     } else {
-        set->plain_grid = std::max(1, groups);
+        S.plain_grid = std::max(1, groups);
     }
 
     // Batched form (see plan_ext_batched).
-    std::vector<uint32_t> order;
-    std::vector<WorkItem> items;
     std::vector<uint8_t> any_n(n);
     for (int k = 0; k < n; ++k) any_n[k] = qn_flag[k] | tn_flag[k];
-    set->batched_scratch = plan_ext_batched(eng, n, ql.data(), tl.data(), vh0.data(), vw.data(),
-                                            any_n.data(), max_mat, set->params.std_mat != 0,
-                                            order, items, set->launches);
+    S.batched_scratch = plan_ext_batched(eng, tune, n, P.ql.data(), P.tl.data(), P.h0.data(),
+                                         P.w.data(), any_n.data(), max_mat,
+                                         P.params.std_mat != 0, P.order, P.items, S.launches);
     // Work items of any later re-plan (hipsw_ext_autotune) fit: 64 jobs or more per item.
-    const size_t items_cap = (size_t)n / 64 + set->launches.size() + 1;
-    set->items_cap = items_cap;
-    // One device allocation for the whole set.
-    const size_t bytes = span<uint32_t>(words) + span<uint32_t>(nwords) + 4 * span<uint32_t>(n) +
-                         6 * span<int32_t>(n) + span<int32_t>((size_t)n * 6) +
-                         span<uint32_t>(n) + span<WorkItem>(items_cap) + span<PkMeta>(n);
-    if (!eng->check(hipMalloc(&set->dmem, bytes), "hipMalloc(ext set)")) {
-        delete set;
-        return nullptr;
-    }
-    set->dmem_bytes = bytes;
-    char *cur = reinterpret_cast<char *>(set->dmem);
-    uint32_t *d_seq = carve<uint32_t>(cur, words);
-    uint32_t *d_nmask = carve<uint32_t>(cur, nwords);
+    S.items_cap = (size_t)n / 64 + S.launches.size() + 1;
+    P.pk.resize(n);
+    build_pk_meta(n, P.order.data(), P.qw.data(), P.tw.data(), P.qnw.data(), P.tnw.data(),
+                  P.ql.data(), P.tl.data(), P.h0.data(), P.w.data(), P.eb.data(), P.pk.data());
+    P.arena_bytes = span<uint32_t>(words) + span<uint32_t>(nwords) + 4 * span<uint32_t>(n) +
+                    6 * span<int32_t>(n) + span<int32_t>((size_t)n * 6) + span<uint32_t>(n) +
+                    span<WorkItem>(S.items_cap) + span<PkMeta>(n);
+    return HIPSW_OK;
+}
+
+hipError_t ext_bind_upload(const ExtPrep &P, void *dmem, hipsw_ext_set *set, hipStream_t st)
+{
+    const hipsw_ext_set &S = P.shape;
+    const int n = P.n;
+    set->n = n;
+    set->params = P.params;
+    set->plain_tier = S.plain_tier;
+    set->plain_cols = S.plain_cols;
+    set->plain_global = S.plain_global;
+    set->plain_grid = S.plain_grid;
+    set->plain_scratch = S.plain_scratch;
+    set->launches = S.launches;
+    set->batched_scratch = S.batched_scratch;
+    set->items_cap = S.items_cap;
+    for (int k = 0; k < 3; ++k) set->tier_jobs[k] = S.tier_jobs[k];
+    set->dmem = dmem;
+    set->dmem_bytes = P.arena_bytes;
+    char *cur = reinterpret_cast<char *>(dmem);
+    uint32_t *d_seq = carve<uint32_t>(cur, P.seq.size());
+    uint32_t *d_nmask = carve<uint32_t>(cur, P.nmask.size());
     uint32_t *d_qw = carve<uint32_t>(cur, n);
     uint32_t *d_tw = carve<uint32_t>(cur, n);
     uint32_t *d_qn = carve<uint32_t>(cur, n);
@@ -329,63 +395,103 @@ extern "C" hipsw_ext_set *hipsw_ext_upload(hipsw_engine *eng, int32_t n,
     int32_t *d_h0 = carve<int32_t>(cur, n);
     int32_t *d_w = carve<int32_t>(cur, n);
     int32_t *d_eb = carve<int32_t>(cur, n);
-    int32_t *d_unused = carve<int32_t>(cur, std::max(n, 2));   // 8-byte cell counter
+    int32_t *d_cells = carve<int32_t>(cur, std::max(n, 2));   // 8-byte cell counter
     int32_t *d_out = carve<int32_t>(cur, (size_t)n * 6);
     set->d_order = carve<uint32_t>(cur, n);
-    set->d_items = carve<WorkItem>(cur, items_cap);
+    set->d_items = carve<WorkItem>(cur, S.items_cap);
     PkMeta *d_pk = carve<PkMeta>(cur, n);
-    std::vector<PkMeta> pk(n);
-    build_pk_meta(n, order.data(), qw.data(), tw.data(), qnw.data(), tnw.data(), ql.data(), tl.data(),
-                  vh0.data(), vw.data(), veb.data(), pk.data());
-    bool ok = true;
+    set->dev = ExtDev{d_seq, d_nmask, d_qw, d_tw, d_qn, d_tn, d_ql, d_tl, d_h0, d_w, d_eb, d_out,
+                      reinterpret_cast<unsigned long long *>(d_cells), d_pk};
+    hipError_t e = hipSuccess;
     auto up = [&](void *dst, const void *src, size_t b) {
-        if (ok && b) ok = eng->check(hipMemcpyAsync(dst, src, b, hipMemcpyHostToDevice, eng->stream), "upload");
+        if (e == hipSuccess && b) e = hipMemcpyAsync(dst, src, b, hipMemcpyHostToDevice, st);
     };
-    up(d_seq, seq.data(), words * 4);
-    up(d_nmask, nmask.data(), nwords * 4);
-    up(d_qw, qw.data(), (size_t)n * 4);
-    up(d_tw, tw.data(), (size_t)n * 4);
-    up(d_qn, qnw.data(), (size_t)n * 4);
-    up(d_tn, tnw.data(), (size_t)n * 4);
-    up(d_ql, ql.data(), (size_t)n * 4);
-    up(d_tl, tl.data(), (size_t)n * 4);
-    up(d_h0, vh0.data(), (size_t)n * 4);
-    up(d_w, vw.data(), (size_t)n * 4);
-    up(d_eb, veb.data(), (size_t)n * 4);
-    up(set->d_order, order.data(), (size_t)n * 4);
-    up(set->d_items, items.data(), items.size() * sizeof(WorkItem));
-    up(d_pk, pk.data(), (size_t)n * sizeof(PkMeta));
-    if (ok) ok = eng->check(hipStreamSynchronize(eng->stream), "upload sync");
-    if (!ok) {
-        hipFree(set->dmem);
+    up(d_seq, P.seq.data(), P.seq.size() * 4);
+    up(d_nmask, P.nmask.data(), P.nmask.size() * 4);
+    up(d_qw, P.qw.data(), (size_t)n * 4);
+    up(d_tw, P.tw.data(), (size_t)n * 4);
+    up(d_qn, P.qnw.data(), (size_t)n * 4);
+    up(d_tn, P.tnw.data(), (size_t)n * 4);
+    up(d_ql, P.ql.data(), (size_t)n * 4);
+    up(d_tl, P.tl.data(), (size_t)n * 4);
+    up(d_h0, P.h0.data(), (size_t)n * 4);
+    up(d_w, P.w.data(), (size_t)n * 4);
+    up(d_eb, P.eb.data(), (size_t)n * 4);
+    up(set->d_order, P.order.data(), (size_t)n * 4);
+    up(set->d_items, P.items.data(), P.items.size() * sizeof(WorkItem));
+    up(d_pk, P.pk.data(), (size_t)n * sizeof(PkMeta));
+    return e;
+}
+
+hipError_t ext_enqueue(const hipsw_ext_set *set, int mode, void *gs, uint32_t *counters,
+                       hipStream_t st, hipEvent_t e0, hipEvent_t e1)
+{
+    hipError_t e = launch_zero_u32(reinterpret_cast<uint32_t *>(set->dev.cells), 2, st);
+    if (e == hipSuccess && mode == HIPSW_MODE_BATCHED) e = launch_zero_u32(counters, 64, st);
+    if (e == hipSuccess && e0) e = hipEventRecord(e0, st);
+    if (e != hipSuccess) return e;
+    if (mode == HIPSW_MODE_PLAIN) {
+        e = launch_ext_plain(set->plain_tier, set->plain_global, set->dev, set->params, nullptr,
+                             set->n, set->plain_cols, set->plain_grid, gs, st);
+    } else {
+        for (size_t b = 0; e == hipSuccess && b < set->launches.size(); ++b)
+            e = launch_ext_bin(set->launches[b], set->dev, set->params, set->d_order, set->d_items,
+                               counters + b, gs, st);
+    }
+    if (e == hipSuccess && e1) e = hipEventRecord(e1, st);
+    return e;
+}
+
+}  // namespace hipsw
+
+extern "C" hipsw_ext_set *hipsw_ext_upload(hipsw_engine *eng, int32_t n,
+                                           const uint8_t *queries, const uint32_t *q_off,
+                                           const uint16_t *q_len, const uint8_t *targets,
+                                           const uint32_t *t_off, const uint16_t *t_len,
+                                           const int32_t *h0, const int32_t *w,
+                                           const int32_t *end_bonus, const hipsw_scoring *sc)
+{
+    if (!eng) return nullptr;
+    ExtPrep P;
+    std::string err;
+    if (ext_prepare(eng, eng->tune_snapshot(), n, queries, q_off, q_len, targets, t_off, t_len,
+                    h0, w, end_bonus, sc, P, err) != HIPSW_OK) {
+        eng->set_error("hipsw_ext_upload: " + err);
+        return nullptr;
+    }
+    std::lock_guard<std::recursive_mutex> lock(eng->mu);
+    hipSetDevice(eng->device);
+    auto *set = new hipsw_ext_set();
+    void *dmem = nullptr;
+    if (!eng->check(hipMalloc(&dmem, P.arena_bytes), "hipMalloc(ext set)")) {
         delete set;
         return nullptr;
     }
-    set->dev = ExtDev{d_seq, d_nmask, d_qw, d_tw, d_qn, d_tn, d_ql, d_tl, d_h0, d_w, d_eb, d_out,
-                      reinterpret_cast<unsigned long long *>(d_unused), d_pk};
+    set->own_dmem = true;
+    if (!eng->check(ext_bind_upload(P, dmem, set, eng->main.stream), "upload") ||
+        !eng->check(hipStreamSynchronize(eng->main.stream), "upload sync")) {
+        hipFree(dmem);
+        delete set;
+        return nullptr;
+    }
     return set;
 }
 
 // The batched form of a set as one graph: the counter resets, then the bins
-// back to back on the engine stream.  Captured once per plan (and again when
-// the engine's scratch region has moved); replays cost one launch per run.
-// The work-queue counters are reset by a kernel inside the graph (no memset
-// node).  Running the bins on concurrent streams instead, so that one bin's
-// tail overlaps the next one's start, measured slower on the uneven 4M set
-// (26.8 against 25.5 ms): every persistent launch already fills the device.
+// back to back on the engine's main stream.  Captured once per plan (and again
+// when the scratch region has moved); replays cost one launch per run.  The
+// work-queue counters are reset by a kernel inside the graph (no memset node).
+// Running the bins on concurrent streams instead, so that one bin's tail
+// overlaps the next one's start, measured slower on the uneven 4M set (26.8
+// against 25.5 ms): every persistent launch already fills the device.
 static bool build_batched_graph(hipsw_engine *eng, hipsw_ext_set *set, void *gs)
 {
     hipGraph_t g = nullptr;
-    bool ok = eng->check(hipStreamBeginCapture(eng->stream, hipStreamCaptureModeThreadLocal),
-                         "begin capture");
-    if (ok) ok = eng->check(launch_zero_u32(reinterpret_cast<uint32_t *>(set->dev.cells), 2, eng->stream),
-                            "capture zero");
-    if (ok) ok = eng->check(launch_zero_u32(eng->counters, 64, eng->stream), "capture zero");
-    for (size_t b = 0; ok && b < set->launches.size(); ++b)
-        ok = eng->check(launch_ext_bin(set->launches[b], set->dev, set->params, set->d_order,
-                                       set->d_items, eng->counters + b, gs, eng->stream),
-                        "capture ext_persistent");
-    const hipError_t ee = hipStreamEndCapture(eng->stream, &g);
+    hipStream_t st = eng->main.stream;
+    bool ok = eng->check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (ok) ok = eng->check(ext_enqueue(set, HIPSW_MODE_BATCHED, gs, eng->main.counters, st, nullptr,
+                                        nullptr), "capture ext_persistent");
+    const hipError_t ee = hipStreamEndCapture(st, &g);
     ok = ok && eng->check(ee, "end capture");
     if (ok) ok = eng->check(hipGraphInstantiate(&set->graph, g, nullptr, nullptr, 0), "instantiate");
     if (g) hipGraphDestroy(g);
@@ -397,6 +503,20 @@ static bool build_batched_graph(hipsw_engine *eng, hipsw_ext_set *set, void *gs)
     return true;
 }
 
+static void fill_ext_stats(hipsw_stats *stats, const hipsw_ext_set *set, float ms, double t0,
+                           int64_t launches, unsigned long long cells)
+{
+    if (!stats) return;
+    memset(stats, 0, sizeof *stats);
+    stats->kernel_ms = ms;
+    stats->total_ms = now_ms() - t0;
+    stats->n_jobs = set->n;
+    stats->launches = launches;
+    stats->promoted = set->tier_jobs[TIER_I32];
+    for (int k = 0; k < 3; ++k) stats->tier_jobs[k] = set->tier_jobs[k];
+    stats->cells = (int64_t)cells;
+}
+
 extern "C" int hipsw_ext_run(hipsw_engine *eng, hipsw_ext_set *set, int32_t mode,
                              hipsw_stats *stats)
 {
@@ -404,31 +524,25 @@ extern "C" int hipsw_ext_run(hipsw_engine *eng, hipsw_ext_set *set, int32_t mode
     if (mode != HIPSW_MODE_PLAIN && mode != HIPSW_MODE_BATCHED) return HIPSW_E_INVALID;
     std::lock_guard<std::recursive_mutex> lock(eng->mu);
     hipSetDevice(eng->device);
+    ExecCtx &c = eng->main;
     const double t0 = now_ms();
     int64_t launches = 0;
+    float ms = 0.f;
+    unsigned long long cells = 0;
     if (set->n > 0) {
+        if (mode == HIPSW_MODE_BATCHED && set->launches.size() > 64) {
+            eng->set_error("hipsw_ext_run: too many bins");
+            return HIPSW_E_UNSUPPORTED;
+        }
         const size_t need = mode == HIPSW_MODE_PLAIN ? (set->plain_global ? set->plain_scratch : 0)
                                                      : set->batched_scratch;
         void *gs = nullptr;
-        if (need) {
-            gs = eng->scratch_get(need);
-            if (!gs) {
-                eng->last_error = "hipsw_ext_run: scratch allocation failed";
-                return HIPSW_E_DEVICE;
-            }
-        }
-        if (mode == HIPSW_MODE_BATCHED && set->launches.size() > 64) {
-            eng->last_error = "hipsw_ext_run: too many bins";
-            return HIPSW_E_UNSUPPORTED;
-        }
-        if (!eng->check(launch_zero_u32(reinterpret_cast<uint32_t *>(set->dev.cells), 2, eng->stream),
-                        "launch zero"))
+        if (need && !(gs = c.scratch.get(need))) {
+            eng->set_error("hipsw_ext_run: scratch allocation failed");
             return HIPSW_E_DEVICE;
-        if (!eng->check(hipEventRecord(eng->ev0, eng->stream), "event")) return HIPSW_E_DEVICE;
+        }
         if (mode == HIPSW_MODE_PLAIN) {
-            if (!eng->check(launch_ext_plain(set->plain_tier, set->plain_global, set->dev,
-                                             set->params, nullptr, set->n, set->plain_cols,
-                                             set->plain_grid, gs, eng->stream),
+            if (!eng->check(ext_enqueue(set, mode, gs, c.counters, c.stream, c.ev0, c.ev1),
                             "launch ext_plain"))
                 return HIPSW_E_DEVICE;
             launches = 1;
@@ -438,27 +552,19 @@ extern "C" int hipsw_ext_run(hipsw_engine *eng, hipsw_ext_set *set, int32_t mode
                 set->graph = nullptr;
             }
             if (!set->graph && !build_batched_graph(eng, set, gs)) return HIPSW_E_DEVICE;
-            if (!eng->check(hipGraphLaunch(set->graph, eng->stream), "graph launch"))
+            if (!eng->check(hipEventRecord(c.ev0, c.stream), "event") ||
+                !eng->check(hipGraphLaunch(set->graph, c.stream), "graph launch") ||
+                !eng->check(hipEventRecord(c.ev1, c.stream), "event"))
                 return HIPSW_E_DEVICE;
             launches = (int64_t)set->launches.size();
         }
-        if (!eng->check(hipEventRecord(eng->ev1, eng->stream), "event")) return HIPSW_E_DEVICE;
-        if (!eng->check(hipEventSynchronize(eng->ev1), "kernel")) return HIPSW_E_DEVICE;
+        if (!eng->check(hipEventSynchronize(c.ev1), "kernel")) return HIPSW_E_DEVICE;
+        hipEventElapsedTime(&ms, c.ev0, c.ev1);
+        if (stats && !eng->check(hipMemcpy(&cells, set->dev.cells, sizeof cells, hipMemcpyDeviceToHost),
+                                 "cells"))
+            return HIPSW_E_DEVICE;
     }
-    if (stats) {
-        float ms = 0.f;
-        if (set->n > 0) hipEventElapsedTime(&ms, eng->ev0, eng->ev1);
-        stats->kernel_ms = ms;
-        stats->total_ms = now_ms() - t0;
-        stats->n_jobs = set->n;
-        stats->launches = launches;
-        stats->promoted = set->tier_jobs[TIER_I32];
-        for (int k = 0; k < 3; ++k) stats->tier_jobs[k] = set->tier_jobs[k];
-        unsigned long long c = 0;
-        if (set->n > 0)
-            hipMemcpy(&c, set->dev.cells, sizeof c, hipMemcpyDeviceToHost);
-        stats->cells = (int64_t)c;
-    }
+    fill_ext_stats(stats, set, ms, t0, launches, cells);
     return HIPSW_OK;
 }
 
@@ -481,12 +587,57 @@ extern "C" void hipsw_ext_free(hipsw_engine *eng, hipsw_ext_set *set)
     if (eng) {
         std::lock_guard<std::recursive_mutex> lock(eng->mu);
         hipSetDevice(eng->device);
-        hipStreamSynchronize(eng->stream);
+        hipStreamSynchronize(eng->main.stream);
         if (set->graph) hipGraphExecDestroy(set->graph);
-        if (set->dmem) hipFree(set->dmem);
+        if (set->dmem && set->own_dmem) hipFree(set->dmem);
     }
     delete set;
 }
+
+namespace hipsw {
+
+// One-shot run of a prepared set on a leased context: the job arrays go to
+// the context's grow-only arena, the bins are launched directly (a graph is
+// not worth capturing for a set that runs once), and the results come back
+// on the context's stream.  `arena`, `gs_fixed` and `counters` (nullable)
+// replace the context's buffers.
+int ext_run_once(hipsw_engine *eng, ExecCtx &c, const ExtPrep &P, int mode, void *arena,
+                 void *gs_fixed, uint32_t *counters, int32_t *out, hipsw_stats *stats, double t0)
+{
+    hipsw_ext_set set;
+    if (!arena && !(arena = c.arena.get(P.arena_bytes))) {
+        eng->set_error("hipsw_banded_extend: arena allocation failed");
+        return HIPSW_E_DEVICE;
+    }
+    if (!eng->check(ext_bind_upload(P, arena, &set, c.stream), "upload")) return HIPSW_E_DEVICE;
+    if (mode == HIPSW_MODE_BATCHED && set.launches.size() > 64) {
+        eng->set_error("hipsw_banded_extend: too many bins");
+        return HIPSW_E_UNSUPPORTED;
+    }
+    const size_t need = mode == HIPSW_MODE_PLAIN ? (set.plain_global ? set.plain_scratch : 0)
+                                                 : set.batched_scratch;
+    void *gs = gs_fixed;
+    if (need && !gs && !(gs = c.scratch.get(need))) {
+        eng->set_error("hipsw_banded_extend: scratch allocation failed");
+        return HIPSW_E_DEVICE;
+    }
+    unsigned long long cells = 0;
+    if (!eng->check(ext_enqueue(&set, mode, gs, counters ? counters : c.counters, c.stream, c.ev0,
+                                c.ev1), "launch extension") ||
+        !eng->check(hipMemcpyAsync(out, set.dev.out, (size_t)P.n * 6 * sizeof(int32_t),
+                                   hipMemcpyDeviceToHost, c.stream), "download") ||
+        !eng->check(hipMemcpyAsync(&cells, set.dev.cells, sizeof cells, hipMemcpyDeviceToHost,
+                                   c.stream), "download cells") ||
+        !eng->check(hipStreamSynchronize(c.stream), "extension"))
+        return HIPSW_E_DEVICE;
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c.ev0, c.ev1);
+    fill_ext_stats(stats, &set, ms, t0,
+                   mode == HIPSW_MODE_PLAIN ? 1 : (int64_t)set.launches.size(), cells);
+    return HIPSW_OK;
+}
+
+}  // namespace hipsw
 
 extern "C" int hipsw_banded_extend(hipsw_engine *eng, int32_t n, const uint8_t *queries,
                                    const uint32_t *q_off, const uint16_t *q_len,
@@ -497,20 +648,20 @@ extern "C" int hipsw_banded_extend(hipsw_engine *eng, int32_t n, const uint8_t *
 {
     if (!eng) return HIPSW_E_INVALID;
     if (n == 0) return HIPSW_OK;
-    if (n < 0 || !out) return HIPSW_E_INVALID;
-    std::lock_guard<std::recursive_mutex> lock(eng->mu);
+    if (n < 0 || !out || (mode != HIPSW_MODE_PLAIN && mode != HIPSW_MODE_BATCHED))
+        return HIPSW_E_INVALID;
     const double t0 = now_ms();
-    hipsw_ext_set *set = hipsw_ext_upload(eng, n, queries, q_off, q_len, targets, t_off, t_len,
-                                          h0, w, end_bonus, sc);
-    if (!set) return eng->last_error.find("hipMalloc") != std::string::npos ||
-                             eng->last_error.find("upload") == 0
-                         ? HIPSW_E_DEVICE
-                         : HIPSW_E_INVALID;
-    int rc = hipsw_ext_run(eng, set, mode, stats);
-    if (rc == HIPSW_OK) rc = hipsw_ext_download(eng, set, out);
-    hipsw_ext_free(eng, set);
-    if (stats) stats->total_ms = now_ms() - t0;
-    return rc;
+    ExtPrep P;
+    std::string err;
+    const int rc = ext_prepare(eng, eng->tune_snapshot(), n, queries, q_off, q_len, targets, t_off,
+                               t_len, h0, w, end_bonus, sc, P, err);
+    if (rc != HIPSW_OK) {
+        eng->set_error("hipsw_banded_extend: " + err);
+        return rc;
+    }
+    SlotLease lease(eng);
+    hipSetDevice(eng->device);
+    return ext_run_once(eng, lease.ctx(), P, mode, nullptr, nullptr, nullptr, out, stats, t0);
 }
 
 // Autotuner: for every 8-bit bin of the set that the register-query kernels
@@ -522,32 +673,33 @@ extern "C" int hipsw_ext_set_layout(hipsw_engine *eng, int32_t length_class, int
                                    int32_t lds_blocks)
 {
     if (!eng || length_class < 0 || length_class > 2) return HIPSW_E_INVALID;
-    std::lock_guard<std::recursive_mutex> lock(eng->mu);
     if (kind == 0) {
-        eng->tuned_kind[length_class] = 0;
-        eng->tuned_lb[length_class] = 0;
+        std::lock_guard<std::mutex> lk(eng->tune_mu);
+        eng->tune.kind[length_class] = 0;
+        eng->tune.lb[length_class] = 0;
         return HIPSW_OK;
     }
     const int cols = length_class == 0 ? 64 : length_class == 1 ? 160 : 256;   // class's widest
     if ((kind != EXT_PACKED && kind != EXT_LANE) || !ext_pk_lb_supported(kind, cols, lds_blocks)) {
-        eng->last_error = "hipsw_ext_set_layout: layout not built for this length class";
+        eng->set_error("hipsw_ext_set_layout: layout not built for this length class");
         return HIPSW_E_INVALID;
     }
-    eng->tuned_kind[length_class] = kind;
-    eng->tuned_lb[length_class] = lds_blocks;
+    std::lock_guard<std::mutex> lk(eng->tune_mu);
+    eng->tune.kind[length_class] = kind;
+    eng->tune.lb[length_class] = lds_blocks;
     return HIPSW_OK;
 }
 
 extern "C" int hipsw_ext_set_layout16(hipsw_engine *eng, int32_t length_class, int32_t lds_blocks)
 {
     if (!eng || length_class < 0 || length_class > 2) return HIPSW_E_INVALID;
-    std::lock_guard<std::recursive_mutex> lock(eng->mu);
     const int cols = length_class == 0 ? 64 : length_class == 1 ? 160 : 256;   // class's widest
     if (lds_blocks != 0 && !ext_pk_lb_supported(EXT_LANE, cols, lds_blocks)) {
-        eng->last_error = "hipsw_ext_set_layout16: layout not built for this length class";
+        eng->set_error("hipsw_ext_set_layout16: layout not built for this length class");
         return HIPSW_E_INVALID;
     }
-    eng->tuned_lb16[length_class] = lds_blocks;
+    std::lock_guard<std::mutex> lk(eng->tune_mu);
+    eng->tune.lb16[length_class] = lds_blocks;
     return HIPSW_OK;
 }
 
@@ -595,8 +747,8 @@ extern "C" int hipsw_ext_autotune(hipsw_engine *eng, hipsw_ext_set *set, int32_t
             const size_t need = ext_layout_launches(eng, one, items);
             K = one[0];
             void *gs = nullptr;
-            if (need && !(gs = eng->scratch_get(need))) {
-                eng->last_error = "hipsw_ext_autotune: scratch allocation failed";
+            if (need && !(gs = eng->main.scratch.get(need))) {
+                eng->set_error("hipsw_ext_autotune: scratch allocation failed");
                 rc = HIPSW_E_DEVICE;
                 break;
             }
@@ -608,17 +760,18 @@ extern "C" int hipsw_ext_autotune(hipsw_engine *eng, hipsw_ext_set *set, int32_t
             }
             float t = 1e30f;
             for (int rep = 0; rep < 2; ++rep) {     // the first run warms caches and code
-                if (!eng->check(launch_zero_u32(eng->counters, 1, eng->stream), "zero") ||
-                    !eng->check(hipEventRecord(eng->ev0, eng->stream), "event") ||
+                ExecCtx &c = eng->main;
+                if (!eng->check(launch_zero_u32(c.counters, 1, c.stream), "zero") ||
+                    !eng->check(hipEventRecord(c.ev0, c.stream), "event") ||
                     !eng->check(launch_ext_bin(K, set->dev, set->params, set->d_order, d_tmp,
-                                               eng->counters, gs, eng->stream), "tune launch") ||
-                    !eng->check(hipEventRecord(eng->ev1, eng->stream), "event") ||
-                    !eng->check(hipEventSynchronize(eng->ev1), "tune sync")) {
+                                               c.counters, gs, c.stream), "tune launch") ||
+                    !eng->check(hipEventRecord(c.ev1, c.stream), "event") ||
+                    !eng->check(hipEventSynchronize(c.ev1), "tune sync")) {
                     rc = HIPSW_E_DEVICE;
                     break;
                 }
                 float e = 0;
-                hipEventElapsedTime(&e, eng->ev0, eng->ev1);
+                hipEventElapsedTime(&e, eng->main.ev0, eng->main.ev1);
                 t = std::min(t, e);
             }
             if (rc != HIPSW_OK) break;
@@ -633,11 +786,12 @@ extern "C" int hipsw_ext_autotune(hipsw_engine *eng, hipsw_ext_set *set, int32_t
             if (report && report[r].kind == win.kind && report[r].lds_blocks == win.lb) report[r].chosen = 1;
         L.kind = win.kind;
         L.lb = win.lb;
+        std::lock_guard<std::mutex> lk(eng->tune_mu);
         if (L.tier == TIER_U8) {
-            eng->tuned_kind[L.cls] = win.kind;
-            eng->tuned_lb[L.cls] = win.lb;
+            eng->tune.kind[L.cls] = win.kind;
+            eng->tune.lb[L.cls] = win.lb;
         } else {
-            eng->tuned_lb16[L.cls] = win.lb;
+            eng->tune.lb16[L.cls] = win.lb;
         }
     }
     hipFree(d_tmp);
@@ -654,5 +808,22 @@ extern "C" int hipsw_ext_autotune(hipsw_engine *eng, hipsw_ext_set *set, int32_t
                               hipMemcpyHostToDevice), "tune re-plan"))
         return HIPSW_E_DEVICE;
     if (n_report) *n_report = nrep;
+    return HIPSW_OK;
+}
+
+extern "C" int hipsw_ext_plan(hipsw_engine *eng, const hipsw_ext_set *set,
+                              hipsw_plan_entry *entries, int32_t cap, int32_t *n_entries)
+{
+    if (!eng || !set || cap < 0 || (cap > 0 && !entries)) return HIPSW_E_INVALID;
+    std::lock_guard<std::recursive_mutex> lock(eng->mu);
+    int k = 0;
+    for (const ExtLaunch &L : set->launches) {
+        if (k < cap) {
+            entries[k] = hipsw_plan_entry{L.tier, L.cls, L.has_n, L.job_end - L.job_begin, L.kind,
+                                          L.kind == EXT_UNPACKED ? 0 : L.lb, L.item_count, L.grid};
+        }
+        ++k;
+    }
+    if (n_entries) *n_entries = k;
     return HIPSW_OK;
 }

@@ -34,24 +34,22 @@ extern "C" int hipsw_local_sw(hipsw_engine *eng, int32_t n, const uint8_t *queri
 {
     if (!eng) return HIPSW_E_INVALID;
     if (n == 0) return HIPSW_OK;
-    std::lock_guard<std::recursive_mutex> lock(eng->mu);
     if (n < 0 || !out6 || !ok || !queries || !q_off || !q_len || !targets || !t_off || !t_len ||
         !sc) {
-        eng->last_error = "hipsw_local_sw: invalid argument";
+        eng->set_error("hipsw_local_sw: invalid argument");
         return HIPSW_E_INVALID;
     }
     if (mismatch_penalty < 0 || mismatch_penalty > 255 || sc->o_ins + sc->e_ins < 0 ||
         sc->o_ins + sc->e_ins > 255 || sc->e_ins < 0 || sc->e_ins > 255) {
         // LocalSWAligner converts bias, gapOE and gapE to UInt8 (:53, :176-177)
-        eng->last_error = "hipsw_local_sw: penalties must fit in 8 bits";
+        eng->set_error("hipsw_local_sw: penalties must fit in 8 bits");
         return HIPSW_E_UNSUPPORTED;
     }
-    hipSetDevice(eng->device);
     const double t0 = now_ms();
     PackedJobs pj;
     std::string err;
     if (!pack_jobs(n, queries, q_off, q_len, targets, t_off, t_len, pj, err)) {
-        eng->last_error = "hipsw_local_sw: " + err;
+        eng->set_error("hipsw_local_sw: " + err);
         return HIPSW_E_INVALID;
     }
     LocalParams p;
@@ -86,8 +84,14 @@ extern "C" int hipsw_local_sw(hipsw_engine *eng, int32_t n, const uint8_t *queri
     const size_t bytes = span<uint32_t>(words) + span<uint32_t>(nwords) + 4 * span<uint32_t>(n) +
                          2 * span<int32_t>(n) + span<int32_t>((size_t)n * 6) + span<int32_t>(n) +
                          2 * span<uint32_t>(n) + span<uint8_t>(colmax_bytes);
-    void *dmem = nullptr;
-    if (!eng->check(hipMalloc(&dmem, bytes), "hipMalloc(local set)")) return HIPSW_E_DEVICE;
+    SlotLease lease(eng);
+    ExecCtx &c = lease.ctx();
+    hipSetDevice(eng->device);
+    void *dmem = c.arena.get(bytes);
+    if (!dmem) {
+        eng->set_error("hipsw_local_sw: arena allocation failed");
+        return HIPSW_E_DEVICE;
+    }
     char *cur = reinterpret_cast<char *>(dmem);
     uint32_t *d_seq = carve<uint32_t>(cur, words);
     uint32_t *d_nmask = carve<uint32_t>(cur, nwords);
@@ -100,7 +104,7 @@ extern "C" int hipsw_local_sw(hipsw_engine *eng, int32_t n, const uint8_t *queri
     uint8_t *d_colmax = carve<uint8_t>(cur, colmax_bytes);
     bool good = true;
     auto up = [&](void *dst, const void *src, size_t b) {
-        if (good && b) good = eng->check(hipMemcpyAsync(dst, src, b, hipMemcpyHostToDevice, eng->stream), "upload");
+        if (good && b) good = eng->check(hipMemcpyAsync(dst, src, b, hipMemcpyHostToDevice, c.stream), "upload");
     };
     up(d_seq, pj.seq.data(), words * 4);
     up(d_nmask, pj.nmask.data(), nwords * 4);
@@ -118,21 +122,22 @@ extern "C" int hipsw_local_sw(hipsw_engine *eng, int32_t n, const uint8_t *queri
     double kernel_ms = 0;
     int64_t launches = 0;
     std::vector<int32_t> status(n, LOCAL_NIL);
-    if (good) good = eng->check(hipMemsetAsync(d_status, 0, (size_t)n * 4, eng->stream), "memset");
+    if (good) good = eng->check(hipMemsetAsync(d_status, 0, (size_t)n * 4, c.stream), "memset");
     if (good && !striped.empty()) {
-        good = eng->check(hipEventRecord(eng->ev0, eng->stream), "event") &&
+        good = eng->check(hipEventRecord(c.ev0, c.stream), "event") &&
                eng->check(launch_local_striped(dev, p, d_ord_s, grp_off, d_colmax,
-                                               p.min_sub > 0 ? tmax : 0, eng->stream),
+                                               p.min_sub > 0 ? tmax : 0, c.stream),
                           "launch local_striped") &&
-               eng->check(hipEventRecord(eng->ev1, eng->stream), "event") &&
-               eng->check(hipEventSynchronize(eng->ev1), "local_striped");
+               eng->check(hipEventRecord(c.ev1, c.stream), "event") &&
+               eng->check(hipEventSynchronize(c.ev1), "local_striped");
         if (good) {
             float ms = 0;
-            hipEventElapsedTime(&ms, eng->ev0, eng->ev1);
+            hipEventElapsedTime(&ms, c.ev0, c.ev1);
             kernel_ms += ms;
             ++launches;
-            good = eng->check(hipMemcpy(status.data(), d_status, (size_t)n * 4, hipMemcpyDeviceToHost),
-                              "download status");
+            good = eng->check(hipMemcpyAsync(status.data(), d_status, (size_t)n * 4,
+                                             hipMemcpyDeviceToHost, c.stream), "download status") &&
+                   eng->check(hipStreamSynchronize(c.stream), "download status");
         }
         if (good)
             for (uint32_t k : striped)
@@ -143,27 +148,29 @@ extern "C" int hipsw_local_sw(hipsw_engine *eng, int32_t n, const uint8_t *queri
         for (uint32_t k : scalar) { qc = std::max(qc, pj.ql[k]); tc = std::max(tc, pj.tl[k]); }
         const int blocks = (int)((scalar.size() + 63) / 64);
         const size_t sbytes = (size_t)blocks * 64 * (2 * (tc + 1) + qc) * sizeof(int);
-        int *scr = reinterpret_cast<int *>(eng->scratch_get(sbytes));
+        int *scr = reinterpret_cast<int *>(c.scratch.get(sbytes));
         good = scr != nullptr;
-        if (!good) eng->last_error = "hipsw_local_sw: scratch allocation failed";
+        if (!good) eng->set_error("hipsw_local_sw: scratch allocation failed");
         if (good) good = eng->check(hipMemcpyAsync(d_ord_c, scalar.data(), scalar.size() * 4,
-                                                   hipMemcpyHostToDevice, eng->stream), "upload");
+                                                   hipMemcpyHostToDevice, c.stream), "upload");
         if (good)
-            good = eng->check(hipEventRecord(eng->ev0, eng->stream), "event") &&
+            good = eng->check(hipEventRecord(c.ev0, c.stream), "event") &&
                    eng->check(launch_local_scalar(dev, p, d_ord_c, (int)scalar.size(), scr, tc, qc,
-                                                  eng->stream), "launch local_scalar") &&
-                   eng->check(hipEventRecord(eng->ev1, eng->stream), "event") &&
-                   eng->check(hipEventSynchronize(eng->ev1), "local_scalar");
+                                                  c.stream), "launch local_scalar") &&
+                   eng->check(hipEventRecord(c.ev1, c.stream), "event") &&
+                   eng->check(hipEventSynchronize(c.ev1), "local_scalar");
         if (good) {
             float ms = 0;
-            hipEventElapsedTime(&ms, eng->ev0, eng->ev1);
+            hipEventElapsedTime(&ms, c.ev0, c.ev1);
             kernel_ms += ms;
             ++launches;
         }
     }
-    if (good) good = eng->check(hipMemcpy(out6, d_out, (size_t)n * 6 * 4, hipMemcpyDeviceToHost), "download");
-    if (good) good = eng->check(hipMemcpy(status.data(), d_status, (size_t)n * 4, hipMemcpyDeviceToHost), "download");
-    hipFree(dmem);
+    if (good) good = eng->check(hipMemcpyAsync(out6, d_out, (size_t)n * 6 * 4, hipMemcpyDeviceToHost,
+                                               c.stream), "download");
+    if (good) good = eng->check(hipMemcpyAsync(status.data(), d_status, (size_t)n * 4,
+                                               hipMemcpyDeviceToHost, c.stream), "download");
+    if (good) good = eng->check(hipStreamSynchronize(c.stream), "download");
     if (!good) return HIPSW_E_DEVICE;
     for (int k = 0; k < n; ++k) {
         ok[k] = status[k] == LOCAL_OK ? 1 : 0;

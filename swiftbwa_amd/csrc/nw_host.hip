@@ -21,7 +21,8 @@ struct hipsw_nw_set {
     NwParams params;
     std::vector<int32_t> ql, tl, w0, ts;
     int max_pen = 7;
-    void *dmem = nullptr;
+    void *dmem = nullptr;            // owned when own_dmem (device-resident sets)
+    bool own_dmem = false;
     NwDev dev;
     int32_t *d_w = nullptr;
     uint32_t *d_order = nullptr;
@@ -57,72 +58,82 @@ int s_class(int S)
 
 }  // namespace
 
-extern "C" hipsw_nw_set *hipsw_nw_upload(hipsw_engine *eng, int32_t n, const uint8_t *queries,
-                                         const uint32_t *q_off, const uint16_t *q_len,
-                                         const uint8_t *targets, const uint32_t *t_off,
-                                         const uint16_t *t_len, const int32_t *w,
-                                         const int32_t *true_score, int32_t max_retries,
-                                         int32_t cap, const hipsw_scoring *sc)
+namespace {
+
+// Host half of an NW set: validation, scoring, packed sequences.
+struct NwPrep {
+    int n = 0, max_retries = 3, cap = 512, max_pen = 7;
+    NwParams params;
+    PackedJobs pj;
+    std::vector<int32_t> w0, ts;
+    size_t items_cap = 0, arena_bytes = 0;
+};
+
+int nw_prepare(int n, const uint8_t *queries, const uint32_t *q_off, const uint16_t *q_len,
+               const uint8_t *targets, const uint32_t *t_off, const uint16_t *t_len,
+               const int32_t *w, const int32_t *true_score, int32_t max_retries, int32_t cap,
+               const hipsw_scoring *sc, NwPrep &P, std::string &err)
 {
-    if (!eng) return nullptr;
-    std::lock_guard<std::recursive_mutex> lock(eng->mu);
     if (n < 0 || cap < 1 || max_retries < 0 ||
         (n > 0 && (!queries || !q_off || !q_len || !targets || !t_off || !t_len || !w ||
                    !true_score))) {
-        eng->last_error = "hipsw_nw_upload: invalid argument";
-        return nullptr;
+        err = "invalid argument";
+        return HIPSW_E_INVALID;
     }
     if (!scoring_valid(sc)) {
-        eng->last_error = "hipsw_nw_upload: gap extension penalties must be > 0";
-        return nullptr;
+        err = "gap extension penalties must be > 0";
+        return HIPSW_E_INVALID;
     }
-    hipSetDevice(eng->device);
-    auto *set = new hipsw_nw_set();
-    set->n = n;
-    set->max_retries = max_retries;
-    set->cap = cap;
-    memset(&set->params, 0, sizeof set->params);
+    P.n = n;
+    P.max_retries = max_retries;
+    P.cap = cap;
+    memset(&P.params, 0, sizeof P.params);
     int mat_abs = 0;
     for (int k = 0; k < 25; ++k) {
-        set->params.mat[k] = sc->mat[k];
+        P.params.mat[k] = sc->mat[k];
         mat_abs = std::max(mat_abs, std::abs((int)sc->mat[k]));
     }
-    set->params.o_del = sc->o_del; set->params.e_del = sc->e_del;
-    set->params.o_ins = sc->o_ins; set->params.e_ins = sc->e_ins;
-    set->params.oe_del = sc->o_del + sc->e_del;
-    set->params.oe_ins = sc->o_ins + sc->e_ins;
-    set->max_pen = std::max(mat_abs, std::max(set->params.oe_del, set->params.oe_ins));
-
-    PackedJobs pj;
-    std::string err;
-    if (!pack_jobs(n, queries, q_off, q_len, targets, t_off, t_len, pj, err)) {
-        eng->last_error = "hipsw_nw_upload: " + err;
-        delete set;
-        return nullptr;
-    }
-    set->ql = pj.ql;
-    set->tl = pj.tl;
-    set->w0.assign(w, w + n);
-    set->ts.assign(true_score, true_score + n);
+    P.params.o_del = sc->o_del; P.params.e_del = sc->e_del;
+    P.params.o_ins = sc->o_ins; P.params.e_ins = sc->e_ins;
+    P.params.oe_del = sc->o_del + sc->e_del;
+    P.params.oe_ins = sc->o_ins + sc->e_ins;
+    P.max_pen = std::max(mat_abs, std::max(P.params.oe_del, P.params.oe_ins));
     for (int k = 0; k < n; ++k) {
-        if (set->w0[k] < 1) {
+        if (w[k] < 1) {
             // GlobalAligner's first-row loop `for j in 1...initEnd` needs w >= 1
-            eng->last_error = "hipsw_nw_upload: w must be >= 1";
-            delete set;
-            return nullptr;
+            err = "w must be >= 1";
+            return HIPSW_E_INVALID;
         }
     }
-    const size_t words = pj.seq.size(), nwords = pj.nmask.size();
-    const size_t items_cap = (size_t)n / 64 + 64;
-    const size_t bytes = span<uint32_t>(words) + span<uint32_t>(nwords) + 4 * span<uint32_t>(n) +
-                         3 * span<int32_t>(n) + span<int32_t>(n) + span<uint32_t>((size_t)n * cap) +
-                         2 * span<int32_t>(n) + span<unsigned long long>(1) +
-                         span<uint32_t>(n) + span<WorkItem>(items_cap);
-    if (!eng->check(hipMalloc(&set->dmem, bytes), "hipMalloc(nw set)")) {
-        delete set;
-        return nullptr;
-    }
-    char *cur = reinterpret_cast<char *>(set->dmem);
+    if (!pack_jobs(n, queries, q_off, q_len, targets, t_off, t_len, P.pj, err))
+        return HIPSW_E_INVALID;
+    P.w0.assign(w, w + n);
+    P.ts.assign(true_score, true_score + n);
+    const size_t words = P.pj.seq.size(), nwords = P.pj.nmask.size();
+    P.items_cap = (size_t)n / 64 + 64;
+    P.arena_bytes = span<uint32_t>(words) + span<uint32_t>(nwords) + 4 * span<uint32_t>(n) +
+                    3 * span<int32_t>(n) + span<int32_t>(n) + span<uint32_t>((size_t)n * cap) +
+                    2 * span<int32_t>(n) + span<unsigned long long>(1) + span<uint32_t>(n) +
+                    span<WorkItem>(P.items_cap);
+    return HIPSW_OK;
+}
+
+hipError_t nw_bind_upload(NwPrep &P, void *dmem, hipsw_nw_set *set, hipStream_t st)
+{
+    const int n = P.n, cap = P.cap;
+    set->n = n;
+    set->max_retries = P.max_retries;
+    set->cap = cap;
+    set->params = P.params;
+    set->max_pen = P.max_pen;
+    set->ql = P.pj.ql;
+    set->tl = P.pj.tl;
+    set->w0 = P.w0;
+    set->ts = P.ts;
+    set->final_w = P.w0;
+    set->dmem = dmem;
+    const size_t words = P.pj.seq.size(), nwords = P.pj.nmask.size();
+    char *cur = reinterpret_cast<char *>(dmem);
     uint32_t *d_seq = carve<uint32_t>(cur, words);
     uint32_t *d_nmask = carve<uint32_t>(cur, nwords);
     uint32_t *d_qw = carve<uint32_t>(cur, n);
@@ -138,28 +149,53 @@ extern "C" hipsw_nw_set *hipsw_nw_upload(hipsw_engine *eng, int32_t n, const uin
     int32_t *d_flags = carve<int32_t>(cur, n);
     unsigned long long *d_cells = carve<unsigned long long>(cur, 1);
     set->d_order = carve<uint32_t>(cur, n);
-    set->d_items = carve<WorkItem>(cur, items_cap);
-    bool ok = true;
+    set->d_items = carve<WorkItem>(cur, P.items_cap);
+    set->dev = NwDev{d_seq, d_nmask, d_qw, d_tw, d_qn, d_tn, d_ql, d_tl, set->d_w, d_score,
+                     d_cig, d_ncig, d_flags, d_cells, cap};
+    hipError_t e = hipSuccess;
     auto up = [&](void *dst, const void *src, size_t b) {
-        if (ok && b) ok = eng->check(hipMemcpyAsync(dst, src, b, hipMemcpyHostToDevice, eng->stream), "upload");
+        if (e == hipSuccess && b) e = hipMemcpyAsync(dst, src, b, hipMemcpyHostToDevice, st);
     };
-    up(d_seq, pj.seq.data(), words * 4);
-    up(d_nmask, pj.nmask.data(), nwords * 4);
-    up(d_qw, pj.qw.data(), (size_t)n * 4);
-    up(d_tw, pj.tw.data(), (size_t)n * 4);
-    up(d_qn, pj.qnw.data(), (size_t)n * 4);
-    up(d_tn, pj.tnw.data(), (size_t)n * 4);
-    up(d_ql, pj.ql.data(), (size_t)n * 4);
-    up(d_tl, pj.tl.data(), (size_t)n * 4);
-    if (ok) ok = eng->check(hipStreamSynchronize(eng->stream), "upload sync");
-    if (!ok) {
-        hipFree(set->dmem);
+    up(d_seq, P.pj.seq.data(), words * 4);
+    up(d_nmask, P.pj.nmask.data(), nwords * 4);
+    up(d_qw, P.pj.qw.data(), (size_t)n * 4);
+    up(d_tw, P.pj.tw.data(), (size_t)n * 4);
+    up(d_qn, P.pj.qnw.data(), (size_t)n * 4);
+    up(d_tn, P.pj.tnw.data(), (size_t)n * 4);
+    up(d_ql, P.pj.ql.data(), (size_t)n * 4);
+    up(d_tl, P.pj.tl.data(), (size_t)n * 4);
+    return e;
+}
+
+}  // namespace
+
+extern "C" hipsw_nw_set *hipsw_nw_upload(hipsw_engine *eng, int32_t n, const uint8_t *queries,
+                                         const uint32_t *q_off, const uint16_t *q_len,
+                                         const uint8_t *targets, const uint32_t *t_off,
+                                         const uint16_t *t_len, const int32_t *w,
+                                         const int32_t *true_score, int32_t max_retries,
+                                         int32_t cap, const hipsw_scoring *sc)
+{
+    if (!eng) return nullptr;
+    NwPrep P;
+    std::string err;
+    if (nw_prepare(n, queries, q_off, q_len, targets, t_off, t_len, w, true_score, max_retries,
+                   cap, sc, P, err) != HIPSW_OK) {
+        eng->set_error("hipsw_nw_upload: " + err);
+        return nullptr;
+    }
+    std::lock_guard<std::recursive_mutex> lock(eng->mu);
+    hipSetDevice(eng->device);
+    void *dmem = nullptr;
+    if (!eng->check(hipMalloc(&dmem, P.arena_bytes), "hipMalloc(nw set)")) return nullptr;
+    auto *set = new hipsw_nw_set();
+    set->own_dmem = true;
+    if (!eng->check(nw_bind_upload(P, dmem, set, eng->main.stream), "upload") ||
+        !eng->check(hipStreamSynchronize(eng->main.stream), "upload sync")) {
+        hipFree(dmem);
         delete set;
         return nullptr;
     }
-    set->dev = NwDev{d_seq, d_nmask, d_qw, d_tw, d_qn, d_tn, d_ql, d_tl, set->d_w, d_score,
-                     d_cig, d_ncig, d_flags, d_cells, cap};
-    set->final_w = set->w0;
     return set;
 }
 
@@ -259,13 +295,10 @@ void hipsw::nw_plan_pass(const hipsw_engine *eng, const int32_t *qlv, const int3
     }
 }
 
-extern "C" int hipsw_nw_run(hipsw_engine *eng, hipsw_nw_set *set, int32_t mode,
-                            hipsw_stats *stats)
+// The passes of the CPU retry loop over a bound set, on context c.
+static int nw_run_ctx(hipsw_engine *eng, ExecCtx &c, hipsw_nw_set *set, int mode,
+                      hipsw_stats *stats)
 {
-    if (!eng || !set) return HIPSW_E_INVALID;
-    if (mode != HIPSW_MODE_PLAIN && mode != HIPSW_MODE_BATCHED) return HIPSW_E_INVALID;
-    std::lock_guard<std::recursive_mutex> lock(eng->mu);
-    hipSetDevice(eng->device);
     const double t0 = now_ms();
     const int n = set->n;
     double kernel_ms = 0;
@@ -279,7 +312,7 @@ extern "C" int hipsw_nw_run(hipsw_engine *eng, hipsw_nw_set *set, int32_t mode,
     std::vector<WorkItem> items;
     std::vector<NwLaunch> launches;
     if (n > 0 && !eng->check(launch_zero_u32(reinterpret_cast<uint32_t *>(set->dev.cells), 2,
-                                             eng->stream), "zero"))
+                                             c.stream), "zero"))
         return HIPSW_E_DEVICE;
     for (int pass = 0; !active.empty(); ++pass) {
         if (pass == 0 && set->plan_ready[mode]) {
@@ -302,7 +335,7 @@ extern "C" int hipsw_nw_run(hipsw_engine *eng, hipsw_nw_set *set, int32_t mode,
             }
         }
         if (launches.size() > 64) {
-            eng->last_error = "hipsw_nw_run: too many bins";
+            eng->set_error("hipsw_nw_run: too many bins");
             return HIPSW_E_UNSUPPORTED;
         }
         // Scratch: backpointers [row][c/8][lane] per resident wave, plus DP
@@ -320,21 +353,21 @@ extern "C" int hipsw_nw_run(hipsw_engine *eng, hipsw_nw_set *set, int32_t mode,
                 gbytes = std::max(gbytes, (size_t)L.grid * L.S * 64 * (L.tier == NW_I16 ? 4 : 8));
         }
         const size_t need = zbytes + ((gbytes + 255) & ~(size_t)255) + 256;
-        if (need > eng->nw_dev_bytes) {
-            eng->free_nw_buffers();
-            if (!eng->check(hipMalloc(&eng->nw_dev, need), "hipMalloc(nw scratch)")) return HIPSW_E_DEVICE;
-            eng->nw_dev_bytes = need;
+        void *nwbuf = c.nw.get(need);
+        if (!nwbuf) {
+            eng->set_error("hipsw_nw_run: scratch allocation failed");
+            return HIPSW_E_DEVICE;
         }
-        uint32_t *zbase = reinterpret_cast<uint32_t *>(eng->nw_dev);
-        void *gstate = reinterpret_cast<char *>(eng->nw_dev) + ((zbytes + 255) & ~(size_t)255);
+        uint32_t *zbase = reinterpret_cast<uint32_t *>(nwbuf);
+        void *gstate = reinterpret_cast<char *>(nwbuf) + ((zbytes + 255) & ~(size_t)255);
         bool ok = eng->check(hipMemcpyAsync(set->d_w, w.data(), (size_t)n * 4,
-                                            hipMemcpyHostToDevice, eng->stream), "upload w");
+                                            hipMemcpyHostToDevice, c.stream), "upload w");
         if (ok) ok = eng->check(hipMemcpyAsync(set->d_order, order.data(), order.size() * 4,
-                                               hipMemcpyHostToDevice, eng->stream), "upload order");
+                                               hipMemcpyHostToDevice, c.stream), "upload order");
         if (ok) ok = eng->check(hipMemcpyAsync(set->d_items, items.data(), items.size() * sizeof(WorkItem),
-                                               hipMemcpyHostToDevice, eng->stream), "upload items");
-        if (ok) ok = eng->check(launch_zero_u32(eng->counters, 64, eng->stream), "zero");
-        if (ok) ok = eng->check(hipEventRecord(eng->ev0, eng->stream), "event");
+                                               hipMemcpyHostToDevice, c.stream), "upload items");
+        if (ok) ok = eng->check(launch_zero_u32(c.counters, 64, c.stream), "zero");
+        if (ok) ok = eng->check(hipEventRecord(c.ev0, c.stream), "event");
         int k = 0;
         for (const NwLaunch &L : launches) {
             if (!ok) break;
@@ -344,9 +377,9 @@ extern "C" int hipsw_nw_run(hipsw_engine *eng, hipsw_nw_set *set, int32_t mode,
                 const WorkItem &last = items[L.item_begin + L.item_count - 1];
                 const int nj = (int)(last.start + last.count - first.start);
                 ok = eng->check(launch_nw_wide(set->dev, set->params, set->d_order + first.start, nj,
-                                               eng->counters + k, L.qmax,
+                                               c.counters + k, L.qmax,
                                                reinterpret_cast<uint8_t *>(zbase), L.zmax,
-                                               L.zmax <= 49152, L.grid, eng->stream),
+                                               L.zmax <= 49152, L.grid, c.stream),
                                 "launch nw wide");
                 ++k;
                 ++launches_total;
@@ -355,21 +388,22 @@ extern "C" int hipsw_nw_run(hipsw_engine *eng, hipsw_nw_set *set, int32_t mode,
             const int ncw = (L.ncol_max + 7) / 8;
             ok = eng->check(launch_nw(L.tier, L.global_state, set->dev, set->params, set->d_order,
                                       set->d_items + L.item_begin, L.item_count,
-                                      eng->counters + k, L.S, zbase, L.zrows, ncw, L.grid,
+                                      c.counters + k, L.S, zbase, L.zrows, ncw, L.grid,
                                       pass == 0 && mode == HIPSW_MODE_BATCHED, gstate, L.qw8,
-                                      eng->stream),
+                                      c.stream),
                             "launch nw");
             ++k;
             ++launches_total;
         }
-        if (ok) ok = eng->check(hipEventRecord(eng->ev1, eng->stream), "event");
-        if (ok) ok = eng->check(hipEventSynchronize(eng->ev1), "nw kernel");
+        if (ok) ok = eng->check(hipEventRecord(c.ev1, c.stream), "event");
+        if (ok) ok = eng->check(hipEventSynchronize(c.ev1), "nw kernel");
         if (!ok) return HIPSW_E_DEVICE;
         float ms = 0.f;
-        hipEventElapsedTime(&ms, eng->ev0, eng->ev1);
+        hipEventElapsedTime(&ms, c.ev0, c.ev1);
         kernel_ms += ms;
-        if (!eng->check(hipMemcpy(score.data(), set->dev.score, (size_t)n * 4, hipMemcpyDeviceToHost),
-                        "download scores"))
+        if (!eng->check(hipMemcpyAsync(score.data(), set->dev.score, (size_t)n * 4,
+                                       hipMemcpyDeviceToHost, c.stream), "download scores") ||
+            !eng->check(hipStreamSynchronize(c.stream), "download scores"))
             return HIPSW_E_DEVICE;
         // CIGARGenerator.swift:88-96: retry while score < trueScore.
         std::vector<uint32_t> next;
@@ -389,10 +423,51 @@ extern "C" int hipsw_nw_run(hipsw_engine *eng, hipsw_nw_set *set, int32_t mode,
         stats->total_ms = now_ms() - t0;
         stats->n_jobs = n;
         stats->launches = launches_total;
-        unsigned long long c = 0;
-        if (n > 0) hipMemcpy(&c, set->dev.cells, sizeof c, hipMemcpyDeviceToHost);
-        stats->cells = (int64_t)c;
+        unsigned long long cells = 0;
+        if (n > 0 && (!eng->check(hipMemcpyAsync(&cells, set->dev.cells, sizeof cells,
+                                                 hipMemcpyDeviceToHost, c.stream), "cells") ||
+                      !eng->check(hipStreamSynchronize(c.stream), "cells")))
+            return HIPSW_E_DEVICE;
+        stats->cells = (int64_t)cells;
     }
+    return HIPSW_OK;
+}
+
+extern "C" int hipsw_nw_run(hipsw_engine *eng, hipsw_nw_set *set, int32_t mode,
+                            hipsw_stats *stats)
+{
+    if (!eng || !set) return HIPSW_E_INVALID;
+    if (mode != HIPSW_MODE_PLAIN && mode != HIPSW_MODE_BATCHED) return HIPSW_E_INVALID;
+    std::lock_guard<std::recursive_mutex> lock(eng->mu);
+    hipSetDevice(eng->device);
+    return nw_run_ctx(eng, eng->main, set, mode, stats);
+}
+
+// Results of a bound set into the caller's arrays, copied on stream st.
+static int nw_download_ctx(hipsw_engine *eng, hipStream_t st, hipsw_nw_set *set,
+                           int32_t *out_score, uint32_t *out_cigar, int32_t *out_n_cigar,
+                           int32_t *out_flags, int32_t *out_final_w)
+{
+    const int n = set->n;
+    if (n == 0) return HIPSW_OK;
+    bool ok = true;
+    auto down = [&](void *dst, const void *src, size_t b) {
+        if (ok && dst) ok = eng->check(hipMemcpyAsync(dst, src, b, hipMemcpyDeviceToHost, st), "download");
+    };
+    std::vector<int32_t> ncig(n);
+    down(out_score, set->dev.score, (size_t)n * 4);
+    down(out_cigar, set->dev.cigar, (size_t)n * set->cap * 4);
+    down(ncig.data(), set->dev.n_cigar, (size_t)n * 4);
+    down(out_flags, set->dev.flags, (size_t)n * 4);
+    if (ok) ok = eng->check(hipStreamSynchronize(st), "download");
+    if (!ok) return HIPSW_E_DEVICE;
+    if (out_n_cigar) memcpy(out_n_cigar, ncig.data(), (size_t)n * 4);
+    if (out_cigar)                                       // entries past the CIGAR are zero
+        for (int k = 0; k < n; ++k) {
+            const int used = std::min(std::max(ncig[k], 0), set->cap);
+            std::fill(out_cigar + (size_t)k * set->cap + used, out_cigar + (size_t)(k + 1) * set->cap, 0u);
+        }
+    if (out_final_w) memcpy(out_final_w, set->final_w.data(), (size_t)n * 4);
     return HIPSW_OK;
 }
 
@@ -402,26 +477,9 @@ extern "C" int hipsw_nw_download(hipsw_engine *eng, hipsw_nw_set *set, int32_t *
 {
     if (!eng || !set) return HIPSW_E_INVALID;
     std::lock_guard<std::recursive_mutex> lock(eng->mu);
-    const int n = set->n;
-    if (n == 0) return HIPSW_OK;
     hipSetDevice(eng->device);
-    bool ok = true;
-    auto down = [&](void *dst, const void *src, size_t b) {
-        if (ok && dst) ok = eng->check(hipMemcpy(dst, src, b, hipMemcpyDeviceToHost), "download");
-    };
-    std::vector<int32_t> ncig(n);
-    down(out_score, set->dev.score, (size_t)n * 4);
-    down(out_cigar, set->dev.cigar, (size_t)n * set->cap * 4);
-    down(ncig.data(), set->dev.n_cigar, (size_t)n * 4);
-    if (ok && out_n_cigar) memcpy(out_n_cigar, ncig.data(), (size_t)n * 4);
-    if (ok && out_cigar)                                       // entries past the CIGAR are zero
-        for (int k = 0; k < n; ++k) {
-            const int used = std::min(std::max(ncig[k], 0), set->cap);
-            std::fill(out_cigar + (size_t)k * set->cap + used, out_cigar + (size_t)(k + 1) * set->cap, 0u);
-        }
-    down(out_flags, set->dev.flags, (size_t)n * 4);
-    if (out_final_w) memcpy(out_final_w, set->final_w.data(), (size_t)n * 4);
-    return ok ? HIPSW_OK : HIPSW_E_DEVICE;
+    return nw_download_ctx(eng, eng->main.stream, set, out_score, out_cigar, out_n_cigar,
+                           out_flags, out_final_w);
 }
 
 extern "C" void hipsw_nw_free(hipsw_engine *eng, hipsw_nw_set *set)
@@ -430,8 +488,8 @@ extern "C" void hipsw_nw_free(hipsw_engine *eng, hipsw_nw_set *set)
     if (eng) {
         std::lock_guard<std::recursive_mutex> lock(eng->mu);
         hipSetDevice(eng->device);
-        hipStreamSynchronize(eng->stream);
-        if (set->dmem) hipFree(set->dmem);
+        hipStreamSynchronize(eng->main.stream);
+        if (set->dmem && set->own_dmem) hipFree(set->dmem);
     }
     delete set;
 }
@@ -447,16 +505,32 @@ extern "C" int hipsw_global_nw(hipsw_engine *eng, int32_t n, const uint8_t *quer
 {
     if (!eng) return HIPSW_E_INVALID;
     if (n == 0) return HIPSW_OK;
-    if (n < 0 || !out_score || !out_cigar || !out_n_cigar || !out_flags) return HIPSW_E_INVALID;
-    std::lock_guard<std::recursive_mutex> lock(eng->mu);
+    if (n < 0 || !out_score || !out_cigar || !out_n_cigar || !out_flags ||
+        (mode != HIPSW_MODE_PLAIN && mode != HIPSW_MODE_BATCHED))
+        return HIPSW_E_INVALID;
     const double t0 = now_ms();
-    hipsw_nw_set *set = hipsw_nw_upload(eng, n, queries, q_off, q_len, targets, t_off, t_len, w,
-                                        true_score, max_retries, cap, sc);
-    if (!set) return HIPSW_E_INVALID;
-    int rc = hipsw_nw_run(eng, set, mode, stats);
+    NwPrep P;
+    std::string err;
+    const int rc0 = nw_prepare(n, queries, q_off, q_len, targets, t_off, t_len, w, true_score,
+                               max_retries, cap, sc, P, err);
+    if (rc0 != HIPSW_OK) {
+        eng->set_error("hipsw_global_nw: " + err);
+        return rc0;
+    }
+    SlotLease lease(eng);
+    ExecCtx &c = lease.ctx();
+    hipSetDevice(eng->device);
+    void *dmem = c.arena.get(P.arena_bytes);
+    if (!dmem) {
+        eng->set_error("hipsw_global_nw: arena allocation failed");
+        return HIPSW_E_DEVICE;
+    }
+    hipsw_nw_set set;
+    if (!eng->check(nw_bind_upload(P, dmem, &set, c.stream), "upload")) return HIPSW_E_DEVICE;
+    int rc = nw_run_ctx(eng, c, &set, mode, stats);
     if (rc == HIPSW_OK)
-        rc = hipsw_nw_download(eng, set, out_score, out_cigar, out_n_cigar, out_flags, out_final_w);
-    hipsw_nw_free(eng, set);
+        rc = nw_download_ctx(eng, c.stream, &set, out_score, out_cigar, out_n_cigar, out_flags,
+                             out_final_w);
     if (stats) stats->total_ms = now_ms() - t0;
     return rc;
 }

@@ -93,6 +93,7 @@ struct ChunkReq {
     std::vector<uint32_t> nw_cig_off;            // n_nw + 1 offsets into nw_cig
     int64_t cells = 0;
     double host_ms = 0;                          // host packing and planning of the chunk
+    int64_t dev_peak = 0;                        // largest device footprint of the run
 };
 
 struct Slot {
@@ -297,9 +298,9 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
         // windows come from the .pac (no N); queries may hold N
         std::vector<uint8_t> any_n(r.n_ext);
         for (int k = 0; k < r.n_ext; ++k) any_n[k] = qnw[k] != kNoN || tnw[k] != kNoN;
-        ext_scratch = plan_ext_batched(eng, r.n_ext, ql.data(), tl.data(), r.h0.data(), r.w.data(),
-                                       any_n.data(), ep.max_mat, ep.std_mat != 0, ext_order,
-                                       ext_items, ext_launch);
+        ext_scratch = plan_ext_batched(eng, eng->tune_snapshot(), r.n_ext, ql.data(), tl.data(),
+                                       r.h0.data(), r.w.data(), any_n.data(), ep.max_mat,
+                                       ep.std_mat != 0, ext_order, ext_items, ext_launch);
     }
     // local SW split
     std::vector<uint32_t> loc_s, loc_c;
@@ -336,6 +337,10 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
     r.host_ms = now_ms_s() - t_host0;
     void *dmem = nullptr;
     if (hipMallocAsync(&dmem, bytes, st) != hipSuccess) return HIPSW_E_DEVICE;
+    // the phases' scratch buffers are freed in stream order before the next
+    // phase allocates, so the footprint is the arena plus the largest of them
+    auto note_phase = [&](size_t b) { r.dev_peak = std::max<int64_t>(r.dev_peak, (int64_t)(bytes + b)); };
+    note_phase(0);
     char *cur = reinterpret_cast<char *>(dmem);
     uint32_t *d_seq = carve<uint32_t>(cur, words), *d_nmask = carve<uint32_t>(cur, nwords);
     uint32_t *d_qw = carve<uint32_t>(cur, n), *d_tw = carve<uint32_t>(cur, n);
@@ -386,6 +391,7 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
         up(d_pk, hpk, (size_t)r.n_ext * sizeof(PkMeta));
         void *gs = nullptr;
         if (ok && ext_scratch) ok = hipMallocAsync(&gs, ext_scratch, st) == hipSuccess;
+        note_phase(ext_scratch);
         if (ok) ok = launch_zero_u32(sl.counters, 64, st) == hipSuccess;
         const ExtDev ed{d_seq, d_nmask, d_qw, d_tw, d_qn, d_tn, d_ql, d_tl, d_h0, d_w, d_eb, d_out6,
                         d_cells, d_pk};
@@ -435,7 +441,9 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
             const int blocks = (int)((loc_c.size() + 63) / 64);
             void *scr = nullptr;
             up(d_order2, loc_c.data(), loc_c.size() * 4);
-            if (ok) ok = hipMallocAsync(&scr, (size_t)blocks * 64 * (2 * (tc + 1) + qc) * 4, st) == hipSuccess;
+            const size_t scr_b = (size_t)blocks * 64 * (2 * (tc + 1) + qc) * 4;
+            if (ok) ok = hipMallocAsync(&scr, scr_b, st) == hipSuccess;
+            note_phase(scr_b);
             if (ok) ok = launch_local_scalar(ld, lp, d_order2, (int)loc_c.size(), (int *)scr, tc, qc, st) == hipSuccess;
             if (scr) hipFreeAsync(scr, st);
         }
@@ -477,6 +485,7 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
             up(d_order, order.data(), order.size() * 4);
             up(d_items, items.data(), items.size() * sizeof(WorkItem));
             if (ok) ok = hipMallocAsync(&scr, zb + gb + 512, st) == hipSuccess;
+            note_phase(zb + gb + 512);
             if (ok) ok = launch_zero_u32(sl.counters, 64, st) == hipSuccess;
             int k = 0;
             for (const NwLaunch &L : launches) {
@@ -537,6 +546,7 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
         void *cm = nullptr;
         const size_t off_b = ((size_t)(r.n_nw + 1) * 4 + 255) & ~(size_t)255;
         ok = hipMallocAsync(&cm, off_b + r.nw_cig.size() * 4, st) == hipSuccess;
+        note_phase(off_b + r.nw_cig.size() * 4);
         uint32_t *d_off = static_cast<uint32_t *>(cm);
         uint32_t *d_dense = reinterpret_cast<uint32_t *>(static_cast<char *>(cm) + off_b);
         if (ok) ok = hipMemcpyAsync(d_off, r.nw_cig_off.data(), (size_t)(r.n_nw + 1) * 4,
@@ -573,13 +583,64 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
     return HIPSW_OK;
 }
 
-size_t estimate_bytes(const ChunkReq &r, int cap)
+// Device bytes a chunk can hold at once, as run_chunk allocates them: its
+// arena (every term of run_chunk's arena size, with the requested window
+// lengths as the fetched ones' upper bound) plus the largest phase scratch --
+// the phases free theirs in stream order before the next one allocates.  The
+// phase bounds hold for any plan: the extension bound considers every built
+// LDS/HBM split (ext_scratch_bound), local SW assumes every task may fall
+// back to the scalar kernel, and each NW pass is bounded with the longest
+// query's column count and the longest target's rows for every wave.
+size_t estimate_bytes(const hipsw_server *srv, const ChunkReq &r)
 {
     const size_t n = (size_t)(r.n_ext + r.n_loc + r.n_nw);
-    size_t words = 64;
-    for (size_t k = 0; k < n; ++k) words += (r.q_len[k] + 15) / 16 + (r.t_len_req[k] + 15) / 16;
-    // arena + outputs + generous scratch (NW backpointers, long-query state)
-    return words * 4 + n * 160 + (size_t)r.n_nw * cap * 4 + (size_t)r.n_nw * 300 * 32 + (64 << 20);
+    const int cap = srv->nw_cap;
+    uint64_t words = 64, nwords = 64;
+    int loc_q = 1, loc_t = 1, nw_q = 1, nw_t = 1;
+    for (size_t k = 0; k < n; ++k) {
+        words += (r.q_len[k] + 15) / 16 + (r.t_len_req[k] + 15) / 16;
+        nwords += (r.q_len[k] + 31) / 32;
+        if ((int)k >= r.n_ext && (int)k < r.n_ext + r.n_loc) {
+            loc_q = std::max(loc_q, (int)r.q_len[k]);
+            loc_t = std::max(loc_t, (int)r.t_len_req[k]);
+        } else if ((int)k >= r.n_ext + r.n_loc) {
+            nw_q = std::max(nw_q, (int)r.q_len[k]);
+            nw_t = std::max(nw_t, (int)r.t_len_req[k]);
+        }
+    }
+    const size_t colmax_b = r.loc_min_sub > 0 ? (size_t)r.n_loc * loc_t : 1;
+    const size_t arena = span<uint32_t>(words) + span<uint32_t>(nwords) + 4 * span<uint32_t>(n) +
+                         5 * span<int32_t>(n) + span<Window>(n) + span<int32_t>(n * 6) +
+                         span<int32_t>(n) + span<uint32_t>((size_t)r.n_nw * cap + 1) +
+                         3 * span<int32_t>(n) + 2 * span<uint32_t>(n) +
+                         span<WorkItem>(n / 64 + 64) + span<uint8_t>(colmax_b) +
+                         span<PkMeta>(r.n_ext) + span<unsigned long long>(2);
+    size_t phase = 0;
+    if (r.n_ext) {
+        const ExtParams ep = make_params(&srv->sc);
+        std::vector<int32_t> ql(r.q_len.begin(), r.q_len.begin() + r.n_ext);
+        std::vector<uint8_t> qn(r.n_ext);
+        for (int k = 0; k < r.n_ext; ++k) qn[k] = has_n(r.qbytes.get() + r.q_off[k], r.q_len[k]);
+        phase = std::max(phase, ext_scratch_bound(srv->eng, r.n_ext, ql.data(), r.h0.data(), qn.data(),
+                                                  ep.max_mat, ep.std_mat != 0));
+    }
+    if (r.n_loc) {
+        const size_t blocks = ((size_t)r.n_loc + 63) / 64;
+        phase = std::max(phase, blocks * 64 * (2 * ((size_t)loc_t + 1) + loc_q) * 4);
+    }
+    if (r.n_nw) {
+        // lane-per-job launches: per wave, tlen rows of ceil(ncol/8) dwords
+        // per lane (ncol <= qlen), plus HBM row state of S <= qlen + 10 slots;
+        // wave-per-job launches: a tlen x ncol byte matrix per wave
+        const size_t waves = ((size_t)r.n_nw + 63) / 64 + 16;
+        const size_t lane = waves * (size_t)nw_t * ((nw_q + 7) / 8) * 64 * 4 +
+                            waves * ((size_t)nw_q + 10) * 64 * 8 + 1024;
+        const size_t wide = std::min<size_t>((size_t)r.n_nw, (size_t)srv->eng->n_cu * 8) *
+                            (size_t)nw_t * nw_q + 1024;
+        phase = std::max(phase, std::max(lane, wide));
+        phase = std::max(phase, ((size_t)(r.n_nw + 1) * 4 + 255) + (size_t)r.n_nw * cap * 4);
+    }
+    return arena + phase;
 }
 
 void slot_loop(hipsw_server *srv, Slot *sl)
@@ -664,11 +725,21 @@ extern "C" int hipsw_server_set_reference(hipsw_server *srv, const uint8_t *pac,
     std::unique_lock<std::mutex> lk(srv->mu);
     if (srv->running || !srv->queue.empty()) return HIPSW_E_PENDING;
     hipSetDevice(srv->eng->device);
+    // the new reference goes to a buffer of its own; the server's reference
+    // changes only once it is complete, and a failure leaves no reference
+    // (submits are then refused) rather than a stale length over new memory
+    const size_t bytes = (size_t)(n_bases + 3) / 4;
+    uint8_t *fresh = nullptr;
+    const bool ok = hipMalloc((void **)&fresh, bytes) == hipSuccess &&
+                    hipMemcpy(fresh, pac, bytes, hipMemcpyHostToDevice) == hipSuccess;
     if (srv->d_pac) hipFree(srv->d_pac);
     srv->d_pac = nullptr;
-    const size_t bytes = (size_t)(n_bases + 3) / 4;
-    if (hipMalloc((void **)&srv->d_pac, bytes) != hipSuccess) return HIPSW_E_DEVICE;
-    if (hipMemcpy(srv->d_pac, pac, bytes, hipMemcpyHostToDevice) != hipSuccess) return HIPSW_E_DEVICE;
+    srv->gl = 0;
+    if (!ok) {
+        if (fresh) hipFree(fresh);
+        return HIPSW_E_DEVICE;
+    }
+    srv->d_pac = fresh;
     srv->gl = n_bases;
     return HIPSW_OK;
 }
@@ -732,7 +803,7 @@ extern "C" int64_t hipsw_server_submit(hipsw_server *srv, const hipsw_chunk *c)
     r->eb.assign(c->ext_eb, c->ext_eb + c->n_ext);
     r->nw_w.assign(c->nw_w, c->nw_w + c->n_nw);
     r->nw_ts.assign(c->nw_true_score, c->nw_true_score + c->n_nw);
-    r->est_bytes = estimate_bytes(*r, srv->nw_cap);
+    r->est_bytes = estimate_bytes(srv, *r);
     std::lock_guard<std::mutex> lk(srv->mu);
     if ((int64_t)r->est_bytes > srv->budget) return HIPSW_E_BUDGET;
     r->ticket = srv->next_ticket++;
@@ -769,6 +840,8 @@ extern "C" int hipsw_server_wait(hipsw_server *srv, int64_t ticket, hipsw_chunk_
         res->run_ms = r->t_done - r->t_start;
         res->host_ms = r->host_ms;
         res->cells = r->cells;
+        res->est_bytes = (int64_t)r->est_bytes;
+        res->dev_peak_bytes = r->dev_peak;
         if (r->status == HIPSW_OK) {
             const int cap = srv->nw_cap;
             if (res->ext_out && r->n_ext) memcpy(res->ext_out, r->ext_out.data(), (size_t)r->n_ext * 24);

@@ -78,7 +78,14 @@ class HipswChunkResult(C.Structure):
     _fields_ = [("ext_out", _P), ("loc_out", _P), ("loc_ok", _P), ("nw_score", _P),
                 ("nw_cigar", _P), ("nw_n_cigar", _P), ("nw_flags", _P),
                 ("latency_ms", C.c_double), ("queue_ms", C.c_double), ("run_ms", C.c_double),
-                ("cells", C.c_int64), ("host_ms", C.c_double), ("nw_cigar_off", _P)]
+                ("cells", C.c_int64), ("host_ms", C.c_double), ("nw_cigar_off", _P),
+                ("est_bytes", C.c_int64), ("dev_peak_bytes", C.c_int64)]
+
+
+class HipswPlanEntry(C.Structure):
+    _fields_ = [("tier", C.c_int32), ("length_class", C.c_int32), ("has_n", C.c_int32),
+                ("jobs", C.c_int32), ("kind", C.c_int32), ("lds_blocks", C.c_int32),
+                ("items", C.c_int32), ("grid", C.c_int32)]
 
 
 # Every symbol include/hipsw.h declares, with (restype, argtypes).
@@ -96,6 +103,7 @@ _SIGS = {
     "hipsw_ext_upload": (_P, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                               C.POINTER(HipswScoring)]),
     "hipsw_ext_run": (C.c_int, [_P, _P, C.c_int32, C.POINTER(HipswStats)]),
+    "hipsw_ext_plan": (C.c_int, [_P, _P, _P, C.c_int32, _P]),
     "hipsw_ext_download": (C.c_int, [_P, _P, _P]),
     "hipsw_ext_free": (None, [_P, _P]),
     "hipsw_global_nw": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P, _P, C.c_int32,

@@ -158,7 +158,8 @@ class Server:
         out = {"ext": ext, "loc": loc, "loc_ok": lok, "nw_score": nws,
                "nw_n_cigar": nwn, "nw_flags": nwf, "latency_ms": r.latency_ms,
                "queue_ms": r.queue_ms, "run_ms": r.run_ms, "cells": r.cells,
-               "host_ms": r.host_ms}
+               "host_ms": r.host_ms, "est_bytes": r.est_bytes,
+               "dev_peak_bytes": r.dev_peak_bytes}
         if compact:
             out["nw_cigar_ops"] = nwc[:int(off[-1])]
             out["nw_cigar_off"] = off

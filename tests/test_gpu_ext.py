@@ -188,11 +188,37 @@ def test_autotune_both_layouts_exact(engine):
         t8 = [e for e in ents if e.tier == 0 and e.kind == 1]
         assert {e.lds_blocks for e in t8 if e.length_class == 1} == {10, 6, 5}
         assert {e.lds_blocks for e in t8 if e.length_class == 2} == {10, 8, 6}
+        t16 = [e for e in ents if e.tier == 1]
+        for c, splits in ((1, {10, 6, 5}), (2, {10, 8, 6})):
+            if any(e.length_class == c for e in t16):
+                assert {e.lds_blocks for e in t16 if e.length_class == c} == splits
         assert lib.hipsw_ext_download(engine.handle, st, native.ptr(out)) == 0
         assert np.array_equal(out, ref)                       # lane-layout results
         assert lib.hipsw_ext_run(engine.handle, st, native.MODE_BATCHED, C.byref(s)) == 0
         assert lib.hipsw_ext_download(engine.handle, st, native.ptr(out)) == 0
         assert np.array_equal(out, ref)                       # re-planned set
+    finally:
+        lib.hipsw_ext_free(engine.handle, st)
+
+
+def plan_of(engine, js, sp=None):
+    """The batched plan hipsw_ext_plan reports for js: one dict per bin."""
+    import ctypes as C
+    from swiftbwa_amd.metalsw import _ext_arrays
+    lib = engine.lib
+    a = _ext_arrays(js)
+    sc = (sp or ScoringParameters()).to_c()
+    st = lib.hipsw_ext_upload(engine.handle, js["n"], native.ptr(a["queries"]),
+                              native.ptr(a["q_off"]), native.ptr(a["q_len"]),
+                              native.ptr(a["targets"]), native.ptr(a["t_off"]),
+                              native.ptr(a["t_len"]), native.ptr(a["h0"]), native.ptr(a["w"]),
+                              native.ptr(a["end_bonus"]), C.byref(sc))
+    assert st
+    try:
+        ents = (native.HipswPlanEntry * 64)()
+        k = C.c_int32()
+        assert lib.hipsw_ext_plan(engine.handle, st, ents, 64, C.byref(k)) == 0
+        return [{f: getattr(e, f) for f, _ in native.HipswPlanEntry._fields_} for e in ents[:k.value]]
     finally:
         lib.hipsw_ext_free(engine.handle, st)
 
@@ -237,6 +263,13 @@ def test_every_layout_exact(engine):
         for combo in combos:
             for cls, (kind, lb) in enumerate(combo):
                 assert lib.hipsw_ext_set_layout(engine.handle, cls, kind, lb) == 0
+            # the pinned layout reached every 8-bit register-query bin
+            seen = set()
+            for b in plan_of(engine, big):
+                if b["tier"] == 0 and b["kind"] in (1, 2) and not b["has_n"]:
+                    assert (b["kind"], b["lds_blocks"]) == combo[b["length_class"]], (b, combo)
+                    seen.add(b["length_class"])
+            assert seen == {0, 1, 2}
             check(engine, big, modes=(native.MODE_BATCHED,))
     finally:
         for cls in range(3):
@@ -272,6 +305,11 @@ def test_every_layout16_exact(engine):
         for combo in ((4, 10, 10), (4, 6, 8), (4, 5, 6)):
             for cls, lb in enumerate(combo):
                 assert lib.hipsw_ext_set_layout16(engine.handle, cls, lb) == 0
+            # the pinned split reached every 16-bit bin (one job per lane)
+            bins = [b for b in plan_of(engine, js) if b["tier"] == 1]
+            assert {b["length_class"] for b in bins} == {0, 1, 2}
+            for b in bins:
+                assert b["kind"] == 2 and b["lds_blocks"] == combo[b["length_class"]], (b, combo)
             check(engine, js, modes=(native.MODE_BATCHED,))
     finally:
         for cls in range(3):

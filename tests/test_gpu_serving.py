@@ -108,3 +108,76 @@ def test_queries_with_n(engine, ref):
             assert np.array_equal(res["nw_cigar"][k, :m], rc[k, :m]), k
     finally:
         srv.close()
+
+
+def _long_query_chunk(ref, seed, n_ext, n_loc):
+    """A chunk whose extensions have 160-255-base queries, half of them with a
+    seed score that puts them in the 16-bit tier, plus 150-bp rescue tasks in
+    600-bp windows: the bins with the largest HBM column state."""
+    rng = np.random.default_rng(seed)
+    gl = len(ref)
+    ext = {k: [] for k in ("q", "q_len", "t_pos", "t_len", "t_rev", "a", "b", "c")}
+    for k in range(n_ext):
+        L = int(rng.integers(160, 256))
+        p = int(rng.integers(1000, gl - 2000))
+        q = ref[p:p + L].copy()
+        flip = rng.random(L) < 0.02
+        q[flip] = (q[flip] + 1) % 4
+        ext["q"].append(q)
+        ext["q_len"].append(L)
+        ext["t_pos"].append(p)
+        ext["t_len"].append(L + int(rng.integers(0, 100)))
+        ext["t_rev"].append(k & 1)
+        ext["a"].append(int(rng.integers(19, 60)) if k % 2 else int(rng.integers(300, 900)))
+        ext["b"].append(100)
+        ext["c"].append(5)
+    loc = {k: [] for k in ("q", "q_len", "t_pos", "t_len")}
+    for k in range(n_loc):
+        p = int(rng.integers(1000, gl - 2000))
+        q = ref[p + 200:p + 350].copy()
+        loc["q"].append(q)
+        loc["q_len"].append(150)
+        loc["t_pos"].append(p)
+        loc["t_len"].append(600)
+
+    def pack(d, n):
+        out = {"n": n, "q": np.concatenate(d["q"]) if n else np.zeros(1, np.uint8)}
+        ql = np.array(d["q_len"], np.uint16)
+        off = np.zeros(n, np.uint32)
+        if n:
+            off[1:] = np.cumsum(ql.astype(np.int64))[:-1]
+        out["q_off"], out["q_len"] = off, ql
+        out["t_pos"] = np.array(d["t_pos"], np.int64)
+        out["t_len"] = np.array(d["t_len"], np.uint16)
+        for f, dt in (("t_rev", np.uint8), ("a", np.int32), ("b", np.int32), ("c", np.int32)):
+            out[f] = np.array(d.get(f, [0] * n), dt)
+        return out
+
+    nw = serving.make_chunk(ref, seed + 1, 200)["nw"]
+    return {"pairs": n_ext, "ext": pack(ext, n_ext), "loc": pack(loc, n_loc), "nw": nw}
+
+
+def test_budget_estimate_bounds_peak(engine, ref):
+    """Each chunk's admission estimate bounds the device memory its run
+    holds at once (arena plus the largest phase scratch), also when the
+    160-255-base bins of both score tiers run the layouts with the most HBM
+    column state; results stay the oracle's."""
+    lib = engine.lib
+    srv = serving.Server(engine, slots=2, hbm_budget=16 << 30)
+    try:
+        srv.set_reference(serving.pack_pac(ref), len(ref))
+        assert lib.hipsw_ext_set_layout(engine.handle, 2, 1, 6) == 0
+        assert lib.hipsw_ext_set_layout16(engine.handle, 2, 6) == 0
+        chunks = [_long_query_chunk(ref, 700 + k, n, m)
+                  for k, (n, m) in enumerate([(300, 50), (4000, 2000), (20000, 6000)])]
+        chunks.append(serving.make_chunk(ref, 710, 3000))
+        for ch in chunks:
+            r = srv.wait(srv.submit(ch))
+            assert 0 < r["dev_peak_bytes"] <= r["est_bytes"], (r["dev_peak_bytes"], r["est_bytes"])
+            sets = orc.chunk_to_job_sets(ref, ch)
+            ext_ref, _ = orc.ksw_extend_batch(sets["ext"], nthreads=8)
+            assert np.array_equal(r["ext"], ext_ref)
+    finally:
+        lib.hipsw_ext_set_layout(engine.handle, 2, 0, 0)
+        lib.hipsw_ext_set_layout16(engine.handle, 2, 0)
+        srv.close()
