@@ -105,6 +105,26 @@ int hipsw_banded_extend(hipsw_engine *eng, int32_t n,
                         const hipsw_scoring *sc, int32_t mode, int32_t *out,
                         hipsw_stats *stats);
 
+/* Allocation-free variant of hipsw_banded_extend: same arguments, same
+ * results.  No counterpart in the reference; it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * hosts that forbid hipMalloc).  Every device buffer beyond the caller's
+ * inputs and outputs -- the packed job arrays, the batched order, work items
+ * and descriptors, the counters and the HBM column state -- lives in one
+ * static device array of 64 MiB per device, held until the process ends; the
+ * call makes no hipMalloc/hipFree.  A set whose layout needs more than that
+ * is refused with HIPSW_E_BUDGET (hipsw_last_error gives the bytes needed)
+ * before anything runs.  Calls on one device share the array: the library
+ * serialises them with a per-device mutex, so overlapping calls from
+ * several threads wait rather than race. */
+int hipsw_banded_extend_noalloc(hipsw_engine *eng, int32_t n,
+                                const uint8_t *queries, const uint32_t *q_off,
+                                const uint16_t *q_len, const uint8_t *targets,
+                                const uint32_t *t_off, const uint16_t *t_len,
+                                const int32_t *h0, const int32_t *w, const int32_t *end_bonus,
+                                const hipsw_scoring *sc, int32_t mode, int32_t *out,
+                                hipsw_stats *stats);
+
 /* Device-resident job sets: upload once, run many times (benchmarks, serving).
  * hipsw_ext_run leaves results on the device; hipsw_ext_download copies them
  * out.  No reference counterpart: MetalSW re-uploads every call. */

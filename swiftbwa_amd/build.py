@@ -16,7 +16,7 @@ INCLUDE = os.path.join(ROOT, "include")
 
 HIP_SOURCES = ["engine.hip", "ext_host.hip", "ext_kernel.hip", "nw_host.hip", "nw_kernel.hip",
                "local_host.hip", "local_kernel.hip", "server.hip", "cigar_host.hip",
-               "driver_host.hip"]
+               "driver_host.hip", "noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

@@ -340,6 +340,11 @@ extern "C" hipsw_engine *hipsw_engine_create(int device)
         }
         eng->free_slots.push_back(k);
     }
+    eng->noalloc_scratch = noalloc_scratch_address(device);
+    if (!eng->noalloc_scratch) {
+        delete eng;
+        return nullptr;
+    }
     return eng;
 }
 

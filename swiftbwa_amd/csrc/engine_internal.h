@@ -72,6 +72,9 @@ struct hipsw_engine {
     // autotuner table, read by every planner as a copy
     mutable std::mutex tune_mu;
     hipsw::TuneTable tune;
+    // device address of the allocation-free entry point's static scratch
+    // array on this engine's device (hipGetSymbolAddress at creation)
+    void *noalloc_scratch = nullptr;
 
     hipsw::TuneTable tune_snapshot() const;
     // Error text of the calling thread's last failed call on this engine.
@@ -220,6 +223,11 @@ hipError_t ext_enqueue(const hipsw_ext_set *set, int mode, void *gs, uint32_t *c
 
 // Error text of the calling thread's last failed call on eng.
 const char *thread_error(const hipsw_engine *eng);
+
+// Device address of the allocation-free entry point's static scratch array
+// on `device` (noalloc.hip), looked up once per device with the device
+// current; nullptr on failure.
+void *noalloc_scratch_address(int device);
 
 // One-shot run of a prepared set on context c (hipsw_banded_extend): the
 // job arrays go to `arena` (the context's grow-only arena when null), HBM

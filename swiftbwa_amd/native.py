@@ -100,6 +100,9 @@ _SIGS = {
     "hipsw_banded_extend": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                       C.POINTER(HipswScoring), C.c_int32, _P,
                                       C.POINTER(HipswStats)]),
+    "hipsw_banded_extend_noalloc": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P, _P,
+                                              _P, C.POINTER(HipswScoring), C.c_int32, _P,
+                                              C.POINTER(HipswStats)]),
     "hipsw_ext_upload": (_P, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                               C.POINTER(HipswScoring)]),
     "hipsw_ext_run": (C.c_int, [_P, _P, C.c_int32, C.POINTER(HipswStats)]),
