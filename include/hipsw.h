@@ -46,6 +46,10 @@ extern "C" {
 /* Dispatch modes of the extension and NW entry points. */
 #define HIPSW_MODE_PLAIN      0   /* one launch per call, input order            */
 #define HIPSW_MODE_BATCHED    1   /* area-binned persistent launch, tuned per bin */
+#define HIPSW_MODE_INPUT_ORDER 2  /* hipsw_ext_run only: the batched form's kernel over the
+                                     jobs in input order, one launch, one work item per
+                                     block -- no binning, sorting or work stealing (a
+                                     measurement baseline) */
 
 typedef struct hipsw_engine hipsw_engine;
 

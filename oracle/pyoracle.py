@@ -37,14 +37,38 @@ def build():
 
 
 _lib = None
+_lib_path = LIB_PATH
+
+
+def use_library(path):
+    """Loads the oracle from another build of the same sources (bench.py's
+    CPU baseline compiles one with -march=native on the measuring host).
+    Must be called before the first oracle call."""
+    global _lib_path
+    if _lib is not None and path != _lib_path:
+        raise RuntimeError("oracle library already loaded from %s" % _lib_path)
+    _lib_path = path
+
+
+def build_native(out_dir):
+    """Compiles the oracle sources with -O3 -march=native into out_dir and
+    returns the library path (None when no C compiler is available)."""
+    out = os.path.join(out_dir, "libswor_native.so")
+    cmd = ["gcc", "-O3", "-march=native", "-std=c11", "-fPIC", "-shared", "-o", out,
+           os.path.join(HERE, "sw_oracle.c"), os.path.join(HERE, "e2e_oracle.c"), "-lpthread", "-lm"]
+    try:
+        subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    except (OSError, subprocess.CalledProcessError):
+        return None
+    return out
 
 
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        if _lib_path == LIB_PATH and not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(_lib_path)
         P = C.c_void_p
         L.or_scoring_default.argtypes = [C.POINTER(Scoring)]
         L.or_scoring_fill_matrix.argtypes = [C.POINTER(Scoring)]

@@ -188,6 +188,14 @@ struct hipsw_ext_set {
     // changes
     hipGraphExec_t graph = nullptr;
     void *graph_scratch = nullptr;
+    // input-order form (HIPSW_MODE_INPUT_ORDER): the batched form's register-
+    // query kernel over the jobs in input order, one work item per block;
+    // io_ok is false when some job needs the generic kernel.  Its descriptors
+    // and items are built on the device at the first such run.
+    bool io_ok = false;
+    hipsw::ExtLaunch io_launch{};
+    size_t io_scratch = 0;
+    void *io_dmem = nullptr;
 };
 
 namespace hipsw {

@@ -351,7 +351,34 @@ int ext_prepare(const hipsw_engine *eng, const TuneTable &tune, int n, const uin
 
     // Batched form (see plan_ext_batched).
     std::vector<uint8_t> any_n(n);
-    for (int k = 0; k < n; ++k) any_n[k] = qn_flag[k] | tn_flag[k];
+    bool some_n = false;
+    for (int k = 0; k < n; ++k) {
+        any_n[k] = qn_flag[k] | tn_flag[k];
+        some_n |= any_n[k] != 0;
+    }
+    // Input-order form: one launch of the register-query kernel at the set's
+    // widest tier and longest query, items in input order.
+    if (n > 0 && !some_n && P.params.std_mat && qmax <= 255 && plain_tier != TIER_I32) {
+        ExtLaunch &L = S.io_launch;
+        L.tier = plain_tier;
+        L.kind = plain_tier == TIER_U8 ? EXT_PACKED : EXT_LANE;
+        L.cls = class_of(qmax);
+        L.has_n = 0;
+        L.cols = qmax + 1;
+        L.lb = ext_pk_default_lb(L.cols);
+        L.global_state = false;
+        L.job_begin = 0;
+        L.job_end = n;
+        if (ext_pk_lds_bytes(L.cols, L.lb) <= (size_t)eng->lds_per_block) {
+            const int per_item = L.kind == EXT_PACKED ? 128 : 64;
+            L.item_begin = 0;
+            L.item_count = (n + per_item - 1) / per_item;
+            L.grid = L.item_count;                  // one item per block
+            L.scratch_bytes = (size_t)L.grid * ext_pk_hbm_bytes_per_wave(L.cols, L.lb);
+            S.io_scratch = (L.scratch_bytes + 255) & ~(size_t)255;
+            S.io_ok = true;
+        }
+    }
     S.batched_scratch = plan_ext_batched(eng, tune, n, P.ql.data(), P.tl.data(), P.h0.data(),
                                          P.w.data(), any_n.data(), max_mat,
                                          P.params.std_mat != 0, P.order, P.items, S.launches);
@@ -380,6 +407,9 @@ hipError_t ext_bind_upload(const ExtPrep &P, void *dmem, hipsw_ext_set *set, hip
     set->launches = S.launches;
     set->batched_scratch = S.batched_scratch;
     set->items_cap = S.items_cap;
+    set->io_ok = S.io_ok;
+    set->io_launch = S.io_launch;
+    set->io_scratch = S.io_scratch;
     for (int k = 0; k < 3; ++k) set->tier_jobs[k] = S.tier_jobs[k];
     set->dmem = dmem;
     set->dmem_bytes = P.arena_bytes;
@@ -423,9 +453,36 @@ hipError_t ext_bind_upload(const ExtPrep &P, void *dmem, hipsw_ext_set *set, hip
     return e;
 }
 
+// Descriptors and work items of the input-order form: job k's descriptor at
+// position k, item i = jobs [i * per_item, ...).
+__global__ void build_io_kernel(ExtDev d, int n, int per_item, PkMeta *pk, WorkItem *items)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        pk[k] = PkMeta{(uint32_t)k, d.q_word[k], d.t_word[k], (uint16_t)d.q_len[k],
+                       (uint16_t)d.t_len[k], d.h0[k], d.w[k], d.eb[k], 0u};
+        if (k % per_item == 0)
+            items[k / per_item] = WorkItem{(uint32_t)k, (uint32_t)min(per_item, n - k)};
+    }
+}
+
+static size_t io_pk_bytes(int n) { return ((size_t)n * sizeof(PkMeta) + 255) & ~(size_t)255; }
+
 hipError_t ext_enqueue(const hipsw_ext_set *set, int mode, void *gs, uint32_t *counters,
                        hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
+    if (mode == HIPSW_MODE_INPUT_ORDER) {
+        if (!set->io_ok || !set->io_dmem) return hipErrorInvalidValue;
+        ExtDev d = set->dev;
+        d.pk = static_cast<const PkMeta *>(set->io_dmem);
+        const WorkItem *items = reinterpret_cast<const WorkItem *>(
+            static_cast<const char *>(set->io_dmem) + io_pk_bytes(set->n));
+        hipError_t e = launch_zero_u32(reinterpret_cast<uint32_t *>(set->dev.cells), 2, st);
+        if (e == hipSuccess && e0) e = hipEventRecord(e0, st);
+        if (e == hipSuccess)
+            e = launch_ext_bin(set->io_launch, d, set->params, nullptr, items, nullptr, gs, st);
+        if (e == hipSuccess && e1) e = hipEventRecord(e1, st);
+        return e;
+    }
     hipError_t e = launch_zero_u32(reinterpret_cast<uint32_t *>(set->dev.cells), 2, st);
     if (e == hipSuccess && mode == HIPSW_MODE_BATCHED) e = launch_zero_u32(counters, 64, st);
     if (e == hipSuccess && e0) e = hipEventRecord(e0, st);
@@ -521,7 +578,8 @@ extern "C" int hipsw_ext_run(hipsw_engine *eng, hipsw_ext_set *set, int32_t mode
                              hipsw_stats *stats)
 {
     if (!eng || !set) return HIPSW_E_INVALID;
-    if (mode != HIPSW_MODE_PLAIN && mode != HIPSW_MODE_BATCHED) return HIPSW_E_INVALID;
+    if (mode != HIPSW_MODE_PLAIN && mode != HIPSW_MODE_BATCHED && mode != HIPSW_MODE_INPUT_ORDER)
+        return HIPSW_E_INVALID;
     std::lock_guard<std::recursive_mutex> lock(eng->mu);
     hipSetDevice(eng->device);
     ExecCtx &c = eng->main;
@@ -534,16 +592,36 @@ extern "C" int hipsw_ext_run(hipsw_engine *eng, hipsw_ext_set *set, int32_t mode
             eng->set_error("hipsw_ext_run: too many bins");
             return HIPSW_E_UNSUPPORTED;
         }
+        if (mode == HIPSW_MODE_INPUT_ORDER) {
+            if (!set->io_ok) {
+                eng->set_error("hipsw_ext_run: the input-order form needs every job in the 8- or "
+                               "16-bit tier, the standard matrix, no N and queries below 256 bases");
+                return HIPSW_E_UNSUPPORTED;
+            }
+            if (!set->io_dmem) {
+                const int per_item = set->io_launch.kind == EXT_PACKED ? 128 : 64;
+                const size_t b = io_pk_bytes(set->n) + (size_t)set->io_launch.item_count * sizeof(WorkItem);
+                if (!eng->check(hipMalloc(&set->io_dmem, b), "hipMalloc(input-order form)"))
+                    return HIPSW_E_DEVICE;
+                hipLaunchKernelGGL(build_io_kernel, dim3(std::min((set->n + 255) / 256, 4096)), dim3(256),
+                                   0, c.stream, set->dev, set->n, per_item,
+                                   static_cast<PkMeta *>(set->io_dmem),
+                                   reinterpret_cast<WorkItem *>(static_cast<char *>(set->io_dmem) +
+                                                                io_pk_bytes(set->n)));
+                if (!eng->check(hipGetLastError(), "build input-order form")) return HIPSW_E_DEVICE;
+            }
+        }
         const size_t need = mode == HIPSW_MODE_PLAIN ? (set->plain_global ? set->plain_scratch : 0)
-                                                     : set->batched_scratch;
+                            : mode == HIPSW_MODE_BATCHED ? set->batched_scratch
+                                                         : set->io_scratch;
         void *gs = nullptr;
         if (need && !(gs = c.scratch.get(need))) {
             eng->set_error("hipsw_ext_run: scratch allocation failed");
             return HIPSW_E_DEVICE;
         }
-        if (mode == HIPSW_MODE_PLAIN) {
+        if (mode != HIPSW_MODE_BATCHED) {
             if (!eng->check(ext_enqueue(set, mode, gs, c.counters, c.stream, c.ev0, c.ev1),
-                            "launch ext_plain"))
+                            "launch extension"))
                 return HIPSW_E_DEVICE;
             launches = 1;
         } else {
@@ -590,6 +668,7 @@ extern "C" void hipsw_ext_free(hipsw_engine *eng, hipsw_ext_set *set)
         hipStreamSynchronize(eng->main.stream);
         if (set->graph) hipGraphExecDestroy(set->graph);
         if (set->dmem && set->own_dmem) hipFree(set->dmem);
+        if (set->io_dmem) hipFree(set->io_dmem);
     }
     delete set;
 }

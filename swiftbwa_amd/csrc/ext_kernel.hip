@@ -1545,7 +1545,8 @@ __device__ void ext_wave_ln(const ExtDev &d, const ExtParams &p, const PkMeta *m
 }
 
 // Persistent kernel of the one-job-per-lane form: items of up to 64 jobs, no
-// N base (the planner bins N-holding jobs apart).
+// N base (the planner bins N-holding jobs apart).  With a null counter every
+// block runs the one item of its index (the input-order form).
 template <bool SYM, int NQ, int LB>
 __global__ __launch_bounds__(64) void ext_persistent_ln_kernel(ExtDev d, ExtParams p,
                                                                const WorkItem *items, int n_items,
@@ -1555,10 +1556,14 @@ __global__ __launch_bounds__(64) void ext_persistent_ln_kernel(ExtDev d, ExtPara
     uint32_t *col = reinterpret_cast<uint32_t *>(lds_raw) + threadIdx.x * 4;
     uint32_t *gcol = gscratch ? gscratch + (size_t)blockIdx.x * ext_hyb_words(NQ, LB) + threadIdx.x * 4
                               : nullptr;
-    for (;;) {
+    for (int it = 0;; ++it) {
         int item = 0;
-        if (threadIdx.x == 0) item = (int)atomicAdd(counter, 1u);
-        item = __builtin_amdgcn_readfirstlane(item);
+        if (counter) {                 // work stealing
+            if (threadIdx.x == 0) item = (int)atomicAdd(counter, 1u);
+            item = __builtin_amdgcn_readfirstlane(item);
+        } else {                       // static: block b runs item b only
+            item = it == 0 ? (int)blockIdx.x : n_items;
+        }
         if (item >= n_items) break;
         const WorkItem wi = items[item];
         ext_wave_ln<SYM, NQ, LB>(d, p, d.pk + wi.start, (int)wi.count, col, gcol);
@@ -1578,10 +1583,14 @@ __global__ __launch_bounds__(64) void ext_persistent_pk_kernel(ExtDev d, ExtPara
     uint32_t *col = reinterpret_cast<uint32_t *>(lds_raw) + threadIdx.x * 4;
     uint32_t *gcol = gscratch ? gscratch + (size_t)blockIdx.x * ext_hyb_words(NQ, LB) + threadIdx.x * 4
                               : nullptr;
-    for (;;) {
+    for (int it = 0;; ++it) {
         int item = 0;
-        if (threadIdx.x == 0) item = (int)atomicAdd(counter, 1u);
-        item = __builtin_amdgcn_readfirstlane(item);
+        if (counter) {                 // work stealing
+            if (threadIdx.x == 0) item = (int)atomicAdd(counter, 1u);
+            item = __builtin_amdgcn_readfirstlane(item);
+        } else {                       // static: block b runs item b only
+            item = it == 0 ? (int)blockIdx.x : n_items;
+        }
         if (item >= n_items) break;
         const WorkItem wi = items[item];
         ext_wave_pk<SYM, NQ, LB>(d, p, d.pk + wi.start, (int)wi.count, col, gcol);

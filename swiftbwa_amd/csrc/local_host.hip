@@ -177,12 +177,23 @@ extern "C" int hipsw_local_sw(hipsw_engine *eng, int32_t n, const uint8_t *queri
         if (!ok[k]) memset(out6 + (size_t)k * 6, 0, 6 * sizeof(int32_t));
     }
     if (stats) {
+        // Cells as LocalSWAligner's loops count them (LocalSWAligner.swift:209,
+        // :385): the forward pass's qlen x tlen, again for a task redone by the
+        // scalar pass, plus the reverse pass over the (qe+1) x (te+1) prefixes
+        // of a found alignment.  A striped pass that stops at an overflow is
+        // counted whole.
+        int64_t cells = 0;
+        for (uint32_t k : striped) cells += (int64_t)pj.ql[k] * pj.tl[k];
+        for (uint32_t k : scalar) cells += (int64_t)pj.ql[k] * pj.tl[k];
+        for (int k = 0; k < n; ++k)
+            if (ok[k]) cells += (int64_t)(out6[(size_t)k * 6 + 2] + 1) * (out6[(size_t)k * 6 + 4] + 1);
         memset(stats, 0, sizeof *stats);
         stats->kernel_ms = kernel_ms;
         stats->total_ms = now_ms() - t0;
         stats->n_jobs = n;
         stats->launches = launches;
         stats->promoted = (int64_t)scalar.size();
+        stats->cells = cells;
     }
     return HIPSW_OK;
 }

@@ -21,6 +21,7 @@ HIPSW_E_PENDING = -5
 
 MODE_PLAIN = 0
 MODE_BATCHED = 1
+MODE_INPUT_ORDER = 2      # hipsw_ext_run only: batched kernel, input order, no binning
 
 
 class HipswScoring(C.Structure):

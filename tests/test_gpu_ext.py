@@ -396,3 +396,47 @@ def test_batched_graph_replays(engine):
     finally:
         lib.hipsw_ext_free(engine.handle, s1)
         lib.hipsw_ext_free(engine.handle, s2)
+
+
+def test_input_order_form_exact(engine):
+    """HIPSW_MODE_INPUT_ORDER (the batched form's own kernel over the jobs in
+    input order, one work item per block): oracle results and cell counts on
+    an 8-bit set (packed kernel) and on a set with 16-bit jobs (one job per
+    lane); refused for a set that needs the generic kernel."""
+    import ctypes as C
+    from swiftbwa_amd.metalsw import _ext_arrays
+    sp = ScoringParameters()
+    sc = sp.to_c()
+    lib = engine.lib
+    for js in (wl.generate(wl.EXT_UNIFORM, 41, 20000), wl.generate(wl.EXT_MIXED, 42, 30000)):
+        ref, cells = orc.ksw_extend_batch(js, oracle_sc(sp), nthreads=8)
+        a = _ext_arrays(js)
+        st = lib.hipsw_ext_upload(engine.handle, js["n"], native.ptr(a["queries"]),
+                                  native.ptr(a["q_off"]), native.ptr(a["q_len"]),
+                                  native.ptr(a["targets"]), native.ptr(a["t_off"]),
+                                  native.ptr(a["t_len"]), native.ptr(a["h0"]), native.ptr(a["w"]),
+                                  native.ptr(a["end_bonus"]), C.byref(sc))
+        assert st
+        try:
+            out = np.zeros((js["n"], 6), np.int32)
+            for rep in range(2):
+                s = native.HipswStats()
+                assert lib.hipsw_ext_run(engine.handle, st, 2, C.byref(s)) == 0, engine.last_error()
+                assert lib.hipsw_ext_download(engine.handle, st, native.ptr(out)) == 0
+                assert np.array_equal(out, ref)
+                assert s.cells == cells and s.launches == 1
+        finally:
+            lib.hipsw_ext_free(engine.handle, st)
+    bad = wl.generate(wl.EXT_UNIFORM, 43, 500)
+    bad["queries"] = bad["queries"].copy()
+    bad["queries"][5] = 4                                   # an N: generic kernel only
+    a = _ext_arrays(bad)
+    st = lib.hipsw_ext_upload(engine.handle, bad["n"], native.ptr(a["queries"]),
+                              native.ptr(a["q_off"]), native.ptr(a["q_len"]),
+                              native.ptr(a["targets"]), native.ptr(a["t_off"]),
+                              native.ptr(a["t_len"]), native.ptr(a["h0"]), native.ptr(a["w"]),
+                              native.ptr(a["end_bonus"]), C.byref(sc))
+    try:
+        assert lib.hipsw_ext_run(engine.handle, st, 2, None) == native.HIPSW_E_UNSUPPORTED
+    finally:
+        lib.hipsw_ext_free(engine.handle, st)

@@ -44,6 +44,17 @@ def test_rescue_windows(engine):
     check(engine, wl.generate(wl.LOCAL, 5, 20000), min_sub=19)
 
 
+def test_cell_count(engine):
+    """hipsw_local_sw counts the cells the reference's loops visit (forward
+    pass, scalar redo, reverse pass), equal to the oracle's count on tasks
+    that do not overflow the striped pass."""
+    js = wl.generate(wl.LOCAL, 7, 4000)
+    _, _, cells = orc.local_batch(js, oracle_sc(ScoringParameters()), min_sub_score=19, nthreads=8)
+    st = {}
+    engine.local_sw(js, ScoringParameters(), min_sub_score=19, stats=st)
+    assert st["cells"] == cells
+
+
 def test_rescue_windows_no_score2(engine):
     check(engine, wl.generate(wl.LOCAL, 6, 5000), min_sub=0)
 
