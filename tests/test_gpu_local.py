@@ -110,3 +110,34 @@ def test_all_stripe_counts(engine):
     # into larger ones (padded waves) and full groups (exact waves)
     check(engine, _random(np.random.default_rng(35), 6000, 250, 320), min_sub=15)
 
+
+
+def test_pinned_vectors(engine):
+    """The hand-derived vectors that pin the oracle where the reference's
+    striped pass differs from textbook SW (tests/oracle_pins.py): one-pass
+    lazy-F, its stop on a tie, stripe-then-lane ties, a padding-lane
+    maximum under free gaps, and the score2 exclusion radius."""
+    from tests import oracle_pins as P
+    sp = ScoringParameters()
+    free = ScoringParameters(gapOpenPenalty=0, gapExtendPenalty=0, gapOpenPenaltyDeletion=0,
+                             gapExtendPenaltyDeletion=0)
+    q22 = [0, 1, 1, 1, 1, 0, 0, 0, 1, 1, 1, 0, 0, 1, 0, 0, 1, 0, 1, 1, 1, 0]
+    t20 = [0, 1, 1, 1, 1, 0, 0, 0, 1, 1, 0, 1, 0, 0, 1, 0, 1, 1, 1, 0]
+    tie = [1] * 20
+    tie[3] = tie[4] = 0
+    cases = [(P.X20 + [3] * 4 + P.Y20, [3] * 5 + P.X20 + P.Y20 + [3] * 5, sp, 0, 29),
+             (P.X20 + [3] * 5 + P.Y20, [3] * 5 + P.X20 + P.Y20 + [3] * 5, sp, 0, 23),
+             (q22, t20, sp, 0, (10, 0, 9, 0, 9, 0)),
+             (tie, [0], sp, 0, (1, 4, 4, 0, 0, 0)),
+             ([1] * 17 + [0], [1, 0], free, 0, (2, 0, 0, 0, 1, 0))]
+    for g, s2 in ((5, 0), (6, 25)):
+        cases.append((P.Q30, [3] * 12 + P.Q30 + [3] * g + P.Q30[:25] + [3] * 12, sp, 19,
+                      (30, 0, 29, 12, 41, s2)))
+    for q, t, sc, ms, want in cases:
+        r = LocalSWDispatcher.dispatchBatch([LocalSWTask(bytes(q), bytes(t), sc)], engine,
+                                            minSubScore=ms)[0]
+        got = (r.score, r.queryBegin, r.queryEnd, r.targetBegin, r.targetEnd, r.score2)
+        if isinstance(want, int):
+            assert got[0] == want, (q, t, got)
+        else:
+            assert got == want, (q, t, got, want)

@@ -142,3 +142,35 @@ def test_wave_per_job_edges(engine):
     sp = ScoringParameters(gapOpenPenalty=0, gapExtendPenalty=2, gapOpenPenaltyDeletion=3,
                            gapExtendPenaltyDeletion=1, mismatchPenalty=3)
     check(engine, _edges(np.random.default_rng(25)), sp)
+
+
+def test_pinned_tie_vectors(engine):
+    """Equal-score path choices pinned in tests/oracle_pins.py: the lambda
+    fixture's deletion inside AAC AAC (leftmost, 20M3D20M), homopolymer
+    deletion / insertion (1D4M / 1I4M), and binary pairs with many equal
+    paths under cheap gaps -- each against the oracle, op for op."""
+    import itertools
+    from tests import oracle_pins as P
+    from tests.ref_restatements import global_tb_full
+    seq = "".join(l.strip() for l in open(P.os.path.join(P.HERE, "golden", "lambda", "lambda.fa"))
+                  if not l.startswith(">"))
+    enc = {"A": 0, "C": 1, "G": 2, "T": 3}
+    ref = [enc[c] for c in seq[30030:30073]]
+    read = ref[:20] + ref[23:]
+    sp = ScoringParameters()
+    T = lambda q, t, w, s=sp: GlobalSWTask(bytes(q), bytes(t), -(1 << 30), w, s)
+    res = GlobalSWDispatcher.dispatchBatch([T(read, ref, 10), T([0, 0, 0, 1], [0, 0, 0, 0, 1], 3),
+                                            T([0, 0, 0, 0, 1], [0, 0, 0, 1], 3)], engine)
+    assert res[0].cigar == [20 << 4, 3 << 4 | 2, 20 << 4] and res[0].score == 31
+    assert res[1].cigar == [1 << 4 | 2, 4 << 4]
+    assert res[2].cigar == [1 << 4 | 1, 4 << 4]
+    cheap = ScoringParameters(mismatchPenalty=1, gapOpenPenalty=1, gapExtendPenalty=1,
+                              gapOpenPenaltyDeletion=1, gapExtendPenaltyDeletion=1)
+    osc = oracle_sc(cheap)
+    seqs = [list(s) for n in range(1, 5) for s in itertools.product((0, 1), repeat=n)]
+    pairs = [(q, t) for q in seqs for t in seqs]
+    res = GlobalSWDispatcher.dispatchBatch([T(q, t, 2, cheap) for q, t in pairs], engine)
+    assert len(res) == len(pairs)
+    for (q, t), r in zip(pairs, res):
+        want = global_tb_full(q, t, list(osc.mat), 1, 1, 1, 1, 2)
+        assert (r.score, r.cigar) == want, (q, t, r, want)
