@@ -143,6 +143,22 @@ def test_full_size_uniform_properties(engine):
     assert np.array_equal(a[np.sort(idx)], ref)
 
 
+def test_full_size_uneven_4M_every_row(engine):
+    """The whole uneven 4,000,000-job set (the benchmarked one): every row and
+    the cell count against the oracle, batched and plain."""
+    import os
+    js = wl.generate(wl.EXT_MIXED, 4, 4000000)
+    ref, cells = orc.ksw_extend_batch(js, nthreads=min(16, os.cpu_count() or 1))
+    sp = ScoringParameters()
+    for mode in MODES:
+        print("uneven 4M mode %d" % mode, flush=True)
+        st = {}
+        got = engine.banded_extend(js, sp, mode=mode, stats=st)
+        bad = np.nonzero((got != ref).any(axis=1))[0]
+        assert len(bad) == 0, (mode, len(bad), bad[:5])
+        assert st["cells"] == cells
+
+
 def test_autotune_both_layouts_exact(engine):
     """The autotuner times the packed and the one-job-per-lane layouts on each
     8-bit bin; the lane layout runs last, so the sampled jobs' results on the

@@ -174,3 +174,24 @@ def test_pinned_tie_vectors(engine):
     for (q, t), r in zip(pairs, res):
         want = global_tb_full(q, t, list(osc.mat), 1, 1, 1, 1, 2)
         assert (r.score, r.cigar) == want, (q, t, r, want)
+
+
+def test_full_size_500k_every_row(engine):
+    """The whole 500,000-pair NW set (the benchmarked one): every pair's
+    score, op count, CIGAR ops and the cell count against the oracle, in the
+    batched and the plain form."""
+    import os
+    js = wl.generate(wl.NW, 3, 500000)
+    sp = ScoringParameters()
+    cap = 512
+    rs, rc, rn, cells = orc.global_batch(js, oracle_sc(sp), max_retries=3, cap=cap,
+                                         nthreads=min(16, os.cpu_count() or 1))
+    used = np.arange(cap)[None, :] < np.minimum(rn, cap)[:, None]
+    for mode in MODES:
+        print("NW 500k mode %d" % mode, flush=True)
+        st = {}
+        s, c, n, fl, _ = engine.global_nw(js, sp, max_retries=3, cap=cap, mode=mode, stats=st)
+        assert np.array_equal(s, rs), mode
+        assert np.array_equal(n, rn), mode
+        assert not ((c != rc) & used).any(), mode
+        assert st["cells"] == cells, (mode, st["cells"], cells)
