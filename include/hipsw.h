@@ -87,6 +87,18 @@ void          hipsw_engine_destroy(hipsw_engine *eng);
 const char   *hipsw_last_error(const hipsw_engine *eng);
 /* MetalSWEngine.minBatchSize (MetalSWEngine.swift:39). */
 int32_t       hipsw_min_batch_size(void);
+/* Fault injection, for testing callers' fallbacks (no reference counterpart).
+ * Arms one fault: the countdown-th device allocation (site 1), kernel launch
+ * (site 2) or host<->device copy (site 3) that a call on this engine makes from
+ * now on fails as if the HIP runtime had refused it; the call then returns
+ * HIPSW_E_DEVICE with "injected" in hipsw_last_error, after draining its
+ * stream, and the engine stays usable.  The fault disarms once it fires;
+ * countdown 0 (or site 0) disarms it. */
+#define HIPSW_FAULT_NONE   0
+#define HIPSW_FAULT_ALLOC  1
+#define HIPSW_FAULT_LAUNCH 2
+#define HIPSW_FAULT_COPY   3
+int           hipsw_fault_inject(hipsw_engine *eng, int32_t site, int32_t countdown);
 /* Device name and compute-unit count, for reports. */
 int           hipsw_device_info(const hipsw_engine *eng, char *name, int name_cap,
                                 int32_t *n_cu);

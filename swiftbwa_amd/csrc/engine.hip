@@ -43,6 +43,15 @@ bool hipsw_engine::check(hipError_t e, const char *what) const
     return fail(what, e);
 }
 
+bool hipsw_engine::inject(int site)
+{
+    std::lock_guard<std::mutex> lk(fault_mu);
+    if (fault_site == 0 || fault_site != site) return false;
+    if (--fault_countdown > 0) return false;
+    fault_site = 0;
+    return true;
+}
+
 TuneTable hipsw_engine::tune_snapshot() const
 {
     std::lock_guard<std::mutex> lk(tune_mu);
@@ -363,6 +372,16 @@ extern "C" const char *hipsw_last_error(const hipsw_engine *eng)
 }
 
 extern "C" int32_t hipsw_min_batch_size(void) { return 32; }
+
+extern "C" int hipsw_fault_inject(hipsw_engine *eng, int32_t site, int32_t countdown)
+{
+    if (!eng || site < HIPSW_FAULT_NONE || site > HIPSW_FAULT_COPY || countdown < 0)
+        return HIPSW_E_INVALID;
+    std::lock_guard<std::mutex> lk(eng->fault_mu);
+    eng->fault_site = countdown > 0 ? site : HIPSW_FAULT_NONE;
+    eng->fault_countdown = countdown;
+    return HIPSW_OK;
+}
 
 extern "C" int hipsw_device_info(const hipsw_engine *eng, char *name, int name_cap, int32_t *n_cu)
 {

@@ -75,8 +75,14 @@ struct hipsw_engine {
     // device address of the allocation-free entry point's static scratch
     // array on this engine's device (hipGetSymbolAddress at creation)
     void *noalloc_scratch = nullptr;
+    // fault injection (hipsw_fault_inject): the armed site and its countdown
+    std::mutex fault_mu;
+    int fault_site = 0, fault_countdown = 0;
 
     hipsw::TuneTable tune_snapshot() const;
+    // True when an armed fault of `site` fires at this check (it then disarms);
+    // the caller fails the step as if the HIP runtime had.
+    bool inject(int site);
     // Error text of the calling thread's last failed call on this engine.
     void set_error(const std::string &msg) const;
     bool fail(const char *what, hipError_t e) const;

@@ -683,32 +683,44 @@ namespace hipsw {
 int ext_run_once(hipsw_engine *eng, ExecCtx &c, const ExtPrep &P, int mode, void *arena,
                  void *gs_fixed, uint32_t *counters, int32_t *out, hipsw_stats *stats, double t0)
 {
-    hipsw_ext_set set;
-    if (!arena && !(arena = c.arena.get(P.arena_bytes))) {
-        eng->set_error("hipsw_banded_extend: arena allocation failed");
+    // every failure drains the stream first: copies from the caller's and the
+    // prepared host arrays may still be queued when a later step fails
+    // (what: the failed step; a runtime failure keeps the runtime's text)
+    auto fail = [&](const char *what, bool runtime = false) {
+        const std::string rt = runtime ? std::string(" (") + thread_error(eng) + ")" : "";
+        (void)hipStreamSynchronize(c.stream);
+        eng->set_error(std::string("hipsw_banded_extend: ") + what + rt);
         return HIPSW_E_DEVICE;
-    }
-    if (!eng->check(ext_bind_upload(P, arena, &set, c.stream), "upload")) return HIPSW_E_DEVICE;
+    };
+    hipsw_ext_set set;
+    if (!arena && eng->inject(HIPSW_FAULT_ALLOC)) return fail("arena allocation failed (injected)");
+    if (!arena && !(arena = c.arena.get(P.arena_bytes))) return fail("arena allocation failed");
+    if (eng->inject(HIPSW_FAULT_COPY)) return fail("upload failed (injected)");
+    if (!eng->check(ext_bind_upload(P, arena, &set, c.stream), "upload"))
+        return fail("upload failed", true);
     if (mode == HIPSW_MODE_BATCHED && set.launches.size() > 64) {
+        (void)hipStreamSynchronize(c.stream);
         eng->set_error("hipsw_banded_extend: too many bins");
         return HIPSW_E_UNSUPPORTED;
     }
     const size_t need = mode == HIPSW_MODE_PLAIN ? (set.plain_global ? set.plain_scratch : 0)
                                                  : set.batched_scratch;
     void *gs = gs_fixed;
-    if (need && !gs && !(gs = c.scratch.get(need))) {
-        eng->set_error("hipsw_banded_extend: scratch allocation failed");
-        return HIPSW_E_DEVICE;
-    }
+    if (need && !gs && eng->inject(HIPSW_FAULT_ALLOC))
+        return fail("scratch allocation failed (injected)");
+    if (need && !gs && !(gs = c.scratch.get(need))) return fail("scratch allocation failed");
     unsigned long long cells = 0;
+    if (eng->inject(HIPSW_FAULT_LAUNCH)) return fail("kernel launch failed (injected)");
     if (!eng->check(ext_enqueue(&set, mode, gs, counters ? counters : c.counters, c.stream, c.ev0,
-                                c.ev1), "launch extension") ||
-        !eng->check(hipMemcpyAsync(out, set.dev.out, (size_t)P.n * 6 * sizeof(int32_t),
+                                c.ev1), "launch extension"))
+        return fail("kernel launch failed", true);
+    if (eng->inject(HIPSW_FAULT_COPY)) return fail("download failed (injected)");
+    if (!eng->check(hipMemcpyAsync(out, set.dev.out, (size_t)P.n * 6 * sizeof(int32_t),
                                    hipMemcpyDeviceToHost, c.stream), "download") ||
         !eng->check(hipMemcpyAsync(&cells, set.dev.cells, sizeof cells, hipMemcpyDeviceToHost,
                                    c.stream), "download cells") ||
         !eng->check(hipStreamSynchronize(c.stream), "extension"))
-        return HIPSW_E_DEVICE;
+        return fail("download failed", true);
     float ms = 0.f;
     hipEventElapsedTime(&ms, c.ev0, c.ev1);
     fill_ext_stats(stats, &set, ms, t0,

@@ -87,9 +87,9 @@ extern "C" int hipsw_local_sw(hipsw_engine *eng, int32_t n, const uint8_t *queri
     SlotLease lease(eng);
     ExecCtx &c = lease.ctx();
     hipSetDevice(eng->device);
-    void *dmem = c.arena.get(bytes);
+    void *dmem = eng->inject(HIPSW_FAULT_ALLOC) ? nullptr : c.arena.get(bytes);
     if (!dmem) {
-        eng->set_error("hipsw_local_sw: arena allocation failed");
+        eng->set_error("hipsw_local_sw: arena allocation failed (injected or out of memory)");
         return HIPSW_E_DEVICE;
     }
     char *cur = reinterpret_cast<char *>(dmem);
@@ -103,6 +103,10 @@ extern "C" int hipsw_local_sw(hipsw_engine *eng, int32_t n, const uint8_t *queri
     uint32_t *d_ord_s = carve<uint32_t>(cur, n), *d_ord_c = carve<uint32_t>(cur, n);
     uint8_t *d_colmax = carve<uint8_t>(cur, colmax_bytes);
     bool good = true;
+    if (eng->inject(HIPSW_FAULT_COPY)) {
+        eng->set_error("hipsw_local_sw: upload failed (injected)");
+        good = false;
+    }
     auto up = [&](void *dst, const void *src, size_t b) {
         if (good && b) good = eng->check(hipMemcpyAsync(dst, src, b, hipMemcpyHostToDevice, c.stream), "upload");
     };
@@ -123,6 +127,10 @@ extern "C" int hipsw_local_sw(hipsw_engine *eng, int32_t n, const uint8_t *queri
     int64_t launches = 0;
     std::vector<int32_t> status(n, LOCAL_NIL);
     if (good) good = eng->check(hipMemsetAsync(d_status, 0, (size_t)n * 4, c.stream), "memset");
+    if (good && eng->inject(HIPSW_FAULT_LAUNCH)) {
+        eng->set_error("hipsw_local_sw: kernel launch failed (injected)");
+        good = false;
+    }
     if (good && !striped.empty()) {
         good = eng->check(hipEventRecord(c.ev0, c.stream), "event") &&
                eng->check(launch_local_striped(dev, p, d_ord_s, grp_off, d_colmax,
@@ -148,9 +156,9 @@ extern "C" int hipsw_local_sw(hipsw_engine *eng, int32_t n, const uint8_t *queri
         for (uint32_t k : scalar) { qc = std::max(qc, pj.ql[k]); tc = std::max(tc, pj.tl[k]); }
         const int blocks = (int)((scalar.size() + 63) / 64);
         const size_t sbytes = (size_t)blocks * 64 * (2 * (tc + 1) + qc) * sizeof(int);
-        int *scr = reinterpret_cast<int *>(c.scratch.get(sbytes));
+        int *scr = eng->inject(HIPSW_FAULT_ALLOC) ? nullptr : reinterpret_cast<int *>(c.scratch.get(sbytes));
         good = scr != nullptr;
-        if (!good) eng->set_error("hipsw_local_sw: scratch allocation failed");
+        if (!good) eng->set_error("hipsw_local_sw: scratch allocation failed (injected or out of memory)");
         if (good) good = eng->check(hipMemcpyAsync(d_ord_c, scalar.data(), scalar.size() * 4,
                                                    hipMemcpyHostToDevice, c.stream), "upload");
         if (good)
@@ -166,12 +174,19 @@ extern "C" int hipsw_local_sw(hipsw_engine *eng, int32_t n, const uint8_t *queri
             ++launches;
         }
     }
+    if (good && eng->inject(HIPSW_FAULT_COPY)) {
+        eng->set_error("hipsw_local_sw: download failed (injected)");
+        good = false;
+    }
     if (good) good = eng->check(hipMemcpyAsync(out6, d_out, (size_t)n * 6 * 4, hipMemcpyDeviceToHost,
                                                c.stream), "download");
     if (good) good = eng->check(hipMemcpyAsync(status.data(), d_status, (size_t)n * 4,
                                                hipMemcpyDeviceToHost, c.stream), "download");
     if (good) good = eng->check(hipStreamSynchronize(c.stream), "download");
-    if (!good) return HIPSW_E_DEVICE;
+    if (!good) {
+        (void)hipStreamSynchronize(c.stream);   // queued copies read host arrays freed on return
+        return HIPSW_E_DEVICE;
+    }
     for (int k = 0; k < n; ++k) {
         ok[k] = status[k] == LOCAL_OK ? 1 : 0;
         if (!ok[k]) memset(out6 + (size_t)k * 6, 0, 6 * sizeof(int32_t));

@@ -353,9 +353,10 @@ static int nw_run_ctx(hipsw_engine *eng, ExecCtx &c, hipsw_nw_set *set, int mode
                 gbytes = std::max(gbytes, (size_t)L.grid * L.S * 64 * (L.tier == NW_I16 ? 4 : 8));
         }
         const size_t need = zbytes + ((gbytes + 255) & ~(size_t)255) + 256;
-        void *nwbuf = c.nw.get(need);
+        void *nwbuf = eng->inject(HIPSW_FAULT_ALLOC) ? nullptr : c.nw.get(need);
         if (!nwbuf) {
-            eng->set_error("hipsw_nw_run: scratch allocation failed");
+            (void)hipStreamSynchronize(c.stream);
+            eng->set_error("hipsw_nw_run: scratch allocation failed (injected or out of memory)");
             return HIPSW_E_DEVICE;
         }
         uint32_t *zbase = reinterpret_cast<uint32_t *>(nwbuf);
@@ -366,6 +367,10 @@ static int nw_run_ctx(hipsw_engine *eng, ExecCtx &c, hipsw_nw_set *set, int mode
                                                hipMemcpyHostToDevice, c.stream), "upload order");
         if (ok) ok = eng->check(hipMemcpyAsync(set->d_items, items.data(), items.size() * sizeof(WorkItem),
                                                hipMemcpyHostToDevice, c.stream), "upload items");
+        if (ok && eng->inject(HIPSW_FAULT_LAUNCH)) {
+            eng->set_error("hipsw_nw_run: kernel launch failed (injected)");
+            ok = false;
+        }
         if (ok) ok = eng->check(launch_zero_u32(c.counters, 64, c.stream), "zero");
         if (ok) ok = eng->check(hipEventRecord(c.ev0, c.stream), "event");
         int k = 0;
@@ -397,10 +402,17 @@ static int nw_run_ctx(hipsw_engine *eng, ExecCtx &c, hipsw_nw_set *set, int mode
         }
         if (ok) ok = eng->check(hipEventRecord(c.ev1, c.stream), "event");
         if (ok) ok = eng->check(hipEventSynchronize(c.ev1), "nw kernel");
-        if (!ok) return HIPSW_E_DEVICE;
+        if (!ok) {
+            (void)hipStreamSynchronize(c.stream);
+            return HIPSW_E_DEVICE;
+        }
         float ms = 0.f;
         hipEventElapsedTime(&ms, c.ev0, c.ev1);
         kernel_ms += ms;
+        if (eng->inject(HIPSW_FAULT_COPY)) {
+            eng->set_error("hipsw_nw_run: download failed (injected)");
+            return HIPSW_E_DEVICE;
+        }
         if (!eng->check(hipMemcpyAsync(score.data(), set->dev.score, (size_t)n * 4,
                                        hipMemcpyDeviceToHost, c.stream), "download scores") ||
             !eng->check(hipStreamSynchronize(c.stream), "download scores"))
@@ -450,6 +462,10 @@ static int nw_download_ctx(hipsw_engine *eng, hipStream_t st, hipsw_nw_set *set,
 {
     const int n = set->n;
     if (n == 0) return HIPSW_OK;
+    if (eng->inject(HIPSW_FAULT_COPY)) {
+        eng->set_error("hipsw_nw_download: download failed (injected)");
+        return HIPSW_E_DEVICE;
+    }
     bool ok = true;
     auto down = [&](void *dst, const void *src, size_t b) {
         if (ok && dst) ok = eng->check(hipMemcpyAsync(dst, src, b, hipMemcpyDeviceToHost, st), "download");
@@ -460,7 +476,10 @@ static int nw_download_ctx(hipsw_engine *eng, hipStream_t st, hipsw_nw_set *set,
     down(ncig.data(), set->dev.n_cigar, (size_t)n * 4);
     down(out_flags, set->dev.flags, (size_t)n * 4);
     if (ok) ok = eng->check(hipStreamSynchronize(st), "download");
-    if (!ok) return HIPSW_E_DEVICE;
+    if (!ok) {
+        (void)hipStreamSynchronize(st);
+        return HIPSW_E_DEVICE;
+    }
     if (out_n_cigar) memcpy(out_n_cigar, ncig.data(), (size_t)n * 4);
     if (out_cigar)                                       // entries past the CIGAR are zero
         for (int k = 0; k < n; ++k) {
@@ -520,13 +539,20 @@ extern "C" int hipsw_global_nw(hipsw_engine *eng, int32_t n, const uint8_t *quer
     SlotLease lease(eng);
     ExecCtx &c = lease.ctx();
     hipSetDevice(eng->device);
-    void *dmem = c.arena.get(P.arena_bytes);
+    void *dmem = eng->inject(HIPSW_FAULT_ALLOC) ? nullptr : c.arena.get(P.arena_bytes);
     if (!dmem) {
-        eng->set_error("hipsw_global_nw: arena allocation failed");
+        eng->set_error("hipsw_global_nw: arena allocation failed (injected or out of memory)");
         return HIPSW_E_DEVICE;
     }
     hipsw_nw_set set;
-    if (!eng->check(nw_bind_upload(P, dmem, &set, c.stream), "upload")) return HIPSW_E_DEVICE;
+    if (eng->inject(HIPSW_FAULT_COPY)) {
+        eng->set_error("hipsw_global_nw: upload failed (injected)");
+        return HIPSW_E_DEVICE;
+    }
+    if (!eng->check(nw_bind_upload(P, dmem, &set, c.stream), "hipsw_global_nw: upload")) {
+        (void)hipStreamSynchronize(c.stream);
+        return HIPSW_E_DEVICE;
+    }
     int rc = nw_run_ctx(eng, c, &set, mode, stats);
     if (rc == HIPSW_OK)
         rc = nw_download_ctx(eng, c.stream, &set, out_score, out_cigar, out_n_cigar, out_flags,

@@ -336,7 +336,8 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
                          span<unsigned long long>(2);
     r.host_ms = now_ms_s() - t_host0;
     void *dmem = nullptr;
-    if (hipMallocAsync(&dmem, bytes, st) != hipSuccess) return HIPSW_E_DEVICE;
+    if (eng->inject(HIPSW_FAULT_ALLOC) || hipMallocAsync(&dmem, bytes, st) != hipSuccess)
+        return HIPSW_E_DEVICE;
     // the phases' scratch buffers are freed in stream order before the next
     // phase allocates, so the footprint is the arena plus the largest of them
     auto note_phase = [&](size_t b) { r.dev_peak = std::max<int64_t>(r.dev_peak, (int64_t)(bytes + b)); };
@@ -379,6 +380,7 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
                            srv->d_pac, srv->gl, d_win, n, d_seq);   // 8 windows per block
         ok = hipGetLastError() == hipSuccess;
     }
+    if (ok && eng->inject(HIPSW_FAULT_LAUNCH)) ok = false;
     // --- extension ---
     if (ok && r.n_ext) {
         up(d_order, ext_order.data(), ext_order.size() * 4);
@@ -528,6 +530,7 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
     auto down = [&](void *dst, const void *src, size_t b) {
         if (ok && b) ok = hipMemcpyAsync(dst, src, b, hipMemcpyDeviceToHost, st) == hipSuccess;
     };
+    if (ok && eng->inject(HIPSW_FAULT_COPY)) ok = false;
     // only the rows each kind reads: extension and local results, local
     // status, NW score / op count / flags
     down(out6, d_out6, (size_t)nw0 * 6 * 4);
@@ -560,8 +563,10 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
         if (cm) hipFreeAsync(cm, st);
     }
     hipFreeAsync(dmem, st);
-    if (ok) ok = hipStreamSynchronize(st) == hipSuccess;
-    if (!ok) return HIPSW_E_DEVICE;
+    // drained on every path: the next chunk of this slot reuses the pinned
+    // staging that queued copies may still read or write
+    const bool drained = hipStreamSynchronize(st) == hipSuccess;
+    if (!ok || !drained) return HIPSW_E_DEVICE;
     // result rows out of the pinned staging, in parallel
     r.ext_out.resize((size_t)r.n_ext * 6);
     parallel_for(r.n_ext, host_threads(), [&](int64_t lo, int64_t hi) {

@@ -95,6 +95,12 @@ class HipSWEngine:
                 cls._shared = None
         return cls._shared
 
+    def fault_inject(self, site, countdown=1):
+        """Arms hipsw_fault_inject: the countdown-th allocation / launch / copy
+        of a later call fails (a test hook for callers' fallbacks)."""
+        if self.lib.hipsw_fault_inject(self.handle, int(site), int(countdown)):
+            raise ValueError("bad fault site")
+
     def last_error(self) -> str:
         return (self.lib.hipsw_last_error(self.handle) or b"").decode()
 

@@ -19,6 +19,8 @@ HIPSW_E_UNSUPPORTED = -3
 HIPSW_E_BUDGET = -4
 HIPSW_E_PENDING = -5
 
+FAULT_NONE, FAULT_ALLOC, FAULT_LAUNCH, FAULT_COPY = 0, 1, 2, 3
+
 MODE_PLAIN = 0
 MODE_BATCHED = 1
 MODE_INPUT_ORDER = 2      # hipsw_ext_run only: batched kernel, input order, no binning
@@ -97,6 +99,7 @@ _SIGS = {
     "hipsw_engine_destroy": (None, [_P]),
     "hipsw_last_error": (C.c_char_p, [_P]),
     "hipsw_min_batch_size": (C.c_int32, []),
+    "hipsw_fault_inject": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "hipsw_device_info": (C.c_int, [_P, C.c_char_p, C.c_int, _I32P]),
     "hipsw_banded_extend": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                       C.POINTER(HipswScoring), C.c_int32, _P,
