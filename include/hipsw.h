@@ -248,6 +248,33 @@ int hipsw_cigar_generate(hipsw_engine *eng, int32_t n,
                          int64_t *out_pos, uint32_t *out_cigar, int32_t *out_n_cigar,
                          char *out_md, int32_t *out_flags, hipsw_stats *stats);
 
+/* The same generator in two host steps around an NW the caller runs (the
+ * serving loop's NW jobs, targets fetched from the resident .pac).
+ * hipsw_cigar_route writes each region's band (out_w) and route: 0 the fast
+ * path, 1 NW (query segment vs reference segment, band out_w, up to 3
+ * doublings while the score is below true_score, as hipsw_global_nw with
+ * max_retries 3 runs it), 2 an empty segment.  hipsw_cigar_finish takes the
+ * NW results of the route-1 regions in region order -- score, the used ops
+ * back to back at nw_cigar[nw_cig_off[s] ..), the full op count, the flags,
+ * and the op cap the NW ran with -- and writes what hipsw_cigar_generate
+ * writes. */
+int hipsw_cigar_route(int32_t n, const uint16_t *q_len, const uint16_t *r_len,
+                      const int32_t *qb, const int32_t *qe, const int32_t *true_score,
+                      const int32_t *initial_w, const hipsw_scoring *sc, int32_t match_score,
+                      int32_t mismatch_penalty, int32_t *out_w, uint8_t *out_route);
+int hipsw_cigar_finish(int32_t n,
+                       const uint8_t *qsegs, const uint32_t *q_off, const uint16_t *q_len,
+                       const uint8_t *rsegs, const uint32_t *r_off, const uint16_t *r_len,
+                       const int32_t *qb, const int32_t *qe, const int32_t *read_len,
+                       const uint8_t *is_reverse, const int32_t *true_score,
+                       const int64_t *ref_pos, const uint8_t *route,
+                       const int32_t *nw_score, const uint32_t *nw_cigar,
+                       const uint32_t *nw_cig_off, const int32_t *nw_n_cigar,
+                       const int32_t *nw_flags, int32_t nw_cap, int32_t cap, int32_t md_cap,
+                       int32_t *out_score, int32_t *out_nm, int64_t *out_pos,
+                       uint32_t *out_cigar, int32_t *out_n_cigar, char *out_md,
+                       int32_t *out_flags);
+
 /* ---- chain extension (replaces ExtensionAligner.extend,
  *      ExtensionAligner.swift:114-346, for a batch of chains) ----
  * Chain c owns seeds [chain_seed_off[c], chain_seed_off[c+1]) (MemSeed fields
@@ -287,6 +314,38 @@ int hipsw_extend_chains(hipsw_engine *eng, int32_t n_chains, const uint32_t *cha
                         hipsw_get_reference_fn get_ref, void *ref_ctx, const hipsw_scoring *sc,
                         const hipsw_ext_opts *opt, hipsw_region *out, int32_t *out_n,
                         hipsw_stats *stats);
+
+/* The same driver as a four-pass state machine, for callers that run the
+ * extensions themselves (the serving loop: hipsw_chunk windows fetched on the
+ * device).  hipsw_chain_begin copies the chains; then four times
+ * hipsw_chain_next (the pass's jobs: queries, and targets as getReference
+ * windows (t_pos, t_len), reversed after fetching when t_rev) and
+ * hipsw_chain_feed (n x 6 results in SWResult order, as hipsw_banded_extend
+ * returns them); then hipsw_chain_finish writes the regions as
+ * hipsw_extend_chains does.  The pass's arrays stay valid until the next
+ * call on the driver.  Passes: left try 0, left try 1 (w doubled), right
+ * try 0 (h0 = the left score), right try 1; a pass may have no jobs. */
+typedef struct hipsw_chain_driver hipsw_chain_driver;
+
+typedef struct {
+    int32_t index;                  /* 0..3 */
+    int32_t n;
+    const uint8_t *q; const uint32_t *q_off; const uint16_t *q_len;
+    const int64_t *t_pos; const int32_t *t_len; const uint8_t *t_rev;
+    const int32_t *h0, *w, *eb;
+} hipsw_chain_pass;
+
+hipsw_chain_driver *hipsw_chain_begin(int32_t n_chains, const uint32_t *chain_seed_off,
+                                      const int64_t *seed_rbeg, const int32_t *seed_qbeg,
+                                      const int32_t *seed_len, const int32_t *seed_score,
+                                      const uint32_t *chain_read, int32_t n_reads,
+                                      const uint8_t *reads, const uint32_t *read_off,
+                                      const int32_t *read_len, const hipsw_scoring *sc,
+                                      const hipsw_ext_opts *opt);
+int  hipsw_chain_next(hipsw_chain_driver *drv, hipsw_chain_pass *pass);
+int  hipsw_chain_feed(hipsw_chain_driver *drv, const int32_t *out6);
+int  hipsw_chain_finish(hipsw_chain_driver *drv, hipsw_region *out, int32_t *out_n);
+void hipsw_chain_free(hipsw_chain_driver *drv);
 
 /* Device-resident NW job sets (benchmarks, serving); see hipsw_ext_upload. */
 typedef struct hipsw_nw_set hipsw_nw_set;
@@ -343,6 +402,13 @@ typedef struct {
     const uint8_t *nw_q; const uint32_t *nw_q_off; const uint16_t *nw_q_len;
     const int64_t *nw_t_pos; const uint16_t *nw_t_len;
     const int32_t *nw_w, *nw_true_score;
+    /* Non-NULL: the local-SW targets are mate-rescue windows in BWT space,
+     * [loc_t_pos, loc_t_pos + loc_t_len), which the server clamps as
+     * PackedReference.fetchSequence does (PackedReference.swift:72-121) to
+     * the reference sequence holding the window's start; a job whose clamp is
+     * empty or lands on a sequence other than loc_rid[k] (MateRescue.swift:
+     * 105-111) is skipped (loc_ok 0, loc_win -1). */
+    const int32_t *loc_rid;
 } hipsw_chunk;
 
 typedef struct {
@@ -365,6 +431,9 @@ typedef struct {
     int64_t est_bytes;              /* device bytes the chunk was admitted with  */
     int64_t dev_peak_bytes;         /* largest device footprint of its run; never
                                        above est_bytes                           */
+    int64_t *loc_win;               /* n_loc clamped window starts in BWT space
+                                       (fetchSequence's clampedBegin), -1 for a
+                                       skipped job; written when loc_rid was set */
 } hipsw_chunk_result;
 
 hipsw_server *hipsw_server_create(hipsw_engine *eng, int32_t n_slots, int64_t hbm_budget,
@@ -372,6 +441,10 @@ hipsw_server *hipsw_server_create(hipsw_engine *eng, int32_t n_slots, int64_t hb
                                   int32_t mismatch_penalty, int32_t nw_cap);
 /* Uploads the 2-bit .pac (4 bases per byte, first base in the high bits). */
 int     hipsw_server_set_reference(hipsw_server *srv, const uint8_t *pac, int64_t n_bases);
+/* The reference's sequences (ReferenceMetadata.annotations: offset, length),
+ * which must tile [0, n_bases); set_reference resets them to one sequence. */
+int     hipsw_server_set_contigs(hipsw_server *srv, int32_t n, const int64_t *off,
+                                 const int64_t *len);
 /* Copies the chunk and queues it; returns a ticket (> 0) or a negative status
  * (HIPSW_E_BUDGET when the chunk alone exceeds the budget). */
 int64_t hipsw_server_submit(hipsw_server *srv, const hipsw_chunk *chunk);

@@ -30,6 +30,12 @@ def _lib():
     sig = {
         "or_e2e_create": (P, [P, I64, C.c_char_p, C.POINTER(po.Scoring)]),
         "or_e2e_free": (None, [P]),
+        "or_e2e_set_contigs": (C.c_int, [P, C.c_int, P, P, C.c_char_p]),
+        "or_e2e_seq_id": (C.c_int, [P, I64]),
+        "or_e2e_fork": (P, [P]),
+        "or_e2e_pe_rescue_jobs_raw": (I64, [P, C.c_int]),
+        "or_e2e_pe_get_rescue_windows": (None, [P, P, P, P]),
+        "or_e2e_pe_set_rescue_windows": (None, [P, P]),
         "or_e2e_sa": (I64, [P, I64]),
         "or_e2e_bwt": (C.c_int, [P, I64]),
         "or_e2e_counts": (None, [P, P]),
@@ -117,10 +123,20 @@ def parse_sam(text):
 
 
 class E2E:
-    """One reference sequence and one read batch at a time."""
+    """One indexed reference and one read batch at a time.  `contigs`, a list
+    of (name, length) tiling the reference, splits it into sequences as an
+    index built from a multi-sequence FASTA does; without it the reference is
+    one sequence called `name`."""
 
-    def __init__(self, fwd, name="ref", sc=None):
+    def __init__(self, fwd, name="ref", sc=None, contigs=None, _base=None):
         self.L = _lib()
+        if _base is not None:                  # fork(): the base's index, own batch
+            self.sc, self.fwd, self.gl = _base.sc, _base.fwd, _base.gl
+            self.contigs = _base.contigs
+            self._base = _base
+            self.h = self.L.or_e2e_fork(_base.h)
+            return
+        self.contigs = [(name, len(fwd))] if not contigs else list(contigs)
         self.sc = sc or po.scoring()
         self.fwd = np.ascontiguousarray(fwd, np.uint8)
         if np.any(self.fwd > 3):
@@ -128,9 +144,25 @@ class E2E:
         self.h = self.L.or_e2e_create(self.fwd.ctypes.data, len(self.fwd), name.encode(),
                                       C.byref(self.sc))
         self.gl = len(self.fwd)
+        if contigs:
+            lens = np.array([n for _, n in contigs], np.int64)
+            offs = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int64)
+            names = b"".join(str(nm).encode() + b"\0" for nm, _ in contigs)
+            if self.L.or_e2e_set_contigs(self.h, len(contigs), offs.ctypes.data, lens.ctypes.data,
+                                         names) != 0:
+                self.close()
+                raise ValueError("contig lengths must tile the reference")
+
+    def fork(self):
+        """A handle over this index with its own batch state (for batches in
+        flight on several threads); this handle must stay open meanwhile."""
+        return E2E(None, _base=self)
+
+    def seq_id(self, pos):
+        return self.L.or_e2e_seq_id(self.h, pos)
 
     def close(self):
-        if self.h:
+        if getattr(self, "h", None):
             self.L.or_e2e_free(self.h)
             self.h = None
 
@@ -278,6 +310,31 @@ class E2E:
         return {"n": int(n), "queries": q, "q_off": qo[:n].astype(np.uint32),
                 "q_len": ql[:n].astype(np.uint16), "targets": t,
                 "t_off": to[:n].astype(np.uint32), "t_len": tl[:n].astype(np.uint16)}
+
+    def rescue_windows(self, direction):
+        """The candidates of one rescue direction before fetchSequence: mate
+        queries and raw BWT-space windows [rb, re) with the template's
+        sequence id (the caller clamps and fetches); see set_rescue_windows."""
+        n = self.L.or_e2e_pe_rescue_jobs_raw(self.h, int(direction))
+        qb_, tb_ = C.c_int64(), C.c_int64()
+        self.L.or_e2e_pe_rescue_sizes(self.h, C.byref(qb_), C.byref(tb_))
+        q = np.zeros(max(qb_.value, 1), np.uint8)
+        t = np.zeros(1, np.uint8)
+        m = max(n, 1)
+        qo, to = np.zeros(m, np.int64), np.zeros(m, np.int64)
+        ql, tl = np.zeros(m, np.int32), np.zeros(m, np.int32)
+        self.L.or_e2e_pe_get_rescue_jobs(self.h, q.ctypes.data, qo.ctypes.data, ql.ctypes.data,
+                                         t.ctypes.data, to.ctypes.data, tl.ctypes.data)
+        rb, re = np.zeros(m, np.int64), np.zeros(m, np.int64)
+        rid = np.zeros(m, np.int32)
+        self.L.or_e2e_pe_get_rescue_windows(self.h, rb.ctypes.data, re.ctypes.data, rid.ctypes.data)
+        return {"n": int(n), "queries": q, "q_off": qo[:n].astype(np.uint32),
+                "q_len": ql[:n].astype(np.uint16), "rb": rb[:n], "re": re[:n], "rid": rid[:n]}
+
+    def set_rescue_windows(self, clamped):
+        clamped = np.ascontiguousarray(clamped, np.int64)
+        if len(clamped):
+            self.L.or_e2e_pe_set_rescue_windows(self.h, clamped.ctypes.data)
 
     def rescue_cpu(self):
         self.L.or_e2e_pe_rescue_cpu(self.h)

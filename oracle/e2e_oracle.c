@@ -125,6 +125,7 @@ typedef struct {                                   /* OrientationStats (InsertSi
 typedef struct {                                   /* MateRescue.RescueCandidate */
     int pair, mate_read, is_rev, rid, mate_len;
     int64_t clamped;
+    int64_t raw_rb, raw_re;                        /* the window before fetchSequence */
     int64_t q_off, t_off;
     int32_t q_len, t_len;
     int32_t out6[6];
@@ -147,6 +148,11 @@ struct or_e2e {
     int64_t sentinel;
     int32_t *occ;                   /* (n + 2) x 4 prefix counts */
     char name[256];
+    /* reference sequences (ReferenceMetadata.annotations): offsets into the
+       forward text, lengths, names; one sequence unless or_e2e_set_contigs */
+    int n_ctg;
+    int64_t *ctg_off, *ctg_len;
+    char (*ctg_name)[256];
     /* batch */
     int n_reads;
     uint8_t *reads;
@@ -170,6 +176,8 @@ struct or_e2e {
     int64_t n_resc, resc_cap;
     uint8_t *rq, *rt;
     int64_t rq_n, rq_cap, rt_n, rt_cap;
+    int raw_resc;                   /* rescue windows left to the caller's fetchSequence */
+    int shared;                     /* index borrowed from another handle (or_e2e_fork) */
 };
 
 #define VEC_PUSH(vec, x)                                                              \
@@ -235,8 +243,55 @@ or_e2e_t *or_e2e_create(const uint8_t *fwd, int64_t gl, const char *name, const 
     for (int64_t i = 0; i < e->n; ++i) f[e->text[i]]++;
     e->cnt[0] = 1;                                             /* count[c] + 1 (BWT.swift:10-11) */
     for (int c = 0; c < 4; ++c) e->cnt[c + 1] = e->cnt[c] + f[c];
+    e->n_ctg = 1;
+    e->ctg_off = (int64_t *)calloc(1, sizeof(int64_t));
+    e->ctg_len = (int64_t *)malloc(sizeof(int64_t));
+    e->ctg_len[0] = gl;
+    e->ctg_name = (char (*)[256])calloc(1, 256);
+    snprintf(e->ctg_name[0], 256, "%s", e->name);
     return e;
 }
+
+/* Several reference sequences concatenated in the forward text, as the
+ * index's annotations describe them (offset, length, name per sequence;
+ * names NUL-separated).  Returns 0, or -1 when they do not tile [0, gl). */
+int or_e2e_set_contigs(or_e2e_t *e, int n, const int64_t *off, const int64_t *len, const char *names)
+{
+    if (n < 1) return -1;
+    int64_t at = 0;
+    for (int k = 0; k < n; ++k) {
+        if (off[k] != at || len[k] <= 0) return -1;
+        at += len[k];
+    }
+    if (at != e->gl) return -1;
+    free(e->ctg_off); free(e->ctg_len); free(e->ctg_name);
+    e->n_ctg = n;
+    e->ctg_off = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    e->ctg_len = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    e->ctg_name = (char (*)[256])calloc((size_t)n, 256);
+    for (int k = 0; k < n; ++k) {
+        e->ctg_off[k] = off[k];
+        e->ctg_len[k] = len[k];
+        snprintf(e->ctg_name[k], 256, "%s", names);
+        names += strlen(names) + 1;
+    }
+    return 0;
+}
+
+/* ReferenceMetadata.sequenceID (ReferenceMetadata.swift:77-89): the last
+ * sequence whose offset is <= pos */
+static int seq_id(const or_e2e_t *e, int64_t pos)
+{
+    int lo = 0, hi = e->n_ctg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (e->ctg_off[mid] <= pos) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+int or_e2e_seq_id(const or_e2e_t *e, int64_t pos) { return seq_id(e, pos); }
 
 static void batch_clear(or_e2e_t *e)
 {
@@ -265,8 +320,30 @@ void or_e2e_free(or_e2e_t *e)
     batch_clear(e);
     free(e->jobs); free(e->cq); free(e->cr); free(e->calls); free(e->lq); free(e->lt);
     free(e->resc); free(e->rq); free(e->rt);
-    free(e->fwd); free(e->text); free(e->bwt); free(e->sa); free(e->occ);
+    if (!e->shared) {
+        free(e->ctg_off); free(e->ctg_len); free(e->ctg_name);
+        free(e->fwd); free(e->text); free(e->bwt); free(e->sa); free(e->occ);
+    }
     free(e);
+}
+
+/* A handle with its own batch state over the index of `base`, which must
+ * outlive it and keep its sequences unchanged; batches on different handles
+ * may run in different threads. */
+or_e2e_t *or_e2e_fork(const or_e2e_t *base)
+{
+    or_e2e_t *e = (or_e2e_t *)calloc(1, sizeof *e);
+    e->sc = base->sc;
+    e->gl = base->gl; e->n = base->n;
+    e->fwd = base->fwd; e->text = base->text; e->bwt = base->bwt; e->sa = base->sa;
+    memcpy(e->cnt, base->cnt, sizeof e->cnt);
+    e->sentinel = base->sentinel;
+    e->occ = base->occ;
+    memcpy(e->name, base->name, sizeof e->name);
+    e->n_ctg = base->n_ctg;
+    e->ctg_off = base->ctg_off; e->ctg_len = base->ctg_len; e->ctg_name = base->ctg_name;
+    e->shared = 1;
+    return e;
 }
 
 int64_t or_e2e_sa(const or_e2e_t *e, int64_t i) { return e->sa[i]; }
@@ -450,8 +527,7 @@ static int seed_before(const void *a, const void *b, void *ctx)
     return ((const seed_t *)a)->rbeg < ((const seed_t *)b)->rbeg;
 }
 
-/* SeedChainer.chain (SeedChainer.swift:16-107); one reference sequence, so
- * every seed's rid is 0 and no contig is ALT */
+/* SeedChainer.chain (SeedChainer.swift:16-107); no contig is ALT */
 static void seed_chain(const or_e2e_t *e, const smem_vec *sm, chain_vec *out)
 {
     for (int i = 0; i < sm->n; ++i) {
@@ -463,7 +539,11 @@ static void seed_chain(const or_e2e_t *e, const smem_vec *sm, chain_vec *out)
         int64_t count = 0;
         for (int64_t sa_idx = 0; sa_idx < occs && count < MAX_OCC; sa_idx += step, ++count) {
             int64_t ref_pos = e->sa[s->k + sa_idx];
-            const int32_t rid = 0;
+            /* :44-52: reverse-strand positions map to forward coordinates,
+               and the sequence is the one holding min(rbeg, gl - 1) */
+            int64_t rbeg = ref_pos;
+            if (rbeg >= e->gl) rbeg = e->gl * 2 - 1 - rbeg - (int64_t)seed_len + 1;
+            const int32_t rid = seq_id(e, rbeg < e->gl - 1 ? rbeg : e->gl - 1);
             seed_t seed = {ref_pos, s->qb, seed_len, seed_len * e->sc.a};
             int merged = 0;
             for (int ci = out->n - 1; ci >= 0; --ci) {
@@ -978,7 +1058,11 @@ static int64_t add_cigar_job(or_e2e_t *e, int r, const region_t *g)
     e->cq_n += j.q_len;
     int64_t rb = g->rb, re = g->re;
     if (rev) { int64_t frb = 2 * gl - re, fre = 2 * gl - rb; rb = frb; re = fre; }
-    if (re > gl) re = gl;                                      /* chromosome end (one sequence) */
+    {                                                          /* :866-872 chromosome end */
+        const int rid = seq_id(e, rb);
+        const int64_t chr_end = e->ctg_off[rid] + e->ctg_len[rid];
+        if (re > chr_end) re = chr_end;
+    }
     int64_t ref_len = re - rb;
     int64_t safe = ref_len < gl - rb ? ref_len : gl - rb;
     j.r_off = e->cr_n;
@@ -1197,6 +1281,8 @@ typedef struct {
     int rev;
     int64_t ref_len;                   /* reference consumed */
     int dummy;                         /* cigarString "*" */
+    int rid;                           /* decodePosition(pos) */
+    int64_t lpos;
 } cinfo_t;
 
 static cinfo_t cinfo_of(const or_e2e_t *e, const region_t *g)
@@ -1204,7 +1290,7 @@ static cinfo_t cinfo_of(const or_e2e_t *e, const region_t *g)
     cinfo_t c;
     memset(&c, 0, sizeof c);
     c.md = "";
-    if (g->cig < 0) { c.dummy = 1; return c; }
+    if (g->cig < 0) { c.dummy = 1; return c; }                 /* pos 0: rid 0, local 0 */
     const cjob_t *j = &e->jobs[g->cig];
     c.cig = j->cigar; c.n = j->n_cigar; c.nm = j->nm; c.md = j->md ? j->md : "";
     c.pos = j->pos; c.rev = j->rev;
@@ -1212,6 +1298,8 @@ static cinfo_t cinfo_of(const or_e2e_t *e, const region_t *g)
         uint32_t op = c.cig[k] & 0xF;
         if (op == 0 || op == 2 || op == 3 || op == 7 || op == 8) c.ref_len += c.cig[k] >> 4;
     }
+    c.rid = seq_id(e, c.pos);                                  /* ReferenceMetadata.swift:92-96 */
+    c.lpos = c.pos - e->ctg_off[c.rid];
     return c;
 }
 
@@ -1295,7 +1383,8 @@ static void classify(const or_e2e_t *e, const region_t *R, int nreg, int is_r1, 
         for (int k = 0; k < nxa; ++k) {
             const region_t *g = xa_src[k];
             cinfo_t c = cinfo_of(e, g);
-            sb_printf(&xa, "%s,%c%lld,", e->name, c.rev ? '-' : '+', (long long)(c.pos + 1));
+            sb_printf(&xa, "%s,%c%lld,", e->ctg_name[c.rid], c.rev ? '-' : '+',
+                      (long long)(c.lpos + 1));
             cinfo_str(&xa, &c, 0);
             sb_printf(&xa, ",%d;", c.nm);
         }
@@ -1306,7 +1395,7 @@ static void classify(const or_e2e_t *e, const region_t *R, int nreg, int is_r1, 
             sbuf_t sa = {0};
             for (int x = 0; x < nonsec; ++x) {
                 if (x == k) continue;
-                sb_printf(&sa, "%s,%lld,%c,", e->name, (long long)(S[x].ci.pos + 1),
+                sb_printf(&sa, "%s,%lld,%c,", e->ctg_name[S[x].ci.rid], (long long)(S[x].ci.lpos + 1),
                           S[x].ci.rev ? '-' : '+');
                 cinfo_str(&sa, &S[x].ci, 0);
                 sb_printf(&sa, ",%d,%d;", S[x].mapq, S[x].ci.nm);
@@ -1317,7 +1406,8 @@ static void classify(const or_e2e_t *e, const region_t *R, int nreg, int is_r1, 
 }
 
 /* buildRecord / buildUnmappedRecord (SAMOutputBuilder.swift:103-296, 370-438)
- * rendered as SAM text; one reference sequence (tid 0). */
+ * rendered as SAM text: RNAME/POS from decodePosition, RNEXT "=" when the
+ * mate is on the same sequence (htslib's SAM text form). */
 static void write_record(sbuf_t *b, const or_e2e_t *e, const char *name, const uint8_t *read,
                          const uint8_t *qual, int rl, const spec_t *s, const region_t *R)
 {
@@ -1332,7 +1422,8 @@ static void write_record(sbuf_t *b, const or_e2e_t *e, const char *name, const u
     const int64_t mpos = pe && !pe->mate_unmapped ? pe->mate_pos : -1;
     if (s->reg < 0) {
         flag |= 0x4;
-        sb_printf(b, "%s\t%d\t*\t0\t0\t*\t%s\t%lld\t0\t", name, flag, mtid < 0 ? "*" : e->name,
+        sb_printf(b, "%s\t%d\t*\t0\t0\t*\t%s\t%lld\t0\t", name, flag,
+                  mtid < 0 ? "*" : e->ctg_name[mtid],
                   (long long)(mpos + 1));
         seq_qual(b, read, qual, rl, 0, 0, 0);
         sb_put(b, "\n", 1);
@@ -1351,9 +1442,11 @@ static void write_record(sbuf_t *b, const or_e2e_t *e, const char *name, const u
         if (c->n > 1 && (c->cig[c->n - 1] & 0xF) == 4) t1 = (int)(c->cig[c->n - 1] >> 4);
         if (!(t0 < rl - t1)) { t0 = 0; t1 = 0; }
     }
-    sb_printf(b, "%s\t%d\t%s\t%lld\t%d\t", name, flag, e->name, (long long)(c->pos + 1), s->mapq);
+    sb_printf(b, "%s\t%d\t%s\t%lld\t%d\t", name, flag, e->ctg_name[c->rid], (long long)(c->lpos + 1),
+              s->mapq);
     cinfo_str(b, c, s->supp);
-    sb_printf(b, "\t%s\t%lld\t%lld\t", mtid < 0 ? "*" : "=", (long long)(mpos + 1),
+    sb_printf(b, "\t%s\t%lld\t%lld\t",
+              mtid < 0 ? "*" : mtid == c->rid ? "=" : e->ctg_name[mtid], (long long)(mpos + 1),
               (long long)(pe ? pe->tlen : 0));
     seq_qual(b, read, qual, rl, rev, t0, t1);
     int32_t xs = g->sub > g->csub ? g->sub : g->csub;
@@ -1582,16 +1675,32 @@ static void collect_rescue(or_e2e_t *e, int i, int dir)
             if (rb < 0) rb = 0;
             if (re > 2 * gl) re = 2 * gl;
             if (!(rb < re)) continue;
-            int64_t cb, ce, clamped;
-            const int fwd = rb < gl;
-            if (fwd) { cb = rb; ce = re < gl ? re : gl; }
-            else {
-                int64_t fb = 2 * gl - re, fe = 2 * gl - rb;
-                cb = fb > 0 ? fb : 0; ce = fe < gl ? fe : gl;
+            if (e->raw_resc) {                                 /* the caller fetches */
+                e->resc = (rcand_t *)grow(e->resc, &e->resc_cap, e->n_resc + 1, sizeof(rcand_t));
+                rcand_t *c = &e->resc[e->n_resc++];
+                memset(c, 0, sizeof *c);
+                c->pair = i; c->mate_read = mr; c->is_rev = is_rev; c->rid = a->rid;
+                c->mate_len = mlen; c->raw_rb = rb; c->raw_re = re; c->clamped = -1;
+                e->rq = (uint8_t *)grow(e->rq, &e->rq_cap, e->rq_n + mlen + 1, 1);
+                c->q_off = e->rq_n; c->q_len = mlen;
+                for (int k = 0; k < mlen; ++k) {
+                    uint8_t x = is_rev ? mread[mlen - 1 - k] : mread[k];
+                    e->rq[e->rq_n + k] = is_rev ? (x < 4 ? (uint8_t)(3 - x) : 4) : x;
+                }
+                e->rq_n += mlen;
+                c->t_off = e->rt_n; c->t_len = 0;
+                continue;
             }
+            /* PackedReference.fetchSequence (PackedReference.swift:72-121):
+               clamp to the sequence holding the window's start, on its strand */
+            const int fwd = rb < gl;
+            const int64_t fb = fwd ? rb : 2 * gl - re, fe = fwd ? re : 2 * gl - rb;
+            const int rid = seq_id(e, fb);
+            const int64_t lo_c = e->ctg_off[rid], hi_c = lo_c + e->ctg_len[rid];
+            const int64_t cb = fb > lo_c ? fb : lo_c, ce = fe < hi_c ? fe : hi_c;
             if (!(cb < ce)) continue;
-            clamped = fwd ? cb : 2 * gl - ce;
-            /* one sequence: rid 0 == a->rid */
+            if (rid != a->rid) continue;                       /* MateRescue.swift:111, :256 */
+            const int64_t clamped = fwd ? cb : 2 * gl - ce;
             e->resc = (rcand_t *)grow(e->resc, &e->resc_cap, e->n_resc + 1, sizeof(rcand_t));
             rcand_t *c = &e->resc[e->n_resc++];
             memset(c, 0, sizeof *c);
@@ -1617,9 +1726,36 @@ static void collect_rescue(or_e2e_t *e, int i, int dir)
 int64_t or_e2e_pe_rescue_jobs(or_e2e_t *e, int dir)
 {
     e->n_resc = 0; e->rq_n = e->rt_n = 0;
+    e->raw_resc = 0;
     if (!e->do_rescue) return 0;
     for (int i = 0; i < e->n_pairs; ++i) collect_rescue(e, i, dir);
     return e->n_resc;
+}
+
+/* The same candidates with their windows before fetchSequence: BWT-space
+ * [rb, re) and the template's sequence id, for a caller that clamps and
+ * fetches them itself; the target arena stays empty. */
+int64_t or_e2e_pe_rescue_jobs_raw(or_e2e_t *e, int dir)
+{
+    e->n_resc = 0; e->rq_n = e->rt_n = 0;
+    e->raw_resc = 1;
+    if (e->do_rescue)
+        for (int i = 0; i < e->n_pairs; ++i) collect_rescue(e, i, dir);
+    e->raw_resc = 0;
+    return e->n_resc;
+}
+
+void or_e2e_pe_get_rescue_windows(const or_e2e_t *e, int64_t *rb, int64_t *re, int32_t *rid)
+{
+    for (int64_t i = 0; i < e->n_resc; ++i) {
+        rb[i] = e->resc[i].raw_rb; re[i] = e->resc[i].raw_re; rid[i] = e->resc[i].rid;
+    }
+}
+
+/* fetchSequence's clampedBegin of each raw candidate (ignored where ok = 0). */
+void or_e2e_pe_set_rescue_windows(or_e2e_t *e, const int64_t *clamped)
+{
+    for (int64_t i = 0; i < e->n_resc; ++i) e->resc[i].clamped = clamped[i];
 }
 
 void or_e2e_pe_rescue_sizes(const or_e2e_t *e, int64_t *q_bytes, int64_t *t_bytes)
@@ -1843,7 +1979,8 @@ static void pe_pair(pe_t *pe, int is_r1, int proper, const cinfo_t *mate, int64_
 {
     memset(pe, 0, sizeof *pe);
     pe->on = 1; pe->is_r1 = is_r1; pe->proper = proper;
-    pe->mate_tid = 0; pe->mate_pos = mate->pos; pe->mate_rev = mate->rev; pe->mate_unmapped = 0;
+    pe->mate_tid = mate->rid; pe->mate_pos = mate->lpos; pe->mate_rev = mate->rev;
+    pe->mate_unmapped = 0;
     pe->tlen = tlen; pe->has_mc = 1; pe->mate_cig = *mate;
 }
 
@@ -1896,7 +2033,7 @@ static void pair_records(or_e2e_t *e, int i, const char *name, const uint8_t *qu
             }
             cinfo_t c1 = cinfo_of(e, &P1[0]), c2 = cinfo_of(e, &P2[0]);
             int64_t t1, t2;
-            compute_tlen(c1.pos, c1.rev, c1.ref_len, c2.pos, c2.rev, c2.ref_len, &t1, &t2);
+            compute_tlen(c1.lpos, c1.rev, c1.ref_len, c2.lpos, c2.rev, c2.ref_len, &t1, &t2);
             pe_t pe1, pe2;
             pe_pair(&pe1, 1, d.proper, &c2, t1);
             pe_pair(&pe2, 0, d.proper, &c1, t2);
@@ -1931,11 +2068,13 @@ static void pair_records(or_e2e_t *e, int i, const char *name, const uint8_t *qu
             int64_t t1 = 0, t2 = 0;
             int ori;
             int64_t ins;
-            if (infer_orientation(e, &v1->v[0], &v2->v[0], &ori, &ins)) {
-                const ori_stats_t *st = &e->dist[ori];
-                if (!st->failed && ins >= st->proper_low && ins <= st->proper_high) proper = 1;
+            if (c1.rid == c2.rid) {                            /* :1752-1771 */
+                if (infer_orientation(e, &v1->v[0], &v2->v[0], &ori, &ins)) {
+                    const ori_stats_t *st = &e->dist[ori];
+                    if (!st->failed && ins >= st->proper_low && ins <= st->proper_high) proper = 1;
+                }
+                compute_tlen(c1.lpos, c1.rev, c1.ref_len, c2.lpos, c2.rev, c2.ref_len, &t1, &t2);
             }
-            compute_tlen(c1.pos, c1.rev, c1.ref_len, c2.pos, c2.rev, c2.ref_len, &t1, &t2);
             pe_t pe1, pe2;
             pe_pair(&pe1, 1, proper, &c2, t1);
             pe_pair(&pe2, 0, proper, &c1, t2);
@@ -1966,10 +2105,10 @@ mapped_unmapped: {
         memset(&ms, 0, sizeof ms);
         memset(&us, 0, sizeof us);
         ms.r1 = m_is_r1; ms.reg = 0; ms.ci = c; ms.mapq = mapq_of(e, g); ms.primary = 1;
-        ms.pe.on = 1; ms.pe.is_r1 = m_is_r1; ms.pe.mate_tid = 0; ms.pe.mate_pos = c.pos;
+        ms.pe.on = 1; ms.pe.is_r1 = m_is_r1; ms.pe.mate_tid = c.rid; ms.pe.mate_pos = c.lpos;
         ms.pe.mate_unmapped = 1;
         us.r1 = !m_is_r1; us.reg = -1;
-        us.pe.on = 1; us.pe.is_r1 = !m_is_r1; us.pe.mate_tid = 0; us.pe.mate_pos = c.pos;
+        us.pe.on = 1; us.pe.is_r1 = !m_is_r1; us.pe.mate_tid = c.rid; us.pe.mate_pos = c.lpos;
         us.pe.mate_rev = c.rev; us.pe.mate_unmapped = 0;
         if (m_is_r1) { VEC_PUSH(sv, ms); VEC_PUSH(sv, us); }
         else { VEC_PUSH(sv, us); VEC_PUSH(sv, ms); }

@@ -60,6 +60,13 @@ typedef struct or_e2e or_e2e_t;
 /* Builds the index of one sequence (bases 0..3, gl of them) named `name`. */
 or_e2e_t *or_e2e_create(const uint8_t *fwd, int64_t gl, const char *name, const or_scoring_t *sc);
 void or_e2e_free(or_e2e_t *e);
+/* Splits the indexed text into n sequences (the index's annotations: offset,
+ * length, NUL-separated names) that tile [0, gl); 0, or -1 if they do not. */
+int or_e2e_set_contigs(or_e2e_t *e, int n, const int64_t *off, const int64_t *len, const char *names);
+/* A handle with its own batch state sharing base's index (base outlives it). */
+or_e2e_t *or_e2e_fork(const or_e2e_t *base);
+/* ReferenceMetadata.sequenceID of a forward-text position. */
+int or_e2e_seq_id(const or_e2e_t *e, int64_t pos);
 
 /* Index inspection for the tests: SA row i (0..2gl), BWT char of row i (4 =
  * '$'), count tuple (5 values), occ(pos, c). */
@@ -139,6 +146,13 @@ void or_e2e_pe_get_rescue_jobs(const or_e2e_t *e, uint8_t *q, int64_t *q_off, in
  * results (n x {score, qb, qe, tb, te, score2} and the success flags). */
 void or_e2e_pe_rescue_cpu(or_e2e_t *e);
 void or_e2e_pe_set_rescue(or_e2e_t *e, const int32_t *out6, const int32_t *ok);
+/* Raw candidates (windows before PackedReference.fetchSequence, BWT-space
+ * [rb, re), with the template's sequence id); the caller clamps and fetches,
+ * then passes each candidate's clampedBegin and its local-SW result (ok 0 for
+ * a window the clamp dropped). */
+int64_t or_e2e_pe_rescue_jobs_raw(or_e2e_t *e, int dir);
+void or_e2e_pe_get_rescue_windows(const or_e2e_t *e, int64_t *rb, int64_t *re, int32_t *rid);
+void or_e2e_pe_set_rescue_windows(or_e2e_t *e, const int64_t *clamped);
 /* MateRescue.applyResult of the pending candidates; returns regions added. */
 int64_t or_e2e_pe_apply(or_e2e_t *e);
 /* markSecondary after rescue (only when rescue runs). */

@@ -11,6 +11,7 @@
 #include "engine_internal.h"
 
 #include <algorithm>
+#include <functional>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -90,82 +91,41 @@ std::string md_tag(const std::vector<uint32_t> &cig, const Region &g, int roff)
     return md;
 }
 
-}  // namespace
-
-extern "C" int hipsw_cigar_generate(hipsw_engine *eng, int32_t n, const uint8_t *qsegs,
-                                    const uint32_t *q_off, const uint16_t *q_len,
-                                    const uint8_t *rsegs, const uint32_t *r_off,
-                                    const uint16_t *r_len, const int32_t *qb, const int32_t *qe,
-                                    const int32_t *read_len, const uint8_t *is_reverse,
-                                    const int32_t *true_score, const int32_t *initial_w,
-                                    const int64_t *ref_pos, const hipsw_scoring *sc,
-                                    int32_t match_score, int32_t mismatch_penalty, int32_t cap,
-                                    int32_t md_cap, int32_t *out_score, int32_t *out_nm,
-                                    int64_t *out_pos, uint32_t *out_cigar, int32_t *out_n_cigar,
-                                    char *out_md, int32_t *out_flags, hipsw_stats *stats)
+// Band and route of each region (CIGARGenerator.generate, :44-80).
+void cigar_route(int32_t n, const uint16_t *q_len, const uint16_t *r_len, const int32_t *qb,
+                 const int32_t *qe, const int32_t *true_score, const int32_t *initial_w,
+                 const hipsw_scoring *sc, int32_t match_score, int32_t mismatch_penalty,
+                 int32_t *w, uint8_t *route)
 {
-    if (!eng) return HIPSW_E_INVALID;
-    if (n == 0) return HIPSW_OK;
-    if (n < 0 || cap < 1 || md_cap < 1 || !qsegs || !q_off || !q_len || !rsegs || !r_off ||
-        !r_len || !qb || !qe || !read_len || !is_reverse || !true_score || !initial_w ||
-        !ref_pos || !sc || !out_score || !out_nm || !out_pos || !out_cigar || !out_n_cigar ||
-        !out_md || !out_flags)
-        return HIPSW_E_INVALID;
-    if (!scoring_valid(sc)) return HIPSW_E_INVALID;
-    const double t0 = now_ms();
-
-    // Band and route of each region.
     const int32_t min_indel = sc->o_ins + sc->e_ins + sc->o_del + sc->e_del;   // :65-67
-    std::vector<int32_t> w(n);
-    std::vector<uint8_t> route(n);   // 0 fast path, 1 device NW, 2 empty segment
-    std::vector<uint32_t> dev;       // regions that run NW
     for (int k = 0; k < n; ++k) {
         w[k] = std::max(infer_band(qe[k] - qb[k], r_len[k], true_score[k], match_score,
                                    mismatch_penalty, sc),
                         initial_w[k]);
         const int32_t deficit = (int32_t)q_len[k] * match_score - true_score[k];
-        if (q_len[k] == r_len[k] && q_len[k] > 0 && deficit < min_indel) {
-            route[k] = 0;
-        } else if (q_len[k] == 0 || r_len[k] == 0) {
-            route[k] = 2;   // GlobalAligner returns score 0 and no ops (GlobalAligner.swift:29-31)
-        } else {
-            route[k] = 1;
-            dev.push_back((uint32_t)k);
-        }
+        if (q_len[k] == r_len[k] && q_len[k] > 0 && deficit < min_indel) route[k] = 0;
+        else if (q_len[k] == 0 || r_len[k] == 0) route[k] = 2;   // GlobalAligner.swift:29-31
+        else route[k] = 1;
     }
+}
 
-    // Device NW with the retry loop for the routed subset.  Squeezing drops at
-    // most two ops (merged leading and trailing deletions), so a raw CIGAR
-    // longer than cap + 2 ops cannot fit `cap` and is flagged.
-    const int nd = (int)dev.size();
-    int raw_cap = 1;
-    std::vector<uint32_t> sq_off(nd), sr_off(nd);
-    std::vector<uint16_t> sq_len(nd), sr_len(nd);
-    std::vector<int32_t> sw(nd), sts(nd);
-    for (int s = 0; s < nd; ++s) {
-        const uint32_t k = dev[s];
-        sq_off[s] = q_off[k]; sq_len[s] = q_len[k];
-        sr_off[s] = r_off[k]; sr_len[s] = r_len[k];
-        sw[s] = w[k]; sts[s] = true_score[k];
-        raw_cap = std::max(raw_cap, (int)q_len[k] + (int)r_len[k]);
-    }
-    raw_cap = std::min(raw_cap, cap + 2);
-    std::vector<int32_t> d_score(nd), d_ncig(nd), d_flags(nd);
-    std::vector<uint32_t> d_cig((size_t)nd * raw_cap);
-    hipsw_stats nw_stats;
-    std::memset(&nw_stats, 0, sizeof nw_stats);
-    if (nd) {
-        const int rc = hipsw_global_nw(eng, nd, qsegs, sq_off.data(), sq_len.data(), rsegs,
-                                       sr_off.data(), sr_len.data(), sw.data(), sts.data(), 3,
-                                       raw_cap, sc, HIPSW_MODE_BATCHED, d_score.data(),
-                                       d_cig.data(), d_ncig.data(), d_flags.data(), nullptr,
-                                       &nw_stats);
-        if (rc != HIPSW_OK) return rc;
-    }
-    std::vector<int32_t> slot(n, -1);
-    for (int s = 0; s < nd; ++s) slot[dev[s]] = s;
+// The NW result of a routed region: score, ops (the first min(n_ops, cap) of
+// them), the full op count and the kernel's flags.
+struct NwRow {
+    int32_t score;
+    const uint32_t *ops;
+    int32_t n_ops, cap, flags;
+};
 
-    // postProcess on the host (:117-142), regions in parallel.
+// postProcess on the host (:117-142), regions in parallel.
+void cigar_post(int32_t n, const uint8_t *qsegs, const uint32_t *q_off, const uint16_t *q_len,
+                const uint8_t *rsegs, const uint32_t *r_off, const uint16_t *r_len,
+                const int32_t *qb, const int32_t *qe, const int32_t *read_len,
+                const uint8_t *is_reverse, const int32_t *true_score, const int64_t *ref_pos,
+                const uint8_t *route, const std::function<NwRow(int64_t)> &nw_row, int32_t cap,
+                int32_t md_cap, int32_t *out_score, int32_t *out_nm, int64_t *out_pos,
+                uint32_t *out_cigar, int32_t *out_n_cigar, char *out_md, int32_t *out_flags)
+{
     parallel_for(n, host_threads(), [&](int64_t lo, int64_t hi) {
         std::vector<uint32_t> cig;
         for (int64_t k = lo; k < hi; ++k) {
@@ -179,11 +139,10 @@ extern "C" int hipsw_cigar_generate(hipsw_engine *eng, int32_t n, const uint8_t 
                 score = true_score[k];
             } else {
                 if (route[k] == 1) {
-                    const int s = slot[k];
-                    score = d_score[s];
-                    if (d_ncig[s] > raw_cap || (d_flags[s] & 1)) flags |= 1;
-                    const uint32_t *raw = d_cig.data() + (size_t)s * raw_cap;
-                    cig.assign(raw, raw + std::min(d_ncig[s], raw_cap));
+                    const NwRow r = nw_row(k);
+                    score = r.score;
+                    if (r.n_ops > r.cap || (r.flags & 1)) flags |= 1;
+                    cig.assign(r.ops, r.ops + std::max(0, std::min(r.n_ops, r.cap)));
                 } else {
                     score = 0;
                 }
@@ -221,10 +180,139 @@ extern "C" int hipsw_cigar_generate(hipsw_engine *eng, int32_t n, const uint8_t 
             out_flags[k] = flags;
         }
     });
+}
+
+}  // namespace
+
+extern "C" int hipsw_cigar_generate(hipsw_engine *eng, int32_t n, const uint8_t *qsegs,
+                                    const uint32_t *q_off, const uint16_t *q_len,
+                                    const uint8_t *rsegs, const uint32_t *r_off,
+                                    const uint16_t *r_len, const int32_t *qb, const int32_t *qe,
+                                    const int32_t *read_len, const uint8_t *is_reverse,
+                                    const int32_t *true_score, const int32_t *initial_w,
+                                    const int64_t *ref_pos, const hipsw_scoring *sc,
+                                    int32_t match_score, int32_t mismatch_penalty, int32_t cap,
+                                    int32_t md_cap, int32_t *out_score, int32_t *out_nm,
+                                    int64_t *out_pos, uint32_t *out_cigar, int32_t *out_n_cigar,
+                                    char *out_md, int32_t *out_flags, hipsw_stats *stats)
+{
+    if (!eng) return HIPSW_E_INVALID;
+    if (n == 0) return HIPSW_OK;
+    if (n < 0 || cap < 1 || md_cap < 1 || !qsegs || !q_off || !q_len || !rsegs || !r_off ||
+        !r_len || !qb || !qe || !read_len || !is_reverse || !true_score || !initial_w ||
+        !ref_pos || !sc || !out_score || !out_nm || !out_pos || !out_cigar || !out_n_cigar ||
+        !out_md || !out_flags)
+        return HIPSW_E_INVALID;
+    if (!scoring_valid(sc)) return HIPSW_E_INVALID;
+    const double t0 = now_ms();
+
+    std::vector<int32_t> w(n);
+    std::vector<uint8_t> route(n);   // 0 fast path, 1 device NW, 2 empty segment
+    cigar_route(n, q_len, r_len, qb, qe, true_score, initial_w, sc, match_score, mismatch_penalty,
+                w.data(), route.data());
+    std::vector<uint32_t> dev;       // regions that run NW
+    for (int k = 0; k < n; ++k)
+        if (route[k] == 1) dev.push_back((uint32_t)k);
+
+    // Device NW with the retry loop for the routed subset.  Squeezing drops at
+    // most two ops (merged leading and trailing deletions), so a raw CIGAR
+    // longer than cap + 2 ops cannot fit `cap` and is flagged.
+    const int nd = (int)dev.size();
+    int raw_cap = 1;
+    std::vector<uint32_t> sq_off(nd), sr_off(nd);
+    std::vector<uint16_t> sq_len(nd), sr_len(nd);
+    std::vector<int32_t> sw(nd), sts(nd);
+    for (int s = 0; s < nd; ++s) {
+        const uint32_t k = dev[s];
+        sq_off[s] = q_off[k]; sq_len[s] = q_len[k];
+        sr_off[s] = r_off[k]; sr_len[s] = r_len[k];
+        sw[s] = w[k]; sts[s] = true_score[k];
+        raw_cap = std::max(raw_cap, (int)q_len[k] + (int)r_len[k]);
+    }
+    raw_cap = std::min(raw_cap, cap + 2);
+    std::vector<int32_t> d_score(nd), d_ncig(nd), d_flags(nd);
+    std::vector<uint32_t> d_cig((size_t)nd * raw_cap);
+    hipsw_stats nw_stats;
+    std::memset(&nw_stats, 0, sizeof nw_stats);
+    if (nd) {
+        const int rc = hipsw_global_nw(eng, nd, qsegs, sq_off.data(), sq_len.data(), rsegs,
+                                       sr_off.data(), sr_len.data(), sw.data(), sts.data(), 3,
+                                       raw_cap, sc, HIPSW_MODE_BATCHED, d_score.data(),
+                                       d_cig.data(), d_ncig.data(), d_flags.data(), nullptr,
+                                       &nw_stats);
+        if (rc != HIPSW_OK) return rc;
+    }
+    std::vector<int32_t> slot(n, -1);
+    for (int s = 0; s < nd; ++s) slot[dev[s]] = s;
+    cigar_post(n, qsegs, q_off, q_len, rsegs, r_off, r_len, qb, qe, read_len, is_reverse,
+               true_score, ref_pos, route.data(),
+               [&](int64_t k) {
+                   const int s = slot[k];
+                   return NwRow{d_score[s], d_cig.data() + (size_t)s * raw_cap, d_ncig[s], raw_cap,
+                                d_flags[s]};
+               },
+               cap, md_cap, out_score, out_nm, out_pos, out_cigar, out_n_cigar, out_md, out_flags);
     if (stats) {
         *stats = nw_stats;
         stats->n_jobs = n;
         stats->total_ms = now_ms() - t0;
     }
+    return HIPSW_OK;
+}
+
+extern "C" int hipsw_cigar_route(int32_t n, const uint16_t *q_len, const uint16_t *r_len,
+                                 const int32_t *qb, const int32_t *qe, const int32_t *true_score,
+                                 const int32_t *initial_w, const hipsw_scoring *sc,
+                                 int32_t match_score, int32_t mismatch_penalty, int32_t *out_w,
+                                 uint8_t *out_route)
+{
+    if (n == 0) return HIPSW_OK;
+    if (n < 0 || !q_len || !r_len || !qb || !qe || !true_score || !initial_w || !sc || !out_w ||
+        !out_route || !scoring_valid(sc))
+        return HIPSW_E_INVALID;
+    cigar_route(n, q_len, r_len, qb, qe, true_score, initial_w, sc, match_score, mismatch_penalty,
+                out_w, out_route);
+    return HIPSW_OK;
+}
+
+extern "C" int hipsw_cigar_finish(int32_t n, const uint8_t *qsegs, const uint32_t *q_off,
+                                  const uint16_t *q_len, const uint8_t *rsegs,
+                                  const uint32_t *r_off, const uint16_t *r_len, const int32_t *qb,
+                                  const int32_t *qe, const int32_t *read_len,
+                                  const uint8_t *is_reverse, const int32_t *true_score,
+                                  const int64_t *ref_pos, const uint8_t *route,
+                                  const int32_t *nw_score, const uint32_t *nw_cigar,
+                                  const uint32_t *nw_cig_off, const int32_t *nw_n_cigar,
+                                  const int32_t *nw_flags, int32_t nw_cap, int32_t cap,
+                                  int32_t md_cap, int32_t *out_score, int32_t *out_nm,
+                                  int64_t *out_pos, uint32_t *out_cigar, int32_t *out_n_cigar,
+                                  char *out_md, int32_t *out_flags)
+{
+    if (n == 0) return HIPSW_OK;
+    if (n < 0 || cap < 1 || md_cap < 1 || nw_cap < 1 || !qsegs || !q_off || !q_len || !rsegs ||
+        !r_off || !r_len || !qb || !qe || !read_len || !is_reverse || !true_score || !ref_pos ||
+        !route || !out_score || !out_nm || !out_pos || !out_cigar || !out_n_cigar || !out_md ||
+        !out_flags)
+        return HIPSW_E_INVALID;
+    // NW rows come in region order over the route-1 regions
+    std::vector<int32_t> slot(n, -1);
+    int n_nw = 0;
+    for (int k = 0; k < n; ++k) {
+        if (route[k] > 2) return HIPSW_E_INVALID;
+        if (route[k] == 1) slot[k] = n_nw++;
+    }
+    if (n_nw && (!nw_score || !nw_cigar || !nw_cig_off || !nw_n_cigar || !nw_flags))
+        return HIPSW_E_INVALID;
+    for (int s = 0; s < n_nw; ++s)
+        if (nw_cig_off[s + 1] - nw_cig_off[s] != (uint32_t)std::min(std::max(nw_n_cigar[s], 0), nw_cap))
+            return HIPSW_E_INVALID;
+    cigar_post(n, qsegs, q_off, q_len, rsegs, r_off, r_len, qb, qe, read_len, is_reverse,
+               true_score, ref_pos, route,
+               [&](int64_t k) {
+                   const int s = slot[k];
+                   return NwRow{nw_score[s], nw_cigar + nw_cig_off[s], nw_n_cigar[s], nw_cap,
+                                nw_flags[s]};
+               },
+               cap, md_cap, out_score, out_nm, out_pos, out_cigar, out_n_cigar, out_md, out_flags);
     return HIPSW_OK;
 }

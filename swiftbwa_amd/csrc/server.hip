@@ -87,6 +87,7 @@ struct ChunkReq {
     std::vector<uint8_t> t_rev;
     std::vector<int32_t> h0, w, eb, nw_w, nw_ts;
     int32_t loc_min_sub = 0;
+    std::vector<int64_t> loc_win;                // clamped window start per local job (-1: none)
     // outputs
     std::vector<int32_t> ext_out, loc_out, loc_ok, nw_score, nw_ncig, nw_flags;
     std::vector<uint32_t> nw_cig;                // the used ops of every CIGAR, back to back
@@ -153,6 +154,9 @@ struct hipsw_server {
     int32_t match_score = 1, mismatch = 4;
     uint8_t *d_pac = nullptr;
     int64_t gl = 0;                              // genome length (one strand)
+    // reference sequences (ReferenceMetadata.annotations): offsets into the
+    // forward strand and lengths, tiling [0, gl)
+    std::vector<int64_t> ctg_off, ctg_len;
     std::vector<std::unique_ptr<Slot>> slots;
     std::mutex mu;
     std::condition_variable cv, done_cv;
@@ -187,6 +191,33 @@ int fetched_len(int64_t pos, int len, int64_t gl)
     }
     if (pos < 0) return 0;
     return (int)std::max<int64_t>(0, std::min<int64_t>(len, gl - pos));
+}
+
+// PackedReference.fetchSequence (Sources/FMIndex/PackedReference.swift:72-121):
+// the BWT-space window [rb, re) clamped to the reference sequence that holds
+// its start, on its strand.  Returns false for an empty clamp; otherwise the
+// clamped BWT start and length and the sequence id.
+bool fetch_sequence(const hipsw_server *srv, int64_t rb, int64_t re, int64_t *begin, int32_t *len,
+                    int32_t *rid)
+{
+    if (!(rb < re)) return false;
+    const int64_t gl = srv->gl;
+    const bool fwd = rb < gl;
+    const int64_t fb = fwd ? rb : 2 * gl - re, fe = fwd ? re : 2 * gl - rb;
+    // ReferenceMetadata.sequenceID (ReferenceMetadata.swift:77-89)
+    int lo = 0, hi = (int)srv->ctg_off.size() - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (srv->ctg_off[mid] <= fb) lo = mid;
+        else hi = mid - 1;
+    }
+    const int64_t cb = std::max(fb, srv->ctg_off[lo]);
+    const int64_t ce = std::min(fe, srv->ctg_off[lo] + srv->ctg_len[lo]);
+    if (!(cb < ce)) return false;
+    *begin = fwd ? cb : 2 * gl - ce;
+    *len = (int32_t)(ce - cb);
+    *rid = lo;
+    return true;
 }
 
 double now_ms_s() { return hipsw::now_ms(); }
@@ -746,6 +777,26 @@ extern "C" int hipsw_server_set_reference(hipsw_server *srv, const uint8_t *pac,
     }
     srv->d_pac = fresh;
     srv->gl = n_bases;
+    srv->ctg_off.assign(1, 0);
+    srv->ctg_len.assign(1, n_bases);
+    return HIPSW_OK;
+}
+
+extern "C" int hipsw_server_set_contigs(hipsw_server *srv, int32_t n, const int64_t *off,
+                                        const int64_t *len)
+{
+    if (!srv || n < 1 || !off || !len) return HIPSW_E_INVALID;
+    std::unique_lock<std::mutex> lk(srv->mu);
+    if (srv->running || !srv->queue.empty()) return HIPSW_E_PENDING;
+    if (!srv->d_pac) return HIPSW_E_INVALID;
+    int64_t at = 0;
+    for (int k = 0; k < n; ++k) {
+        if (off[k] != at || len[k] <= 0) return HIPSW_E_INVALID;
+        at += len[k];
+    }
+    if (at != srv->gl) return HIPSW_E_INVALID;
+    srv->ctg_off.assign(off, off + n);
+    srv->ctg_len.assign(len, len + n);
     return HIPSW_OK;
 }
 
@@ -799,6 +850,25 @@ extern "C" int64_t hipsw_server_submit(hipsw_server *srv, const hipsw_chunk *c)
             memcpy(r->qbytes.get() + r->q_off[j], sr.q + sr.qo[k], sr.ql[k]);
         }
     });
+    if (c->loc_rid) {
+        // mate-rescue windows: fetchSequence clamping, and the template's
+        // sequence must hold the window (MateRescue.swift:105-111, :252-256)
+        std::lock_guard<std::mutex> lk(srv->mu);
+        r->loc_win.assign(r->n_loc, -1);
+        for (int k = 0; k < r->n_loc; ++k) {
+            const int64_t j = b1 + k;
+            int64_t begin;
+            int32_t len, rid;
+            if (fetch_sequence(srv, r->t_pos[j], r->t_pos[j] + r->t_len_req[j], &begin, &len, &rid) &&
+                rid == c->loc_rid[k]) {
+                r->loc_win[k] = begin;
+                r->t_pos[j] = begin;
+                r->t_len_req[j] = (uint16_t)len;
+            } else {
+                r->t_len_req[j] = 0;                              // skipped: ok = 0
+            }
+        }
+    }
     if (c->n_ext && (!c->ext_h0 || !c->ext_w || !c->ext_eb)) return HIPSW_E_INVALID;
     for (int k = 0; k < c->n_ext; ++k)
         if (c->ext_w[k] < 0) return HIPSW_E_INVALID;   // band width must be >= 0
@@ -852,6 +922,8 @@ extern "C" int hipsw_server_wait(hipsw_server *srv, int64_t ticket, hipsw_chunk_
             if (res->ext_out && r->n_ext) memcpy(res->ext_out, r->ext_out.data(), (size_t)r->n_ext * 24);
             if (res->loc_out && r->n_loc) memcpy(res->loc_out, r->loc_out.data(), (size_t)r->n_loc * 24);
             if (res->loc_ok && r->n_loc) memcpy(res->loc_ok, r->loc_ok.data(), (size_t)r->n_loc * 4);
+            if (res->loc_win && r->n_loc && !r->loc_win.empty())
+                memcpy(res->loc_win, r->loc_win.data(), (size_t)r->n_loc * 8);
             if (res->nw_score && r->n_nw) memcpy(res->nw_score, r->nw_score.data(), (size_t)r->n_nw * 4);
             if (res->nw_n_cigar && r->n_nw) memcpy(res->nw_n_cigar, r->nw_ncig.data(), (size_t)r->n_nw * 4);
             if (res->nw_flags && r->n_nw) memcpy(res->nw_flags, r->nw_flags.data(), (size_t)r->n_nw * 4);

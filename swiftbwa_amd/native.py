@@ -73,7 +73,8 @@ class HipswChunk(C.Structure):
                 ("n_loc", C.c_int32), ("loc_q", _P), ("loc_q_off", _P), ("loc_q_len", _P),
                 ("loc_t_pos", _P), ("loc_t_len", _P), ("loc_min_sub", C.c_int32),
                 ("n_nw", C.c_int32), ("nw_q", _P), ("nw_q_off", _P), ("nw_q_len", _P),
-                ("nw_t_pos", _P), ("nw_t_len", _P), ("nw_w", _P), ("nw_true_score", _P)]
+                ("nw_t_pos", _P), ("nw_t_len", _P), ("nw_w", _P), ("nw_true_score", _P),
+                ("loc_rid", _P)]
 
 
 class HipswChunkResult(C.Structure):
@@ -82,7 +83,14 @@ class HipswChunkResult(C.Structure):
                 ("nw_cigar", _P), ("nw_n_cigar", _P), ("nw_flags", _P),
                 ("latency_ms", C.c_double), ("queue_ms", C.c_double), ("run_ms", C.c_double),
                 ("cells", C.c_int64), ("host_ms", C.c_double), ("nw_cigar_off", _P),
-                ("est_bytes", C.c_int64), ("dev_peak_bytes", C.c_int64)]
+                ("est_bytes", C.c_int64), ("dev_peak_bytes", C.c_int64), ("loc_win", _P)]
+
+
+class HipswChainPass(C.Structure):
+    _P = C.c_void_p
+    _fields_ = [("index", C.c_int32), ("n", C.c_int32), ("q", _P), ("q_off", _P),
+                ("q_len", _P), ("t_pos", _P), ("t_len", _P), ("t_rev", _P), ("h0", _P),
+                ("w", _P), ("eb", _P)]
 
 
 class HipswPlanEntry(C.Structure):
@@ -120,7 +128,17 @@ _SIGS = {
                                        _P, _P, _P, C.POINTER(HipswScoring), C.c_int32, C.c_int32,
                                        C.c_int32, C.c_int32, _P, _P, _P, _P, _P, _P, _P,
                                        C.POINTER(HipswStats)]),
+    "hipsw_cigar_route": (C.c_int, [C.c_int32, _P, _P, _P, _P, _P, _P, C.POINTER(HipswScoring),
+                                    C.c_int32, C.c_int32, _P, _P]),
+    "hipsw_cigar_finish": (C.c_int, [C.c_int32] + [_P] * 18 + [C.c_int32, C.c_int32, C.c_int32]
+                           + [_P] * 7),
     "hipsw_get_reference_pac": (C.c_int32, [_P, C.c_int64, C.c_int32, _P]),
+    "hipsw_chain_begin": (_P, [C.c_int32, _P, _P, _P, _P, _P, _P, C.c_int32, _P, _P, _P,
+                               C.POINTER(HipswScoring), C.POINTER(HipswExtOpts)]),
+    "hipsw_chain_next": (C.c_int, [_P, C.POINTER(HipswChainPass)]),
+    "hipsw_chain_feed": (C.c_int, [_P, _P]),
+    "hipsw_chain_finish": (C.c_int, [_P, _P, _P]),
+    "hipsw_chain_free": (None, [_P]),
     "hipsw_extend_chains": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                       _P, C.POINTER(HipswScoring), C.POINTER(HipswExtOpts), _P,
                                       _P, C.POINTER(HipswStats)]),
@@ -138,6 +156,7 @@ _SIGS = {
     "hipsw_server_create": (_P, [_P, C.c_int32, C.c_int64, C.POINTER(HipswScoring),
                                  C.c_int32, C.c_int32, C.c_int32]),
     "hipsw_server_set_reference": (C.c_int, [_P, _P, C.c_int64]),
+    "hipsw_server_set_contigs": (C.c_int, [_P, C.c_int32, _P, _P]),
     "hipsw_server_submit": (C.c_int64, [_P, C.POINTER(HipswChunk)]),
     "hipsw_server_poll": (C.c_int, [_P, C.c_int64]),
     "hipsw_server_wait": (C.c_int, [_P, C.c_int64, C.POINTER(HipswChunkResult)]),
