@@ -410,6 +410,7 @@ typedef struct {
     uint32_t *q_off; uint16_t *q_len;
     int64_t *t_pos; uint16_t *t_len; uint8_t *t_rev;
     int32_t *a, *b, *c;              /* ext: h0, w, eb; nw: w, true_score */
+    int32_t *r;                      /* read of the job: 2 * pair + mate */
 } wl_jobs_t;
 
 typedef struct {
@@ -430,6 +431,7 @@ static void jobs_alloc(wl_jobs_t *j, int64_t n, int qcap)
     j->a = (int32_t *)calloc((size_t)n + 1, sizeof(int32_t));
     j->b = (int32_t *)calloc((size_t)n + 1, sizeof(int32_t));
     j->c = (int32_t *)calloc((size_t)n + 1, sizeof(int32_t));
+    j->r = (int32_t *)calloc((size_t)n + 1, sizeof(int32_t));
 }
 
 static int64_t jobs_add(wl_jobs_t *j, const uint8_t *q, int ql, int64_t tpos, int tlen, int rev)
@@ -492,15 +494,18 @@ wl_chunk_t *wl_chunk(const uint8_t *ref, int64_t gl, uint64_t seed, int64_t pair
                 int L = o + cal_max_gap(o, 100);
                 int64_t k = jobs_add(&c->ext, qrev, o, rbeg - L, L, 1);
                 c->ext.a[k] = 30; c->ext.b[k] = 100; c->ext.c[k] = 5;
+                c->ext.r[k] = (int32_t)(2 * p + mt);
             }
             int rem = rl - o - 30;
             if (rem > 0) {
                 int R = rem + cal_max_gap(rem, 100);
                 int64_t k = jobs_add(&c->ext, rd + o + 30, rem, rbeg + 30, R, 0);
                 c->ext.a[k] = 30 + o; c->ext.b[k] = 100; c->ext.c[k] = 5;
+                c->ext.r[k] = (int32_t)(2 * p + mt);
             }
             int64_t k = jobs_add(&c->nw, rd, rl, locus[mt], 150, 0);
             c->nw.a[k] = 5;
+            c->nw.r[k] = (int32_t)(2 * p + mt);
             c->nw.b[k] = rl - 30;
         }
         /* rescue mate 2 (forward orientation) in a 600-bp window after mate 1 */
@@ -508,7 +513,7 @@ wl_chunk_t *wl_chunk(const uint8_t *ref, int64_t gl, uint64_t seed, int64_t pair
         int64_t wstart = p1 + 400 - 150 - 225;
         if (wstart < 0) wstart = 0;
         if (wstart + 600 > gl) wstart = gl - 600;
-        jobs_add(&c->loc, tmp, l2m, wstart, 600, 0);
+        c->loc.r[jobs_add(&c->loc, tmp, l2m, wstart, 600, 0)] = (int32_t)(2 * p + 1);
     }
     return c;
 }
@@ -526,6 +531,7 @@ void *wl_chunk_field(wl_chunk_t *c, int kind, int which)
     case 6: return j->a;
     case 7: return j->b;
     case 8: return j->c;
+    case 9: return j->r;
     default: return NULL;
     }
 }
@@ -539,7 +545,7 @@ int64_t wl_chunk_count(wl_chunk_t *c, int kind, int which)
 static void jobs_free(wl_jobs_t *j)
 {
     free(j->q); free(j->q_off); free(j->q_len); free(j->t_pos); free(j->t_len); free(j->t_rev);
-    free(j->a); free(j->b); free(j->c);
+    free(j->a); free(j->b); free(j->c); free(j->r);
 }
 
 void wl_chunk_free(wl_chunk_t *c)

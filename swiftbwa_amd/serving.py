@@ -77,7 +77,7 @@ def pack_pac(ref):
 
 _JOB_FIELDS = [("q", np.uint8), ("q_off", np.uint32), ("q_len", np.uint16), ("t_pos", np.int64),
                ("t_len", np.uint16), ("t_rev", np.uint8), ("a", np.int32), ("b", np.int32),
-               ("c", np.int32)]
+               ("c", np.int32), ("r", np.int32)]
 
 
 def make_chunk(ref, seed, pairs):
@@ -214,9 +214,10 @@ class Server:
             self.h = None
 
 
-def run_poisson(server, chunks, rate_per_s, seed=0):
+def run_poisson(server, chunks, rate_per_s, seed=0, on_result=None):
     """Submits `chunks` at Poisson arrival times (rate chunks/s) from one thread
-    and collects them from another; returns per-chunk latency records."""
+    and collects them from another; returns per-chunk latency records.
+    on_result(k, results) sees each chunk's results as they are collected."""
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate_per_s, len(chunks)) if rate_per_s > 0 else np.zeros(len(chunks))
     tickets = [None] * len(chunks)
@@ -239,6 +240,8 @@ def run_poisson(server, chunks, rate_per_s, seed=0):
     for k in range(len(chunks)):
         ready.acquire()
         r = server.wait(tickets[k], compact=True)
+        if on_result is not None:
+            on_result(k, r)
         out.append({"pairs": chunks[k]["pairs"], "latency_ms": r["latency_ms"],
                     "queue_ms": r["queue_ms"], "run_ms": r["run_ms"], "cells": r["cells"],
                     "host_ms": r["host_ms"]})

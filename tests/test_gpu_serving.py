@@ -181,3 +181,38 @@ def test_budget_estimate_bounds_peak(engine, ref):
         lib.hipsw_ext_set_layout(engine.handle, 2, 0, 0)
         lib.hipsw_ext_set_layout16(engine.handle, 2, 0)
         srv.close()
+
+
+def test_rescue_windows_clamped_to_sequences(engine, ref):
+    """Raw mate-rescue windows (loc_rid set) on a reference split into
+    sequences: the server's fetchSequence clamp and sequence check, then local
+    SW, equal the restatement in tests/serving_pipeline.py followed by the
+    oracle -- clamped starts, skipped windows and every result row."""
+    import serving_pipeline as SP
+    lens = serving.contig_lengths(len(ref), 9)
+    offs = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int64)
+    ch = serving.make_chunk(ref, 77, 3000)
+    loc = ch["loc"]
+    rng = np.random.default_rng(7)
+    # push a third of the windows onto sequence ends, both strands
+    k = rng.choice(loc["n"], loc["n"] // 3, replace=False)
+    ends = np.concatenate([offs[1:], [len(ref)]])
+    loc["t_pos"][k] = ends[rng.integers(0, len(ends), len(k))] - rng.integers(0, 600, len(k))
+    flip = k[::2]
+    loc["t_pos"][flip] = 2 * len(ref) - (loc["t_pos"][flip] + loc["t_len"][flip])
+    fwd_start = np.where(loc["t_pos"] < len(ref), loc["t_pos"], 2 * len(ref) - loc["t_pos"] - loc["t_len"])
+    loc["rid"] = (np.searchsorted(offs, fwd_start + 300, side="right") - 1).astype(np.int32)
+    only = {"ext": None, "loc": loc, "nw": None}
+    srv = serving.Server(engine, slots=2, hbm_budget=2 << 30)
+    try:
+        srv.set_reference(serving.pack_pac(ref), len(ref))
+        srv.set_contigs(lens)
+        got = srv.wait(srv.submit(only))
+    finally:
+        srv.close()
+    exp = SP.CpuServer(ref, lens)
+    want = exp.wait(exp.submit(only))
+    assert np.array_equal(got["loc_win"], want["loc_win"])
+    assert (want["loc_win"] == -1).sum() > 10 and (want["loc_win"] != loc["t_pos"]).sum() > 100
+    assert np.array_equal(got["loc_ok"], want["loc_ok"])
+    assert np.array_equal(got["loc"], want["loc"])
