@@ -29,6 +29,9 @@
 #define _GNU_SOURCE
 #include "e2e_oracle.h"
 
+#include <pthread.h>
+#include <unistd.h>
+
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -212,6 +215,117 @@ static int suffix_cmp(const void *a, const void *b, void *ctx)
     return (int)t[x] - (int)t[y];
 }
 
+/* Suffix order of text[0, n) (then '$'): rows 1..n of sa.  Suffixes go to
+ * 65,536 buckets by their first 8 bases (one counting pass and one scatter,
+ * each over text slices in parallel); each bucket is sorted by its suffixes'
+ * first 32 bases packed in 64 bits, and runs of equal keys by the full
+ * comparison.  A suffix shorter than 32 bases is padded with A (code 0): its
+ * key is then <= that of every suffix it prefixes, and equal keys are
+ * resolved by suffix_cmp, which sorts '$' first -- the order is qsort's. */
+typedef struct { uint64_t key; int64_t pos; } skey_t;
+
+static int skey_cmp(const void *a, const void *b)
+{
+    const skey_t *x = (const skey_t *)a, *y = (const skey_t *)b;
+    return x->key < y->key ? -1 : x->key > y->key;
+}
+
+typedef struct {
+    const or_e2e_t *e;
+    int64_t *sa;                     /* rows 1..n */
+    int64_t lo, hi;                  /* text slice (count, scatter) */
+    int64_t *cnt;                    /* 65,536 per slice */
+    int64_t *bstart;                 /* 65,537 bucket starts (rows) */
+    int64_t *next_bucket;            /* shared work counter (sort) */
+    pthread_mutex_t *mu;
+    int phase;
+} sa_work_t;
+
+static uint64_t key_at(const uint8_t *t, int64_t n, int64_t i)
+{
+    uint64_t k = 0;
+    for (int j = 0; j < 32; ++j) k = (k << 2) | (i + j < n ? t[i + j] : 0);
+    return k;
+}
+
+static void *sa_worker(void *arg)
+{
+    sa_work_t *w = (sa_work_t *)arg;
+    const or_e2e_t *e = w->e;
+    const uint8_t *t = e->text;
+    const int64_t n = e->n;
+    if (w->phase == 0 || w->phase == 1) {
+        if (w->lo >= w->hi) return NULL;
+        uint64_t k = key_at(t, n, w->lo);
+        for (int64_t i = w->lo; i < w->hi; ++i) {
+            if (i > w->lo) k = (k << 2) | (i + 31 < n ? t[i + 31] : 0);
+            const uint32_t b = (uint32_t)(k >> 48);
+            if (w->phase == 0) w->cnt[b]++;
+            else w->sa[w->cnt[b]++] = i;
+        }
+        return NULL;
+    }
+    skey_t *buf = NULL;
+    int64_t cap = 0;
+    for (;;) {
+        pthread_mutex_lock(w->mu);
+        const int64_t b = (*w->next_bucket)++;
+        pthread_mutex_unlock(w->mu);
+        if (b >= 65536) break;
+        const int64_t r0 = w->bstart[b], r1 = w->bstart[b + 1], m = r1 - r0;
+        if (m < 2) continue;
+        if (m > cap) { cap = m; buf = (skey_t *)realloc(buf, sizeof(skey_t) * (size_t)cap); }
+        for (int64_t k = 0; k < m; ++k) buf[k] = (skey_t){key_at(t, n, w->sa[r0 + k]), w->sa[r0 + k]};
+        qsort(buf, (size_t)m, sizeof(skey_t), skey_cmp);
+        for (int64_t k = 0; k < m; ++k) w->sa[r0 + k] = buf[k].pos;
+        for (int64_t a = 0; a < m;) {                          /* runs of equal keys */
+            int64_t z = a + 1;
+            while (z < m && buf[z].key == buf[a].key) ++z;
+            if (z - a > 1) qsort_r(w->sa + r0 + a, (size_t)(z - a), sizeof(int64_t), suffix_cmp, (void *)e);
+            a = z;
+        }
+    }
+    free(buf);
+    return NULL;
+}
+
+static void suffix_sort(or_e2e_t *e)
+{
+    const int64_t n = e->n;
+    int nt = (int)sysconf(_SC_NPROCESSORS_ONLN);
+    nt = nt < 1 ? 1 : nt > 16 ? 16 : nt;
+    if (n < (1 << 16)) nt = 1;
+    int64_t *sa = e->sa + 1;
+    sa_work_t w[16];
+    pthread_t th[16];
+    int64_t *cnt = (int64_t *)calloc((size_t)nt * 65536, sizeof(int64_t));
+    int64_t *bstart = (int64_t *)calloc(65537, sizeof(int64_t));
+    int64_t next = 0;
+    pthread_mutex_t mu = PTHREAD_MUTEX_INITIALIZER;
+    for (int ph = 0; ph < 3; ++ph) {
+        for (int k = 0; k < nt; ++k) {
+            w[k] = (sa_work_t){e, sa, n * k / nt, n * (k + 1) / nt, cnt + (size_t)k * 65536, bstart,
+                               &next, &mu, ph};
+            pthread_create(&th[k], NULL, sa_worker, &w[k]);
+        }
+        for (int k = 0; k < nt; ++k) pthread_join(th[k], NULL);
+        if (ph == 0) {                 /* slice counts -> each slice's write cursors */
+            int64_t at = 0;
+            for (int b = 0; b < 65536; ++b) {
+                bstart[b] = at;
+                for (int k = 0; k < nt; ++k) {
+                    const int64_t c = cnt[(size_t)k * 65536 + b];
+                    cnt[(size_t)k * 65536 + b] = at;
+                    at += c;
+                }
+            }
+            bstart[65536] = at;
+        }
+    }
+    free(cnt);
+    free(bstart);
+}
+
 or_e2e_t *or_e2e_create(const uint8_t *fwd, int64_t gl, const char *name, const or_scoring_t *sc)
 {
     if (gl <= 0) return NULL;
@@ -227,18 +341,22 @@ or_e2e_t *or_e2e_create(const uint8_t *fwd, int64_t gl, const char *name, const 
     for (int64_t i = 0; i < gl; ++i) e->text[gl + i] = (uint8_t)(3 - (fwd[gl - 1 - i] & 3));
     e->sa = (int64_t *)malloc(sizeof(int64_t) * (size_t)(e->n + 1));
     e->sa[0] = e->n;                                           /* the '$' suffix */
-    for (int64_t i = 0; i < e->n; ++i) e->sa[i + 1] = i;
-    qsort_r(e->sa + 1, (size_t)e->n, sizeof(int64_t), suffix_cmp, e);
+    suffix_sort(e);
     e->bwt = (uint8_t *)malloc((size_t)(e->n + 1));
     e->sentinel = -1;
     for (int64_t i = 0; i <= e->n; ++i) {
         if (e->sa[i] > 0) e->bwt[i] = e->text[e->sa[i] - 1];
         else { e->bwt[i] = 4; e->sentinel = i; }
     }
-    e->occ = (int32_t *)calloc((size_t)(e->n + 2) * 4, sizeof(int32_t));
-    for (int64_t i = 0; i <= e->n; ++i)
-        for (int c = 0; c < 4; ++c)
-            e->occ[(i + 1) * 4 + c] = e->occ[i * 4 + c] + (e->bwt[i] == c);
+    e->occ = (int32_t *)malloc(sizeof(int32_t) * (size_t)(e->n + 2) * 4);
+    {
+        int32_t o[4] = {0, 0, 0, 0};
+        for (int64_t i = 0; i <= e->n; ++i) {
+            memcpy(e->occ + i * 4, o, sizeof o);
+            if (e->bwt[i] < 4) o[e->bwt[i]]++;
+        }
+        memcpy(e->occ + (e->n + 1) * 4, o, sizeof o);
+    }
     int64_t f[4] = {0, 0, 0, 0};
     for (int64_t i = 0; i < e->n; ++i) f[e->text[i]]++;
     e->cnt[0] = 1;                                             /* count[c] + 1 (BWT.swift:10-11) */
@@ -670,6 +788,37 @@ static void chain_filter(const or_e2e_t *e, chain_vec *cv)
     cv->n = n2;
 }
 
+/* BWAMemAligner.alignRead phases 1-3 (:130-200) for reads [lo, hi) */
+typedef struct {
+    const or_e2e_t *e;
+    chain_vec *per;
+    int lo, hi;
+} seed_work_t;
+
+static void *seed_worker(void *arg)
+{
+    seed_work_t *w = (seed_work_t *)arg;
+    const or_e2e_t *e = w->e;
+    for (int r = w->lo; r < w->hi; ++r) {
+        const uint8_t *q = e->reads + e->read_off[r];
+        const int qlen = e->read_len[r];
+        smem_vec sm = {0};
+        if (qlen > 0) find_all_smems(e, q, qlen, e->sc.min_seed_len, 1, &sm);
+        if (sm.n == 0) { free(sm.v); continue; }               /* :140 */
+        internal_reseed(e, q, qlen, &sm);                      /* :143 */
+        /* -y reseeding (:149-171) is off at the default reseedLength 0 */
+        stable_sort(sm.v, (size_t)sm.n, sizeof(smem_t), smem_before, NULL);   /* :174-177 */
+        float fr = frac_rep(&sm, qlen);                        /* :181 */
+        chain_vec cv = {0};
+        seed_chain(e, &sm, &cv);                               /* :184-192 */
+        for (int i = 0; i < cv.n; ++i) cv.v[i].frac_rep = fr;  /* :195-197 */
+        chain_filter(e, &cv);                                  /* :200 */
+        w->per[r] = cv;
+        free(sm.v);
+    }
+    return NULL;
+}
+
 int64_t or_e2e_seed(or_e2e_t *e, int n_reads, const uint8_t *bases, const int64_t *off,
                     const int32_t *len)
 {
@@ -688,30 +837,31 @@ int64_t or_e2e_seed(or_e2e_t *e, int n_reads, const uint8_t *bases, const int64_
         o += len[i];
     }
     e->regs = (region_vec *)calloc((size_t)(n_reads > 0 ? n_reads : 1), sizeof(region_vec));
+    /* phases 1-3 of each read (independent), on threads over read slices */
+    chain_vec *per = (chain_vec *)calloc((size_t)(n_reads > 0 ? n_reads : 1), sizeof(chain_vec));
+    int nt = (int)sysconf(_SC_NPROCESSORS_ONLN);
+    nt = nt < 1 ? 1 : nt > 16 ? 16 : nt;
+    if (n_reads < 256) nt = 1;
+    seed_work_t w[16];
+    pthread_t th[16];
+    for (int k = 0; k < nt; ++k) {
+        w[k] = (seed_work_t){e, per, (int)((int64_t)n_reads * k / nt), (int)((int64_t)n_reads * (k + 1) / nt)};
+        if (nt > 1) pthread_create(&th[k], NULL, seed_worker, &w[k]);
+        else seed_worker(&w[k]);
+    }
+    if (nt > 1)
+        for (int k = 0; k < nt; ++k) pthread_join(th[k], NULL);
     int32_t *cr = NULL;
     int64_t cr_cap = 0;
     for (int r = 0; r < n_reads; ++r) {
-        const uint8_t *q = e->reads + e->read_off[r];
-        const int qlen = e->read_len[r];
-        smem_vec sm = {0};
-        if (qlen > 0) find_all_smems(e, q, qlen, e->sc.min_seed_len, 1, &sm);
-        if (sm.n == 0) { free(sm.v); continue; }               /* :140 */
-        internal_reseed(e, q, qlen, &sm);                      /* :143 */
-        /* -y reseeding (:149-171) is off at the default reseedLength 0 */
-        stable_sort(sm.v, (size_t)sm.n, sizeof(smem_t), smem_before, NULL);   /* :174-177 */
-        float fr = frac_rep(&sm, qlen);                        /* :181 */
-        chain_vec cv = {0};
-        seed_chain(e, &sm, &cv);                               /* :184-192 */
-        for (int i = 0; i < cv.n; ++i) cv.v[i].frac_rep = fr;  /* :195-197 */
-        chain_filter(e, &cv);                                  /* :200 */
-        for (int i = 0; i < cv.n; ++i) {
-            VEC_PUSH(e->chains, cv.v[i]);
+        for (int i = 0; i < per[r].n; ++i) {
+            VEC_PUSH(e->chains, per[r].v[i]);
             cr = (int32_t *)grow(cr, &cr_cap, e->chains.n, sizeof(int32_t));
             cr[e->chains.n - 1] = r;
         }
-        free(cv.v);
-        free(sm.v);
+        free(per[r].v);
     }
+    free(per);
     e->chain_read = cr;
     return e->chains.n;
 }
