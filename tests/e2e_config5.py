@@ -13,7 +13,8 @@ reference's paired-end path; its chain-extension passes, rescue local SW
 (windows clamped on the server) and CIGAR NW run on the GPU through one
 4-slot server, all chunks in flight together (saturation) and then as a
 Poisson stream at half the saturation rate.  The SAM of the --check smallest
-chunks is compared byte for byte with the harness's all-CPU run.
+chunks (all of them by default) is compared byte for byte with the harness's
+all-CPU run.
 """
 import argparse
 import json
@@ -38,7 +39,7 @@ def main(argv=None):
     ap.add_argument("--ref-bp", type=int, default=250_000_000)
     ap.add_argument("--sequences", type=int, default=24)
     ap.add_argument("--chunks", type=int, default=12)
-    ap.add_argument("--check", type=int, default=3)
+    ap.add_argument("--check", type=int, default=12)
     ap.add_argument("--load", type=float, default=0.5)
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
