@@ -5,7 +5,11 @@
 // table written while other threads plan, the allocation-free entry point,
 // injected failures, and the serving loop's worker threads with concurrent
 // submitters.  Every result is compared with the same call run alone first.
-// Exit 0 when everything matched, 1 on a mismatch, 2 without a device.
+// The chain driver's passes and the CIGAR route / finish steps run first, on
+// the host alone, from several threads.
+// Exit 0 when everything matched, 1 on a mismatch, 2 without a device (after
+// the host-only steps matched).
+#include <algorithm>
 #include <atomic>
 #include <cstdio>
 #include <cstring>
@@ -164,16 +168,155 @@ bool same(const ChunkOut &a, const ChunkOut &b)
            a.cig == b.cig;
 }
 
+// Host-only steps (no device): the chain driver's pass state machine and the
+// CIGAR route / finish steps, driven with synthetic extension and NW results,
+// each thread on its own objects; every thread's output must equal the
+// single-thread output.
+struct HostOut {
+    std::vector<hipsw_region> regs;
+    std::vector<int32_t> nreg, score, nm, ncig, flags;
+    std::vector<int64_t> pos;
+    std::vector<uint32_t> cig;
+    std::vector<char> md;
+    bool operator==(const HostOut &o) const
+    {
+        return nreg == o.nreg && score == o.score && nm == o.nm && ncig == o.ncig &&
+               flags == o.flags && pos == o.pos && cig == o.cig && md == o.md &&
+               std::memcmp(regs.data(), o.regs.data(), regs.size() * sizeof(hipsw_region)) == 0;
+    }
+};
+
+HostOut host_steps(uint32_t seed)
+{
+    HostOut o;
+    uint64_t x = seed * 0x9E3779B97F4A7C15ull + 1;
+    auto rnd = [&](uint32_t m) { x ^= x << 13; x ^= x >> 7; x ^= x << 17; return (uint32_t)(x % m); };
+    const int n_reads = 64, rl = 150, n_chains = 96;
+    std::vector<uint8_t> reads((size_t)n_reads * rl);
+    for (auto &b : reads) b = (uint8_t)rnd(4);
+    std::vector<uint32_t> read_off(n_reads), chain_read(n_chains), off(n_chains + 1, 0);
+    std::vector<int32_t> read_len(n_reads, rl);
+    for (int r = 0; r < n_reads; ++r) read_off[r] = (uint32_t)(r * rl);
+    std::vector<int64_t> rbeg;
+    std::vector<int32_t> qbeg, len, score;
+    for (int c = 0; c < n_chains; ++c) {
+        chain_read[c] = (uint32_t)(c % n_reads);
+        const int ns = 1 + (int)rnd(4);
+        const int64_t base = 100000 + (int64_t)rnd(1000000);
+        for (int k = 0; k < ns; ++k) {
+            const int qb = (int)rnd(120), l = 19 + (int)rnd(12);
+            rbeg.push_back(base + qb);
+            qbeg.push_back(qb);
+            len.push_back(std::min(l, rl - qb));
+            score.push_back(std::min(l, rl - qb));
+        }
+        off[c + 1] = (uint32_t)rbeg.size();
+    }
+    const hipsw_scoring sc = scoring();
+    const hipsw_ext_opts opt = {1, 4, 100, 5, 5, 19};
+    hipsw_chain_driver *d = hipsw_chain_begin(n_chains, off.data(), rbeg.data(), qbeg.data(), len.data(),
+                                              score.data(), chain_read.data(), n_reads, reads.data(),
+                                              read_off.data(), read_len.data(), &sc, &opt);
+    expect(d != nullptr, "chain begin");
+    for (int p = 0; d && p < 4; ++p) {
+        hipsw_chain_pass ps;
+        expect(hipsw_chain_next(d, &ps) == HIPSW_OK && ps.index == p, "chain next");
+        std::vector<int32_t> res((size_t)ps.n * 6);
+        for (int k = 0; k < ps.n; ++k) {            // synthetic SWResult rows
+            const int32_t ql = ps.q_len[k], tl = ps.t_len[k];
+            res[k * 6 + 0] = ps.h0[k] + (k % 5 ? (int32_t)ql / 2 : 0);
+            res[k * 6 + 1] = ql / 2;
+            res[k * 6 + 2] = std::min(tl, ql / 2 + 1);
+            res[k * 6 + 3] = std::min(tl, ql);
+            res[k * 6 + 4] = k % 3 ? ps.h0[k] + ql - 3 : 0;
+            res[k * 6 + 5] = (int32_t)(k % 90);
+        }
+        expect(hipsw_chain_feed(d, res.data()) == HIPSW_OK, "chain feed");
+    }
+    o.regs.assign(rbeg.size(), hipsw_region{});
+    o.nreg.assign(n_chains, 0);
+    expect(d && hipsw_chain_finish(d, o.regs.data(), o.nreg.data()) == HIPSW_OK, "chain finish");
+    hipsw_chain_free(d);
+    // CIGAR route and finish over synthetic regions, NW rows "qlenM" or "1I..M"
+    const int n = 80, cap = 64, md_cap = 256;
+    std::vector<uint8_t> q, r;
+    std::vector<uint32_t> qo(n), ro(n);
+    std::vector<uint16_t> ql(n), rlen(n);
+    std::vector<int32_t> qb(n), qe(n), rdl(n, rl), ts(n), iw(n, 5);
+    std::vector<uint8_t> rev(n);
+    std::vector<int64_t> rp(n);
+    for (int k = 0; k < n; ++k) {
+        qo[k] = (uint32_t)q.size();
+        ro[k] = (uint32_t)r.size();
+        ql[k] = (uint16_t)(20 + rnd(100));
+        rlen[k] = (uint16_t)(ql[k] + (k % 4 == 0 ? rnd(5) : 0));
+        for (int i = 0; i < ql[k]; ++i) q.push_back((uint8_t)rnd(4));
+        for (int i = 0; i < rlen[k]; ++i) r.push_back(i < ql[k] && rnd(10) ? q[qo[k] + i] : (uint8_t)rnd(4));
+        qb[k] = (int)rnd(10);
+        qe[k] = qb[k] + ql[k];
+        rdl[k] = qe[k] + (int)rnd(10);
+        ts[k] = ql[k] - (int)rnd(12);
+        rev[k] = (uint8_t)(k & 1);
+        rp[k] = 5000 + k * 300;
+    }
+    std::vector<int32_t> w(n);
+    std::vector<uint8_t> route(n);
+    expect(hipsw_cigar_route(n, ql.data(), rlen.data(), qb.data(), qe.data(), ts.data(), iw.data(),
+                             &sc, 1, 4, w.data(), route.data()) == HIPSW_OK, "cigar route");
+    std::vector<int32_t> ns, nn, nf;
+    std::vector<uint32_t> ops, noff(1, 0);
+    for (int k = 0; k < n; ++k) {
+        if (route[k] != 1) continue;
+        ns.push_back(ts[k]);
+        if (rlen[k] > ql[k]) {
+            ops.push_back((uint32_t)(rlen[k] - ql[k]) << 4 | 2u);
+            ops.push_back((uint32_t)ql[k] << 4);
+            nn.push_back(2);
+        } else {
+            ops.push_back((uint32_t)ql[k] << 4);
+            nn.push_back(1);
+        }
+        nf.push_back(0);
+        noff.push_back((uint32_t)ops.size());
+    }
+    if (ns.empty()) { ns.push_back(0); nn.push_back(0); nf.push_back(0); ops.push_back(0); }
+    o.score.assign(n, 0); o.nm.assign(n, 0); o.ncig.assign(n, 0); o.flags.assign(n, 0);
+    o.pos.assign(n, 0); o.cig.assign((size_t)n * cap, 0); o.md.assign((size_t)n * md_cap, 0);
+    expect(hipsw_cigar_finish(n, q.data(), qo.data(), ql.data(), r.data(), ro.data(), rlen.data(),
+                              qb.data(), qe.data(), rdl.data(), rev.data(), ts.data(), rp.data(),
+                              route.data(), ns.data(), ops.data(), noff.data(), nn.data(), nf.data(),
+                              512, cap, md_cap, o.score.data(), o.nm.data(), o.pos.data(),
+                              o.cig.data(), o.ncig.data(), o.md.data(), o.flags.data()) == HIPSW_OK,
+           "cigar finish");
+    return o;
+}
+
+void host_only_steps()
+{
+    std::puts("step: host-only chain passes and CIGAR route / finish from 8 threads");
+    std::fflush(stdout);
+    std::vector<HostOut> ref;
+    for (uint32_t s = 0; s < 4; ++s) ref.push_back(host_steps(s));
+    std::vector<std::thread> th;
+    for (int t = 0; t < 8; ++t)
+        th.emplace_back([&, t] {
+            for (int it = 0; it < 3; ++it) expect(host_steps((uint32_t)(t % 4)) == ref[t % 4], "host steps thread");
+        });
+    for (auto &x : th) x.join();
+}
+
 }  // namespace
 
 int main()
 {
+    host_only_steps();
     hipsw_engine *eng = hipsw_engine_create(0);
     if (!eng) {
         int nd = -1;
         const int e = hipGetDeviceCount(&nd);
-        std::printf("no HIP device (hipGetDeviceCount: %d devices, %s)\n", nd, hipGetErrorString(e));
-        return 2;
+        std::printf("no HIP device (hipGetDeviceCount: %d devices, %s); host-only steps: %d mismatches\n",
+                    nd, hipGetErrorString(e), g_bad.load());
+        return g_bad ? 1 : 2;
     }
     Jobs E = gen(4, 71, 6000), U = gen(2, 72, 4000), N = gen(3, 73, 3000), L = gen(5, 74, 2000);
     std::vector<int32_t> rE, rEp, rU, rN, rL;
