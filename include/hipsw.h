@@ -360,6 +360,13 @@ int  hipsw_nw_download(hipsw_engine *eng, hipsw_nw_set *set, int32_t *out_score,
                        uint32_t *out_cigar, int32_t *out_n_cigar, int32_t *out_flags,
                        int32_t *out_final_w);
 void hipsw_nw_free(hipsw_engine *eng, hipsw_nw_set *set);
+/* Row-state form of the NW fill's 16-bit tier on this engine (no reference
+ * counterpart; results are identical either way): form 0, the default, keeps
+ * a lane's row state in registers when its launch's bands are at most 128
+ * columns wide; form 1 keeps it in an LDS ring (HBM for long queries)
+ * throughout.  Applies to NW runs and serving chunks started afterwards.
+ * HIPSW_E_INVALID for another form. */
+int  hipsw_nw_set_state(hipsw_engine *eng, int32_t form);
 
 /* ---- mate-rescue local SW (replaces LocalSWDispatcher.dispatchBatch,
  *      LocalSWDispatcher.swift:31-95) ----

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <condition_variable>
 #include <functional>
 #include <memory>
@@ -69,6 +70,9 @@ struct hipsw_engine {
     std::vector<int> free_slots;
     std::mutex slot_mu;
     std::condition_variable slot_cv;
+    // NW fill of the 16-bit tier: false = row state in registers where the
+    // band fits (hipsw_nw_set_state), true = the LDS ring throughout
+    std::atomic<bool> nw_ring{false};
     // autotuner table, read by every planner as a copy
     mutable std::mutex tune_mu;
     hipsw::TuneTable tune;

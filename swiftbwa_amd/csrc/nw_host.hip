@@ -30,6 +30,7 @@ struct hipsw_nw_set {
     std::vector<int32_t> final_w;
     // first-pass plans, per mode (the retry passes are planned on the fly)
     bool plan_ready[2] = {false, false};
+    bool plan_ring[2] = {false, false};   // the engine's NW state form the plan was made for
     std::vector<uint32_t> plan_order[2];
     std::vector<hipsw::WorkItem> plan_items[2];
     std::vector<hipsw::NwLaunch> plan_launches[2];
@@ -235,9 +236,19 @@ void hipsw::nw_plan_pass(const hipsw_engine *eng, const int32_t *qlv, const int3
         L.item_count = ic;
         const size_t lds = nw_lds_bytes(tr, false, s_max, L.qw8);
         L.global_state = lds > (size_t)eng->lds_per_block;
-        const size_t lds_used = nw_lds_bytes(tr, L.global_state, s_max, L.qw8);
-        const int wpc = std::max(1, std::min(16, (int)(eng->lds_per_cu / (lds_used + 512))));
-        L.grid = std::max(1, std::min(ic, eng->n_cu * wpc));
+        L.reg = 0;
+        const int kreg = tr == NW_I16 && !eng->nw_ring ? nw_reg_cols(c_max) : 0;
+        const int reg_bpc = kreg && nw_reg_lds_bytes(L.qw8) <= (size_t)eng->lds_per_block
+                                ? nw_reg_blocks_per_cu(kreg, L.qw8) : 0;
+        if (reg_bpc > 0) {
+            L.reg = kreg;
+            L.global_state = false;
+            L.grid = std::max(1, std::min(ic, eng->n_cu * std::min(reg_bpc, 32)));
+        } else {
+            const size_t lds_used = nw_lds_bytes(tr, L.global_state, s_max, L.qw8);
+            const int wpc = std::max(1, std::min(16, (int)(eng->lds_per_cu / (lds_used + 512))));
+            L.grid = std::max(1, std::min(ic, eng->n_cu * wpc));
+        }
         if (L.wide) {
             // one wave per job; flag matrices in LDS when they fit
             int jobs = 0;
@@ -264,14 +275,23 @@ void hipsw::nw_plan_pass(const hipsw_engine *eng, const int32_t *qlv, const int3
         add_launch(tr, smax, cmax, zr, 0, (int)items.size(), qm, zm);
         return;
     }
-    // Batched: bins by (tier, S class); inside a bin lanes of one wave share
-    // the band width and have similar target lengths (ncol desc, tlen desc).
+    // Batched: bins by (tier, S class); inside a bin the lanes of one wave
+    // run the same number of 8-cell blocks per row and have similar target
+    // lengths (blocks desc, tlen desc).  With the register-state form every
+    // 16-bit job whose band it holds goes to one bin, so one persistent
+    // launch balances them all (its items longest first) instead of each
+    // band class ending on its own tail.
+#ifndef NW_MERGE_REG
+#define NW_MERGE_REG 1
+#endif
+    const bool merge_reg = NW_MERGE_REG && !eng->nw_ring;
     std::vector<uint32_t> keys(m), idx(m);
     for (int a = 0; a < m; ++a) {
-        const uint32_t bin = (uint32_t)(tier[a] * 8 + s_class(S[a]));
-        const uint32_t nc = (uint32_t)std::min(ncol[a], 4095);
+        const int sc = merge_reg && tier[a] == NW_I16 && nw_reg_cols(ncol[a]) ? 0 : s_class(S[a]);
+        const uint32_t bin = (uint32_t)(tier[a] * 8 + sc);
+        const uint32_t nb = (uint32_t)std::min((ncol[a] + 7) / 8, 4095);
         const uint32_t tl = (uint32_t)std::min(tlv[active[a]], 65535);
-        keys[a] = (bin << 28) | ((4095u - nc) << 16) | (65535u - tl);
+        keys[a] = (bin << 28) | ((4095u - nb) << 16) | (65535u - tl);
         idx[a] = (uint32_t)a;
     }
     radix_sort_by_key(keys, idx);
@@ -291,6 +311,26 @@ void hipsw::nw_plan_pass(const hipsw_engine *eng, const int32_t *qlv, const int3
         const int ib = (int)items.size();
         for (int g = s; g < e; g += 64) items.push_back(WorkItem{(uint32_t)g, (uint32_t)std::min(64, e - g)});
         add_launch((int)bin / 8, smax, cmax, zr, ib, (int)items.size() - ib, qm, zm);
+        // Items longest first (a wave's time is its longest target times its
+        // widest band in blocks), so the persistent waves end together.  The
+        // wave-per-job form reads the bin's jobs as one contiguous range of
+        // the order and keeps its items as they are.
+        if (!launches.back().wide) {
+            std::vector<std::pair<int64_t, WorkItem>> cost;
+            for (int t = ib; t < (int)items.size(); ++t) {
+                int tm = 0, nb = 0;
+                for (uint32_t u = items[t].start; u < items[t].start + items[t].count; ++u) {
+                    tm = std::max(tm, tlv[active[idx[u]]]);
+                    nb = std::max(nb, (ncol[idx[u]] + 7) / 8);
+                }
+                cost.push_back({(int64_t)tm * nb, items[t]});
+            }
+            std::stable_sort(cost.begin(), cost.end(),
+                             [](const std::pair<int64_t, WorkItem> &x, const std::pair<int64_t, WorkItem> &y) {
+                                 return x.first > y.first;
+                             });
+            for (size_t t = 0; t < cost.size(); ++t) items[ib + t] = cost[t].second;
+        }
         s = e;
     }
 }
@@ -315,7 +355,7 @@ static int nw_run_ctx(hipsw_engine *eng, ExecCtx &c, hipsw_nw_set *set, int mode
                                              c.stream), "zero"))
         return HIPSW_E_DEVICE;
     for (int pass = 0; !active.empty(); ++pass) {
-        if (pass == 0 && set->plan_ready[mode]) {
+        if (pass == 0 && set->plan_ready[mode] && set->plan_ring[mode] == eng->nw_ring.load()) {
             order = set->plan_order[mode];
             items = set->plan_items[mode];
             launches = set->plan_launches[mode];
@@ -332,6 +372,7 @@ static int nw_run_ctx(hipsw_engine *eng, ExecCtx &c, hipsw_nw_set *set, int mode
                 set->plan_items[mode] = items;
                 set->plan_launches[mode] = launches;
                 set->plan_ready[mode] = true;
+                set->plan_ring[mode] = eng->nw_ring.load();
             }
         }
         if (launches.size() > 64) {
@@ -391,7 +432,7 @@ static int nw_run_ctx(hipsw_engine *eng, ExecCtx &c, hipsw_nw_set *set, int mode
                 continue;
             }
             const int ncw = (L.ncol_max + 7) / 8;
-            ok = eng->check(launch_nw(L.tier, L.global_state, set->dev, set->params, set->d_order,
+            ok = eng->check(launch_nw(L.tier, L.global_state, L.reg, set->dev, set->params, set->d_order,
                                       set->d_items + L.item_begin, L.item_count,
                                       c.counters + k, L.S, zbase, L.zrows, ncw, L.grid,
                                       pass == 0 && mode == HIPSW_MODE_BATCHED, gstate, L.qw8,
@@ -559,4 +600,11 @@ extern "C" int hipsw_global_nw(hipsw_engine *eng, int32_t n, const uint8_t *quer
                              out_final_w);
     if (stats) stats->total_ms = now_ms() - t0;
     return rc;
+}
+
+extern "C" int hipsw_nw_set_state(hipsw_engine *eng, int32_t form)
+{
+    if (!eng || form < 0 || form > 1) return HIPSW_E_INVALID;
+    eng->nw_ring.store(form == 1);
+    return HIPSW_OK;
 }

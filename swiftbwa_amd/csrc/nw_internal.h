@@ -41,7 +41,14 @@ struct NwParams {
 // LDS bytes of one block: scoring table, the staged query (qw8 words of 8
 // 4-bit codes per lane) and, unless held in HBM, the S-slot row state.
 size_t nw_lds_bytes(int tier, bool global_state, int S, int qw8);
-hipError_t launch_nw(int tier, bool global_state, const NwDev &d, const NwParams &p,
+// Register-state form of the 16-bit tier: the instance (K = 32, 64, 96 or 128
+// columns) for bands of at most ncol_max columns, 0 when none holds them; its
+// LDS bytes; resident blocks per CU (registers and LDS), 0 if unknown.
+int nw_reg_cols(int ncol_max);
+size_t nw_reg_lds_bytes(int qw8);
+int nw_reg_blocks_per_cu(int K, int qw8);
+// reg != 0 runs the register-state form with K = reg (16-bit tier, reg >= 8 * ncw).
+hipError_t launch_nw(int tier, bool global_state, int reg, const NwDev &d, const NwParams &p,
                      const uint32_t *order, const WorkItem *items, int n_items,
                      uint32_t *counter, int S, uint32_t *zbase, int zrows, int ncw, int grid,
                      bool persistent, void *gstate, int qw8, hipStream_t s);
@@ -59,6 +66,7 @@ hipError_t launch_nw_wide(const NwDev &d, const NwParams &p, const uint32_t *job
 struct NwLaunch {
     int tier, S, ncol_max, zrows, item_begin, item_count, grid;
     bool global_state;
+    int reg;     // register-state form: K columns (0: the LDS/HBM ring)
     int qw8;     // staged query words (8 bases each) per lane
     int wide;    // 1: run the bin's jobs with the wave-per-job form
     int qmax;    // longest query of the bin

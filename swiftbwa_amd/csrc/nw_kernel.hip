@@ -305,6 +305,170 @@ __device__ void nw_wave(const NwDev &d, const NwParams &p, const uint32_t *idx, 
     });
 }
 
+// The 16-bit tier with the row state in registers, for bands of at most K
+// columns (K a multiple of 8).  R[k] holds the packed (h, e) of column beg + k
+// of the lane's current row: the row's h_diag is R[0] (R[1] when the band
+// start advances, which it does by one column per row once i > w'), cell
+// beg + k reads the previous row's column beg + k + 1 at R[k + 1] (R[k + 2]
+// when advancing) and writes R[k + 1], so the frame moves with the band by a
+// per-lane select instead of a ring.  Every row runs the wave's ceil(cmax / 8)
+// blocks (cmax the wave's widest band, KC = 8 ceil(cmax / 8) cells); cells
+// past a lane's band end copy what they read, so the columns right of the
+// band keep the values they had in the first row (the sentinel: they lie
+// beyond min(w', qlen)) and R[KC + 1], never written, shifts that sentinel in.
+// Only the query and the scoring table stay in LDS.  Flags, traceback and
+// output are those of nw_wave.
+template <int K>
+__device__ void nw_wave_reg(const NwDev &d, const NwParams &p, const uint32_t *idx, int count,
+                            uint32_t *zreg, int zrows, int ncw, const int8_t *smat, uint32_t *qlds,
+                            int qw8)
+{
+    using St = NwSt<NW_I16>;
+    constexpr int kNegInf = St::kNeg;
+    const int lane = threadIdx.x & 63;
+    const bool valid = lane < count;
+    const int job = valid ? (int)idx[lane] : 0;
+    const int qlen = valid ? d.q_len[job] : 0;
+    const int tlen = valid ? d.t_len[job] : 0;
+    const bool nonempty = valid && qlen > 0 && tlen > 0;     // :29-31
+    int w = valid ? min(d.w[job], 1 << 20) : 0;
+    const int dl = qlen > tlen ? qlen - tlen : tlen - qlen;
+    w = max(w, dl);                                            // :47
+    const int ncol = min(qlen, 2 * w + 1);                     // :48
+    const uint32_t qw0 = valid ? d.q_word[job] : 0;
+    const uint32_t tw0 = valid ? d.t_word[job] : 0;
+    const uint32_t qn0 = valid ? d.qn_word[job] : kNoN;
+    const uint32_t tn0 = valid ? d.tn_word[job] : kNoN;
+
+    // Query staged as in nw_wave: 4-bit codes, 8 per word, qlds[b*64 + lane].
+    uint32_t *ql = qlds + lane;
+    for (int b = 0; b < qw8; ++b) {
+        uint32_t x = 0;
+        if (nonempty && b * 8 < qlen) {
+            x = (d.seq[qw0 + (b >> 1)] >> ((b & 1) * 16)) & 0xFFFFu;
+            x = (x | (x << 8)) & 0x00FF00FFu;
+            x = (x | (x << 4)) & 0x0F0F0F0Fu;
+            x = (x | (x << 2)) & 0x33333333u;
+            if (qn0 != kNoN) {
+                const uint32_t nb = (d.nmask[qn0 + (b >> 2)] >> ((b & 3) * 8)) & 0xFFu;
+                for (int k = 0; k < 8; ++k)
+                    if ((nb >> k) & 1u) x = (x & ~(0xFu << (4 * k))) | (4u << (4 * k));
+            }
+        }
+        ql[b * 64] = x;
+    }
+
+    // First row (:75-84) in the frame of beg = 0: R[s] is column s.
+    uint32_t R[K + 2];
+    {
+        const int init_end = min(w, qlen);
+#pragma unroll
+        for (int s = 0; s < K + 2; ++s) {
+            const int hv = s == 0 ? 0 : (s <= init_end ? -(p.o_ins + p.e_ins * s) : kNegInf);
+            R[s] = St::pack(hv, kNegInf);
+        }
+    }
+    const int cmax = wave_max_nw(nonempty ? ncol : 0);
+    uint32_t *zl = zreg + lane;                                // word (i, cw) at zl[(i*ncw+cw)*64]
+    const int tmax = wave_max_nw(nonempty ? tlen : 0);
+    const int qlim = qw8 - 1;
+
+    uint32_t tw = 0, tn = 0;
+    uint64_t ncells = 0;
+    for (int i = 0; i < tmax; ++i) {                           // :89
+        const bool act = nonempty && i < tlen;
+        if ((i & 15) == 0 && act) tw = d.seq[tw0 + (i >> 4)];
+        int t = (tw >> ((i & 15) * 2)) & 3;
+        if (tn0 != kNoN) {
+            if ((i & 31) == 0 && act) tn = d.nmask[tn0 + (i >> 5)];
+            if ((tn >> (i & 31)) & 1) t = 4;
+        }
+        const int t5 = t * 5;
+        const int beg = max(0, i - w);                         // :90-91
+        const int end = min(qlen, i + w + 1);
+        const int width = act ? end - beg : 0;                 // cells this row
+        ncells += (uint64_t)max(width, 0);
+        const bool adv = act && i > w;                         // beg = i - w moved by one
+        int f = kNegInf, h_diag, h_left;
+        {                                                      // :96-104
+            const uint32_t v0 = adv ? R[1] : R[0];
+            h_diag = St::h(v0);
+            h_left = beg == 0 ? 0 - (p.oe_del + p.e_del * i) : kNegInf;
+            if (act) R[0] = St::pack(h_left, kNegInf);         // column beg's e is never read again
+        }
+#pragma unroll
+        for (int cb = 0; cb < K / 8; ++cb) {                   // :108-141
+            if (cb * 8 >= cmax) break;
+            const int j0 = beg + cb * 8;
+            const int wq = max(0, min(j0 >> 3, qlim));
+            const uint64_t win = (uint64_t)ql[wq * 64] | ((uint64_t)ql[min(wq + 1, qlim) * 64] << 32);
+            const uint32_t codes = (uint32_t)(win >> ((j0 & 7) * 4));
+            int sc[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) sc[k] = smat[t5 + (int)((codes >> (4 * k)) & 0xFu)];
+            uint32_t zword = 0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int kk = cb * 8 + k;
+                const uint32_t v = adv ? R[kk + 2] : R[kk + 1];
+                const bool inb = kk < width;
+                const int h_cur = h_diag + sc[k];
+                const int hd = St::h(v);
+                const int e_open = hd - p.oe_del;
+                const int e_ext = St::e(v) - p.e_del;
+                const int ev = max(e_open, e_ext);
+                const int f_open = h_left - p.oe_ins;
+                const int f_ext = f - p.e_ins;
+                f = max(f_open, f_ext);
+                const int h1 = max(ev, h_cur);
+                const int hn = max(f, h1);
+                zword = __builtin_amdgcn_alignbit(zword, (uint32_t)(h1 - f), 31);        // f > h1
+                zword = __builtin_amdgcn_alignbit(zword, (uint32_t)(h_cur - ev), 31);    // ev > h_cur
+                zword = __builtin_amdgcn_alignbit(zword, (uint32_t)(f_ext - f_open), 31);
+                zword = __builtin_amdgcn_alignbit(zword, (uint32_t)(e_ext - e_open), 31);
+                R[kk + 1] = inb ? __builtin_amdgcn_perm((uint32_t)ev, (uint32_t)hn, 0x05040100u) : v;
+                // the new state is computed here, not sunk to the row's end:
+                // sinking keeps every cell's h and e live across the row and
+                // the old and new arrays in separate registers
+                asm volatile("" : "+v"(R[kk + 1]));
+                h_diag = hd;
+                h_left = hn;
+            }
+            {
+                const int nv = min(max(width - cb * 8, 0), 8);
+                zword &= nv >= 8 ? 0xFFFFFFFFu : ~(0xFFFFFFFFu >> (4 * nv)) & (nv ? 0xFFFFFFFFu : 0u);
+            }
+            zl[((size_t)i * ncw + cb) * 64] = zword;       // i < tmax <= zrows
+        }
+    }
+    if (d.cells) {
+        unsigned long long c = ncells;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+        if (lane == 0) atomicAdd(d.cells, c);
+    }
+
+    if (!valid) return;
+    int32_t *o_score = d.score + job;
+    if (!nonempty) {
+        *o_score = 0;
+        d.n_cigar[job] = 0;
+        d.flags[job] = 0;
+        return;
+    }
+    // score = h[qlen] (:144): column qlen of the last row is R[qlen - beg_last].
+    {
+        const int sidx = qlen - max(0, tlen - 1 - w);
+        uint32_t sv = R[0];
+#pragma unroll
+        for (int s = 1; s < K + 2; ++s) sv = s == sidx ? R[s] : sv;
+        *o_score = St::h(sv);
+    }
+    nw_traceback(d, job, qlen, tlen, w, ncol, [&](int i, int c) {
+        return (zl[((size_t)i * ncw + (c >> 3)) * 64] >> ((7 - (c & 7)) * 4)) & 0xFu;
+    });
+}
+
 __device__ __forceinline__ const int8_t *nw_load_smat(const NwParams &p)
 {
     extern __shared__ __align__(16) unsigned char nw_lds[];
@@ -347,6 +511,37 @@ __global__ __launch_bounds__(64) void nw_kernel(NwDev d, NwParams p, const uint3
         nw_wave<T>(d, p, idx, (int)wi.count, slots, S, zreg, zrows, ncw, smat, qlds, qw8);
         __syncthreads();
         if (!persistent) it += gridDim.x;      // plain form: static round-robin
+    }
+}
+
+// The register-state form of nw_kernel (16-bit tier, bands of at most K
+// columns); LDS holds the scoring table and the staged query only.
+template <int K>
+__global__ __launch_bounds__(64) void nw_reg_kernel(NwDev d, NwParams p, const uint32_t *order,
+                                                    const WorkItem *items, int n_items,
+                                                    uint32_t *counter, uint32_t *zbase, int zrows,
+                                                    int ncw, int persistent, int qw8)
+{
+    extern __shared__ __align__(16) unsigned char nw_lds[];
+    const int8_t *smat = nw_load_smat(p);
+    uint32_t *qlds = reinterpret_cast<uint32_t *>(nw_lds + 32);
+    uint32_t *zreg = zbase + (size_t)blockIdx.x * zrows * ncw * 64;
+    __shared__ uint32_t idx[64];
+    __shared__ int item_sh;
+    int it = blockIdx.x;
+    for (;;) {
+        if (persistent) {
+            if (threadIdx.x == 0) item_sh = (int)atomicAdd(counter, 1u);
+            __syncthreads();
+            it = item_sh;
+        }
+        if (it >= n_items) break;
+        const WorkItem wi = items[it];
+        if (threadIdx.x < wi.count) idx[threadIdx.x] = order[wi.start + threadIdx.x];
+        __syncthreads();
+        nw_wave_reg<K>(d, p, idx, (int)wi.count, zreg, zrows, ncw, smat, qlds, qw8);
+        __syncthreads();
+        if (!persistent) it += gridDim.x;
     }
 }
 
@@ -545,12 +740,61 @@ static hipError_t nw_t(const NwDev &d, const NwParams &p, const uint32_t *order,
     return hipGetLastError();
 }
 
-hipError_t launch_nw(int tier, bool global_state, const NwDev &d, const NwParams &p,
+int nw_reg_cols(int ncol_max)
+{
+    return ncol_max <= 32 ? 32 : ncol_max <= 64 ? 64 : ncol_max <= 96 ? 96 : ncol_max <= 128 ? 128 : 0;
+}
+
+size_t nw_reg_lds_bytes(int qw8) { return 32 + (size_t)qw8 * 256; }
+
+template <int K>
+static int nw_reg_occ(int qw8)
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, nw_reg_kernel<K>, 64,
+                                                     nw_reg_lds_bytes(qw8)) != hipSuccess)
+        return 0;
+    return nb;
+}
+
+int nw_reg_blocks_per_cu(int K, int qw8)
+{
+    switch (K) {
+    case 32: return nw_reg_occ<32>(qw8);
+    case 64: return nw_reg_occ<64>(qw8);
+    case 96: return nw_reg_occ<96>(qw8);
+    case 128: return nw_reg_occ<128>(qw8);
+    default: return 0;
+    }
+}
+
+template <int K>
+static hipError_t nw_reg_t(const NwDev &d, const NwParams &p, const uint32_t *order,
+                           const WorkItem *items, int n_items, uint32_t *counter, uint32_t *zbase,
+                           int zrows, int ncw, int grid, bool persistent, int qw8, hipStream_t s)
+{
+    hipLaunchKernelGGL((nw_reg_kernel<K>), dim3(grid), dim3(64), nw_reg_lds_bytes(qw8), s, d, p,
+                       order, items, n_items, counter, zbase, zrows, ncw, persistent ? 1 : 0, qw8);
+    return hipGetLastError();
+}
+
+hipError_t launch_nw(int tier, bool global_state, int reg, const NwDev &d, const NwParams &p,
                      const uint32_t *order, const WorkItem *items, int n_items,
                      uint32_t *counter, int S, uint32_t *zbase, int zrows, int ncw, int grid,
                      bool persistent, void *gstate, int qw8, hipStream_t s)
 {
     if (n_items <= 0) return hipSuccess;
+    if (reg) {
+        // the kernel's blocks cover 8 * ncw >= every band of the launch
+        if (tier != NW_I16 || reg < 8 * ncw) return hipErrorInvalidValue;
+        switch (reg) {
+        case 32: return nw_reg_t<32>(d, p, order, items, n_items, counter, zbase, zrows, ncw, grid, persistent, qw8, s);
+        case 64: return nw_reg_t<64>(d, p, order, items, n_items, counter, zbase, zrows, ncw, grid, persistent, qw8, s);
+        case 96: return nw_reg_t<96>(d, p, order, items, n_items, counter, zbase, zrows, ncw, grid, persistent, qw8, s);
+        case 128: return nw_reg_t<128>(d, p, order, items, n_items, counter, zbase, zrows, ncw, grid, persistent, qw8, s);
+        default: return hipErrorInvalidValue;
+        }
+    }
     switch (tier * 2 + (global_state ? 1 : 0)) {
     case 0: return nw_t<NW_I16, false>(d, p, order, items, n_items, counter, S, zbase, zrows, ncw, grid, persistent, gstate, qw8, s);
     case 1: return nw_t<NW_I16, true>(d, p, order, items, n_items, counter, S, zbase, zrows, ncw, grid, persistent, gstate, qw8, s);

@@ -533,7 +533,7 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
                     ++k;
                     continue;
                 }
-                ok = launch_nw(L.tier, L.global_state, nd, np, d_order, d_items + L.item_begin,
+                ok = launch_nw(L.tier, L.global_state, L.reg, nd, np, d_order, d_items + L.item_begin,
                                L.item_count, sl.counters + k, L.S, (uint32_t *)scr, L.zrows,
                                (L.ncol_max + 7) / 8, L.grid, pass == 0, (char *)scr + ((zb + 255) & ~(size_t)255),
                                L.qw8, st) == hipSuccess;

@@ -101,6 +101,12 @@ class HipSWEngine:
         if self.lib.hipsw_fault_inject(self.handle, int(site), int(countdown)):
             raise ValueError("bad fault site")
 
+    def set_nw_state(self, form):
+        """Row-state form of the NW fill's 16-bit tier: 0 registers where the
+        band fits (default), 1 the LDS ring throughout."""
+        if self.lib.hipsw_nw_set_state(self.handle, int(form)):
+            raise ValueError("hipsw_nw_set_state: invalid form %r" % (form,))
+
     def last_error(self) -> str:
         return (self.lib.hipsw_last_error(self.handle) or b"").decode()
 

@@ -150,6 +150,7 @@ _SIGS = {
     "hipsw_nw_run": (C.c_int, [_P, _P, C.c_int32, C.POINTER(HipswStats)]),
     "hipsw_nw_download": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     "hipsw_nw_free": (None, [_P, _P]),
+    "hipsw_nw_set_state": (C.c_int, [_P, C.c_int32]),
     "hipsw_local_sw": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, C.c_int32, C.c_int32,
                                  C.c_int32, C.POINTER(HipswScoring), _P, _P,
                                  C.POINTER(HipswStats)]),

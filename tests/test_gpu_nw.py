@@ -18,14 +18,20 @@ def oracle_sc(sp):
                        e_del=sp.gapExtendPenaltyDeletion, zdrop=sp.zDrop)
 
 
-def check(engine, js, sp=None, max_retries=3, cap=512):
+def check(engine, js, sp=None, max_retries=3, cap=512, forms=(0, 1)):
+    """Every mode, and the 16-bit tier's row state in registers (form 0) and in
+    the LDS ring (form 1)."""
     sp = sp or ScoringParameters()
     rs, rc, rn, cells = orc.global_batch(js, oracle_sc(sp), max_retries=max_retries, cap=cap,
                                          nthreads=8)
-    for mode in MODES:
+    for form, mode in [(f, m) for f in forms for m in MODES]:
+        engine.set_nw_state(form)
         st = {}
-        s, c, n, fl, fw = engine.global_nw(js, sp, max_retries=max_retries, cap=cap, mode=mode,
-                                           stats=st)
+        try:
+            s, c, n, fl, fw = engine.global_nw(js, sp, max_retries=max_retries, cap=cap,
+                                               mode=mode, stats=st)
+        finally:
+            engine.set_nw_state(0)
         bad = np.nonzero(s != rs)[0]
         assert len(bad) == 0, "mode %d score mismatch on %d jobs, first %d: %d vs %d" % (
             mode, len(bad), bad[0], s[bad[0]], rs[bad[0]])
@@ -135,6 +141,57 @@ def _edges(rng):
     return {"n": n, "queries": np.concatenate(qs), "targets": np.concatenate(ts), "q_off": qo,
             "t_off": to, "q_len": np.array(ql, np.uint16), "t_len": np.array(tl, np.uint16),
             "w": np.array(w, np.int32), "true_score": np.array(tsc, np.int32)}
+
+
+def _band_edges(rng):
+    """Bands of 1..130 columns around the register-state instances' limits
+    (32, 64, 96, 128 columns, and the 8-column block edges): odd widths
+    2w+1 on 150-base queries, even and odd widths set by short queries under
+    wide bands, targets shorter and longer than the query (w' = max(w, |qlen -
+    tlen|)), N bases in a few; one bin holds several widths, so lanes of one
+    wave have different bands, band ramps and band ends."""
+    qs, ts, ql, tl, w, tsc = [], [], [], [], [], []
+    widths = (1, 2, 7, 8, 9, 15, 16, 17, 30, 31, 32, 33, 34, 62, 63, 64, 65, 66, 94, 95, 96, 97,
+              98, 126, 127, 128, 129, 130)
+    for c in widths:
+        for kind in range(6):
+            if kind < 3:                     # ncol = 2w + 1 (or qlen when shorter)
+                q_len, band = 150, max(1, (c - 1) // 2)
+                t_len = q_len + (0, -min(band, 20), min(band, 20))[kind]
+            else:                            # ncol = qlen
+                q_len, band = c, c + 3
+                t_len = max(1, q_len + (0, -5, 9)[kind - 3])
+            q = rng.integers(0, 4, q_len).astype(np.uint8)
+            t = np.resize(q, t_len).copy()
+            t[rng.random(t_len) < 0.06] = rng.integers(0, 4)
+            if rng.random() < 0.2:
+                q[rng.random(q_len) < 0.02] = 4
+                t[rng.random(t_len) < 0.02] = 4
+            qs.append(q); ts.append(t); ql.append(q_len); tl.append(t_len)
+            w.append(band); tsc.append(int(rng.integers(-20, q_len + 1)))
+    n = len(ql)
+    qo = np.zeros(n, np.uint32); to = np.zeros(n, np.uint32)
+    qo[1:] = np.cumsum(ql)[:-1]; to[1:] = np.cumsum(tl)[:-1]
+    return {"n": n, "queries": np.concatenate(qs), "targets": np.concatenate(ts), "q_off": qo,
+            "t_off": to, "q_len": np.array(ql, np.uint16), "t_len": np.array(tl, np.uint16),
+            "w": np.array(w, np.int32), "true_score": np.array(tsc, np.int32)}
+
+
+def test_register_state_band_edges(engine):
+    js = _band_edges(np.random.default_rng(26))
+    check(engine, js)
+    # the same pairs many times over: full bins, so every register-state
+    # instance runs whole waves of mixed widths in the batched form
+    rep = 40
+    big = {"n": js["n"] * rep,
+           "queries": np.tile(js["queries"], rep), "targets": np.tile(js["targets"], rep),
+           "q_off": (js["q_off"][None, :].astype(np.int64) +
+                     len(js["queries"]) * np.arange(rep)[:, None]).ravel().astype(np.uint32),
+           "t_off": (js["t_off"][None, :].astype(np.int64) +
+                     len(js["targets"]) * np.arange(rep)[:, None]).ravel().astype(np.uint32),
+           "q_len": np.tile(js["q_len"], rep), "t_len": np.tile(js["t_len"], rep),
+           "w": np.tile(js["w"], rep), "true_score": np.tile(js["true_score"], rep)}
+    check(engine, big, forms=(0,))
 
 
 def test_wave_per_job_edges(engine):
