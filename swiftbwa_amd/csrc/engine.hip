@@ -88,23 +88,30 @@ void DevBuf::release()
 bool ExecCtx::init()
 {
     return hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) == hipSuccess &&
+           hipStreamCreateWithFlags(&side, hipStreamNonBlocking) == hipSuccess &&
            hipEventCreate(&ev0) == hipSuccess && hipEventCreate(&ev1) == hipSuccess &&
+           hipEventCreateWithFlags(&evf, hipEventDisableTiming) == hipSuccess &&
+           hipEventCreateWithFlags(&evj, hipEventDisableTiming) == hipSuccess &&
            hipMalloc((void **)&counters, 64 * sizeof(uint32_t)) == hipSuccess;
 }
 
 void ExecCtx::destroy()
 {
     if (stream) hipStreamSynchronize(stream);
+    if (side) hipStreamSynchronize(side);
     arena.release();
     scratch.release();
     nw.release();
     if (counters) hipFree(counters);
     if (ev0) hipEventDestroy(ev0);
     if (ev1) hipEventDestroy(ev1);
+    if (evf) hipEventDestroy(evf);
+    if (evj) hipEventDestroy(evj);
+    if (side) hipStreamDestroy(side);
     if (stream) hipStreamDestroy(stream);
     counters = nullptr;
-    ev0 = ev1 = nullptr;
-    stream = nullptr;
+    ev0 = ev1 = evf = evj = nullptr;
+    stream = side = nullptr;
 }
 
 SlotLease::SlotLease(hipsw_engine *eng) : eng_(eng), idx_(-1)

@@ -34,6 +34,11 @@ struct DevBuf {
 struct ExecCtx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // a second stream for launches that run beside the main one (small NW
+    // bins next to the large one), forked from and joined back to `stream`
+    // with two events
+    hipStream_t side = nullptr;
+    hipEvent_t evf = nullptr, evj = nullptr;
     uint32_t *counters = nullptr;   // 64 work-queue counters
     DevBuf arena;                   // job arrays of a one-shot call
     DevBuf scratch;                 // HBM DP state (extension hybrid layouts, local SW scalar)

@@ -238,7 +238,7 @@ void hipsw::nw_plan_pass(const hipsw_engine *eng, const int32_t *qlv, const int3
         L.global_state = lds > (size_t)eng->lds_per_block;
         L.reg = 0;
         const int kreg = tr == NW_I16 && !eng->nw_ring ? nw_reg_cols(c_max) : 0;
-        const int reg_bpc = kreg && nw_reg_lds_bytes(L.qw8) <= (size_t)eng->lds_per_block
+        const int reg_bpc = kreg && nw_reg_lds_bytes(kreg, L.qw8) <= (size_t)eng->lds_per_block
                                 ? nw_reg_blocks_per_cu(kreg, L.qw8) : 0;
         if (reg_bpc > 0) {
             L.reg = kreg;
@@ -277,17 +277,37 @@ void hipsw::nw_plan_pass(const hipsw_engine *eng, const int32_t *qlv, const int3
     }
     // Batched: bins by (tier, S class); inside a bin the lanes of one wave
     // run the same number of 8-cell blocks per row and have similar target
-    // lengths (blocks desc, tlen desc).  With the register-state form every
-    // 16-bit job whose band it holds goes to one bin, so one persistent
-    // launch balances them all (its items longest first) instead of each
-    // band class ending on its own tail.
-#ifndef NW_MERGE_REG
-#define NW_MERGE_REG 1
-#endif
-    const bool merge_reg = NW_MERGE_REG && !eng->nw_ring;
+    // lengths (blocks desc, tlen desc).  With the register-state form the
+    // 16-bit jobs go by register instance (bands of <= 32, 64, 96, 128
+    // columns, 5 to 2 waves per SIMD): the classes up to the smallest one
+    // that leaves at most 1/32 of their cells above it share one bin, so one
+    // persistent launch balances them (its items longest first) instead of
+    // each class ending on its own tail, and the few wider jobs do not pull
+    // the shared launch down to a larger instance's occupancy.
+    const bool reg_form = !eng->nw_ring;
+    std::vector<int> rcls(m, -1);
+    int kstar = -1;
+    if (reg_form) {
+        int64_t cells[4] = {0, 0, 0, 0}, total = 0;
+        for (int a = 0; a < m; ++a) {
+            const int K = tier[a] == NW_I16 ? nw_reg_cols(ncol[a]) : 0;
+            if (!K) continue;
+            rcls[a] = K / 32 - 1;
+            const int64_t c = (int64_t)tlv[active[a]] * ncol[a];
+            cells[rcls[a]] += c;
+            total += c;
+        }
+        int64_t above = total;
+        for (kstar = 0; kstar < 4; ++kstar) {
+            above -= cells[kstar];
+            if (above * 32 <= total) break;
+        }
+    }
     std::vector<uint32_t> keys(m), idx(m);
     for (int a = 0; a < m; ++a) {
-        const int sc = merge_reg && tier[a] == NW_I16 && nw_reg_cols(ncol[a]) ? 0 : s_class(S[a]);
+        int sc = s_class(S[a]);
+        if (rcls[a] >= 0) sc = rcls[a] <= kstar ? 0 : rcls[a];           // 0..3
+        else if (reg_form && tier[a] == NW_I16) sc = 3 + s_class(S[a]);  // 6, 7: S > 130
         const uint32_t bin = (uint32_t)(tier[a] * 8 + sc);
         const uint32_t nb = (uint32_t)std::min((ncol[a] + 7) / 8, 4095);
         const uint32_t tl = (uint32_t)std::min(tlv[active[a]], 65535);
@@ -414,20 +434,41 @@ static int nw_run_ctx(hipsw_engine *eng, ExecCtx &c, hipsw_nw_set *set, int mode
         }
         if (ok) ok = eng->check(launch_zero_u32(c.counters, 64, c.stream), "zero");
         if (ok) ok = eng->check(hipEventRecord(c.ev0, c.stream), "event");
-        int k = 0;
+        // Small bins sent to the wave-per-job form with their flags in LDS
+        // touch no shared scratch: they run on the side stream, queued first,
+        // beside the lane-per-job launches.
+        bool forked = false;
+        auto side_ok = [&](const NwLaunch &L) {
+            return L.wide && L.zmax <= 49152 && launches.size() > 1;
+        };
+        auto run_wide = [&](const NwLaunch &L, int k, hipStream_t ls) {
+            // the bin's jobs are order[items[ib].start ..) contiguously
+            const WorkItem &first = items[L.item_begin];
+            const WorkItem &last = items[L.item_begin + L.item_count - 1];
+            const int nj = (int)(last.start + last.count - first.start);
+            return eng->check(launch_nw_wide(set->dev, set->params, set->d_order + first.start, nj,
+                                             c.counters + k, L.qmax,
+                                             reinterpret_cast<uint8_t *>(zbase), L.zmax,
+                                             L.zmax <= 49152, L.grid, ls),
+                              "launch nw wide");
+        };
+        for (size_t li = 0; ok && li < launches.size(); ++li) {
+            if (!side_ok(launches[li])) continue;
+            if (!forked) {
+                ok = eng->check(hipEventRecord(c.evf, c.stream), "event") &&
+                     eng->check(hipStreamWaitEvent(c.side, c.evf, 0), "fork");
+                forked = true;
+            }
+            if (ok) ok = run_wide(launches[li], (int)li, c.side);
+            ++launches_total;
+        }
+        int k = -1;
         for (const NwLaunch &L : launches) {
+            ++k;
             if (!ok) break;
+            if (side_ok(L)) continue;
             if (L.wide) {
-                // the bin's jobs are order[items[ib].start ..) contiguously
-                const WorkItem &first = items[L.item_begin];
-                const WorkItem &last = items[L.item_begin + L.item_count - 1];
-                const int nj = (int)(last.start + last.count - first.start);
-                ok = eng->check(launch_nw_wide(set->dev, set->params, set->d_order + first.start, nj,
-                                               c.counters + k, L.qmax,
-                                               reinterpret_cast<uint8_t *>(zbase), L.zmax,
-                                               L.zmax <= 49152, L.grid, c.stream),
-                                "launch nw wide");
-                ++k;
+                ok = run_wide(L, k, c.stream);
                 ++launches_total;
                 continue;
             }
@@ -438,13 +479,19 @@ static int nw_run_ctx(hipsw_engine *eng, ExecCtx &c, hipsw_nw_set *set, int mode
                                       pass == 0 && mode == HIPSW_MODE_BATCHED, gstate, L.qw8,
                                       c.stream),
                             "launch nw");
-            ++k;
             ++launches_total;
+        }
+        if (forked) {
+            // join even after a failed launch, so nothing is left queued on the side
+            const bool j = eng->check(hipEventRecord(c.evj, c.side), "event") &&
+                           eng->check(hipStreamWaitEvent(c.stream, c.evj, 0), "join");
+            ok = ok && j;
         }
         if (ok) ok = eng->check(hipEventRecord(c.ev1, c.stream), "event");
         if (ok) ok = eng->check(hipEventSynchronize(c.ev1), "nw kernel");
         if (!ok) {
             (void)hipStreamSynchronize(c.stream);
+            (void)hipStreamSynchronize(c.side);
             return HIPSW_E_DEVICE;
         }
         float ms = 0.f;

@@ -45,7 +45,7 @@ size_t nw_lds_bytes(int tier, bool global_state, int S, int qw8);
 // columns) for bands of at most ncol_max columns, 0 when none holds them; its
 // LDS bytes; resident blocks per CU (registers and LDS), 0 if unknown.
 int nw_reg_cols(int ncol_max);
-size_t nw_reg_lds_bytes(int qw8);
+size_t nw_reg_lds_bytes(int K, int qw8);
 int nw_reg_blocks_per_cu(int K, int qw8);
 // reg != 0 runs the register-state form with K = reg (16-bit tier, reg >= 8 * ncw).
 hipError_t launch_nw(int tier, bool global_state, int reg, const NwDev &d, const NwParams &p,

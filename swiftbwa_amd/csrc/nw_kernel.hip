@@ -49,6 +49,25 @@ __device__ __forceinline__ int wave_max_nw(int v)
     return v;
 }
 
+// Wave-uniform minima of two per-lane u32 (all 64 lanes active), the two
+// chains interleaved; fused v_min_u32_dpp steps (quad_perm and mirror steps within rows, row_bcast:15/31 into row 3; lane 63 holds the result).
+__device__ __forceinline__ void wave_min2_u32(uint32_t &a, uint32_t &b)
+{
+#define HIPSW_NW_MIN2(CTRL, RM)                                                               \
+    a = min(a, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)a, CTRL, RM, 0xF, false));      \
+    b = min(b, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)b, CTRL, RM, 0xF, false));
+    HIPSW_NW_MIN2(0xB1, 0xF)
+    HIPSW_NW_MIN2(0x4E, 0xF)
+    HIPSW_NW_MIN2(0x141, 0xF)
+    HIPSW_NW_MIN2(0x140, 0xF)
+    HIPSW_NW_MIN2(0x142, 0xA)
+    HIPSW_NW_MIN2(0x143, 0xC)
+#undef HIPSW_NW_MIN2
+    a = __builtin_amdgcn_readlane(a, 63);
+    b = __builtin_amdgcn_readlane(b, 63);
+}
+
+
 // Traceback (GlobalAligner.swift:160-232) of one job by one lane, with
 // CIGARBuilder merging (CIGARBuilder.swift:4-38); ops come out last-first.
 // They go to a ring of `cap` slots, so when the CIGAR has more than cap ops
@@ -311,17 +330,26 @@ __device__ void nw_wave(const NwDev &d, const NwParams &p, const uint32_t *idx, 
 // start advances, which it does by one column per row once i > w'), cell
 // beg + k reads the previous row's column beg + k + 1 at R[k + 1] (R[k + 2]
 // when advancing) and writes R[k + 1], so the frame moves with the band by a
-// per-lane select instead of a ring.  Every row runs the wave's ceil(cmax / 8)
-// blocks (cmax the wave's widest band, KC = 8 ceil(cmax / 8) cells); cells
-// past a lane's band end copy what they read, so the columns right of the
-// band keep the values they had in the first row (the sentinel: they lie
-// beyond min(w', qlen)) and R[KC + 1], never written, shifts that sentinel in.
-// Only the query and the scoring table stay in LDS.  Flags, traceback and
-// output are those of nw_wave.
+// per-lane select instead of a ring.  A row runs the 8-cell blocks that hold
+// every active lane's band and the cell after it (at most ceil(cmax / 8),
+// cmax the wave's widest band); cells past a lane's band end copy what they
+// read, so the columns right of the band keep the values they had in the
+// first row (the sentinel: they lie beyond min(w', qlen)), and entries past
+// the blocks a row runs hold that sentinel before and after the row.
+//
+// Two forms of a block: on rows where every active lane advances, a block
+// that lies inside every active lane's band runs without the shift select and
+// without masks (a lane past its last row may then overwrite its state, so
+// each lane's score h[qlen] is taken at its last row); other blocks select
+// and mask per lane.  The score of a cell comes from the target base's row of
+// the matrix held in two registers (codes 0-3 in the low word, 4 in the
+// high), looked up four cells at a time with a byte permute of the staged
+// query codes.  Only the query and the scoring tables stay in LDS.  Flags,
+// traceback and output are those of nw_wave.
 template <int K>
 __device__ void nw_wave_reg(const NwDev &d, const NwParams &p, const uint32_t *idx, int count,
-                            uint32_t *zreg, int zrows, int ncw, const int8_t *smat, uint32_t *qlds,
-                            int qw8)
+                            uint32_t *zreg, int zrows, int ncw, const uint32_t *rowtab,
+                            uint32_t *qlds, int qw8)
 {
     using St = NwSt<NW_I16>;
     constexpr int kNegInf = St::kNeg;
@@ -340,8 +368,11 @@ __device__ void nw_wave_reg(const NwDev &d, const NwParams &p, const uint32_t *i
     const uint32_t qn0 = valid ? d.qn_word[job] : kNoN;
     const uint32_t tn0 = valid ? d.tn_word[job] : kNoN;
 
-    // Query staged as in nw_wave: 4-bit codes, 8 per word, qlds[b*64 + lane].
+    // Query staged as in nw_wave: 4-bit codes, 8 per word, qlds[b*64 + lane],
+    // then K/8 zero words: a row reads words beg/8 .. beg/8 + ceil(cmax/8)
+    // with beg <= qlen - 1, so at most word qw8 - 1 + K/8.
     uint32_t *ql = qlds + lane;
+    for (int b = qw8; b < qw8 + K / 8; ++b) ql[b * 64] = 0;
     for (int b = 0; b < qw8; ++b) {
         uint32_t x = 0;
         if (nonempty && b * 8 < qlen) {
@@ -369,9 +400,12 @@ __device__ void nw_wave_reg(const NwDev &d, const NwParams &p, const uint32_t *i
         }
     }
     const int cmax = wave_max_nw(nonempty ? ncol : 0);
+    const int wmax = wave_max_nw(nonempty ? w : 0);
     uint32_t *zl = zreg + lane;                                // word (i, cw) at zl[(i*ncw+cw)*64]
     const int tmax = wave_max_nw(nonempty ? tlen : 0);
-    const int qlim = qw8 - 1;
+    // score = h[qlen] (:144): column qlen of the last row is R[qlen - beg_last]
+    const int sidx = qlen - max(0, tlen - 1 - w);
+    int score = 0;
 
     uint32_t tw = 0, tn = 0;
     uint64_t ncells = 0;
@@ -383,12 +417,30 @@ __device__ void nw_wave_reg(const NwDev &d, const NwParams &p, const uint32_t *i
             if ((i & 31) == 0 && act) tn = d.nmask[tn0 + (i >> 5)];
             if ((tn >> (i & 31)) & 1) t = 4;
         }
-        const int t5 = t * 5;
+        const uint32_t tab_lo = rowtab[2 * t], tab_hi = rowtab[2 * t + 1];
         const int beg = max(0, i - w);                         // :90-91
         const int end = min(qlen, i + w + 1);
         const int width = act ? end - beg : 0;                 // cells this row
         ncells += (uint64_t)max(width, 0);
         const bool adv = act && i > w;                         // beg = i - w moved by one
+        // Blocks [0, nfull) lie inside every active lane's band; on rows
+        // past every lane's ramp they run unmasked.
+        // A row runs the blocks holding every active lane's band and the
+        // cell after it (the shift carries the sentinel, and the traceback
+        // may read the flags one cell past the band, which must be zero),
+        // at most ceil(cmax / 8); the entries beyond hold the sentinel in
+        // both frames.
+        uint32_t wlo = act ? (uint32_t)width : 0xFFFFFFFFu;
+        uint32_t whi = act ? ~(uint32_t)width : 0xFFFFFFFFu;
+        wave_min2_u32(wlo, whi);
+        const int wmin = (int)wlo, wmaxr = (int)~whi;
+        const int nblk = min((wmaxr + 8) >> 3, (cmax + 7) >> 3);
+        const int nfull = i > wmax ? min(wmin, cmax) >> 3 : 0;
+        // this row's query words: from beg / 8 (0 for lanes without a row)
+        const int qb = act ? beg : 0;
+        const uint32_t *qrow = ql + (qb >> 3) * 64;
+        const uint32_t qsh = (uint32_t)(qb & 7) * 4;
+        uint32_t qlo = qrow[0];
         int f = kNegInf, h_diag, h_left;
         {                                                      // :96-104
             const uint32_t v0 = adv ? R[1] : R[0];
@@ -398,47 +450,88 @@ __device__ void nw_wave_reg(const NwDev &d, const NwParams &p, const uint32_t *i
         }
 #pragma unroll
         for (int cb = 0; cb < K / 8; ++cb) {                   // :108-141
-            if (cb * 8 >= cmax) break;
-            const int j0 = beg + cb * 8;
-            const int wq = max(0, min(j0 >> 3, qlim));
-            const uint64_t win = (uint64_t)ql[wq * 64] | ((uint64_t)ql[min(wq + 1, qlim) * 64] << 32);
-            const uint32_t codes = (uint32_t)(win >> ((j0 & 7) * 4));
-            int sc[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) sc[k] = smat[t5 + (int)((codes >> (4 * k)) & 0xFu)];
-            uint32_t zword = 0;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const int kk = cb * 8 + k;
-                const uint32_t v = adv ? R[kk + 2] : R[kk + 1];
-                const bool inb = kk < width;
-                const int h_cur = h_diag + sc[k];
-                const int hd = St::h(v);
-                const int e_open = hd - p.oe_del;
-                const int e_ext = St::e(v) - p.e_del;
-                const int ev = max(e_open, e_ext);
-                const int f_open = h_left - p.oe_ins;
-                const int f_ext = f - p.e_ins;
-                f = max(f_open, f_ext);
-                const int h1 = max(ev, h_cur);
-                const int hn = max(f, h1);
-                zword = __builtin_amdgcn_alignbit(zword, (uint32_t)(h1 - f), 31);        // f > h1
-                zword = __builtin_amdgcn_alignbit(zword, (uint32_t)(h_cur - ev), 31);    // ev > h_cur
-                zword = __builtin_amdgcn_alignbit(zword, (uint32_t)(f_ext - f_open), 31);
-                zword = __builtin_amdgcn_alignbit(zword, (uint32_t)(e_ext - e_open), 31);
-                R[kk + 1] = inb ? __builtin_amdgcn_perm((uint32_t)ev, (uint32_t)hn, 0x05040100u) : v;
-                // the new state is computed here, not sunk to the row's end:
-                // sinking keeps every cell's h and e live across the row and
-                // the old and new arrays in separate registers
-                asm volatile("" : "+v"(R[kk + 1]));
-                h_diag = hd;
-                h_left = hn;
+            // a guard, not a break: with an exit the loop has a run-time trip
+            // count, is not unrolled, and R goes to scratch memory
+            if (cb < nblk) {
+                const uint32_t qhi = qrow[(cb + 1) * 64];
+                const uint32_t codes = __builtin_amdgcn_alignbit(qhi, qlo, qsh);
+                qlo = qhi;
+                // codes of cells 0..7 as byte selectors, then the row's scores
+                const uint32_t xe = codes & 0x0F0F0F0Fu;           // cells 0, 2, 4, 6
+                const uint32_t xo = (codes >> 4) & 0x0F0F0F0Fu;    // cells 1, 3, 5, 7
+                const uint32_t s03 = __builtin_amdgcn_perm(tab_hi, tab_lo,
+                                                           __builtin_amdgcn_perm(xo, xe, 0x05010400u));
+                const uint32_t s47 = __builtin_amdgcn_perm(tab_hi, tab_lo,
+                                                           __builtin_amdgcn_perm(xo, xe, 0x07030602u));
+                uint32_t zword = 0;
+                if (cb < nfull) {
+    #pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        const int kk = cb * 8 + k;
+                        const uint32_t v = R[kk + 2];
+                        const int sc = (int)(int8_t)((k < 4 ? s03 : s47) >> (8 * (k & 3)));
+                        const int h_cur = h_diag + sc;
+                        const int hd = St::h(v);
+                        const int e_open = hd - p.oe_del;
+                        const int e_ext = St::e(v) - p.e_del;
+                        const int ev = max(e_open, e_ext);
+                        const int f_open = h_left - p.oe_ins;
+                        const int f_ext = f - p.e_ins;
+                        f = max(f_open, f_ext);
+                        const int h1 = max(ev, h_cur);
+                        const int hn = max(f, h1);
+                        zword = __builtin_amdgcn_alignbit(zword, (uint32_t)(h1 - f), 31);
+                        zword = __builtin_amdgcn_alignbit(zword, (uint32_t)(h_cur - ev), 31);
+                        zword = __builtin_amdgcn_alignbit(zword, (uint32_t)(f_ext - f_open), 31);
+                        zword = __builtin_amdgcn_alignbit(zword, (uint32_t)(e_ext - e_open), 31);
+                        R[kk + 1] = __builtin_amdgcn_perm((uint32_t)ev, (uint32_t)hn, 0x05040100u);
+                        // the new state is computed here, not sunk to the row's
+                        // end: sinking keeps every cell's h and e live across the
+                        // row and the old and new arrays in separate registers
+                        asm volatile("" : "+v"(R[kk + 1]));
+                        h_diag = hd;
+                        h_left = hn;
+                    }
+                } else {
+    #pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        const int kk = cb * 8 + k;
+                        const uint32_t v = adv ? R[kk + 2] : R[kk + 1];
+                        const bool inb = kk < width;
+                        const int sc = (int)(int8_t)((k < 4 ? s03 : s47) >> (8 * (k & 3)));
+                        const int h_cur = h_diag + sc;
+                        const int hd = St::h(v);
+                        const int e_open = hd - p.oe_del;
+                        const int e_ext = St::e(v) - p.e_del;
+                        const int ev = max(e_open, e_ext);
+                        const int f_open = h_left - p.oe_ins;
+                        const int f_ext = f - p.e_ins;
+                        f = max(f_open, f_ext);
+                        const int h1 = max(ev, h_cur);
+                        const int hn = max(f, h1);
+                        zword = __builtin_amdgcn_alignbit(zword, (uint32_t)(h1 - f), 31);        // f > h1
+                        zword = __builtin_amdgcn_alignbit(zword, (uint32_t)(h_cur - ev), 31);    // ev > h_cur
+                        zword = __builtin_amdgcn_alignbit(zword, (uint32_t)(f_ext - f_open), 31);
+                        zword = __builtin_amdgcn_alignbit(zword, (uint32_t)(e_ext - e_open), 31);
+                        R[kk + 1] = inb ? __builtin_amdgcn_perm((uint32_t)ev, (uint32_t)hn, 0x05040100u) : v;
+                        asm volatile("" : "+v"(R[kk + 1]));
+                        h_diag = hd;
+                        h_left = hn;
+                    }
+                    // cells past the band end record no flags
+                    const int nv = min(max(width - cb * 8, 0), 8);
+                    zword &= nv >= 8 ? 0xFFFFFFFFu : ~(0xFFFFFFFFu >> (4 * nv)) & (nv ? 0xFFFFFFFFu : 0u);
+                }
+                zl[((size_t)i * ncw + cb) * 64] = zword;           // i < tmax <= zrows
             }
-            {
-                const int nv = min(max(width - cb * 8, 0), 8);
-                zword &= nv >= 8 ? 0xFFFFFFFFu : ~(0xFFFFFFFFu >> (4 * nv)) & (nv ? 0xFFFFFFFFu : 0u);
-            }
-            zl[((size_t)i * ncw + cb) * 64] = zword;       // i < tmax <= zrows
+        }
+        // lanes on their last row take h[qlen] before a later unmasked block
+        // may overwrite their state
+        if (__builtin_amdgcn_ballot_w64(act && i == tlen - 1)) {
+            const int si = act && i == tlen - 1 ? sidx : -1;
+#pragma unroll
+            for (int s = 0; s < K + 2; ++s)
+                if (s == si) score = St::h(R[s]);
         }
     }
     if (d.cells) {
@@ -449,21 +542,13 @@ __device__ void nw_wave_reg(const NwDev &d, const NwParams &p, const uint32_t *i
     }
 
     if (!valid) return;
-    int32_t *o_score = d.score + job;
     if (!nonempty) {
-        *o_score = 0;
+        d.score[job] = 0;
         d.n_cigar[job] = 0;
         d.flags[job] = 0;
         return;
     }
-    // score = h[qlen] (:144): column qlen of the last row is R[qlen - beg_last].
-    {
-        const int sidx = qlen - max(0, tlen - 1 - w);
-        uint32_t sv = R[0];
-#pragma unroll
-        for (int s = 1; s < K + 2; ++s) sv = s == sidx ? R[s] : sv;
-        *o_score = St::h(sv);
-    }
+    d.score[job] = score;
     nw_traceback(d, job, qlen, tlen, w, ncol, [&](int i, int c) {
         return (zl[((size_t)i * ncw + (c >> 3)) * 64] >> ((7 - (c & 7)) * 4)) & 0xFu;
     });
@@ -515,16 +600,33 @@ __global__ __launch_bounds__(64) void nw_kernel(NwDev d, NwParams p, const uint3
 }
 
 // The register-state form of nw_kernel (16-bit tier, bands of at most K
-// columns); LDS holds the scoring table and the staged query only.
+// columns); LDS holds the scoring table and the staged query only.  Each
+// instance is held to the register budget of 5, 4, 3 or 2 waves per SIMD
+// (K = 32, 64, 96, 128): the row state takes K + 2 registers, and the few
+// values the limit pushes out are per-job ones, reloaded outside the cells.
 template <int K>
-__global__ __launch_bounds__(64) void nw_reg_kernel(NwDev d, NwParams p, const uint32_t *order,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 32 ? 5 : K <= 64 ? 4 : K <= 96 ? 3 : 2)))
+void nw_reg_kernel(NwDev d, NwParams p, const uint32_t *order,
                                                     const WorkItem *items, int n_items,
                                                     uint32_t *counter, uint32_t *zbase, int zrows,
                                                     int ncw, int persistent, int qw8)
 {
     extern __shared__ __align__(16) unsigned char nw_lds[];
-    const int8_t *smat = nw_load_smat(p);
-    uint32_t *qlds = reinterpret_cast<uint32_t *>(nw_lds + 32);
+    // rows of the scoring matrix by target code t: word 2t holds the scores
+    // of query codes 0-3 (bytes 0-3), word 2t + 1 that of code 4 (byte 0)
+    uint32_t *rowtab = reinterpret_cast<uint32_t *>(nw_lds);
+    if (threadIdx.x < 10) {
+        const int t = threadIdx.x >> 1;
+        uint32_t v = 0;
+        if (threadIdx.x & 1) {
+            v = (uint8_t)p.mat[t * 5 + 4];
+        } else {
+            for (int c = 3; c >= 0; --c) v = (v << 8) | (uint8_t)p.mat[t * 5 + c];
+        }
+        rowtab[threadIdx.x] = v;
+    }
+    __syncthreads();
+    uint32_t *qlds = reinterpret_cast<uint32_t *>(nw_lds + 64);
     uint32_t *zreg = zbase + (size_t)blockIdx.x * zrows * ncw * 64;
     __shared__ uint32_t idx[64];
     __shared__ int item_sh;
@@ -539,7 +641,7 @@ __global__ __launch_bounds__(64) void nw_reg_kernel(NwDev d, NwParams p, const u
         const WorkItem wi = items[it];
         if (threadIdx.x < wi.count) idx[threadIdx.x] = order[wi.start + threadIdx.x];
         __syncthreads();
-        nw_wave_reg<K>(d, p, idx, (int)wi.count, zreg, zrows, ncw, smat, qlds, qw8);
+        nw_wave_reg<K>(d, p, idx, (int)wi.count, zreg, zrows, ncw, rowtab, qlds, qw8);
         __syncthreads();
         if (!persistent) it += gridDim.x;
     }
@@ -745,14 +847,14 @@ int nw_reg_cols(int ncol_max)
     return ncol_max <= 32 ? 32 : ncol_max <= 64 ? 64 : ncol_max <= 96 ? 96 : ncol_max <= 128 ? 128 : 0;
 }
 
-size_t nw_reg_lds_bytes(int qw8) { return 32 + (size_t)qw8 * 256; }
+size_t nw_reg_lds_bytes(int K, int qw8) { return 64 + (size_t)(qw8 + K / 8) * 256; }
 
 template <int K>
 static int nw_reg_occ(int qw8)
 {
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, nw_reg_kernel<K>, 64,
-                                                     nw_reg_lds_bytes(qw8)) != hipSuccess)
+                                                     nw_reg_lds_bytes(K, qw8)) != hipSuccess)
         return 0;
     return nb;
 }
@@ -773,7 +875,7 @@ static hipError_t nw_reg_t(const NwDev &d, const NwParams &p, const uint32_t *or
                            const WorkItem *items, int n_items, uint32_t *counter, uint32_t *zbase,
                            int zrows, int ncw, int grid, bool persistent, int qw8, hipStream_t s)
 {
-    hipLaunchKernelGGL((nw_reg_kernel<K>), dim3(grid), dim3(64), nw_reg_lds_bytes(qw8), s, d, p,
+    hipLaunchKernelGGL((nw_reg_kernel<K>), dim3(grid), dim3(64), nw_reg_lds_bytes(K, qw8), s, d, p,
                        order, items, n_items, counter, zbase, zrows, ncw, persistent ? 1 : 0, qw8);
     return hipGetLastError();
 }

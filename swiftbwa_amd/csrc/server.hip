@@ -100,6 +100,9 @@ struct ChunkReq {
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    // small NW bins run beside the large one on `side` (fork / join events)
+    hipStream_t side = nullptr;
+    hipEvent_t evf = nullptr, evj = nullptr;
     uint32_t *counters = nullptr;
     // pinned host staging of the slot's chunk (query words, extension
     // descriptors, result rows), grown on demand and reused: copies from it
@@ -520,24 +523,49 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
             if (ok) ok = hipMallocAsync(&scr, zb + gb + 512, st) == hipSuccess;
             note_phase(zb + gb + 512);
             if (ok) ok = launch_zero_u32(sl.counters, 64, st) == hipSuccess;
-            int k = 0;
+            if (launches.size() > 64) ok = false;
+            // as hipsw_nw_run: small bins in the wave-per-job form with their
+            // flags in LDS run on the side stream, queued first
+            auto side_ok = [&](const NwLaunch &L) {
+                return L.wide && L.zmax <= 49152 && launches.size() > 1;
+            };
+            auto run_wide = [&](const NwLaunch &L, int k, hipStream_t ls) {
+                const WorkItem &first = items[L.item_begin];
+                const WorkItem &last = items[L.item_begin + L.item_count - 1];
+                return launch_nw_wide(nd, np, d_order + first.start,
+                                      (int)(last.start + last.count - first.start), sl.counters + k,
+                                      L.qmax, (uint8_t *)scr, L.zmax, L.zmax <= 49152, L.grid,
+                                      ls) == hipSuccess;
+            };
+            bool forked = false;
+            for (size_t li = 0; ok && li < launches.size(); ++li) {
+                if (!side_ok(launches[li])) continue;
+                if (!forked) {
+                    ok = hipEventRecord(sl.evf, st) == hipSuccess &&
+                         hipStreamWaitEvent(sl.side, sl.evf, 0) == hipSuccess;
+                    forked = true;
+                }
+                if (ok) ok = run_wide(launches[li], (int)li, sl.side);
+            }
+            int k = -1;
             for (const NwLaunch &L : launches) {
-                if (!ok || k >= 64) { ok = false; break; }
+                ++k;
+                if (!ok) break;
+                if (side_ok(L)) continue;
                 if (L.wide) {
-                    const WorkItem &first = items[L.item_begin];
-                    const WorkItem &last = items[L.item_begin + L.item_count - 1];
-                    ok = launch_nw_wide(nd, np, d_order + first.start,
-                                        (int)(last.start + last.count - first.start), sl.counters + k,
-                                        L.qmax, (uint8_t *)scr, L.zmax, L.zmax <= 49152, L.grid,
-                                        st) == hipSuccess;
-                    ++k;
+                    ok = run_wide(L, k, st);
                     continue;
                 }
                 ok = launch_nw(L.tier, L.global_state, L.reg, nd, np, d_order, d_items + L.item_begin,
                                L.item_count, sl.counters + k, L.S, (uint32_t *)scr, L.zrows,
                                (L.ncol_max + 7) / 8, L.grid, pass == 0, (char *)scr + ((zb + 255) & ~(size_t)255),
                                L.qw8, st) == hipSuccess;
-                ++k;
+            }
+            if (forked) {
+                // joined even after a failed launch: scr is freed in stream order
+                const bool j = hipEventRecord(sl.evj, sl.side) == hipSuccess &&
+                               hipStreamWaitEvent(st, sl.evj, 0) == hipSuccess;
+                ok = ok && j;
             }
             if (scr) hipFreeAsync(scr, st);
             // the NW scores only, into the slot's pinned score row
@@ -745,6 +773,9 @@ extern "C" hipsw_server *hipsw_server_create(hipsw_engine *eng, int32_t n_slots,
     for (int s = 0; s < n_slots; ++s) {
         auto sl = std::make_unique<Slot>();
         if (hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking) != hipSuccess ||
+            hipStreamCreateWithFlags(&sl->side, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&sl->evf, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&sl->evj, hipEventDisableTiming) != hipSuccess ||
             hipMalloc((void **)&sl->counters, 64 * sizeof(uint32_t)) != hipSuccess) {
             delete srv;
             return nullptr;
@@ -979,6 +1010,12 @@ extern "C" void hipsw_server_destroy(hipsw_server *srv)
             hipStreamSynchronize(sl->stream);
             hipStreamDestroy(sl->stream);
         }
+        if (sl->side) {
+            hipStreamSynchronize(sl->side);
+            hipStreamDestroy(sl->side);
+        }
+        if (sl->evf) hipEventDestroy(sl->evf);
+        if (sl->evj) hipEventDestroy(sl->evj);
         if (sl->counters) hipFree(sl->counters);
         if (sl->pin) (void)hipHostFree(sl->pin);
     }
