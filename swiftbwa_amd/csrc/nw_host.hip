@@ -27,6 +27,15 @@ struct hipsw_nw_set {
     int32_t *d_w = nullptr;
     uint32_t *d_order = nullptr;
     WorkItem *d_items = nullptr;
+    // retry passes: true scores, the doubled bands, the pass's order and
+    // items, and the jobs the device selects for the next pass (count last)
+    int32_t *d_ts = nullptr, *d_w_r = nullptr;
+    uint32_t *d_order_r = nullptr, *d_next = nullptr;
+    WorkItem *d_items_r = nullptr;
+    // which mode's first-pass plan d_order / d_items hold (-1: none), and
+    // whether d_w holds w0
+    int p0_on_device = -1;
+    bool w0_on_device = false;
     std::vector<int32_t> final_w;
     // first-pass plans, per mode (the retry passes are planned on the fly)
     bool plan_ready[2] = {false, false};
@@ -115,7 +124,8 @@ int nw_prepare(int n, const uint8_t *queries, const uint32_t *q_off, const uint1
     P.arena_bytes = span<uint32_t>(words) + span<uint32_t>(nwords) + 4 * span<uint32_t>(n) +
                     3 * span<int32_t>(n) + span<int32_t>(n) + span<uint32_t>((size_t)n * cap) +
                     2 * span<int32_t>(n) + span<unsigned long long>(1) + span<uint32_t>(n) +
-                    span<WorkItem>(P.items_cap);
+                    span<WorkItem>(P.items_cap) + 2 * span<int32_t>(n) + span<uint32_t>(n) +
+                    span<WorkItem>(P.items_cap) + span<uint32_t>((size_t)n + 1);
     return HIPSW_OK;
 }
 
@@ -151,6 +161,13 @@ hipError_t nw_bind_upload(NwPrep &P, void *dmem, hipsw_nw_set *set, hipStream_t 
     unsigned long long *d_cells = carve<unsigned long long>(cur, 1);
     set->d_order = carve<uint32_t>(cur, n);
     set->d_items = carve<WorkItem>(cur, P.items_cap);
+    set->d_ts = carve<int32_t>(cur, n);
+    set->d_w_r = carve<int32_t>(cur, n);
+    set->d_order_r = carve<uint32_t>(cur, n);
+    set->d_items_r = carve<WorkItem>(cur, P.items_cap);
+    set->d_next = carve<uint32_t>(cur, (size_t)n + 1);
+    set->p0_on_device = -1;
+    set->w0_on_device = false;
     set->dev = NwDev{d_seq, d_nmask, d_qw, d_tw, d_qn, d_tn, d_ql, d_tl, set->d_w, d_score,
                      d_cig, d_ncig, d_flags, d_cells, cap};
     hipError_t e = hipSuccess;
@@ -165,6 +182,7 @@ hipError_t nw_bind_upload(NwPrep &P, void *dmem, hipsw_nw_set *set, hipStream_t 
     up(d_tn, P.pj.tnw.data(), (size_t)n * 4);
     up(d_ql, P.pj.ql.data(), (size_t)n * 4);
     up(d_tl, P.pj.tl.data(), (size_t)n * 4);
+    up(set->d_ts, P.ts.data(), (size_t)n * 4);
     return e;
 }
 
@@ -364,10 +382,8 @@ static int nw_run_ctx(hipsw_engine *eng, ExecCtx &c, hipsw_nw_set *set, int mode
     double kernel_ms = 0;
     int64_t launches_total = 0;
     std::vector<int32_t> w = set->w0;
-    std::vector<int> tries(n, 0);
     std::vector<uint32_t> active(n);
     std::iota(active.begin(), active.end(), 0u);
-    std::vector<int32_t> score(n);
     std::vector<uint32_t> order;
     std::vector<WorkItem> items;
     std::vector<NwLaunch> launches;
@@ -393,6 +409,7 @@ static int nw_run_ctx(hipsw_engine *eng, ExecCtx &c, hipsw_nw_set *set, int mode
                 set->plan_launches[mode] = launches;
                 set->plan_ready[mode] = true;
                 set->plan_ring[mode] = eng->nw_ring.load();
+                set->p0_on_device = -1;                        // the device holds an older plan
             }
         }
         if (launches.size() > 64) {
@@ -422,12 +439,33 @@ static int nw_run_ctx(hipsw_engine *eng, ExecCtx &c, hipsw_nw_set *set, int mode
         }
         uint32_t *zbase = reinterpret_cast<uint32_t *>(nwbuf);
         void *gstate = reinterpret_cast<char *>(nwbuf) + ((zbytes + 255) & ~(size_t)255);
-        bool ok = eng->check(hipMemcpyAsync(set->d_w, w.data(), (size_t)n * 4,
-                                            hipMemcpyHostToDevice, c.stream), "upload w");
-        if (ok) ok = eng->check(hipMemcpyAsync(set->d_order, order.data(), order.size() * 4,
-                                               hipMemcpyHostToDevice, c.stream), "upload order");
-        if (ok) ok = eng->check(hipMemcpyAsync(set->d_items, items.data(), items.size() * sizeof(WorkItem),
+        // The first pass reads d_w (w0), d_order and d_items, uploaded once
+        // per plan; a retry pass reads d_w_r (its bands, doubled on the
+        // device) and its own small order and items.
+        bool ok = true;
+        uint32_t *d_ord = pass == 0 ? set->d_order : set->d_order_r;
+        WorkItem *d_it = pass == 0 ? set->d_items : set->d_items_r;
+        NwDev dev = set->dev;
+        if (pass == 0) {
+            if (!set->w0_on_device) {
+                ok = eng->check(hipMemcpyAsync(set->d_w, set->w0.data(), (size_t)n * 4,
+                                               hipMemcpyHostToDevice, c.stream), "upload w");
+                set->w0_on_device = ok;
+            }
+            if (ok && set->p0_on_device != mode) {
+                ok = eng->check(hipMemcpyAsync(d_ord, order.data(), order.size() * 4,
+                                               hipMemcpyHostToDevice, c.stream), "upload order") &&
+                     eng->check(hipMemcpyAsync(d_it, items.data(), items.size() * sizeof(WorkItem),
                                                hipMemcpyHostToDevice, c.stream), "upload items");
+                set->p0_on_device = ok ? mode : -1;
+            }
+        } else {
+            dev.w = set->d_w_r;
+            ok = eng->check(hipMemcpyAsync(d_ord, order.data(), order.size() * 4,
+                                           hipMemcpyHostToDevice, c.stream), "upload order") &&
+                 eng->check(hipMemcpyAsync(d_it, items.data(), items.size() * sizeof(WorkItem),
+                                           hipMemcpyHostToDevice, c.stream), "upload items");
+        }
         if (ok && eng->inject(HIPSW_FAULT_LAUNCH)) {
             eng->set_error("hipsw_nw_run: kernel launch failed (injected)");
             ok = false;
@@ -435,8 +473,10 @@ static int nw_run_ctx(hipsw_engine *eng, ExecCtx &c, hipsw_nw_set *set, int mode
         if (ok) ok = eng->check(launch_zero_u32(c.counters, 64, c.stream), "zero");
         if (ok) ok = eng->check(hipEventRecord(c.ev0, c.stream), "event");
         // Small bins sent to the wave-per-job form with their flags in LDS
-        // touch no shared scratch: they run on the side stream, queued first,
-        // beside the lane-per-job launches.
+        // touch no shared scratch: they run on the side stream, queued first
+        // (their waves start before the lane-per-job launch fills the GPU and
+        // end inside it; queued after it, they wait for its first waves to
+        // retire and run past its end).
         bool forked = false;
         auto side_ok = [&](const NwLaunch &L) {
             return L.wide && L.zmax <= 49152 && launches.size() > 1;
@@ -446,7 +486,7 @@ static int nw_run_ctx(hipsw_engine *eng, ExecCtx &c, hipsw_nw_set *set, int mode
             const WorkItem &first = items[L.item_begin];
             const WorkItem &last = items[L.item_begin + L.item_count - 1];
             const int nj = (int)(last.start + last.count - first.start);
-            return eng->check(launch_nw_wide(set->dev, set->params, set->d_order + first.start, nj,
+            return eng->check(launch_nw_wide(dev, set->params, d_ord + first.start, nj,
                                              c.counters + k, L.qmax,
                                              reinterpret_cast<uint8_t *>(zbase), L.zmax,
                                              L.zmax <= 49152, L.grid, ls),
@@ -473,8 +513,8 @@ static int nw_run_ctx(hipsw_engine *eng, ExecCtx &c, hipsw_nw_set *set, int mode
                 continue;
             }
             const int ncw = (L.ncol_max + 7) / 8;
-            ok = eng->check(launch_nw(L.tier, L.global_state, L.reg, set->dev, set->params, set->d_order,
-                                      set->d_items + L.item_begin, L.item_count,
+            ok = eng->check(launch_nw(L.tier, L.global_state, L.reg, dev, set->params, d_ord,
+                                      d_it + L.item_begin, L.item_count,
                                       c.counters + k, L.S, zbase, L.zrows, ncw, L.grid,
                                       pass == 0 && mode == HIPSW_MODE_BATCHED, gstate, L.qw8,
                                       c.stream),
@@ -501,20 +541,27 @@ static int nw_run_ctx(hipsw_engine *eng, ExecCtx &c, hipsw_nw_set *set, int mode
             eng->set_error("hipsw_nw_run: download failed (injected)");
             return HIPSW_E_DEVICE;
         }
-        if (!eng->check(hipMemcpyAsync(score.data(), set->dev.score, (size_t)n * 4,
-                                       hipMemcpyDeviceToHost, c.stream), "download scores") ||
-            !eng->check(hipStreamSynchronize(c.stream), "download scores"))
+        // CIGARGenerator.swift:88-96: retry while score < trueScore, at most
+        // max_retries times (every job of pass p has been tried p times).
+        for (uint32_t j : active) set->final_w[j] = w[j];
+        if (pass >= set->max_retries) break;
+        uint32_t *d_count = set->d_next + n;
+        uint32_t cnt = 0;
+        if (!eng->check(launch_zero_u32(d_count, 1, c.stream), "zero") ||
+            !eng->check(launch_nw_select(d_ord, (int)active.size(), set->dev.score, set->d_ts,
+                                         pass == 0 ? set->d_w : set->d_w_r, set->d_w_r,
+                                         set->d_next, d_count, c.stream), "retry select") ||
+            !eng->check(hipMemcpyAsync(&cnt, d_count, 4, hipMemcpyDeviceToHost, c.stream),
+                        "download retry count") ||
+            !eng->check(hipStreamSynchronize(c.stream), "retry count"))
             return HIPSW_E_DEVICE;
-        // CIGARGenerator.swift:88-96: retry while score < trueScore.
-        std::vector<uint32_t> next;
-        for (uint32_t j : active) {
-            set->final_w[j] = w[j];
-            if (score[j] < set->ts[j] && tries[j] < set->max_retries) {
-                w[j] = w[j] < (1 << 20) ? w[j] * 2 : w[j];
-                tries[j] += 1;
-                next.push_back(j);
-            }
-        }
+        std::vector<uint32_t> next(cnt);
+        if (cnt && (!eng->check(hipMemcpyAsync(next.data(), set->d_next, (size_t)cnt * 4,
+                                               hipMemcpyDeviceToHost, c.stream), "download retries") ||
+                    !eng->check(hipStreamSynchronize(c.stream), "retries")))
+            return HIPSW_E_DEVICE;
+        std::sort(next.begin(), next.end());                   // the device appends in no fixed order
+        for (uint32_t j : next) w[j] = w[j] < (1 << 20) ? w[j] * 2 : w[j];
         active.swap(next);
     }
     if (stats) {

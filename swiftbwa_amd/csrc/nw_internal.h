@@ -53,6 +53,12 @@ hipError_t launch_nw(int tier, bool global_state, int reg, const NwDev &d, const
                      uint32_t *counter, int S, uint32_t *zbase, int zrows, int ncw, int grid,
                      bool persistent, void *gstate, int qw8, hipStream_t s);
 
+// Retry selection on the device: jobs of list[0..m) with score < ts go to
+// next[0..*count) (count zeroed by the caller) with w_next[j] = 2 w_cur[j].
+hipError_t launch_nw_select(const uint32_t *list, int m, const int32_t *score, const int32_t *ts,
+                            const int32_t *w_cur, int32_t *w_next, uint32_t *next,
+                            uint32_t *count, hipStream_t s);
+
 // Wave-per-job form (nw_kernel.hip): jobs[0..n), query lengths <= 512 (the
 // register chunks of the instance chosen by nw_wide_kc(qmax); 0 = none fits),
 // o_ins >= 0.  Flag matrices of at most zmax bytes each, in LDS (zlds) or at

@@ -808,6 +808,35 @@ __global__ __launch_bounds__(64) void nw_wide_kernel(NwDev d, NwParams p, const 
     }
 }
 
+// The CPU retry loop's test (CIGARGenerator.swift:88-96) on the device: of
+// the m jobs list[0..m), those whose score is below their true score are
+// appended to next[0..*count) (in no fixed order), with their band doubled
+// into w_next (w < 2^20: the cap of the band the kernels accept).
+__global__ void nw_select_kernel(const uint32_t *list, int m, const int32_t *score,
+                                 const int32_t *ts, const int32_t *w_cur, int32_t *w_next,
+                                 uint32_t *next, uint32_t *count)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const uint32_t j = list[k];
+    if (score[j] < ts[j]) {
+        const uint32_t slot = atomicAdd(count, 1u);
+        next[slot] = j;
+        const int32_t wv = w_cur[j];
+        w_next[j] = wv < (1 << 20) ? wv * 2 : wv;
+    }
+}
+
+hipError_t launch_nw_select(const uint32_t *list, int m, const int32_t *score, const int32_t *ts,
+                            const int32_t *w_cur, int32_t *w_next, uint32_t *next,
+                            uint32_t *count, hipStream_t s)
+{
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(nw_select_kernel, dim3((m + 255) / 256), dim3(256), 0, s, list, m, score,
+                       ts, w_cur, w_next, next, count);
+    return hipGetLastError();
+}
+
 int nw_wide_kc(int qmax) { return qmax <= 128 ? 2 : qmax <= 256 ? 4 : qmax <= 512 ? 8 : 0; }
 
 hipError_t launch_nw_wide(const NwDev &d, const NwParams &p, const uint32_t *jobs, int n,
