@@ -194,18 +194,48 @@ ExtParams make_params(const hipsw_scoring *sc)
     return p;
 }
 
+namespace {
+
+constexpr uint64_t kHigh6 = 0xFCFCFCFCFCFCFCFCull;   // any byte >= 4
+
+inline uint64_t load8(const uint8_t *p)
+{
+    uint64_t x;
+    memcpy(&x, p, 8);
+    return x;
+}
+
+// Eight codes 0..3, one per byte, to 16 bits (byte k at bits 2k..2k+1).
+inline uint32_t gather8(uint64_t x)
+{
+    x = (x | (x >> 6)) & 0x000F000F000F000Full;
+    x = (x | (x >> 12)) & 0x000000FF000000FFull;
+    return (uint32_t)((x | (x >> 24)) & 0xFFFFu);
+}
+
+}  // namespace
+
 // 2-bit packing of one sequence into `dst` (16 bases per word, base k at bits
-// 2k..2k+1; N -> code 0 plus an N bit).  Returns false on a code > 4.
+// 2k..2k+1; N -> code 0 plus an N bit).  Returns false on a code > 4.  Whole
+// words of codes 0..3 go eight bytes at a time; a word holding an N (or a bad
+// code) and the last partial word go byte by byte.
 bool pack2(const uint8_t *src, int len, uint32_t *dst, uint32_t *ndst)
 {
     const int words = (len + 15) / 16;
     for (int wdx = 0; wdx < words; ++wdx) {
-        uint32_t v = 0;
         const int lo = wdx * 16, hi = std::min(len, lo + 16);
+        if (hi - lo == 16) {
+            const uint64_t a = load8(src + lo), b = load8(src + lo + 8);
+            if (((a | b) & kHigh6) == 0) {
+                dst[wdx] = gather8(a) | (gather8(b) << 16);
+                continue;
+            }
+        }
+        uint32_t v = 0;
         for (int k = lo; k < hi; ++k) {
-            const uint8_t b = src[k];
-            if (b > 4) return false;
-            v |= (uint32_t)(b & 3) << ((k - lo) * 2);
+            const uint8_t c = src[k];
+            if (c > 4) return false;
+            v |= (uint32_t)(c & 3) << ((k - lo) * 2);
         }
         dst[wdx] = v;
     }
@@ -224,7 +254,10 @@ bool pack2(const uint8_t *src, int len, uint32_t *dst, uint32_t *ndst)
 
 bool has_n(const uint8_t *src, int len)
 {
-    for (int k = 0; k < len; ++k)
+    int k = 0;
+    for (; k + 8 <= len; k += 8)
+        if (load8(src + k) & kHigh6) return true;
+    for (; k < len; ++k)
         if (src[k] >= 4) return true;
     return false;
 }

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -160,16 +161,27 @@ size_t ext_layout_launches(const hipsw_engine *eng, std::vector<ExtLaunch> &laun
 size_t ext_scratch_bound(const hipsw_engine *eng, int n, const int32_t *ql, const int32_t *h0,
                          const uint8_t *has_n, int max_mat, bool std_mat)
 {
-    // per (tier, class, N) bin: its job count and longest query
+    // per (tier, class, N) bin: its job count and longest query (per-thread
+    // tallies merged under a lock)
     constexpr int kBins = 3 * kClasses * 2;
     int64_t cnt[kBins] = {0};
     int cmax[kBins] = {0};
-    for (int k = 0; k < n; ++k) {
-        const int b = (tier_of(h0[k], ql[k], max_mat) * kClasses + class_of(ql[k])) * 2 +
-                      (has_n && has_n[k] ? 1 : 0);
-        cnt[b]++;
-        cmax[b] = std::max(cmax[b], ql[k]);
-    }
+    std::mutex mu;
+    parallel_for(n, host_threads(), [&](int64_t lo, int64_t hi) {
+        int64_t c[kBins] = {0};
+        int m[kBins] = {0};
+        for (int64_t k = lo; k < hi; ++k) {
+            const int b = (tier_of(h0[k], ql[k], max_mat) * kClasses + class_of(ql[k])) * 2 +
+                          (has_n && has_n[k] ? 1 : 0);
+            c[b]++;
+            m[b] = std::max(m[b], ql[k]);
+        }
+        std::lock_guard<std::mutex> lk(mu);
+        for (int b = 0; b < kBins; ++b) {
+            cnt[b] += c[b];
+            cmax[b] = std::max(cmax[b], m[b]);
+        }
+    });
     size_t bound = 0;
     for (int b = 0; b < kBins; ++b) {
         if (!cnt[b]) continue;

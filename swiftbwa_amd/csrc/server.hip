@@ -71,20 +71,42 @@ __global__ void fetch_windows_kernel(const uint8_t *pac, int64_t gl, const Windo
 
 }  // namespace hipsw
 
+// A per-job host array without value-initialisation: a chunk's arrays are
+// filled in parallel, so zeroing them first on one thread would cost more
+// than the fill (tens of MB per large chunk).
+template <typename T> struct RawVec {
+    std::unique_ptr<T[]> p;
+    RawVec() = default;
+    explicit RawVec(size_t n) : p(new T[n > 0 ? n : 1]) {}
+    void reset(size_t n) { p.reset(new T[n > 0 ? n : 1]); }
+    T *data() { return p.get(); }
+    const T *data() const { return p.get(); }
+    T &operator[](size_t i) { return p[i]; }
+    const T &operator[](size_t i) const { return p[i]; }
+};
+
 struct ChunkReq {
     int64_t ticket = 0;
     double t_submit = 0, t_start = 0, t_done = 0;
     size_t est_bytes = 0;
     int status = HIPSW_OK;
     bool done = false;
-    // inputs (owned copies)
+    // inputs (owned copies), per job in the order extensions, local SW, NW.
+    // The queries are packed at submit, in the device arena's layout: job k's
+    // 2-bit words follow the words of the jobs before it (ceil(len / 16)
+    // each), and its N mask, if it holds an N, follows the masks of the
+    // earlier jobs holding one (ceil(len / 32) words each).
     int n_ext = 0, n_loc = 0, n_nw = 0;
-    std::unique_ptr<uint8_t[]> qbytes;           // all queries, one arena (q_off / q_len)
-    std::vector<uint32_t> q_off;                 // per job (ext, then loc, then nw)
-    std::vector<uint16_t> q_len;
-    std::vector<int64_t> t_pos;
-    std::vector<uint16_t> t_len_req;
-    std::vector<uint8_t> t_rev;
+    RawVec<uint16_t> q_len;
+    RawVec<uint8_t> qhas_n;
+    RawVec<uint32_t> qwords, nmask;
+    uint64_t n_qwords = 0, n_nmask = 0;
+    RawVec<int64_t> t_pos;
+    RawVec<uint16_t> t_len_req;
+    RawVec<uint8_t> t_rev;
+    // sums the admission estimate needs, taken at submit
+    uint64_t est_words = 0;                      // query + requested target words
+    int est_loc_q = 1, est_loc_t = 1, est_nw_q = 1, est_nw_t = 1;
     std::vector<int32_t> h0, w, eb, nw_w, nw_ts;
     int32_t loc_min_sub = 0;
     std::vector<int64_t> loc_win;                // clamped window start per local job (-1: none)
@@ -114,17 +136,6 @@ struct Slot {
     std::thread th;
 };
 
-// A per-job host array without value-initialisation: a chunk's arrays are
-// filled in parallel, so zeroing them first on one thread would cost more
-// than the fill (tens of MB per large chunk).
-template <typename T> struct RawVec {
-    std::unique_ptr<T[]> p;
-    explicit RawVec(size_t n) : p(new T[n > 0 ? n : 1]) {}
-    T *data() { return p.get(); }
-    const T *data() const { return p.get(); }
-    T &operator[](size_t i) { return p[i]; }
-    const T &operator[](size_t i) const { return p[i]; }
-};
 
 // A view of an array inside a slot's pinned staging.
 template <typename T> struct PinArr {
@@ -249,7 +260,7 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
     // Sequence arena: every query's words, then every target's words (written
     // by the window fetch on the device), then 64 zero words of padding, so
     // the host packs and uploads the queries only.
-    RawVec<uint8_t> qhas_n(n);
+    const uint8_t *qhas_n = r.qhas_n.data();
     // word offsets: per-block sums in parallel, a scan over the blocks, then
     // each block's offsets in parallel.  The blocks are parallel_for's own
     // chunks of the jobs (one thread each; one call covers them all when the
@@ -266,7 +277,6 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
     blocks([&](int64_t b, int64_t k0, int64_t k1) {
         uint64_t sq = 0, stt = 0, sn = 0;
         for (int64_t k = k0; k < k1; ++k) {
-            qhas_n[k] = has_n(r.qbytes.get() + r.q_off[k], r.q_len[k]);
             const int q = r.q_len[k], t = fetched_len(r.t_pos[k], r.t_len_req[k], srv->gl);
             sq += (q + 15) / 16;
             stt += (t + 15) / 16;
@@ -313,15 +323,17 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
             win[k] = Window{r.t_pos[k], tw[k], tl[k], (int32_t)r.t_rev[k]};
         }
     });
-    std::vector<uint32_t> nmask(nwords, 0);
-    std::atomic<bool> packed_ok{true};
-    parallel_for(n, host_threads(), [&](int64_t lo, int64_t hi) {
-        for (int64_t k = lo; k < hi; ++k)
-            if (!pack2(r.qbytes.get() + r.q_off[k], ql[k], hq + qw[k],
-                       qnw[k] != kNoN ? nmask.data() + qnw[k] : nullptr))
-                packed_ok = false;
-    });
-    if (!packed_ok) return HIPSW_E_INVALID;
+    // the query words, packed at submit in this layout, into the pinned staging
+    if (qwords != r.n_qwords || nwords - 64 != r.n_nmask) return HIPSW_E_INVALID;
+    {
+        const int64_t cw = 1 << 18;                            // 1 MiB pieces
+        parallel_for((int64_t)((qwords + cw - 1) / cw), host_threads(), [&](int64_t lo, int64_t hi) {
+            for (int64_t p = lo; p < hi; ++p) {
+                const uint64_t a = (uint64_t)p * cw, e = std::min<uint64_t>(qwords, a + cw);
+                memcpy(hq + a, r.qwords.data() + a, (e - a) * 4);
+            }
+        });
+    }
     // extension plan (jobs [0, n_ext))
     const ExtParams ep = make_params(&srv->sc);
     std::vector<uint32_t> ext_order;
@@ -404,7 +416,8 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
     };
     up(d_seq, hq, (size_t)qwords * 4);
     if (ok) ok = launch_zero_u32(d_seq + qwords + twords, 64, st) == hipSuccess;
-    up(d_nmask, nmask.data(), nwords * 4);
+    up(d_nmask, r.nmask.data(), (nwords - 64) * 4);
+    if (ok) ok = launch_zero_u32(d_nmask + (nwords - 64), 64, st) == hipSuccess;
     // qw .. win: one copy of the pinned block (same carve order and spans as
     // the arena, d_qw .. d_win)
     up(d_qw, qw.data(), meta_b);
@@ -659,19 +672,8 @@ size_t estimate_bytes(const hipsw_server *srv, const ChunkReq &r)
 {
     const size_t n = (size_t)(r.n_ext + r.n_loc + r.n_nw);
     const int cap = srv->nw_cap;
-    uint64_t words = 64, nwords = 64;
-    int loc_q = 1, loc_t = 1, nw_q = 1, nw_t = 1;
-    for (size_t k = 0; k < n; ++k) {
-        words += (r.q_len[k] + 15) / 16 + (r.t_len_req[k] + 15) / 16;
-        nwords += (r.q_len[k] + 31) / 32;
-        if ((int)k >= r.n_ext && (int)k < r.n_ext + r.n_loc) {
-            loc_q = std::max(loc_q, (int)r.q_len[k]);
-            loc_t = std::max(loc_t, (int)r.t_len_req[k]);
-        } else if ((int)k >= r.n_ext + r.n_loc) {
-            nw_q = std::max(nw_q, (int)r.q_len[k]);
-            nw_t = std::max(nw_t, (int)r.t_len_req[k]);
-        }
-    }
+    const uint64_t words = 64 + r.est_words, nwords = 64 + r.n_nmask;
+    const int loc_q = r.est_loc_q, loc_t = r.est_loc_t, nw_q = r.est_nw_q, nw_t = r.est_nw_t;
     const size_t colmax_b = r.loc_min_sub > 0 ? (size_t)r.n_loc * loc_t : 1;
     const size_t arena = span<uint32_t>(words) + span<uint32_t>(nwords) + 4 * span<uint32_t>(n) +
                          5 * span<int32_t>(n) + span<Window>(n) + span<int32_t>(n * 6) +
@@ -682,11 +684,12 @@ size_t estimate_bytes(const hipsw_server *srv, const ChunkReq &r)
     size_t phase = 0;
     if (r.n_ext) {
         const ExtParams ep = make_params(&srv->sc);
-        std::vector<int32_t> ql(r.q_len.begin(), r.q_len.begin() + r.n_ext);
-        std::vector<uint8_t> qn(r.n_ext);
-        for (int k = 0; k < r.n_ext; ++k) qn[k] = has_n(r.qbytes.get() + r.q_off[k], r.q_len[k]);
-        phase = std::max(phase, ext_scratch_bound(srv->eng, r.n_ext, ql.data(), r.h0.data(), qn.data(),
-                                                  ep.max_mat, ep.std_mat != 0));
+        RawVec<int32_t> ql(r.n_ext);
+        parallel_for(r.n_ext, host_threads(), [&](int64_t lo, int64_t hi) {
+            for (int64_t k = lo; k < hi; ++k) ql[k] = r.q_len[k];
+        });
+        phase = std::max(phase, ext_scratch_bound(srv->eng, r.n_ext, ql.data(), r.h0.data(),
+                                                  r.qhas_n.data(), ep.max_mat, ep.std_mat != 0));
     }
     if (r.n_loc) {
         const size_t blocks = ((size_t)r.n_loc + 63) / 64;
@@ -855,50 +858,107 @@ extern "C" int64_t hipsw_server_submit(hipsw_server *srv, const hipsw_chunk *c)
     for (const Src &sr : src)
         if (sr.cnt && (!sr.q || !sr.qo || !sr.ql || !sr.tp || !sr.tl)) return HIPSW_E_INVALID;
     const int64_t n = (int64_t)r->n_ext + r->n_loc + r->n_nw;
-    r->q_off.resize(n);
-    r->q_len.resize(n);
-    r->t_pos.resize(n);
-    r->t_len_req.resize(n);
-    r->t_rev.resize(n);
-    uint64_t tot = 0;
-    for (int kind = 0, j = 0; kind < 3; ++kind)
-        for (int k = 0; k < src[kind].cnt; ++k, ++j) {
-            r->q_off[j] = (uint32_t)tot;
-            tot += src[kind].ql[k];
-        }
-    if (tot > 0xFFFFFFFFull) return HIPSW_E_INVALID;
-    r->qbytes.reset(new uint8_t[tot > 0 ? tot : 1]);
     const int64_t b1 = r->n_ext, b2 = b1 + r->n_loc;
-    parallel_for(n, host_threads(), [&](int64_t lo, int64_t hi) {
-        for (int64_t j = lo; j < hi; ++j) {
-            const int kind = j < b1 ? 0 : j < b2 ? 1 : 2;
-            const int64_t k = j - (kind == 0 ? 0 : kind == 1 ? b1 : b2);
-            const Src &sr = src[kind];
-            r->q_len[j] = sr.ql[k];
+    r->q_len.reset(n);
+    r->qhas_n.reset(n);
+    r->t_pos.reset(n);
+    r->t_len_req.reset(n);
+    r->t_rev.reset(n);
+    // Pass 1 over blocks of jobs: the fields, the N test and each block's
+    // word counts; then offsets by a scan over the blocks; pass 2 packs each
+    // query from the caller's bytes straight into the chunk's words.
+    const int nth = host_threads();
+    const int64_t bs = std::max<int64_t>(1, (n + nth - 1) / nth), nb = n > 0 ? (n + bs - 1) / bs : 0;
+    auto blocks = [&](const std::function<void(int64_t, int64_t, int64_t)> &body) {
+        parallel_for(n, nth, [&](int64_t lo, int64_t hi) {
+            for (int64_t b = lo / bs; b * bs < hi; ++b)
+                body(b, std::max(lo, b * bs), std::min(hi, (b + 1) * bs));
+        });
+    };
+    struct BlockSums {
+        uint64_t q = 0, nm = 0, t = 0;
+        int loc_q = 1, loc_t = 1, nw_q = 1, nw_t = 1;
+    };
+    std::vector<BlockSums> bsum(nb + 1);
+    auto src_of = [&](int64_t j, int64_t &k) -> const Src & {
+        const int kind = j < b1 ? 0 : j < b2 ? 1 : 2;
+        k = j - (kind == 0 ? 0 : kind == 1 ? b1 : b2);
+        return src[kind];
+    };
+    blocks([&](int64_t b, int64_t j0, int64_t j1) {
+        BlockSums &S = bsum[b + 1];
+        for (int64_t j = j0; j < j1; ++j) {
+            int64_t k;
+            const Src &sr = src_of(j, k);
+            const int q = sr.ql[k];
+            r->q_len[j] = (uint16_t)q;
             r->t_pos[j] = sr.tp[k];
             r->t_len_req[j] = sr.tl[k];
             r->t_rev[j] = sr.tr ? sr.tr[k] : 0;
-            memcpy(r->qbytes.get() + r->q_off[j], sr.q + sr.qo[k], sr.ql[k]);
+            r->qhas_n[j] = has_n(sr.q + sr.qo[k], q);
+            S.q += (q + 15) / 16;
+            S.t += (sr.tl[k] + 15) / 16;
+            if (r->qhas_n[j]) S.nm += (q + 31) / 32;
+            if (j >= b1 && j < b2) {
+                S.loc_q = std::max(S.loc_q, q);
+                S.loc_t = std::max(S.loc_t, (int)sr.tl[k]);
+            } else if (j >= b2) {
+                S.nw_q = std::max(S.nw_q, q);
+                S.nw_t = std::max(S.nw_t, (int)sr.tl[k]);
+            }
         }
     });
+    for (int64_t b = 0; b < nb; ++b) {
+        BlockSums &A = bsum[b + 1];
+        const BlockSums &P = bsum[b];
+        A.q += P.q; A.nm += P.nm; A.t += P.t;
+        A.loc_q = std::max(A.loc_q, P.loc_q); A.loc_t = std::max(A.loc_t, P.loc_t);
+        A.nw_q = std::max(A.nw_q, P.nw_q); A.nw_t = std::max(A.nw_t, P.nw_t);
+    }
+    const BlockSums &T = bsum[nb];
+    if (T.q + T.t > 0xFFFFFF00ull) return HIPSW_E_INVALID;
+    r->n_qwords = T.q;
+    r->n_nmask = T.nm;
+    r->est_words = T.q + T.t;
+    r->est_loc_q = T.loc_q; r->est_loc_t = T.loc_t; r->est_nw_q = T.nw_q; r->est_nw_t = T.nw_t;
+    r->qwords.reset(T.q);
+    r->nmask.reset(T.nm);
+    std::atomic<bool> packed_ok{true};
+    blocks([&](int64_t b, int64_t j0, int64_t j1) {
+        uint64_t q = bsum[b].q, m = bsum[b].nm;
+        for (int64_t j = j0; j < j1; ++j) {
+            int64_t k;
+            const Src &sr = src_of(j, k);
+            const int len = r->q_len[j];
+            if (!pack2(sr.q + sr.qo[k], len, r->qwords.data() + q,
+                       r->qhas_n[j] ? r->nmask.data() + m : nullptr))
+                packed_ok = false;
+            q += (len + 15) / 16;
+            if (r->qhas_n[j]) m += (len + 31) / 32;
+        }
+    });
+    if (!packed_ok) return HIPSW_E_INVALID;                 // a query byte above 4
     if (c->loc_rid) {
         // mate-rescue windows: fetchSequence clamping, and the template's
         // sequence must hold the window (MateRescue.swift:105-111, :252-256)
         std::lock_guard<std::mutex> lk(srv->mu);
         r->loc_win.assign(r->n_loc, -1);
-        for (int k = 0; k < r->n_loc; ++k) {
-            const int64_t j = b1 + k;
-            int64_t begin;
-            int32_t len, rid;
-            if (fetch_sequence(srv, r->t_pos[j], r->t_pos[j] + r->t_len_req[j], &begin, &len, &rid) &&
-                rid == c->loc_rid[k]) {
-                r->loc_win[k] = begin;
-                r->t_pos[j] = begin;
-                r->t_len_req[j] = (uint16_t)len;
-            } else {
-                r->t_len_req[j] = 0;                              // skipped: ok = 0
+        parallel_for(r->n_loc, nth, [&](int64_t lo, int64_t hi) {
+            for (int64_t k = lo; k < hi; ++k) {
+                const int64_t j = b1 + k;
+                int64_t begin;
+                int32_t len, rid;
+                if (fetch_sequence(srv, r->t_pos[j], r->t_pos[j] + r->t_len_req[j], &begin, &len,
+                                   &rid) &&
+                    rid == c->loc_rid[k]) {
+                    r->loc_win[k] = begin;
+                    r->t_pos[j] = begin;
+                    r->t_len_req[j] = (uint16_t)len;
+                } else {
+                    r->t_len_req[j] = 0;                          // skipped: ok = 0
+                }
             }
-        }
+        });
     }
     if (c->n_ext && (!c->ext_h0 || !c->ext_w || !c->ext_eb)) return HIPSW_E_INVALID;
     for (int k = 0; k < c->n_ext; ++k)
