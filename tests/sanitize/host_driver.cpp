@@ -412,5 +412,6 @@ int main()
     hipsw_engine_destroy(eng);
     std::printf("%s: %d mismatches\n", g_bad ? "SANITIZE DRIVER FAILED" : "SANITIZE DRIVER OK",
                 g_bad.load());
+    std::fflush(stdout);   // the verdict is out before the runtime's own teardown
     return g_bad ? 1 : 0;
 }
