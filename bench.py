@@ -588,8 +588,10 @@ def main():
                                                PEAK_PK16_TOPS, kernel="binned persistent bins")
     if nw:
         others["nw_500k_batched"] = roofline(nw["cells"], nw["batched"]["kernel_ms"], OPS_NW,
-                                             PEAK_I32_TOPS, kernel="nw_kernel + nw_wide_kernel",
-                                             note="int32 VALU peak: the fill runs 32-bit arithmetic")
+                                             PEAK_I32_TOPS,
+                                             kernel="nw_reg_kernel<96> + nw_wide_kernel (retry passes, wide bands)",
+                                             note="int32 VALU peak: the fill runs 32-bit arithmetic on "
+                                                  "16-bit state")
     if local:
         others["local_sw_200k"] = roofline(local["cells"], local["kernel_ms"], OPS_LOC, PEAK_I32_TOPS,
                                            kernel="local_striped_kernel<10> (+ scalar fallback)",
