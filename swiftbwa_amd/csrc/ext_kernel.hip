@@ -1025,44 +1025,173 @@ __device__ __forceinline__ uint32_t nz_bits(const u32x4 &va, const u32x4 &vb)
            (w32(vmin(u2(m2), one)) << 2) | (w32(vmin(u2(m3), one)) << 3);
 }
 
-// Row start of one half (:92-98, the h1 of :99-107 is made by the caller):
+// Row-level state of the packed form.  Each field pair holds job A in the low
+// and job B in the high 16-bit half, so the per-row bookkeeping of both jobs
+// is one packed instruction per step.  All values fit 16 bits: qlen <= 255,
+// every score of the 8-bit tier <= 255, rows and t_len <= 65535; -1 (max_i,
+// max_j, gscore, gtle before their first update) is 0xFFFF.
+struct PkRow {
+    uint32_t alive;                        // 0xFFFF per live half
+    us2 qlen, tlen, h0, w, beg, end;       // w capped (:52-62), at most 65535
+    us2 maxv, maxi, maxj, gsc, gtle, maxoff;
+    us2 nc16;                              // cells of this and earlier rows, < 65536
+    uint32_t ncells;                       // both jobs' cells flushed from nc16
+    uint32_t twe[2], two[2];               // target words of even / odd 16-row blocks
+    uint32_t tw0[2], tnw[2], qw0[2];
+};
+
+__device__ __forceinline__ us2 bfi16(uint32_t m, us2 a, us2 b) { return u2((m & w32(a)) | (~m & w32(b))); }
+// 0xFFFF in each half where a < b (signed halves, |a - b| < 32768)
+__device__ __forceinline__ uint32_t lt16(us2 a, us2 b)
+{
+    return w32(__builtin_bit_cast(
+        us2, (__builtin_bit_cast(ss2, a) - __builtin_bit_cast(ss2, b)) >> ss2{15, 15}));
+}
+// 0xFFFF in each half where a != 0 (one = the opaque constant 0x00010001)
+__device__ __forceinline__ uint32_t nz16(us2 a, us2 one) { return w32(splat(0) - vmin(a, one)); }
+// both halves of v set to min(v.lo, v.hi)
+__device__ __forceinline__ uint32_t hmin16(uint32_t v)
+{
+    return w32(vmin(u2(v), u2(__builtin_amdgcn_alignbit(v, v, 16))));
+}
+
+__device__ __forceinline__ void pkr_setup(PkRow &R, const ExtDev &d, const ExtParams &p,
+                                          const PkMeta *meta, int lane, int count)
+{
+    unsigned short ql[2], tl[2], h0[2], wc[2];
+    uint32_t al = 0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int slot = lane + 64 * h;
+        const bool valid = slot < count;
+        const PkMeta m = meta[valid ? slot : 0];
+        const int qlen = valid ? m.q_len : 0, tlen = valid ? m.t_len : 0;
+        int w = valid ? m.w : 0;
+        const int eb = valid ? m.eb : 0;
+        const bool nonempty = valid && qlen > 0 && tlen > 0;    // :33-35
+        if (nonempty) {                                          // w cap, :52-62
+            int mi = (int)((double)(qlen * p.max_mat + eb - p.o_ins) / (double)p.e_ins + 1.0);
+            w = min(w, max(1, mi));
+            int md = (int)((double)(qlen * p.max_mat + eb - p.o_del) / (double)p.e_del + 1.0);
+            w = min(w, max(1, md));
+            al |= 0xFFFFu << (16 * h);
+        }
+        // a band wider than 65535 clamps nothing (i - w < 0 and i + w + 1 > qlen)
+        ql[h] = (unsigned short)qlen;
+        tl[h] = (unsigned short)tlen;
+        h0[h] = (unsigned short)(valid ? m.h0 : 0);
+        wc[h] = (unsigned short)min(max(w, 0), 65535);
+        R.qw0[h] = valid ? m.q_word : 0u;
+        R.tw0[h] = valid ? m.t_word : 0u;
+        // target words of rows 0-15 and 16-31 (the second is a prefetch);
+        // word indexes stop at the job's last word + 1, inside the padded arena
+        R.tnw[h] = (uint32_t)(tlen + 15) >> 4;
+        R.twe[h] = d.seq[R.tw0[h]];
+        R.two[h] = d.seq[R.tw0[h] + min(1u, R.tnw[h])];
+    }
+    R.alive = al;
+    R.qlen = us2{ql[0], ql[1]};
+    R.tlen = us2{tl[0], tl[1]};
+    R.h0 = us2{h0[0], h0[1]};
+    R.w = us2{wc[0], wc[1]};
+    R.beg = splat(0);                                            // :79-87
+    R.end = R.qlen;
+    R.maxv = R.h0;
+    R.maxi = R.maxj = R.gsc = R.gtle = splat(0xFFFF);
+    R.maxoff = splat(0);
+    R.nc16 = splat(0);
+    R.ncells = 0;
+}
+
+// SWResult of half h (:183-190); an empty or absent job writes zeros.
+__device__ __forceinline__ void pkr_finish(const PkRow &R, const ExtDev &d, const PkMeta *meta,
+                                           int lane, int count, int h)
+{
+    const int slot = lane + 64 * h;
+    if (slot >= count) return;
+    const int sh = 16 * h;
+    auto s16 = [&](us2 v) { return (int)(short)(w32(v) >> sh); };     // -1 kept
+    auto u16 = [&](us2 v) { return (int)((w32(v) >> sh) & 0xFFFF); };
+    int32_t *o = d.out + (size_t)meta[slot].job * 6;
+    if (u16(R.qlen) > 0 && u16(R.tlen) > 0) {
+        const int mi = u16(R.maxi), gt = u16(R.gtle);
+        o[0] = u16(R.maxv);
+        o[1] = s16(R.maxj) + 1;
+        o[2] = mi == 0xFFFF ? 0 : mi + 1;
+        o[3] = gt == 0xFFFF ? 0 : gt + 1;
+        o[4] = s16(R.gsc);
+        o[5] = u16(R.maxoff);
+    } else {
+        o[0] = 0; o[1] = 0; o[2] = 0; o[3] = 0; o[4] = 0; o[5] = 0;
+    }
+}
+
+// Row start of both jobs (:92-98; the h1 of :99-107 is made by the caller):
 // band clamp -- the column the w-clamp drops is cleared, which keeps every
 // entry below beg at zero -- band end, cell count.
 template <int LC>
-__device__ __forceinline__ void pk2_row_start(PkJob &J, int i, uint32_t *col, uint32_t *gcol, int hs)
+__device__ __forceinline__ void pkr_row_start(PkRow &R, int i, uint32_t *col, uint32_t *gcol)
 {
-    J.tb = (((i >> 4) & 1 ? J.two : J.twe) >> ((i & 15) * 2)) & 3u;   // this row's target base
-    const int wb = i - J.w;
-    if (J.alive && J.beg < wb) {
-        // beg = i - w (:96) drops column beg: with w >= 0, beg >= i - 1 - w
-        // after the previous row, so exactly one column per row
-        store16<LC>(col, gcol, J.beg, hs, 0);
-        J.beg = wb;
-    }
-    J.end = min(J.end, i + J.w + 1);                           // :97-98 (end <= qlen holds)
-    J.ncells += (J.alive && J.end > J.beg) ? (uint32_t)(J.end - J.beg) : 0u;
+    // beg = i - w (:96) drops column beg: with w >= 0, beg >= i - 1 - w after
+    // the previous row, so exactly one column per row
+    const us2 nb = vmax(R.beg, vsubs(splat(i), R.w));
+    const uint32_t mv = (w32(nb) ^ w32(R.beg)) & R.alive;
+    if (mv & 0xFFFFu) store16<LC>(col, gcol, (int)(w32(R.beg) & 0xFFFF), 0, 0);
+    if (mv >> 16) store16<LC>(col, gcol, (int)(w32(R.beg) >> 16), 1, 0);
+    R.beg = bfi16(R.alive, nb, R.beg);
+    R.end = vmin(R.end, __builtin_elementwise_add_sat(R.w, splat(i + 1)));   // :97-98 (end <= qlen holds)
+    R.nc16 = R.nc16 + u2(w32(vsubs(R.end, R.beg)) & R.alive);
 }
 
-// Row end of one half (:145-172): band-end entry, global-score column, row
-// maximum, z-drop, and the last-row test of the next row (:90).
-template <int LC>
-__device__ __forceinline__ void pk2_row_end(PkJob &J, const ExtParams &p, int i, int h1, int key,
-                                            uint32_t *col, uint32_t *gcol, int hs)
+// Row end of both jobs (:145-172): band-end entry, global-score column, row
+// maximum, z-drop, and the last-row test of the next row (:90).  `key` is
+// the row maximum H * 256 + j per half (0 when the row's maximum is 0).
+template <bool SYM, int LC>
+__device__ __forceinline__ void pkr_row_end(PkRow &R, const ExtParams &p, int i, us2 h1, us2 key,
+                                            us2 one, uint32_t *col, uint32_t *gcol)
 {
-    store16<LC>(col, gcol, J.end, hs, (uint16_t)h1);          // h[end] = h1, e[end] = 0
-    const int rm = key >> 8, rj = key & 0xFF;
-    const bool at_q = J.alive && J.end == J.qlen;              // :149-152
-    J.gtle = (at_q && !(J.gsc > h1)) ? i : J.gtle;
-    J.gsc = at_q ? max(J.gsc, h1) : J.gsc;
-    const bool up = J.alive && rm > J.maxv;                    // :158-163
-    const int di = i - J.max_i, dj = rj - J.max_j;             // :164-172
-    const int drop = di > dj ? J.maxv - rm - (di - dj) * p.e_del : J.maxv - rm - (dj - di) * p.e_ins;
-    const bool zd = p.zdrop > 0 && drop > p.zdrop;
-    J.max_off = up ? max(J.max_off, abs(rj - i)) : J.max_off;
-    J.max_i = up ? i : J.max_i;
-    J.max_j = up ? rj : J.max_j;
-    J.maxv = up ? rm : J.maxv;
-    J.alive = J.alive && rm != 0 && (up || !zd) && i + 1 < J.tlen;   // :155, :171, :90
+    // h[end] = h1, e[end] = 0; a finished job's half is never read again, so
+    // its store needs no guard (end <= qlen keeps it in bounds)
+    store16<LC>(col, gcol, (int)(w32(R.end) & 0xFFFF), 0, (uint16_t)(w32(h1) & 0xFFFF));
+    store16<LC>(col, gcol, (int)(w32(R.end) >> 16), 1, (uint16_t)(w32(h1) >> 16));
+    const us2 rm = key >> splat(8), rj = u2(w32(key) & 0x00FF00FFu);
+    const us2 si = splat(i);
+    // :149-152: at the query end, gtle = i and gscore = h1 unless gscore > h1
+    const uint32_t atq = R.alive & ~nz16(u2(w32(R.end) ^ w32(R.qlen)), one);
+    const uint32_t c = atq & ~lt16(h1, R.gsc);
+    R.gtle = bfi16(c, si, R.gtle);
+    R.gsc = bfi16(c, h1, R.gsc);
+    // :158-163: a new maximum
+    const uint32_t up = R.alive & lt16(R.maxv, rm);
+    // :164-172: z-drop against the current maximum.  drop > zdrop is
+    // tested as dd * e < maxv - rm - zdrop with dd = |di - dj|; di saturates
+    // at 1023 and dd at 255 and e at 255, which keeps each sign and each
+    // outcome (maxv - rm <= 255), and dd * e within 16 bits.
+    const us2 di = vmin(si - R.maxi, splat(1023));                // i - max_i >= 1
+    const ss2 dd = __builtin_bit_cast(ss2, di) - __builtin_bit_cast(ss2, rj - R.maxj);
+    const ss2 nd = ss2{0, 0} - dd;
+    const us2 ad = vmin(__builtin_bit_cast(us2, __builtin_elementwise_max(dd, nd)), splat(255));
+    us2 ev;
+    if (SYM) {
+        ev = splat(min(p.e_del, 255));
+    } else {
+        const uint32_t pos = w32(__builtin_bit_cast(us2, nd >> ss2{15, 15}));   // di > dj
+        ev = bfi16(pos, splat(min(p.e_del, 255)), splat(min(p.e_ins, 255)));
+    }
+    const int zd_lim = p.zdrop > 0 ? p.zdrop : 0x7FFF;         // zdrop <= 0: never
+    const ss2 r = __builtin_elementwise_max(
+        __builtin_bit_cast(ss2, R.maxv) - __builtin_bit_cast(ss2, rm) - ss2{(short)zd_lim, (short)zd_lim},
+        ss2{0, 0});
+    const uint32_t zd = nz16(vsubs(__builtin_bit_cast(us2, r), ad * ev), one);
+    // :158-163 updates
+    const us2 off = u2(w32(vsubs(si, rj)) | w32(vsubs(rj, si)));   // |j - i|
+    R.maxoff = bfi16(up, vmax(R.maxoff, off), R.maxoff);
+    R.maxi = bfi16(up, si, R.maxi);
+    R.maxj = bfi16(up, rj, R.maxj);
+    R.maxv = bfi16(up, rm, R.maxv);
+    // :155 (a zero row maximum ends the job), :171 (z-drop), :90 (last row)
+    const uint32_t keep = nz16(vmin(rm, vsubs(R.tlen, splat(i + 1))), one);
+    R.alive = R.alive & keep & (up | ~zd);
 }
 
 // Band narrowing of both halves (:175-180), a 4-column group per trip and
@@ -1071,54 +1200,59 @@ __device__ __forceinline__ void pk2_row_end(PkJob &J, const ExtParams &p, int i,
 // column end holds (h1, 0), so only a zero h1 searches -- then
 // end = min(that + 2, qlen); none found: beg - 1.
 template <int LC>
-__device__ __forceinline__ void pk2_narrow(PkJob &A, PkJob &B, int h1a, int h1b,
-                                           const uint32_t *col, const uint32_t *gcol)
+__device__ __forceinline__ void pkr_narrow(PkRow &R, us2 h1, const uint32_t *col, const uint32_t *gcol)
 {
     constexpr int GL = LC / 4;
-    bool fa = A.alive && A.beg < A.end, fb = B.alive && B.beg < B.end;
-    int ga = A.beg >> 2, gb = B.beg >> 2;
-    uint32_t ma = 0xFu << (A.beg & 3), mb = 0xFu << (B.beg & 3);   // columns >= beg
+    const bool la = (R.alive & 0xFFFFu) != 0, lb = (R.alive >> 16) != 0;
+    int begA = (int)(w32(R.beg) & 0xFFFF), begB = (int)(w32(R.beg) >> 16);
+    const int endA = (int)(w32(R.end) & 0xFFFF), endB = (int)(w32(R.end) >> 16);
+    bool fa = la && begA < endA, fb = lb && begB < endB;
+    int ga = begA >> 2, gb = begB >> 2;
+    uint32_t ma = 0xFu << (begA & 3), mb = 0xFu << (begB & 3);   // columns >= beg
     while (__builtin_amdgcn_ballot_w64(fa || fb)) {
         // a lane done searching re-reads group 0 (its own index may have run
         // past the state)
         const uint32_t z = nz_bits(group_rd<GL>(col, gcol, fa ? ga : 0), group_rd<GL>(col, gcol, fb ? gb : 0));
         const uint32_t za = z & ma, zb = (z >> 16) & mb;
         if (fa) {
-            if (za) { A.beg = min(4 * ga + (int)__builtin_ctz(za), A.end); fa = false; }
-            else { ++ga; fa = 4 * ga < A.end; if (!fa) A.beg = A.end; }
+            if (za) { begA = min(4 * ga + (int)__builtin_ctz(za), endA); fa = false; }
+            else { ++ga; fa = 4 * ga < endA; if (!fa) begA = endA; }
         }
         if (fb) {
-            if (zb) { B.beg = min(4 * gb + (int)__builtin_ctz(zb), B.end); fb = false; }
-            else { ++gb; fb = 4 * gb < B.end; if (!fb) B.beg = B.end; }
+            if (zb) { begB = min(4 * gb + (int)__builtin_ctz(zb), endB); fb = false; }
+            else { ++gb; fb = 4 * gb < endB; if (!fb) begB = endB; }
         }
         ma = mb = 0xFu;
     }
     // backward: search from column end - 1 down to beg when h1 == 0
-    int ka = A.end, kb = B.end;                                 // h1 != 0: k = end
-    bool sa = A.alive && h1a == 0, sb = B.alive && h1b == 0;
+    int ka = endA, kb = endB;                                   // h1 != 0: k = end
+    bool sa = la && (w32(h1) & 0xFFFFu) == 0, sb = lb && (w32(h1) >> 16) == 0;
     ka -= sa;
     kb -= sb;
-    sa = sa && ka >= A.beg;
-    sb = sb && kb >= B.beg;
+    sa = sa && ka >= begA;
+    sb = sb && kb >= begB;
     int gka = max(ka, 0) >> 2, gkb = max(kb, 0) >> 2;
     while (__builtin_amdgcn_ballot_w64(sa || sb)) {
         const uint32_t z = nz_bits(group_rd<GL>(col, gcol, sa ? gka : 0), group_rd<GL>(col, gcol, sb ? gkb : 0));
         // columns of the group in [beg, k]
-        const int la = A.beg - 4 * gka, lb = B.beg - 4 * gkb;
-        const uint32_t ra = (0xFu >> (3 - min(ka - 4 * gka, 3))) & (0xFu << max(min(la, 4), 0));
-        const uint32_t rb = (0xFu >> (3 - min(kb - 4 * gkb, 3))) & (0xFu << max(min(lb, 4), 0));
+        const int xa = begA - 4 * gka, xb = begB - 4 * gkb;
+        const uint32_t ra = (0xFu >> (3 - min(ka - 4 * gka, 3))) & (0xFu << max(min(xa, 4), 0));
+        const uint32_t rb = (0xFu >> (3 - min(kb - 4 * gkb, 3))) & (0xFu << max(min(xb, 4), 0));
         const uint32_t za = z & ra & 0xFu, zb = (z >> 16) & rb & 0xFu;
         if (sa) {
             if (za) { ka = 4 * gka + 31 - (int)__builtin_clz(za); sa = false; }
-            else { --gka; ka = 4 * gka + 3; sa = ka >= A.beg; if (!sa) ka = A.beg - 1; }
+            else { --gka; ka = 4 * gka + 3; sa = ka >= begA; if (!sa) ka = begA - 1; }
         }
         if (sb) {
             if (zb) { kb = 4 * gkb + 31 - (int)__builtin_clz(zb); sb = false; }
-            else { --gkb; kb = 4 * gkb + 3; sb = kb >= B.beg; if (!sb) kb = B.beg - 1; }
+            else { --gkb; kb = 4 * gkb + 3; sb = kb >= begB; if (!sb) kb = begB - 1; }
         }
     }
-    if (A.alive) A.end = min(ka + 2, A.qlen);
-    if (B.alive) B.end = min(kb + 2, B.qlen);
+    const int qa = (int)(w32(R.qlen) & 0xFFFF), qb = (int)(w32(R.qlen) >> 16);
+    const us2 nbeg = us2{(unsigned short)begA, (unsigned short)begB};
+    const us2 nend = us2{(unsigned short)min(ka + 2, qa), (unsigned short)min(kb + 2, qb)};
+    R.beg = bfi16(R.alive, nbeg, R.beg);
+    R.end = bfi16(R.alive, nend, R.end);
 }
 
 }  // namespace
@@ -1138,9 +1272,11 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
     constexpr int LC = HYB ? LB * 16 : 0;                      // ColRef: columns in LDS
     constexpr int GL = LC / 4;                                 // groups in LDS (0: all)
     const int lane = threadIdx.x & 63;
-    PkJob A, B;
-    pk_setup(A, d, p, meta, lane, count);
-    pk_setup(B, d, p, meta, lane + 64, count);
+    PkRow R;
+    pkr_setup(R, d, p, meta, lane, count);
+    const int qlA = (int)(w32(R.qlen) & 0xFFFF), qlB = (int)(w32(R.qlen) >> 16);
+    const int h0A = (int)(w32(R.h0) & 0xFFFF), h0B = (int)(w32(R.h0) >> 16);
+    const bool liveA = (R.alive & 0xFFFFu) != 0, liveB = (R.alive >> 16) != 0;
     // Per 16-base query word, the "differs from base c" bit masks of every
     // c, held in registers for the whole item: N01 = ne0 | ne1 << 16 and
     // N23 = ne2 | ne3 << 16.  A row picks its target base's mask with one
@@ -1148,17 +1284,17 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
     uint32_t NA01[NQ], NA23[NQ], NB01[NQ], NB23[NQ];
 #pragma unroll
     for (int b = 0; b < NQ; ++b) {
-        const uint32_t qa = b * 16 < A.qlen ? d.seq[A.qw0 + b] : 0u;
-        const uint32_t qb = b * 16 < B.qlen ? d.seq[B.qw0 + b] : 0u;
+        const uint32_t qa = b * 16 < qlA ? d.seq[R.qw0[0] + b] : 0u;
+        const uint32_t qb = b * 16 < qlB ? d.seq[R.qw0[1] + b] : 0u;
         NA01[b] = mismatch16(qa, 0u) | (mismatch16(qa, 0x55555555u) << 16);
         NA23[b] = mismatch16(qa, 0xAAAAAAAAu) | (mismatch16(qa, 0xFFFFFFFFu) << 16);
         NB01[b] = mismatch16(qb, 0u) | (mismatch16(qb, 0x55555555u) << 16);
         NB23[b] = mismatch16(qb, 0xAAAAAAAAu) | (mismatch16(qb, 0xFFFFFFFFu) << 16);
     }
-    const int qmax = uniform_max(max(A.alive ? A.qlen : 0, B.alive ? B.qlen : 0));
+    const int qmax = uniform_max(max(liveA ? qlA : 0, liveB ? qlB : 0));
     for (int j = 0; j <= qmax; ++j) {                          // first row, :68-77
-        const int ha = (A.alive && j <= A.qlen) ? (j == 0 ? A.h0 : max(0, A.h0 - p.o_ins - p.e_ins * j)) : 0;
-        const int hb = (B.alive && j <= B.qlen) ? (j == 0 ? B.h0 : max(0, B.h0 - p.o_ins - p.e_ins * j)) : 0;
+        const int ha = (liveA && j <= qlA) ? (j == 0 ? h0A : max(0, h0A - p.o_ins - p.e_ins * j)) : 0;
+        const int hb = (liveB && j <= qlB) ? (j == 0 ? h0B : max(0, h0B - p.o_ins - p.e_ins * j)) : 0;
         const uint32_t v = (uint32_t)ha | ((uint32_t)hb << 16);
         // j is wave-uniform: one LDS or one global store (no generic access)
         if (!HYB || j < LC) ((__attribute__((address_space(3))) uint32_t *)col)[cpos<uint32_t>(j)] = v;
@@ -1175,8 +1311,8 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
     // wave: h0 - o_del - e_del * (i + 1) > 0 needs i < ceil((h0 - o_del) / e_del).
     int h1rows;
     {
-        const int ra = A.alive ? max(0, (A.h0 - p.o_del + p.e_del - 1) / p.e_del) : 0;
-        const int rb = B.alive ? max(0, (B.h0 - p.o_del + p.e_del - 1) / p.e_del) : 0;
+        const int ra = liveA ? max(0, (h0A - p.o_del + p.e_del - 1) / p.e_del) : 0;
+        const int rb = liveB ? max(0, (h0B - p.o_del + p.e_del - 1) / p.e_del) : 0;
         uint32_t x = ~(uint32_t)max(ra, rb), y = ~0u, z = ~0u;
         wave_min3(x, y, z);
         h1rows = (int)~x;
@@ -1188,35 +1324,36 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
             // of block i/16 - 1 (it is first read 16 rows from now)
             const int nb = (i >> 4) + 1;
             if (nb & 1) {
-                A.two = d.seq[A.tw0 + min(nb, A.tnw)];
-                B.two = d.seq[B.tw0 + min(nb, B.tnw)];
+                R.two[0] = d.seq[R.tw0[0] + min((uint32_t)nb, R.tnw[0])];
+                R.two[1] = d.seq[R.tw0[1] + min((uint32_t)nb, R.tnw[1])];
             } else {
-                A.twe = d.seq[A.tw0 + min(nb, A.tnw)];
-                B.twe = d.seq[B.tw0 + min(nb, B.tnw)];
+                R.twe[0] = d.seq[R.tw0[0] + min((uint32_t)nb, R.tnw[0])];
+                R.twe[1] = d.seq[R.tw0[1] + min((uint32_t)nb, R.tnw[1])];
             }
         }
-        pk2_row_start<LC>(A, i, col, gcol, 0);
-        pk2_row_start<LC>(B, i, col, gcol, 1);
-        // wave band: lowest live start, highest and lowest live end; no live
-        // job leaves the first at ~0
-        uint32_t rlo = min(A.alive ? (uint32_t)A.beg : ~0u, B.alive ? (uint32_t)B.beg : ~0u);
-        uint32_t rhi = min(A.alive ? ~(uint32_t)A.end : ~0u, B.alive ? ~(uint32_t)B.end : ~0u);
-        uint32_t rmh = min(A.alive ? (uint32_t)A.end : ~0u, B.alive ? (uint32_t)B.end : ~0u);
+        pkr_row_start<LC>(R, i, col, gcol);
+        // wave band: lowest live start, highest and lowest live end (each
+        // lane's two halves folded first); no live job leaves the first at
+        // 0xFFFF
+        uint32_t rlo = hmin16(w32(R.beg) | ~R.alive);
+        uint32_t rhi = hmin16(~(w32(R.end) & R.alive));
+        uint32_t rmh = hmin16(w32(R.end) | ~R.alive);
         wave_min3(rlo, rhi, rmh);
-        if (rlo == ~0u) break;                                 // every job has ended
-        const int jlo = (int)rlo, jhi = (int)~rhi, mhi = (int)rmh;
+        if ((rlo & 0xFFFFu) == 0xFFFFu) break;                 // every job has ended
+        const int jlo = (int)(rlo & 0xFFFFu), jhi = (int)(~rhi & 0xFFFFu), mhi = (int)(rmh & 0xFFFFu);
         // this row's target base of each job: mask register (N23 for G/T) and shift
-        const bool hiA = (A.tb & 2u) != 0, hiB = (B.tb & 2u) != 0;
-        const uint32_t shA = (A.tb & 1u) * 16, shB = (B.tb & 1u) * 16;
+        const uint32_t tbA = (((i >> 4) & 1 ? R.two[0] : R.twe[0]) >> ((i & 15) * 2)) & 3u;
+        const uint32_t tbB = (((i >> 4) & 1 ? R.two[1] : R.twe[1]) >> ((i & 15) * 2)) & 3u;
+        const bool hiA = (tbA & 2u) != 0, hiB = (tbB & 2u) != 0;
+        const uint32_t shA = (tbA & 1u) * 16, shB = (tbB & 1u) * 16;
         us2 h1 = splat(0);
-        if (i < h1rows) {                                      // :99-107
-            const int ha = (A.alive && A.beg == 0) ? max(0, A.h0 - (p.o_del + p.e_del * (i + 1))) : 0;
-            const int hb = (B.alive && B.beg == 0) ? max(0, B.h0 - (p.o_del + p.e_del * (i + 1))) : 0;
-            h1 = us2{(unsigned short)ha, (unsigned short)hb};
+        if (i < h1rows) {                                      // :99-107, beg == 0 only
+            const us2 t = vsubs(R.h0, splat(min(p.o_del + p.e_del * (i + 1), 65535)));
+            h1 = u2(w32(t) & R.alive & ~nz16(R.beg, ONE));
         }
         us2 f = splat(0), key = splat(0);
         // per-half band end (columns >= it commit nothing); 0 for a finished job
-        const us2 L = us2{(unsigned short)(A.alive ? A.end : 0), (unsigned short)(B.alive ? B.end : 0)};
+        const us2 L = u2(w32(R.end) & R.alive);
 
         // Each row sweeps the 4-column groups from the wave's lowest band start
         // to its highest band end in blocks of 16 columns (the last one 4, 8
@@ -1408,20 +1545,21 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
             }
         }
 
-        const uint32_t kw = w32(key), hw = w32(h1);
-        const int fa = (int)(hw & 0xFFFF), fb = (int)(hw >> 16);
-        pk2_row_end<LC>(A, p, i, fa, (int)(kw & 0xFFFF), col, gcol, 0);
-        pk2_row_end<LC>(B, p, i, fb, (int)(kw >> 16), col, gcol, 1);
-        pk2_narrow<LC>(A, B, fa, fb, col, gcol);
+        pkr_row_end<SYM, LC>(R, p, i, h1, key, ONE, col, gcol);
+        pkr_narrow<LC>(R, h1, col, gcol);
+        if ((i & 255) == 255) {                                // keep each half of nc16 below 65536
+            R.ncells += (w32(R.nc16) & 0xFFFFu) + (w32(R.nc16) >> 16);
+            R.nc16 = splat(0);
+        }
     }
     if (d.cells) {
-        unsigned long long c = (unsigned long long)A.ncells + B.ncells;
+        unsigned long long c = R.ncells + (w32(R.nc16) & 0xFFFFu) + (w32(R.nc16) >> 16);
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
         if (lane == 0) atomicAdd(d.cells, c);
     }
-    pk_finish(A, d);
-    pk_finish(B, d);
+    pkr_finish(R, d, meta, lane, count, 0);
+    pkr_finish(R, d, meta, lane, count, 1);
 }
 
 // Extends jobs meta[0..count) (count <= 64) with one wave, one per lane, for
