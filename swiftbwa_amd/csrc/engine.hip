@@ -159,6 +159,17 @@ void parallel_for(int64_t n, int threads, const std::function<void(int64_t, int6
     for (auto &t : th) t.join();
 }
 
+void parallel_chunks(int chunks, const std::function<void(int)> &fn)
+{
+    if (chunks <= 1) {
+        if (chunks == 1) fn(0);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int c = 0; c < chunks; ++c) th.emplace_back([&fn, c] { fn(c); });
+    for (auto &t : th) t.join();
+}
+
 int host_threads()
 {
     unsigned h = std::thread::hardware_concurrency();

@@ -17,6 +17,25 @@
 
 namespace hipsw {
 
+// std::vector whose growth leaves new elements of trivial types
+// uninitialised, for host arrays whose every element is written before it is
+// read: a resize then costs no fill pass.
+template <class T> struct NoInitAlloc : std::allocator<T> {
+    template <class U> struct rebind {
+        using other = NoInitAlloc<U>;
+    };
+    NoInitAlloc() = default;
+    template <class U> NoInitAlloc(const NoInitAlloc<U> &) noexcept {}
+    template <class U> void construct(U *p) noexcept { ::new ((void *)p) U; }
+    template <class U, class... A> void construct(U *p, A &&...a)
+    {
+        ::new ((void *)p) U(std::forward<A>(a)...);
+    }
+};
+template <class T> using NoFillVec = std::vector<T, NoInitAlloc<T>>;
+
+struct ExtPrep;
+
 // A grow-only device buffer (hipMalloc only when a request outgrows it).
 struct DevBuf {
     void *p = nullptr;
@@ -78,6 +97,11 @@ struct hipsw_engine {
     // NW fill of the 16-bit tier: false = row state in registers where the
     // band fits (hipsw_nw_set_state), true = the LDS ring throughout
     std::atomic<bool> nw_ring{false};
+    // host halves of one-shot extension calls, kept for reuse: a call takes
+    // one, packs and plans into it and gives it back, so repeated calls
+    // neither fault in nor fill fresh host memory (at most kEngineSlots kept)
+    std::mutex prep_mu;
+    std::vector<std::unique_ptr<hipsw::ExtPrep>> prep_pool;
     // autotuner table, read by every planner as a copy
     mutable std::mutex tune_mu;
     hipsw::TuneTable tune;
@@ -120,6 +144,9 @@ constexpr int kEngineSlots = 4;
 
 double now_ms();
 void parallel_for(int64_t n, int threads, const std::function<void(int64_t, int64_t)> &fn);
+// fn(c) for c in [0, chunks), one thread each (in the caller's thread when
+// chunks == 1)
+void parallel_chunks(int chunks, const std::function<void(int)> &fn);
 int host_threads();
 bool scoring_valid(const hipsw_scoring *sc);
 ExtParams make_params(const hipsw_scoring *sc);
@@ -221,12 +248,27 @@ namespace hipsw {
 struct ExtPrep {
     int n = 0;
     ExtParams params;
-    std::vector<uint32_t> seq, nmask, qw, tw, qnw, tnw, order;
-    std::vector<int32_t> ql, tl, h0, w, eb;
+    NoFillVec<uint32_t> seq, qw, tw, qnw, tnw;
+    std::vector<uint32_t> nmask, order;
+    NoFillVec<int32_t> ql, tl, h0, w, eb;
     std::vector<WorkItem> items;
-    std::vector<PkMeta> pk;
+    NoFillVec<PkMeta> pk;
     hipsw_ext_set shape;             // tiers, plans and scratch sizes (no device pointers)
     size_t arena_bytes = 0;          // device bytes of the set's arrays
+};
+// An ExtPrep taken from the engine's pool (a new one when the pool is empty)
+// for the duration of one call; given back on destruction.
+class PrepLease {
+public:
+    explicit PrepLease(hipsw_engine *eng);
+    ~PrepLease();
+    ExtPrep &get() { return *p_; }
+    PrepLease(const PrepLease &) = delete;
+    PrepLease &operator=(const PrepLease &) = delete;
+
+private:
+    hipsw_engine *eng_;
+    std::unique_ptr<ExtPrep> p_;
 };
 // Validates, packs and plans n jobs.  Returns HIPSW_OK or a negative status
 // with the reason in err.

@@ -283,39 +283,62 @@ int ext_prepare(const hipsw_engine *eng, const TuneTable &tune, int n, const uin
             return HIPSW_E_INVALID;
         }
     }
+    // P may come back from an earlier call: every array below is rewritten
+    // in full (the no-fill ones element by element), the shape reset
+    P.shape = hipsw_ext_set();
     hipsw_ext_set &S = P.shape;
     P.n = S.n = n;
     P.params = S.params = make_params(sc);
     const int max_mat = P.params.max_mat;
     const int threads = host_threads();
 
-    // Word layout: every query and target starts on a 16-base word boundary.
-    P.qw.assign(n, 0); P.tw.assign(n, 0); P.qnw.assign(n, 0); P.tnw.assign(n, 0);
+    // Word layout: every query and target starts on a 16-base word boundary,
+    // sequences back to back.  Chunks of jobs sum their words, the chunk
+    // totals are scanned, then each chunk writes its jobs' offsets.
+    P.qw.resize(n); P.tw.resize(n); P.qnw.resize(n); P.tnw.resize(n);
     std::vector<uint8_t> qn_flag(n), tn_flag(n);
-    parallel_for(n, threads, [&](int64_t lo, int64_t hi) {
-        for (int64_t k = lo; k < hi; ++k) {
+    const int nch = (threads > 1 && n >= 4096) ? threads : 1;
+    const int64_t chunk = ((int64_t)n + nch - 1) / nch;
+    std::vector<uint64_t> cw(nch + 1, 0), cn(nch + 1, 0);
+    parallel_chunks(nch, [&](int c) {
+        uint64_t a = 0, b = 0;
+        for (int64_t k = c * chunk, e = std::min<int64_t>(n, (c + 1) * chunk); k < e; ++k) {
             qn_flag[k] = has_n(queries + q_off[k], q_len[k]);
             tn_flag[k] = has_n(targets + t_off[k], t_len[k]);
+            a += (q_len[k] + 15) / 16 + (t_len[k] + 15) / 16;
+            b += (qn_flag[k] ? (q_len[k] + 31) / 32 : 0) + (tn_flag[k] ? (t_len[k] + 31) / 32 : 0);
         }
+        cw[c + 1] = a;
+        cn[c + 1] = b;
     });
-    uint64_t words = 0, nwords = 0;
-    for (int k = 0; k < n; ++k) {
-        P.qw[k] = (uint32_t)words;
-        words += (q_len[k] + 15) / 16;
-        P.tw[k] = (uint32_t)words;
-        words += (t_len[k] + 15) / 16;
-        P.qnw[k] = qn_flag[k] ? (uint32_t)nwords : kNoN;
-        if (qn_flag[k]) nwords += (q_len[k] + 31) / 32;
-        P.tnw[k] = tn_flag[k] ? (uint32_t)nwords : kNoN;
-        if (tn_flag[k]) nwords += (t_len[k] + 31) / 32;
+    for (int c = 0; c < nch; ++c) {
+        cw[c + 1] += cw[c];
+        cn[c + 1] += cn[c];
     }
+    uint64_t words = cw[nch], nwords = cn[nch];
     if (words + 64 > 0xFFFFFFF0ull) {
         err = "job set exceeds 2^32 sequence words";
         return HIPSW_E_UNSUPPORTED;
     }
+    parallel_chunks(nch, [&](int c) {
+        uint64_t a = cw[c], b = cn[c];
+        for (int64_t k = c * chunk, e = std::min<int64_t>(n, (c + 1) * chunk); k < e; ++k) {
+            P.qw[k] = (uint32_t)a;
+            a += (q_len[k] + 15) / 16;
+            P.tw[k] = (uint32_t)a;
+            a += (t_len[k] + 15) / 16;
+            P.qnw[k] = qn_flag[k] ? (uint32_t)b : kNoN;
+            if (qn_flag[k]) b += (q_len[k] + 31) / 32;
+            P.tnw[k] = tn_flag[k] ? (uint32_t)b : kNoN;
+            if (tn_flag[k]) b += (t_len[k] + 31) / 32;
+        }
+    });
     words += 64;    // tail padding: whole-word reads past the last sequence stay in bounds
     nwords += 64;
-    P.seq.assign(words, 0);
+    // every sequence word is written by the packing below; only the padding
+    // is cleared here
+    P.seq.resize(words);
+    std::fill(P.seq.end() - 64, P.seq.end(), 0u);
     P.nmask.assign(nwords, 0);
     P.ql.resize(n); P.tl.resize(n); P.h0.resize(n); P.w.resize(n); P.eb.resize(n);
     std::atomic<bool> bad{false};
@@ -339,13 +362,34 @@ int ext_prepare(const hipsw_engine *eng, const TuneTable &tune, int n, const uin
         return HIPSW_E_INVALID;
     }
 
-    // Exact score tier per job.
+    // Exact score tier per job; jobs holding an N.
+    std::vector<uint8_t> any_n(n);
+    std::vector<int64_t> ctier((size_t)nch * 3, 0);
+    std::vector<int> cq(nch, 0);
+    std::vector<uint8_t> csome(nch, 0);
+    parallel_chunks(nch, [&](int c) {
+        int64_t t3[3] = {0, 0, 0};
+        int q = 0;
+        uint8_t some = 0;
+        for (int64_t k = c * chunk, e = std::min<int64_t>(n, (c + 1) * chunk); k < e; ++k) {
+            t3[tier_of(P.h0[k], P.ql[k], max_mat)]++;
+            q = std::max(q, (int)P.ql[k]);
+            any_n[k] = qn_flag[k] | tn_flag[k];
+            some |= any_n[k];
+        }
+        for (int t = 0; t < 3; ++t) ctier[(size_t)c * 3 + t] = t3[t];
+        cq[c] = q;
+        csome[c] = some;
+    });
     int plain_tier = TIER_U8, qmax = 0;
-    for (int k = 0; k < n; ++k) {
-        const int tr = tier_of(P.h0[k], P.ql[k], max_mat);
-        S.tier_jobs[tr]++;
-        plain_tier = std::max(plain_tier, tr);
-        qmax = std::max(qmax, P.ql[k]);
+    bool some_n = false;
+    for (int c = 0; c < nch; ++c) {
+        for (int t = 0; t < 3; ++t) {
+            S.tier_jobs[t] += ctier[(size_t)c * 3 + t];
+            if (ctier[(size_t)c * 3 + t]) plain_tier = std::max(plain_tier, t);
+        }
+        qmax = std::max(qmax, cq[c]);
+        some_n |= csome[c] != 0;
     }
 
     // Plain form: input order, one tier and one LDS size for the whole launch.
@@ -362,12 +406,6 @@ int ext_prepare(const hipsw_engine *eng, const TuneTable &tune, int n, const uin
     }
 
     // Batched form (see plan_ext_batched).
-    std::vector<uint8_t> any_n(n);
-    bool some_n = false;
-    for (int k = 0; k < n; ++k) {
-        any_n[k] = qn_flag[k] | tn_flag[k];
-        some_n |= any_n[k] != 0;
-    }
     // Input-order form: one launch of the register-query kernel at the set's
     // widest tier and longest query, items in input order.
     if (n > 0 && !some_n && P.params.std_mat && qmax <= 255 && plain_tier != TIER_I32) {
@@ -397,12 +435,33 @@ int ext_prepare(const hipsw_engine *eng, const TuneTable &tune, int n, const uin
     // Work items of any later re-plan (hipsw_ext_autotune) fit: 64 jobs or more per item.
     S.items_cap = (size_t)n / 64 + S.launches.size() + 1;
     P.pk.resize(n);
-    build_pk_meta(n, P.order.data(), P.qw.data(), P.tw.data(), P.qnw.data(), P.tnw.data(),
-                  P.ql.data(), P.tl.data(), P.h0.data(), P.w.data(), P.eb.data(), P.pk.data());
+    parallel_for(n, threads, [&](int64_t lo, int64_t hi) {
+        build_pk_meta((int)(hi - lo), P.order.data() + lo, P.qw.data(), P.tw.data(), P.qnw.data(),
+                      P.tnw.data(), P.ql.data(), P.tl.data(), P.h0.data(), P.w.data(),
+                      P.eb.data(), P.pk.data() + lo);
+    });
     P.arena_bytes = span<uint32_t>(words) + span<uint32_t>(nwords) + 4 * span<uint32_t>(n) +
                     6 * span<int32_t>(n) + span<int32_t>((size_t)n * 6) + span<uint32_t>(n) +
                     span<WorkItem>(S.items_cap) + span<PkMeta>(n);
     return HIPSW_OK;
+}
+
+PrepLease::PrepLease(hipsw_engine *eng) : eng_(eng)
+{
+    {
+        std::lock_guard<std::mutex> lk(eng->prep_mu);
+        if (!eng->prep_pool.empty()) {
+            p_ = std::move(eng->prep_pool.back());
+            eng->prep_pool.pop_back();
+        }
+    }
+    if (!p_) p_.reset(new ExtPrep());
+}
+
+PrepLease::~PrepLease()
+{
+    std::lock_guard<std::mutex> lk(eng_->prep_mu);
+    if (eng_->prep_pool.size() < (size_t)kEngineSlots) eng_->prep_pool.push_back(std::move(p_));
 }
 
 hipError_t ext_bind_upload(const ExtPrep &P, void *dmem, hipsw_ext_set *set, hipStream_t st)
@@ -754,7 +813,8 @@ extern "C" int hipsw_banded_extend(hipsw_engine *eng, int32_t n, const uint8_t *
     if (n < 0 || !out || (mode != HIPSW_MODE_PLAIN && mode != HIPSW_MODE_BATCHED))
         return HIPSW_E_INVALID;
     const double t0 = now_ms();
-    ExtPrep P;
+    PrepLease prep(eng);
+    ExtPrep &P = prep.get();
     std::string err;
     const int rc = ext_prepare(eng, eng->tune_snapshot(), n, queries, q_off, q_len, targets, t_off,
                                t_len, h0, w, end_bonus, sc, P, err);

@@ -70,7 +70,8 @@ extern "C" int hipsw_banded_extend_noalloc(hipsw_engine *eng, int32_t n, const u
         return HIPSW_E_DEVICE;
     }
     const double t0 = now_ms();
-    ExtPrep P;
+    PrepLease prep(eng);
+    ExtPrep &P = prep.get();
     std::string err;
     const int rc = ext_prepare(eng, eng->tune_snapshot(), n, queries, q_off, q_len, targets, t_off,
                                t_len, h0, w, end_bonus, sc, P, err);
