@@ -10,6 +10,7 @@
 #include "engine_internal.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -278,35 +279,57 @@ bool pack_jobs(int n, const uint8_t *queries, const uint32_t *q_off, const uint1
                PackedJobs &pj, std::string &err)
 {
     const int threads = host_threads();
-    pj.qw.assign(n, 0); pj.tw.assign(n, 0); pj.qnw.assign(n, kNoN); pj.tnw.assign(n, kNoN);
-    pj.ql.assign(n, 0); pj.tl.assign(n, 0);
+    pj.qw.resize(n); pj.tw.resize(n); pj.qnw.resize(n); pj.tnw.resize(n);
+    pj.ql.resize(n); pj.tl.resize(n);
     std::vector<uint8_t> qn_flag(n), tn_flag(n);
-    parallel_for(n, threads, [&](int64_t lo, int64_t hi) {
-        for (int64_t k = lo; k < hi; ++k) {
+    // sequences back to back from word boundaries: chunks of jobs sum their
+    // words, the chunk totals are scanned, then each chunk writes its offsets
+    const int nch = (threads > 1 && n >= 4096) ? threads : 1;
+    const int64_t chunk = ((int64_t)n + nch - 1) / nch;
+    std::vector<uint64_t> cw(nch + 1, 0), cn(nch + 1, 0);
+    parallel_chunks(nch, [&](int c) {
+        uint64_t a = 0, b = 0;
+        for (int64_t k = c * chunk, e = std::min<int64_t>(n, (c + 1) * chunk); k < e; ++k) {
             qn_flag[k] = has_n(queries + q_off[k], q_len[k]);
             tn_flag[k] = has_n(targets + t_off[k], t_len[k]);
+            a += (q_len[k] + 15) / 16 + (t_len[k] + 15) / 16;
+            b += (qn_flag[k] ? (q_len[k] + 31) / 32 : 0) + (tn_flag[k] ? (t_len[k] + 31) / 32 : 0);
         }
+        cw[c + 1] = a;
+        cn[c + 1] = b;
     });
-    uint64_t words = 0, nwords = 0;
-    for (int k = 0; k < n; ++k) {
-        pj.ql[k] = q_len[k];
-        pj.tl[k] = t_len[k];
-        pj.qw[k] = (uint32_t)words;
-        words += (q_len[k] + 15) / 16;
-        pj.tw[k] = (uint32_t)words;
-        words += (t_len[k] + 15) / 16;
-        if (qn_flag[k]) { pj.qnw[k] = (uint32_t)nwords; nwords += (q_len[k] + 31) / 32; }
-        if (tn_flag[k]) { pj.tnw[k] = (uint32_t)nwords; nwords += (t_len[k] + 31) / 32; }
+    for (int c = 0; c < nch; ++c) {
+        cw[c + 1] += cw[c];
+        cn[c + 1] += cn[c];
     }
+    uint64_t words = cw[nch], nwords = cn[nch];
     if (words + 64 > 0xFFFFFFF0ull) {
         err = "job set exceeds 2^32 sequence words";
         return false;
     }
+    parallel_chunks(nch, [&](int c) {
+        uint64_t a = cw[c], b = cn[c];
+        for (int64_t k = c * chunk, e = std::min<int64_t>(n, (c + 1) * chunk); k < e; ++k) {
+            pj.ql[k] = q_len[k];
+            pj.tl[k] = t_len[k];
+            pj.qw[k] = (uint32_t)a;
+            a += (q_len[k] + 15) / 16;
+            pj.tw[k] = (uint32_t)a;
+            a += (t_len[k] + 15) / 16;
+            pj.qnw[k] = qn_flag[k] ? (uint32_t)b : kNoN;
+            if (qn_flag[k]) b += (q_len[k] + 31) / 32;
+            pj.tnw[k] = tn_flag[k] ? (uint32_t)b : kNoN;
+            if (tn_flag[k]) b += (t_len[k] + 31) / 32;
+        }
+    });
     words += 64;
     nwords += 64;
-    pj.seq.assign(words, 0);
+    // every sequence word is written by the packing below; only the padding
+    // is cleared here
+    pj.seq.resize(words);
+    std::fill(pj.seq.end() - 64, pj.seq.end(), 0u);
     pj.nmask.assign(nwords, 0);
-    bool bad = false;
+    std::atomic<bool> bad{false};
     parallel_for(n, threads, [&](int64_t lo, int64_t hi) {
         for (int64_t k = lo; k < hi; ++k) {
             if (!pack2(queries + q_off[k], q_len[k], pj.seq.data() + pj.qw[k],

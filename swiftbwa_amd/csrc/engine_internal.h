@@ -155,8 +155,9 @@ bool has_n(const uint8_t *src, int len);
 
 // 2-bit packed query/target arenas of a job set (word-aligned per sequence).
 struct PackedJobs {
-    std::vector<uint32_t> seq, nmask, qw, tw, qnw, tnw;
-    std::vector<int32_t> ql, tl;
+    NoFillVec<uint32_t> seq, qw, tw, qnw, tnw;
+    std::vector<uint32_t> nmask;
+    NoFillVec<int32_t> ql, tl;
 };
 // Packs n jobs; returns false (with a message) on a base code > 4 or an arena
 // beyond 2^32 words.

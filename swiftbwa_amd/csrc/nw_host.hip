@@ -137,8 +137,8 @@ hipError_t nw_bind_upload(NwPrep &P, void *dmem, hipsw_nw_set *set, hipStream_t 
     set->cap = cap;
     set->params = P.params;
     set->max_pen = P.max_pen;
-    set->ql = P.pj.ql;
-    set->tl = P.pj.tl;
+    set->ql.assign(P.pj.ql.begin(), P.pj.ql.end());
+    set->tl.assign(P.pj.tl.begin(), P.pj.tl.end());
     set->w0 = P.w0;
     set->ts = P.ts;
     set->final_w = P.w0;
