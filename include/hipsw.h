@@ -23,6 +23,10 @@
  *    pack, upload, run and copy back concurrently; a call waits only when every
  *    context is leased.  Calls on device-resident sets (hipsw_ext_* / hipsw_nw_*)
  *    and the autotuner run on the engine's own stream under its mutex.
+ *  - hipsw_banded_extend and hipsw_banded_extend_noalloc pack and plan into
+ *    host buffers the engine keeps for reuse (at most four, grow-only, about
+ *    190 bytes per job of the largest call; freed by hipsw_engine_destroy),
+ *    so repeated calls do not fault in fresh host memory.
  *  - hipsw_last_error reports the calling thread's last failure on the engine
  *    (errno-like), so concurrent callers do not see each other's messages.
  */
