@@ -100,6 +100,19 @@ def test_random_small_with_n(engine):
     check(engine, _random_jobs(rng, 3000, 40, 60, n_rate=0.05))
 
 
+def test_one_shot_reuses_host_buffers(engine):
+    """hipsw_banded_extend packs into host buffers the engine keeps between
+    calls and does not clear them: a large set, then a smaller one of other
+    shapes (N bases, empty jobs, longer queries), then the large one again,
+    each equal to the oracle, so no word of an earlier call leaks into a
+    later one."""
+    big = wl.generate(wl.EXT_MIXED, 21, 40000)
+    rng = np.random.default_rng(22)
+    small = _random_jobs(rng, 2500, 200, 260, n_rate=0.03)
+    for js in (big, small, big, small):
+        check(engine, js, modes=[native.MODE_BATCHED])
+
+
 def test_int32_tier_and_negative_h0(engine):
     rng = np.random.default_rng(12)
     js = _random_jobs(rng, 1500, 80, 120, h0_range=(-20, 70000))
