@@ -147,7 +147,8 @@ def autotune(eng, st, max_jobs):
     if eng.lib.hipsw_ext_autotune(eng.handle, st, int(max_jobs), rep, 32, C.byref(nrep)):
         raise RuntimeError(eng.last_error())
     classes = ["0-63", "64-159", "160-255", "256+"]
-    kinds = {1: "packed (2 jobs/lane, int16x2)", 2: "one job/lane (int32)"}
+    kinds = {1: "packed (2 jobs/lane, int16x2)", 2: "one job/lane (int32)",
+             3: "packed (2 jobs/lane, int16x2, 16-bit h/e planes)"}
     out = []
     tiers = {0: "8-bit", 1: "16-bit", 2: "32-bit"}
     for e in rep[:nrep.value]:

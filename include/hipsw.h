@@ -186,15 +186,25 @@ int  hipsw_ext_autotune(hipsw_engine *eng, hipsw_ext_set *set, int32_t max_jobs,
  * plans made afterwards.  HIPSW_E_INVALID for a layout not built. */
 int  hipsw_ext_set_layout(hipsw_engine *eng, int32_t length_class, int32_t kind,
                           int32_t lds_blocks);
-/* The same for the 16-bit tier, whose bins run the one-job-per-lane kernel
- * (no reference counterpart): lds_blocks among the splits built for the class
+/* The same for the 16-bit tier with the one-job-per-lane kernel (no
+ * reference counterpart): lds_blocks among the splits built for the class
  * (class 0: 4; class 1: 10, 6, 5; class 2: 10, 8, 6), 0 for the default (all
  * column state in LDS up to 160 columns).  Applies to plans made afterwards.
  * HIPSW_E_INVALID for a layout not built. */
 int  hipsw_ext_set_layout16(hipsw_engine *eng, int32_t length_class, int32_t lds_blocks);
+/* The 16-bit tier's layout with its kernel kind (no reference counterpart):
+ * kind 0 restores the default (one job per lane), 2 = one job per lane with
+ * lds_blocks as above, 3 = two jobs per lane in 16-bit halves (h and e
+ * planes) with lds_blocks among 4, 2 (class 0) or 5, 3, 2 (classes 1, 2), 0
+ * for its default.  Kind 3 runs a bin only when every job's score bound
+ * h0 + qlen * max(mat) is below 2048 and e_del, e_ins <= 31; other bins keep
+ * the one-job-per-lane kernel.  Applies to plans made afterwards. */
+int  hipsw_ext_set_layout16_kind(hipsw_engine *eng, int32_t length_class, int32_t kind,
+                                 int32_t lds_blocks);
 /* The batched plan of a set, one entry per bin (no reference counterpart):
  * tier (0 8-bit, 1 16-bit, 2 32-bit), query-length class, has_n, jobs, kind
- * (0 one job per lane, generic; 1 packed; 2 one job per lane, register query),
+ * (0 one job per lane, generic; 1 packed; 2 one job per lane, register query;
+ * 3 packed, 16-bit tier),
  * lds_blocks (16-column blocks of column state in LDS, 0 for the generic
  * kind), work items and grid.  Lets a host check which layout a pinned or
  * tuned choice gave each bin. */

@@ -67,12 +67,14 @@ struct ExecCtx {
 };
 
 // The autotuner's layout choice per query-length class: kind[c] 0 = default
-// (packed form) or an ExtKind for the 8-bit tier, and the 16-column blocks of
+// (packed form) or an ExtKind for the 8-bit tier, kind16[c] 0 = default (one
+// job per lane) or an ExtKind for the 16-bit tier, and the 16-column blocks of
 // column state held in LDS for the 8-bit (lb) and 16-bit (lb16) tiers, 0 =
 // default.  Planners read a copy taken under the table's own mutex.
 struct TuneTable {
     int kind[4] = {0, 0, 0, 0};
     int lb[4] = {0, 0, 0, 0};
+    int kind16[4] = {0, 0, 0, 0};
     int lb16[4] = {0, 0, 0, 0};
 };
 
@@ -178,6 +180,8 @@ struct ExtLaunch {
     int job_begin, job_end;   // the bin's range in the batched order
     int cols;
     int lb;               // packed / one-per-lane: 16-column blocks of state in LDS
+    bool w16_ok;          // 16-bit tier: the two-jobs-per-lane form may run the bin
+                          // (every score below 2048, e_del and e_ins <= 31)
     int item_begin, item_count;
     int grid;
     size_t scratch_bytes; // HBM DP state of the launch (0: all in LDS); the
@@ -187,7 +191,7 @@ struct ExtLaunch {
 
 size_t plan_ext_batched(const hipsw_engine *eng, const TuneTable &tune, int n, const int32_t *ql,
                         const int32_t *tl, const int32_t *h0, const int32_t *w,
-                        const uint8_t *has_n, int max_mat, bool std_mat,
+                        const uint8_t *has_n, int max_mat, bool std_mat, int e_max,
                         std::vector<uint32_t> &order, std::vector<WorkItem> &items,
                         std::vector<ExtLaunch> &launches);
 // Work items and grid of a bin for its kind.

@@ -76,7 +76,7 @@ namespace hipsw {
 // global-state scratch bytes the launches need.
 size_t plan_ext_batched(const hipsw_engine *eng, const TuneTable &tune, int n, const int32_t *ql,
                         const int32_t *tl, const int32_t *h0, const int32_t *w,
-                        const uint8_t *has_n, int max_mat, bool std_mat,
+                        const uint8_t *has_n, int max_mat, bool std_mat, int e_max,
                         std::vector<uint32_t> &order, std::vector<WorkItem> &items,
                         std::vector<ExtLaunch> &launches)
 {
@@ -99,8 +99,10 @@ size_t plan_ext_batched(const hipsw_engine *eng, const TuneTable &tune, int n, c
     for (int s = 0; s < n;) {
         const uint32_t bin = keys[s] >> 27;
         int e = s, cmax = 0;
+        int64_t bmax = 0;                                      // the bin's largest score bound
         while (e < n && (keys[e] >> 27) == bin) {
             cmax = std::max(cmax, ql[idx[e]]);
+            bmax = std::max(bmax, (int64_t)h0[idx[e]] + (int64_t)ql[idx[e]] * max_mat);
             ++e;
         }
         ExtLaunch L;
@@ -112,15 +114,16 @@ size_t plan_ext_batched(const hipsw_engine *eng, const TuneTable &tune, int n, c
         const bool fast = !with_n && std_mat && cmax <= 255 && L.tier != TIER_I32 &&
                           ext_pk_lds_bytes(L.cols, ext_pk_default_lb(L.cols)) <=
                               (size_t)eng->lds_per_block;
+        L.w16_ok = fast && L.tier == TIER_U16 && bmax <= 2047 && e_max <= 31;
         L.kind = !fast ? EXT_UNPACKED
-                       : L.tier == TIER_U16 ? EXT_LANE
-                                            : tune.kind[L.cls] > 0 ? tune.kind[L.cls] : EXT_PACKED;
+                 : L.tier == TIER_U16 ? (tune.kind16[L.cls] == EXT_PACKED16 && L.w16_ok ? EXT_PACKED16 : EXT_LANE)
+                                      : tune.kind[L.cls] > 0 ? tune.kind[L.cls] : EXT_PACKED;
         L.lb = 0;
         if (L.kind != EXT_UNPACKED) {
             const int t = L.tier == TIER_U8    ? tune.lb[L.cls]
                           : L.tier == TIER_U16 ? tune.lb16[L.cls]
                                                : 0;
-            L.lb = t > 0 && ext_pk_lb_supported(L.kind, L.cols, t) ? t : ext_pk_default_lb(L.cols);
+            L.lb = t > 0 && ext_pk_lb_supported(L.kind, L.cols, t) ? t : ext_pk_default_lb_k(L.kind, L.cols);
         }
         L.global_state = L.kind == EXT_UNPACKED &&
                          ext_lds_bytes(L.tier, false, L.cols) > (size_t)eng->lds_per_block;
@@ -137,7 +140,7 @@ size_t plan_ext_batched(const hipsw_engine *eng, const TuneTable &tune, int n, c
 static int pk_waves_per_cu(const hipsw_engine *eng, const ExtLaunch &L)
 {
     const size_t per_cu = eng->lds_per_cu > 0 ? (size_t)eng->lds_per_cu : 65536;
-    return (int)std::max<size_t>(1, std::min<size_t>(8, per_cu / ext_pk_lds_bytes(L.cols, L.lb)));
+    return (int)std::max<size_t>(1, std::min<size_t>(8, per_cu / ext_pk_lds_bytes_k(L.kind, L.cols, L.lb)));
 }
 
 size_t ext_layout_launches(const hipsw_engine *eng, std::vector<ExtLaunch> &launches,
@@ -152,7 +155,7 @@ size_t ext_layout_launches(const hipsw_engine *eng, std::vector<ExtLaunch> &laun
         if (L.kind == EXT_UNPACKED)
             L.scratch_bytes = L.global_state ? (size_t)L.grid * L.cols * 64 * ext_state_bytes(L.tier) : 0;
         else
-            L.scratch_bytes = (size_t)L.grid * ext_pk_hbm_bytes_per_wave(L.cols, L.lb);
+            L.scratch_bytes = (size_t)L.grid * ext_pk_hbm_bytes_per_wave_k(L.kind, L.cols, L.lb);
         need = std::max(need, (L.scratch_bytes + 255) & ~(size_t)255);
     }
     return need;
@@ -202,7 +205,8 @@ size_t ext_scratch_bound(const hipsw_engine *eng, int n, const int32_t *ql, cons
             L.global_state = ext_lds_bytes(L.tier, false, L.cols) > (size_t)eng->lds_per_block;
             cand.push_back(L);
         } else {
-            for (int kind : {EXT_PACKED, EXT_LANE}) {
+            for (int kind : {EXT_PACKED, EXT_LANE, EXT_PACKED16}) {
+                if (kind == EXT_PACKED16 && L.tier != TIER_U16) continue;
                 int lbs[5];
                 const int nlb = ext_pk_lb_candidates(kind, L.cols, lbs);
                 for (int c = 0; c < nlb; ++c) {
@@ -225,7 +229,7 @@ size_t ext_scratch_bound(const hipsw_engine *eng, int n, const int32_t *ql, cons
 
 void ext_bin_items(const hipsw_engine *eng, ExtLaunch &L, std::vector<WorkItem> &items)
 {
-    const int per_item = L.kind == EXT_PACKED ? 128 : 64;
+    const int per_item = L.kind == EXT_PACKED || L.kind == EXT_PACKED16 ? 128 : 64;
     L.item_begin = (int)items.size();
     for (int g = L.job_begin; g < L.job_end; g += per_item)
         items.push_back(WorkItem{(uint32_t)g, (uint32_t)std::min(per_item, L.job_end - g)});
@@ -251,6 +255,9 @@ hipError_t launch_ext_bin(const ExtLaunch &L, const ExtDev &d, const ExtParams &
     case EXT_LANE:
         return launch_ext_persistent_ln(d, p, items + L.item_begin, L.item_count, counter, L.cols,
                                         L.lb, L.grid, gs, s);
+    case EXT_PACKED16:
+        return launch_ext_persistent_pk16(d, p, items + L.item_begin, L.item_count, counter, L.cols,
+                                          L.lb, L.grid, gs, s);
     default:
         return launch_ext_persistent(L.tier, L.global_state, d, p, order, items + L.item_begin,
                                      L.item_count, counter, L.cols, L.grid, gs, s);
@@ -431,7 +438,9 @@ int ext_prepare(const hipsw_engine *eng, const TuneTable &tune, int n, const uin
     }
     S.batched_scratch = plan_ext_batched(eng, tune, n, P.ql.data(), P.tl.data(), P.h0.data(),
                                          P.w.data(), any_n.data(), max_mat,
-                                         P.params.std_mat != 0, P.order, P.items, S.launches);
+                                         P.params.std_mat != 0,
+                                         std::max(P.params.e_del, P.params.e_ins), P.order,
+                                         P.items, S.launches);
     // Work items of any later re-plan (hipsw_ext_autotune) fit: 64 jobs or more per item.
     S.items_cap = (size_t)n / 64 + S.launches.size() + 1;
     P.pk.resize(n);
@@ -862,6 +871,27 @@ extern "C" int hipsw_ext_set_layout16(hipsw_engine *eng, int32_t length_class, i
         return HIPSW_E_INVALID;
     }
     std::lock_guard<std::mutex> lk(eng->tune_mu);
+    eng->tune.kind16[length_class] = 0;
+    eng->tune.lb16[length_class] = lds_blocks;
+    return HIPSW_OK;
+}
+
+extern "C" int hipsw_ext_set_layout16_kind(hipsw_engine *eng, int32_t length_class, int32_t kind,
+                                           int32_t lds_blocks)
+{
+    if (!eng || length_class < 0 || length_class > 2) return HIPSW_E_INVALID;
+    if (kind != 0 && kind != EXT_LANE && kind != EXT_PACKED16) {
+        eng->set_error("hipsw_ext_set_layout16_kind: kind must be 0, 2 (one job per lane) or 3 (two jobs per lane)");
+        return HIPSW_E_INVALID;
+    }
+    const int cols = length_class == 0 ? 64 : length_class == 1 ? 160 : 256;   // class's widest
+    const int k = kind == 0 ? EXT_LANE : kind;
+    if (lds_blocks != 0 && !ext_pk_lb_supported(k, cols, lds_blocks)) {
+        eng->set_error("hipsw_ext_set_layout16_kind: layout not built for this length class");
+        return HIPSW_E_INVALID;
+    }
+    std::lock_guard<std::mutex> lk(eng->tune_mu);
+    eng->tune.kind16[length_class] = kind == 0 ? 0 : k;
     eng->tune.lb16[length_class] = lds_blocks;
     return HIPSW_OK;
 }
@@ -895,6 +925,10 @@ extern "C" int hipsw_ext_autotune(hipsw_engine *eng, hipsw_ext_set *set, int32_t
         } else {
             const int nlb = ext_pk_lb_candidates(EXT_LANE, L.cols, lbs);
             for (int c = 0; c < nlb; ++c) cands.push_back(Cand{EXT_LANE, lbs[c]});
+            if (L.w16_ok) {
+                const int n16 = ext_pk_lb_candidates(EXT_PACKED16, L.cols, lbs);
+                for (int c = 0; c < n16; ++c) cands.push_back(Cand{EXT_PACKED16, lbs[c]});
+            }
         }
         ExtLaunch S = L;
         S.job_end = std::min(L.job_end, L.job_begin + max_jobs);
@@ -954,6 +988,7 @@ extern "C" int hipsw_ext_autotune(hipsw_engine *eng, hipsw_ext_set *set, int32_t
             eng->tune.kind[L.cls] = win.kind;
             eng->tune.lb[L.cls] = win.lb;
         } else {
+            eng->tune.kind16[L.cls] = win.kind;
             eng->tune.lb16[L.cls] = win.lb;
         }
     }

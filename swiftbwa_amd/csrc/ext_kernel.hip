@@ -1126,35 +1126,52 @@ __device__ __forceinline__ void pkr_finish(const PkRow &R, const ExtDev &d, cons
     }
 }
 
+// Column state of the packed forms.  8-bit tier: one u32 per (column, lane)
+// holding the bytes (hA, eA, hB, eB), a job's entry the 16-bit half
+// h | e << 8.  16-bit tier (W16): two planes, h words (hA | hB << 16) and e
+// words (eA | eB << 16), the e plane after the h plane in LDS and in the
+// wave's HBM region.  Stores (h, e = 0) into half hs of column j.
+template <int LC, bool W16>
+__device__ __forceinline__ void store_he(uint32_t *col, uint32_t *gcol, uint32_t *colE, uint32_t *gcolE,
+                                         int j, int hs, uint16_t h)
+{
+    store16<LC>(col, gcol, j, hs, h);
+    if (W16) store16<LC>(colE, gcolE, j, hs, 0);
+}
+
 // Row start of both jobs (:92-98; the h1 of :99-107 is made by the caller):
 // band clamp -- the column the w-clamp drops is cleared, which keeps every
 // entry below beg at zero -- band end, cell count.
-template <int LC>
-__device__ __forceinline__ void pkr_row_start(PkRow &R, int i, uint32_t *col, uint32_t *gcol)
+template <int LC, bool W16>
+__device__ __forceinline__ void pkr_row_start(PkRow &R, int i, uint32_t *col, uint32_t *gcol,
+                                              uint32_t *colE, uint32_t *gcolE)
 {
     // beg = i - w (:96) drops column beg: with w >= 0, beg >= i - 1 - w after
     // the previous row, so exactly one column per row
     const us2 nb = vmax(R.beg, vsubs(splat(i), R.w));
     const uint32_t mv = (w32(nb) ^ w32(R.beg)) & R.alive;
-    if (mv & 0xFFFFu) store16<LC>(col, gcol, (int)(w32(R.beg) & 0xFFFF), 0, 0);
-    if (mv >> 16) store16<LC>(col, gcol, (int)(w32(R.beg) >> 16), 1, 0);
+    if (mv & 0xFFFFu) store_he<LC, W16>(col, gcol, colE, gcolE, (int)(w32(R.beg) & 0xFFFF), 0, 0);
+    if (mv >> 16) store_he<LC, W16>(col, gcol, colE, gcolE, (int)(w32(R.beg) >> 16), 1, 0);
     R.beg = bfi16(R.alive, nb, R.beg);
     R.end = vmin(R.end, __builtin_elementwise_add_sat(R.w, splat(i + 1)));   // :97-98 (end <= qlen holds)
     R.nc16 = R.nc16 + u2(w32(vsubs(R.end, R.beg)) & R.alive);
 }
 
 // Row end of both jobs (:145-172): band-end entry, global-score column, row
-// maximum, z-drop, and the last-row test of the next row (:90).  `key` is
-// the row maximum H * 256 + j per half (0 when the row's maximum is 0).
-template <bool SYM, int LC>
-__device__ __forceinline__ void pkr_row_end(PkRow &R, const ExtParams &p, int i, us2 h1, us2 key,
-                                            us2 one, uint32_t *col, uint32_t *gcol)
+// maximum, z-drop, and the last-row test of the next row (:90).  rm / rj:
+// the row maximum per half and its (last) column; rm = 0 when the row's
+// maximum is 0.  Scores are below 2^8 (8-bit tier) or 2^11 (W16).
+template <bool SYM, int LC, bool W16>
+__device__ __forceinline__ void pkr_row_end(PkRow &R, const ExtParams &p, int i, us2 h1, us2 rm,
+                                            us2 rj, us2 one, uint32_t *col, uint32_t *gcol,
+                                            uint32_t *colE, uint32_t *gcolE)
 {
     // h[end] = h1, e[end] = 0; a finished job's half is never read again, so
     // its store needs no guard (end <= qlen keeps it in bounds)
-    store16<LC>(col, gcol, (int)(w32(R.end) & 0xFFFF), 0, (uint16_t)(w32(h1) & 0xFFFF));
-    store16<LC>(col, gcol, (int)(w32(R.end) >> 16), 1, (uint16_t)(w32(h1) >> 16));
-    const us2 rm = key >> splat(8), rj = u2(w32(key) & 0x00FF00FFu);
+    store_he<LC, W16>(col, gcol, colE, gcolE, (int)(w32(R.end) & 0xFFFF), 0, (uint16_t)(w32(h1) & 0xFFFF));
+    store_he<LC, W16>(col, gcol, colE, gcolE, (int)(w32(R.end) >> 16), 1, (uint16_t)(w32(h1) >> 16));
+    // score bound of the tier minus one: every maxv - rm - zdrop <= KCL - 1
+    constexpr int KCL = W16 ? 2047 : 255;
     const us2 si = splat(i);
     // :149-152: at the query end, gtle = i and gscore = h1 unless gscore > h1
     const uint32_t atq = R.alive & ~nz16(u2(w32(R.end) ^ w32(R.qlen)), one);
@@ -1165,12 +1182,13 @@ __device__ __forceinline__ void pkr_row_end(PkRow &R, const ExtParams &p, int i,
     const uint32_t up = R.alive & lt16(R.maxv, rm);
     // :164-172: z-drop against the current maximum.  drop > zdrop is
     // tested as dd * e < maxv - rm - zdrop with dd = |di - dj|; di saturates
-    // at 1023 and dd at 255 and e at 255, which keeps each sign and each
-    // outcome (maxv - rm <= 255), and dd * e within 16 bits.
-    const us2 di = vmin(si - R.maxi, splat(1023));                // i - max_i >= 1
+    // at 2 * KCL + 256 and dd at KCL, which keeps each sign and each outcome
+    // (maxv - rm - zdrop < KCL), and dd * e within 16 bits: e <= 255 in the
+    // 8-bit tier, e <= 31 for W16 (the host's condition for that form).
+    const us2 di = vmin(si - R.maxi, splat(2 * KCL + 256));        // i - max_i >= 1
     const ss2 dd = __builtin_bit_cast(ss2, di) - __builtin_bit_cast(ss2, rj - R.maxj);
     const ss2 nd = ss2{0, 0} - dd;
-    const us2 ad = vmin(__builtin_bit_cast(us2, __builtin_elementwise_max(dd, nd)), splat(255));
+    const us2 ad = vmin(__builtin_bit_cast(us2, __builtin_elementwise_max(dd, nd)), splat(KCL));
     us2 ev;
     if (SYM) {
         ev = splat(min(p.e_del, 255));
@@ -1199,10 +1217,18 @@ __device__ __forceinline__ void pkr_row_end(PkRow &R, const ExtParams &p, int i,
 // end.  Backward: last column in [beg, end] whose (h, e) is non-zero --
 // column end holds (h1, 0), so only a zero h1 searches -- then
 // end = min(that + 2, qlen); none found: beg - 1.
-template <int LC>
-__device__ __forceinline__ void pkr_narrow(PkRow &R, us2 h1, const uint32_t *col, const uint32_t *gcol)
+template <int LC, bool W16>
+__device__ __forceinline__ void pkr_narrow(PkRow &R, us2 h1, const uint32_t *col, const uint32_t *gcol,
+                                           const uint32_t *colE, const uint32_t *gcolE)
 {
     constexpr int GL = LC / 4;
+    // group g of this lane, both planes OR-ed for W16 (a column is non-zero
+    // when its h or its e is)
+    auto grp = [&](int g) {
+        u32x4 v = group_rd<GL>(col, gcol, g);
+        if (W16) v |= group_rd<GL>(colE, gcolE, g);
+        return v;
+    };
     const bool la = (R.alive & 0xFFFFu) != 0, lb = (R.alive >> 16) != 0;
     int begA = (int)(w32(R.beg) & 0xFFFF), begB = (int)(w32(R.beg) >> 16);
     const int endA = (int)(w32(R.end) & 0xFFFF), endB = (int)(w32(R.end) >> 16);
@@ -1212,7 +1238,7 @@ __device__ __forceinline__ void pkr_narrow(PkRow &R, us2 h1, const uint32_t *col
     while (__builtin_amdgcn_ballot_w64(fa || fb)) {
         // a lane done searching re-reads group 0 (its own index may have run
         // past the state)
-        const uint32_t z = nz_bits(group_rd<GL>(col, gcol, fa ? ga : 0), group_rd<GL>(col, gcol, fb ? gb : 0));
+        const uint32_t z = nz_bits(grp(fa ? ga : 0), grp(fb ? gb : 0));
         const uint32_t za = z & ma, zb = (z >> 16) & mb;
         if (fa) {
             if (za) { begA = min(4 * ga + (int)__builtin_ctz(za), endA); fa = false; }
@@ -1233,7 +1259,7 @@ __device__ __forceinline__ void pkr_narrow(PkRow &R, us2 h1, const uint32_t *col
     sb = sb && kb >= begB;
     int gka = max(ka, 0) >> 2, gkb = max(kb, 0) >> 2;
     while (__builtin_amdgcn_ballot_w64(sa || sb)) {
-        const uint32_t z = nz_bits(group_rd<GL>(col, gcol, sa ? gka : 0), group_rd<GL>(col, gcol, sb ? gkb : 0));
+        const uint32_t z = nz_bits(grp(sa ? gka : 0), grp(sb ? gkb : 0));
         // columns of the group in [beg, k]
         const int xa = begA - 4 * gka, xb = begB - 4 * gkb;
         const uint32_t ra = (0xFu >> (3 - min(ka - 4 * gka, 3))) & (0xFu << max(min(xa, 4), 0));
@@ -1264,13 +1290,17 @@ __device__ __forceinline__ void pkr_narrow(PkRow &R, us2 h1, const uint32_t *col
 // e_del == e_ins (one shared gap-open term).  `col` is this lane's word of
 // column 0 (group layout, see cpos) and `gcol` its word of column 16 * LB in
 // the wave's HBM region (used when LB < NQ).
-template <bool SYM, int NQ, int LB>
+template <bool SYM, int NQ, int LB, bool W16>
 __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *meta, int count,
                             uint32_t *col, uint32_t *gcol)
 {
     constexpr bool HYB = LB < NQ;
     constexpr int LC = HYB ? LB * 16 : 0;                      // ColRef: columns in LDS
     constexpr int GL = LC / 4;                                 // groups in LDS (0: all)
+    // W16: the e plane follows the h plane (LDS: after the LDS groups; HBM:
+    // after the wave's h region)
+    uint32_t *colE = W16 ? col + (size_t)(HYB ? LB : NQ) * 1024 : nullptr;
+    uint32_t *gcolE = W16 && HYB ? gcol + ext_hyb_words(NQ, LB) : nullptr;
     const int lane = threadIdx.x & 63;
     PkRow R;
     pkr_setup(R, d, p, meta, lane, count);
@@ -1295,10 +1325,16 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
     for (int j = 0; j <= qmax; ++j) {                          // first row, :68-77
         const int ha = (liveA && j <= qlA) ? (j == 0 ? h0A : max(0, h0A - p.o_ins - p.e_ins * j)) : 0;
         const int hb = (liveB && j <= qlB) ? (j == 0 ? h0B : max(0, h0B - p.o_ins - p.e_ins * j)) : 0;
+        // 8-bit: bytes (hA, eA = 0, hB, eB = 0); W16: the h plane word
         const uint32_t v = (uint32_t)ha | ((uint32_t)hb << 16);
         // j is wave-uniform: one LDS or one global store (no generic access)
-        if (!HYB || j < LC) ((__attribute__((address_space(3))) uint32_t *)col)[cpos<uint32_t>(j)] = v;
-        else ((__attribute__((address_space(1))) uint32_t *)gcol)[cpos<uint32_t>(j - LC)] = v;
+        if (!HYB || j < LC) {
+            ((__attribute__((address_space(3))) uint32_t *)col)[cpos<uint32_t>(j)] = v;
+            if (W16) ((__attribute__((address_space(3))) uint32_t *)colE)[cpos<uint32_t>(j)] = 0u;
+        } else {
+            ((__attribute__((address_space(1))) uint32_t *)gcol)[cpos<uint32_t>(j - LC)] = v;
+            if (W16) ((__attribute__((address_space(1))) uint32_t *)gcolE)[cpos<uint32_t>(j - LC)] = 0u;
+        }
     }
     const ss2 Ssc = ss2{(short)p.match_sc, (short)p.match_sc};
     const ss2 nAB = ss2{(short)(p.mis_sc - p.match_sc), (short)(p.mis_sc - p.match_sc)};
@@ -1307,6 +1343,7 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
     const uint32_t LOW = 0x00FF00FFu;
     const us2 ONE = u2(opaque(0x00010001u));
     const us2 C256 = u2(opaque(0x01000100u));   // H * 256 + k: one packed mad
+    const us2 C16 = u2(opaque(0x00100010u));    // W16: H * 16 + k
     // Rows whose band-start h1 (:101-104) can be non-zero for some job of the
     // wave: h0 - o_del - e_del * (i + 1) > 0 needs i < ceil((h0 - o_del) / e_del).
     int h1rows;
@@ -1331,7 +1368,7 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
                 R.twe[1] = d.seq[R.tw0[1] + min((uint32_t)nb, R.tnw[1])];
             }
         }
-        pkr_row_start<LC>(R, i, col, gcol);
+        pkr_row_start<LC, W16>(R, i, col, gcol, colE, gcolE);
         // wave band: lowest live start, highest and lowest live end (each
         // lane's two halves folded first); no live job leaves the first at
         // 0xFFFF
@@ -1352,6 +1389,7 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
             h1 = u2(w32(t) & R.alive & ~nz16(R.beg, ONE));
         }
         us2 f = splat(0), key = splat(0);
+        uint32_t keyA = 0, keyB = 0;                           // W16: H * 256 + j per job
         // per-half band end (columns >= it commit nothing); 0 for a finished job
         const us2 L = u2(w32(R.end) & R.alive);
 
@@ -1372,7 +1410,8 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
         // max(diag, E) and the new E (:128-129); (2) the running F and h1
         // (:121-123, :130-131); (3) the row-maximum candidates H * 256 + k
         // (:124-125).  MASKED: commit only left of each half's band end.
-        auto block = [&](auto NC, auto MK, auto KD, uint32_t (&wv)[16], int c0, uint32_t mm) {
+        auto block = [&](auto NC, auto MK, auto KD, uint32_t (&wv)[16], uint32_t (&we)[16], int c0,
+                         uint32_t mm) {
             constexpr int N = decltype(NC)::value;
             constexpr bool MASKED = decltype(MK)::value;
             constexpr int KIND = decltype(KD)::value;   // 0 LDS, 2 HBM, 1 either (per group)
@@ -1381,8 +1420,13 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
             for (int k = 0; k < N; ++k) {
                 const uint32_t ab = (mm >> k) & 0x00010001u;          // per-half mismatch flag
                 s[k] = __builtin_bit_cast(us2, __builtin_bit_cast(ss2, ab) * nAB + Ssc);
-                M[k] = u2(wv[k] & LOW);
-                E[k] = u2(__builtin_amdgcn_perm(0u, wv[k], 0x0C030C01u));   // bytes 1, 3
+                if (W16) {
+                    M[k] = u2(wv[k]);
+                    E[k] = u2(we[k]);
+                } else {
+                    M[k] = u2(wv[k] & LOW);
+                    E[k] = u2(__builtin_amdgcn_perm(0u, wv[k], 0x0C030C01u));   // bytes 1, 3
+                }
             }
 #pragma unroll
             for (int k = 0; k < N; ++k) s[k] = s[k] * vmin(M[k], ONE) + M[k];   // M ? M + s : 0
@@ -1405,9 +1449,14 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
                 for (int k = 0; k < N; ++k) {
                     const us2 H = vmax(dgE[k], f);
                     f = smax(__builtin_bit_cast(ss2, tf[k]), vsubs(f, eI));
-                    wv[k] = __builtin_amdgcn_perm(w32(ne[k]), w32(h1), 0x06020400u);
+                    if (W16) {
+                        wv[k] = w32(h1);
+                        we[k] = w32(ne[k]);
+                    } else {
+                        wv[k] = __builtin_amdgcn_perm(w32(ne[k]), w32(h1), 0x06020400u);
+                    }
                     h1 = H;
-                    Hs[k] = H * C256 + splat(k);
+                    Hs[k] = H * (W16 ? C16 : C256) + splat(k);
                 }
             } else {
                 const us2 Lb = vsubs(L, splat(c0));
@@ -1423,34 +1472,69 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
                     const uint32_t hm = hms[k];
                     const us2 H = vmax(dgE[k], f);
                     f = smax(__builtin_bit_cast(ss2, tf[k]), vsubs(f, eI));
-                    const uint32_t nw = __builtin_amdgcn_perm(w32(ne[k]), w32(h1), 0x06020400u);
-                    wv[k] = (nw & hm) | (wv[k] & ~hm);
+                    if (W16) {
+                        wv[k] = (w32(h1) & hm) | (wv[k] & ~hm);
+                        we[k] = (w32(ne[k]) & hm) | (we[k] & ~hm);
+                    } else {
+                        const uint32_t nw = __builtin_amdgcn_perm(w32(ne[k]), w32(h1), 0x06020400u);
+                        wv[k] = (nw & hm) | (wv[k] & ~hm);
+                    }
                     h1 = u2((w32(H) & hm) | (w32(h1) & ~hm));
-                    Hs[k] = u2(w32(H * C256 + splat(k)) & hm);
+                    Hs[k] = u2(w32(H * (W16 ? C16 : C256) + splat(k)) & hm);
                 }
             }
-            if constexpr (KIND == 1) store_groups<GL, N / 4>(col, gcol, c0 >> 2, wv);
-            else store_groups_k<KIND, GL, N / 4>(col, gcol, c0 >> 2, wv);
+            if constexpr (KIND == 1) {
+                store_groups<GL, N / 4>(col, gcol, c0 >> 2, wv);
+                if (W16) store_groups<GL, N / 4>(colE, gcolE, c0 >> 2, we);
+            } else {
+                store_groups_k<KIND, GL, N / 4>(col, gcol, c0 >> 2, wv);
+                if (W16) store_groups_k<KIND, GL, N / 4>(colE, gcolE, c0 >> 2, we);
+            }
 #pragma unroll
             for (int st = 8; st >= 1; st >>= 1)
 #pragma unroll
                 for (int k = 0; k < st; ++k)
                     if (k + st < N) Hs[k] = vmax(Hs[k], Hs[k + st]);
-            key = vmax(key, Hs[0] + splat(c0));
+            if (W16) {
+                // the block's H * 16 + k per job, as H * 256 + (c0 + k)
+                const uint32_t bk = w32(Hs[0]);
+                const uint32_t ka = ((bk & 0xFFF0u) << 4) | (c0 + (bk & 15u));
+                const uint32_t kb = ((bk >> 12) & 0xFFF00u) | (c0 + ((bk >> 16) & 15u));
+                keyA = max(keyA, ka);
+                keyB = max(keyB, kb);
+            } else {
+                key = vmax(key, Hs[0] + splat(c0));
+            }
         };
         using N16 = std::integral_constant<int, 16>;
         using Unmasked = std::integral_constant<bool, false>;
         using Masked = std::integral_constant<bool, true>;
         using KAny = std::integral_constant<int, 1>;
-        auto full = [&](auto KD, uint32_t (&wv)[16], int c0, uint32_t mm) {
-            if (c0 + 16 <= mhi) block(N16{}, Unmasked{}, KD, wv, c0, mm);   // left of every live band end
-            else block(N16{}, Masked{}, KD, wv, c0, mm);
+        // a block's column words: the state words (8-bit) or the h plane
+        // (W16) in h, the e plane (W16 only) in e
+        struct WSet {
+            uint32_t h[16], e[16];       // e unused (and dropped) in the 8-bit tier
+        };
+        auto full = [&](auto KD, WSet &w, int c0, uint32_t mm) {
+            if (c0 + 16 <= mhi) block(N16{}, Unmasked{}, KD, w.h, w.e, c0, mm);   // left of every live band end
+            else block(N16{}, Masked{}, KD, w.h, w.e, c0, mm);
         };
         // partial block of r = 1..3 groups (always at a band end or a memory boundary)
-        auto part = [&](auto KD, int r, uint32_t (&wv)[16], int c0, uint32_t mm) {
-            if (r == 1) block(std::integral_constant<int, 4>{}, Masked{}, KD, wv, c0, mm);
-            else if (r == 2) block(std::integral_constant<int, 8>{}, Masked{}, KD, wv, c0, mm);
-            else block(std::integral_constant<int, 12>{}, Masked{}, KD, wv, c0, mm);
+        auto part = [&](auto KD, int r, WSet &w, int c0, uint32_t mm) {
+            if (r == 1) block(std::integral_constant<int, 4>{}, Masked{}, KD, w.h, w.e, c0, mm);
+            else if (r == 2) block(std::integral_constant<int, 8>{}, Masked{}, KD, w.h, w.e, c0, mm);
+            else block(std::integral_constant<int, 12>{}, Masked{}, KD, w.h, w.e, c0, mm);
+        };
+        // groups g .. g + 3 (clamped to glast) of both planes; KIND as in block
+        auto ld = [&](auto KD, int g, int glast, WSet &w) {
+            constexpr int KIND = decltype(KD)::value;
+            if constexpr (KIND == 1) {
+                load_groups<GL>(col, gcol, g, glast, w.h);
+                if constexpr (W16) load_groups<GL>(colE, gcolE, g, glast, w.e);
+            } else {
+                load_groups_k<KIND, GL>(col, gcol, g, glast, w.h);
+                if constexpr (W16) load_groups_k<KIND, GL>(colE, gcolE, g, glast, w.e);
+            }
         };
         if (jlo < jhi) {
             const int g0 = jlo >> 2, gend = (jhi + 3) >> 2;
@@ -1473,39 +1557,33 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
             // phase repeats its last group: same memory, unused); then the
             // partial block of the remaining 1-3 groups, whose words the last
             // trip has already read.  W0 holds the first block on entry.
-            auto phase = [&](auto KD, int &g, int gstop, uint32_t (&W0)[16], uint32_t (&W1)[16]) {
-                constexpr int KIND = decltype(KD)::value;
+            auto phase = [&](auto KD, int &g, int gstop, WSet &W0, WSet &W1) {
                 const int gl = gstop - 1;
                 const int nf = (gstop - g) >> 2, r = (gstop - g) & 3;
                 bool in_w1 = false;
                 for (int t = 0; t < nf;) {
-                    if constexpr (KIND == 1) load_groups<GL>(col, gcol, min(g + 4, gl), gl, W1);
-                    else load_groups_k<KIND, GL>(col, gcol, min(g + 4, gl), gl, W1);
+                    ld(KD, min(g + 4, gl), gl, W1);
                     full(KD, W0, 4 * g, next_mm());
                     g += 4;
                     if (++t >= nf) {
                         in_w1 = true;
                         break;
                     }
-                    if constexpr (KIND == 1) load_groups<GL>(col, gcol, min(g + 4, gl), gl, W0);
-                    else load_groups_k<KIND, GL>(col, gcol, min(g + 4, gl), gl, W0);
+                    ld(KD, min(g + 4, gl), gl, W0);
                     full(KD, W1, 4 * g, next_mm());
                     g += 4;
                     ++t;
                 }
                 if (r) {
-                    if (in_w1) {
-#pragma unroll
-                        for (int k = 0; k < 16; ++k) W0[k] = W1[k];
-                    }
+                    if (in_w1) W0 = W1;
                     part(KD, r, W0, 4 * g, next_mm());
                     g += r;
                 }
             };
-            uint32_t W0[16], W1[16];
+            WSet W0, W1;
             int g = g0;
             if constexpr (!HYB) {
-                load_groups<GL>(col, gcol, g, gend - 1, W0);
+                ld(KAny{}, g, gend - 1, W0);
                 phase(KAny{}, g, gend, W0, W1);
             } else {
                 // LDS groups [g0, min(GL, gend)) first, then HBM groups
@@ -1513,7 +1591,7 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
                 // into W2 before the LDS phase so its latency hides there.
                 using KL = std::integral_constant<int, 0>;
                 using KH = std::integral_constant<int, 2>;
-                uint32_t W2[16];
+                WSet W2;
                 const int gh = max(g0, GL);
                 // Retire the previous row's global accesses (its last HBM trip
                 // read a block it did not use) so that W2's reads are the only
@@ -1523,10 +1601,10 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
                 // so this wait is normally free.  0x0F70: vmcnt(0), others
                 // unchanged.
                 __builtin_amdgcn_s_waitcnt(0x0F70);
-                if (gend > GL) load_groups_k<2, GL>(col, gcol, gh, gend - 1, W2);
+                if (gend > GL) ld(KH{}, gh, gend - 1, W2);
                 if (g0 < GL) {
                     const int gs = min(GL, gend);
-                    load_groups_k<0, GL>(col, gcol, g, gs - 1, W0);
+                    ld(KL{}, g, gs - 1, W0);
                     phase(KL{}, g, gs, W0, W1);
                 }
                 if (gend > GL) {
@@ -1538,15 +1616,22 @@ __device__ void ext_wave_pk(const ExtDev &d, const ExtParams &p, const PkMeta *m
                         q = GL >> 2;
                         mcur = qmask(q);
                     }
-#pragma unroll
-                    for (int k = 0; k < 16; ++k) W0[k] = W2[k];
+                    W0 = W2;
                     phase(KH{}, g, gend, W0, W1);
                 }
             }
         }
 
-        pkr_row_end<SYM, LC>(R, p, i, h1, key, ONE, col, gcol);
-        pkr_narrow<LC>(R, h1, col, gcol);
+        us2 rm, rj;
+        if (W16) {
+            rm = us2{(unsigned short)(keyA >> 8), (unsigned short)(keyB >> 8)};
+            rj = us2{(unsigned short)(keyA & 0xFFu), (unsigned short)(keyB & 0xFFu)};
+        } else {
+            rm = key >> splat(8);
+            rj = u2(w32(key) & 0x00FF00FFu);
+        }
+        pkr_row_end<SYM, LC, W16>(R, p, i, h1, rm, rj, ONE, col, gcol, colE, gcolE);
+        pkr_narrow<LC, W16>(R, h1, col, gcol, colE, gcolE);
         if ((i & 255) == 255) {                                // keep each half of nc16 below 65536
             R.ncells += (w32(R.nc16) & 0xFFFFu) + (w32(R.nc16) >> 16);
             R.nc16 = splat(0);
@@ -1712,14 +1797,14 @@ __global__ __launch_bounds__(64) void ext_persistent_ln_kernel(ExtDev d, ExtPara
 // (the planner bins N-holding jobs apart).  The block's LDS is exactly the
 // column state (no static shared memory), so a CU holds as many waves as their
 // LDS shares allow (150-bp queries: four held whole, six as 96 + 64 columns).
-template <bool SYM, int NQ, int LB>
+template <bool SYM, int NQ, int LB, bool W16>
 __global__ __launch_bounds__(64) void ext_persistent_pk_kernel(ExtDev d, ExtParams p,
                                                                const WorkItem *items, int n_items,
                                                                uint32_t *counter, uint32_t *gscratch)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     uint32_t *col = reinterpret_cast<uint32_t *>(lds_raw) + threadIdx.x * 4;
-    uint32_t *gcol = gscratch ? gscratch + (size_t)blockIdx.x * ext_hyb_words(NQ, LB) + threadIdx.x * 4
+    uint32_t *gcol = gscratch ? gscratch + (size_t)blockIdx.x * ext_hyb_words(NQ, LB) * (W16 ? 2 : 1) + threadIdx.x * 4
                               : nullptr;
     for (int it = 0;; ++it) {
         int item = 0;
@@ -1731,7 +1816,7 @@ __global__ __launch_bounds__(64) void ext_persistent_pk_kernel(ExtDev d, ExtPara
         }
         if (item >= n_items) break;
         const WorkItem wi = items[item];
-        ext_wave_pk<SYM, NQ, LB>(d, p, d.pk + wi.start, (int)wi.count, col, gcol);
+        ext_wave_pk<SYM, NQ, LB, W16>(d, p, d.pk + wi.start, (int)wi.count, col, gcol);
     }
 }
 
@@ -1757,11 +1842,20 @@ int ext_pk_nq(int cols)
 
 int ext_pk_default_lb(int cols) { return std::min(ext_pk_nq(cols), 10); }
 
+// The 16-bit packed form keeps two planes (8 KiB of LDS per 16-column
+// block), so its default holds at most 5 blocks in LDS.
+int ext_pk_default_lb_k(int kind, int cols)
+{
+    return kind == EXT_PACKED16 ? std::min(ext_pk_nq(cols), 5) : ext_pk_default_lb(cols);
+}
+
 // Instances built: blocks held in LDS per query-word class.
 bool ext_pk_lb_supported(int kind, int cols, int lb)
 {
     if (cols < 1 || cols > 256) return false;
     const int nq = ext_pk_nq(cols);
+    if (kind == EXT_PACKED16)
+        return (nq == 4 && (lb == 4 || lb == 2)) || (nq > 4 && (lb == 5 || lb == 3 || lb == 2));
     if (lb == ext_pk_default_lb(cols)) return true;
     if (kind != EXT_PACKED && kind != EXT_LANE) return false;
     return (nq == 10 && (lb == 5 || lb == 6)) || (nq == 16 && (lb == 6 || lb == 8));
@@ -1769,10 +1863,10 @@ bool ext_pk_lb_supported(int kind, int cols, int lb)
 
 int ext_pk_lb_candidates(int kind, int cols, int *out)
 {
-    static const int cand[] = {10, 8, 6, 5, 4};
+    static const int cand[] = {10, 8, 6, 5, 4, 3, 2};
     int n = 0;
     for (int lb : cand)
-        if (ext_pk_lb_supported(kind, cols, lb)) out[n++] = lb;
+        if (n < 5 && ext_pk_lb_supported(kind, cols, lb)) out[n++] = lb;
     return n;
 }
 
@@ -1786,12 +1880,26 @@ size_t ext_pk_hbm_bytes_per_wave(int cols, int lb)
     return (cols + 15) / 16 > lb ? ext_hyb_words(ext_pk_nq(cols), lb) * 4 : 0;
 }
 
-#define HIPSW_EXT_LAUNCH(KER, S, Q, B)                                                          \
-    hipLaunchKernelGGL((KER<S, Q, B>), dim3(grid), dim3(64), lds, s, d, p, items, n_items,      \
-                       counter, gs)
-#define HIPSW_EXT_SYM(KER, Q, B)                                                                \
+size_t ext_pk_lds_bytes_k(int kind, int cols, int lb)
+{
+    // the 16-bit form's e plane starts a whole h plane (min(NQ, LB) blocks of
+    // 4 KiB) after the h plane, whatever the bin's longest query
+    if (kind == EXT_PACKED16) return (size_t)2 * std::min(ext_pk_nq(cols), lb) * 4096;
+    return ext_pk_lds_bytes(cols, lb);
+}
+
+size_t ext_pk_hbm_bytes_per_wave_k(int kind, int cols, int lb)
+{
+    return ext_pk_hbm_bytes_per_wave(cols, lb) * (kind == EXT_PACKED16 ? 2 : 1);
+}
+
+#define HIPSW_EXT_LAUNCH(KER, S, Q, B, ...)                                                     \
+    hipLaunchKernelGGL((KER<S, Q, B __VA_ARGS__>), dim3(grid), dim3(64), lds, s, d, p, items,   \
+                       n_items, counter, gs)
+#define HIPSW_EXT_SYM(KER, Q, B, ...)                                                           \
     do {                                                                                        \
-        if (sym) HIPSW_EXT_LAUNCH(KER, true, Q, B); else HIPSW_EXT_LAUNCH(KER, false, Q, B);   \
+        if (sym) HIPSW_EXT_LAUNCH(KER, true, Q, B, __VA_ARGS__);                               \
+        else HIPSW_EXT_LAUNCH(KER, false, Q, B, __VA_ARGS__);                                  \
     } while (0)
 
 hipError_t launch_ext_persistent_ln(const ExtDev &d, const ExtParams &p, const WorkItem *items,
@@ -1826,13 +1934,35 @@ hipError_t launch_ext_persistent_pk(const ExtDev &d, const ExtParams &p, const W
     const bool sym = p.o_del == p.o_ins && p.e_del == p.e_ins;
     const int nq = ext_pk_nq(cols);             // query words of the longest query
     const size_t lds = ext_pk_lds_bytes(cols, lb);
-    if (nq == 4) HIPSW_EXT_SYM(ext_persistent_pk_kernel, 4, 4);
-    else if (nq == 10 && lb == 10) HIPSW_EXT_SYM(ext_persistent_pk_kernel, 10, 10);
-    else if (nq == 10 && lb == 6) HIPSW_EXT_SYM(ext_persistent_pk_kernel, 10, 6);
-    else if (nq == 10) HIPSW_EXT_SYM(ext_persistent_pk_kernel, 10, 5);
-    else if (lb == 10) HIPSW_EXT_SYM(ext_persistent_pk_kernel, 16, 10);
-    else if (lb == 8) HIPSW_EXT_SYM(ext_persistent_pk_kernel, 16, 8);
-    else HIPSW_EXT_SYM(ext_persistent_pk_kernel, 16, 6);
+    if (nq == 4) HIPSW_EXT_SYM(ext_persistent_pk_kernel, 4, 4, , false);
+    else if (nq == 10 && lb == 10) HIPSW_EXT_SYM(ext_persistent_pk_kernel, 10, 10, , false);
+    else if (nq == 10 && lb == 6) HIPSW_EXT_SYM(ext_persistent_pk_kernel, 10, 6, , false);
+    else if (nq == 10) HIPSW_EXT_SYM(ext_persistent_pk_kernel, 10, 5, , false);
+    else if (lb == 10) HIPSW_EXT_SYM(ext_persistent_pk_kernel, 16, 10, , false);
+    else if (lb == 8) HIPSW_EXT_SYM(ext_persistent_pk_kernel, 16, 8, , false);
+    else HIPSW_EXT_SYM(ext_persistent_pk_kernel, 16, 6, , false);
+    return hipGetLastError();
+}
+
+hipError_t launch_ext_persistent_pk16(const ExtDev &d, const ExtParams &p, const WorkItem *items,
+                                      int n_items, uint32_t *counter, int cols, int lb, int grid,
+                                      void *gscratch, hipStream_t s)
+{
+    if (n_items <= 0) return hipSuccess;
+    if (!ext_pk_lb_supported(EXT_PACKED16, cols, lb)) return hipErrorInvalidValue;
+    if (ext_pk_hbm_bytes_per_wave_k(EXT_PACKED16, cols, lb) && !gscratch) return hipErrorInvalidValue;
+    uint32_t *gs = static_cast<uint32_t *>(gscratch);
+    const bool sym = p.o_del == p.o_ins && p.e_del == p.e_ins;
+    const int nq = ext_pk_nq(cols);
+    const size_t lds = ext_pk_lds_bytes_k(EXT_PACKED16, cols, lb);
+    if (nq == 4 && lb == 4) HIPSW_EXT_SYM(ext_persistent_pk_kernel, 4, 4, , true);
+    else if (nq == 4) HIPSW_EXT_SYM(ext_persistent_pk_kernel, 4, 2, , true);
+    else if (nq == 10 && lb == 5) HIPSW_EXT_SYM(ext_persistent_pk_kernel, 10, 5, , true);
+    else if (nq == 10 && lb == 3) HIPSW_EXT_SYM(ext_persistent_pk_kernel, 10, 3, , true);
+    else if (nq == 10) HIPSW_EXT_SYM(ext_persistent_pk_kernel, 10, 2, , true);
+    else if (lb == 5) HIPSW_EXT_SYM(ext_persistent_pk_kernel, 16, 5, , true);
+    else if (lb == 3) HIPSW_EXT_SYM(ext_persistent_pk_kernel, 16, 3, , true);
+    else HIPSW_EXT_SYM(ext_persistent_pk_kernel, 16, 2, , true);
     return hipGetLastError();
 }
 #undef HIPSW_EXT_SYM

@@ -80,16 +80,20 @@ enum ExtKind {
     EXT_UNPACKED = 0,     // one job per lane, any matrix / N / tier (ext_persistent_kernel)
     EXT_PACKED = 1,       // two jobs per lane, 8-bit tier, items of 128 jobs
     EXT_LANE = 2,         // one job per lane, 8/16-bit tier, packed-form structure
+    EXT_PACKED16 = 3,     // two jobs per lane, 16-bit tier (scores < 2048), items of 128 jobs
 };
 // Packed and one-per-lane forms: a bin whose longest query needs `cols`
 // columns runs the instance for ext_pk_nq(cols) query words with `lb`
 // 16-column blocks of its state in LDS and the rest in HBM.
 int ext_pk_nq(int cols);
 int ext_pk_default_lb(int cols);
+int ext_pk_default_lb_k(int kind, int cols);
 bool ext_pk_lb_supported(int kind, int cols, int lb);
 int ext_pk_lb_candidates(int kind, int cols, int *out);   // out: up to 5 entries
 size_t ext_pk_lds_bytes(int cols, int lb);                // LDS bytes per wave
 size_t ext_pk_hbm_bytes_per_wave(int cols, int lb);       // HBM state bytes per resident wave
+size_t ext_pk_lds_bytes_k(int kind, int cols, int lb);    // the same for a kind (EXT_PACKED16: two planes)
+size_t ext_pk_hbm_bytes_per_wave_k(int kind, int cols, int lb);
 // Descriptors of the packed form, in `order`.
 void build_pk_meta(int n, const uint32_t *order, const uint32_t *q_word, const uint32_t *t_word,
                    const uint32_t *qn_word, const uint32_t *tn_word, const int32_t *q_len,
@@ -98,6 +102,11 @@ void build_pk_meta(int n, const uint32_t *order, const uint32_t *q_word, const u
 hipError_t launch_ext_persistent_pk(const ExtDev &d, const ExtParams &p, const WorkItem *items,
                                     int n_items, uint32_t *counter, int cols, int lb, int grid,
                                     void *gscratch, hipStream_t s);
+// Two jobs per lane in the 16-bit tier (h and e planes of 16-bit pairs):
+// scores below 2048 and e_del, e_ins <= 31 (checked by the planner).
+hipError_t launch_ext_persistent_pk16(const ExtDev &d, const ExtParams &p, const WorkItem *items,
+                                      int n_items, uint32_t *counter, int cols, int lb, int grid,
+                                      void *gscratch, hipStream_t s);
 // One-job-per-lane form (8- and 16-bit tiers, no N, standard matrix, items of
 // <= 64 jobs, descriptors in d.pk); same LDS size as the packed form.
 hipError_t launch_ext_persistent_ln(const ExtDev &d, const ExtParams &p, const WorkItem *items,

@@ -346,7 +346,8 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
         for (int k = 0; k < r.n_ext; ++k) any_n[k] = qnw[k] != kNoN || tnw[k] != kNoN;
         ext_scratch = plan_ext_batched(eng, eng->tune_snapshot(), r.n_ext, ql.data(), tl.data(),
                                        r.h0.data(), r.w.data(), any_n.data(), ep.max_mat,
-                                       ep.std_mat != 0, ext_order, ext_items, ext_launch);
+                                       ep.std_mat != 0, std::max(ep.e_del, ep.e_ins), ext_order,
+                                       ext_items, ext_launch);
     }
     // local SW split
     std::vector<uint32_t> loc_s, loc_c;

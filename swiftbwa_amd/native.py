@@ -145,6 +145,7 @@ _SIGS = {
     "hipsw_ext_autotune": (C.c_int, [_P, _P, C.c_int32, _P, C.c_int32, _P]),
     "hipsw_ext_set_layout": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32]),
     "hipsw_ext_set_layout16": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "hipsw_ext_set_layout16_kind": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32]),
     "hipsw_nw_upload": (_P, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P, _P, C.c_int32,
                              C.c_int32, C.POINTER(HipswScoring)]),
     "hipsw_nw_run": (C.c_int, [_P, _P, C.c_int32, C.POINTER(HipswStats)]),

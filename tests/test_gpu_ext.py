@@ -207,20 +207,25 @@ def test_autotune_both_layouts_exact(engine):
             es = [e for e in ents if e.tier == tier and e.length_class == c]
             assert sum(e.chosen for e in es) == 1
             for e in es:
-                assert e.jobs > 0 and e.ms > 0 and 4 <= e.lds_blocks <= 10
+                assert e.jobs > 0 and e.ms > 0
+                assert 4 <= e.lds_blocks <= 10 if e.kind != 3 else 2 <= e.lds_blocks <= 5
             if tier == 0:
                 assert es[-1].kind == 2                      # one job per lane runs last
                 assert {e.kind for e in es} == {1, 2}
-            else:                                            # 16-bit tier: one job per lane
-                assert tier == 1 and {e.kind for e in es} == {2}
+            else:                                            # 16-bit tier: both forms (scores < 2048)
+                assert tier == 1 and {e.kind for e in es} == {2, 3}
         # every LDS/HBM split of the 64-159 and 160-255 classes was timed
         t8 = [e for e in ents if e.tier == 0 and e.kind == 1]
         assert {e.lds_blocks for e in t8 if e.length_class == 1} == {10, 6, 5}
         assert {e.lds_blocks for e in t8 if e.length_class == 2} == {10, 8, 6}
-        t16 = [e for e in ents if e.tier == 1]
+        t16 = [e for e in ents if e.tier == 1 and e.kind == 2]
         for c, splits in ((1, {10, 6, 5}), (2, {10, 8, 6})):
             if any(e.length_class == c for e in t16):
                 assert {e.lds_blocks for e in t16 if e.length_class == c} == splits
+        p16 = [e for e in ents if e.tier == 1 and e.kind == 3]
+        for c, splits in ((0, {4, 2}), (1, {5, 3, 2}), (2, {5, 3, 2})):
+            if any(e.length_class == c for e in p16):
+                assert {e.lds_blocks for e in p16 if e.length_class == c} == splits
         assert lib.hipsw_ext_download(engine.handle, st, native.ptr(out)) == 0
         assert np.array_equal(out, ref)                       # lane-layout results
         assert lib.hipsw_ext_run(engine.handle, st, native.MODE_BATCHED, C.byref(s)) == 0
@@ -340,6 +345,67 @@ def test_every_layout16_exact(engine):
             for b in bins:
                 assert b["kind"] == 2 and b["lds_blocks"] == combo[b["length_class"]], (b, combo)
             check(engine, js, modes=(native.MODE_BATCHED,))
+    finally:
+        for cls in range(3):
+            lib.hipsw_ext_set_layout16(engine.handle, cls, 0)
+
+
+def test_every_layout16_packed_exact(engine):
+    """The 16-bit tier's two-jobs-per-lane kernel (h and e planes of 16-bit
+    halves) at every built split, pinned per class through
+    hipsw_ext_set_layout16_kind; bins whose score bound reaches 2048, or a
+    scoring with a gap extension above 31, keep the one-job-per-lane kernel."""
+    rng = np.random.default_rng(24)
+    qs, ts = [], []
+    for ql in (5, 40, 63, 64, 79, 80, 81, 95, 96, 97, 120, 159, 160, 175, 176, 177, 200, 250):
+        for rep in range(30):
+            q = rng.integers(0, 4, ql).astype(np.uint8)
+            t = q.copy()
+            flip = rng.random(ql) < 0.03
+            t[flip] = (t[flip] + 1) % 4
+            qs.append(q)
+            ts.append(np.concatenate([t, rng.integers(0, 4, rep * 4).astype(np.uint8)]))
+    n = len(qs)
+    ql = np.array([len(x) for x in qs], np.uint16); tl = np.array([len(x) for x in ts], np.uint16)
+    qo = np.zeros(n, np.uint32); to = np.zeros(n, np.uint32)
+    qo[1:] = np.cumsum(ql.astype(np.int64))[:-1]; to[1:] = np.cumsum(tl.astype(np.int64))[:-1]
+    # 16-bit tier in every class, bounds h0 + qlen up to 1,999 (below 2,048)
+    h0 = np.maximum(rng.integers(200, 1750, n), 256 - ql.astype(np.int64)).astype(np.int32)
+    js = {"n": n, "queries": np.concatenate(qs), "q_off": qo, "q_len": ql,
+          "targets": np.concatenate(ts), "t_off": to, "t_len": tl,
+          "h0": h0, "w": rng.choice(np.array([10, 50, 100, 300], np.int32), n),
+          "end_bonus": np.full(n, 5, np.int32)}
+    lib = engine.lib
+    assert lib.hipsw_ext_set_layout16_kind(engine.handle, 1, 3, 4) == native.HIPSW_E_INVALID
+    assert lib.hipsw_ext_set_layout16_kind(engine.handle, 0, 3, 5) == native.HIPSW_E_INVALID
+    assert lib.hipsw_ext_set_layout16_kind(engine.handle, 1, 1, 5) == native.HIPSW_E_INVALID
+    try:
+        for combo in ((4, 5, 5), (2, 3, 3), (4, 2, 2), (0, 0, 0)):
+            for cls, lb in enumerate(combo):
+                assert lib.hipsw_ext_set_layout16_kind(engine.handle, cls, 3, lb) == 0
+            bins = [b for b in plan_of(engine, js) if b["tier"] == 1]
+            assert {b["length_class"] for b in bins} == {0, 1, 2}
+            for b in bins:
+                want = combo[b["length_class"]] or (4 if b["length_class"] == 0 else 5)
+                assert b["kind"] == 3 and b["lds_blocks"] == want, (b, combo)
+            check(engine, js, modes=(native.MODE_BATCHED,))
+            # other scorings: a mismatch-heavy matrix and asymmetric gaps
+            check(engine, js, ScoringParameters(matchScore=2, mismatchPenalty=3, zDrop=40),
+                  modes=(native.MODE_BATCHED,))
+            check(engine, js, ScoringParameters(gapOpenPenalty=3, gapExtendPenalty=2,
+                                                gapOpenPenaltyDeletion=8, gapExtendPenaltyDeletion=3,
+                                                zDrop=30), modes=(native.MODE_BATCHED,))
+        # a bound of 2,048 or more in a class keeps that class on one job per lane
+        big = dict(js)
+        big["h0"] = js["h0"].copy()
+        big["h0"][ql.astype(np.int64) >= 160] = 3000
+        for b in [b for b in plan_of(engine, big) if b["tier"] == 1]:
+            assert b["kind"] == (2 if b["length_class"] == 2 else 3), b
+        check(engine, big, modes=(native.MODE_BATCHED,))
+        # so does a gap extension above 31
+        sp_e = ScoringParameters(gapExtendPenalty=40, gapExtendPenaltyDeletion=40)
+        for b in [b for b in plan_of(engine, js, sp_e) if b["tier"] == 1]:
+            assert b["kind"] == 2, b
     finally:
         for cls in range(3):
             lib.hipsw_ext_set_layout16(engine.handle, cls, 0)
