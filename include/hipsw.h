@@ -375,11 +375,15 @@ int  hipsw_nw_download(hipsw_engine *eng, hipsw_nw_set *set, int32_t *out_score,
                        int32_t *out_final_w);
 void hipsw_nw_free(hipsw_engine *eng, hipsw_nw_set *set);
 /* Row-state form of the NW fill's 16-bit tier on this engine (no reference
- * counterpart; results are identical either way): form 0, the default, keeps
- * a lane's row state in registers when its launch's bands are at most 128
- * columns wide; form 1 keeps it in an LDS ring (HBM for long queries)
- * throughout.  Applies to NW runs and serving chunks started afterwards.
- * HIPSW_E_INVALID for another form. */
+ * counterpart; results are identical in every form): form 0, the default,
+ * keeps a lane's row state in registers when its launch's bands are at most
+ * 128 columns wide, and fills two jobs per lane in int16 pairs when the bands
+ * of at most 64 columns, but not those of at most 32, hold all but 1/32 of
+ * those cells; form 1 keeps the row state in an LDS ring (HBM for long
+ * queries) throughout; form 2 keeps it
+ * in registers, one job per lane; form 3 as form 2 but two jobs per lane for
+ * every band of at most 64 columns.  Applies to NW runs and serving chunks
+ * started afterwards.  HIPSW_E_INVALID for another form. */
 int  hipsw_nw_set_state(hipsw_engine *eng, int32_t form);
 
 /* ---- mate-rescue local SW (replaces LocalSWDispatcher.dispatchBatch,

@@ -96,9 +96,12 @@ struct hipsw_engine {
     std::vector<int> free_slots;
     std::mutex slot_mu;
     std::condition_variable slot_cv;
-    // NW fill of the 16-bit tier: false = row state in registers where the
-    // band fits (hipsw_nw_set_state), true = the LDS ring throughout
-    std::atomic<bool> nw_ring{false};
+    // NW fill of the 16-bit tier (hipsw_nw_set_state): 0 = row state in
+    // registers where the band fits, two jobs per lane when the bands of at
+    // most 64 columns, but not those of at most 32, hold all but 1/32 of those
+    // cells; 1 = the LDS ring throughout; 2 = registers, one job per lane;
+    // 3 = registers, two jobs per lane for every band of at most 64 columns
+    std::atomic<int> nw_form{0};
     // host halves of one-shot extension calls, kept for reuse: a call takes
     // one, packs and plans into it and gives it back, so repeated calls
     // neither fault in nor fill fresh host memory (at most kEngineSlots kept)

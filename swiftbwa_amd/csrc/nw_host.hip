@@ -39,7 +39,7 @@ struct hipsw_nw_set {
     std::vector<int32_t> final_w;
     // first-pass plans, per mode (the retry passes are planned on the fly)
     bool plan_ready[2] = {false, false};
-    bool plan_ring[2] = {false, false};   // the engine's NW state form the plan was made for
+    int plan_form[2] = {0, 0};            // the engine's NW state form the plan was made for
     std::vector<uint32_t> plan_order[2];
     std::vector<hipsw::WorkItem> plan_items[2];
     std::vector<hipsw::NwLaunch> plan_launches[2];
@@ -238,8 +238,9 @@ void hipsw::nw_plan_pass(const hipsw_engine *eng, const int32_t *qlv, const int3
     order.clear();
     items.clear();
     launches.clear();
+    const int form = eng->nw_form.load();
     auto add_launch = [&](int tr, int s_max, int c_max, int z_rows, int ib, int ic, int q_max,
-                          size_t z_max) {
+                          size_t z_max, bool pk) {
         NwLaunch L;
         L.qmax = q_max;
         L.zmax = z_max;
@@ -255,11 +256,14 @@ void hipsw::nw_plan_pass(const hipsw_engine *eng, const int32_t *qlv, const int3
         const size_t lds = nw_lds_bytes(tr, false, s_max, L.qw8);
         L.global_state = lds > (size_t)eng->lds_per_block;
         L.reg = 0;
-        const int kreg = tr == NW_I16 && !eng->nw_ring ? nw_reg_cols(c_max) : 0;
-        const int reg_bpc = kreg && nw_reg_lds_bytes(kreg, L.qw8) <= (size_t)eng->lds_per_block
-                                ? nw_reg_blocks_per_cu(kreg, L.qw8) : 0;
+        const int kreg = tr == NW_I16 && form != 1 ? nw_reg_cols(c_max) : 0;
+        int reg_bpc = 0;
+        if (pk && kreg && kreg <= 64 && nw_pk_lds_bytes(kreg, L.qw8) <= (size_t)eng->lds_per_block)
+            reg_bpc = nw_pk_blocks_per_cu(kreg, L.qw8);
+        else if (!pk && kreg && nw_reg_lds_bytes(kreg, L.qw8) <= (size_t)eng->lds_per_block)
+            reg_bpc = nw_reg_blocks_per_cu(kreg, L.qw8);
         if (reg_bpc > 0) {
-            L.reg = kreg;
+            L.reg = pk ? -kreg : kreg;
             L.global_state = false;
             L.grid = std::max(1, std::min(ic, eng->n_cu * std::min(reg_bpc, 32)));
         } else {
@@ -290,7 +294,7 @@ void hipsw::nw_plan_pass(const hipsw_engine *eng, const int32_t *qlv, const int3
             order.push_back(active[a]);
         }
         for (int g = 0; g < m; g += 64) items.push_back(WorkItem{(uint32_t)g, (uint32_t)std::min(64, m - g)});
-        add_launch(tr, smax, cmax, zr, 0, (int)items.size(), qm, zm);
+        add_launch(tr, smax, cmax, zr, 0, (int)items.size(), qm, zm, false);
         return;
     }
     // Batched: bins by (tier, S class); inside a bin the lanes of one wave
@@ -302,29 +306,55 @@ void hipsw::nw_plan_pass(const hipsw_engine *eng, const int32_t *qlv, const int3
     // persistent launch balances them (its items longest first) instead of
     // each class ending on its own tail, and the few wider jobs do not pull
     // the shared launch down to a larger instance's occupancy.
-    const bool reg_form = !eng->nw_ring;
+    // Two jobs per lane (bins 4, 5): form 0 runs the merged bin that way when
+    // it is the 64-column instance (with bands of at most 32 columns only, one
+    // job per lane at 5 waves per SIMD is faster; the wider classes, at most
+    // 1/32 of the cells, go to bins of their own beside it); form 3 always runs the
+    // classes of at most 64 columns that way, merged among themselves, and
+    // merges the wider ones as before (bins 2, 3).  Both persistent launches
+    // then share the GPU, the packed one on the side stream.
+    const bool reg_form = form != 1;
+    bool pk_form = form == 3;
     std::vector<int> rcls(m, -1);
-    int kstar = -1;
+    int kstar = -1, kpk = -1;
     if (reg_form) {
-        int64_t cells[4] = {0, 0, 0, 0}, total = 0;
+        int64_t cells[4] = {0, 0, 0, 0};
         for (int a = 0; a < m; ++a) {
             const int K = tier[a] == NW_I16 ? nw_reg_cols(ncol[a]) : 0;
             if (!K) continue;
             rcls[a] = K / 32 - 1;
-            const int64_t c = (int64_t)tlv[active[a]] * ncol[a];
-            cells[rcls[a]] += c;
-            total += c;
+            cells[rcls[a]] += (int64_t)tlv[active[a]] * ncol[a];
         }
-        int64_t above = total;
-        for (kstar = 0; kstar < 4; ++kstar) {
-            above -= cells[kstar];
-            if (above * 32 <= total) break;
+        // the smallest class of [lo, hi] with at most 1/32 of the group's cells above it
+        auto merge_class = [&](int lo, int hi) {
+            int64_t total = 0;
+            for (int k = lo; k <= hi; ++k) total += cells[k];
+            int64_t above = total;
+            int k = lo;
+            for (; k < hi; ++k) {
+                above -= cells[k];
+                if (above * 32 <= total) break;
+            }
+            return k;
+        };
+        if (pk_form) {
+            kpk = merge_class(0, 1);
+            kstar = merge_class(2, 3);
+        } else {
+            kstar = merge_class(0, 3);
+            if (form == 0 && kstar == 1) {
+                pk_form = true;
+                kpk = kstar;
+                kstar = 2;                                     // classes 2, 3 in bins of their own
+            }
         }
     }
     std::vector<uint32_t> keys(m), idx(m);
     for (int a = 0; a < m; ++a) {
         int sc = s_class(S[a]);
-        if (rcls[a] >= 0) sc = rcls[a] <= kstar ? 0 : rcls[a];           // 0..3
+        if (rcls[a] >= 0 && pk_form && rcls[a] <= 1) sc = rcls[a] <= kpk ? 4 : 5;   // 4, 5
+        else if (rcls[a] >= 0 && pk_form) sc = rcls[a] <= kstar ? 2 : 3;            // 2, 3
+        else if (rcls[a] >= 0) sc = rcls[a] <= kstar ? 0 : rcls[a];                 // 0..3
         else if (reg_form && tier[a] == NW_I16) sc = 3 + s_class(S[a]);  // 6, 7: S > 130
         const uint32_t bin = (uint32_t)(tier[a] * 8 + sc);
         const uint32_t nb = (uint32_t)std::min((ncol[a] + 7) / 8, 4095);
@@ -347,8 +377,10 @@ void hipsw::nw_plan_pass(const hipsw_engine *eng, const int32_t *qlv, const int3
             ++e;
         }
         const int ib = (int)items.size();
-        for (int g = s; g < e; g += 64) items.push_back(WorkItem{(uint32_t)g, (uint32_t)std::min(64, e - g)});
-        add_launch((int)bin / 8, smax, cmax, zr, ib, (int)items.size() - ib, qm, zm);
+        const bool pk = pk_form && bin / 8 == NW_I16 && (bin % 8 == 4 || bin % 8 == 5);
+        const int G = pk ? 128 : 64;
+        for (int g = s; g < e; g += G) items.push_back(WorkItem{(uint32_t)g, (uint32_t)std::min(G, e - g)});
+        add_launch((int)bin / 8, smax, cmax, zr, ib, (int)items.size() - ib, qm, zm, pk);
         // Items longest first (a wave's time is its longest target times its
         // widest band in blocks), so the persistent waves end together.  The
         // wave-per-job form reads the bin's jobs as one contiguous range of
@@ -391,7 +423,7 @@ static int nw_run_ctx(hipsw_engine *eng, ExecCtx &c, hipsw_nw_set *set, int mode
                                              c.stream), "zero"))
         return HIPSW_E_DEVICE;
     for (int pass = 0; !active.empty(); ++pass) {
-        if (pass == 0 && set->plan_ready[mode] && set->plan_ring[mode] == eng->nw_ring.load()) {
+        if (pass == 0 && set->plan_ready[mode] && set->plan_form[mode] == eng->nw_form.load()) {
             order = set->plan_order[mode];
             items = set->plan_items[mode];
             launches = set->plan_launches[mode];
@@ -408,7 +440,7 @@ static int nw_run_ctx(hipsw_engine *eng, ExecCtx &c, hipsw_nw_set *set, int mode
                 set->plan_items[mode] = items;
                 set->plan_launches[mode] = launches;
                 set->plan_ready[mode] = true;
-                set->plan_ring[mode] = eng->nw_ring.load();
+                set->plan_form[mode] = eng->nw_form.load();
                 set->p0_on_device = -1;                        // the device holds an older plan
             }
         }
@@ -418,19 +450,21 @@ static int nw_run_ctx(hipsw_engine *eng, ExecCtx &c, hipsw_nw_set *set, int mode
         }
         // Scratch: backpointers [row][c/8][lane] per resident wave, plus DP
         // state for launches whose state does not fit in LDS.  Launches run in
-        // order on one stream, so they share one region.
-        size_t zbytes = 0, gbytes = 0;
+        // order on their stream: those of one stream share one region; the
+        // side stream's two-jobs-per-lane launches have a region of their own.
+        size_t zbytes = 0, zside = 0, gbytes = 0;
         for (const NwLaunch &L : launches) {
             if (L.wide) {
                 if (L.zmax > 49152) zbytes = std::max(zbytes, (size_t)L.grid * L.zmax);
                 continue;
             }
-            const size_t ncw = (size_t)(L.ncol_max + 7) / 8;
-            zbytes = std::max(zbytes, (size_t)L.grid * L.zrows * ncw * 64 * 4);
+            if (nw_side_launch(L, launches)) zside = std::max(zside, nw_launch_zbytes(L));
+            else zbytes = std::max(zbytes, nw_launch_zbytes(L));
             if (L.global_state)
                 gbytes = std::max(gbytes, (size_t)L.grid * L.S * 64 * (L.tier == NW_I16 ? 4 : 8));
         }
-        const size_t need = zbytes + ((gbytes + 255) & ~(size_t)255) + 256;
+        zbytes = (zbytes + 255) & ~(size_t)255;
+        const size_t need = zbytes + zside + ((gbytes + 255) & ~(size_t)255) + 256;
         void *nwbuf = eng->inject(HIPSW_FAULT_ALLOC) ? nullptr : c.nw.get(need);
         if (!nwbuf) {
             (void)hipStreamSynchronize(c.stream);
@@ -438,7 +472,8 @@ static int nw_run_ctx(hipsw_engine *eng, ExecCtx &c, hipsw_nw_set *set, int mode
             return HIPSW_E_DEVICE;
         }
         uint32_t *zbase = reinterpret_cast<uint32_t *>(nwbuf);
-        void *gstate = reinterpret_cast<char *>(nwbuf) + ((zbytes + 255) & ~(size_t)255);
+        uint32_t *zbase_side = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(nwbuf) + zbytes);
+        void *gstate = reinterpret_cast<char *>(nwbuf) + zbytes + ((zside + 255) & ~(size_t)255);
         // The first pass reads d_w (w0), d_order and d_items, uploaded once
         // per plan; a retry pass reads d_w_r (its bands, doubled on the
         // device) and its own small order and items.
@@ -476,11 +511,11 @@ static int nw_run_ctx(hipsw_engine *eng, ExecCtx &c, hipsw_nw_set *set, int mode
         // touch no shared scratch: they run on the side stream, queued first
         // (their waves start before the lane-per-job launch fills the GPU and
         // end inside it; queued after it, they wait for its first waves to
-        // retire and run past its end).
+        // retire and run past its end).  The two-jobs-per-lane bins go there
+        // too, with their own backpointer region: the two persistent launches
+        // then fill each other's tails.
         bool forked = false;
-        auto side_ok = [&](const NwLaunch &L) {
-            return L.wide && L.zmax <= 49152 && launches.size() > 1;
-        };
+        auto side_ok = [&](const NwLaunch &L) { return nw_side_launch(L, launches); };
         auto run_wide = [&](const NwLaunch &L, int k, hipStream_t ls) {
             // the bin's jobs are order[items[ib].start ..) contiguously
             const WorkItem &first = items[L.item_begin];
@@ -499,7 +534,15 @@ static int nw_run_ctx(hipsw_engine *eng, ExecCtx &c, hipsw_nw_set *set, int mode
                      eng->check(hipStreamWaitEvent(c.side, c.evf, 0), "fork");
                 forked = true;
             }
-            if (ok) ok = run_wide(launches[li], (int)li, c.side);
+            if (ok && launches[li].wide) ok = run_wide(launches[li], (int)li, c.side);
+            else if (ok) ok = eng->check(launch_nw(launches[li].tier, false, launches[li].reg, dev,
+                                                   set->params, d_ord, d_it + launches[li].item_begin,
+                                                   launches[li].item_count, c.counters + li, 0,
+                                                   zbase_side, launches[li].zrows,
+                                                   (launches[li].ncol_max + 7) / 8, launches[li].grid,
+                                                   pass == 0 && mode == HIPSW_MODE_BATCHED, nullptr,
+                                                   launches[li].qw8, c.side),
+                                         "launch nw (side)");
             ++launches_total;
         }
         int k = -1;
@@ -698,7 +741,7 @@ extern "C" int hipsw_global_nw(hipsw_engine *eng, int32_t n, const uint8_t *quer
 
 extern "C" int hipsw_nw_set_state(hipsw_engine *eng, int32_t form)
 {
-    if (!eng || form < 0 || form > 1) return HIPSW_E_INVALID;
-    eng->nw_ring.store(form == 1);
+    if (!eng || form < 0 || form > 3) return HIPSW_E_INVALID;
+    eng->nw_form.store(form);
     return HIPSW_OK;
 }

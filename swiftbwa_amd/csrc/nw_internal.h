@@ -47,7 +47,11 @@ size_t nw_lds_bytes(int tier, bool global_state, int S, int qw8);
 int nw_reg_cols(int ncol_max);
 size_t nw_reg_lds_bytes(int K, int qw8);
 int nw_reg_blocks_per_cu(int K, int qw8);
-// reg != 0 runs the register-state form with K = reg (16-bit tier, reg >= 8 * ncw).
+// Two jobs per lane (K = 32, 64, 96): LDS bytes and resident blocks per CU.
+size_t nw_pk_lds_bytes(int K, int qw8);
+int nw_pk_blocks_per_cu(int K, int qw8);
+// reg > 0 runs the register-state form with K = reg, reg < 0 its two-jobs-per-lane
+// form with K = -reg (16-bit tier, |reg| >= 8 * ncw; items of up to 128 jobs).
 hipError_t launch_nw(int tier, bool global_state, int reg, const NwDev &d, const NwParams &p,
                      const uint32_t *order, const WorkItem *items, int n_items,
                      uint32_t *counter, int S, uint32_t *zbase, int zrows, int ncw, int grid,
@@ -78,6 +82,23 @@ struct NwLaunch {
     int qmax;    // longest query of the bin
     size_t zmax; // wave-per-job form: largest tlen * ncol of the bin
 };
+
+// Backpointer bytes of a lane-per-job launch: one word per row, 8-column
+// block and job slot (64 per wave, 128 with two jobs per lane) per resident wave.
+inline size_t nw_launch_zbytes(const NwLaunch &L)
+{
+    const size_t ncw = (size_t)(L.ncol_max + 7) / 8;
+    return (size_t)L.grid * L.zrows * ncw * (L.reg < 0 ? 128 : 64) * 4;
+}
+
+// Launches of a pass queued on the side stream, ahead of the main stream's:
+// wave-per-job bins with their flags in LDS, and two-jobs-per-lane bins
+// (backpointers in a region of their own), when the pass has other launches.
+inline bool nw_side_launch(const NwLaunch &L, const std::vector<NwLaunch> &all)
+{
+    if (all.size() < 2) return false;
+    return (L.wide && L.zmax <= 49152) || (!L.wide && L.reg < 0);
+}
 
 }  // namespace hipsw
 

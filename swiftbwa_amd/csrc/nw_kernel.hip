@@ -21,6 +21,8 @@
 #include "hipsw_internal.h"
 #include "nw_internal.h"
 
+#include <type_traits>
+
 namespace hipsw {
 
 template <int T> struct NwSt;
@@ -648,6 +650,327 @@ void nw_reg_kernel(NwDev d, NwParams p, const uint32_t *order,
 }
 
 // ---------------------------------------------------------------------------
+// Two jobs per lane: the register-state form with int16 pair arithmetic.
+//
+// Lane l fills jobs idx[l] (half 0 of every pair) and idx[l + 64] (half 1).
+// H[k] and E[k] hold column beg + k of both jobs' current rows as int16 pairs
+// (the frame of nw_wave_reg, moved per half), so every step of the cell --
+// the diagonal add, the E and F updates and the maxima -- is one
+// v_pk_*_i16 for both jobs.  The 16-bit tier's bound keeps every value in
+// [-16384 - 8000, 8000], so the pair differences below do not wrap.
+//
+// Scores: per job and 8-cell block, four byte-permute lookups of the target
+// base's matrix row put the scores of cells (0, 4), (1, 5), (2, 6), (3, 7) in
+// bytes 1 and 3; one more permute per cell joins both jobs' scores as a
+// sign-extended int16 pair (selectors 8-11 replicate the sign of bytes 1, 3,
+// 5, 7).
+//
+// Flags: the four flag bits of the cell (those of nw_traceback) are the sign
+// bits of four pair differences; a permute turns the signs of two of them into
+// byte masks (flag a of job 0, flag b of job 0, flag a of job 1, flag b of
+// job 1), and bit c of each byte is kept for cell c.
+// Per row and block each job stores one word: byte f holds flag f of its 8
+// cells.  The words of a wave are [row][block][128 job slots].
+typedef short nws2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ nws2 nw_s2(uint32_t v) { return __builtin_bit_cast(nws2, v); }
+__device__ __forceinline__ uint32_t nw_u(nws2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ uint32_t nw_pair(int a, int b)
+{
+    return ((uint32_t)a & 0xFFFFu) | ((uint32_t)b << 16);
+}
+__device__ __forceinline__ uint32_t nw_sel(uint32_t m, uint32_t x, uint32_t y) { return y ^ ((x ^ y) & m); }
+
+// 4 flag bits of cell c (0..7) of a packed block word, in nw_traceback's order
+__device__ __forceinline__ uint32_t nw_pk_flags(uint32_t z, int c)
+{
+    return ((((z >> c) & 0x01010101u) * 0x01020408u) >> 24) & 0xFu;
+}
+
+template <int K>
+__device__ void nw_wave_pk(const NwDev &d, const NwParams &p, const uint32_t *idx, int count,
+                           uint32_t *zreg, int zrows, int ncw, const uint32_t *rowtab,
+                           uint32_t *qlds, int qw8)
+{
+    constexpr int NEG = NwSt<NW_I16>::kNeg;
+    const int lane = threadIdx.x & 63;
+    uint32_t *ql = qlds + lane;
+    int qlen[2], tlen[2], wj[2];
+    bool ne[2];
+    uint32_t tw0[2], tn0[2];
+    int cm = 0, wm = 0, tm = 0;
+#pragma unroll
+    for (int J = 0; J < 2; ++J) {
+        const int slot = lane + 64 * J;
+        const bool valid = slot < count;
+        const int job = valid ? (int)idx[slot] : 0;
+        qlen[J] = valid ? d.q_len[job] : 0;
+        tlen[J] = valid ? d.t_len[job] : 0;
+        ne[J] = valid && qlen[J] > 0 && tlen[J] > 0;           // :29-31
+        int w = valid ? min(d.w[job], 1 << 20) : 0;
+        w = max(w, abs(qlen[J] - tlen[J]));                    // :47
+        wj[J] = w;
+        const int ncol = min(qlen[J], 2 * w + 1);              // :48
+        tw0[J] = valid ? d.t_word[job] : 0;
+        tn0[J] = valid ? d.tn_word[job] : kNoN;
+        const uint32_t qw0 = valid ? d.q_word[job] : 0;
+        const uint32_t qn0 = valid ? d.qn_word[job] : kNoN;
+        if (ne[J]) {
+            cm = max(cm, ncol);
+            wm = max(wm, w);
+            tm = max(tm, tlen[J]);
+        }
+        // query J at words J * qw8 .. J * qw8 + qw8 - 1, 4-bit codes
+        for (int b = 0; b < qw8; ++b) {
+            uint32_t x = 0;
+            if (ne[J] && b * 8 < qlen[J]) {
+                x = (d.seq[qw0 + (b >> 1)] >> ((b & 1) * 16)) & 0xFFFFu;
+                x = (x | (x << 8)) & 0x00FF00FFu;
+                x = (x | (x << 4)) & 0x0F0F0F0Fu;
+                x = (x | (x << 2)) & 0x33333333u;
+                if (qn0 != kNoN) {
+                    const uint32_t nb = (d.nmask[qn0 + (b >> 2)] >> ((b & 3) * 8)) & 0xFFu;
+                    for (int k = 0; k < 8; ++k)
+                        if ((nb >> k) & 1u) x = (x & ~(0xFu << (4 * k))) | (4u << (4 * k));
+                }
+            }
+            ql[(J * qw8 + b) * 64] = x;
+        }
+    }
+    // A row reads words beg/8 .. beg/8 + ceil(cmax/8) of its job's area, at
+    // most J * qw8 + qw8 - 1 + K/8; cells those words feed beyond the band are
+    // not kept, so job 0's reads may run into job 1's words.
+    for (int b = 2 * qw8; b < 2 * qw8 + K / 8; ++b) ql[b * 64] = 0;
+    const int cmax = wave_max_nw(cm);
+    const int wmax = wave_max_nw(wm);
+    const int tmax = wave_max_nw(tm);
+
+    // First row (:75-84) in the frame of beg = 0: H[s] is column s.
+    uint32_t H[K + 2], E[K + 2];
+    {
+        const int ie0 = min(wj[0], qlen[0]), ie1 = min(wj[1], qlen[1]);
+#pragma unroll
+        for (int s = 0; s < K + 2; ++s) {
+            const int h0 = s == 0 ? 0 : (s <= ie0 ? -(p.o_ins + p.e_ins * s) : NEG);
+            const int h1 = s == 0 ? 0 : (s <= ie1 ? -(p.o_ins + p.e_ins * s) : NEG);
+            H[s] = nw_pair(h0, h1);
+            E[s] = nw_pair(NEG, NEG);
+        }
+    }
+    // score = h[qlen] (:144): column qlen of the last row is H[qlen - beg_last]
+    const int sidx0 = qlen[0] - max(0, tlen[0] - 1 - wj[0]);
+    const int sidx1 = qlen[1] - max(0, tlen[1] - 1 - wj[1]);
+    uint32_t score = 0;
+    const nws2 OED = {(short)p.oe_del, (short)p.oe_del}, ED = {(short)p.e_del, (short)p.e_del};
+    const nws2 OEI = {(short)p.oe_ins, (short)p.oe_ins}, EI = {(short)p.e_ins, (short)p.e_ins};
+    uint32_t *zl = zreg + lane;                  // word (i, cw, J) at zl[(i*ncw+cw)*128 + 64 J]
+    uint32_t tw[2] = {0, 0}, tn[2] = {0, 0};
+    uint32_t ncells = 0;
+    for (int i = 0; i < tmax; ++i) {                           // :89
+        uint32_t tabl[2], tabh[2], qlo[2], qsh[2];
+        const uint32_t *qrow[2];
+        int width[2], hl[2];
+        bool act[2], adv[2];
+#pragma unroll
+        for (int J = 0; J < 2; ++J) {
+            act[J] = ne[J] && i < tlen[J];
+            if ((i & 15) == 0 && act[J]) tw[J] = d.seq[tw0[J] + (i >> 4)];
+            int t = (tw[J] >> ((i & 15) * 2)) & 3;
+            if (tn0[J] != kNoN) {
+                if ((i & 31) == 0 && act[J]) tn[J] = d.nmask[tn0[J] + (i >> 5)];
+                if ((tn[J] >> (i & 31)) & 1) t = 4;
+            }
+            tabl[J] = rowtab[2 * t];
+            tabh[J] = rowtab[2 * t + 1];
+            const int beg = max(0, i - wj[J]);                 // :90-91
+            const int end = min(qlen[J], i + wj[J] + 1);
+            width[J] = act[J] ? end - beg : 0;
+            ncells += (uint32_t)width[J];
+            adv[J] = act[J] && i > wj[J];
+            hl[J] = beg == 0 ? 0 - (p.oe_del + p.e_del * i) : NEG;   // :96-104
+            const int qb = act[J] ? beg : 0;
+            qrow[J] = ql + (J * qw8 + (qb >> 3)) * 64;
+            qsh[J] = (uint32_t)(qb & 7) * 4;
+            qlo[J] = qrow[J][0];
+        }
+        // blocks: those holding every active band and the cell after it
+        uint32_t wlo = min(act[0] ? (uint32_t)width[0] : 0xFFFFFFFFu,
+                           act[1] ? (uint32_t)width[1] : 0xFFFFFFFFu);
+        uint32_t whi = min(act[0] ? ~(uint32_t)width[0] : 0xFFFFFFFFu,
+                           act[1] ? ~(uint32_t)width[1] : 0xFFFFFFFFu);
+        wave_min2_u32(wlo, whi);
+        const int wmin = (int)wlo, wmaxr = (int)~whi;
+        const int nblk = min((wmaxr + 8) >> 3, (cmax + 7) >> 3);
+        const int nfull = i > wmax ? min(wmin, cmax) >> 3 : 0;
+        const uint32_t advM = (adv[0] ? 0xFFFFu : 0u) | (adv[1] ? 0xFFFF0000u : 0u);
+        const uint32_t actM = (act[0] ? 0xFFFFu : 0u) | (act[1] ? 0xFFFF0000u : 0u);
+        const uint32_t wpair = nw_pair(width[0], width[1]);
+        uint32_t h_diag = nw_sel(advM, H[1], H[0]);
+        uint32_t h_left = nw_pair(hl[0], hl[1]);
+        H[0] = nw_sel(actM, h_left, H[0]);                     // column beg's e is never read again
+        uint32_t f = nw_pair(NEG, NEG);
+#pragma unroll
+        for (int cb = 0; cb < K / 8; ++cb) {                   // :108-141
+            if (cb < nblk) {                                   // a guard: see nw_wave_reg
+                uint32_t lk[2][4];
+#pragma unroll
+                for (int J = 0; J < 2; ++J) {
+                    const uint32_t qhi = qrow[J][(cb + 1) * 64];
+                    const uint32_t codes = __builtin_amdgcn_alignbit(qhi, qlo[J], qsh[J]);
+                    qlo[J] = qhi;
+                    const uint32_t xe = codes & 0x0F0F0F0Fu;           // cells 0, 2, 4, 6
+                    const uint32_t xo = (codes >> 4) & 0x0F0F0F0Fu;    // cells 1, 3, 5, 7
+                    lk[J][0] = __builtin_amdgcn_perm(tabh[J], tabl[J], xe << 8);   // bytes 1, 3: cells 0, 4
+                    lk[J][1] = __builtin_amdgcn_perm(tabh[J], tabl[J], xo << 8);   // 1, 5
+                    lk[J][2] = __builtin_amdgcn_perm(tabh[J], tabl[J], xe);        // 2, 6
+                    lk[J][3] = __builtin_amdgcn_perm(tabh[J], tabl[J], xo);        // 3, 7
+                }
+                uint32_t accP = 0, accQ = 0;
+                auto cell = [&](auto full_t, int k) {
+                    constexpr bool full = decltype(full_t)::value;
+                    const int kk = cb * 8 + k;
+                    const nws2 sc = nw_s2(__builtin_amdgcn_perm(lk[1][k & 3], lk[0][k & 3],
+                                                                k < 4 ? 0x0A050801u : 0x0B070903u));
+                    const uint32_t vh = full ? H[kk + 2] : nw_sel(advM, H[kk + 2], H[kk + 1]);
+                    const uint32_t ve = full ? E[kk + 2] : nw_sel(advM, E[kk + 2], E[kk + 1]);
+                    const nws2 hd = nw_s2(vh);
+                    const nws2 h_cur = nw_s2(h_diag) + sc;                     // :110
+                    const nws2 e_open = hd - OED, e_ext = nw_s2(ve) - ED;      // :116-120
+                    const nws2 ev = __builtin_elementwise_max(e_open, e_ext);
+                    const nws2 f_open = nw_s2(h_left) - OEI, f_ext = nw_s2(f) - EI;   // :121-127
+                    const nws2 fv = __builtin_elementwise_max(f_open, f_ext);
+                    const nws2 h1 = __builtin_elementwise_max(ev, h_cur);      // :129-134
+                    const nws2 hn = __builtin_elementwise_max(fv, h1);
+                    // flags (E opened, F opened, H from E, H from F): byte masks
+                    // from the signs of the differences, [f0 or f2, f1 or f3] of
+                    // job 0 then of job 1, one bit per cell
+                    const uint32_t P = __builtin_amdgcn_perm(nw_u(f_ext - f_open), nw_u(e_ext - e_open), 0x0B090A08u);
+                    const uint32_t Q = __builtin_amdgcn_perm(nw_u(h1 - fv), nw_u(h_cur - ev), 0x0B090A08u);
+                    const uint32_t bit = 0x01010101u << k;
+                    if constexpr (full) {
+                        accP |= P & bit;
+                        accQ |= Q & bit;
+                        H[kk + 1] = nw_u(hn);
+                        E[kk + 1] = nw_u(ev);
+                    } else {
+                        // halves whose band ends before this cell keep what they read
+                        const uint32_t inM = nw_u((nws2{(short)kk, (short)kk} - nw_s2(wpair)) >> nws2{15, 15});
+                        const uint32_t m = inM & bit;
+                        accP |= P & m;
+                        accQ |= Q & m;
+                        H[kk + 1] = nw_sel(inM, nw_u(hn), vh);
+                        E[kk + 1] = nw_sel(inM, nw_u(ev), ve);
+                    }
+                    // pinned where computed (see nw_wave_reg)
+                    asm volatile("" : "+v"(H[kk + 1]));
+                    asm volatile("" : "+v"(E[kk + 1]));
+                    h_diag = vh;
+                    // the next cell's left neighbour; in a half whose band has
+                    // ended every later cell of the row is discarded as well
+                    h_left = H[kk + 1];
+                    f = nw_u(fv);
+                };
+                if (cb < nfull) {
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) cell(std::true_type{}, k);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) cell(std::false_type{}, k);
+                }
+                uint32_t *zw = zl + ((size_t)i * ncw + cb) * 128;      // i < tmax <= zrows
+                zw[0] = __builtin_amdgcn_perm(accQ, accP, 0x05040100u);
+                zw[64] = __builtin_amdgcn_perm(accQ, accP, 0x07060302u);
+            }
+        }
+        // jobs on their last row take h[qlen] before a later unmasked block
+        // may overwrite their state: a select tree over the bits of the index
+        const bool end0 = act[0] && i == tlen[0] - 1, end1 = act[1] && i == tlen[1] - 1;
+        if (__builtin_amdgcn_ballot_w64(end0 || end1)) {
+            const int s0 = end0 ? sidx0 : 0, s1 = end1 ? sidx1 : 0;
+            uint32_t T[K + 2];
+#pragma unroll
+            for (int s = 0; s < K + 2; ++s) T[s] = H[s];
+#pragma unroll
+            for (int b = 0; (1 << b) < K + 2; ++b) {
+                const uint32_t mb = (((s0 >> b) & 1) ? 0xFFFFu : 0u) | (((s1 >> b) & 1) ? 0xFFFF0000u : 0u);
+#pragma unroll
+                for (int s = 0; s + (1 << b) < K + 2; s += 2 << b) T[s] = nw_sel(mb, T[s + (1 << b)], T[s]);
+            }
+            const uint32_t em = (end0 ? 0xFFFFu : 0u) | (end1 ? 0xFFFF0000u : 0u);
+            score = nw_sel(em, T[0], score);
+        }
+    }
+    if (d.cells) {
+        unsigned long long c = ncells;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+        if (lane == 0) atomicAdd(d.cells, c);
+    }
+#pragma unroll
+    for (int J = 0; J < 2; ++J) {
+        const int slot = lane + 64 * J;
+        if (slot >= count) continue;
+        const int job = (int)idx[slot];
+        if (!ne[J]) {
+            d.score[job] = 0;
+            d.n_cigar[job] = 0;
+            d.flags[job] = 0;
+            continue;
+        }
+        d.score[job] = (int)(int16_t)(score >> (16 * J));
+        const int ncol = min(qlen[J], 2 * wj[J] + 1);
+        const uint32_t *zj = zl + 64 * J;
+        nw_traceback(d, job, qlen[J], tlen[J], wj[J], ncol, [&](int i, int c) {
+            return nw_pk_flags(zj[((size_t)i * ncw + (c >> 3)) * 128], c & 7);
+        });
+    }
+}
+
+// The two-jobs-per-lane form: work items of up to 128 jobs; LDS holds the
+// scoring table and both staged queries.
+size_t nw_pk_lds_bytes(int K, int qw8) { return 64 + (size_t)(2 * qw8 + K / 8) * 256; }
+
+template <int K>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 32 ? 3 : 2)))
+void nw_pk_kernel(NwDev d, NwParams p, const uint32_t *order, const WorkItem *items, int n_items,
+                  uint32_t *counter, uint32_t *zbase, int zrows, int ncw, int persistent, int qw8)
+{
+    extern __shared__ __align__(16) unsigned char nw_lds[];
+    uint32_t *rowtab = reinterpret_cast<uint32_t *>(nw_lds);
+    if (threadIdx.x < 10) {
+        const int t = threadIdx.x >> 1;
+        uint32_t v = 0;
+        if (threadIdx.x & 1) {
+            v = (uint8_t)p.mat[t * 5 + 4];
+        } else {
+            for (int c = 3; c >= 0; --c) v = (v << 8) | (uint8_t)p.mat[t * 5 + c];
+        }
+        rowtab[threadIdx.x] = v;
+    }
+    __syncthreads();
+    uint32_t *qlds = reinterpret_cast<uint32_t *>(nw_lds + 64);
+    uint32_t *zreg = zbase + (size_t)blockIdx.x * zrows * ncw * 128;
+    __shared__ uint32_t idx[128];
+    __shared__ int item_sh;
+    int it = blockIdx.x;
+    for (;;) {
+        if (persistent) {
+            if (threadIdx.x == 0) item_sh = (int)atomicAdd(counter, 1u);
+            __syncthreads();
+            it = item_sh;
+        }
+        if (it >= n_items) break;
+        const WorkItem wi = items[it];
+        if (threadIdx.x < wi.count) idx[threadIdx.x] = order[wi.start + threadIdx.x];
+        if (threadIdx.x + 64 < wi.count) idx[threadIdx.x + 64] = order[wi.start + threadIdx.x + 64];
+        __syncthreads();
+        nw_wave_pk<K>(d, p, idx, (int)wi.count, zreg, zrows, ncw, rowtab, qlds, qw8);
+        __syncthreads();
+        if (!persistent) it += gridDim.x;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // One wave per job ("wide" form), for jobs whose band is too wide to leave to
 // one lane: the bins holding few of them and the band-doubling retry passes,
 // whose time a lane-per-job kernel spends on its single slowest job.
@@ -909,12 +1232,52 @@ static hipError_t nw_reg_t(const NwDev &d, const NwParams &p, const uint32_t *or
     return hipGetLastError();
 }
 
+template <int K>
+static int nw_pk_occ(int qw8)
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, nw_pk_kernel<K>, 64,
+                                                     nw_pk_lds_bytes(K, qw8)) != hipSuccess)
+        return 0;
+    return nb;
+}
+
+int nw_pk_blocks_per_cu(int K, int qw8)
+{
+    switch (K) {
+    case 32: return nw_pk_occ<32>(qw8);
+    case 64: return nw_pk_occ<64>(qw8);
+    case 96: return nw_pk_occ<96>(qw8);
+    default: return 0;
+    }
+}
+
+template <int K>
+static hipError_t nw_pk_t(const NwDev &d, const NwParams &p, const uint32_t *order,
+                          const WorkItem *items, int n_items, uint32_t *counter, uint32_t *zbase,
+                          int zrows, int ncw, int grid, bool persistent, int qw8, hipStream_t s)
+{
+    hipLaunchKernelGGL((nw_pk_kernel<K>), dim3(grid), dim3(64), nw_pk_lds_bytes(K, qw8), s, d, p,
+                       order, items, n_items, counter, zbase, zrows, ncw, persistent ? 1 : 0, qw8);
+    return hipGetLastError();
+}
+
 hipError_t launch_nw(int tier, bool global_state, int reg, const NwDev &d, const NwParams &p,
                      const uint32_t *order, const WorkItem *items, int n_items,
                      uint32_t *counter, int S, uint32_t *zbase, int zrows, int ncw, int grid,
                      bool persistent, void *gstate, int qw8, hipStream_t s)
 {
     if (n_items <= 0) return hipSuccess;
+    if (reg < 0) {
+        // two jobs per lane, K = -reg; blocks cover 8 * ncw >= every band
+        if (tier != NW_I16 || -reg < 8 * ncw) return hipErrorInvalidValue;
+        switch (-reg) {
+        case 32: return nw_pk_t<32>(d, p, order, items, n_items, counter, zbase, zrows, ncw, grid, persistent, qw8, s);
+        case 64: return nw_pk_t<64>(d, p, order, items, n_items, counter, zbase, zrows, ncw, grid, persistent, qw8, s);
+        case 96: return nw_pk_t<96>(d, p, order, items, n_items, counter, zbase, zrows, ncw, grid, persistent, qw8, s);
+        default: return hipErrorInvalidValue;
+        }
+    }
     if (reg) {
         // the kernel's blocks cover 8 * ncw >= every band of the launch
         if (tier != NW_I16 || reg < 8 * ncw) return hipErrorInvalidValue;

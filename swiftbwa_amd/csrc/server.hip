@@ -521,28 +521,29 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
             nw_plan_pass(eng, ql.data(), tl.data(), max_pen, active, wcur,
                          pass == 0 ? HIPSW_MODE_BATCHED : HIPSW_MODE_PLAIN, order, items, launches,
                          np.o_ins >= 0 ? (pass == 0 ? 1 : 2) : 0);
-            size_t zb = 0, gb = 0;
+            size_t zb = 0, zs = 0, gb = 0;
             for (const NwLaunch &L : launches) {
                 if (L.wide) {
                     if (L.zmax > 49152) zb = std::max(zb, (size_t)L.grid * L.zmax);
                     continue;
                 }
-                zb = std::max(zb, (size_t)L.grid * L.zrows * ((L.ncol_max + 7) / 8) * 64 * 4);
+                if (nw_side_launch(L, launches)) zs = std::max(zs, nw_launch_zbytes(L));
+                else zb = std::max(zb, nw_launch_zbytes(L));
                 if (L.global_state) gb = std::max(gb, (size_t)L.grid * L.S * 64 * (L.tier == NW_I16 ? 4 : 8));
             }
+            zb = (zb + 255) & ~(size_t)255;
+            zs = (zs + 255) & ~(size_t)255;
             void *scr = nullptr;
             up(d_w + nw0, wcur.data() + nw0, (size_t)r.n_nw * 4);   // NW rows only
             up(d_order, order.data(), order.size() * 4);
             up(d_items, items.data(), items.size() * sizeof(WorkItem));
-            if (ok) ok = hipMallocAsync(&scr, zb + gb + 512, st) == hipSuccess;
-            note_phase(zb + gb + 512);
+            if (ok) ok = hipMallocAsync(&scr, zb + zs + gb + 512, st) == hipSuccess;
+            note_phase(zb + zs + gb + 512);
             if (ok) ok = launch_zero_u32(sl.counters, 64, st) == hipSuccess;
             if (launches.size() > 64) ok = false;
             // as hipsw_nw_run: small bins in the wave-per-job form with their
             // flags in LDS run on the side stream, queued first
-            auto side_ok = [&](const NwLaunch &L) {
-                return L.wide && L.zmax <= 49152 && launches.size() > 1;
-            };
+            auto side_ok = [&](const NwLaunch &L) { return nw_side_launch(L, launches); };
             auto run_wide = [&](const NwLaunch &L, int k, hipStream_t ls) {
                 const WorkItem &first = items[L.item_begin];
                 const WorkItem &last = items[L.item_begin + L.item_count - 1];
@@ -559,7 +560,13 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
                          hipStreamWaitEvent(sl.side, sl.evf, 0) == hipSuccess;
                     forked = true;
                 }
-                if (ok) ok = run_wide(launches[li], (int)li, sl.side);
+                const NwLaunch &L = launches[li];
+                if (ok && L.wide) ok = run_wide(L, (int)li, sl.side);
+                else if (ok)
+                    ok = launch_nw(L.tier, false, L.reg, nd, np, d_order, d_items + L.item_begin,
+                                   L.item_count, sl.counters + li, 0, (uint32_t *)((char *)scr + zb),
+                                   L.zrows, (L.ncol_max + 7) / 8, L.grid, pass == 0, nullptr, L.qw8,
+                                   sl.side) == hipSuccess;
             }
             int k = -1;
             for (const NwLaunch &L : launches) {
@@ -572,7 +579,7 @@ int run_chunk(hipsw_server *srv, Slot &sl, ChunkReq &r)
                 }
                 ok = launch_nw(L.tier, L.global_state, L.reg, nd, np, d_order, d_items + L.item_begin,
                                L.item_count, sl.counters + k, L.S, (uint32_t *)scr, L.zrows,
-                               (L.ncol_max + 7) / 8, L.grid, pass == 0, (char *)scr + ((zb + 255) & ~(size_t)255),
+                               (L.ncol_max + 7) / 8, L.grid, pass == 0, (char *)scr + zb + zs,
                                L.qw8, st) == hipSuccess;
             }
             if (forked) {

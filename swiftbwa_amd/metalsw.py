@@ -103,7 +103,11 @@ class HipSWEngine:
 
     def set_nw_state(self, form):
         """Row-state form of the NW fill's 16-bit tier: 0 registers where the
-        band fits (default), 1 the LDS ring throughout."""
+        band fits, two jobs per lane when the bands of at most 64 columns, but
+        not those of at most 32, hold all but 1/32 of those cells (default); 1
+        the LDS ring throughout; 2
+        registers, one job per lane; 3 two jobs per lane for every band of at
+        most 64 columns."""
         if self.lib.hipsw_nw_set_state(self.handle, int(form)):
             raise ValueError("hipsw_nw_set_state: invalid form %r" % (form,))
 

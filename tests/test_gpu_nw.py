@@ -18,9 +18,10 @@ def oracle_sc(sp):
                        e_del=sp.gapExtendPenaltyDeletion, zdrop=sp.zDrop)
 
 
-def check(engine, js, sp=None, max_retries=3, cap=512, forms=(0, 1)):
-    """Every mode, and the 16-bit tier's row state in registers (form 0) and in
-    the LDS ring (form 1)."""
+def check(engine, js, sp=None, max_retries=3, cap=512, forms=(0, 1, 2, 3)):
+    """Every mode, and the 16-bit tier's row state in registers (form 0 chooses;
+    form 2 one job per lane; form 3 two jobs per lane in int16 pairs wherever
+    the band is at most 64 columns) and in the LDS ring (form 1)."""
     sp = sp or ScoringParameters()
     rs, rc, rn, cells = orc.global_batch(js, oracle_sc(sp), max_retries=max_retries, cap=cap,
                                          nthreads=8)
@@ -69,6 +70,22 @@ def test_known_answers(engine):
 
 def test_nw_subset(engine):
     check(engine, wl.generate(wl.NW, 3, 20000))
+
+
+def test_narrow_bands_two_jobs_per_lane(engine):
+    """Pairs whose bands are at most 64 columns: the default form fills them
+    two jobs per lane (one int16 pair per column and row); every form equal
+    to the oracle, with scorings whose E, F and H flags tie differently."""
+    js = wl.generate(wl.NW, 5, 30000)
+    q = js["q_len"].astype(int); t = js["t_len"].astype(int)
+    wp = np.maximum(np.minimum(js["w"].astype(int), 1 << 20), np.abs(q - t))
+    js = wl.slice_jobs(js, np.nonzero(np.minimum(q, 2 * wp + 1) <= 64)[0])
+    check(engine, js)
+    check(engine, js, ScoringParameters(gapOpenPenalty=0, gapExtendPenalty=2, gapOpenPenaltyDeletion=3,
+                                        gapExtendPenaltyDeletion=1, mismatchPenalty=3))
+    check(engine, js, ScoringParameters(matchScore=2, mismatchPenalty=2, gapOpenPenalty=1,
+                                        gapExtendPenalty=1, gapOpenPenaltyDeletion=1,
+                                        gapExtendPenaltyDeletion=1))
 
 
 def test_forced_retries(engine):
@@ -191,7 +208,7 @@ def test_register_state_band_edges(engine):
                      len(js["targets"]) * np.arange(rep)[:, None]).ravel().astype(np.uint32),
            "q_len": np.tile(js["q_len"], rep), "t_len": np.tile(js["t_len"], rep),
            "w": np.tile(js["w"], rep), "true_score": np.tile(js["true_score"], rep)}
-    check(engine, big, forms=(0,))
+    check(engine, big, forms=(0, 2, 3))
 
 
 def test_wave_per_job_edges(engine):
