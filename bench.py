@@ -281,6 +281,67 @@ def leg_nw(eng, args, modes=None):
     return res, js, outs.get("batched")
 
 
+def nw_narrow_set(js):
+    """The pairs of an NW set whose band is at most 64 columns (GlobalAligner.swift:47-48)."""
+    from swiftbwa_amd import workload as wl
+    q = js["q_len"].astype(np.int64)
+    t = js["t_len"].astype(np.int64)
+    wp = np.maximum(np.minimum(js["w"].astype(np.int64), 1 << 20), np.abs(q - t))
+    return wl.slice_jobs(js, np.nonzero(np.minimum(q, 2 * wp + 1) <= 64)[0])
+
+
+def leg_nw_narrow(eng, args, js, forms=((0, "two_jobs_per_lane"), (2, "one_job_per_lane"))):
+    """NW pairs with bands of at most 64 columns (367,300 of the 500,000): the
+    default row-state form (two jobs per lane in int16 pairs) against one job
+    per lane, batched, with identical results required."""
+    from swiftbwa_amd import native
+    from swiftbwa_amd.metalsw import ScoringParameters, _ext_arrays
+    lib = eng.lib
+    a = _ext_arrays(js)
+    ts = np.ascontiguousarray(js["true_score"], np.int32)
+    sc = ScoringParameters().to_c()
+    cap = 512
+    st = lib.hipsw_nw_upload(eng.handle, js["n"], native.ptr(a["queries"]), native.ptr(a["q_off"]),
+                             native.ptr(a["q_len"]), native.ptr(a["targets"]),
+                             native.ptr(a["t_off"]), native.ptr(a["t_len"]), native.ptr(a["w"]),
+                             native.ptr(ts), 3, cap, C.byref(sc))
+    if not st:
+        raise RuntimeError(eng.last_error())
+    res, outs = {"pairs": js["n"]}, {}
+    try:
+        for form, name in forms:
+            eng.set_nw_state(form)
+            s = native.HipswStats()
+            kms = []
+            for k in range(args.warmup + args.steps):
+                if lib.hipsw_nw_run(eng.handle, st, native.MODE_BATCHED, C.byref(s)):
+                    raise RuntimeError(eng.last_error())
+                if k >= args.warmup:
+                    kms.append(s.kernel_ms)
+            n = js["n"]
+            o = {"score": np.zeros(n, np.int32), "cigar": np.zeros((n, cap), np.uint32),
+                 "n_cigar": np.zeros(n, np.int32), "flags": np.zeros(n, np.int32)}
+            if lib.hipsw_nw_download(eng.handle, st, native.ptr(o["score"]), native.ptr(o["cigar"]),
+                                     native.ptr(o["n_cigar"]), native.ptr(o["flags"]), None):
+                raise RuntimeError(eng.last_error())
+            outs[name] = o
+            res["cells"] = int(s.cells)
+            res[name] = {"form": form, "kernel_ms": round(float(np.mean(kms)), 3),
+                         "kernel_gcups": round(s.cells / (np.mean(kms) * 1e-3) / 1e9, 2)}
+            log("nw narrow (form %d): kernel %.3f ms" % (form, np.mean(kms)))
+    finally:
+        eng.set_nw_state(0)
+        lib.hipsw_nw_free(eng.handle, st)
+    if len(outs) == 2:
+        x, y = list(outs.values())
+        res["rows_differing_between_forms"] = int(
+            ((x["score"] != y["score"]) | (x["n_cigar"] != y["n_cigar"]) |
+             (x["cigar"] != y["cigar"]).any(axis=1)).sum())
+        n0, n1 = [f[1] for f in forms]
+        res["speedup"] = round(res[n1]["kernel_ms"] / res[n0]["kernel_ms"], 3)
+    return res
+
+
 def leg_local(eng, args):
     """Mate-rescue local SW: 200,000 150-bp mates against 600-bp windows
     (score2 on, minSubScore 19), one-shot hipsw_local_sw calls."""
@@ -446,6 +507,7 @@ def profile_leg(args):
     ext:   the headline job set, batched form, `warmup` + `steps` passes.
     mixed: the uneven-size job set (4,000,000 jobs), batched form, likewise.
     nw:    the NW + CIGAR set (500,000 pairs), batched form, likewise.
+    nw_narrow: its pairs with bands of at most 64 columns, default form, likewise.
     local: the 200,000 rescue windows, `steps` one-shot calls.
     copy:  a 256 MiB device-to-device copy, `steps` times -- the known-size
            reference that scales the FETCH_SIZE / WRITE_SIZE counters."""
@@ -460,6 +522,9 @@ def profile_leg(args):
             raise RuntimeError(eng.last_error())
     if args.profile_leg == "nw":
         leg_nw(eng, args, modes=(("batched", native.MODE_BATCHED),))
+    elif args.profile_leg == "nw_narrow":
+        leg_nw_narrow(eng, args, nw_narrow_set(wl.generate(wl.NW, 3, args.nw_n)),
+                      forms=((0, "two_jobs_per_lane"),))
     elif args.profile_leg == "local":
         js = wl.generate(wl.LOCAL, 5, args.local_n)
         for k in range(args.steps):
@@ -521,7 +586,8 @@ def main():
     ap.add_argument("--serve-budget-gb", type=int, default=64)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--mixed-n", type=int, default=4000000)
-    ap.add_argument("--profile-leg", choices=["ext", "mixed", "nw", "local", "copy"], default=None)
+    ap.add_argument("--profile-leg", choices=["ext", "mixed", "nw", "nw_narrow", "local", "copy"],
+                    default=None)
     ap.add_argument("--tune", action="store_true", help="profile leg mixed: autotune first")
     ap.add_argument("--pk-lds-blocks", type=int, default=0,
                     help="profile leg: 16-column blocks of packed column state in LDS for "
@@ -555,6 +621,7 @@ def main():
     if args.nw_n > 0:
         nw, nw_js, nw_out = leg_nw(eng, args)
         sets["nw"] = (nw_js, nw_out)
+        nw["narrow_bands_le64"] = leg_nw_narrow(eng, args, nw_narrow_set(nw_js))
     if args.mixed_n > 0:
         mjs = wl.generate(wl.EXT_MIXED, 4, args.mixed_n)
         mixed, mixed_out = ext_forms(eng, mjs, sp, args, 100000, "mixed")
@@ -593,6 +660,12 @@ def main():
                                              kernel="nw_reg_kernel<96> + nw_wide_kernel (retry passes, wide bands)",
                                              note="int32 VALU peak: the fill runs 32-bit arithmetic on "
                                                   "16-bit state")
+        nn = nw.get("narrow_bands_le64")
+        if nn and "two_jobs_per_lane" in nn:
+            others["nw_narrow_bands_two_jobs_per_lane"] = roofline(
+                nn["cells"], nn["two_jobs_per_lane"]["kernel_ms"], OPS_NW, PEAK_PK16_TOPS,
+                kernel="nw_pk_kernel<64> + nw_wide_kernel (retry passes, wide bands)",
+                note="int16-pair arithmetic: packed-int16 VALU peak")
     if local:
         others["local_sw_200k"] = roofline(local["cells"], local["kernel_ms"], OPS_LOC, PEAK_I32_TOPS,
                                            kernel="local_striped_kernel<10> (+ scalar fallback)",
@@ -633,6 +706,7 @@ def main():
         ok &= mixed["rows_differing_between_forms"] == 0 and mixed["rows_violating_invariants"] == 0
     if nw:
         ok &= nw.get("rows_differing_between_forms", 0) == 0
+        ok &= nw["narrow_bands_le64"].get("rows_differing_between_forms", 0) == 0
     if not args.no_cpu:
         log("cpu baseline and oracle checks (%d threads)" % args.cpu_threads)
         rec["cpu_baseline"] = cpu_baseline(sets, args.cpu_threads)
