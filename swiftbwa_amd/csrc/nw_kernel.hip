@@ -78,9 +78,13 @@ __device__ __forceinline__ void wave_min2_u32(uint32_t &a, uint32_t &b)
 // bit 0 E opened from H (e_open > e_ext), bit 1 F opened from H
 // (f_open > f_ext), bit 2 H taken from E, bit 3 H taken from F -- the
 // reference's z byte with bits 0 and 1 negated (:114-136).
+//
+// wr: this lane writes the output.  A wave may walk one job's traceback with
+// every lane at once (wave-uniform state, flags read as uniform values) and
+// lane 0 alone writing.
 template <class ZF>
 __device__ __forceinline__ void nw_traceback(const NwDev &d, int job, int qlen, int tlen, int w,
-                                             int ncol, ZF zf)
+                                             int ncol, ZF zf, bool wr = true)
 {
     uint32_t *cig = d.cigar + (size_t)job * d.cap;
     int i = tlen - 1, jj = qlen - 1, state = 0;
@@ -89,7 +93,7 @@ __device__ __forceinline__ void nw_traceback(const NwDev &d, int job, int qlen, 
     const int cap = d.cap;
     int slot = 0;
     auto push = [&]() {
-        cig[slot] = (cur_len << 4) | cur_op;
+        if (wr) cig[slot] = (cur_len << 4) | cur_op;
         slot = slot + 1 == cap ? 0 : slot + 1;
         ++nops;
     };
@@ -127,6 +131,7 @@ __device__ __forceinline__ void nw_traceback(const NwDev &d, int job, int qlen, 
     if (cur_len) push();
     // Final order = reverse of emission order (:230).  With the ring, slots
     // [0, p) and [p, kept) are each reversed in place, p = nops mod cap.
+    if (!wr) return;
     const int kept = min(nops, cap);
     const int p_split = nops > cap ? nops % cap : kept;
     for (int a = 0, b = p_split - 1; a < b; ++a, --b) {
@@ -991,8 +996,11 @@ void nw_pk_kernel(NwDev d, NwParams p, const uint32_t *order, const WorkItem *it
 // encoding); they go to LDS
 // (bytes, [row][j - beg] as the reference's z) when tlen * ncol fits, else to
 // the wave's HBM region, and lane 0 walks the traceback from there.
-template <int KC>
-__device__ void nw_wide_job(const NwDev &d, const NwParams &p, int job, uint8_t *z)
+typedef __attribute__((address_space(3))) uint8_t nw_lds_u8;
+typedef __attribute__((address_space(1))) uint8_t nw_glb_u8;
+
+template <int KC, typename ZB>
+__device__ void nw_wide_job(const NwDev &d, const NwParams &p, int job, ZB *z)
 {
     constexpr int NEG = INT32_MIN / 2;                         // :52
     constexpr int LOW = NEG - (1 << 28);                       // below every scanned term
@@ -1103,10 +1111,12 @@ __device__ void nw_wide_job(const NwDev &d, const NwParams &p, int job, uint8_t 
 #pragma unroll
     for (int k = 0; k < KC; ++k)
         if (k == kq) score = __builtin_amdgcn_readlane(Hc[k], (qlen - 1) & 63);
-    __syncthreads();                                           // flags visible to lane 0
-    if (lane != 0) return;
-    d.score[job] = score;
-    nw_traceback(d, job, qlen, tlen, w, ncol, [&](int i, int c) { return (uint32_t)z[(size_t)i * ncol + c]; });
+    __syncthreads();                                           // flags visible to every lane
+    if (lane == 0) d.score[job] = score;
+    // every lane walks the traceback (its state is wave-uniform), lane 0 writes
+    nw_traceback(d, job, qlen, tlen, w, ncol, [&](int i, int c) {
+        return (uint32_t)__builtin_amdgcn_readfirstlane((int)z[(size_t)i * ncol + c]);
+    }, lane == 0);
 }
 
 // Jobs jobs[0..n) of one pass, one wave each, taken from an atomic counter.
@@ -1118,7 +1128,10 @@ __global__ __launch_bounds__(64) void nw_wide_kernel(NwDev d, NwParams p, const 
                                                      size_t zmax, int zlds)
 {
     extern __shared__ __align__(16) unsigned char nw_lds[];
-    uint8_t *z = zlds ? reinterpret_cast<uint8_t *>(nw_lds) : zg + (size_t)blockIdx.x * zmax;
+    // the flag matrix in LDS or in this block's HBM region, addressed in its
+    // own address space (ds_ / global_ accesses rather than flat ones)
+    nw_lds_u8 *zl = (nw_lds_u8 *)nw_lds;
+    nw_glb_u8 *zh = (nw_glb_u8 *)(zg + (size_t)blockIdx.x * zmax);
     __shared__ int item_sh;
     for (;;) {
         if (threadIdx.x == 0) item_sh = (int)atomicAdd(counter, 1u);
@@ -1126,7 +1139,8 @@ __global__ __launch_bounds__(64) void nw_wide_kernel(NwDev d, NwParams p, const 
         const int it = item_sh;
         __syncthreads();
         if (it >= n) break;
-        nw_wide_job<KC>(d, p, (int)jobs[it], z);
+        if (zlds) nw_wide_job<KC>(d, p, (int)jobs[it], zl);
+        else nw_wide_job<KC>(d, p, (int)jobs[it], zh);
         __syncthreads();
     }
 }
