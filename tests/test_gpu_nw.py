@@ -130,6 +130,51 @@ def test_long_and_int32_tier(engine):
     check(engine, _random(np.random.default_rng(22), 200, 900, 1000), sp)
 
 
+def test_int16_bound_edge(engine):
+    """Pairs at the 16-bit tier's edge, (qlen + tlen) * max_pen + 64 <= 8000:
+    with max_pen = 23 (o = 20, e = 3, mismatch 9), qlen + tlen = 345 is the
+    largest 16-bit pair and 346 goes to int32.  Identical, random and indel
+    pairs with bands of at most 64 columns, so the int16-pair form (form 3)
+    fills them where the sentinel's drift and the score range are widest."""
+    rng = np.random.default_rng(31)
+    sp = ScoringParameters(gapOpenPenalty=20, gapExtendPenalty=3, gapOpenPenaltyDeletion=20,
+                           gapExtendPenaltyDeletion=3, mismatchPenalty=9)
+    qs, ts, w = [], [], []
+    for total in (345, 346):
+        for kind in range(4):
+            for band in (1, 3, 10, 31):
+                for dl in (0, 1, 5):
+                    ql = (total - dl) // 2
+                    tl = total - ql
+                    q = rng.integers(0, 4, ql).astype(np.uint8)
+                    if kind == 0:                                   # identical prefix
+                        t = np.concatenate([q, rng.integers(0, 4, tl - ql)]).astype(np.uint8)
+                    elif kind == 1:                                 # unrelated
+                        t = rng.integers(0, 4, tl).astype(np.uint8)
+                    elif kind == 2:                                 # an insertion in the middle
+                        t = np.concatenate([q[:ql // 2], rng.integers(0, 4, tl - ql), q[ql // 2:]]).astype(np.uint8)
+                    else:                                           # 5% substitutions, Ns
+                        t = np.concatenate([q, rng.integers(0, 4, tl - ql)]).astype(np.uint8)
+                        t[rng.random(tl) < 0.05] = rng.integers(0, 4)
+                        q[rng.random(ql) < 0.02] = 4
+                    qs.append(q); ts.append(t); w.append(band)
+    n = len(qs)
+    ql = np.array([len(x) for x in qs]); tl = np.array([len(x) for x in ts])
+    qo = np.zeros(n, np.uint32); to = np.zeros(n, np.uint32)
+    qo[1:] = np.cumsum(ql)[:-1]; to[1:] = np.cumsum(tl)[:-1]
+    js = {"n": n, "queries": np.concatenate(qs), "targets": np.concatenate(ts), "q_off": qo,
+          "t_off": to, "q_len": ql.astype(np.uint16), "t_len": tl.astype(np.uint16),
+          "w": np.array(w, np.int32), "true_score": np.full(n, -(1 << 20), np.int32)}
+    check(engine, js, sp)
+    # many copies: full waves in every form, both halves of the pairs used
+    rep = 64
+    big = {k: (np.tile(v, rep) if isinstance(v, np.ndarray) else v) for k, v in js.items()}
+    big["n"] = n * rep
+    big["q_off"] = (qo[None, :].astype(np.int64) + len(js["queries"]) * np.arange(rep)[:, None]).ravel().astype(np.uint32)
+    big["t_off"] = (to[None, :].astype(np.int64) + len(js["targets"]) * np.arange(rep)[:, None]).ravel().astype(np.uint32)
+    check(engine, big, sp, forms=(0, 2, 3))
+
+
 def test_small_cap_flags(engine):
     check(engine, _random(np.random.default_rng(23), 500, 60, 70), cap=2)
 
