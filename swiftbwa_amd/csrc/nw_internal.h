@@ -92,12 +92,17 @@ inline size_t nw_launch_zbytes(const NwLaunch &L)
 }
 
 // Launches of a pass queued on the side stream, ahead of the main stream's:
-// wave-per-job bins with their flags in LDS, and two-jobs-per-lane bins
-// (backpointers in a region of their own), when the pass has other launches.
+// wave-per-job bins with their flags in LDS, when the pass has other
+// launches, and two-jobs-per-lane bins (backpointers in a region of their
+// own) when a one-job-per-lane launch holds the main stream.
 inline bool nw_side_launch(const NwLaunch &L, const std::vector<NwLaunch> &all)
 {
     if (all.size() < 2) return false;
-    return (L.wide && L.zmax <= 49152) || (!L.wide && L.reg < 0);
+    if (L.wide) return L.zmax <= 49152;
+    if (L.reg >= 0) return false;
+    for (const NwLaunch &o : all)
+        if (!o.wide && o.reg >= 0) return true;
+    return false;
 }
 
 }  // namespace hipsw
