@@ -1797,8 +1797,12 @@ __global__ __launch_bounds__(64) void ext_persistent_ln_kernel(ExtDev d, ExtPara
 // (the planner bins N-holding jobs apart).  The block's LDS is exactly the
 // column state (no static shared memory), so a CU holds as many waves as their
 // LDS shares allow (150-bp queries: four held whole, six as 96 + 64 columns).
+// The 8-bit form is held to 256 registers, two waves per SIMD (its 16-query-
+// word instances would otherwise take 257 and run one); the 16-bit form needs
+// 294-316 and runs one.
 template <bool SYM, int NQ, int LB, bool W16>
-__global__ __launch_bounds__(64) void ext_persistent_pk_kernel(ExtDev d, ExtParams p,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(!W16 && NQ == 16 ? 2 : 1)))
+void ext_persistent_pk_kernel(ExtDev d, ExtParams p,
                                                                const WorkItem *items, int n_items,
                                                                uint32_t *counter, uint32_t *gscratch)
 {
