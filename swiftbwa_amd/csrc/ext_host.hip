@@ -66,10 +66,13 @@ size_t span(size_t count)
 namespace hipsw {
 
 // Batched form: bin by (tier, query-length class, N present); inside a bin
-// order by estimated DP area, largest first, and within an area step of 256
-// cells by h0, largest first (h0 sets how fast a band's left edge moves, so
-// jobs sharing a wave keep similar bands); cut into work items; one persistent
-// launch per bin.  Bins without N, with the standard matrix and queries below
+// order by band w, widest first (in steps of 4 columns, 255 and wider as one),
+// then by estimated DP area, largest first (steps of 256 cells, 2 M and more
+// as one), and within an area step by h0, largest first.  A wave sweeps the union of its jobs' bands row
+// by row, and a row's band starts at i - w (:90-91): jobs of one w share its
+// left edge, and h0 sets how fast that edge moves once z-drop and zero cells
+// narrow it.  Then cut into work items (the bin stays close to longest
+// first); one persistent launch per bin.  Bins without N, with the standard matrix and queries below
 // 256 bases run the register-query kernels: the packed form for the 8-bit
 // tier (unless the engine's tuning table picks the one-job-per-lane form for
 // that class), the one-job-per-lane form for the 16-bit tier.  Returns the
@@ -81,15 +84,16 @@ size_t plan_ext_batched(const hipsw_engine *eng, const TuneTable &tune, int n, c
                         std::vector<ExtLaunch> &launches)
 {
     std::vector<uint32_t> keys(n), idx(n);
-    const uint64_t amax = (1ull << 27) - 1;
     parallel_for(n, host_threads(), [&](int64_t lo, int64_t hi) {
         for (int64_t k = lo; k < hi; ++k) {
             const uint32_t bin = (uint32_t)((tier_of(h0[k], ql[k], max_mat) * kClasses + class_of(ql[k])) * 2 +
                                             (has_n && has_n[k] ? 1 : 0));
-            const uint64_t band = (uint64_t)std::min<int64_t>(ql[k], 2 * (int64_t)std::max(w[k], 0) + 1);
-            const uint64_t area = std::min<uint64_t>((uint64_t)tl[k] * band, amax);
+            const int64_t wk = std::max(w[k], 0);
+            const uint64_t band = (uint64_t)std::min<int64_t>(ql[k], 2 * wk + 1);
+            const uint32_t wq = (uint32_t)std::min<int64_t>(wk, 255) >> 2;               // 6 bits
+            const uint32_t aq = (uint32_t)std::min<uint64_t>((uint64_t)tl[k] * band >> 8, 8191);   // 13 bits
             const uint32_t h0c = (uint32_t)std::min(std::max(h0[k], 0), 255);
-            keys[k] = (bin << 27) | ((uint32_t)(amax - area) & ~0xFFu) | (255u - h0c);
+            keys[k] = (bin << 27) | ((63u - wq) << 21) | ((8191u - aq) << 8) | (255u - h0c);
             idx[k] = (uint32_t)k;
         }
     });
