@@ -347,6 +347,29 @@ bool pack_jobs(int n, const uint8_t *queries, const uint32_t *q_off, const uint1
     return true;
 }
 
+void *pinned_or_heap_alloc(size_t bytes)
+{
+    constexpr size_t kHdr = 64, kPinMax = size_t(1) << 30;
+    void *raw = nullptr;
+    uint32_t pinned = 0;
+    if (bytes <= kPinMax && hipHostMalloc(&raw, bytes + kHdr, hipHostMallocDefault) == hipSuccess) {
+        pinned = 1;
+    } else {
+        (void)hipGetLastError();                               // clear a failed pin
+        raw = ::operator new(bytes + kHdr);
+    }
+    *static_cast<uint32_t *>(raw) = pinned;
+    return static_cast<char *>(raw) + kHdr;
+}
+
+void pinned_or_heap_free(void *p) noexcept
+{
+    if (!p) return;
+    void *raw = static_cast<char *>(p) - 64;
+    if (*static_cast<uint32_t *>(raw)) (void)hipHostFree(raw);
+    else ::operator delete(raw);
+}
+
 void radix_sort_by_key(std::vector<uint32_t> &keys, std::vector<uint32_t> &idx)
 {
     // Stable LSD sort, four 8-bit digits.  The jobs are cut into the chunks

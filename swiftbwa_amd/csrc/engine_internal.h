@@ -34,6 +34,29 @@ template <class T> struct NoInitAlloc : std::allocator<T> {
 };
 template <class T> using NoFillVec = std::vector<T, NoInitAlloc<T>>;
 
+// Page-locked host memory (hipHostMalloc) for arrays the device copies from:
+// their uploads run as direct DMA rather than through the runtime's pageable
+// staging.  Elements are default-initialised like NoInitAlloc's.  Blocks
+// above 1 GiB, and every block when pinning fails (no device), come from the
+// ordinary heap; a 64-byte header records which.
+void *pinned_or_heap_alloc(size_t bytes);
+void pinned_or_heap_free(void *p) noexcept;
+template <class T> struct PinnedAlloc {
+    using value_type = T;
+    PinnedAlloc() = default;
+    template <class U> PinnedAlloc(const PinnedAlloc<U> &) noexcept {}
+    T *allocate(size_t n) { return static_cast<T *>(pinned_or_heap_alloc(n * sizeof(T))); }
+    void deallocate(T *p, size_t) noexcept { pinned_or_heap_free(p); }
+    template <class U> void construct(U *p) noexcept { ::new ((void *)p) U; }
+    template <class U, class... A> void construct(U *p, A &&...a)
+    {
+        ::new ((void *)p) U(std::forward<A>(a)...);
+    }
+    template <class U> bool operator==(const PinnedAlloc<U> &) const noexcept { return true; }
+    template <class U> bool operator!=(const PinnedAlloc<U> &) const noexcept { return false; }
+};
+template <class T> using PinnedVec = std::vector<T, PinnedAlloc<T>>;
+
 struct ExtPrep;
 
 // A grow-only device buffer (hipMalloc only when a request outgrows it).
@@ -256,11 +279,12 @@ namespace hipsw {
 struct ExtPrep {
     int n = 0;
     ExtParams params;
-    NoFillVec<uint32_t> seq, qw, tw, qnw, tnw;
+    // the arrays uploaded as they are: page-locked (kept for reuse by PrepLease)
+    PinnedVec<uint32_t> seq, qw, tw, qnw, tnw;
     std::vector<uint32_t> nmask, order;
-    NoFillVec<int32_t> ql, tl, h0, w, eb;
+    PinnedVec<int32_t> ql, tl, h0, w, eb;
     std::vector<WorkItem> items;
-    NoFillVec<PkMeta> pk;
+    PinnedVec<PkMeta> pk;
     hipsw_ext_set shape;             // tiers, plans and scratch sizes (no device pointers)
     size_t arena_bytes = 0;          // device bytes of the set's arrays
 };

@@ -593,7 +593,8 @@ extern "C" hipsw_ext_set *hipsw_ext_upload(hipsw_engine *eng, int32_t n,
                                            const int32_t *end_bonus, const hipsw_scoring *sc)
 {
     if (!eng) return nullptr;
-    ExtPrep P;
+    PrepLease prep(eng);                                       // page-locked host halves, reused
+    ExtPrep &P = prep.get();
     std::string err;
     if (ext_prepare(eng, eng->tune_snapshot(), n, queries, q_off, q_len, targets, t_off, t_len,
                     h0, w, end_bonus, sc, P, err) != HIPSW_OK) {
