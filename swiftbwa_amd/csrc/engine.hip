@@ -264,6 +264,43 @@ bool pack2(const uint8_t *src, int len, uint32_t *dst, uint32_t *ndst)
     return true;
 }
 
+int pack2_scan(const uint8_t *src, int len, uint32_t *dst)
+{
+    const int words = (len + 15) / 16;
+    int any_n = 0;
+    for (int wdx = 0; wdx < words; ++wdx) {
+        const int lo = wdx * 16, hi = std::min(len, lo + 16);
+        if (hi - lo == 16) {
+            const uint64_t a = load8(src + lo), b = load8(src + lo + 8);
+            if (((a | b) & kHigh6) == 0) {
+                dst[wdx] = gather8(a) | (gather8(b) << 16);
+                continue;
+            }
+        }
+        uint32_t v = 0;
+        for (int k = lo; k < hi; ++k) {
+            const uint8_t c = src[k];
+            if (c > 4) return -1;
+            any_n |= c == 4;
+            v |= (uint32_t)(c & 3) << ((k - lo) * 2);
+        }
+        dst[wdx] = v;
+    }
+    return any_n;
+}
+
+void nmask_fill(const uint8_t *src, int len, uint32_t *ndst)
+{
+    const int nw = (len + 31) / 32;
+    for (int wdx = 0; wdx < nw; ++wdx) {
+        uint32_t v = 0;
+        const int lo = wdx * 32, hi = std::min(len, lo + 32);
+        for (int k = lo; k < hi; ++k)
+            if (src[k] == 4) v |= 1u << (k - lo);
+        ndst[wdx] = v;
+    }
+}
+
 bool has_n(const uint8_t *src, int len)
 {
     int k = 0;

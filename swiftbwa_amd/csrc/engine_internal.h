@@ -180,6 +180,11 @@ bool scoring_valid(const hipsw_scoring *sc);
 ExtParams make_params(const hipsw_scoring *sc);
 bool pack2(const uint8_t *src, int len, uint32_t *dst, uint32_t *ndst);
 bool has_n(const uint8_t *src, int len);
+// pack2 without the N mask, in the same pass telling whether the sequence
+// holds an N: -1 for a code above 4, 1 with an N, 0 without.
+int pack2_scan(const uint8_t *src, int len, uint32_t *dst);
+// The N mask words of a sequence (bit k of word k / 32 set where base k is N).
+void nmask_fill(const uint8_t *src, int len, uint32_t *ndst);
 
 // 2-bit packed query/target arenas of a job set (word-aligned per sequence).
 struct PackedJobs {

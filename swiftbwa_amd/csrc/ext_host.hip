@@ -305,52 +305,41 @@ int ext_prepare(const hipsw_engine *eng, const TuneTable &tune, int n, const uin
 
     // Word layout: every query and target starts on a 16-base word boundary,
     // sequences back to back.  Chunks of jobs sum their words, the chunk
-    // totals are scanned, then each chunk writes its jobs' offsets.
+    // totals are scanned, then each chunk writes its jobs' offsets.  The
+    // packing pass reads each base once and also finds the sequences holding
+    // an N; their mask words (placed by a second scan of the flags) are
+    // filled afterwards from those sequences alone.
     P.qw.resize(n); P.tw.resize(n); P.qnw.resize(n); P.tnw.resize(n);
     std::vector<uint8_t> qn_flag(n), tn_flag(n);
     const int nch = (threads > 1 && n >= 4096) ? threads : 1;
     const int64_t chunk = ((int64_t)n + nch - 1) / nch;
     std::vector<uint64_t> cw(nch + 1, 0), cn(nch + 1, 0);
     parallel_chunks(nch, [&](int c) {
-        uint64_t a = 0, b = 0;
-        for (int64_t k = c * chunk, e = std::min<int64_t>(n, (c + 1) * chunk); k < e; ++k) {
-            qn_flag[k] = has_n(queries + q_off[k], q_len[k]);
-            tn_flag[k] = has_n(targets + t_off[k], t_len[k]);
+        uint64_t a = 0;
+        for (int64_t k = c * chunk, e = std::min<int64_t>(n, (c + 1) * chunk); k < e; ++k)
             a += (q_len[k] + 15) / 16 + (t_len[k] + 15) / 16;
-            b += (qn_flag[k] ? (q_len[k] + 31) / 32 : 0) + (tn_flag[k] ? (t_len[k] + 31) / 32 : 0);
-        }
         cw[c + 1] = a;
-        cn[c + 1] = b;
     });
-    for (int c = 0; c < nch; ++c) {
-        cw[c + 1] += cw[c];
-        cn[c + 1] += cn[c];
-    }
-    uint64_t words = cw[nch], nwords = cn[nch];
+    for (int c = 0; c < nch; ++c) cw[c + 1] += cw[c];
+    uint64_t words = cw[nch];
     if (words + 64 > 0xFFFFFFF0ull) {
         err = "job set exceeds 2^32 sequence words";
         return HIPSW_E_UNSUPPORTED;
     }
     parallel_chunks(nch, [&](int c) {
-        uint64_t a = cw[c], b = cn[c];
+        uint64_t a = cw[c];
         for (int64_t k = c * chunk, e = std::min<int64_t>(n, (c + 1) * chunk); k < e; ++k) {
             P.qw[k] = (uint32_t)a;
             a += (q_len[k] + 15) / 16;
             P.tw[k] = (uint32_t)a;
             a += (t_len[k] + 15) / 16;
-            P.qnw[k] = qn_flag[k] ? (uint32_t)b : kNoN;
-            if (qn_flag[k]) b += (q_len[k] + 31) / 32;
-            P.tnw[k] = tn_flag[k] ? (uint32_t)b : kNoN;
-            if (tn_flag[k]) b += (t_len[k] + 31) / 32;
         }
     });
     words += 64;    // tail padding: whole-word reads past the last sequence stay in bounds
-    nwords += 64;
     // every sequence word is written by the packing below; only the padding
     // is cleared here
     P.seq.resize(words);
     std::fill(P.seq.end() - 64, P.seq.end(), 0u);
-    P.nmask.assign(nwords, 0);
     P.ql.resize(n); P.tl.resize(n); P.h0.resize(n); P.w.resize(n); P.eb.resize(n);
     std::atomic<bool> bad{false};
     parallel_for(n, threads, [&](int64_t lo, int64_t hi) {
@@ -360,18 +349,41 @@ int ext_prepare(const hipsw_engine *eng, const TuneTable &tune, int n, const uin
             P.h0[k] = h0[k];
             P.w[k] = w[k];
             P.eb[k] = end_bonus[k];
-            if (!pack2(queries + q_off[k], q_len[k], P.seq.data() + P.qw[k],
-                       qn_flag[k] ? P.nmask.data() + P.qnw[k] : nullptr))
-                bad = true;
-            if (!pack2(targets + t_off[k], t_len[k], P.seq.data() + P.tw[k],
-                       tn_flag[k] ? P.nmask.data() + P.tnw[k] : nullptr))
-                bad = true;
+            const int rq = pack2_scan(queries + q_off[k], q_len[k], P.seq.data() + P.qw[k]);
+            const int rt = pack2_scan(targets + t_off[k], t_len[k], P.seq.data() + P.tw[k]);
+            if (rq < 0 || rt < 0) bad = true;
+            qn_flag[k] = rq > 0;
+            tn_flag[k] = rt > 0;
         }
     });
     if (bad) {
         err = "base code > 4";
         return HIPSW_E_INVALID;
     }
+    parallel_chunks(nch, [&](int c) {
+        uint64_t b = 0;
+        for (int64_t k = c * chunk, e = std::min<int64_t>(n, (c + 1) * chunk); k < e; ++k)
+            b += (qn_flag[k] ? (q_len[k] + 31) / 32 : 0) + (tn_flag[k] ? (t_len[k] + 31) / 32 : 0);
+        cn[c + 1] = b;
+    });
+    for (int c = 0; c < nch; ++c) cn[c + 1] += cn[c];
+    const uint64_t nwords = cn[nch] + 64;
+    P.nmask.assign(nwords, 0);
+    parallel_chunks(nch, [&](int c) {
+        uint64_t b = cn[c];
+        for (int64_t k = c * chunk, e = std::min<int64_t>(n, (c + 1) * chunk); k < e; ++k) {
+            P.qnw[k] = qn_flag[k] ? (uint32_t)b : kNoN;
+            if (qn_flag[k]) {
+                nmask_fill(queries + q_off[k], q_len[k], P.nmask.data() + b);
+                b += (q_len[k] + 31) / 32;
+            }
+            P.tnw[k] = tn_flag[k] ? (uint32_t)b : kNoN;
+            if (tn_flag[k]) {
+                nmask_fill(targets + t_off[k], t_len[k], P.nmask.data() + b);
+                b += (t_len[k] + 31) / 32;
+            }
+        }
+    });
 
     // Exact score tier per job; jobs holding an N.
     std::vector<uint8_t> any_n(n);
