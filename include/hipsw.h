@@ -23,10 +23,13 @@
  *    pack, upload, run and copy back concurrently; a call waits only when every
  *    context is leased.  Calls on device-resident sets (hipsw_ext_* / hipsw_nw_*)
  *    and the autotuner run on the engine's own stream under its mutex.
- *  - hipsw_banded_extend and hipsw_banded_extend_noalloc pack and plan into
- *    host buffers the engine keeps for reuse (at most four, grow-only, about
- *    190 bytes per job of the largest call; freed by hipsw_engine_destroy),
- *    so repeated calls do not fault in fresh host memory.
+ *  - hipsw_banded_extend, hipsw_banded_extend_noalloc and hipsw_ext_upload
+ *    pack and plan into host buffers the engine keeps for reuse (at most four,
+ *    grow-only, about 190 bytes per job of the largest call; freed by
+ *    hipsw_engine_destroy), so repeated calls do not fault in fresh host
+ *    memory.  The arrays copied to the device (about 140 bytes per 150-base
+ *    job) are page-locked (hipHostMalloc) when the runtime grants it, so their
+ *    upload is a direct DMA; blocks above 1 GiB stay pageable.
  *  - hipsw_last_error reports the calling thread's last failure on the engine
  *    (errno-like), so concurrent callers do not see each other's messages.
  */
